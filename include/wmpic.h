@@ -1,0 +1,154 @@
+/*
+ * wmpic.h -- C ABI of libwmpic, the MI355X-native implementation of the
+ * WumingPIC2D per-step hot path (push, deposit, sort, boundaries, implicit
+ * field solve).
+ *
+ * Conventions
+ *  - Every function returns an int status: WM_OK (0) or one of WM_ERR_*.
+ *    wm_last_error() returns the message; for the reference's fatal stops
+ *    the message is the reference's own text.
+ *  - Arrays passed by host pointer use the reference's column-major layout
+ *    and index bounds (first index fastest), exactly as the Fortran callers
+ *    hold them:
+ *      up, gp  (ndim, np, nys:nye, nsp)        real(8)
+ *      uf, df  (6, nxgs-2:nxge+2, nys-2:nye+2)  real(8)
+ *      uj      (3, nxgs-2:nxge+2, nys-2:nye+2)  real(8)
+ *      cumcnt  (nxgs:nxge+1, nys:nye, nsp)      integer(4)
+ *      np2     (nys:nye, nsp)                   integer(4)
+ *      phi     (nxs-1:nxe+1, nys-1:nye+1)       real(8)
+ *  - One context per device and grid (one MPI rank of the reference).  All
+ *    calls on a context are made from one host thread; each call is
+ *    synchronous with respect to the host-visible results it returns.
+ *  - The host-array entry points (wm_particle_solv ... wm_bcp_phi) keep the
+ *    reference's call semantics: inputs are read from host memory, outputs are
+ *    written to host memory, persistent state (the CG warm start of
+ *    field__fdtd_i) stays on the device.  The device-resident entry points
+ *    (wm_state_*, wm_step) keep particles and fields in HBM across steps.
+ *
+ * Citations name the reference routine each entry point replaces
+ * (files of WumingCode/WumingPIC2D).
+ */
+#ifndef WMPIC_H
+#define WMPIC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WM_OK 0
+#define WM_ERR_NOT_INIT 1   /* "Initialize first by calling ..." particle.f90:63-66,
+                               field.f90:100-103, sort.f90:47-50,
+                               boundary_periodic.f90:69-72 */
+#define WM_ERR_CG_ITEMAX 2  /* "stop at cgm after ite_max" field.f90:427-430 */
+#define WM_ERR_NP_OVER 3    /* "memory over (np2 > np)" boundary_periodic.f90:231-234 */
+#define WM_ERR_RANGE 4      /* a particle outside the cells the reference indexes */
+#define WM_ERR_ARG 5        /* inconsistent arguments */
+#define WM_ERR_HIP 6        /* HIP runtime error */
+#define WM_ERR_NOMEM 7      /* device allocation failed */
+#define WM_ERR_NODEV 8      /* no usable GPU */
+
+/* Boundary families for wm_field_fdtd_i's set_boundary_* callbacks. */
+#define WM_BC_PERIODIC 0    /* boundary_periodic__dfield/__curre/__phi */
+
+typedef struct wm_ctx wm_ctx;
+
+/* ---- context (no reference counterpart: holds what the reference keeps in
+ *      module SAVE variables, on the device) ---- */
+int wm_create(wm_ctx **ctx, int device);
+int wm_destroy(wm_ctx *ctx);
+const char *wm_last_error(const wm_ctx *ctx);
+/* Device capacity per (species, row) and migration buffer per row and
+ * direction.  The reference sizes rows by np (proj/weibel/app.f90:248);
+ * the device may use less.  0 selects defaults (np, np/8). */
+int wm_set_capacity(wm_ctx *ctx, int cap_per_row, int mover_cap);
+/* CG relative tolerance; the reference fixes 1e-6 (field.f90:339). */
+int wm_set_cg_tolerance(wm_ctx *ctx, double err);
+
+/* ---- mpi_set__init (mpi_set.f90:19-49), single rank ---- */
+int wm_mpi_set_init(int nygs, int nyge, int nproc, int *nys, int *nye,
+                    int *nup, int *ndown, int *nrank, int *nsize);
+
+/* ---- module initialisers ---- */
+/* particle__init (particle.f90:18-45) */
+int wm_particle_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs,
+                     int nxge, int nygs, int nyge, int nys, int nye,
+                     double delx, double delt, double c, const double *q,
+                     const double *r);
+/* field__init (field.f90:22-63); mnpr/ncomw/opsum/nerr are MPI handles that a
+ * single device context does not use. */
+int wm_field_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
+                  int nygs, int nyge, int nys, int nye, int mnpr, int ncomw,
+                  int opsum, int nerr, double delx, double delt, double c,
+                  const double *q, const double *r, double gfac);
+/* sort__init (sort.f90:16-33) */
+int wm_sort_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
+                 int nygs, int nyge, int nys, int nye);
+/* boundary_periodic__init (boundary_periodic.f90:26-58) */
+int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
+                int nygs, int nyge, int nys, int nye, int nup, int ndown,
+                int mnpi, int mnpr, int ncomw, int nerr, const int *nstat,
+                int nstat_len, double delx, double delt, double c);
+
+/* ---- host-array entry points (reference call semantics) ---- */
+/* particle__solv(gp,up,uf,cumcnt,nxs,nxe) (particle.f90:48-177) */
+int wm_particle_solv(wm_ctx *ctx, double *gp, const double *up,
+                     const double *uf, const int *cumcnt, int nxs, int nxe);
+/* field__fdtd_i(uf,up,gp,cumcnt,nxs,nxe,bc_dfield,bc_curre,bc_phi)
+ * (field.f90:66-186).  The three boundary callbacks of the reference are
+ * selected by bc_family and run on the device. */
+int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up,
+                    const double *gp, const int *cumcnt, int nxs, int nxe,
+                    int bc_family);
+/* boundary_periodic__particle_x(up,np2) (boundary_periodic.f90:61-96) */
+int wm_bcp_particle_x(wm_ctx *ctx, double *up, const int *np2);
+/* boundary_periodic__particle_y(up,np2) (boundary_periodic.f90:99-248) */
+int wm_bcp_particle_y(wm_ctx *ctx, double *up, int *np2);
+/* sort__bucket(gp,up,cumcnt,np2,nxs,nxe) (sort.f90:36-82): gp is the output */
+int wm_sort_bucket(wm_ctx *ctx, double *gp, const double *up, int *cumcnt,
+                   const int *np2, int nxs, int nxe);
+/* boundary_periodic__dfield(df,nxs,nxe,nys,nye,nxgs,nxge)
+ * (boundary_periodic.f90:251-354) */
+int wm_bcp_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye,
+                  int nxgs, int nxge);
+/* boundary_periodic__curre(uj,nxs,nxe,nys,nye,nxgs,nxge)
+ * (boundary_periodic.f90:357-508) */
+int wm_bcp_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye,
+                 int nxgs, int nxge);
+/* boundary_periodic__phi(phi,nxs,nxe,nys,nye,l) (boundary_periodic.f90:511-568) */
+int wm_bcp_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye,
+               int l);
+
+/* ---- device-resident state and step ---- */
+/* Ingest the reference's state (as after proj/weibel/app.f90:362 gp = up):
+ * up (np2 active per row, sorted with cumcnt) and uf. */
+int wm_state_upload(wm_ctx *ctx, const double *up, const int *np2,
+                    const int *cumcnt, const double *uf);
+/* Materialise the device state into the reference layout. */
+int wm_state_download(wm_ctx *ctx, double *up, int *np2, int *cumcnt,
+                      double *uf);
+/* Synthetic counter-streaming pair plasma generated on the device:
+ * ppc particles per cell per species, positions uniform in each row,
+ * u = (0, 0, +/-udrift) + N(0, vth) per component, co-located species,
+ * fields zero.  Same formula as wumingpic2d_amd.weibel.init_state. */
+int wm_state_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth,
+                         double udrift);
+/* nsteps steps of the reference loop body proj/weibel/app.f90:102-106:
+ * particle__solv, field__fdtd_i, particle_x, particle_y, sort__bucket. */
+int wm_step(wm_ctx *ctx, int nsteps);
+/* energy_history sums (proj/weibel/app.f90:479-545) of the resident state:
+ * out[0..nsp-1] kinetic per species, out[nsp] E, out[nsp+1] B. */
+int wm_energy(wm_ctx *ctx, double *out);
+/* CG iterations per component (l = 1..3) of the last field solve. */
+int wm_cg_iters(wm_ctx *ctx, int *iters3);
+/* GPU time (ms) of each phase of the last wm_step call, summed over its
+ * steps: [0] push+deposit, [1] current gather+fold, [2] field solve,
+ * [3] particle boundaries, [4] sort.  Returns the number written. */
+int wm_phase_times(wm_ctx *ctx, double *ms, int n);
+/* Total active particles (sum of np2). */
+int wm_particle_count(wm_ctx *ctx, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,20 @@
+#!/bin/bash
+# Compiles the reference's own particle and sort modules, exactly as they lie in
+# the reference tree, into oracle/_ref/libwmref.so.  Nothing else is compiled:
+# field.f90 and boundary_periodic.f90 `use mpi`, and this image has no MPI
+# module in the compiler's format, so those two files are not buildable here.
+# The library is single-threaded (no -fopenmp) and is test infrastructure only.
+set -euo pipefail
+REF=${WM_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+if [ ! -f "$REF/common/particle.f90" ]; then
+  echo "build_ref: no reference tree at $REF; skipping" >&2
+  exit 0
+fi
+mkdir -p "$OUT"
+amdflang -O2 -fPIC -shared -module-dir "$OUT" \
+  "$REF/common/particle.f90" "$REF/common/sort.f90" \
+  -o "$OUT/libwmref.so.tmp"
+mv "$OUT/libwmref.so.tmp" "$OUT/libwmref.so"
+echo "build_ref: built $OUT/libwmref.so"

@@ -1,0 +1,47 @@
+"""Shared helpers of the test suite: contexts built the way the reference
+apps initialise their modules (proj/weibel/app.f90:331-347)."""
+import numpy as np
+
+import wumingpic2d_amd as wm
+from wumingpic2d_amd import weibel
+
+
+def grid_args(p):
+    return (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"],
+            p["nye"])
+
+
+def make_ctx(p, cap=0, mcap=0, init=("particle", "field", "sort", "bc")):
+    ctx = wm.Context(0)
+    if cap or mcap:
+        ctx.set_capacity(cap, mcap)
+    g = grid_args(p)
+    if "bc" in init:
+        ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
+    if "particle" in init:
+        ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
+    if "field" in init:
+        ctx.field__init(*g, 0, 0, 0, 0, p["delx"], p["delt"], p["c"], p["q"], p["r"], p["gfac"])
+    if "sort" in init:
+        ctx.sort__init(*g)
+    return ctx
+
+
+def state(nx, ny, ppc, seed, **kw):
+    p = weibel.grid_params(nx, ny, ppc, **{k: v for k, v in kw.items() if k in ("np_cap", "delt", "wpe")})
+    up, np2, cc, uf = weibel.init_state(p, seed, **{k: v for k, v in kw.items() if k in ("vth", "udrift")})
+    return p, up, np2, cc, uf
+
+
+def random_fields(uf, seed, amp=0.05):
+    rng = np.random.default_rng(seed)
+    uf[...] = rng.normal(0.0, amp, uf.shape)
+    return uf
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+def active(up, np2, jl, s, ndim=5):
+    return up[:ndim, :np2[jl, s], jl, s]

@@ -1,0 +1,316 @@
+"""wumingpic2d_amd -- MI355X-native hot path of WumingPIC2D.
+
+The package is a thin host layer over ``libwmpic.so`` (hand-written HIP
+kernels for gfx950 behind the C ABI of ``include/wmpic.h``).  ``Context``
+mirrors the reference's module procedures one for one (same names, argument
+order and meaning, same fatal-error messages raised as :class:`WmError`):
+
+=====================================  ======================================
+reference (WumingCode/WumingPIC2D)     Context method
+=====================================  ======================================
+common/particle.f90:18   particle__init          ``particle__init``
+common/particle.f90:48   particle__solv          ``particle__solv``
+common/field.f90:22      field__init             ``field__init``
+common/field.f90:66      field__fdtd_i           ``field__fdtd_i``
+common/sort.f90:16       sort__init              ``sort__init``
+common/sort.f90:36       sort__bucket            ``sort__bucket``
+common/boundary_periodic.f90:26  __init          ``boundary_periodic__init``
+common/boundary_periodic.f90:61  __particle_x    ``boundary_periodic__particle_x``
+common/boundary_periodic.f90:99  __particle_y    ``boundary_periodic__particle_y``
+common/boundary_periodic.f90:251 __dfield        ``boundary_periodic__dfield``
+common/boundary_periodic.f90:357 __curre         ``boundary_periodic__curre``
+common/boundary_periodic.f90:511 __phi           ``boundary_periodic__phi``
+common/mpi_set.f90:19    mpi_set__init           :func:`mpi_set__init`
+=====================================  ======================================
+
+Arrays are numpy arrays in Fortran order with the reference's shapes, e.g.
+``up`` is ``(ndim, np, ny, nsp)`` float64 and ``cumcnt`` is
+``(nx+1, ny, nsp)`` int32.  The device-resident methods (``state_upload``,
+``init_weibel``, ``step``, ``energy``) keep the whole run in HBM.
+
+There is no CPU fallback: importing works without a GPU, but every compute
+call needs the built library and a visible device and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwmpic.so")
+
+WM_OK = 0
+WM_ERR_NOT_INIT = 1
+WM_ERR_CG_ITEMAX = 2
+WM_ERR_NP_OVER = 3
+WM_ERR_RANGE = 4
+WM_ERR_ARG = 5
+WM_ERR_HIP = 6
+WM_ERR_NOMEM = 7
+WM_ERR_NODEV = 8
+BC_PERIODIC = 0
+
+# Symbols declared in include/wmpic.h (checked by the CPU test suite).
+EXPORTS = (
+    "wm_create", "wm_destroy", "wm_last_error", "wm_set_capacity", "wm_set_cg_tolerance",
+    "wm_mpi_set_init", "wm_particle_init", "wm_field_init", "wm_sort_init", "wm_bcp_init",
+    "wm_particle_solv", "wm_field_fdtd_i", "wm_bcp_particle_x", "wm_bcp_particle_y",
+    "wm_sort_bucket", "wm_bcp_dfield", "wm_bcp_curre", "wm_bcp_phi", "wm_state_upload",
+    "wm_state_download", "wm_state_init_weibel", "wm_step", "wm_energy", "wm_cg_iters",
+    "wm_phase_times", "wm_particle_count",
+)
+
+
+class WmError(RuntimeError):
+    """A non-zero status from libwmpic; ``code`` is the WM_ERR_* value."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load libwmpic.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise WmError(WM_ERR_NODEV, f"libwmpic is not built ({p} missing); run __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    _declare(lib)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    sig = {
+        "wm_create": [ctypes.POINTER(_P), _I],
+        "wm_destroy": [_P],
+        "wm_set_capacity": [_P, _I, _I],
+        "wm_set_cg_tolerance": [_P, _D],
+        "wm_mpi_set_init": [_I, _I, _I] + [ctypes.POINTER(_I)] * 6,
+        "wm_particle_init": [_P] + [_I] * 9 + [_D, _D, _D, _P, _P],
+        "wm_field_init": [_P] + [_I] * 13 + [_D, _D, _D, _P, _P, _D],
+        "wm_sort_init": [_P] + [_I] * 9,
+        "wm_bcp_init": [_P] + [_I] * 15 + [_P, _I, _D, _D, _D],
+        "wm_particle_solv": [_P, _P, _P, _P, _P, _I, _I],
+        "wm_field_fdtd_i": [_P, _P, _P, _P, _P, _I, _I, _I],
+        "wm_bcp_particle_x": [_P, _P, _P],
+        "wm_bcp_particle_y": [_P, _P, _P],
+        "wm_sort_bucket": [_P, _P, _P, _P, _P, _I, _I],
+        "wm_bcp_dfield": [_P, _P] + [_I] * 6,
+        "wm_bcp_curre": [_P, _P] + [_I] * 6,
+        "wm_bcp_phi": [_P, _P] + [_I] * 5,
+        "wm_state_upload": [_P, _P, _P, _P, _P],
+        "wm_state_download": [_P, _P, _P, _P, _P],
+        "wm_state_init_weibel": [_P, ctypes.c_uint64, _I, _D, _D],
+        "wm_step": [_P, _I],
+        "wm_energy": [_P, _P],
+        "wm_cg_iters": [_P, _P],
+        "wm_phase_times": [_P, _P, _I],
+        "wm_particle_count": [_P, ctypes.POINTER(ctypes.c_longlong)],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = _I
+    lib.wm_last_error.argtypes = [_P]
+    lib.wm_last_error.restype = ctypes.c_char_p
+
+
+def _ptr(a: np.ndarray | None, dtype, name: str):
+    if a is None:
+        return None
+    if not isinstance(a, np.ndarray) or a.dtype != np.dtype(dtype):
+        raise TypeError(f"{name} must be a numpy array of {np.dtype(dtype)}")
+    if not a.flags.f_contiguous:
+        raise TypeError(f"{name} must be Fortran-contiguous (the reference's column-major layout)")
+    return a.ctypes.data
+
+
+def mpi_set__init(nygs: int, nyge: int, nproc: int):
+    """Single-rank ``mpi_set__init`` (common/mpi_set.f90:19-49).
+
+    Returns ``dict(nys, nye, nup, ndown, nrank, nsize)``; ``nproc`` must be 1
+    (the reference aborts on a rank-count mismatch, mpi_set.f90:30-34)."""
+    lib = load_library()
+    out = [_I() for _ in range(6)]
+    st = lib.wm_mpi_set_init(nygs, nyge, nproc, *[ctypes.byref(o) for o in out])
+    if st != WM_OK:
+        raise WmError(st, "error in proc no.")
+    keys = ("nys", "nye", "nup", "ndown", "nrank", "nsize")
+    return {k: o.value for k, o in zip(keys, out)}
+
+
+class Context:
+    """One device, one grid: the reference's module SAVE state, in HBM."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = _P()
+        st = self._lib.wm_create(ctypes.byref(h), int(device))
+        if st != WM_OK:
+            raise WmError(st, f"no usable GPU {device} (status {st})")
+        self._h = h
+        self.device = device
+        self.grid: dict | None = None
+
+    # ------------------------------------------------------------ plumbing
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.wm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st: int) -> None:
+        if st != WM_OK:
+            msg = self._lib.wm_last_error(self._h)
+            raise WmError(st, msg.decode() if msg else f"status {st}")
+
+    def set_capacity(self, cap_per_row: int, mover_cap: int = 0) -> None:
+        """Device slots per (species, row) and migration buffer per row."""
+        self._check(self._lib.wm_set_capacity(self._h, int(cap_per_row), int(mover_cap)))
+
+    def set_cg_tolerance(self, err: float) -> None:
+        self._check(self._lib.wm_set_cg_tolerance(self._h, float(err)))
+
+    def _remember(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye):
+        self.grid = dict(ndim=ndim, np=np_, nsp=nsp, nxgs=nxgs, nxge=nxge, nygs=nygs,
+                         nyge=nyge, nys=nys, nye=nye, nx=nxge - nxgs + 1, ny=nye - nys + 1)
+
+    # ------------------------------------------------------ initialisers
+    def particle__init(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c,
+                       q: Sequence[float], r: Sequence[float]) -> None:
+        qa = np.ascontiguousarray(q, dtype=np.float64)
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.wm_particle_init(self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge,
+                                               nys, nye, delx, delt, c, qa.ctypes.data,
+                                               ra.ctypes.data))
+        self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+
+    def field__init(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, mnpr, ncomw, opsum,
+                    nerr, delx, delt, c, q, r, gfac) -> None:
+        qa = np.ascontiguousarray(q, dtype=np.float64)
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.wm_field_init(self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys,
+                                            nye, mnpr, ncomw, opsum, nerr, delx, delt, c,
+                                            qa.ctypes.data, ra.ctypes.data, gfac))
+        self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+
+    def sort__init(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye) -> None:
+        self._check(self._lib.wm_sort_init(self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys,
+                                           nye))
+        self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+
+    def boundary_periodic__init(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, nup,
+                                ndown, mnpi, mnpr, ncomw, nerr, nstat, delx, delt, c) -> None:
+        ns = np.ascontiguousarray(nstat, dtype=np.int32)
+        self._check(self._lib.wm_bcp_init(self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys,
+                                          nye, nup, ndown, mnpi, mnpr, ncomw, nerr,
+                                          ns.ctypes.data, ns.size, delx, delt, c))
+        self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+
+    # ------------------------------------------- host-array entry points
+    def particle__solv(self, gp, up, uf, cumcnt, nxs, nxe) -> None:
+        self._check(self._lib.wm_particle_solv(self._h, _ptr(gp, np.float64, "gp"),
+                                               _ptr(up, np.float64, "up"),
+                                               _ptr(uf, np.float64, "uf"),
+                                               _ptr(cumcnt, np.int32, "cumcnt"), nxs, nxe))
+
+    def field__fdtd_i(self, uf, up, gp, cumcnt, nxs, nxe, bc_family: int = BC_PERIODIC) -> None:
+        self._check(self._lib.wm_field_fdtd_i(self._h, _ptr(uf, np.float64, "uf"),
+                                              _ptr(up, np.float64, "up"),
+                                              _ptr(gp, np.float64, "gp"),
+                                              _ptr(cumcnt, np.int32, "cumcnt"), nxs, nxe,
+                                              bc_family))
+
+    def boundary_periodic__particle_x(self, up, np2) -> None:
+        self._check(self._lib.wm_bcp_particle_x(self._h, _ptr(up, np.float64, "up"),
+                                                _ptr(np2, np.int32, "np2")))
+
+    def boundary_periodic__particle_y(self, up, np2) -> None:
+        self._check(self._lib.wm_bcp_particle_y(self._h, _ptr(up, np.float64, "up"),
+                                                _ptr(np2, np.int32, "np2")))
+
+    def sort__bucket(self, gp, up, cumcnt, np2, nxs, nxe) -> None:
+        self._check(self._lib.wm_sort_bucket(self._h, _ptr(gp, np.float64, "gp"),
+                                             _ptr(up, np.float64, "up"),
+                                             _ptr(cumcnt, np.int32, "cumcnt"),
+                                             _ptr(np2, np.int32, "np2"), nxs, nxe))
+
+    def boundary_periodic__dfield(self, df, nxs, nxe, nys, nye, nxgs, nxge) -> None:
+        self._check(self._lib.wm_bcp_dfield(self._h, _ptr(df, np.float64, "df"), nxs, nxe, nys,
+                                            nye, nxgs, nxge))
+
+    def boundary_periodic__curre(self, uj, nxs, nxe, nys, nye, nxgs, nxge) -> None:
+        self._check(self._lib.wm_bcp_curre(self._h, _ptr(uj, np.float64, "uj"), nxs, nxe, nys,
+                                           nye, nxgs, nxge))
+
+    def boundary_periodic__phi(self, phi, nxs, nxe, nys, nye, l) -> None:
+        self._check(self._lib.wm_bcp_phi(self._h, _ptr(phi, np.float64, "phi"), nxs, nxe, nys,
+                                         nye, l))
+
+    # ------------------------------------------------- resident run
+    def state_upload(self, up, np2, cumcnt, uf) -> None:
+        self._check(self._lib.wm_state_upload(self._h, _ptr(up, np.float64, "up"),
+                                              _ptr(np2, np.int32, "np2"),
+                                              _ptr(cumcnt, np.int32, "cumcnt"),
+                                              _ptr(uf, np.float64, "uf")))
+
+    def state_download(self, up=None, np2=None, cumcnt=None, uf=None) -> None:
+        self._check(self._lib.wm_state_download(self._h, _ptr(up, np.float64, "up"),
+                                                _ptr(np2, np.int32, "np2"),
+                                                _ptr(cumcnt, np.int32, "cumcnt"),
+                                                _ptr(uf, np.float64, "uf")))
+
+    def init_weibel(self, seed: int, ppc: int, vth: float, udrift: float) -> None:
+        self._check(self._lib.wm_state_init_weibel(self._h, int(seed), int(ppc), float(vth),
+                                                   float(udrift)))
+
+    def step(self, nsteps: int = 1) -> None:
+        self._check(self._lib.wm_step(self._h, int(nsteps)))
+
+    def energy(self) -> np.ndarray:
+        nsp = self.grid["nsp"]
+        out = np.zeros(nsp + 2)
+        self._check(self._lib.wm_energy(self._h, out.ctypes.data))
+        return out
+
+    def cg_iters(self) -> np.ndarray:
+        out = np.zeros(3, dtype=np.int32)
+        self._check(self._lib.wm_cg_iters(self._h, out.ctypes.data))
+        return out
+
+    def phase_times(self) -> np.ndarray:
+        out = np.zeros(8)
+        n = self._lib.wm_phase_times(self._h, out.ctypes.data, 8)
+        return out[:max(n, 0)]
+
+    def particle_count(self) -> int:
+        c = ctypes.c_longlong()
+        self._check(self._lib.wm_particle_count(self._h, ctypes.byref(c)))
+        return c.value

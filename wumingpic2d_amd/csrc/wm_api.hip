@@ -1,0 +1,782 @@
+// wm_api.hip -- the C ABI of libwmpic: contexts, module initialisers, the
+// host-array entry points with the reference's call semantics, and the
+// device-resident step.
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "wm_internal.h"
+
+size_t wm_deposit_part_elems(int nx, int ny);
+size_t wm_cg_partials(int nx, int ny);
+
+void wm_set_error(wm_ctx *ctx, int code, const char *fmt, ...) {
+  if (!ctx) return;
+  ctx->status = code;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(ctx->msg, sizeof(ctx->msg), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+// --------------------------------------------------- layout conversion
+// AoS staging rows (seg, k) of ndim doubles <-> SoA banks.
+__global__ void k_aos_to_soa(int ndim, int w, int cap, const int *__restrict__ cnt,
+                             const double *__restrict__ st, PBank B, int64_t *id) {
+  const int seg = blockIdx.y;
+  const int n = cnt[seg];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const double *r = st + ((size_t)seg * w + k) * ndim;
+    const size_t p = (size_t)seg * cap + k;
+    B.x[p] = r[0];
+    B.y[p] = r[1];
+    B.ux[p] = r[2];
+    B.uy[p] = r[3];
+    B.uz[p] = r[4];
+    if (ndim >= 6) id[p] = __double_as_longlong(r[5]);
+  }
+}
+__global__ void k_soa_to_aos(int ndim, int w, int cap, const int *__restrict__ cnt,
+                             double *__restrict__ st, PBank B, const int64_t *id) {
+  const int seg = blockIdx.y;
+  const int n = cnt[seg];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    double *r = st + ((size_t)seg * w + k) * ndim;
+    const size_t p = (size_t)seg * cap + k;
+    r[0] = B.x[p];
+    r[1] = B.y[p];
+    r[2] = B.ux[p];
+    r[3] = B.uy[p];
+    r[4] = B.uz[p];
+    if (ndim >= 6) r[5] = __longlong_as_double(id[p]);
+  }
+}
+// Field arrays: component-fastest (ncomp, gx, gy) <-> component planes.
+__global__ void k_aos_to_planes(int ncomp, size_t n, size_t ps, const double *__restrict__ a,
+                                double *__restrict__ pl) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int c = 0; c < ncomp; ++c) pl[(size_t)c * ps + t] = a[t * ncomp + c];
+}
+__global__ void k_planes_to_aos(int ncomp, size_t n, size_t ps, double *__restrict__ a,
+                                const double *__restrict__ pl) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int c = 0; c < ncomp; ++c) a[t * ncomp + c] = pl[(size_t)c * ps + t];
+}
+__global__ void k_fill_int(int *a, size_t n, int v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+__global__ void k_fill_dbl(double *a, size_t n, double v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// ------------------------------------------------------------ helpers
+int nseg_of(const wm_ctx *ctx) { return ctx->g.nsp * ctx->g.ny; }
+
+int check_grid(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+               int nys, int nye, double delx, double delt, double c, const double *q,
+               const double *r, bool have_phys) {
+  if (ndim < 5 || ndim > 6 || np <= 0 || nsp <= 0 || nsp > WM_MAXSP || nxge < nxgs + 3 ||
+      nyge < nygs || nye < nys) {
+    wm_set_error(ctx, WM_ERR_ARG, "invalid grid: ndim=%d np=%d nsp=%d", ndim, np, nsp);
+    return WM_ERR_ARG;
+  }
+  if (nys != nygs || nye != nyge) {
+    wm_set_error(ctx, WM_ERR_ARG,
+                 "one device context holds the whole y range (nys=nygs, nye=nyge): got %d:%d of %d:%d",
+                 nys, nye, nygs, nyge);
+    return WM_ERR_ARG;
+  }
+  if (have_phys && delx != 1.0) {
+    // sort.f90:60,72 bins by int(x), i.e. assumes unit cells
+    wm_set_error(ctx, WM_ERR_ARG, "delx must be 1 (sort__bucket bins particles by int(x))");
+    return WM_ERR_ARG;
+  }
+  wm_grid g{};
+  g.ndim = ndim; g.np = np; g.nsp = nsp;
+  g.nxgs = nxgs; g.nxge = nxge; g.nygs = nygs; g.nyge = nyge; g.nys = nys; g.nye = nye;
+  g.nx = nxge - nxgs + 1;
+  g.ny = nye - nys + 1;
+  if (ctx->have_grid) {
+    const wm_grid &h = ctx->g;
+    if (h.ndim != ndim || h.np != np || h.nsp != nsp || h.nxgs != nxgs || h.nxge != nxge ||
+        h.nygs != nygs || h.nyge != nyge || h.nys != nys || h.nye != nye) {
+      wm_set_error(ctx, WM_ERR_ARG, "module initialisers disagree on the grid");
+      return WM_ERR_ARG;
+    }
+  } else {
+    g.delx = 1.0;
+    g.delt = 1.0;
+    g.c = 1.0;
+    g.gfac = 0.501;
+    for (int s = 0; s < WM_MAXSP; ++s) g.r[s] = 1.0;
+    ctx->g = g;
+    ctx->have_grid = true;
+  }
+  if (have_phys) {
+    ctx->g.delx = delx;
+    ctx->g.delt = delt;
+    ctx->g.c = c;
+    for (int s = 0; s < nsp; ++s) {
+      ctx->g.q[s] = q[s];
+      ctx->g.r[s] = r[s];
+    }
+  }
+  return WM_OK;
+}
+
+template <class T>
+int dmalloc(wm_ctx *ctx, T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+  if (e != hipSuccess) {
+    wm_set_error(ctx, WM_ERR_NOMEM, "device allocation of %zu bytes failed: %s", sizeof(T) * n,
+                 hipGetErrorString(e));
+    *p = nullptr;
+    return WM_ERR_NOMEM;
+  }
+  return WM_OK;
+}
+
+int alloc_bank(wm_ctx *ctx, PBank &b, size_t n) {
+  WM_TRY(dmalloc(ctx, &b.x, n));
+  WM_TRY(dmalloc(ctx, &b.y, n));
+  WM_TRY(dmalloc(ctx, &b.ux, n));
+  WM_TRY(dmalloc(ctx, &b.uy, n));
+  WM_TRY(dmalloc(ctx, &b.uz, n));
+  return WM_OK;
+}
+void free_bank(PBank &b) {
+  (void)hipFree(b.x);
+  (void)hipFree(b.y);
+  (void)hipFree(b.ux);
+  (void)hipFree(b.uy);
+  (void)hipFree(b.uz);
+  b = PBank{};
+}
+
+int fill_zero(wm_ctx *ctx, double *a, size_t n) {
+  hipLaunchKernelGGL(k_fill_dbl, dim3(wm_cdiv((long long)n, 256)), dim3(256), 0, ctx->stream, a,
+                     n, 0.0);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int ensure_stage(wm_ctx *ctx, size_t elems) {
+  if (ctx->stage_elems >= elems) return WM_OK;
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  ctx->stage = nullptr;
+  ctx->stage_elems = 0;
+  WM_TRY(dmalloc(ctx, &ctx->stage, elems));
+  ctx->stage_elems = elems;
+  return WM_OK;
+}
+
+// Upload the active records (cnt[seg] per segment) of a reference-layout
+// particle array into bank `b` and the current ID array.
+int upload_particles(wm_ctx *ctx, const double *up, const int *cnt_host, int b) {
+  const wm_grid &g = ctx->g;
+  const int nseg = nseg_of(ctx);
+  int w = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (cnt_host[s] < 0 || cnt_host[s] > g.np) {
+      wm_set_error(ctx, WM_ERR_RANGE, "particle count %d of row segment %d outside 0..np=%d",
+                   cnt_host[s], s, g.np);
+      return WM_ERR_RANGE;
+    }
+    if (cnt_host[s] > ctx->cap) {
+      wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np): %d particles, device capacity %d",
+                   cnt_host[s], ctx->cap);
+      return WM_ERR_NP_OVER;
+    }
+    w = std::max(w, cnt_host[s]);
+  }
+  w = std::max(w, 1);
+  const size_t elems = (size_t)nseg * w * g.ndim;
+  WM_TRY(ensure_stage(ctx, elems));
+  std::vector<double> h(elems);
+  for (int s = 0; s < nseg; ++s)
+    memcpy(&h[(size_t)s * w * g.ndim], up + (size_t)s * g.np * g.ndim,
+           sizeof(double) * (size_t)cnt_host[s] * g.ndim);
+  int *dcnt = ctx->cnt_scratch;
+  WM_HIP(hipMemcpyAsync(ctx->stage, h.data(), sizeof(double) * elems, hipMemcpyHostToDevice,
+                        ctx->stream));
+  WM_HIP(hipMemcpyAsync(dcnt, cnt_host, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
+                     ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
+                     ctx->id[ctx->idcur]);
+  WM_HIP(hipGetLastError());
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+// Write the active records of bank `b` (cnt_host per segment) into a
+// reference-layout host array; other slots are left untouched.
+int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b) {
+  const wm_grid &g = ctx->g;
+  const int nseg = nseg_of(ctx);
+  int w = 1;
+  for (int s = 0; s < nseg; ++s) w = std::max(w, cnt_host[s]);
+  if (w > g.np) {
+    wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np): %d > %d", w, g.np);
+    return WM_ERR_NP_OVER;
+  }
+  const size_t elems = (size_t)nseg * w * g.ndim;
+  WM_TRY(ensure_stage(ctx, elems));
+  int *dcnt = ctx->cnt_scratch;
+  WM_HIP(hipMemcpyAsync(dcnt, cnt_host, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
+                     ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
+                     ctx->id[ctx->idcur]);
+  WM_HIP(hipGetLastError());
+  std::vector<double> h(elems);
+  WM_HIP(hipMemcpyAsync(h.data(), ctx->stage, sizeof(double) * elems, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < nseg; ++s)
+    memcpy(up + (size_t)s * g.np * g.ndim, &h[(size_t)s * w * g.ndim],
+           sizeof(double) * (size_t)cnt_host[s] * g.ndim);
+  return WM_OK;
+}
+
+// Field array (ncomp, gx, gy) host <-> device planes.
+int upload_field(wm_ctx *ctx, const double *a, int ncomp, double *planes) {
+  const size_t n = (size_t)ctx->fg.gx * ctx->fg.gy;
+  WM_TRY(ensure_stage(ctx, n * ncomp));
+  WM_HIP(hipMemcpyAsync(ctx->stage, a, sizeof(double) * n * ncomp, hipMemcpyHostToDevice,
+                        ctx->stream));
+  hipLaunchKernelGGL(k_aos_to_planes, dim3(wm_cdiv((long long)n, 256)), dim3(256), 0, ctx->stream,
+                     ncomp, n, ctx->fg.ps, ctx->stage, planes);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+int download_field(wm_ctx *ctx, double *a, int ncomp, const double *planes) {
+  const size_t n = (size_t)ctx->fg.gx * ctx->fg.gy;
+  WM_TRY(ensure_stage(ctx, n * ncomp));
+  hipLaunchKernelGGL(k_planes_to_aos, dim3(wm_cdiv((long long)n, 256)), dim3(256), 0, ctx->stream,
+                     ncomp, n, ctx->fg.ps, ctx->stage, planes);
+  WM_HIP(hipGetLastError());
+  WM_HIP(hipMemcpyAsync(a, ctx->stage, sizeof(double) * n * ncomp, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int upload_cumcnt(wm_ctx *ctx, const int *cumcnt) {
+  const size_t n = (size_t)nseg_of(ctx) * (ctx->g.nx + 1);
+  // cumcnt must be a prefix table inside the row capacity (sort.f90:64-69)
+  for (int s = 0; s < nseg_of(ctx); ++s) {
+    const int *c = cumcnt + (size_t)s * (ctx->g.nx + 1);
+    bool ok = c[0] >= 0 && c[ctx->g.nx] <= std::min(ctx->g.np, ctx->cap);
+    for (int i = 0; ok && i < ctx->g.nx; ++i) ok = c[i + 1] >= c[i];
+    if (!ok) {
+      wm_set_error(ctx, WM_ERR_RANGE, "cumcnt of row segment %d is not a prefix table within capacity", s);
+      return WM_ERR_RANGE;
+    }
+  }
+  WM_HIP(hipMemcpyAsync(ctx->cumcnt, cumcnt, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  return WM_OK;
+}
+
+std::vector<int> counts_from_cumcnt(const wm_ctx *ctx, const int *cumcnt) {
+  const int nseg = nseg_of(ctx);
+  std::vector<int> c(nseg);
+  for (int s = 0; s < nseg; ++s) c[s] = cumcnt[(size_t)s * (ctx->g.nx + 1) + ctx->g.nx];
+  return c;
+}
+
+int require(wm_ctx *ctx, bool flag, const char *what) {
+  if (!flag) {
+    wm_set_error(ctx, WM_ERR_NOT_INIT, "Initialize first by calling %s()", what);
+    return WM_ERR_NOT_INIT;
+  }
+  return WM_OK;
+}
+
+int check_bounds(wm_ctx *ctx, int nxs, int nxe) {
+  if (nxs != ctx->g.nxgs || nxe != ctx->g.nxge) {
+    wm_set_error(ctx, WM_ERR_ARG, "nxs:nxe = %d:%d must equal nxgs:nxge = %d:%d on one rank", nxs,
+                 nxe, ctx->g.nxgs, ctx->g.nxge);
+    return WM_ERR_ARG;
+  }
+  return WM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- allocation
+int wm_alloc_device(wm_ctx *ctx) {
+  if (ctx->alloc_done) return WM_OK;
+  if (!ctx->have_grid) {
+    wm_set_error(ctx, WM_ERR_NOT_INIT, "Initialize first by calling particle__init()");
+    return WM_ERR_NOT_INIT;
+  }
+  const wm_grid &g = ctx->g;
+  if (ctx->cap <= 0) ctx->cap = g.np;
+  if (ctx->mcap <= 0) ctx->mcap = std::max(1024, ctx->cap / 8);
+  const int nseg = g.nsp * g.ny;
+  const size_t slots = (size_t)nseg * ctx->cap;
+  WM_TRY(alloc_bank(ctx, ctx->bank[0], slots));
+  WM_TRY(alloc_bank(ctx, ctx->bank[1], slots));
+  WM_TRY(dmalloc(ctx, &ctx->id[0], slots));
+  WM_TRY(dmalloc(ctx, &ctx->id[1], slots));
+  WM_TRY(dmalloc(ctx, &ctx->np2, nseg));
+  WM_TRY(dmalloc(ctx, &ctx->cumcnt, (size_t)nseg * (g.nx + 1)));
+  const size_t ms = (size_t)nseg * ctx->mcap;
+  WM_TRY(dmalloc(ctx, &ctx->hole, 2 * ms));
+  WM_TRY(dmalloc(ctx, &ctx->nhole, nseg));
+  WM_TRY(dmalloc(ctx, &ctx->nup, nseg));
+  WM_TRY(dmalloc(ctx, &ctx->ndn, nseg));
+  WM_TRY(alloc_bank(ctx, ctx->mv_up, ms));
+  WM_TRY(alloc_bank(ctx, ctx->mv_dn, ms));
+  WM_TRY(dmalloc(ctx, &ctx->mvid_up, ms));
+  WM_TRY(dmalloc(ctx, &ctx->mvid_dn, ms));
+
+  FieldGeom &fg = ctx->fg;
+  fg.nx = g.nx;
+  fg.ny = g.ny;
+  fg.gx = g.nx + 4;
+  fg.gy = g.ny + 4;
+  fg.ps = ((size_t)fg.gx * fg.gy + 63) / 64 * 64;
+  WM_TRY(dmalloc(ctx, &ctx->uf, 6 * fg.ps));
+  WM_TRY(dmalloc(ctx, &ctx->df, 6 * fg.ps));
+  WM_TRY(dmalloc(ctx, &ctx->uj, 3 * fg.ps));
+  WM_TRY(dmalloc(ctx, &ctx->tmp, 6 * (size_t)(g.nx + 2) * (g.ny + 2)));
+  const size_t n = (size_t)g.nx * g.ny;
+  WM_TRY(dmalloc(ctx, &ctx->gkl, 3 * n));
+  WM_TRY(dmalloc(ctx, &ctx->cg_r, 3 * n));
+  WM_TRY(dmalloc(ctx, &ctx->cg_p[0], 3 * n));
+  WM_TRY(dmalloc(ctx, &ctx->cg_s, 4));
+  WM_TRY(dmalloc(ctx, &ctx->red, wm_cg_partials(g.nx, g.ny)));
+  WM_TRY(dmalloc(ctx, &ctx->red_cnt, 8));
+  ctx->dep_part_elems = wm_deposit_part_elems(g.nx, g.ny);
+  WM_TRY(dmalloc(ctx, &ctx->dep_part, ctx->dep_part_elems));
+  WM_TRY(dmalloc(ctx, &ctx->diag, (size_t)nseg * 8 + 2 * (size_t)g.ny + 64));
+  WM_TRY(dmalloc(ctx, &ctx->derr, 1));
+  WM_TRY(dmalloc(ctx, &ctx->cnt_scratch, nseg));
+  WM_TRY(dmalloc(ctx, &ctx->fscratch, 6 * fg.ps));
+
+  WM_HIP(hipMemsetAsync(ctx->red_cnt, 0, 8 * sizeof(unsigned), ctx->stream));
+  WM_HIP(hipMemsetAsync(ctx->derr, 0, sizeof(int), ctx->stream));
+  WM_HIP(hipMemsetAsync(ctx->np2, 0, sizeof(int) * nseg, ctx->stream));
+  WM_HIP(hipMemsetAsync(ctx->cumcnt, 0, sizeof(int) * (size_t)nseg * (g.nx + 1), ctx->stream));
+  // field.f90:105-119: df, gkl, uj start at zero
+  WM_TRY(fill_zero(ctx, ctx->uf, 6 * fg.ps));
+  WM_TRY(fill_zero(ctx, ctx->df, 6 * fg.ps));
+  WM_TRY(fill_zero(ctx, ctx->uj, 3 * fg.ps));
+  WM_TRY(fill_zero(ctx, ctx->gkl, 3 * n));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->alloc_done = true;
+  return WM_OK;
+}
+
+int wm_check_derr(wm_ctx *ctx, const char *where) {
+  int h = 0;
+  WM_HIP(hipMemcpyAsync(&h, ctx->derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (!h) return WM_OK;
+  WM_HIP(hipMemsetAsync(ctx->derr, 0, sizeof(int), ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (h & (DERR_NP_OVER | DERR_MOVER_OVER)) {
+    wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np) in %s (%s)", where,
+                 (h & DERR_NP_OVER) ? "row capacity" : "migration buffer");
+    return WM_ERR_NP_OVER;
+  }
+  wm_set_error(ctx, WM_ERR_RANGE, "%s: particle outside the cells the reference indexes (%s)",
+               where, (h & DERR_JUMP) ? "moved more than one cell" : "cell index out of range");
+  return WM_ERR_RANGE;
+}
+
+// ================================================================== C ABI
+extern "C" {
+
+int wm_create(wm_ctx **out, int device) {
+  if (!out) return WM_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev)
+    return WM_ERR_NODEV;
+  wm_ctx *ctx = new wm_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return WM_ERR_NODEV;
+  }
+  for (int k = 0; k < 16; ++k) (void)hipEventCreate(&ctx->ev[k]);
+  *out = ctx;
+  return WM_OK;
+}
+
+int wm_destroy(wm_ctx *ctx) {
+  if (!ctx) return WM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_bank(ctx->bank[0]);
+  free_bank(ctx->bank[1]);
+  free_bank(ctx->mv_up);
+  free_bank(ctx->mv_dn);
+  void *ptrs[] = {ctx->id[0], ctx->id[1], ctx->np2, ctx->cumcnt, ctx->hole, ctx->nhole,
+                  ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
+                  ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_s, ctx->red,
+                  ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
+                  ctx->cnt_scratch, ctx->fscratch};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 16; ++k) (void)hipEventDestroy(ctx->ev[k]);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return WM_OK;
+}
+
+const char *wm_last_error(const wm_ctx *ctx) { return ctx ? ctx->msg : "no context"; }
+
+int wm_set_capacity(wm_ctx *ctx, int cap_per_row, int mover_cap) {
+  if (!ctx) return WM_ERR_ARG;
+  if (ctx->alloc_done) {
+    wm_set_error(ctx, WM_ERR_ARG, "capacity must be set before the first device call");
+    return WM_ERR_ARG;
+  }
+  if (cap_per_row < 0 || mover_cap < 0) return WM_ERR_ARG;
+  ctx->cap = cap_per_row;
+  ctx->mcap = mover_cap;
+  return WM_OK;
+}
+
+int wm_set_cg_tolerance(wm_ctx *ctx, double err) {
+  if (!ctx || !(err > 0.0)) return WM_ERR_ARG;
+  ctx->cg_err = err;
+  return WM_OK;
+}
+
+int wm_mpi_set_init(int nygs, int nyge, int nproc, int *nys, int *nye, int *nup, int *ndown,
+                    int *nrank, int *nsize) {
+  // mpi_set.f90:19-49 with nsize = 1 (one device holds the whole domain)
+  if (nproc != 1) return WM_ERR_ARG;   // mpi_set.f90:30-34 aborts on a mismatch
+  const int iwork1 = (nyge - nygs + 1) / 1;
+  *nrank = 0;
+  *nsize = 1;
+  *nys = 0 * iwork1 + nygs;
+  *nye = *nys + iwork1 - 1;
+  *nup = 0;     // nrank == nsize-1 -> periodic neighbour is rank 0
+  *ndown = 0;   // nrank == 0       -> nsize-1 = 0
+  return WM_OK;
+}
+
+int wm_particle_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs,
+                     int nyge, int nys, int nye, double delx, double delt, double c,
+                     const double *q, const double *r) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(check_grid(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r, true));
+  ctx->pinit = true;
+  return WM_OK;
+}
+
+int wm_field_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+                  int nys, int nye, int mnpr, int ncomw, int opsum, int nerr, double delx,
+                  double delt, double c, const double *q, const double *r, double gfac) {
+  (void)mnpr; (void)ncomw; (void)opsum; (void)nerr;
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(check_grid(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r, true));
+  ctx->g.gfac = gfac;
+  ctx->finit = true;
+  return WM_OK;
+}
+
+int wm_sort_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+                 int nys, int nye) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(check_grid(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, 1.0, 1.0, 1.0, nullptr,
+                    nullptr, false));
+  if (ctx->g.nx > 16384) {
+    wm_set_error(ctx, WM_ERR_ARG, "sort: nx=%d exceeds the 16384 cells one LDS histogram holds",
+                 ctx->g.nx);
+    return WM_ERR_ARG;
+  }
+  ctx->sinit = true;
+  return WM_OK;
+}
+
+int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+                int nys, int nye, int nup, int ndown, int mnpi, int mnpr, int ncomw, int nerr,
+                const int *nstat, int nstat_len, double delx, double delt, double c) {
+  (void)mnpi; (void)mnpr; (void)ncomw; (void)nerr; (void)nstat; (void)nstat_len;
+  if (!ctx) return WM_ERR_ARG;
+  if (nup != 0 || ndown != 0) {
+    wm_set_error(ctx, WM_ERR_ARG, "one device context is one periodic rank (nup = ndown = 0)");
+    return WM_ERR_ARG;
+  }
+  const double *q = ctx->have_grid ? ctx->g.q : nullptr;
+  const double *r = ctx->have_grid ? ctx->g.r : nullptr;
+  WM_TRY(check_grid(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r,
+                    q != nullptr));
+  if (!q) {
+    ctx->g.delx = delx;
+    ctx->g.delt = delt;
+    ctx->g.c = c;
+  }
+  ctx->binit = true;
+  return WM_OK;
+}
+
+// ------------------------------------------------- host-array entry points
+int wm_particle_solv(wm_ctx *ctx, double *gp, const double *up, const double *uf,
+                     const int *cumcnt, int nxs, int nxe) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->pinit, "particle__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(wm_alloc_device(ctx));
+  const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
+  WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
+  WM_TRY(upload_cumcnt(ctx, cumcnt));
+  WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
+  WM_TRY(wm_k_cell_fields(ctx));
+  WM_TRY(wm_k_push(ctx, 0, 1));
+  WM_TRY(wm_check_derr(ctx, "particle__solv"));
+  WM_TRY(download_particles(ctx, gp, cnt.data(), 1));
+  // gp(6,:,:,:) = up(6,:,:,:) over the whole capacity (particle.f90:171-175)
+  if (ctx->g.ndim == 6) {
+    const size_t n = (size_t)ctx->g.np * ctx->g.ny * ctx->g.nsp;
+    for (size_t t = 0; t < n; ++t) gp[6 * t + 5] = up[6 * t + 5];
+  }
+  return WM_OK;
+}
+
+int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
+                    const int *cumcnt, int nxs, int nxe, int bc_family) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->finit, "field__init"));
+  if (bc_family != WM_BC_PERIODIC) {
+    wm_set_error(ctx, WM_ERR_ARG, "boundary family %d is not available", bc_family);
+    return WM_ERR_ARG;
+  }
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(wm_alloc_device(ctx));
+  const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
+  WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
+  WM_TRY(upload_particles(ctx, gp, cnt.data(), 1));
+  WM_TRY(upload_cumcnt(ctx, cumcnt));
+  WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
+  WM_TRY(wm_k_deposit(ctx, 0, 1));
+  WM_TRY(wm_k_deposit_gather(ctx));
+  WM_TRY(wm_k_bc_curre(ctx, ctx->uj));
+  WM_TRY(wm_check_derr(ctx, "field__fdtd_i"));
+  WM_TRY(wm_k_field_solve(ctx));
+  WM_TRY(download_field(ctx, uf, 6, ctx->uf));
+  return WM_OK;
+}
+
+int wm_bcp_particle_x(wm_ctx *ctx, double *up, const int *np2) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(upload_particles(ctx, up, np2, 0));
+  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg_of(ctx), hipMemcpyHostToDevice,
+                        ctx->stream));
+  WM_TRY(wm_k_particle_x(ctx, 0));
+  WM_TRY(wm_check_derr(ctx, "boundary_periodic__particle_x"));
+  return download_particles(ctx, up, np2, 0);
+}
+
+int wm_bcp_particle_y(wm_ctx *ctx, double *up, int *np2) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(upload_particles(ctx, up, np2, 0));
+  const int nseg = nseg_of(ctx);
+  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(wm_k_particle_y(ctx, 0, false));
+  WM_TRY(wm_check_derr(ctx, "boundary_periodic__particle_y"));
+  std::vector<int> n2(nseg);
+  WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < nseg; ++s)
+    if (n2[s] > ctx->g.np) {
+      wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np) %d %d %d %d", ctx->g.np, n2[s],
+                   ctx->g.nys + s % ctx->g.ny, 1 + s / ctx->g.ny);
+      return WM_ERR_NP_OVER;
+    }
+  WM_TRY(download_particles(ctx, up, n2.data(), 0));
+  memcpy(np2, n2.data(), sizeof(int) * nseg);
+  return WM_OK;
+}
+
+int wm_sort_bucket(wm_ctx *ctx, double *gp, const double *up, int *cumcnt, const int *np2, int nxs,
+                   int nxe) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->sinit, "sort__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(wm_alloc_device(ctx));
+  const int nseg = nseg_of(ctx);
+  WM_TRY(upload_particles(ctx, up, np2, 1));
+  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(wm_k_sort(ctx, 1, 0));
+  WM_TRY(wm_check_derr(ctx, "sort__bucket"));
+  WM_HIP(hipMemcpyAsync(cumcnt, ctx->cumcnt, sizeof(int) * (size_t)nseg * (ctx->g.nx + 1),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return download_particles(ctx, gp, np2, 0);
+}
+
+int wm_bcp_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye, int nxgs,
+                  int nxge) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  if (nys != ctx->g.nys || nye != ctx->g.nye || nxgs != ctx->g.nxgs || nxge != ctx->g.nxge)
+    return WM_ERR_ARG;
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(upload_field(ctx, df, 6, ctx->fscratch));
+  WM_TRY(wm_k_bc_dfield(ctx, ctx->fscratch));
+  return download_field(ctx, df, 6, ctx->fscratch);
+}
+
+int wm_bcp_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye, int nxgs,
+                 int nxge) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  if (nys != ctx->g.nys || nye != ctx->g.nye || nxgs != ctx->g.nxgs || nxge != ctx->g.nxge)
+    return WM_ERR_ARG;
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(upload_field(ctx, uj, 3, ctx->fscratch));
+  WM_TRY(wm_k_bc_curre(ctx, ctx->fscratch));
+  return download_field(ctx, uj, 3, ctx->fscratch);
+}
+
+int wm_bcp_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye, int l) {
+  (void)l;
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  if (nys != ctx->g.nys || nye != ctx->g.nye) return WM_ERR_ARG;
+  WM_TRY(wm_alloc_device(ctx));
+  const size_t n = (size_t)(ctx->g.nx + 2) * (ctx->g.ny + 2);
+  WM_TRY(ensure_stage(ctx, n));
+  WM_HIP(hipMemcpyAsync(ctx->stage, phi, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(wm_k_bc_phi(ctx, ctx->stage));
+  WM_HIP(hipMemcpyAsync(phi, ctx->stage, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+// ------------------------------------------------------ resident state
+int wm_state_upload(wm_ctx *ctx, const double *up, const int *np2, const int *cumcnt,
+                    const double *uf) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->pinit, "particle__init"));
+  WM_TRY(wm_alloc_device(ctx));
+  const int nseg = nseg_of(ctx);
+  WM_TRY(upload_particles(ctx, up, np2, 0));
+  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(upload_cumcnt(ctx, cumcnt));
+  WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int wm_state_download(wm_ctx *ctx, double *up, int *np2, int *cumcnt, double *uf) {
+  if (!ctx) return WM_ERR_ARG;
+  if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
+  const int nseg = nseg_of(ctx);
+  std::vector<int> n2(nseg);
+  WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
+  if (cumcnt)
+    WM_HIP(hipMemcpyAsync(cumcnt, ctx->cumcnt, sizeof(int) * (size_t)nseg * (ctx->g.nx + 1),
+                          hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (np2) memcpy(np2, n2.data(), sizeof(int) * nseg);
+  if (up) WM_TRY(download_particles(ctx, up, n2.data(), 0));
+  if (uf) WM_TRY(download_field(ctx, uf, 6, ctx->uf));
+  return WM_OK;
+}
+
+int wm_state_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udrift) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->pinit, "particle__init"));
+  WM_TRY(require(ctx, ctx->sinit, "sort__init"));
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(wm_k_init_weibel(ctx, seed, ppc, vth, udrift));
+  WM_TRY(wm_k_sort(ctx, 1, 0));
+  WM_TRY(fill_zero(ctx, ctx->uf, 6 * ctx->fg.ps));
+  WM_TRY(wm_check_derr(ctx, "initial state"));
+  return WM_OK;
+}
+
+int wm_step(wm_ctx *ctx, int nsteps) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->pinit, "particle__init"));
+  WM_TRY(require(ctx, ctx->finit, "field__init"));
+  WM_TRY(require(ctx, ctx->sinit, "sort__init"));
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(wm_alloc_device(ctx));
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int it = 0; it < nsteps; ++it) {
+    hipEvent_t *e = ctx->ev;
+    WM_HIP(hipEventRecord(e[0], ctx->stream));
+    // particle__solv + ele_cur (proj/weibel/app.f90:102, field.f90:121)
+    WM_TRY(wm_k_cell_fields(ctx));
+    WM_TRY(wm_k_push_deposit(ctx, 0, 1));
+    WM_HIP(hipEventRecord(e[1], ctx->stream));
+    WM_TRY(wm_k_deposit_gather(ctx));
+    WM_TRY(wm_k_bc_curre(ctx, ctx->uj));          // field.f90:122
+    WM_HIP(hipEventRecord(e[2], ctx->stream));
+    WM_TRY(wm_k_field_solve(ctx));                // field.f90:125-184
+    WM_HIP(hipEventRecord(e[3], ctx->stream));
+    WM_TRY(wm_k_particle_y(ctx, 1, true));        // app.f90:104-105 (x fused into y pass 1)
+    WM_HIP(hipEventRecord(e[4], ctx->stream));
+    WM_TRY(wm_k_sort(ctx, 1, 0));                 // app.f90:106
+    WM_HIP(hipEventRecord(e[5], ctx->stream));
+    WM_HIP(hipEventSynchronize(e[5]));
+    for (int k = 0; k < 5; ++k) {
+      float ms = 0.f;
+      WM_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+      acc[k] += ms;
+    }
+    WM_TRY(wm_check_derr(ctx, "step"));
+  }
+  for (int k = 0; k < 5; ++k) ctx->phase_ms[k] = acc[k];
+  return WM_OK;
+}
+
+int wm_energy(wm_ctx *ctx, double *out) {
+  if (!ctx || !out) return WM_ERR_ARG;
+  if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
+  return wm_k_energy(ctx, 0, out);
+}
+
+int wm_cg_iters(wm_ctx *ctx, int *iters3) {
+  if (!ctx || !iters3) return WM_ERR_ARG;
+  for (int l = 0; l < 3; ++l) iters3[l] = ctx->last_iters[l];
+  return WM_OK;
+}
+
+int wm_phase_times(wm_ctx *ctx, double *ms, int n) {
+  if (!ctx || !ms) return WM_ERR_ARG;
+  const int m = std::min(n, 5);
+  for (int k = 0; k < m; ++k) ms[k] = ctx->phase_ms[k];
+  return m;
+}
+
+int wm_particle_count(wm_ctx *ctx, long long *count) {
+  if (!ctx || !count) return WM_ERR_ARG;
+  if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
+  const int nseg = nseg_of(ctx);
+  std::vector<int> n2(nseg);
+  WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  long long c = 0;
+  for (int v : n2) c += v;
+  *count = c;
+  return WM_OK;
+}
+
+}  // extern "C"

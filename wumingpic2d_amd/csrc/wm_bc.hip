@@ -1,0 +1,293 @@
+// wm_bc.hip -- periodic particle boundaries (boundary_periodic.f90:61-248).
+//
+// particle_x is elementwise.  particle_y keeps the reference's single-rank
+// result exactly, including the order particles end up in inside each row,
+// with two kernels over (species, row) segments:
+//  pass 1 (one workgroup per segment, in index order): leavers get their y
+//   wrapped in round-down arithmetic and are compacted, order preserving,
+//   into per-row "up" and "down" mover lists; their slots become the row's
+//   ascending hole list (the reference's flag(:,j), :143-169);
+//  pass 2 (one workgroup per segment): the arrival list of row j is the
+//   concatenation the self-SENDRECVs produce (:174-189) -- for rows other
+//   than nys, up-movers of row j-1 then down-movers of row j+1; for row nys,
+//   down-movers of row nys+1 then up-movers of row nye -- and the hole
+//   filling of :193-236 is evaluated in closed form: hole k takes arrival k;
+//   surplus arrivals are appended; with fewer arrivals than holes the row
+//   shrinks by m = H-A to L = np2-m, and the k-th remaining hole below L
+//   takes the k-th non-hole slot of [L, np2) counted from the top.
+#include "wm_internal.h"
+
+namespace {
+
+struct BP {
+  int nx, ny, nxgs, nxge, nygs, nyge, nys, nsp, cap, mcap;
+  double delx, spanx, spany;
+};
+
+__device__ inline double pred_d(double s) {
+  // largest double below s (s finite)
+  if (s == 0.0) return -__longlong_as_double(1LL);
+  long long b = __double_as_longlong(s);
+  b += (s > 0.0) ? -1 : 1;
+  return __longlong_as_double(b);
+}
+
+// a + b rounded toward -inf (boundary_periodic.f90:74, 124), exact via TwoSum.
+__device__ inline double add_rd(double a, double b) {
+  const double s = __dadd_rn(a, b);
+  const double bb = __dsub_rn(s, a);
+  const double err = __dadd_rn(__dsub_rn(a, __dsub_rn(s, bb)), __dsub_rn(b, bb));
+  return err < 0.0 ? pred_d(s) : s;
+}
+
+__device__ inline int wrap_x(double &x, const BP &P) {
+  const int ipos = (int)(x / P.delx);
+  if (ipos < P.nxgs) x = add_rd(x, P.spanx);
+  else if (ipos >= P.nxge + 1) x = add_rd(x, -P.spanx);
+  return ipos;
+}
+
+__global__ void k_particle_x(BP P, const int *__restrict__ np2, double *__restrict__ x,
+                             int *derr) {
+  const int seg = blockIdx.y;
+  const int n = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  bool bad = false;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    double v = x[base + k];
+    const double v0 = v;
+    wrap_x(v, P);
+    if (v != v0) x[base + k] = v;
+    const int inew = (int)(v / P.delx);
+    bad = bad || inew < P.nxgs || inew > P.nxge;
+  }
+  if (bad) atomicOr(derr, DERR_RANGE);
+}
+
+constexpr int YB = 256;
+
+// Exclusive prefix of a per-lane flag over the workgroup; returns the rank
+// and writes the workgroup total into *tot (all threads).
+__device__ inline int block_rank(bool f, int *sh_w, int *tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(f);
+  const int below = __popcll(m & ((1ULL << lane) - 1ULL));
+  if (lane == 0) sh_w[w] = __popcll(m);
+  __syncthreads();
+  int off = 0, all = 0;
+  for (int k = 0; k < YB / 64; ++k) {
+    const int c = sh_w[k];
+    off += (k < w) ? c : 0;
+    all += c;
+  }
+  *tot = all;
+  __syncthreads();
+  return off + below;
+}
+
+// pass 1
+__global__ __launch_bounds__(YB) void k_particle_y1(
+    BP P, bool fuse_x, const int *__restrict__ np2, double *__restrict__ x,
+    double *__restrict__ y, const double *__restrict__ ux, const double *__restrict__ uy,
+    const double *__restrict__ uz, const int64_t *__restrict__ id, int *__restrict__ hole,
+    int *__restrict__ nhole, int *__restrict__ nup, int *__restrict__ ndn, PBank mu,
+    int64_t *mid_u, PBank md, int64_t *mid_d, int *derr) {
+  __shared__ int sh[3][YB / 64];
+  const int seg = blockIdx.x;
+  const int jl = seg % P.ny;
+  const int j = P.nys + jl;
+  const int n = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  const size_t mb = (size_t)seg * P.mcap;
+  int ch = 0, cu = 0, cd = 0;
+  int bad = 0;
+  for (int k0 = 0; k0 < n; k0 += YB) {
+    const int k = k0 + threadIdx.x;
+    bool up = false, dn = false;
+    double xv = 0, yv = 0;
+    if (k < n) {
+      xv = x[base + k];
+      yv = y[base + k];
+      if (fuse_x) {
+        const double x0 = xv;
+        wrap_x(xv, P);
+        if (xv != x0) x[base + k] = xv;
+        const int inew = (int)(xv / P.delx);
+        if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
+      }
+      const int jpos = (int)(yv / P.delx);
+      if (jpos != j) {
+        if (jpos == j + 1) up = true;
+        else if (jpos == j - 1) dn = true;
+        else bad |= DERR_JUMP;
+        if (jpos <= P.nygs - 1) yv = add_rd(yv, P.spany);
+        else if (jpos >= P.nyge + 1) yv = add_rd(yv, -P.spany);
+      }
+    }
+    int th, tu, td;
+    const int rh = block_rank(up || dn, sh[0], &th);
+    const int ru = block_rank(up, sh[1], &tu);
+    const int rd = block_rank(dn, sh[2], &td);
+    if (up || dn) {
+      const int hpos = ch + rh;
+      if (hpos < 2 * P.mcap) hole[2 * mb + hpos] = k;
+      else bad |= DERR_MOVER_OVER;
+    }
+    if (up || dn) {
+      const int pos = up ? cu + ru : cd + rd;
+      if (pos < P.mcap) {
+        const PBank &B = up ? mu : md;
+        int64_t *bid = up ? mid_u : mid_d;
+        const size_t q = mb + pos;
+        B.x[q] = xv;
+        B.y[q] = yv;
+        B.ux[q] = ux[base + k];
+        B.uy[q] = uy[base + k];
+        B.uz[q] = uz[base + k];
+        bid[q] = id[base + k];
+      } else {
+        bad |= DERR_MOVER_OVER;
+      }
+    }
+    ch += th;
+    cu += tu;
+    cd += td;
+  }
+  if (threadIdx.x == 0) {
+    nhole[seg] = min(ch, 2 * P.mcap);
+    nup[seg] = min(cu, P.mcap);
+    ndn[seg] = min(cd, P.mcap);
+  }
+  if (bad) atomicOr(derr, bad);
+}
+
+// pass 2
+__global__ __launch_bounds__(YB) void k_particle_y2(
+    BP P, int *__restrict__ np2, double *__restrict__ x, double *__restrict__ y,
+    double *__restrict__ ux, double *__restrict__ uy, double *__restrict__ uz,
+    int64_t *__restrict__ id, const int *__restrict__ hole, const int *__restrict__ nhole,
+    const int *__restrict__ nup, const int *__restrict__ ndn, PBank mu,
+    const int64_t *mid_u, PBank md, const int64_t *mid_d, int *derr) {
+  __shared__ int sh[YB / 64];
+  const int seg = blockIdx.x;
+  const int s = seg / P.ny, jl = seg % P.ny;
+  const int below = s * P.ny + (jl == 0 ? P.ny - 1 : jl - 1);
+  const int above = s * P.ny + (jl == P.ny - 1 ? 0 : jl + 1);
+  // arrival lists: first F, then S
+  const bool first_row = (jl == 0);
+  const PBank &FB = first_row ? md : mu;
+  const int64_t *Fid = first_row ? mid_d : mid_u;
+  const int Fseg = first_row ? above : below;
+  const int nF = first_row ? ndn[above] : nup[below];
+  const PBank &SB = first_row ? mu : md;
+  const int64_t *Sid = first_row ? mid_u : mid_d;
+  const int Sseg = first_row ? below : above;
+  const int nS = first_row ? nup[below] : ndn[above];
+  const int A = nF + nS;
+  const int H = nhole[seg];
+  const int N = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  const int *hl = hole + 2 * (size_t)seg * P.mcap;
+
+  auto put_arrival = [&](int a, int dst) {
+    const bool f = a < nF;
+    const PBank &B = f ? FB : SB;
+    const int64_t *bid = f ? Fid : Sid;
+    const size_t q = (size_t)(f ? Fseg : Sseg) * P.mcap + (f ? a : a - nF);
+    const size_t d = base + dst;
+    x[d] = B.x[q];
+    y[d] = B.y[q];
+    ux[d] = B.ux[q];
+    uy[d] = B.uy[q];
+    uz[d] = B.uz[q];
+    id[d] = bid[q];
+  };
+
+  const int newn = N + A - H;
+  if (newn > P.cap) {
+    if (threadIdx.x == 0) atomicOr(derr, DERR_NP_OVER);
+    return;
+  }
+  // holes filled by arrivals, and surplus arrivals appended
+  for (int a = threadIdx.x; a < A; a += YB) put_arrival(a, a < H ? hl[a] : N + (a - H));
+  if (A < H) {
+    const int m = H - A;
+    const int L = N - m;
+    // holes >= L form the suffix hl[A + t .. H) of the sorted list
+    for (int q0 = 0; q0 < m; q0 += YB) {
+      const int qi = q0 + threadIdx.x;          // position N-1-qi, descending
+      bool nonhole = false;
+      int src = 0;
+      if (qi < m) {
+        src = N - 1 - qi;
+        // binary search src in hl[A..H)
+        int lo = A, hi = H;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (hl[mid] < src) lo = mid + 1;
+          else hi = mid;
+        }
+        nonhole = !(lo < H && hl[lo] == src);
+      }
+      int tot;
+      const int rank = block_rank(nonhole, sh, &tot);
+      // the nonholes before this tile
+      __shared__ int base_rank;
+      if (q0 == 0 && threadIdx.x == 0) base_rank = 0;
+      __syncthreads();
+      const int nrank = base_rank + rank;
+      if (nonhole && A + nrank < H) {
+        const int dst = hl[A + nrank];   // the nrank-th remaining hole (< L)
+        const size_t sd = base + src, dd = base + dst;
+        x[dd] = x[sd];
+        y[dd] = y[sd];
+        ux[dd] = ux[sd];
+        uy[dd] = uy[sd];
+        uz[dd] = uz[sd];
+        id[dd] = id[sd];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) base_rank += tot;
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) np2[seg] = newn;
+}
+
+BP make_bp(const wm_ctx *ctx) {
+  const wm_grid &g = ctx->g;
+  BP P;
+  P.nx = g.nx; P.ny = g.ny; P.nxgs = g.nxgs; P.nxge = g.nxge; P.nygs = g.nygs;
+  P.nyge = g.nyge; P.nys = g.nys; P.nsp = g.nsp; P.cap = ctx->cap; P.mcap = ctx->mcap;
+  P.delx = g.delx;
+  P.spanx = (double)(g.nxge - g.nxgs + 1) * g.delx;
+  P.spany = (double)(g.nyge - g.nygs + 1) * g.delx;
+  return P;
+}
+
+}  // namespace
+
+int wm_k_particle_x(wm_ctx *ctx, int bank) {
+  BP P = make_bp(ctx);
+  const int nseg = ctx->g.nsp * ctx->g.ny;
+  const int nb = std::min(64, std::max(1, wm_cdiv(ctx->cap, 256)));
+  hipLaunchKernelGGL(k_particle_x, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->np2,
+                     ctx->bank[bank].x, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x) {
+  BP P = make_bp(ctx);
+  const int nseg = ctx->g.nsp * ctx->g.ny;
+  const PBank &B = ctx->bank[bank];
+  int64_t *id = ctx->id[ctx->idcur];
+  hipLaunchKernelGGL(k_particle_y1, dim3(nseg), dim3(YB), 0, ctx->stream, P, fuse_x, ctx->np2,
+                     B.x, B.y, B.ux, B.uy, B.uz, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn,
+                     ctx->mv_up, ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
+  hipLaunchKernelGGL(k_particle_y2, dim3(nseg), dim3(YB), 0, ctx->stream, P, ctx->np2, B.x, B.y,
+                     B.ux, B.uy, B.uz, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
+                     ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}

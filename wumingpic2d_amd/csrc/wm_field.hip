@@ -1,0 +1,547 @@
+// wm_field.hip -- implicit-FDTD field solve (field.f90:66-186, 319-461) and
+// the periodic field/current/potential boundaries (boundary_periodic.f90).
+//
+// Fields are component planes of (nx+4) x (ny+4), x fastest.  The CG works
+// directly on the interior of the delta-B planes of df (which also carry the
+// previous step's solution, the reference's warm start field.f90:98,355) and
+// takes the periodic 1-cell halo of boundary_periodic__phi by index wrap.
+// The three Helmholtz solves are batched in one launch per sweep
+// (blockIdx.y = component); components that have converged skip their work.
+// Dot products are reduced deterministically: fixed per-workgroup trees, then
+// the last workgroup sums the partials in a fixed order and performs the
+// scalar update of the reference on the device.
+#include "wm_internal.h"
+
+#include <math.h>
+
+namespace {
+
+constexpr double PI = 3.141592653589793115997963468544185161590576171875;  // field.f90:13
+
+struct FC {      // field constants, field.f90:53-57
+  double f1, f2, f3, f4, f5, gfac, delt;
+};
+
+FC make_fc(const wm_grid &g) {
+  FC f;
+  f.f1 = g.c * g.delt / g.delx;
+  f.f2 = g.gfac * f.f1 * f.f1;
+  f.f3 = 4.0 * PI * g.delx / g.c;
+  const double t = g.delx / (g.c * g.delt * g.gfac);
+  f.f4 = 4.0 + t * t;
+  f.f5 = t * t;
+  f.gfac = g.gfac;
+  f.delt = g.delt;
+  return f;
+}
+
+// ------------------------------------------------------------ boundaries
+// boundary_periodic__curre (boundary_periodic.f90:357-506), row phase: one
+// thread per (column, component) runs the fold/copy sequence of that column.
+__global__ void k_curre_rows(FieldGeom fg, double *uj) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.gx * 3) return;
+  const int li = t % fg.gx, comp = t / fg.gx;
+  double *U = uj + (size_t)comp * fg.ps + li;
+  const size_t gx = fg.gx;
+  const int ny = fg.ny;
+  // rows: nys-2 -> 0, nys-1 -> 1, nys -> 2, nys+1 -> 3,
+  //       nye-1 -> ny, nye -> ny+1, nye+1 -> ny+2, nye+2 -> ny+3
+  const double r1 = U[0], r2 = U[gx];
+  U[ny * gx] = U[ny * gx] + r1;                 // :391-396
+  U[(ny + 1) * gx] = U[(ny + 1) * gx] + r2;
+  const double s1 = U[(ny + 2) * gx], s2 = U[(ny + 3) * gx];
+  U[2 * gx] = U[2 * gx] + s1;                   // :422-427
+  U[3 * gx] = U[3 * gx] + s2;
+  const double t1 = U[2 * gx], t2 = U[3 * gx];
+  U[(ny + 2) * gx] = t1;                        // :455-460
+  U[(ny + 3) * gx] = t2;
+  const double v1 = U[ny * gx], v2 = U[(ny + 1) * gx];
+  U[0] = v1;                                    // :486-491
+  U[gx] = v2;
+}
+
+// Column phase (boundary_periodic.f90:495-506), one thread per (row, comp).
+__global__ void k_curre_cols(FieldGeom fg, double *uj) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.gy * 3) return;
+  const int lj = t % fg.gy, comp = t / fg.gy;
+  double *R = uj + (size_t)comp * fg.ps + (size_t)lj * fg.gx;
+  const int nx = fg.nx;
+  // cols: nxs-2 -> 0, nxs-1 -> 1, nxs -> 2, nxs+1 -> 3,
+  //       nxe-1 -> nx, nxe -> nx+1, nxe+1 -> nx+2, nxe+2 -> nx+3
+  R[nx] = R[nx] + R[0];
+  R[nx + 1] = R[nx + 1] + R[1];
+  R[2] = R[2] + R[nx + 2];
+  R[3] = R[3] + R[nx + 3];
+  R[0] = R[nx];
+  R[1] = R[nx + 1];
+  R[nx + 2] = R[2];
+  R[nx + 3] = R[3];
+}
+
+// boundary_periodic__dfield (boundary_periodic.f90:263-345): ghost rows of
+// the interior columns.
+__global__ void k_dfield_rows(FieldGeom fg, double *df, int ncomp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.nx * ncomp) return;
+  const int li = 2 + t % fg.nx, comp = t / fg.nx;
+  double *U = df + (size_t)comp * fg.ps + li;
+  const size_t gx = fg.gx;
+  const int ny = fg.ny;
+  U[(ny + 2) * gx] = U[2 * gx];
+  U[(ny + 3) * gx] = U[3 * gx];
+  U[0] = U[ny * gx];
+  U[gx] = U[(ny + 1) * gx];
+}
+
+// boundary_periodic.f90:347-352: ghost columns of every row.
+__global__ void k_dfield_cols(FieldGeom fg, double *df, int ncomp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.gy * ncomp) return;
+  const int lj = t % fg.gy, comp = t / fg.gy;
+  double *R = df + (size_t)comp * fg.ps + (size_t)lj * fg.gx;
+  const int nx = fg.nx;
+  R[0] = R[nx];
+  R[1] = R[nx + 1];
+  R[nx + 2] = R[2];
+  R[nx + 3] = R[3];
+}
+
+// boundary_periodic__phi (boundary_periodic.f90:511-568) on an
+// (nx+2) x (ny+2) array: rows, then columns.
+__global__ void k_phi_rows(int nx, int ny, double *phi) {
+  const int a = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (a > nx) return;
+  const size_t w = nx + 2;
+  phi[a + (size_t)(ny + 1) * w] = phi[a + w];
+  phi[a] = phi[a + (size_t)ny * w];
+}
+__global__ void k_phi_cols(int nx, int ny, double *phi) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > ny + 1) return;
+  const size_t w = nx + 2;
+  phi[(size_t)b * w] = phi[(size_t)b * w + nx];
+  phi[(size_t)b * w + nx + 1] = phi[(size_t)b * w + 1];
+}
+
+// ----------------------------------------------------------- RHS, dE, +=
+// gkl (field.f90:125-146), interior cells, gkl planes nx x ny.
+__global__ void k_rhs(FieldGeom fg, FC f, const double *__restrict__ uf,
+                      const double *__restrict__ uj, double *__restrict__ gkl) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.nx * fg.ny) return;
+  const int ii = t % fg.nx, jj = t / fg.nx;
+  const size_t gx = fg.gx, ps = fg.ps;
+  const size_t c = (size_t)(ii + 2) + gx * (size_t)(jj + 2);
+  const double *B1 = uf, *B2 = uf + ps, *B3 = uf + 2 * ps;
+  const double *E1 = uf + 3 * ps, *E2 = uf + 4 * ps, *E3 = uf + 5 * ps;
+  const double *J1 = uj, *J2 = uj + ps, *J3 = uj + 2 * ps;
+  const size_t n = (size_t)fg.nx * fg.ny;
+  gkl[t] = f.f2 * (B1[c - gx] + B1[c - 1] - 4. * B1[c] + B1[c + 1] + B1[c + gx] +
+                   f.f3 * (-J3[c - gx] + J3[c])) -
+           f.f1 * (-E3[c - gx] + E3[c]);
+  gkl[n + t] = f.f2 * (B2[c - gx] + B2[c - 1] - 4. * B2[c] + B2[c + 1] + B2[c + gx] -
+                       f.f3 * (-J3[c - 1] + J3[c])) +
+               f.f1 * (-E3[c - 1] + E3[c]);
+  gkl[2 * n + t] =
+      f.f2 * (B3[c - gx] + B3[c - 1] - 4. * B3[c] + B3[c + 1] + B3[c + gx] +
+              f.f3 * (-J2[c - 1] + J2[c] + J1[c - gx] - J1[c])) -
+      f.f1 * (-E2[c - 1] + E2[c] + E1[c - gx] - E1[c]);
+}
+
+// delta-E (field.f90:154-171), interior cells.
+__global__ void k_delta_e(FieldGeom fg, FC f, const double *__restrict__ uf,
+                          const double *__restrict__ uj, double *__restrict__ df) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fg.nx * fg.ny) return;
+  const int ii = t % fg.nx, jj = t / fg.nx;
+  const size_t gx = fg.gx, ps = fg.ps;
+  const size_t c = (size_t)(ii + 2) + gx * (size_t)(jj + 2);
+  const double *dB1 = df, *dB2 = df + ps, *dB3 = df + 2 * ps;
+  const double *B1 = uf, *B2 = uf + ps, *B3 = uf + 2 * ps;
+  const double *J1 = uj, *J2 = uj + ps, *J3 = uj + 2 * ps;
+  const double fpt = 4. * PI * f.delt;
+  df[3 * ps + c] = f.f1 * (f.gfac * (-dB3[c] + dB3[c + gx]) + (-B3[c] + B3[c + gx])) - fpt * J1[c];
+  df[4 * ps + c] = -f.f1 * (f.gfac * (-dB3[c] + dB3[c + 1]) + (-B3[c] + B3[c + 1])) - fpt * J2[c];
+  df[5 * ps + c] = f.f1 * (f.gfac * (-dB2[c] + dB2[c + 1] + dB1[c] - dB1[c + gx]) +
+                           (-B2[c] + B2[c + 1] + B1[c] - B1[c + gx])) -
+                   fpt * J3[c];
+}
+
+// uf += df over every cell including ghosts (field.f90:176-184).
+__global__ void k_add(size_t n, double *__restrict__ uf, const double *__restrict__ df) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) uf[t] = uf[t] + df[t];
+}
+
+// -------------------------------------------------------------------- CG
+constexpr int CGB = 256;   // threads per CG workgroup
+
+struct CgArgs {
+  int nx, ny, gx, nblk;    // nblk = workgroups per component
+  size_t ps;               // df plane stride
+  size_t n;                // nx*ny
+  double f4, f5, err;
+  int ite_max;
+};
+
+// Fixed-order workgroup tree reduction of NV values; result in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double (*sh)[CGB]) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = CGB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = sh[k][0];
+}
+
+// Publish this workgroup's NV partials; returns true in every thread of the
+// workgroup that arrived last, after which `tot` holds the fixed-order sums.
+template <int NV>
+__device__ inline bool last_block_total(double (&v)[NV], double (*sh)[CGB],
+                                        double *partials, unsigned *counter,
+                                        int nblk, int bid, double (&tot)[NV]) {
+  block_sum<NV>(v, sh);
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * nblk + bid] = v[k];
+    __threadfence();
+    const unsigned ticket = atomicAdd(counter, 1u);
+    is_last = (ticket == (unsigned)nblk - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double a[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) a[k] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += CGB) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) a[k] += __builtin_nontemporal_load(&partials[(size_t)k * nblk + b]);
+  }
+  block_sum<NV>(a, sh);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = a[k];
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+__device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Initial residual (field.f90:350-385): b = f5*gkl, sum_g = |b|^2,
+// eps = sqrt(sum_g)*err, r = b + N4(phi) - f4*phi with periodic halo, p = r.
+__global__ __launch_bounds__(CGB) void k_cg_init(CgArgs A, const double *__restrict__ gkl,
+                                                 const double *__restrict__ df,
+                                                 double *__restrict__ r, double *__restrict__ p,
+                                                 double *partials, unsigned *counters,
+                                                 CgScal *S) {
+  __shared__ double sh[2][CGB];
+  const int l = blockIdx.y;
+  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (t < A.n) {
+    const int ii = (int)(t % A.nx), jj = (int)(t / A.nx);
+    const double *phi = df + (size_t)l * A.ps;
+    auto PH = [&](int a, int b) {
+      return phi[(size_t)(wrapm(a, A.nx) + 2) + (size_t)A.gx * (size_t)(wrapm(b, A.ny) + 2)];
+    };
+    const double b = A.f5 * gkl[(size_t)l * A.n + t];
+    const double rr = b + PH(ii, jj - 1) + PH(ii - 1, jj) - A.f4 * PH(ii, jj) +
+                      PH(ii + 1, jj) + PH(ii, jj + 1);
+    r[(size_t)l * A.n + t] = rr;
+    p[(size_t)l * A.n + t] = rr;
+    v[0] = b * b;
+    v[1] = rr * rr;
+  }
+  double tot[2];
+  if (last_block_total<2>(v, sh, partials + (size_t)l * 2 * A.nblk, counters + l, A.nblk,
+                          blockIdx.x, tot) &&
+      threadIdx.x == 0) {
+    CgScal s;
+    s.sum_g = tot[0];
+    s.eps = sqrt(tot[0]) * A.err;
+    s.sumr = tot[1];
+    s.sum2 = 0.0;
+    s.av = 0.0;
+    s.bv = 0.0;
+    s.ite = 0;
+    s.err = 0;
+    // the loop runs while sqrt(sumr) > eps (:385) and sum_g > eps (:387)
+    s.active = (sqrt(s.sumr) > s.eps) && (s.sum_g > s.eps);
+    s.pad = 0;
+    S[l] = s;
+  }
+}
+
+// p.Ap sweep (field.f90:395-415): av = sumr/sum2, sum_g = sqrt(sumr) (:426).
+__device__ inline double apply_a(double f4, double pc, double ps_, double pw, double pe,
+                                 double pn) {
+  // -p(i,j-1) - p(i-1,j) + f4*p(i,j) - p(i+1,j) - p(i,j+1), one rounding order
+  double a = __dsub_rn(-ps_, pw);
+  a = __fma_rn(f4, pc, a);
+  a = __dsub_rn(a, pe);
+  return __dsub_rn(a, pn);
+}
+
+__global__ __launch_bounds__(CGB) void k_cg_pap(CgArgs A, const double *__restrict__ p,
+                                                double *partials, unsigned *counters,
+                                                CgScal *S) {
+  __shared__ double sh[1][CGB];
+  const int l = blockIdx.y;
+  if (!S[l].active) return;
+  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  double v[1] = {0.0};
+  if (t < A.n) {
+    const int ii = (int)(t % A.nx), jj = (int)(t / A.nx);
+    const double *P = p + (size_t)l * A.n;
+    auto PP = [&](int a, int b) { return P[(size_t)wrapm(a, A.nx) + (size_t)A.nx * wrapm(b, A.ny)]; };
+    const double pc = P[t];
+    const double ap = apply_a(A.f4, pc, PP(ii, jj - 1), PP(ii - 1, jj), PP(ii + 1, jj), PP(ii, jj + 1));
+    v[0] = pc * ap;
+  }
+  double tot[1];
+  if (last_block_total<1>(v, sh, partials + (size_t)l * 2 * A.nblk, counters + l, A.nblk,
+                          blockIdx.x, tot) &&
+      threadIdx.x == 0) {
+    CgScal s = S[l];
+    s.ite = s.ite + 1;
+    s.sum2 = tot[0];
+    s.av = s.sumr / s.sum2;
+    s.sum_g = sqrt(s.sumr);
+    if (s.ite >= A.ite_max) s.err = 1;
+    S[l] = s;
+  }
+}
+
+// phi += av*p, r -= av*Ap, sum1 = |r|^2, bv = sum1/sumr (field.f90:417-442).
+__global__ __launch_bounds__(CGB) void k_cg_update(CgArgs A, double *__restrict__ df,
+                                                   double *__restrict__ r,
+                                                   const double *__restrict__ p,
+                                                   double *partials, unsigned *counters,
+                                                   CgScal *S) {
+  __shared__ double sh[1][CGB];
+  const int l = blockIdx.y;
+  if (!S[l].active) return;
+  const double av = S[l].av;
+  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  double v[1] = {0.0};
+  if (t < A.n) {
+    const int ii = (int)(t % A.nx), jj = (int)(t / A.nx);
+    const double *P = p + (size_t)l * A.n;
+    auto PP = [&](int a, int b) { return P[(size_t)wrapm(a, A.nx) + (size_t)A.nx * wrapm(b, A.ny)]; };
+    const double pc = P[t];
+    const double ap = apply_a(A.f4, pc, PP(ii, jj - 1), PP(ii - 1, jj), PP(ii + 1, jj), PP(ii, jj + 1));
+    double *phi = df + (size_t)l * A.ps + (size_t)(ii + 2) + (size_t)A.gx * (size_t)(jj + 2);
+    *phi = *phi + av * pc;
+    double *rp = r + (size_t)l * A.n + t;
+    const double rn = *rp - av * ap;
+    *rp = rn;
+    v[0] = rn * rn;
+  }
+  double tot[1];
+  if (last_block_total<1>(v, sh, partials + (size_t)l * 2 * A.nblk, counters + l, A.nblk,
+                          blockIdx.x, tot) &&
+      threadIdx.x == 0) {
+    CgScal s = S[l];
+    s.bv = tot[0] / s.sumr;
+    s.sumr = tot[0];                 // |r|^2 of the next iteration's sweep (:403)
+    s.active = (s.sum_g > s.eps) && !s.err;   // loop test (:387)
+    S[l] = s;
+  }
+}
+
+// p = r + bv*p (field.f90:444-450)
+__global__ __launch_bounds__(CGB) void k_cg_dir(CgArgs A, const double *__restrict__ r,
+                                                double *__restrict__ p, const CgScal *S,
+                                                const int *was_active) {
+  const int l = blockIdx.y;
+  if (!was_active[l]) return;
+  const double bv = S[l].bv;
+  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  if (t < A.n) p[(size_t)l * A.n + t] = r[(size_t)l * A.n + t] + bv * p[(size_t)l * A.n + t];
+}
+
+__global__ void k_snapshot_active(const CgScal *S, int *act) {
+  if (threadIdx.x < 3) act[threadIdx.x] = S[threadIdx.x].active;
+}
+
+// Energy sums (proj/weibel/app.f90:499-535): per-workgroup partials.
+__global__ __launch_bounds__(256) void k_energy_kin(int ny, int cap, int nsp_used,
+                                                    const int *np2, const double *ux,
+                                                    const double *uy, const double *uz,
+                                                    double c, double *part, int nchunk) {
+  __shared__ double sh[1][CGB];
+  const int seg = blockIdx.y;
+  const int n = np2[seg];
+  double v[1] = {0.0};
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += 256 * nchunk) {
+    const size_t p = (size_t)seg * cap + k;
+    const double u2 = ux[p] * ux[p] + uy[p] * uy[p] + uz[p] * uz[p];
+    v[0] += sqrt(1 + u2 / (c * c)) - 1;
+  }
+  block_sum<1>(v, sh);
+  if (threadIdx.x == 0) part[(size_t)seg * nchunk + blockIdx.x] = v[0];
+  (void)ny;
+  (void)nsp_used;
+}
+
+__global__ __launch_bounds__(256) void k_energy_fld(FieldGeom fg, const double *uf,
+                                                    double *part) {
+  __shared__ double sh[2][CGB];
+  const int jj = blockIdx.x;   // one workgroup per interior row
+  double v[2] = {0.0, 0.0};
+  for (int ii = threadIdx.x; ii < fg.nx; ii += 256) {
+    const size_t c = (size_t)(ii + 2) + (size_t)fg.gx * (size_t)(jj + 2);
+    const double b1 = uf[c], b2 = uf[fg.ps + c], b3 = uf[2 * fg.ps + c];
+    const double e1 = uf[3 * fg.ps + c], e2 = uf[4 * fg.ps + c], e3 = uf[5 * fg.ps + c];
+    v[0] += b1 * b1 + b2 * b2 + b3 * b3;
+    v[1] += e1 * e1 + e2 * e2 + e3 * e3;
+  }
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) {
+    part[2 * jj] = v[0];
+    part[2 * jj + 1] = v[1];
+  }
+}
+
+}  // namespace
+
+// -------------------------------------------------------------- launchers
+int wm_k_bc_curre(wm_ctx *ctx, double *uj) {
+  const FieldGeom &fg = ctx->fg;
+  hipLaunchKernelGGL(k_curre_rows, dim3(wm_cdiv(fg.gx * 3, 256)), dim3(256), 0, ctx->stream, fg, uj);
+  hipLaunchKernelGGL(k_curre_cols, dim3(wm_cdiv(fg.gy * 3, 256)), dim3(256), 0, ctx->stream, fg, uj);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_bc_dfield(wm_ctx *ctx, double *df) {
+  const FieldGeom &fg = ctx->fg;
+  hipLaunchKernelGGL(k_dfield_rows, dim3(wm_cdiv(fg.nx * 6, 256)), dim3(256), 0, ctx->stream, fg, df, 6);
+  hipLaunchKernelGGL(k_dfield_cols, dim3(wm_cdiv(fg.gy * 6, 256)), dim3(256), 0, ctx->stream, fg, df, 6);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_bc_phi(wm_ctx *ctx, double *phi) {
+  const int nx = ctx->g.nx, ny = ctx->g.ny;
+  hipLaunchKernelGGL(k_phi_rows, dim3(wm_cdiv(nx, 256)), dim3(256), 0, ctx->stream, nx, ny, phi);
+  hipLaunchKernelGGL(k_phi_cols, dim3(wm_cdiv(ny + 2, 256)), dim3(256), 0, ctx->stream, nx, ny, phi);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+size_t wm_cg_partials(int nx, int ny) {
+  const size_t n = (size_t)nx * ny;
+  return 3 * 2 * (size_t)wm_cdiv((long long)n, CGB);
+}
+
+// RHS -> CG -> bc_dfield -> dE -> bc_dfield -> uf += df (field.f90:125-184).
+// The current uj must already be folded.
+int wm_k_field_solve(wm_ctx *ctx) {
+  const wm_grid &g = ctx->g;
+  const FieldGeom &fg = ctx->fg;
+  const FC f = make_fc(g);
+  const int n = g.nx * g.ny;
+  hipLaunchKernelGGL(k_rhs, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, ctx->uf,
+                     ctx->uj, ctx->gkl);
+  WM_HIP(hipGetLastError());
+
+  CgArgs A;
+  A.nx = g.nx; A.ny = g.ny; A.gx = fg.gx; A.ps = fg.ps; A.n = (size_t)n;
+  A.nblk = wm_cdiv(n, CGB);
+  A.f4 = f.f4; A.f5 = f.f5; A.err = ctx->cg_err; A.ite_max = ctx->ite_max;
+  const dim3 grid(A.nblk, 3);
+  double *p = ctx->cg_p[0];
+  int *act = (int *)(ctx->cg_s + 3);    // 3 ints after the scalars
+  hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, ctx->stream, A, ctx->gkl, ctx->df,
+                     ctx->cg_r, p, ctx->red, ctx->red_cnt, ctx->cg_s);
+  WM_HIP(hipGetLastError());
+  CgScal hs[3];
+  int iters = 0;
+  for (;;) {
+    WM_HIP(hipMemcpyAsync(hs, ctx->cg_s, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    WM_HIP(hipStreamSynchronize(ctx->stream));
+    bool any = false;
+    for (int l = 0; l < 3; ++l) {
+      if (hs[l].err) {
+        wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
+        return WM_ERR_CG_ITEMAX;
+      }
+      any = any || hs[l].active;
+    }
+    if (!any) break;
+    if (++iters > ctx->ite_max + 1) {
+      wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
+      return WM_ERR_CG_ITEMAX;
+    }
+    hipLaunchKernelGGL(k_snapshot_active, dim3(1), dim3(64), 0, ctx->stream, ctx->cg_s, act);
+    hipLaunchKernelGGL(k_cg_pap, grid, dim3(CGB), 0, ctx->stream, A, p, ctx->red, ctx->red_cnt,
+                       ctx->cg_s);
+    hipLaunchKernelGGL(k_cg_update, grid, dim3(CGB), 0, ctx->stream, A, ctx->df, ctx->cg_r, p,
+                       ctx->red, ctx->red_cnt, ctx->cg_s);
+    hipLaunchKernelGGL(k_cg_dir, grid, dim3(CGB), 0, ctx->stream, A, ctx->cg_r, p, ctx->cg_s,
+                       act);
+    WM_HIP(hipGetLastError());
+  }
+  for (int l = 0; l < 3; ++l) ctx->last_iters[l] = hs[l].ite;
+  ctx->cg_iters_total += hs[0].ite + hs[1].ite + hs[2].ite;
+
+  WM_TRY(wm_k_bc_dfield(ctx, ctx->df));
+  hipLaunchKernelGGL(k_delta_e, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, ctx->uf,
+                     ctx->uj, ctx->df);
+  WM_HIP(hipGetLastError());
+  WM_TRY(wm_k_bc_dfield(ctx, ctx->df));
+  const size_t nall = 6 * fg.ps;
+  hipLaunchKernelGGL(k_add, dim3(wm_cdiv((long long)nall, 256)), dim3(256), 0, ctx->stream, nall,
+                     ctx->uf, ctx->df);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_energy(wm_ctx *ctx, int bank, double *out) {
+  const wm_grid &g = ctx->g;
+  const int nseg = g.nsp * g.ny;
+  const int nchunk = 8;
+  double *part = ctx->diag;
+  hipLaunchKernelGGL(k_energy_kin, dim3(nchunk, nseg), dim3(256), 0, ctx->stream, g.ny, ctx->cap,
+                     g.nsp, ctx->np2, ctx->bank[bank].ux, ctx->bank[bank].uy, ctx->bank[bank].uz,
+                     g.c, part, nchunk);
+  hipLaunchKernelGGL(k_energy_fld, dim3(g.ny), dim3(256), 0, ctx->stream, ctx->fg, ctx->uf,
+                     part + (size_t)nseg * nchunk);
+  WM_HIP(hipGetLastError());
+  const size_t np_ = (size_t)nseg * nchunk + 2 * (size_t)g.ny;
+  double *h = (double *)malloc(sizeof(double) * np_);
+  if (!h) return WM_ERR_NOMEM;
+  hipError_t e = hipMemcpyAsync(h, part, sizeof(double) * np_, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free(h);
+    WM_HIP(e);
+  }
+  for (int s = 0; s < g.nsp; ++s) {
+    double v = 0.0;
+    for (int jl = 0; jl < g.ny; ++jl)
+      for (int k = 0; k < nchunk; ++k) v += h[((size_t)s * g.ny + jl) * nchunk + k];
+    out[s] = g.r[s] * v;
+  }
+  double b = 0.0, ee = 0.0;
+  const double *fp = h + (size_t)nseg * nchunk;
+  for (int jl = 0; jl < g.ny; ++jl) {
+    b += fp[2 * jl];
+    ee += fp[2 * jl + 1];
+  }
+  out[g.nsp] = ee / (8 * PI);
+  out[g.nsp + 1] = b / (8 * PI);
+  free(h);
+  return WM_OK;
+}

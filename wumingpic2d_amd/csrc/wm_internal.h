@@ -1,0 +1,171 @@
+// wm_internal.h -- device layout, context and shared helpers of libwmpic.
+//
+// Device layout (MI355X, HBM-resident for the whole run):
+//   Particles: structure-of-arrays, one segment per (species s, row j) of
+//   `cap` slots, segment index seg = s*ny + (j - nys).  Slot k of a segment is
+//   the reference's particle index ii = k+1 of row j (proj/weibel/app.f90:281
+//   up(ndim,np,nys:nye,nsp)), so row-local orders, np2 and cumcnt keep the
+//   reference's meaning exactly.  Two position/momentum banks: bank 0 plays
+//   `up`, bank 1 plays `gp`.  IDs live in their own ping-pong pair.
+//   Fields: one plane per component, x fastest, ghost-inclusive
+//   (nx+4) x (ny+4) for uf/df/uj (the reference's nxgs-2:nxge+2,
+//   nys-2:nye+2 bounds), planes of (nx+2) x (ny+2) for the cell-centred
+//   push fields, nx x ny for the CG vectors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/wmpic.h"
+
+#define WM_MAXSP 8
+
+// ---------------------------------------------------------------- checking
+#define WM_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      wm_set_error(ctx, WM_ERR_HIP, "%s: %s (%s:%d)", #call,                 \
+                   hipGetErrorString(e_), __FILE__, __LINE__);               \
+      return WM_ERR_HIP;                                                     \
+    }                                                                        \
+  } while (0)
+
+#define WM_TRY(expr)          \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != WM_OK) return s_; \
+  } while (0)
+
+// Device-side error bits (set with vector atomics, read back by the host).
+enum : int {
+  DERR_RANGE = 1,     // particle outside the cells the reference indexes
+  DERR_NP_OVER = 2,   // a row exceeds the device capacity
+  DERR_MOVER_OVER = 4,// a row has more migrating particles than the buffer
+  DERR_JUMP = 8,      // a particle moved more than one cell in one step
+  DERR_CG_ITEMAX = 16 // CG reached ite_max
+};
+
+// ------------------------------------------------------------- parameters
+struct wm_grid {
+  int ndim, np, nsp;
+  int nxgs, nxge, nygs, nyge, nys, nye;
+  int nx, ny;
+  double delx, delt, c, gfac;
+  double q[WM_MAXSP], r[WM_MAXSP];
+};
+
+// Field plane geometry (ghost-inclusive, 2 ghost cells).
+struct FieldGeom {
+  int nx, ny;      // interior
+  int gx, gy;      // nx+4, ny+4
+  size_t ps;       // plane stride (elements)
+  // index of reference (i, j) in a ghost-inclusive plane
+  __host__ __device__ inline size_t at(int i, int j, int nxgs, int nys) const {
+    return (size_t)(i - (nxgs - 2)) + (size_t)gx * (size_t)(j - (nys - 2));
+  }
+};
+
+// CG per-component scalar state (device).
+struct CgScal {
+  double sum_g, eps, sumr, sum2, av, bv;
+  int active, ite, err, pad;
+};
+
+struct PBank {
+  double *x, *y, *ux, *uy, *uz;
+};
+
+struct wm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int status = WM_OK;
+  char msg[512] = {0};
+
+  // init flags and parameters (one grid per context)
+  bool pinit = false, finit = false, sinit = false, binit = false;
+  bool have_grid = false;
+  wm_grid g{};
+  double cg_err = 1e-6;    // field.f90:339
+  int ite_max = 100;       // field.f90:337
+
+  // capacities
+  int cap = 0;             // device slots per (species, row)
+  int mcap = 0;            // migrating particles per row and direction
+  bool alloc_done = false;
+
+  // particles
+  PBank bank[2]{};
+  int64_t *id[2]{};
+  int idcur = 0;           // which id array goes with bank 0 / bank 1 data
+  int *np2 = nullptr;      // [nseg]
+  int *cumcnt = nullptr;   // [nseg*(nx+1)]
+
+  // particle_y workspace
+  int *hole = nullptr;     // [nseg*2*mcap]
+  int *nhole = nullptr, *nup = nullptr, *ndn = nullptr;  // [nseg]
+  PBank mv_up{}, mv_dn{};
+  int64_t *mvid_up = nullptr, *mvid_dn = nullptr;
+
+  // fields
+  FieldGeom fg{};
+  double *uf = nullptr, *df = nullptr, *uj = nullptr;   // 6, 6, 3 planes
+  double *tmp = nullptr;   // 6 planes (nx+2)*(ny+2)
+  double *gkl = nullptr;   // 3 planes nx*ny
+  double *cg_phi = nullptr, *cg_r = nullptr, *cg_b = nullptr, *cg_ap = nullptr;
+  double *cg_p[2]{};
+  int cg_pcur = 0;
+  CgScal *cg_s = nullptr;  // [3]
+  double *red = nullptr;   // reduction partials
+  unsigned *red_cnt = nullptr;
+  double *dep_part = nullptr; // deposit tiles
+  size_t dep_part_elems = 0;
+  double *diag = nullptr;  // diagnostics scratch
+  int *derr = nullptr;     // device error bits
+  int *h_derr = nullptr;   // pinned mirror
+
+  // host staging (drop-in host-array entry points)
+  double *stage = nullptr; size_t stage_elems = 0;
+  int *stage_i = nullptr; size_t stage_ielems = 0;
+
+  // timing (per-phase HIP events of the last wm_step call)
+  hipEvent_t ev[16]{};
+  bool ev_ready = false;
+  int last_iters[3] = {0, 0, 0};
+  double phase_ms[8] = {0};
+  int *cnt_scratch = nullptr;    // [nseg] counts for layout conversion
+  double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
+  long long cg_iters_total = 0;
+};
+
+void wm_set_error(wm_ctx *ctx, int code, const char *fmt, ...);
+int wm_alloc_device(wm_ctx *ctx);
+int wm_check_derr(wm_ctx *ctx, const char *where);
+
+// ---------------------------------------------------- kernel launch layer
+// Particles (wm_particle.hip)
+int wm_k_cell_fields(wm_ctx *ctx);
+int wm_k_push(wm_ctx *ctx, int src, int dst);
+int wm_k_deposit(wm_ctx *ctx, int oldb, int newb);
+int wm_k_push_deposit(wm_ctx *ctx, int src, int dst);
+int wm_k_deposit_gather(wm_ctx *ctx);
+// Fields (wm_field.hip)
+int wm_k_bc_curre(wm_ctx *ctx, double *uj);
+int wm_k_bc_dfield(wm_ctx *ctx, double *df);
+int wm_k_bc_phi(wm_ctx *ctx, double *phi);
+int wm_k_field_solve(wm_ctx *ctx);   // RHS .. uf += df (after the current is ready)
+int wm_k_energy(wm_ctx *ctx, int bank, double *out_host);
+// Particle boundaries (wm_bc.hip)
+int wm_k_particle_x(wm_ctx *ctx, int bank);
+int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x);
+// Sort (wm_sort.hip)
+int wm_k_sort(wm_ctx *ctx, int src, int dst);
+// Synthetic initial state (wm_init.hip)
+int wm_k_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udrift);
+
+// --------------------------------------------------------- small helpers
+static inline int wm_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+#define WM_BLOCK 256

@@ -1,0 +1,376 @@
+// wm_particle.hip -- Buneman-Boris push and Esirkepov deposit (FP64 form).
+//
+// Work decomposition (the same for the push, the deposit and the fused
+// push+deposit kernel): one workgroup per (row j, chunk of CHUNK cells); the
+// workgroup's 256 lanes form 32 groups of 8 lanes, group g owns cell
+// i0+g and its 8 lanes stride through that cell's particles of every species
+// (the reference's cell -> species -> particle loop, particle.f90:85-94 and
+// field.f90:212-221).  Particles of one cell are contiguous (sorted), so each
+// group reads 64 contiguous bytes per array per iteration.
+//
+// Push fields come from a per-workgroup LDS tile of the cell-centred fields.
+// Deposit: each lane accumulates the 5x5 stencils of its particles in
+// registers (all in the same cell), the 8 lanes of a group combine by
+// butterfly shuffles, and the workgroup assembles its 5 x (CHUNK+4) current
+// tile in LDS in a fixed order.  Tiles go to a partial buffer that a gather
+// kernel sums in a fixed order, so the deposit is deterministic (no
+// floating-point atomics).
+#include "wm_internal.h"
+
+namespace {
+
+constexpr int CHUNK = 32;             // cells per workgroup
+constexpr int GROUP = 8;              // lanes per cell
+constexpr int TW = CHUNK + 2;         // push tile width (halo 1)
+constexpr int DW = CHUNK + 4;         // deposit tile width (halo 2)
+constexpr int TILE = 5 * DW * 3;      // doubles per deposit tile
+constexpr int NACC = 65;              // Jx 4x5 + Jy 5x4 + Jz 5x5
+
+struct PP {                           // particle-kernel parameters
+  int nx, ny, nxgs, nys, nsp, cap, nchunk;
+  double d_delx, delt, c, qc_scale;   // qc_scale = delx / delt
+  double q[WM_MAXSP], r[WM_MAXSP];
+};
+
+__device__ inline void shape(double dh, double &sm, double &s0, double &sp) {
+  // 2nd-order (TSC) weights: particle.f90:98-100
+  sm = 0.5 * (0.5 - dh) * (0.5 - dh);
+  s0 = 0.75 - dh * dh;
+  sp = 0.5 * (0.5 + dh) * (0.5 + dh);
+}
+
+// Cell-centred field tile: T[(k*3 + b)*TW + col], b = row j-1..j+1,
+// col = i - (i0 - 1).
+__device__ inline void load_tmp_tile(double *T, const double *tmp, int TX,
+                                     size_t tps, int jl, int c0, int ncol) {
+  // tmp planes hold cells (nxs-1 .. nxe+1) x (nys-1 .. nye+1); local column
+  // index of cell i is i - (nxs-1); c0 = local column of i0 - 1.
+  for (int t = threadIdx.x; t < 6 * 3 * TW; t += blockDim.x) {
+    const int col = t % TW;
+    const int b = (t / TW) % 3;
+    const int k = t / (3 * TW);
+    double v = 0.0;
+    if (col < ncol) v = tmp[(size_t)k * tps + (size_t)(c0 + col) + (size_t)TX * (size_t)(jl + b)];
+    T[t] = v;
+  }
+}
+
+struct PushOut { double x, y, ux, uy, uz; };
+
+__device__ inline PushOut boris(const double *T, int gcol, double x, double y,
+                                double ux, double uy, double uz, int i, int j,
+                                const PP &P, double fac1, double txxx, double fac2) {
+  // particle.f90:96-161
+  double sx0, sx1, sx2, sy0, sy1, sy2;
+  shape(x * P.d_delx - 0.5 - i, sx0, sx1, sx2);
+  shape(y * P.d_delx - 0.5 - j, sy0, sy1, sy2);
+  double f[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double *Tk = T + k * 3 * TW + gcol;   // gcol = column of i-1
+    f[k] = (Tk[0] * sx0 + Tk[1] * sx1 + Tk[2] * sx2) * sy0 +
+           (Tk[TW] * sx0 + Tk[TW + 1] * sx1 + Tk[TW + 2] * sx2) * sy1 +
+           (Tk[2 * TW] * sx0 + Tk[2 * TW + 1] * sx1 + Tk[2 * TW + 2] * sx2) * sy2;
+  }
+  const double bpx = f[0], bpy = f[1], bpz = f[2];
+  const double epx = f[3], epy = f[4], epz = f[5];
+  double uvm1 = ux + fac1 * epx;
+  double uvm2 = uy + fac1 * epy;
+  double uvm3 = uz + fac1 * epz;
+  const double gam = sqrt(P.c * P.c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+  const double igam = 1.0 / gam;
+  const double fac1r = fac1 * igam;
+  const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+  const double uvm4 = uvm1 + fac1r * (uvm2 * bpz - uvm3 * bpy);
+  const double uvm5 = uvm2 + fac1r * (uvm3 * bpx - uvm1 * bpz);
+  const double uvm6 = uvm3 + fac1r * (uvm1 * bpy - uvm2 * bpx);
+  uvm1 = uvm1 + fac2r * (uvm5 * bpz - uvm6 * bpy);
+  uvm2 = uvm2 + fac2r * (uvm6 * bpx - uvm4 * bpz);
+  uvm3 = uvm3 + fac2r * (uvm4 * bpy - uvm5 * bpx);
+  PushOut o;
+  o.ux = uvm1 + fac1 * epx;
+  o.uy = uvm2 + fac1 * epy;
+  o.uz = uvm3 + fac1 * epz;
+  const double ig = 1.0 / sqrt(1.0 + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) / (P.c * P.c));
+  o.x = x + o.ux * P.delt * ig;
+  o.y = y + o.uy * P.delt * ig;
+  return o;
+}
+
+// Esirkepov 2nd-order deposit of one particle into the 65 accumulators of
+// its cell (field.f90:224-298).  Returns false when the particle jumped more
+// than one cell.
+__device__ inline bool esirkepov(double *acc, double x0, double y0, double x1,
+                                 double y1, double ux, double uy, double uz,
+                                 int i, int j, double q, const PP &P) {
+  double s0x[5], s0y[5], dsx[5], dsy[5];
+  s0x[0] = s0x[4] = s0y[0] = s0y[4] = 0.0;
+  shape(x0 * P.d_delx - 0.5 - i, s0x[1], s0x[2], s0x[3]);
+  shape(y0 * P.d_delx - 0.5 - j, s0y[1], s0y[2], s0y[3]);
+  bool ok = true;
+  {
+    const int i2 = (int)(x1 * P.d_delx);
+    double a, b, c;
+    shape(x1 * P.d_delx - 0.5 - i2, a, b, c);
+    const int inc = i2 - i;
+    ok = ok && (inc >= -1 && inc <= 1);
+    // s1 placed at offset inc (the smo_1..3 selection of :244-251)
+    dsx[0] = inc < 0 ? a : 0.0;
+    dsx[1] = inc < 0 ? b : (inc == 0 ? a : 0.0);
+    dsx[2] = inc < 0 ? c : (inc == 0 ? b : a);
+    dsx[3] = inc > 0 ? b : (inc == 0 ? c : 0.0);
+    dsx[4] = inc > 0 ? c : 0.0;
+  }
+  {
+    const int j2 = (int)(y1 * P.d_delx);
+    double a, b, c;
+    shape(y1 * P.d_delx - 0.5 - j2, a, b, c);
+    const int inc = j2 - j;
+    ok = ok && (inc >= -1 && inc <= 1);
+    dsy[0] = inc < 0 ? a : 0.0;
+    dsy[1] = inc < 0 ? b : (inc == 0 ? a : 0.0);
+    dsy[2] = inc < 0 ? c : (inc == 0 ? b : a);
+    dsy[3] = inc > 0 ? b : (inc == 0 ? c : 0.0);
+    dsy[4] = inc > 0 ? c : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    dsx[k] -= s0x[k];
+    dsy[k] -= s0y[k];
+  }
+  const double gvz = uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
+  const double qc = q * P.qc_scale;      // q*delx*d_delt
+  // Jx: pjx(ip+1,jp) = pjx(ip,jp) - qc*ds_x(ip)*(s0_y(jp) + ds_y(jp)/2)
+  double Y[5], X[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    Y[k] = s0y[k] + 0.5 * dsy[k];
+    X[k] = s0x[k] + 0.5 * dsx[k];
+  }
+  double w = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    w -= qc * dsx[a];
+#pragma unroll
+    for (int b = 0; b < 5; ++b) acc[a * 5 + b] += w * Y[b];          // tile col a+1
+  }
+  w = 0.0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    w -= qc * dsy[b];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) acc[20 + a * 4 + b] += X[a] * w;     // tile row b+1
+  }
+  const double qv = q * gvz;
+  const double third = 1.0 / 3.0;
+#pragma unroll
+  for (int a = 0; a < 5; ++a) {
+    const double za = qv * X[a];
+    const double zb = qv * (0.5 * s0x[a] + third * dsx[a]);
+#pragma unroll
+    for (int b = 0; b < 5; ++b) acc[40 + a * 5 + b] += za * s0y[b] + zb * dsy[b];
+  }
+  return ok;
+}
+
+// Value of accumulator set `cb` (one cell) at tile offset (a, b) for comp.
+__device__ inline double acc_at(const double *cb, int comp, int a, int b) {
+  if (comp == 0) return a >= 1 ? cb[(a - 1) * 5 + b] : 0.0;
+  if (comp == 1) return b >= 1 ? cb[20 + a * 4 + (b - 1)] : 0.0;
+  return cb[40 + a * 5 + b];
+}
+
+template <bool PUSH, bool DEPOSIT>
+__global__ __launch_bounds__(256) void k_particle(
+    PP P, const int *__restrict__ cumcnt,
+    const double *__restrict__ tmp, int TX, size_t tps,
+    const double *__restrict__ xa, const double *__restrict__ ya,
+    const double *__restrict__ uxa, const double *__restrict__ uya,
+    const double *__restrict__ uza,
+    double *__restrict__ xb, double *__restrict__ yb,
+    double *__restrict__ uxb, double *__restrict__ uyb,
+    double *__restrict__ uzb,
+    double *__restrict__ part, int *__restrict__ derr) {
+  __shared__ double T[PUSH ? 6 * 3 * TW : 1];
+  __shared__ double cellbuf[DEPOSIT ? CHUNK * NACC : 1];
+  const int chunk = blockIdx.x;
+  const int jl = blockIdx.y;                 // row j = nys + jl
+  const int j = P.nys + jl;
+  const int i0 = P.nxgs + chunk * CHUNK;
+  const int ncell = min(CHUNK, P.nx - chunk * CHUNK);
+  if (PUSH) {
+    load_tmp_tile(T, tmp, TX, tps, jl, chunk * CHUNK, min(TW, ncell + 2));
+    __syncthreads();
+  }
+  const int g = threadIdx.x / GROUP;
+  const int sub = threadIdx.x % GROUP;
+  const int i = i0 + g;
+  double acc[DEPOSIT ? NACC : 1];
+  if (DEPOSIT) {
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+  }
+  bool ok = true;
+  if (g < ncell) {
+    for (int s = 0; s < P.nsp; ++s) {
+      const size_t seg = (size_t)s * P.ny + jl;
+      const int *cc = cumcnt + seg * (size_t)(P.nx + 1);
+      const int kb = max(0, cc[i - P.nxgs]), ke = min(P.cap, cc[i - P.nxgs + 1]);
+      const size_t base = seg * (size_t)P.cap;
+      const double q = P.q[s], r = P.r[s];
+      const double fac1 = q / r * 0.5 * P.delt;
+      const double txxx = fac1 * fac1;
+      const double fac2 = q * P.delt / r;
+      for (int k = kb + sub; k < ke; k += GROUP) {
+        const size_t p = base + (size_t)k;
+        double x1, y1, ux, uy, uz;
+        const double x0 = xa[p], y0 = ya[p];
+        if (PUSH) {
+          PushOut o = boris(T, g, x0, y0, uxa[p], uya[p], uza[p], i, j, P, fac1, txxx, fac2);
+          xb[p] = o.x; yb[p] = o.y; uxb[p] = o.ux; uyb[p] = o.uy; uzb[p] = o.uz;
+          x1 = o.x; y1 = o.y; ux = o.ux; uy = o.uy; uz = o.uz;
+        } else {
+          x1 = xb[p]; y1 = yb[p]; ux = uxb[p]; uy = uyb[p]; uz = uzb[p];
+        }
+        if (DEPOSIT) ok = esirkepov(acc, x0, y0, x1, y1, ux, uy, uz, i, j, q, P) && ok;
+      }
+    }
+  }
+  if (!ok) atomicOr(derr, DERR_JUMP);
+  if (DEPOSIT) {
+    // combine the 8 lanes of each group (butterfly: every lane gets the sum)
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) {
+      double v = acc[k];
+      v += __shfl_xor(v, 1, GROUP);
+      v += __shfl_xor(v, 2, GROUP);
+      v += __shfl_xor(v, 4, GROUP);
+      acc[k] = v;
+    }
+    if (sub == 0) {
+#pragma unroll
+      for (int k = 0; k < NACC; ++k) cellbuf[g * NACC + k] = acc[k];
+    }
+    __syncthreads();
+    // assemble the workgroup tile [b][t][comp] in a fixed order
+    double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;
+    for (int e = threadIdx.x; e < TILE; e += blockDim.x) {
+      const int comp = e % 3;
+      const int t = (e / 3) % DW;
+      const int b = e / (3 * DW);
+      double sum = 0.0;
+      const int glo = max(0, t - 4), ghi = min(ncell - 1, t);
+      for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(cellbuf + gg * NACC, comp, t - gg, b);
+      out[e] = sum;
+    }
+  }
+}
+
+// Sum of the deposit tiles into uj (ghost-inclusive), fixed order:
+// source row offset b ascending, chunk ascending.
+__global__ void k_deposit_gather(PP P, FieldGeom fg, const double *__restrict__ part,
+                                 double *__restrict__ uj) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = fg.gx * fg.gy;
+  if (t >= n) return;
+  const int li = t % fg.gx, lj = t / fg.gx;     // ghost-inclusive local indices
+  // local column li <-> reference i = nxgs - 2 + li; tile column of chunk c:
+  // tcol = i - (nxgs + CHUNK*c) + 2 = li - CHUNK*c
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int chi = min(li / CHUNK, P.nchunk - 1);
+  for (int b = 0; b < 5; ++b) {
+    // source row jl with jl + (b - 2) = lj - 2  ->  jl = lj - b
+    const int jl = lj - b;
+    if (jl < 0 || jl >= P.ny) continue;
+    for (int c = max(0, chi - 1); c <= chi; ++c) {
+      const int tc = li - CHUNK * c;
+      if (tc < 0 || tc >= DW) continue;
+      const double *tp = part + ((size_t)jl * P.nchunk + c) * TILE + ((size_t)b * DW + tc) * 3;
+      s0 += tp[0];
+      s1 += tp[1];
+      s2 += tp[2];
+    }
+  }
+  uj[t] = s0;
+  uj[fg.ps + t] = s1;
+  uj[2 * fg.ps + t] = s2;
+}
+
+// Cell-centred fields, particle.f90:68-81, on cells nxs-1..nxe+1, nys-1..nye+1.
+__global__ void k_cell_fields(FieldGeom fg, const double *__restrict__ uf,
+                              double *__restrict__ tmp, int TX, int TY, size_t tps) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= TX * TY) return;
+  const int a = t % TX, b = t / TX;
+  // cell (i, j) = (nxs-1+a, nys-1+b) -> ghost-inclusive local (a+1, b+1)
+  const size_t c0 = (size_t)(a + 1) + (size_t)fg.gx * (size_t)(b + 1);
+  const size_t gx = fg.gx;
+  const double *B1 = uf, *B2 = uf + fg.ps, *B3 = uf + 2 * fg.ps;
+  const double *E1 = uf + 3 * fg.ps, *E2 = uf + 4 * fg.ps, *E3 = uf + 5 * fg.ps;
+  tmp[t] = 0.5 * (B1[c0] + B1[c0 + gx]);
+  tmp[tps + t] = 0.5 * (B2[c0] + B2[c0 + 1]);
+  tmp[2 * tps + t] = 0.25 * (B3[c0] + B3[c0 + 1] + B3[c0 + gx] + B3[c0 + gx + 1]);
+  tmp[3 * tps + t] = 0.5 * (E1[c0] + E1[c0 + 1]);
+  tmp[4 * tps + t] = 0.5 * (E2[c0] + E2[c0 + gx]);
+  tmp[5 * tps + t] = E3[c0];
+}
+
+PP make_pp(const wm_ctx *ctx) {
+  PP P;
+  const wm_grid &g = ctx->g;
+  P.nx = g.nx; P.ny = g.ny; P.nxgs = g.nxgs; P.nys = g.nys; P.nsp = g.nsp;
+  P.cap = ctx->cap;
+  P.nchunk = wm_cdiv(g.nx, CHUNK);
+  P.d_delx = 1.0 / g.delx;
+  P.delt = g.delt;
+  P.c = g.c;
+  P.qc_scale = g.delx * (1.0 / g.delt);
+  for (int s = 0; s < WM_MAXSP; ++s) {
+    P.q[s] = s < g.nsp ? g.q[s] : 0.0;
+    P.r[s] = s < g.nsp ? g.r[s] : 1.0;
+  }
+  return P;
+}
+
+template <bool PUSH, bool DEP>
+int launch_particle(wm_ctx *ctx, int a, int b) {
+  PP P = make_pp(ctx);
+  dim3 grid(P.nchunk, P.ny);
+  const int TX = ctx->g.nx + 2;
+  const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
+  const PBank &A = ctx->bank[a];
+  const PBank &B = ctx->bank[b];
+  hipLaunchKernelGGL((k_particle<PUSH, DEP>), grid, dim3(256), 0, ctx->stream, P,
+                     ctx->cumcnt, ctx->tmp, TX, tps, A.x, A.y, A.ux, A.uy, A.uz,
+                     B.x, B.y, B.ux, B.uy, B.uz, ctx->dep_part, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+}  // namespace
+
+size_t wm_deposit_part_elems(int nx, int ny) {
+  return (size_t)ny * (size_t)wm_cdiv(nx, CHUNK) * (size_t)TILE;
+}
+
+int wm_k_cell_fields(wm_ctx *ctx) {
+  const int TX = ctx->g.nx + 2, TY = ctx->g.ny + 2;
+  const size_t tps = (size_t)TX * TY;
+  hipLaunchKernelGGL(k_cell_fields, dim3(wm_cdiv((long long)TX * TY, 256)), dim3(256), 0,
+                     ctx->stream, ctx->fg, ctx->uf, ctx->tmp, TX, TY, tps);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_push(wm_ctx *ctx, int src, int dst) { return launch_particle<true, false>(ctx, src, dst); }
+int wm_k_deposit(wm_ctx *ctx, int oldb, int newb) { return launch_particle<false, true>(ctx, oldb, newb); }
+int wm_k_push_deposit(wm_ctx *ctx, int src, int dst) { return launch_particle<true, true>(ctx, src, dst); }
+
+int wm_k_deposit_gather(wm_ctx *ctx) {
+  PP P = make_pp(ctx);
+  const int n = ctx->fg.gx * ctx->fg.gy;
+  hipLaunchKernelGGL(k_deposit_gather, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, P,
+                     ctx->fg, ctx->dep_part, ctx->uj);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
