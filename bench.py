@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the WumingPIC2D hot path on one MI355X.
+
+One step = the loop body of proj/weibel/app.f90:102-106 (push, current
+deposit, implicit field solve, particle boundaries, sort) over the whole
+particle population, resident in HBM.  The default workload is the Weibel
+configuration the headline metric is quoted on: 2048 x 2048 cells, 64
+particles per cell per species, 2 species (5.37e8 particles), dt = 0.5,
+counter-streaming beams u_z = +/-0.5 c with 0.05 c thermal spread,
+synthetic state generated on the device.
+
+Prints one JSON line.  ``roofline`` covers the dominant kernel (fused
+push+deposit), timed with HIP events on the library's stream;
+``cpu_baseline`` is the FP64 CPU restatement timed on this host's cores on a
+bounded sample (256 x 256 cells at the same density).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s (push+deposit+sort) & implicit-solve ms: FP32+learned PC vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (nx, ny, ppc per species, capacity factor)
+    "weibel_2048": (2048, 2048, 64, 1.25),
+    "weibel_128": (128, 128, 64, 1.5),
+    "weibel_1024": (1024, 1024, 64, 1.25),
+}
+
+# Algorithmic bytes per particle of the fused push+deposit kernel in the FP64
+# form: read x, y, ux, uy, uz (40 B) and write the same 40 B.
+PUSHDEP_BYTES_PER_PARTICLE = 80.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds: float = 12.0) -> dict:
+    """FP64 CPU restatement (the oracle) on a 256x256, 64 ppc/species sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from wumingpic2d_amd import weibel
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    nx = ny = 256
+    ppc = 64
+    p = weibel.grid_params(nx, ny, ppc, np_cap=int(1.25 * ppc * nx))
+    up, np2, cc, uf = weibel.init_state(p, seed=2, vth=0.05, udrift=0.5)
+    orc = Oracle(p, threads=threads)
+    gp = up.copy(order="F")
+    npart = int(np2.sum())
+    orc.step(up, gp, uf, cc, np2)                    # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while steps < 20:
+        orc.step(up, gp, uf, cc, np2)
+        steps += 1
+        log(f"cpu baseline step {steps}")
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": npart * steps / dt, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"Weibel {nx}x{ny}, {ppc} ppc/species, 2 species ({npart} particles), "
+                      f"{steps} steps after 1 warm-up, FP64 C restatement with {threads} OpenMP threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="weibel_2048", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--seed", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
+              f"{args.gpus}); the domain would split into y-slabs, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import wumingpic2d_amd as wm
+    from wumingpic2d_amd import weibel
+
+    nx, ny, ppc, capf = WORKLOADS[args.workload]
+    p = weibel.grid_params(nx, ny, ppc)
+    cap = int(capf * ppc * nx)
+    ctx = wm.Context(0)
+    ctx.set_capacity(cap, max(1024, cap // 16))
+    g = (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"], p["nye"])
+    ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
+    ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
+    ctx.field__init(*g, 0, 0, 0, 0, p["delx"], p["delt"], p["c"], p["q"], p["r"], p["gfac"])
+    ctx.sort__init(*g)
+    log(f"workload {args.workload}: {nx}x{ny}, {ppc} ppc/species, capacity {cap}/row")
+    ctx.init_weibel(args.seed, ppc, 0.05, 0.5)
+    npart = ctx.particle_count()
+    log(f"{npart} particles generated on the device")
+
+    for w in range(args.warmup):
+        ctx.step(1)
+        log(f"warm-up step {w + 1}/{args.warmup}")
+    phase = np.zeros(6)
+    iters = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.step(1)
+        phase += ctx.phase_times()[:6]
+        iters += int(ctx.cg_iters().sum())
+        if (k + 1) % 5 == 0 or k + 1 == args.steps:
+            log(f"timed step {k + 1}/{args.steps}")
+    elapsed = time.perf_counter() - t0
+    steps = args.steps
+    ms_per_step = 1e3 * elapsed / steps
+    value = npart * steps / elapsed
+    energy = ctx.energy()
+    names = ["cell_fields", "push_deposit", "current_gather_fold", "field_solve",
+             "particle_bc", "sort"]
+    phase_ms = {n: round(float(v) / steps, 4) for n, v in zip(names, phase)}
+    pd_ms = float(phase[1]) / steps
+    achieved = PUSHDEP_BYTES_PER_PARTICLE * npart / (pd_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_particle<PUSH,DEPOSIT> (fused push+deposit)",
+                "bytes_per_particle": PUSHDEP_BYTES_PER_PARTICLE}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} Weibel pair plasma, FP64 form", "nx": nx, "ny": ny,
+                   "ppc_per_species": ppc, "species": 2, "particles": npart, "dt": p["delt"],
+                   "form": "fp64", "parallelism": "single GPU"},
+        "solve_ms_per_step": phase_ms["field_solve"],
+        "cg_iters_per_step": iters / steps,
+        "phase_ms_per_step": phase_ms,
+        "energy_last": [float(e) for e in energy],
+        "roofline": roofline,
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        log("timing the CPU restatement")
+        out["cpu_baseline"] = cpu_baseline()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -142,8 +142,9 @@ int wm_energy(wm_ctx *ctx, double *out);
 /* CG iterations per component (l = 1..3) of the last field solve. */
 int wm_cg_iters(wm_ctx *ctx, int *iters3);
 /* GPU time (ms) of each phase of the last wm_step call, summed over its
- * steps: [0] push+deposit, [1] current gather+fold, [2] field solve,
- * [3] particle boundaries, [4] sort.  Returns the number written. */
+ * steps, from HIP events on the context's stream: [0] cell-centred fields,
+ * [1] push+deposit, [2] current gather+fold, [3] field solve, [4] particle
+ * boundaries, [5] sort.  Returns the number written. */
 int wm_phase_times(wm_ctx *ctx, double *ms, int n);
 /* Total active particles (sum of np2). */
 int wm_particle_count(wm_ctx *ctx, long long *count);

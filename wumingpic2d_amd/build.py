@@ -50,5 +50,34 @@ def build_hip(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FMOD = os.path.join(FORTRAN, "mod")
+FLIB = os.path.join(HERE, "libwmpic_f.so")
+FSRC = ["wmpic_capi.f90", "wm_mpi_set.f90", "wm_particle.f90", "wm_field.f90", "wm_sort.f90",
+        "wm_boundary_periodic.f90"]
+DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "weibel_loop.f90")
+DRIVER = os.path.join(ROOT, "tests", "fortran", "weibel_loop.x")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Drop-in Fortran modules (particle, field, sort, boundary_periodic,
+    mpi_set) over the C ABI -> libwmpic_f.so + .mod files in fortran/mod,
+    and the Fortran test driver of the Weibel time loop."""
+    fc = os.environ.get("FC", "amdflang")
+    os.makedirs(FMOD, exist_ok=True)
+    srcs = [os.path.join(FORTRAN, f) for f in FSRC]
+    tmp = FLIB + ".tmp"
+    _run([fc, "-O2", "-fPIC", "-shared", "-module-dir", FMOD, *srcs, "-o", tmp,
+          f"-L{HERE}", "-lwmpic", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FLIB)
+    _run([fc, "-O2", "-I", FMOD, DRIVER_SRC, "-o", DRIVER + ".tmp", f"-L{HERE}", "-lwmpic_f",
+          "-lwmpic", "-Wl,-rpath,$ORIGIN/../../wumingpic2d_amd"])
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print(f"built {FLIB} and {DRIVER}")
+    return FLIB
+
+
 if __name__ == "__main__":
     build_hip(verbose=True)
+    build_fortran(verbose=True)

@@ -354,6 +354,7 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->gkl, 3 * n));
   WM_TRY(dmalloc(ctx, &ctx->cg_r, 3 * n));
   WM_TRY(dmalloc(ctx, &ctx->cg_p[0], 3 * n));
+  WM_TRY(dmalloc(ctx, &ctx->cg_p[1], 3 * n));
   WM_TRY(dmalloc(ctx, &ctx->cg_s, 4));
   WM_TRY(dmalloc(ctx, &ctx->red, wm_cg_partials(g.nx, g.ny)));
   WM_TRY(dmalloc(ctx, &ctx->red_cnt, 8));
@@ -425,7 +426,7 @@ int wm_destroy(wm_ctx *ctx) {
   free_bank(ctx->mv_dn);
   void *ptrs[] = {ctx->id[0], ctx->id[1], ctx->np2, ctx->cumcnt, ctx->hole, ctx->nhole,
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
-                  ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_s, ctx->red,
+                  ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
                   ctx->cnt_scratch, ctx->fscratch};
   for (void *p : ptrs)
@@ -718,32 +719,33 @@ int wm_step(wm_ctx *ctx, int nsteps) {
   WM_TRY(require(ctx, ctx->sinit, "sort__init"));
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
   WM_TRY(wm_alloc_device(ctx));
-  double acc[5] = {0, 0, 0, 0, 0};
+  double acc[WM_NPHASE] = {0};
   for (int it = 0; it < nsteps; ++it) {
     hipEvent_t *e = ctx->ev;
     WM_HIP(hipEventRecord(e[0], ctx->stream));
     // particle__solv + ele_cur (proj/weibel/app.f90:102, field.f90:121)
     WM_TRY(wm_k_cell_fields(ctx));
-    WM_TRY(wm_k_push_deposit(ctx, 0, 1));
     WM_HIP(hipEventRecord(e[1], ctx->stream));
+    WM_TRY(wm_k_push_deposit(ctx, 0, 1));
+    WM_HIP(hipEventRecord(e[2], ctx->stream));
     WM_TRY(wm_k_deposit_gather(ctx));
     WM_TRY(wm_k_bc_curre(ctx, ctx->uj));          // field.f90:122
-    WM_HIP(hipEventRecord(e[2], ctx->stream));
-    WM_TRY(wm_k_field_solve(ctx));                // field.f90:125-184
     WM_HIP(hipEventRecord(e[3], ctx->stream));
-    WM_TRY(wm_k_particle_y(ctx, 1, true));        // app.f90:104-105 (x fused into y pass 1)
+    WM_TRY(wm_k_field_solve(ctx));                // field.f90:125-184
     WM_HIP(hipEventRecord(e[4], ctx->stream));
-    WM_TRY(wm_k_sort(ctx, 1, 0));                 // app.f90:106
+    WM_TRY(wm_k_particle_y(ctx, 1, true));        // app.f90:104-105 (x fused into y pass 1)
     WM_HIP(hipEventRecord(e[5], ctx->stream));
-    WM_HIP(hipEventSynchronize(e[5]));
-    for (int k = 0; k < 5; ++k) {
+    WM_TRY(wm_k_sort(ctx, 1, 0));                 // app.f90:106
+    WM_HIP(hipEventRecord(e[6], ctx->stream));
+    WM_HIP(hipEventSynchronize(e[6]));
+    for (int k = 0; k < WM_NPHASE; ++k) {
       float ms = 0.f;
       WM_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
       acc[k] += ms;
     }
     WM_TRY(wm_check_derr(ctx, "step"));
   }
-  for (int k = 0; k < 5; ++k) ctx->phase_ms[k] = acc[k];
+  for (int k = 0; k < WM_NPHASE; ++k) ctx->phase_ms[k] = acc[k];
   return WM_OK;
 }
 
@@ -761,7 +763,7 @@ int wm_cg_iters(wm_ctx *ctx, int *iters3) {
 
 int wm_phase_times(wm_ctx *ctx, double *ms, int n) {
   if (!ctx || !ms) return WM_ERR_ARG;
-  const int m = std::min(n, 5);
+  const int m = std::min(n, (int)WM_NPHASE);
   for (int k = 0; k < m; ++k) ms[k] = ctx->phase_ms[k];
   return m;
 }

@@ -64,112 +64,105 @@ __global__ void k_particle_x(BP P, const int *__restrict__ np2, double *__restri
   if (bad) atomicOr(derr, DERR_RANGE);
 }
 
-constexpr int YB = 256;
+constexpr int YW = 64;     // one wavefront per segment
+constexpr int YT = 4;      // tiles in flight
 
-// Exclusive prefix of a per-lane flag over the workgroup; returns the rank
-// and writes the workgroup total into *tot (all threads).
-__device__ inline int block_rank(bool f, int *sh_w, int *tot) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(f);
-  const int below = __popcll(m & ((1ULL << lane) - 1ULL));
-  if (lane == 0) sh_w[w] = __popcll(m);
-  __syncthreads();
-  int off = 0, all = 0;
-  for (int k = 0; k < YB / 64; ++k) {
-    const int c = sh_w[k];
-    off += (k < w) ? c : 0;
-    all += c;
-  }
-  *tot = all;
-  __syncthreads();
-  return off + below;
+__device__ inline int lanes_below(unsigned long long m) {
+  const int lane = threadIdx.x & 63;
+  return __popcll(m & ((1ULL << lane) - 1ULL));
 }
 
-// pass 1
-__global__ __launch_bounds__(YB) void k_particle_y1(
+// pass 1: one wavefront per segment walks it in index order.
+__global__ __launch_bounds__(YW) void k_particle_y1(
     BP P, bool fuse_x, const int *__restrict__ np2, double *__restrict__ x,
     double *__restrict__ y, const double *__restrict__ ux, const double *__restrict__ uy,
     const double *__restrict__ uz, const int64_t *__restrict__ id, int *__restrict__ hole,
     int *__restrict__ nhole, int *__restrict__ nup, int *__restrict__ ndn, PBank mu,
     int64_t *mid_u, PBank md, int64_t *mid_d, int *derr) {
-  __shared__ int sh[3][YB / 64];
   const int seg = blockIdx.x;
+  const int lane = threadIdx.x;
   const int jl = seg % P.ny;
   const int j = P.nys + jl;
   const int n = np2[seg];
   const size_t base = (size_t)seg * P.cap;
   const size_t mb = (size_t)seg * P.mcap;
-  int ch = 0, cu = 0, cd = 0;
+  int cu = 0, cd = 0;
   int bad = 0;
-  for (int k0 = 0; k0 < n; k0 += YB) {
-    const int k = k0 + threadIdx.x;
-    bool up = false, dn = false;
-    double xv = 0, yv = 0;
-    if (k < n) {
-      xv = x[base + k];
-      yv = y[base + k];
-      if (fuse_x) {
-        const double x0 = xv;
-        wrap_x(xv, P);
-        if (xv != x0) x[base + k] = xv;
-        const int inew = (int)(xv / P.delx);
-        if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
-      }
-      const int jpos = (int)(yv / P.delx);
-      if (jpos != j) {
-        if (jpos == j + 1) up = true;
-        else if (jpos == j - 1) dn = true;
-        else bad |= DERR_JUMP;
-        if (jpos <= P.nygs - 1) yv = add_rd(yv, P.spany);
-        else if (jpos >= P.nyge + 1) yv = add_rd(yv, -P.spany);
+  for (int k0 = 0; k0 < n; k0 += YW * YT) {
+    double xv[YT], yv[YT];
+#pragma unroll
+    for (int t = 0; t < YT; ++t) {
+      const int k = k0 + t * YW + lane;
+      xv[t] = 0.0;
+      yv[t] = 0.0;
+      if (k < n) {
+        xv[t] = x[base + k];
+        yv[t] = y[base + k];
       }
     }
-    int th, tu, td;
-    const int rh = block_rank(up || dn, sh[0], &th);
-    const int ru = block_rank(up, sh[1], &tu);
-    const int rd = block_rank(dn, sh[2], &td);
-    if (up || dn) {
-      const int hpos = ch + rh;
-      if (hpos < 2 * P.mcap) hole[2 * mb + hpos] = k;
-      else bad |= DERR_MOVER_OVER;
-    }
-    if (up || dn) {
-      const int pos = up ? cu + ru : cd + rd;
-      if (pos < P.mcap) {
-        const PBank &B = up ? mu : md;
-        int64_t *bid = up ? mid_u : mid_d;
-        const size_t q = mb + pos;
-        B.x[q] = xv;
-        B.y[q] = yv;
-        B.ux[q] = ux[base + k];
-        B.uy[q] = uy[base + k];
-        B.uz[q] = uz[base + k];
-        bid[q] = id[base + k];
-      } else {
-        bad |= DERR_MOVER_OVER;
+#pragma unroll
+    for (int t = 0; t < YT; ++t) {
+      const int k = k0 + t * YW + lane;
+      bool up = false, dn = false;
+      if (k < n) {
+        if (fuse_x) {
+          const double x0 = xv[t];
+          wrap_x(xv[t], P);
+          if (xv[t] != x0) x[base + k] = xv[t];
+          const int inew = (int)(xv[t] / P.delx);
+          if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
+        }
+        const int jpos = (int)(yv[t] / P.delx);
+        if (jpos != j) {
+          if (jpos == j + 1) up = true;
+          else if (jpos == j - 1) dn = true;
+          else bad |= DERR_JUMP;
+          if (jpos <= P.nygs - 1) yv[t] = add_rd(yv[t], P.spany);
+          else if (jpos >= P.nyge + 1) yv[t] = add_rd(yv[t], -P.spany);
+        }
       }
+      const unsigned long long mu_ = __ballot(up), md_ = __ballot(dn);
+      if (up || dn) {
+        // holes in ascending index order: flag(:,j) of :163-164
+        const int hpos = cu + cd + lanes_below(mu_ | md_);
+        if (hpos < 2 * P.mcap) hole[2 * mb + hpos] = k;
+        else bad |= DERR_MOVER_OVER;
+        const int pos = up ? cu + lanes_below(mu_) : cd + lanes_below(md_);
+        if (pos < P.mcap) {
+          const PBank &B = up ? mu : md;
+          int64_t *bid = up ? mid_u : mid_d;
+          const size_t q = mb + pos;
+          B.x[q] = xv[t];
+          B.y[q] = yv[t];
+          B.ux[q] = ux[base + k];
+          B.uy[q] = uy[base + k];
+          B.uz[q] = uz[base + k];
+          bid[q] = id[base + k];
+        } else {
+          bad |= DERR_MOVER_OVER;
+        }
+      }
+      cu += __popcll(mu_);
+      cd += __popcll(md_);
     }
-    ch += th;
-    cu += tu;
-    cd += td;
   }
-  if (threadIdx.x == 0) {
-    nhole[seg] = min(ch, 2 * P.mcap);
+  if (lane == 0) {
+    nhole[seg] = min(cu + cd, 2 * P.mcap);
     nup[seg] = min(cu, P.mcap);
     ndn[seg] = min(cd, P.mcap);
   }
   if (bad) atomicOr(derr, bad);
 }
 
-// pass 2
-__global__ __launch_bounds__(YB) void k_particle_y2(
+// pass 2: one wavefront per segment.
+__global__ __launch_bounds__(YW) void k_particle_y2(
     BP P, int *__restrict__ np2, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ ux, double *__restrict__ uy, double *__restrict__ uz,
     int64_t *__restrict__ id, const int *__restrict__ hole, const int *__restrict__ nhole,
     const int *__restrict__ nup, const int *__restrict__ ndn, PBank mu,
     const int64_t *mid_u, PBank md, const int64_t *mid_d, int *derr) {
-  __shared__ int sh[YB / 64];
   const int seg = blockIdx.x;
+  const int lane = threadIdx.x;
   const int s = seg / P.ny, jl = seg % P.ny;
   const int below = s * P.ny + (jl == 0 ? P.ny - 1 : jl - 1);
   const int above = s * P.ny + (jl == P.ny - 1 ? 0 : jl + 1);
@@ -189,7 +182,14 @@ __global__ __launch_bounds__(YB) void k_particle_y2(
   const size_t base = (size_t)seg * P.cap;
   const int *hl = hole + 2 * (size_t)seg * P.mcap;
 
-  auto put_arrival = [&](int a, int dst) {
+  const int newn = N + A - H;
+  if (newn > P.cap) {
+    if (lane == 0) atomicOr(derr, DERR_NP_OVER);
+    return;
+  }
+  // holes filled by arrivals, and surplus arrivals appended
+  for (int a = lane; a < A; a += YW) {
+    const int dst = a < H ? hl[a] : N + (a - H);
     const bool f = a < nF;
     const PBank &B = f ? FB : SB;
     const int64_t *bid = f ? Fid : Sid;
@@ -201,26 +201,18 @@ __global__ __launch_bounds__(YB) void k_particle_y2(
     uy[d] = B.uy[q];
     uz[d] = B.uz[q];
     id[d] = bid[q];
-  };
-
-  const int newn = N + A - H;
-  if (newn > P.cap) {
-    if (threadIdx.x == 0) atomicOr(derr, DERR_NP_OVER);
-    return;
   }
-  // holes filled by arrivals, and surplus arrivals appended
-  for (int a = threadIdx.x; a < A; a += YB) put_arrival(a, a < H ? hl[a] : N + (a - H));
   if (A < H) {
     const int m = H - A;
     const int L = N - m;
-    // holes >= L form the suffix hl[A + t .. H) of the sorted list
-    for (int q0 = 0; q0 < m; q0 += YB) {
-      const int qi = q0 + threadIdx.x;          // position N-1-qi, descending
+    int done = 0;    // non-hole tail slots already placed
+    for (int q0 = 0; q0 < m; q0 += YW) {
+      const int qi = q0 + lane;          // slot N-1-qi, walking down from the top
       bool nonhole = false;
       int src = 0;
       if (qi < m) {
         src = N - 1 - qi;
-        // binary search src in hl[A..H)
+        // src is a hole iff it is in the sorted list hl[A..H)
         int lo = A, hi = H;
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
@@ -229,13 +221,8 @@ __global__ __launch_bounds__(YB) void k_particle_y2(
         }
         nonhole = !(lo < H && hl[lo] == src);
       }
-      int tot;
-      const int rank = block_rank(nonhole, sh, &tot);
-      // the nonholes before this tile
-      __shared__ int base_rank;
-      if (q0 == 0 && threadIdx.x == 0) base_rank = 0;
-      __syncthreads();
-      const int nrank = base_rank + rank;
+      const unsigned long long mk = __ballot(nonhole);
+      const int nrank = done + lanes_below(mk);
       if (nonhole && A + nrank < H) {
         const int dst = hl[A + nrank];   // the nrank-th remaining hole (< L)
         const size_t sd = base + src, dd = base + dst;
@@ -246,12 +233,11 @@ __global__ __launch_bounds__(YB) void k_particle_y2(
         uz[dd] = uz[sd];
         id[dd] = id[sd];
       }
-      __syncthreads();
-      if (threadIdx.x == 0) base_rank += tot;
-      __syncthreads();
+      done += __popcll(mk);
     }
+    (void)L;
   }
-  if (threadIdx.x == 0) np2[seg] = newn;
+  if (lane == 0) np2[seg] = newn;
 }
 
 BP make_bp(const wm_ctx *ctx) {
@@ -282,10 +268,10 @@ int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x) {
   const int nseg = ctx->g.nsp * ctx->g.ny;
   const PBank &B = ctx->bank[bank];
   int64_t *id = ctx->id[ctx->idcur];
-  hipLaunchKernelGGL(k_particle_y1, dim3(nseg), dim3(YB), 0, ctx->stream, P, fuse_x, ctx->np2,
+  hipLaunchKernelGGL(k_particle_y1, dim3(nseg), dim3(YW), 0, ctx->stream, P, fuse_x, ctx->np2,
                      B.x, B.y, B.ux, B.uy, B.uz, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn,
                      ctx->mv_up, ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
-  hipLaunchKernelGGL(k_particle_y2, dim3(nseg), dim3(YB), 0, ctx->stream, P, ctx->np2, B.x, B.y,
+  hipLaunchKernelGGL(k_particle_y2, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2, B.x, B.y,
                      B.ux, B.uy, B.uz, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
                      ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
   WM_HIP(hipGetLastError());

@@ -180,6 +180,7 @@ constexpr int CGB = 256;   // threads per CG workgroup
 
 struct CgArgs {
   int nx, ny, gx, nblk;    // nblk = workgroups per component
+  int nbx;                 // workgroups per row
   size_t ps;               // df plane stride
   size_t n;                // nx*ny
   double f4, f5, err;
@@ -203,53 +204,73 @@ __device__ inline void block_sum(double (&v)[NV], double (*sh)[CGB]) {
   for (int k = 0; k < NV; ++k) v[k] = sh[k][0];
 }
 
-// Publish this workgroup's NV partials; returns true in every thread of the
-// workgroup that arrived last, after which `tot` holds the fixed-order sums.
+// Publish this workgroup's NV partial sums (fixed-order tree).
 template <int NV>
-__device__ inline bool last_block_total(double (&v)[NV], double (*sh)[CGB],
-                                        double *partials, unsigned *counter,
-                                        int nblk, int bid, double (&tot)[NV]) {
+__device__ inline void block_partials(double (&v)[NV], double (*sh)[CGB], double *partials,
+                                      int nblk, int bid) {
   block_sum<NV>(v, sh);
-  __shared__ int is_last;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) partials[(size_t)k * nblk + bid] = v[k];
-    __threadfence();
-    const unsigned ticket = atomicAdd(counter, 1u);
-    is_last = (ticket == (unsigned)nblk - 1);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// Fixed-order total of NV partial arrays by one workgroup.
+template <int NV>
+__device__ inline void partial_total(const double *partials, int nblk, double (*sh)[CGB],
+                                     double (&tot)[NV]) {
   double a[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) a[k] = 0.0;
   for (int b = threadIdx.x; b < nblk; b += CGB) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) a[k] += __builtin_nontemporal_load(&partials[(size_t)k * nblk + b]);
+    for (int k = 0; k < NV; ++k) a[k] += partials[(size_t)k * nblk + b];
   }
   block_sum<NV>(a, sh);
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = a[k];
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
 }
 
 __device__ inline int wrapm(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-// Initial residual (field.f90:350-385): b = f5*gkl, sum_g = |b|^2,
-// eps = sqrt(sum_g)*err, r = b + N4(phi) - f4*phi with periodic halo, p = r.
+// XCD-aware workgroup order: the dispatcher deals workgroups round-robin to
+// the 8 XCDs; renumber them so that each XCD (own L2) sweeps a contiguous band
+// of rows and the stencil's neighbour rows hit the same L2.
+__device__ inline int xcd_order(int bid, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = bid & 7, s = bid >> 3;
+  return x * q + min(x, r) + s;
+}
+
+// Interior cell of this thread in the row-blocked CG launch: workgroup b
+// covers x in [(b % nbx)*CGB, +CGB) of row b / nbx.
+struct CgCell {
+  int ii, jj, t, b;
+  bool in;
+};
+__device__ inline CgCell cg_cell(const CgArgs &A) {
+  CgCell c;
+  c.b = xcd_order(blockIdx.x, A.nblk);
+  c.jj = c.b / A.nbx;
+  c.ii = (c.b - c.jj * A.nbx) * CGB + threadIdx.x;
+  c.in = c.ii < A.nx;
+  c.t = c.jj * A.nx + c.ii;
+  return c;
+}
+
+// Initial residual (field.f90:350-385): b = f5*gkl, r = b + N4(phi) -
+// f4*phi with the periodic halo of boundary_periodic__phi, p = r; partial
+// sums of b.b and r.r.
 __global__ __launch_bounds__(CGB) void k_cg_init(CgArgs A, const double *__restrict__ gkl,
                                                  const double *__restrict__ df,
                                                  double *__restrict__ r, double *__restrict__ p,
-                                                 double *partials, unsigned *counters,
-                                                 CgScal *S) {
+                                                 double *partials) {
   __shared__ double sh[2][CGB];
   const int l = blockIdx.y;
-  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  const CgCell C = cg_cell(A);
+  const int ii = C.ii, jj = C.jj, t = C.t;
   double v[2] = {0.0, 0.0};
-  if (t < A.n) {
-    const int ii = (int)(t % A.nx), jj = (int)(t / A.nx);
+  if (C.in) {
     const double *phi = df + (size_t)l * A.ps;
     auto PH = [&](int a, int b) {
       return phi[(size_t)(wrapm(a, A.nx) + 2) + (size_t)A.gx * (size_t)(wrapm(b, A.ny) + 2)];
@@ -262,82 +283,63 @@ __global__ __launch_bounds__(CGB) void k_cg_init(CgArgs A, const double *__restr
     v[0] = b * b;
     v[1] = rr * rr;
   }
-  double tot[2];
-  if (last_block_total<2>(v, sh, partials + (size_t)l * 2 * A.nblk, counters + l, A.nblk,
-                          blockIdx.x, tot) &&
-      threadIdx.x == 0) {
-    CgScal s;
-    s.sum_g = tot[0];
-    s.eps = sqrt(tot[0]) * A.err;
-    s.sumr = tot[1];
-    s.sum2 = 0.0;
-    s.av = 0.0;
-    s.bv = 0.0;
-    s.ite = 0;
-    s.err = 0;
-    // the loop runs while sqrt(sumr) > eps (:385) and sum_g > eps (:387)
-    s.active = (sqrt(s.sumr) > s.eps) && (s.sum_g > s.eps);
-    s.pad = 0;
-    S[l] = s;
-  }
+  block_partials<2>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
 }
 
-// p.Ap sweep (field.f90:395-415): av = sumr/sum2, sum_g = sqrt(sumr) (:426).
+// A*p = -p(i,j-1) - p(i-1,j) + f4*p(i,j) - p(i+1,j) - p(i,j+1) (field.f90:400-402),
+// with one fixed rounding sequence so every evaluation of a cell agrees.
 __device__ inline double apply_a(double f4, double pc, double ps_, double pw, double pe,
                                  double pn) {
-  // -p(i,j-1) - p(i-1,j) + f4*p(i,j) - p(i+1,j) - p(i,j+1), one rounding order
   double a = __dsub_rn(-ps_, pw);
   a = __fma_rn(f4, pc, a);
   a = __dsub_rn(a, pe);
   return __dsub_rn(a, pn);
 }
 
-__global__ __launch_bounds__(CGB) void k_cg_pap(CgArgs A, const double *__restrict__ p,
-                                                double *partials, unsigned *counters,
-                                                CgScal *S) {
+// Direction update p = r + bv*p (field.f90:444-450) fused into the next
+// iteration's p.Ap sweep (field.f90:395-415): p_new is formed on the fly at
+// the five stencil points, stored, and dotted with A p_new.  bv = 0 on the
+// first iteration (p = r from the initial residual).
+__global__ __launch_bounds__(CGB) void k_cg_pap(CgArgs A, const double *__restrict__ r,
+                                                const double *__restrict__ pold,
+                                                double *__restrict__ pnew, double *partials,
+                                                const CgScal *S) {
   __shared__ double sh[1][CGB];
   const int l = blockIdx.y;
   if (!S[l].active) return;
-  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  const double bv = S[l].bv;
+  const CgCell C = cg_cell(A);
+  const int ii = C.ii, jj = C.jj, t = C.t;
   double v[1] = {0.0};
-  if (t < A.n) {
-    const int ii = (int)(t % A.nx), jj = (int)(t / A.nx);
-    const double *P = p + (size_t)l * A.n;
-    auto PP = [&](int a, int b) { return P[(size_t)wrapm(a, A.nx) + (size_t)A.nx * wrapm(b, A.ny)]; };
-    const double pc = P[t];
-    const double ap = apply_a(A.f4, pc, PP(ii, jj - 1), PP(ii - 1, jj), PP(ii + 1, jj), PP(ii, jj + 1));
+  if (C.in) {
+    const double *R = r + (size_t)l * A.n, *P = pold + (size_t)l * A.n;
+    auto PN = [&](int a, int b) {
+      const int q = wrapm(a, A.nx) + A.nx * wrapm(b, A.ny);
+      return __fma_rn(bv, P[q], R[q]);
+    };
+    const double pc = PN(ii, jj);
+    const double ap = apply_a(A.f4, pc, PN(ii, jj - 1), PN(ii - 1, jj), PN(ii + 1, jj), PN(ii, jj + 1));
+    pnew[(size_t)l * A.n + t] = pc;
     v[0] = pc * ap;
   }
-  double tot[1];
-  if (last_block_total<1>(v, sh, partials + (size_t)l * 2 * A.nblk, counters + l, A.nblk,
-                          blockIdx.x, tot) &&
-      threadIdx.x == 0) {
-    CgScal s = S[l];
-    s.ite = s.ite + 1;
-    s.sum2 = tot[0];
-    s.av = s.sumr / s.sum2;
-    s.sum_g = sqrt(s.sumr);
-    if (s.ite >= A.ite_max) s.err = 1;
-    S[l] = s;
-  }
+  block_partials<1>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
 }
 
-// phi += av*p, r -= av*Ap, sum1 = |r|^2, bv = sum1/sumr (field.f90:417-442).
+// phi += av*p, r -= av*A p, partial |r|^2 (field.f90:417-439).
 __global__ __launch_bounds__(CGB) void k_cg_update(CgArgs A, double *__restrict__ df,
                                                    double *__restrict__ r,
                                                    const double *__restrict__ p,
-                                                   double *partials, unsigned *counters,
-                                                   CgScal *S) {
+                                                   double *partials, const CgScal *S) {
   __shared__ double sh[1][CGB];
   const int l = blockIdx.y;
   if (!S[l].active) return;
   const double av = S[l].av;
-  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
+  const CgCell C = cg_cell(A);
+  const int ii = C.ii, jj = C.jj, t = C.t;
   double v[1] = {0.0};
-  if (t < A.n) {
-    const int ii = (int)(t % A.nx), jj = (int)(t / A.nx);
+  if (C.in) {
     const double *P = p + (size_t)l * A.n;
-    auto PP = [&](int a, int b) { return P[(size_t)wrapm(a, A.nx) + (size_t)A.nx * wrapm(b, A.ny)]; };
+    auto PP = [&](int a, int b) { return P[wrapm(a, A.nx) + A.nx * wrapm(b, A.ny)]; };
     const double pc = P[t];
     const double ap = apply_a(A.f4, pc, PP(ii, jj - 1), PP(ii - 1, jj), PP(ii + 1, jj), PP(ii, jj + 1));
     double *phi = df + (size_t)l * A.ps + (size_t)(ii + 2) + (size_t)A.gx * (size_t)(jj + 2);
@@ -347,31 +349,57 @@ __global__ __launch_bounds__(CGB) void k_cg_update(CgArgs A, double *__restrict_
     *rp = rn;
     v[0] = rn * rn;
   }
-  double tot[1];
-  if (last_block_total<1>(v, sh, partials + (size_t)l * 2 * A.nblk, counters + l, A.nblk,
-                          blockIdx.x, tot) &&
-      threadIdx.x == 0) {
-    CgScal s = S[l];
-    s.bv = tot[0] / s.sumr;
-    s.sumr = tot[0];                 // |r|^2 of the next iteration's sweep (:403)
-    s.active = (s.sum_g > s.eps) && !s.err;   // loop test (:387)
-    S[l] = s;
+  block_partials<1>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
+}
+
+// Scalar updates of the reference's CG, one workgroup per component.
+// stage 0: after k_cg_init; sum_g = |b|^2, eps = sqrt(sum_g)*err, sumr =
+//   |r|^2, run while sqrt(sumr) > eps and sum_g > eps (field.f90:356-387).
+// stage 1: after k_cg_pap; ite += 1, av = sumr/pAp, sum_g = sqrt(sumr) (the
+//   pre-update residual, :415, :426), error at ite_max (:427).
+// stage 2: after k_cg_update; bv = |r_new|^2/sumr (:442), loop test (:387).
+__global__ __launch_bounds__(CGB) void k_cg_fin(CgArgs A, int stage, const double *partials,
+                                                CgScal *S) {
+  __shared__ double sh[2][CGB];
+  const int l = blockIdx.x;
+  if (stage != 0 && !S[l].active) return;
+  const double *pl = partials + (size_t)l * 2 * A.nblk;
+  if (stage == 0) {
+    double tot[2];
+    partial_total<2>(pl, A.nblk, sh, tot);
+    if (threadIdx.x == 0) {
+      CgScal s;
+      s.sum_g = tot[0];
+      s.eps = sqrt(tot[0]) * A.err;
+      s.sumr = tot[1];
+      s.sum2 = 0.0;
+      s.av = 0.0;
+      s.bv = 0.0;
+      s.ite = 0;
+      s.err = 0;
+      s.active = (sqrt(s.sumr) > s.eps) && (s.sum_g > s.eps);
+      s.pad = 0;
+      S[l] = s;
+    }
+  } else {
+    double tot[1];
+    partial_total<1>(pl, A.nblk, sh, tot);
+    if (threadIdx.x == 0) {
+      CgScal s = S[l];
+      if (stage == 1) {
+        s.ite = s.ite + 1;
+        s.sum2 = tot[0];
+        s.av = s.sumr / s.sum2;
+        s.sum_g = sqrt(s.sumr);
+        if (s.ite >= A.ite_max) s.err = 1;
+      } else {
+        s.bv = tot[0] / s.sumr;
+        s.sumr = tot[0];
+        s.active = (s.sum_g > s.eps) && !s.err;
+      }
+      S[l] = s;
+    }
   }
-}
-
-// p = r + bv*p (field.f90:444-450)
-__global__ __launch_bounds__(CGB) void k_cg_dir(CgArgs A, const double *__restrict__ r,
-                                                double *__restrict__ p, const CgScal *S,
-                                                const int *was_active) {
-  const int l = blockIdx.y;
-  if (!was_active[l]) return;
-  const double bv = S[l].bv;
-  const size_t t = (size_t)blockIdx.x * CGB + threadIdx.x;
-  if (t < A.n) p[(size_t)l * A.n + t] = r[(size_t)l * A.n + t] + bv * p[(size_t)l * A.n + t];
-}
-
-__global__ void k_snapshot_active(const CgScal *S, int *act) {
-  if (threadIdx.x < 3) act[threadIdx.x] = S[threadIdx.x].active;
 }
 
 // Energy sums (proj/weibel/app.f90:499-535): per-workgroup partials.
@@ -458,16 +486,17 @@ int wm_k_field_solve(wm_ctx *ctx) {
 
   CgArgs A;
   A.nx = g.nx; A.ny = g.ny; A.gx = fg.gx; A.ps = fg.ps; A.n = (size_t)n;
-  A.nblk = wm_cdiv(n, CGB);
+  A.nbx = wm_cdiv(g.nx, CGB);
+  A.nblk = A.nbx * g.ny;
   A.f4 = f.f4; A.f5 = f.f5; A.err = ctx->cg_err; A.ite_max = ctx->ite_max;
   const dim3 grid(A.nblk, 3);
-  double *p = ctx->cg_p[0];
-  int *act = (int *)(ctx->cg_s + 3);    // 3 ints after the scalars
+  double *pbuf[2] = {ctx->cg_p[0], ctx->cg_p[1]};
   hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, ctx->stream, A, ctx->gkl, ctx->df,
-                     ctx->cg_r, p, ctx->red, ctx->red_cnt, ctx->cg_s);
+                     ctx->cg_r, pbuf[0], ctx->red);
+  hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 0, ctx->red, ctx->cg_s);
   WM_HIP(hipGetLastError());
   CgScal hs[3];
-  int iters = 0;
+  int iters = 0, cur = 0;
   for (;;) {
     WM_HIP(hipMemcpyAsync(hs, ctx->cg_s, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     WM_HIP(hipStreamSynchronize(ctx->stream));
@@ -484,13 +513,13 @@ int wm_k_field_solve(wm_ctx *ctx) {
       wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
       return WM_ERR_CG_ITEMAX;
     }
-    hipLaunchKernelGGL(k_snapshot_active, dim3(1), dim3(64), 0, ctx->stream, ctx->cg_s, act);
-    hipLaunchKernelGGL(k_cg_pap, grid, dim3(CGB), 0, ctx->stream, A, p, ctx->red, ctx->red_cnt,
-                       ctx->cg_s);
-    hipLaunchKernelGGL(k_cg_update, grid, dim3(CGB), 0, ctx->stream, A, ctx->df, ctx->cg_r, p,
-                       ctx->red, ctx->red_cnt, ctx->cg_s);
-    hipLaunchKernelGGL(k_cg_dir, grid, dim3(CGB), 0, ctx->stream, A, ctx->cg_r, p, ctx->cg_s,
-                       act);
+    hipLaunchKernelGGL(k_cg_pap, grid, dim3(CGB), 0, ctx->stream, A, ctx->cg_r, pbuf[cur],
+                       pbuf[1 - cur], ctx->red, ctx->cg_s);
+    cur = 1 - cur;
+    hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 1, ctx->red, ctx->cg_s);
+    hipLaunchKernelGGL(k_cg_update, grid, dim3(CGB), 0, ctx->stream, A, ctx->df, ctx->cg_r,
+                       pbuf[cur], ctx->red, ctx->cg_s);
+    hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 2, ctx->red, ctx->cg_s);
     WM_HIP(hipGetLastError());
   }
   for (int l = 0; l < 3; ++l) ctx->last_iters[l] = hs[l].ite;

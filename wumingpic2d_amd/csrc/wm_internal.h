@@ -169,3 +169,6 @@ int wm_k_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udr
 static inline int wm_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 #define WM_BLOCK 256
+// phases timed by wm_step: cell-centred fields, push+deposit, current
+// gather+fold, field solve, particle boundaries, sort
+#define WM_NPHASE 6

@@ -97,80 +97,130 @@ __device__ inline PushOut boris(const double *T, int gcol, double x, double y,
   return o;
 }
 
-// Esirkepov 2nd-order deposit of one particle into the 65 accumulators of
-// its cell (field.f90:224-298).  Returns false when the particle jumped more
-// than one cell.
-__device__ inline bool esirkepov(double *acc, double x0, double y0, double x1,
-                                 double y1, double ux, double uy, double uz,
-                                 int i, int j, double q, const PP &P) {
+// Shape data of one particle's move for the Esirkepov deposit
+// (field.f90:224-268): s0 at the old position around cell (i, j), ds = s1
+// shifted by the cell increment minus s0.  Returns false when the particle
+// jumped more than one cell.
+struct Move {
   double s0x[5], s0y[5], dsx[5], dsy[5];
-  s0x[0] = s0x[4] = s0y[0] = s0y[4] = 0.0;
-  shape(x0 * P.d_delx - 0.5 - i, s0x[1], s0x[2], s0x[3]);
-  shape(y0 * P.d_delx - 0.5 - j, s0y[1], s0y[2], s0y[3]);
-  bool ok = true;
-  {
-    const int i2 = (int)(x1 * P.d_delx);
-    double a, b, c;
-    shape(x1 * P.d_delx - 0.5 - i2, a, b, c);
-    const int inc = i2 - i;
-    ok = ok && (inc >= -1 && inc <= 1);
-    // s1 placed at offset inc (the smo_1..3 selection of :244-251)
-    dsx[0] = inc < 0 ? a : 0.0;
-    dsx[1] = inc < 0 ? b : (inc == 0 ? a : 0.0);
-    dsx[2] = inc < 0 ? c : (inc == 0 ? b : a);
-    dsx[3] = inc > 0 ? b : (inc == 0 ? c : 0.0);
-    dsx[4] = inc > 0 ? c : 0.0;
-  }
-  {
-    const int j2 = (int)(y1 * P.d_delx);
-    double a, b, c;
-    shape(y1 * P.d_delx - 0.5 - j2, a, b, c);
-    const int inc = j2 - j;
-    ok = ok && (inc >= -1 && inc <= 1);
-    dsy[0] = inc < 0 ? a : 0.0;
-    dsy[1] = inc < 0 ? b : (inc == 0 ? a : 0.0);
-    dsy[2] = inc < 0 ? c : (inc == 0 ? b : a);
-    dsy[3] = inc > 0 ? b : (inc == 0 ? c : 0.0);
-    dsy[4] = inc > 0 ? c : 0.0;
-  }
+};
+
+__device__ inline void place_s1(double a, double b, double c, int inc, double *ds) {
+  // s1 placed at offset inc (the smo_1..3 selection of field.f90:244-251)
+  ds[0] = inc < 0 ? a : 0.0;
+  ds[1] = inc < 0 ? b : (inc == 0 ? a : 0.0);
+  ds[2] = inc < 0 ? c : (inc == 0 ? b : a);
+  ds[3] = inc > 0 ? b : (inc == 0 ? c : 0.0);
+  ds[4] = inc > 0 ? c : 0.0;
+}
+
+__device__ inline bool make_move(Move &m, double x0, double y0, double x1, double y1, int i,
+                                 int j, const PP &P) {
+  m.s0x[0] = m.s0x[4] = m.s0y[0] = m.s0y[4] = 0.0;
+  shape(x0 * P.d_delx - 0.5 - i, m.s0x[1], m.s0x[2], m.s0x[3]);
+  shape(y0 * P.d_delx - 0.5 - j, m.s0y[1], m.s0y[2], m.s0y[3]);
+  const int i2 = (int)(x1 * P.d_delx), j2 = (int)(y1 * P.d_delx);
+  double a, b, c;
+  shape(x1 * P.d_delx - 0.5 - i2, a, b, c);
+  place_s1(a, b, c, i2 - i, m.dsx);
+  shape(y1 * P.d_delx - 0.5 - j2, a, b, c);
+  place_s1(a, b, c, j2 - j, m.dsy);
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    dsx[k] -= s0x[k];
-    dsy[k] -= s0y[k];
+    m.dsx[k] -= m.s0x[k];
+    m.dsy[k] -= m.s0y[k];
   }
-  const double gvz = uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
-  const double qc = q * P.qc_scale;      // q*delx*d_delt
-  // Jx: pjx(ip+1,jp) = pjx(ip,jp) - qc*ds_x(ip)*(s0_y(jp) + ds_y(jp)/2)
-  double Y[5], X[5];
+  return (i2 - i >= -1 && i2 - i <= 1) && (j2 - j >= -1 && j2 - j <= 1);
+}
+
+// Full 5x5 stencil of one current component (field.f90:274-298) into acc:
+//  C = 0: Jx, acc[a*5+b], tile column a+1 (a = 0..3), pjx(ip+1,jp) =
+//         pjx(ip,jp) - q dx/dt ds_x(ip) (s0_y(jp) + ds_y(jp)/2)
+//  C = 1: Jy, acc[a*4+b], tile row b+1 (b = 0..3), the same along y
+//  C = 2: Jz, acc[a*5+b], q gvz (s0x s0y + dsx s0y/2 + s0x dsy/2 + dsx dsy/3)
+template <int C>
+__device__ inline void stencil(double *acc, const Move &m, double qc, double qv) {
+  if (C == 0) {
+    double w = 0.0;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    Y[k] = s0y[k] + 0.5 * dsy[k];
-    X[k] = s0x[k] + 0.5 * dsx[k];
+    for (int a = 0; a < 4; ++a) {
+      w -= qc * m.dsx[a];
+#pragma unroll
+      for (int b = 0; b < 5; ++b) acc[a * 5 + b] += w * (m.s0y[b] + 0.5 * m.dsy[b]);
+    }
+  } else if (C == 1) {
+    double w = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      w -= qc * m.dsy[b];
+#pragma unroll
+      for (int a = 0; a < 5; ++a) acc[a * 4 + b] += (m.s0x[a] + 0.5 * m.dsx[a]) * w;
+    }
+  } else {
+    const double third = 1.0 / 3.0;
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      const double za = qv * (m.s0x[a] + 0.5 * m.dsx[a]);
+      const double zb = qv * (0.5 * m.s0x[a] + third * m.dsx[a]);
+#pragma unroll
+      for (int b = 0; b < 5; ++b) acc[a * 5 + b] += za * m.s0y[b] + zb * m.dsy[b];
+    }
   }
-  double w = 0.0;
+}
+
+// A particle that stays in its cell has s0 and ds supported on offsets 1..3
+// only, so of the 65 stencil values just the 27 below can be non-zero
+// (every other value is an exact zero in the reference's arithmetic):
+//   Jx tile columns 2..4 x rows 1..3, Jy columns 1..3 x rows 2..4,
+//   Jz columns 1..3 x rows 1..3.
+// core[0..8] Jx (a-1)*3+(b-1) with a = W index 1..3; core[9..17] Jy
+// (a-1)*3+(b-1) with b = W index 1..3; core[18..26] Jz (a-1)*3+(b-1).
+__device__ inline void stencil_core(double *core, double x0, double y0, double x1,
+                                    double y1, double gvz, int i, int j, double q,
+                                    const PP &P) {
+  double sx[3], sy[3], dx[3], dy[3];
+  shape(x0 * P.d_delx - 0.5 - i, sx[0], sx[1], sx[2]);
+  shape(y0 * P.d_delx - 0.5 - j, sy[0], sy[1], sy[2]);
+  shape(x1 * P.d_delx - 0.5 - i, dx[0], dx[1], dx[2]);
+  shape(y1 * P.d_delx - 0.5 - j, dy[0], dy[1], dy[2]);
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    w -= qc * dsx[a];
-#pragma unroll
-    for (int b = 0; b < 5; ++b) acc[a * 5 + b] += w * Y[b];          // tile col a+1
+  for (int k = 0; k < 3; ++k) {
+    dx[k] -= sx[k];
+    dy[k] -= sy[k];
   }
-  w = 0.0;
-#pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    w -= qc * dsy[b];
-#pragma unroll
-    for (int a = 0; a < 5; ++a) acc[20 + a * 4 + b] += X[a] * w;     // tile row b+1
-  }
+  const double qc = q * P.qc_scale;
   const double qv = q * gvz;
+  double X[3], Y[3], wx[3], wy[3];
+  double w1 = 0.0, w2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    X[k] = sx[k] + 0.5 * dx[k];
+    Y[k] = sy[k] + 0.5 * dy[k];
+    w1 -= qc * dx[k];
+    w2 -= qc * dy[k];
+    wx[k] = w1;
+    wy[k] = w2;
+  }
   const double third = 1.0 / 3.0;
 #pragma unroll
-  for (int a = 0; a < 5; ++a) {
+  for (int a = 0; a < 3; ++a) {
     const double za = qv * X[a];
-    const double zb = qv * (0.5 * s0x[a] + third * dsx[a]);
+    const double zb = qv * (0.5 * sx[a] + third * dx[a]);
 #pragma unroll
-    for (int b = 0; b < 5; ++b) acc[40 + a * 5 + b] += za * s0y[b] + zb * dsy[b];
+    for (int b = 0; b < 3; ++b) {
+      core[a * 3 + b] += wx[a] * Y[b];
+      core[9 + a * 3 + b] += X[a] * wy[b];
+      core[18 + a * 3 + b] += za * sy[b] + zb * dy[b];
+    }
   }
-  return ok;
+}
+
+// Index of a core value in the 65-value cell layout of acc_at().
+__device__ inline int core_slot(int c) {
+  const int comp = c / 9, a = (c % 9) / 3 + 1, b = c % 3 + 1;
+  if (comp == 0) return a * 5 + b;             // Jx: W index a -> acc (a, b)
+  if (comp == 1) return 20 + a * 4 + b;        // Jy: acc (a, W index b)
+  return 40 + a * 5 + b;
 }
 
 // Value of accumulator set `cb` (one cell) at tile offset (a, b) for comp.
@@ -180,8 +230,57 @@ __device__ inline double acc_at(const double *cb, int comp, int a, int b) {
   return cb[40 + a * 5 + b];
 }
 
+__device__ inline double group_sum(double v) {
+  v += __shfl_xor(v, 1, GROUP);
+  v += __shfl_xor(v, 2, GROUP);
+  v += __shfl_xor(v, 4, GROUP);
+  return v;
+}
+
+// Contribution of one component of this lane's cell-crossing particles
+// (bit m of mask = particle kb+sub+GROUP*m; particles beyond the mask are
+// re-examined), combined over the group and added by lane 0 to cellbuf.
+template <int C>
+__device__ inline void crossing_pass(double *cb, int sub, unsigned long long mask, int kb,
+                                     int ke, size_t base, const double *xa, const double *ya,
+                                     const double *xb, const double *yb, const double *uxb,
+                                     const double *uyb, const double *uzb, int i, int j,
+                                     double q, const PP &P) {
+  constexpr int N = C == 2 ? 25 : 20;
+  double acc[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = 0.0;
+  const double qc = q * P.qc_scale;
+  auto one = [&](int k, bool known) {
+    const size_t p = base + (size_t)k;
+    Move m;
+    const double x1 = xb[p], y1 = yb[p];
+    const bool cross = (int)(x1 * P.d_delx) != i || (int)(y1 * P.d_delx) != j;
+    if (!known && !cross) return;
+    make_move(m, xa[p], ya[p], x1, y1, i, j, P);
+    double qv = 0.0;
+    if (C == 2) {
+      const double ux = uxb[p], uy = uyb[p], uz = uzb[p];
+      qv = q * (uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c)));
+    }
+    stencil<C>(acc, m, qc, qv);
+  };
+  while (mask) {
+    const int m = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    one(kb + sub + GROUP * m, true);
+  }
+  for (int k = kb + sub + GROUP * 64; k < ke; k += GROUP) one(k, false);
+  const int off = C == 0 ? 0 : (C == 1 ? 20 : 40);
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const double v = group_sum(acc[k]);
+    if (sub == 0) cb[off + k] += v;
+  }
+}
+
 template <bool PUSH, bool DEPOSIT>
-__global__ __launch_bounds__(256) void k_particle(
+__global__ __launch_bounds__(256, 3) void k_particle(
     PP P, const int *__restrict__ cumcnt,
     const double *__restrict__ tmp, int TX, size_t tps,
     const double *__restrict__ xa, const double *__restrict__ ya,
@@ -198,17 +297,18 @@ __global__ __launch_bounds__(256) void k_particle(
   const int j = P.nys + jl;
   const int i0 = P.nxgs + chunk * CHUNK;
   const int ncell = min(CHUNK, P.nx - chunk * CHUNK);
-  if (PUSH) {
-    load_tmp_tile(T, tmp, TX, tps, jl, chunk * CHUNK, min(TW, ncell + 2));
-    __syncthreads();
-  }
+  if (PUSH) load_tmp_tile(T, tmp, TX, tps, jl, chunk * CHUNK, min(TW, ncell + 2));
+  if (DEPOSIT)
+    for (int e = threadIdx.x; e < CHUNK * NACC; e += blockDim.x) cellbuf[e] = 0.0;
+  __syncthreads();
   const int g = threadIdx.x / GROUP;
   const int sub = threadIdx.x % GROUP;
   const int i = i0 + g;
-  double acc[DEPOSIT ? NACC : 1];
+  double *cb = cellbuf + g * NACC;
+  double core[DEPOSIT ? 27 : 1];
   if (DEPOSIT) {
 #pragma unroll
-    for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+    for (int k = 0; k < 27; ++k) core[k] = 0.0;
   }
   bool ok = true;
   if (g < ncell) {
@@ -221,36 +321,66 @@ __global__ __launch_bounds__(256) void k_particle(
       const double fac1 = q / r * 0.5 * P.delt;
       const double txxx = fac1 * fac1;
       const double fac2 = q * P.delt / r;
-      for (int k = kb + sub; k < ke; k += GROUP) {
+      unsigned long long mask = 0ULL;
+      // the next particle's state is loaded while the current one is
+      // computed (one step of software pipelining per lane)
+      int k = kb + sub;
+      double nx0 = 0, ny0 = 0, nux = 0, nuy = 0, nuz = 0;
+      if (k < ke) {
         const size_t p = base + (size_t)k;
+        nx0 = xa[p]; ny0 = ya[p];
+        if (PUSH) { nux = uxa[p]; nuy = uya[p]; nuz = uza[p]; }
+      }
+      for (int m = 0; k < ke; k += GROUP, ++m) {
+        const size_t p = base + (size_t)k;
+        const double x0 = nx0, y0 = ny0, cux = nux, cuy = nuy, cuz = nuz;
+        if (k + GROUP < ke) {
+          const size_t pn = p + GROUP;
+          nx0 = xa[pn]; ny0 = ya[pn];
+          if (PUSH) { nux = uxa[pn]; nuy = uya[pn]; nuz = uza[pn]; }
+        }
         double x1, y1, ux, uy, uz;
-        const double x0 = xa[p], y0 = ya[p];
+        // keep the 54 tile values in LDS instead of hoisting them into
+        // registers for the whole cell loop (occupancy)
+        __asm__ volatile("" ::: "memory");
         if (PUSH) {
-          PushOut o = boris(T, g, x0, y0, uxa[p], uya[p], uza[p], i, j, P, fac1, txxx, fac2);
+          PushOut o = boris(T, g, x0, y0, cux, cuy, cuz, i, j, P, fac1, txxx, fac2);
           xb[p] = o.x; yb[p] = o.y; uxb[p] = o.ux; uyb[p] = o.uy; uzb[p] = o.uz;
           x1 = o.x; y1 = o.y; ux = o.ux; uy = o.uy; uz = o.uz;
         } else {
           x1 = xb[p]; y1 = yb[p]; ux = uxb[p]; uy = uyb[p]; uz = uzb[p];
         }
-        if (DEPOSIT) ok = esirkepov(acc, x0, y0, x1, y1, ux, uy, uz, i, j, q, P) && ok;
+        if (DEPOSIT) {
+          const int i2 = (int)(x1 * P.d_delx), j2 = (int)(y1 * P.d_delx);
+          if (i2 == i && j2 == j) {
+            const double gvz = uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
+            stencil_core(core, x0, y0, x1, y1, gvz, i, j, q, P);
+          } else {
+            ok = ok && (i2 - i >= -1 && i2 - i <= 1 && j2 - j >= -1 && j2 - j <= 1);
+            if (m < 64) mask |= 1ULL << m;
+          }
+        }
+      }
+      if (DEPOSIT) {
+        // add the stay-in-cell stencils of this species over the 8 lanes
+#pragma unroll
+        for (int c = 0; c < 27; ++c) {
+          const double v = group_sum(core[c]);
+          core[c] = 0.0;
+          if (sub == 0) cb[core_slot(c)] += v;
+        }
+      }
+      if (DEPOSIT && __ballot(mask != 0ULL || ke - kb > GROUP * 64)) {
+        // cell-crossing particles: full 5x5 stencils, one component at a time
+        // (stream-ordered after this lane's own writes of the new state)
+        crossing_pass<0>(cb, sub, mask, kb, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
+        crossing_pass<1>(cb, sub, mask, kb, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
+        crossing_pass<2>(cb, sub, mask, kb, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
       }
     }
   }
   if (!ok) atomicOr(derr, DERR_JUMP);
   if (DEPOSIT) {
-    // combine the 8 lanes of each group (butterfly: every lane gets the sum)
-#pragma unroll
-    for (int k = 0; k < NACC; ++k) {
-      double v = acc[k];
-      v += __shfl_xor(v, 1, GROUP);
-      v += __shfl_xor(v, 2, GROUP);
-      v += __shfl_xor(v, 4, GROUP);
-      acc[k] = v;
-    }
-    if (sub == 0) {
-#pragma unroll
-      for (int k = 0; k < NACC; ++k) cellbuf[g * NACC + k] = acc[k];
-    }
     __syncthreads();
     // assemble the workgroup tile [b][t][comp] in a fixed order
     double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;

@@ -1,21 +1,22 @@
 // wm_sort.hip -- stable per-row bucket sort by x-cell (sort.f90:36-82).
 //
-// One workgroup per (species, row) segment.  The cell histogram lives in LDS
-// (nx counters); counting and scattering walk the segment in index order in
-// tiles of 256.  Inside a wavefront, lanes with equal keys are grouped by a
-// ballot loop over the distinct keys (sorted input has one to three per
-// wavefront), which gives each lane its rank among equal keys below it; the
-// four wavefronts of a tile exchange their (key, count) lists through LDS so
-// that the destination of every particle is
-//   start[key] + (equal keys in earlier tiles and wavefronts) + rank,
-// i.e. exactly the reference's stable scatter.  Keys are int(x) as in
-// sort.f90:60 (the reference assumes delx = 1).
+// One wavefront per (species, row) segment, so the ordered walk through the
+// segment needs no workgroup barriers: the cell histogram and the running
+// write offsets live in LDS (nx counters), and the wavefront walks the
+// segment in index order, 64 particles at a time, with the loads of four
+// such tiles in flight together.  Inside a tile, lanes with equal keys are
+// grouped by a ballot loop over the distinct keys (sorted input has one to
+// three per tile), which gives each lane its rank among equal keys below it,
+// so the destination of every particle is
+//   start[key] + (equal keys already written) + rank,
+// exactly the reference's stable scatter.  Keys are int(x) as in sort.f90:60
+// (the reference assumes delx = 1).
 #include "wm_internal.h"
 
 namespace {
 
-constexpr int SB = 256;
-constexpr int NW = SB / 64;
+constexpr int SW = 64;     // one wavefront per segment
+constexpr int BT = 4;      // tiles in flight
 
 struct SP {
   int nx, nxgs, cap;
@@ -24,16 +25,13 @@ struct SP {
 struct WaveMatch {
   int rank;     // equal keys in lower lanes
   int count;    // equal keys in the wavefront
-  int slot;     // index of this key in the wavefront's distinct-key list
   bool leader;  // lowest lane holding the key
-  int ndist;    // distinct keys in the wavefront
 };
 
 __device__ inline WaveMatch wave_match(int key, bool active) {
   const int lane = threadIdx.x & 63;
-  WaveMatch w{0, 0, 0, false, 0};
+  WaveMatch w{0, 0, false};
   unsigned long long rem = __ballot(active);
-  int it = 0;
   while (rem) {
     const int l = __ffsll((long long)rem) - 1;
     const int k = __shfl(key, l);
@@ -41,17 +39,14 @@ __device__ inline WaveMatch wave_match(int key, bool active) {
     if (active && key == k) {
       w.rank = __popcll(m & ((1ULL << lane) - 1ULL));
       w.count = __popcll(m);
-      w.slot = it;
       w.leader = (lane == l);
     }
     rem &= ~m;
-    ++it;
   }
-  w.ndist = it;
   return w;
 }
 
-__global__ __launch_bounds__(SB) void k_sort(SP P, const int *__restrict__ np2,
+__global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2,
                                              const double *__restrict__ xs,
                                              const double *__restrict__ ys,
                                              const double *__restrict__ uxs,
@@ -62,47 +57,48 @@ __global__ __launch_bounds__(SB) void k_sort(SP P, const int *__restrict__ np2,
                                              double *__restrict__ uxd, double *__restrict__ uyd,
                                              double *__restrict__ uzd, int64_t *__restrict__ idd,
                                              int *__restrict__ cumcnt, int *derr) {
-  extern __shared__ int cnt[];             // nx counters, then offsets
-  __shared__ int wl_key[NW][64], wl_cnt[NW][64], wl_n[NW];
-  __shared__ int scan_sh[SB];
+  extern __shared__ int cnt[];             // nx counters, then running offsets
   const int seg = blockIdx.x;
+  const int lane = threadIdx.x;
   const int n = np2[seg];
   const size_t base = (size_t)seg * P.cap;
-  const int w = threadIdx.x >> 6;
   int bad = 0;
-  for (int i = threadIdx.x; i < P.nx; i += SB) cnt[i] = 0;
+  for (int i = lane; i < P.nx; i += SW) cnt[i] = 0;
   __syncthreads();
   // count (sort.f90:59-62)
-  for (int k0 = 0; k0 < n; k0 += SB) {
-    const int k = k0 + threadIdx.x;
-    int key = 0;
-    bool act = k < n;
-    if (act) {
-      key = (int)xs[base + k] - P.nxgs;
-      if (key < 0 || key >= P.nx) {
+  for (int k0 = 0; k0 < n; k0 += SW * BT) {
+    int key[BT];
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int k = k0 + b * SW + lane;
+      key[b] = k < n ? (int)xs[base + k] - P.nxgs : -1;
+    }
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int k = k0 + b * SW + lane;
+      bool act = k < n;
+      if (act && (key[b] < 0 || key[b] >= P.nx)) {
         bad = 1;
         act = false;
       }
+      const WaveMatch m = wave_match(key[b], act);
+      if (act && m.leader) cnt[key[b]] += m.count;
     }
-    const WaveMatch m = wave_match(key, act);
-    if (act && m.leader) atomicAdd(&cnt[key], m.count);
   }
   __syncthreads();
-  // exclusive prefix -> cumcnt(nxs:nxe+1) (sort.f90:64-69); cnt becomes start
+  // exclusive prefix -> cumcnt(nxs:nxe+1) (sort.f90:64-69); cnt becomes the
+  // running write offset of each cell
   {
-    const int per = (P.nx + SB - 1) / SB;
-    const int lo = min(P.nx, threadIdx.x * per), hi = min(P.nx, lo + per);
+    const int per = (P.nx + SW - 1) / SW;
+    const int lo = min(P.nx, lane * per), hi = min(P.nx, lo + per);
     int s = 0;
     for (int i = lo; i < hi; ++i) s += cnt[i];
-    scan_sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < SB; off <<= 1) {
-      const int v = (int)threadIdx.x >= off ? scan_sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      scan_sh[threadIdx.x] += v;
-      __syncthreads();
+    int incl = s;
+    for (int off = 1; off < SW; off <<= 1) {
+      const int v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
     }
-    int run = scan_sh[threadIdx.x] - s;     // exclusive
+    int run = incl - s;
     int *cc = cumcnt + (size_t)seg * (P.nx + 1);
     for (int i = lo; i < hi; ++i) {
       const int c = cnt[i];
@@ -110,40 +106,47 @@ __global__ __launch_bounds__(SB) void k_sort(SP P, const int *__restrict__ np2,
       cnt[i] = run;
       run += c;
     }
-    if (threadIdx.x == SB - 1) cc[P.nx] = scan_sh[SB - 1];
+    if (lane == SW - 1) cc[P.nx] = incl;
   }
   __syncthreads();
-  // stable scatter (sort.f90:71-75)
-  for (int k0 = 0; k0 < n; k0 += SB) {
-    const int k = k0 + threadIdx.x;
-    int key = 0;
-    bool act = k < n;
-    if (act) {
-      key = (int)xs[base + k] - P.nxgs;
-      act = key >= 0 && key < P.nx;
+  // stable scatter (sort.f90:71-75), in index order
+  for (int k0 = 0; k0 < n; k0 += SW * BT) {
+    int key[BT];
+    double vx[BT], vy[BT], vux[BT], vuy[BT], vuz[BT];
+    int64_t vid[BT];
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int k = k0 + b * SW + lane;
+      key[b] = -1;
+      if (k < n) {
+        const size_t s = base + k;
+        vx[b] = xs[s];
+        vy[b] = ys[s];
+        vux[b] = uxs[s];
+        vuy[b] = uys[s];
+        vuz[b] = uzs[s];
+        vid[b] = ids[s];
+        key[b] = (int)vx[b] - P.nxgs;
+      }
     }
-    const WaveMatch m = wave_match(key, act);
-    if (act && m.leader) {
-      wl_key[w][m.slot] = key;
-      wl_cnt[w][m.slot] = m.count;
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const bool act = key[b] >= 0 && key[b] < P.nx;
+      const WaveMatch m = wave_match(key[b], act);
+      if (act) {
+        const size_t d = base + cnt[key[b]] + m.rank;
+        xd[d] = vx[b];
+        yd[d] = vy[b];
+        uxd[d] = vux[b];
+        uyd[d] = vuy[b];
+        uzd[d] = vuz[b];
+        idd[d] = vid[b];
+      }
+      // every lane has read its offset before the leaders advance them
+      __builtin_amdgcn_wave_barrier();
+      if (act && m.leader) cnt[key[b]] += m.count;
+      __builtin_amdgcn_wave_barrier();
     }
-    if ((threadIdx.x & 63) == 0) wl_n[w] = m.ndist;
-    __syncthreads();
-    if (act) {
-      int off = cnt[key] + m.rank;
-      for (int v = 0; v < w; ++v)
-        for (int e = 0; e < wl_n[v]; ++e) off += (wl_key[v][e] == key) ? wl_cnt[v][e] : 0;
-      const size_t s = base + k, d = base + off;
-      xd[d] = xs[s];
-      yd[d] = ys[s];
-      uxd[d] = uxs[s];
-      uyd[d] = uys[s];
-      uzd[d] = uzs[s];
-      idd[d] = ids[s];
-    }
-    __syncthreads();
-    if (act && m.leader) atomicAdd(&cnt[key], m.count);
-    __syncthreads();
   }
   if (bad) atomicOr(derr, DERR_RANGE);
 }
@@ -157,9 +160,13 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst) {
   P.cap = ctx->cap;
   const int nseg = ctx->g.nsp * ctx->g.ny;
   const size_t lds = sizeof(int) * (size_t)P.nx;
+  if (lds > 65536) {
+    wm_set_error(ctx, WM_ERR_ARG, "sort: nx=%d exceeds the 16384 cells one LDS histogram holds", P.nx);
+    return WM_ERR_ARG;
+  }
   const PBank &S = ctx->bank[src];
   const PBank &D = ctx->bank[dst];
-  hipLaunchKernelGGL(k_sort, dim3(nseg), dim3(SB), lds, ctx->stream, P, ctx->np2, S.x, S.y, S.ux,
+  hipLaunchKernelGGL(k_sort, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2, S.x, S.y, S.ux,
                      S.uy, S.uz, ctx->id[ctx->idcur], D.x, D.y, D.ux, D.uy, D.uz,
                      ctx->id[1 - ctx->idcur], ctx->cumcnt, ctx->derr);
   WM_HIP(hipGetLastError());

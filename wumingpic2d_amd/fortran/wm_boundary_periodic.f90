@@ -1,0 +1,73 @@
+! Drop-in module `boundary_periodic` (the public interface of
+! common/boundary_periodic.f90:26-32, 61, 99, 251, 357, 511) for one rank
+! holding the whole domain: the halo exchanges are device kernels.
+module boundary_periodic
+  use iso_c_binding
+  use wmpic_capi
+  implicit none
+  private
+  public :: boundary_periodic__init
+  public :: boundary_periodic__dfield
+  public :: boundary_periodic__particle_x
+  public :: boundary_periodic__particle_y
+  public :: boundary_periodic__curre
+  public :: boundary_periodic__phi
+
+  integer, save :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye
+
+contains
+
+  subroutine boundary_periodic__init(ndim_in,np_in,nsp_in,nxgs_in,nxge_in,nygs_in,nyge_in,nys_in,nye_in, &
+                            nup_in,ndown_in,mnpi_in,mnpr_in,ncomw_in,nerr_in,nstat_in,          &
+                            delx_in,delt_in,c_in)
+    integer, intent(in) :: ndim_in, np_in, nsp_in
+    integer, intent(in) :: nxgs_in, nxge_in, nygs_in, nyge_in, nys_in, nye_in
+    integer, intent(in) :: nup_in, ndown_in, mnpi_in, mnpr_in, ncomw_in, nerr_in, nstat_in(:)
+    real(8), intent(in) :: delx_in, delt_in, c_in
+    integer(c_int) :: ns(max(1, size(nstat_in)))
+    ns = 0
+    if (size(nstat_in) > 0) ns(1:size(nstat_in)) = nstat_in
+    call wmpic_context()
+    call wmpic_check(wm_bcp_init(wm_ctx, ndim_in, np_in, nsp_in, nxgs_in, nxge_in, nygs_in, &
+         nyge_in, nys_in, nye_in, nup_in, ndown_in, mnpi_in, mnpr_in, ncomw_in, nerr_in, ns, &
+         int(size(nstat_in), c_int), delx_in, delt_in, c_in))
+    ndim = ndim_in; np = np_in; nsp = nsp_in
+    nxgs = nxgs_in; nxge = nxge_in; nygs = nygs_in; nyge = nyge_in; nys = nys_in; nye = nye_in
+  end subroutine boundary_periodic__init
+
+  subroutine boundary_periodic__particle_x(up,np2)
+    integer, intent(in)    :: np2(nys:nye,nsp)
+    real(8), intent(inout) :: up(ndim,np,nys:nye,nsp)
+    call wmpic_context()
+    call wmpic_check(wm_bcp_particle_x(wm_ctx, up, np2))
+  end subroutine boundary_periodic__particle_x
+
+  subroutine boundary_periodic__particle_y(up,np2)
+    integer, intent(inout) :: np2(nys:nye,nsp)
+    real(8), intent(inout) :: up(ndim,np,nys:nye,nsp)
+    call wmpic_context()
+    call wmpic_check(wm_bcp_particle_y(wm_ctx, up, np2))
+  end subroutine boundary_periodic__particle_y
+
+  subroutine boundary_periodic__dfield(df,nxs,nxe,nys,nye,nxgs,nxge)
+    integer, intent(in)    :: nxs, nxe, nys, nye, nxgs, nxge
+    real(8), intent(inout) :: df(6,nxgs-2:nxge+2,nys-2:nye+2)
+    call wmpic_context()
+    call wmpic_check(wm_bcp_dfield(wm_ctx, df, nxs, nxe, nys, nye, nxgs, nxge))
+  end subroutine boundary_periodic__dfield
+
+  subroutine boundary_periodic__curre(uj,nxs,nxe,nys,nye,nxgs,nxge)
+    integer, intent(in)    :: nxs, nxe, nys, nye, nxgs, nxge
+    real(8), intent(inout) :: uj(3,nxgs-2:nxge+2,nys-2:nye+2)
+    call wmpic_context()
+    call wmpic_check(wm_bcp_curre(wm_ctx, uj, nxs, nxe, nys, nye, nxgs, nxge))
+  end subroutine boundary_periodic__curre
+
+  subroutine boundary_periodic__phi(phi,nxs,nxe,nys,nye,l)
+    integer, intent(in)    :: nxs, nxe, nys, nye, l
+    real(8), intent(inout) :: phi(nxs-1:nxe+1,nys-1:nye+1)
+    call wmpic_context()
+    call wmpic_check(wm_bcp_phi(wm_ctx, phi, nxs, nxe, nys, nye, l))
+  end subroutine boundary_periodic__phi
+
+end module boundary_periodic
