@@ -35,9 +35,12 @@ WORKLOADS = {
     "weibel_1024": (1024, 1024, 64, 1.25),
 }
 
-# Algorithmic bytes per particle of the fused push+deposit kernel in the FP64
-# form: read x, y, ux, uy, uz (40 B) and write the same 40 B.
+# Algorithmic bytes per particle of the fused push+deposit kernel.  FP64 form:
+# read x, y, ux, uy, uz (40 B) and write the same 40 B.  FP32 form: read the
+# offsets and momenta (20 B; the cell index is implied by the sorted order)
+# and write them with the new cell index (24 B).
 PUSHDEP_BYTES_PER_PARTICLE = 80.0
+PUSHDEP_BYTES_PER_PARTICLE_FP32 = 44.0
 
 
 def log(msg: str) -> None:
@@ -85,7 +88,21 @@ def main() -> int:
     ap.add_argument("--workload", default="weibel_2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+
+        n = 1 << 27                                   # 2^27 doubles = 1 GiB
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); the domain would split into y-slabs, see DESIGN.md", file=sys.stderr)
@@ -100,6 +117,7 @@ def main() -> int:
     p = weibel.grid_params(nx, ny, ppc)
     cap = int(capf * ppc * nx)
     ctx = wm.Context(0)
+    ctx.set_form(wm.FORM_FP32 if args.form == "fp32" else wm.FORM_FP64)
     ctx.set_capacity(cap, max(1024, cap // 16))
     g = (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"], p["nye"])
     ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
@@ -132,11 +150,13 @@ def main() -> int:
              "particle_bc", "sort"]
     phase_ms = {n: round(float(v) / steps, 4) for n, v in zip(names, phase)}
     pd_ms = float(phase[1]) / steps
-    achieved = PUSHDEP_BYTES_PER_PARTICLE * npart / (pd_ms * 1e-3) / 1e9
+    bpp = PUSHDEP_BYTES_PER_PARTICLE if args.form == "fp64" else PUSHDEP_BYTES_PER_PARTICLE_FP32
+    achieved = bpp * npart / (pd_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_particle<PUSH,DEPOSIT> (fused push+deposit)",
-                "bytes_per_particle": PUSHDEP_BYTES_PER_PARTICLE}
+                "kernel": ("k_particle<PUSH,DEPOSIT>" if args.form == "fp64" else "k_push_deposit32")
+                + " (fused push+deposit)",
+                "bytes_per_particle": bpp}
     out = {
         "metric": METRIC,
         "value": value,
@@ -148,11 +168,11 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.form == "fp64" else "f32 particles, f64 fields",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} Weibel pair plasma, FP64 form", "nx": nx, "ny": ny,
+        "config": {"workload": f"{args.workload} Weibel pair plasma, {args.form.upper()} form", "nx": nx, "ny": ny,
                    "ppc_per_species": ppc, "species": 2, "particles": npart, "dt": p["delt"],
-                   "form": "fp64", "parallelism": "single GPU"},
+                   "form": args.form, "parallelism": "single GPU"},
         "solve_ms_per_step": phase_ms["field_solve"],
         "cg_iters_per_step": iters / steps,
         "phase_ms_per_step": phase_ms,

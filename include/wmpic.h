@@ -48,6 +48,10 @@ extern "C" {
 #define WM_ERR_NOMEM 7      /* device allocation failed */
 #define WM_ERR_NODEV 8      /* no usable GPU */
 
+/* Particle storage forms of the device-resident state (wm_set_form). */
+#define WM_FORM_FP64 0      /* absolute x, y and u in FP64, the reference's arithmetic */
+#define WM_FORM_FP32 1      /* int32 cell index + FP32 in-cell offsets + FP32 u */
+
 /* Boundary families for wm_field_fdtd_i's set_boundary_* callbacks. */
 #define WM_BC_PERIODIC 0    /* boundary_periodic__dfield/__curre/__phi */
 
@@ -64,6 +68,12 @@ const char *wm_last_error(const wm_ctx *ctx);
 int wm_set_capacity(wm_ctx *ctx, int cap_per_row, int mover_cap);
 /* CG relative tolerance; the reference fixes 1e-6 (field.f90:339). */
 int wm_set_cg_tolerance(wm_ctx *ctx, double err);
+/* Particle form of the device-resident run (WM_FORM_FP64 default).  In the
+ * FP32 form the push and the per-tile deposit run in FP32 while the current
+ * accumulated across tiles, the field solve (CG dots and norms) and the
+ * energy sums stay FP64; the host-array entry points need the FP64 form.
+ * No reference counterpart (the reference is FP64 throughout). */
+int wm_set_form(wm_ctx *ctx, int form);
 
 /* ---- mpi_set__init (mpi_set.f90:19-49), single rank ---- */
 int wm_mpi_set_init(int nygs, int nyge, int nproc, int *nys, int *nye,

@@ -1,0 +1,108 @@
+"""FP32 particle form (int32 cell + FP32 offsets and momenta) against the FP64
+oracle.  Error bars are the FP32 ones: per-particle state after a few steps
+within a few 1e-6 of the FP64 trajectory, fields within 1e-4 of the field
+scale, energies within 1e-5.  Row counts, the particle order inside rows and
+the cell counts can differ only where an FP32 position lands on the other
+side of a cell or row boundary, so particles are matched by ID."""
+import numpy as np
+import pytest
+
+import wumingpic2d_amd as wm
+from oracle import Oracle
+from wmhelpers import make_ctx, state
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx32(p, **kw):
+    ctx = wm.Context(0)
+    ctx.set_form(wm.FORM_FP32)
+    g = (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"], p["nye"])
+    ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
+    ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
+    ctx.field__init(*g, 0, 0, 0, 0, p["delx"], p["delt"], p["c"], p["q"], p["r"], p["gfac"])
+    ctx.sort__init(*g)
+    return ctx
+
+
+def _by_id(up, np2):
+    rec = {}
+    for s in range(up.shape[3]):
+        for jl in range(up.shape[2]):
+            n = np2[jl, s]
+            ids = up[5, :n, jl, s].view(np.int64)
+            for k in range(n):
+                rec[int(ids[k])] = up[:5, k, jl, s]
+    return rec
+
+
+def test_fp32_initialiser_matches_host_state():
+    p, up, np2, cc, uf = state(32, 8, 8, 41, vth=0.05, udrift=0.5)
+    ctx = _ctx32(p)
+    ctx.init_weibel(41, 8, 0.05, 0.5)
+    gup, gnp2, gcc = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(cc)
+    ctx.state_download(up=gup, np2=gnp2, cumcnt=gcc)
+    assert np.array_equal(gnp2, np2) and np.array_equal(gcc, cc)
+    for s in range(2):
+        for jl in range(p["ny"]):
+            n = np2[jl, s]
+            a, b = up[:, :n, jl, s], gup[:, :n, jl, s]
+            assert np.array_equal(a[5].view(np.int64), b[5].view(np.int64))
+            ix = a[0].astype(np.int64)
+            ox = np.minimum(np.float32(a[0] - ix), np.float32(np.nextafter(np.float32(1), np.float32(0))))
+            assert np.array_equal(b[0], ix + ox.astype(np.float64))
+            assert np.allclose(b[2:5], a[2:5], rtol=2e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("nx,ny,ppc,seed,steps", [(32, 16, 16, 42, 6), (48, 12, 8, 43, 4)])
+def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps):
+    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=0.1)
+    ctx = _ctx32(p)
+    ctx.state_upload(up, np2, cc, uf)
+    orc = Oracle(p)
+    gp = up.copy(order="F")
+    for _ in range(steps):
+        orc.step(up, gp, uf, cc, np2)
+        ctx.step(1)
+    gup, gnp2, guf = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+    ctx.state_download(up=gup, np2=gnp2, uf=guf)
+    assert gnp2.sum() == np2.sum()
+    a, b = _by_id(up, np2), _by_id(gup, gnp2)
+    assert a.keys() == b.keys()
+    dpos = max(np.abs(a[k][:2] - b[k][:2]).max() for k in a)
+    du = max(np.abs(a[k][2:] - b[k][2:]).max() for k in a)
+    assert dpos < 2e-5, dpos
+    assert du < 2e-5, du
+    scale = np.abs(uf).max()
+    assert np.abs(guf - uf).max() < 1e-3 * scale
+    e1, e2 = orc.energy(up, uf, np2), ctx.energy()
+    assert np.allclose(e1[:2], e2[:2], rtol=1e-5)
+    assert np.allclose(e1[2:], e2[2:], rtol=1e-2, atol=1e-9 * e1[:2].sum())
+
+
+def test_fp32_run_is_deterministic():
+    p, up, np2, cc, uf = state(32, 16, 8, 44, vth=0.2)
+    outs = []
+    for _ in range(2):
+        ctx = _ctx32(p)
+        ctx.state_upload(up, np2, cc, uf)
+        ctx.step(5)
+        o, n, f = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+        ctx.state_download(up=o, np2=n, uf=f)
+        outs.append((o, n, f))
+        ctx.close()
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+    for s in range(2):
+        for jl in range(p["ny"]):
+            k = outs[0][1][jl, s]
+            assert np.array_equal(outs[0][0][:, :k, jl, s].view(np.int64),
+                                  outs[1][0][:, :k, jl, s].view(np.int64))
+
+
+def test_fp32_host_array_entry_points_refuse():
+    p, up, np2, cc, uf = state(16, 4, 2, 45)
+    ctx = _ctx32(p)
+    with pytest.raises(wm.WmError) as e:
+        ctx.particle__solv(np.zeros_like(up), up, uf, cc, p["nxgs"], p["nxge"])
+    assert e.value.code == wm.WM_ERR_ARG

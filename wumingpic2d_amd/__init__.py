@@ -52,10 +52,13 @@ WM_ERR_HIP = 6
 WM_ERR_NOMEM = 7
 WM_ERR_NODEV = 8
 BC_PERIODIC = 0
+FORM_FP64 = 0
+FORM_FP32 = 1
 
 # Symbols declared in include/wmpic.h (checked by the CPU test suite).
 EXPORTS = (
     "wm_create", "wm_destroy", "wm_last_error", "wm_set_capacity", "wm_set_cg_tolerance",
+    "wm_set_form",
     "wm_mpi_set_init", "wm_particle_init", "wm_field_init", "wm_sort_init", "wm_bcp_init",
     "wm_particle_solv", "wm_field_fdtd_i", "wm_bcp_particle_x", "wm_bcp_particle_y",
     "wm_sort_bucket", "wm_bcp_dfield", "wm_bcp_curre", "wm_bcp_phi", "wm_state_upload",
@@ -101,6 +104,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_destroy": [_P],
         "wm_set_capacity": [_P, _I, _I],
         "wm_set_cg_tolerance": [_P, _D],
+        "wm_set_form": [_P, _I],
         "wm_mpi_set_init": [_I, _I, _I] + [ctypes.POINTER(_I)] * 6,
         "wm_particle_init": [_P] + [_I] * 9 + [_D, _D, _D, _P, _P],
         "wm_field_init": [_P] + [_I] * 13 + [_D, _D, _D, _P, _P, _D],
@@ -194,6 +198,11 @@ class Context:
     def set_capacity(self, cap_per_row: int, mover_cap: int = 0) -> None:
         """Device slots per (species, row) and migration buffer per row."""
         self._check(self._lib.wm_set_capacity(self._h, int(cap_per_row), int(mover_cap)))
+
+    def set_form(self, form: int) -> None:
+        """FORM_FP64 (reference arithmetic) or FORM_FP32 (int32 cell + FP32
+        offsets and momenta; FP64 current accumulation, solve and sums)."""
+        self._check(self._lib.wm_set_form(self._h, int(form)))
 
     def set_cg_tolerance(self, err: float) -> None:
         self._check(self._lib.wm_set_cg_tolerance(self._h, float(err)))

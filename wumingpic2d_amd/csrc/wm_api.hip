@@ -11,6 +11,7 @@
 #include "wm_internal.h"
 
 size_t wm_deposit_part_elems(int nx, int ny);
+size_t wm_deposit_part_elems32(int nx, int ny);
 size_t wm_cg_partials(int nx, int ny);
 
 void wm_set_error(wm_ctx *ctx, int code, const char *fmt, ...) {
@@ -154,6 +155,24 @@ int alloc_bank(wm_ctx *ctx, PBank &b, size_t n) {
   WM_TRY(dmalloc(ctx, &b.uz, n));
   return WM_OK;
 }
+int alloc_rec32(wm_ctx *ctx, Rec32 &b, size_t n) {
+  WM_TRY(dmalloc(ctx, &b.ci, n));
+  WM_TRY(dmalloc(ctx, &b.ox, n));
+  WM_TRY(dmalloc(ctx, &b.oy, n));
+  WM_TRY(dmalloc(ctx, &b.ux, n));
+  WM_TRY(dmalloc(ctx, &b.uy, n));
+  WM_TRY(dmalloc(ctx, &b.uz, n));
+  return WM_OK;
+}
+void free_rec32(Rec32 &b) {
+  (void)hipFree(b.ci);
+  (void)hipFree(b.ox);
+  (void)hipFree(b.oy);
+  (void)hipFree(b.ux);
+  (void)hipFree(b.uy);
+  (void)hipFree(b.uz);
+  b = Rec32{};
+}
 void free_bank(PBank &b) {
   (void)hipFree(b.x);
   (void)hipFree(b.y);
@@ -210,9 +229,13 @@ int upload_particles(wm_ctx *ctx, const double *up, const int *cnt_host, int b) 
   WM_HIP(hipMemcpyAsync(ctx->stage, h.data(), sizeof(double) * elems, hipMemcpyHostToDevice,
                         ctx->stream));
   WM_HIP(hipMemcpyAsync(dcnt, cnt_host, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_aos_to_soa, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
-                     ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
-                     ctx->id[ctx->idcur]);
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(wm_k_aos_rec32(ctx, true, w, dcnt, ctx->stage, b));
+  } else {
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
+                       ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
+                       ctx->id[ctx->idcur]);
+  }
   WM_HIP(hipGetLastError());
   WM_HIP(hipStreamSynchronize(ctx->stream));
   return WM_OK;
@@ -233,9 +256,13 @@ int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b) {
   WM_TRY(ensure_stage(ctx, elems));
   int *dcnt = ctx->cnt_scratch;
   WM_HIP(hipMemcpyAsync(dcnt, cnt_host, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_soa_to_aos, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
-                     ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
-                     ctx->id[ctx->idcur]);
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(wm_k_aos_rec32(ctx, false, w, dcnt, ctx->stage, b));
+  } else {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
+                       ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
+                       ctx->id[ctx->idcur]);
+  }
   WM_HIP(hipGetLastError());
   std::vector<double> h(elems);
   WM_HIP(hipMemcpyAsync(h.data(), ctx->stage, sizeof(double) * elems, hipMemcpyDeviceToHost,
@@ -301,6 +328,14 @@ int require(wm_ctx *ctx, bool flag, const char *what) {
   return WM_OK;
 }
 
+int require_fp64(wm_ctx *ctx, const char *what) {
+  if (ctx->form != WM_FORM_FP64) {
+    wm_set_error(ctx, WM_ERR_ARG, "%s works on the FP64 particle form (reference arithmetic)", what);
+    return WM_ERR_ARG;
+  }
+  return WM_OK;
+}
+
 int check_bounds(wm_ctx *ctx, int nxs, int nxe) {
   if (nxs != ctx->g.nxgs || nxe != ctx->g.nxge) {
     wm_set_error(ctx, WM_ERR_ARG, "nxs:nxe = %d:%d must equal nxgs:nxge = %d:%d on one rank", nxs,
@@ -321,11 +356,17 @@ int wm_alloc_device(wm_ctx *ctx) {
   }
   const wm_grid &g = ctx->g;
   if (ctx->cap <= 0) ctx->cap = g.np;
+  ctx->cap = (ctx->cap + 15) / 16 * 16;   // segments start on 64-byte sectors
   if (ctx->mcap <= 0) ctx->mcap = std::max(1024, ctx->cap / 8);
   const int nseg = g.nsp * g.ny;
   const size_t slots = (size_t)nseg * ctx->cap;
-  WM_TRY(alloc_bank(ctx, ctx->bank[0], slots));
-  WM_TRY(alloc_bank(ctx, ctx->bank[1], slots));
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(alloc_rec32(ctx, ctx->b32[0], slots));
+    WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
+  } else {
+    WM_TRY(alloc_bank(ctx, ctx->bank[0], slots));
+    WM_TRY(alloc_bank(ctx, ctx->bank[1], slots));
+  }
   WM_TRY(dmalloc(ctx, &ctx->id[0], slots));
   WM_TRY(dmalloc(ctx, &ctx->id[1], slots));
   WM_TRY(dmalloc(ctx, &ctx->np2, nseg));
@@ -335,8 +376,13 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->nhole, nseg));
   WM_TRY(dmalloc(ctx, &ctx->nup, nseg));
   WM_TRY(dmalloc(ctx, &ctx->ndn, nseg));
-  WM_TRY(alloc_bank(ctx, ctx->mv_up, ms));
-  WM_TRY(alloc_bank(ctx, ctx->mv_dn, ms));
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(alloc_rec32(ctx, ctx->mv32_up, ms));
+    WM_TRY(alloc_rec32(ctx, ctx->mv32_dn, ms));
+  } else {
+    WM_TRY(alloc_bank(ctx, ctx->mv_up, ms));
+    WM_TRY(alloc_bank(ctx, ctx->mv_dn, ms));
+  }
   WM_TRY(dmalloc(ctx, &ctx->mvid_up, ms));
   WM_TRY(dmalloc(ctx, &ctx->mvid_dn, ms));
 
@@ -349,7 +395,10 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->uf, 6 * fg.ps));
   WM_TRY(dmalloc(ctx, &ctx->df, 6 * fg.ps));
   WM_TRY(dmalloc(ctx, &ctx->uj, 3 * fg.ps));
-  WM_TRY(dmalloc(ctx, &ctx->tmp, 6 * (size_t)(g.nx + 2) * (g.ny + 2)));
+  if (ctx->form == WM_FORM_FP32)
+    WM_TRY(dmalloc(ctx, &ctx->tmp32, 6 * (size_t)(g.nx + 2) * (g.ny + 2)));
+  else
+    WM_TRY(dmalloc(ctx, &ctx->tmp, 6 * (size_t)(g.nx + 2) * (g.ny + 2)));
   const size_t n = (size_t)g.nx * g.ny;
   WM_TRY(dmalloc(ctx, &ctx->gkl, 3 * n));
   WM_TRY(dmalloc(ctx, &ctx->cg_r, 3 * n));
@@ -358,7 +407,8 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->cg_s, 4));
   WM_TRY(dmalloc(ctx, &ctx->red, wm_cg_partials(g.nx, g.ny)));
   WM_TRY(dmalloc(ctx, &ctx->red_cnt, 8));
-  ctx->dep_part_elems = wm_deposit_part_elems(g.nx, g.ny);
+  ctx->dep_part_elems = ctx->form == WM_FORM_FP32 ? wm_deposit_part_elems32(g.nx, g.ny)
+                                                  : wm_deposit_part_elems(g.nx, g.ny);
   WM_TRY(dmalloc(ctx, &ctx->dep_part, ctx->dep_part_elems));
   WM_TRY(dmalloc(ctx, &ctx->diag, (size_t)nseg * 8 + 2 * (size_t)g.ny + 64));
   WM_TRY(dmalloc(ctx, &ctx->derr, 1));
@@ -424,6 +474,11 @@ int wm_destroy(wm_ctx *ctx) {
   free_bank(ctx->bank[1]);
   free_bank(ctx->mv_up);
   free_bank(ctx->mv_dn);
+  free_rec32(ctx->b32[0]);
+  free_rec32(ctx->b32[1]);
+  free_rec32(ctx->mv32_up);
+  free_rec32(ctx->mv32_dn);
+  if (ctx->tmp32) (void)hipFree(ctx->tmp32);
   void *ptrs[] = {ctx->id[0], ctx->id[1], ctx->np2, ctx->cumcnt, ctx->hole, ctx->nhole,
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
@@ -448,6 +503,16 @@ int wm_set_capacity(wm_ctx *ctx, int cap_per_row, int mover_cap) {
   if (cap_per_row < 0 || mover_cap < 0) return WM_ERR_ARG;
   ctx->cap = cap_per_row;
   ctx->mcap = mover_cap;
+  return WM_OK;
+}
+
+int wm_set_form(wm_ctx *ctx, int form) {
+  if (!ctx || (form != WM_FORM_FP64 && form != WM_FORM_FP32)) return WM_ERR_ARG;
+  if (ctx->alloc_done) {
+    wm_set_error(ctx, WM_ERR_ARG, "the particle form must be set before the first device call");
+    return WM_ERR_ARG;
+  }
+  ctx->form = form;
   return WM_OK;
 }
 
@@ -533,6 +598,7 @@ int wm_particle_solv(wm_ctx *ctx, double *gp, const double *up, const double *uf
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->pinit, "particle__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(require_fp64(ctx, "particle__solv"));
   WM_TRY(wm_alloc_device(ctx));
   const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
   WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
@@ -560,6 +626,7 @@ int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
   }
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(require_fp64(ctx, "field__fdtd_i"));
   WM_TRY(wm_alloc_device(ctx));
   const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
   WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
@@ -578,6 +645,7 @@ int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
 int wm_bcp_particle_x(wm_ctx *ctx, double *up, const int *np2) {
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require_fp64(ctx, "boundary_periodic__particle_x"));
   WM_TRY(wm_alloc_device(ctx));
   WM_TRY(upload_particles(ctx, up, np2, 0));
   WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg_of(ctx), hipMemcpyHostToDevice,
@@ -590,6 +658,7 @@ int wm_bcp_particle_x(wm_ctx *ctx, double *up, const int *np2) {
 int wm_bcp_particle_y(wm_ctx *ctx, double *up, int *np2) {
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require_fp64(ctx, "boundary_periodic__particle_y"));
   WM_TRY(wm_alloc_device(ctx));
   WM_TRY(upload_particles(ctx, up, np2, 0));
   const int nseg = nseg_of(ctx);
@@ -615,6 +684,7 @@ int wm_sort_bucket(wm_ctx *ctx, double *gp, const double *up, int *cumcnt, const
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->sinit, "sort__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(require_fp64(ctx, "sort__bucket"));
   WM_TRY(wm_alloc_device(ctx));
   const int nseg = nseg_of(ctx);
   WM_TRY(upload_particles(ctx, up, np2, 1));
@@ -724,11 +794,12 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     hipEvent_t *e = ctx->ev;
     WM_HIP(hipEventRecord(e[0], ctx->stream));
     // particle__solv + ele_cur (proj/weibel/app.f90:102, field.f90:121)
-    WM_TRY(wm_k_cell_fields(ctx));
+    const bool f32 = ctx->form == WM_FORM_FP32;
+    WM_TRY(f32 ? wm_k_cell_fields32(ctx) : wm_k_cell_fields(ctx));
     WM_HIP(hipEventRecord(e[1], ctx->stream));
-    WM_TRY(wm_k_push_deposit(ctx, 0, 1));
+    WM_TRY(f32 ? wm_k_push_deposit32(ctx, 0, 1) : wm_k_push_deposit(ctx, 0, 1));
     WM_HIP(hipEventRecord(e[2], ctx->stream));
-    WM_TRY(wm_k_deposit_gather(ctx));
+    WM_TRY(f32 ? wm_k_deposit_gather32(ctx) : wm_k_deposit_gather(ctx));
     WM_TRY(wm_k_bc_curre(ctx, ctx->uj));          // field.f90:122
     WM_HIP(hipEventRecord(e[3], ctx->stream));
     WM_TRY(wm_k_field_solve(ctx));                // field.f90:125-184

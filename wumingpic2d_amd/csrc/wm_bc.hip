@@ -65,20 +65,63 @@ __global__ void k_particle_x(BP P, const int *__restrict__ np2, double *__restri
 }
 
 constexpr int YW = 64;     // one wavefront per segment
-constexpr int YT = 4;      // tiles in flight
+constexpr int YT = 4;      // tiles unrolled per walk step
 
 __device__ inline int lanes_below(unsigned long long m) {
   const int lane = threadIdx.x & 63;
   return __popcll(m & ((1ULL << lane) - 1ULL));
 }
 
+// Row change of particle p of row j: -1, 0, +1, or 2 (a jump the reference
+// does not index).  FP64: jpos = int(y/delx) (boundary_periodic.f90:147),
+// with particle_x's round-down wrap of x applied first when fused.
+__device__ inline int row_change(const Rec64 &R, size_t p, int j, bool fuse_x, const BP &P,
+                                 int &bad) {
+  if (fuse_x) {
+    double xv = R.x[p];
+    const double x0 = xv;
+    wrap_x(xv, P);
+    if (xv != x0) R.x[p] = xv;
+    const int inew = (int)(xv / P.delx);
+    if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
+  }
+  const int jpos = (int)(R.y[p] / P.delx);
+  const int d = jpos - j;
+  return (d >= -1 && d <= 1) ? d : 2;
+}
+// FP32: the row offset oy leaves [0, 1) (x wraps in the push).
+__device__ inline int row_change(const Rec32 &R, size_t p, int, bool, const BP &, int &) {
+  const float oy = R.oy[p];
+  return oy < 0.0f ? (oy < -1.0f ? 2 : -1) : (oy >= 1.0f ? (oy >= 2.0f ? 2 : 1) : 0);
+}
+
+// The record of a leaver as it enters the receiving row: FP64 wraps y across
+// the periodic boundary in round-down arithmetic (:150-154); FP32 rebases oy
+// on the new row.
+__device__ inline Rec64::Val arriving(const Rec64 &R, size_t p, int j, int d, const BP &P) {
+  Rec64::Val v = R.load(p);
+  const int jpos = j + d;
+  if (jpos <= P.nygs - 1) v.y = add_rd(v.y, P.spany);
+  else if (jpos >= P.nyge + 1) v.y = add_rd(v.y, -P.spany);
+  return v;
+}
+__device__ inline Rec32::Val arriving(const Rec32 &R, size_t p, int, int d, const BP &) {
+  Rec32::Val v = R.load(p);
+  v.oy = d > 0 ? v.oy - 1.0f : wm_below_one(v.oy + 1.0f);
+  return v;
+}
+
 // pass 1: one wavefront per segment walks it in index order.
-__global__ __launch_bounds__(YW) void k_particle_y1(
-    BP P, bool fuse_x, const int *__restrict__ np2, double *__restrict__ x,
-    double *__restrict__ y, const double *__restrict__ ux, const double *__restrict__ uy,
-    const double *__restrict__ uz, const int64_t *__restrict__ id, int *__restrict__ hole,
-    int *__restrict__ nhole, int *__restrict__ nup, int *__restrict__ ndn, PBank mu,
-    int64_t *mid_u, PBank md, int64_t *mid_d, int *derr) {
+template <class R>
+__global__ __launch_bounds__(YW) void k_particle_y1(BP P, bool fuse_x,
+                                                    const int *__restrict__ np2, R B,
+                                                    const int64_t *__restrict__ id,
+                                                    int *__restrict__ hole,
+                                                    int *__restrict__ nhole,
+                                                    int *__restrict__ nup,
+                                                    int *__restrict__ ndn, R mu,
+                                                    int64_t *mid_u, R md, int64_t *mid_d,
+                                                    int *derr) {
   const int seg = blockIdx.x;
   const int lane = threadIdx.x;
   const int jl = seg % P.ny;
@@ -89,38 +132,17 @@ __global__ __launch_bounds__(YW) void k_particle_y1(
   int cu = 0, cd = 0;
   int bad = 0;
   for (int k0 = 0; k0 < n; k0 += YW * YT) {
-    double xv[YT], yv[YT];
+    int dir[YT];
 #pragma unroll
     for (int t = 0; t < YT; ++t) {
       const int k = k0 + t * YW + lane;
-      xv[t] = 0.0;
-      yv[t] = 0.0;
-      if (k < n) {
-        xv[t] = x[base + k];
-        yv[t] = y[base + k];
-      }
+      dir[t] = k < n ? row_change(B, base + k, j, fuse_x, P, bad) : 0;
     }
 #pragma unroll
     for (int t = 0; t < YT; ++t) {
       const int k = k0 + t * YW + lane;
-      bool up = false, dn = false;
-      if (k < n) {
-        if (fuse_x) {
-          const double x0 = xv[t];
-          wrap_x(xv[t], P);
-          if (xv[t] != x0) x[base + k] = xv[t];
-          const int inew = (int)(xv[t] / P.delx);
-          if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
-        }
-        const int jpos = (int)(yv[t] / P.delx);
-        if (jpos != j) {
-          if (jpos == j + 1) up = true;
-          else if (jpos == j - 1) dn = true;
-          else bad |= DERR_JUMP;
-          if (jpos <= P.nygs - 1) yv[t] = add_rd(yv[t], P.spany);
-          else if (jpos >= P.nyge + 1) yv[t] = add_rd(yv[t], -P.spany);
-        }
-      }
+      if (dir[t] == 2) bad |= DERR_JUMP;
+      const bool up = dir[t] == 1, dn = dir[t] == -1;
       const unsigned long long mu_ = __ballot(up), md_ = __ballot(dn);
       if (up || dn) {
         // holes in ascending index order: flag(:,j) of :163-164
@@ -129,15 +151,9 @@ __global__ __launch_bounds__(YW) void k_particle_y1(
         else bad |= DERR_MOVER_OVER;
         const int pos = up ? cu + lanes_below(mu_) : cd + lanes_below(md_);
         if (pos < P.mcap) {
-          const PBank &B = up ? mu : md;
-          int64_t *bid = up ? mid_u : mid_d;
           const size_t q = mb + pos;
-          B.x[q] = xv[t];
-          B.y[q] = yv[t];
-          B.ux[q] = ux[base + k];
-          B.uy[q] = uy[base + k];
-          B.uz[q] = uz[base + k];
-          bid[q] = id[base + k];
+          (up ? mu : md).store(q, arriving(B, base + k, j, dir[t], P));
+          (up ? mid_u : mid_d)[q] = id[base + k];
         } else {
           bad |= DERR_MOVER_OVER;
         }
@@ -155,24 +171,28 @@ __global__ __launch_bounds__(YW) void k_particle_y1(
 }
 
 // pass 2: one wavefront per segment.
-__global__ __launch_bounds__(YW) void k_particle_y2(
-    BP P, int *__restrict__ np2, double *__restrict__ x, double *__restrict__ y,
-    double *__restrict__ ux, double *__restrict__ uy, double *__restrict__ uz,
-    int64_t *__restrict__ id, const int *__restrict__ hole, const int *__restrict__ nhole,
-    const int *__restrict__ nup, const int *__restrict__ ndn, PBank mu,
-    const int64_t *mid_u, PBank md, const int64_t *mid_d, int *derr) {
+template <class R>
+__global__ __launch_bounds__(YW) void k_particle_y2(BP P, int *__restrict__ np2, R B,
+                                                    int64_t *__restrict__ id,
+                                                    const int *__restrict__ hole,
+                                                    const int *__restrict__ nhole,
+                                                    const int *__restrict__ nup,
+                                                    const int *__restrict__ ndn, R mu,
+                                                    const int64_t *mid_u, R md,
+                                                    const int64_t *mid_d, int *derr) {
   const int seg = blockIdx.x;
   const int lane = threadIdx.x;
   const int s = seg / P.ny, jl = seg % P.ny;
   const int below = s * P.ny + (jl == 0 ? P.ny - 1 : jl - 1);
   const int above = s * P.ny + (jl == P.ny - 1 ? 0 : jl + 1);
-  // arrival lists: first F, then S
+  // arrival lists, in the order the self-SENDRECVs append them (:174-189):
+  // first F, then S
   const bool first_row = (jl == 0);
-  const PBank &FB = first_row ? md : mu;
+  const R FB = first_row ? md : mu;
   const int64_t *Fid = first_row ? mid_d : mid_u;
   const int Fseg = first_row ? above : below;
   const int nF = first_row ? ndn[above] : nup[below];
-  const PBank &SB = first_row ? mu : md;
+  const R SB = first_row ? mu : md;
   const int64_t *Sid = first_row ? mid_u : mid_d;
   const int Sseg = first_row ? below : above;
   const int nS = first_row ? nup[below] : ndn[above];
@@ -191,20 +211,12 @@ __global__ __launch_bounds__(YW) void k_particle_y2(
   for (int a = lane; a < A; a += YW) {
     const int dst = a < H ? hl[a] : N + (a - H);
     const bool f = a < nF;
-    const PBank &B = f ? FB : SB;
-    const int64_t *bid = f ? Fid : Sid;
     const size_t q = (size_t)(f ? Fseg : Sseg) * P.mcap + (f ? a : a - nF);
-    const size_t d = base + dst;
-    x[d] = B.x[q];
-    y[d] = B.y[q];
-    ux[d] = B.ux[q];
-    uy[d] = B.uy[q];
-    uz[d] = B.uz[q];
-    id[d] = bid[q];
+    B.store(base + dst, (f ? FB : SB).load(q));
+    id[base + dst] = (f ? Fid : Sid)[q];
   }
   if (A < H) {
     const int m = H - A;
-    const int L = N - m;
     int done = 0;    // non-hole tail slots already placed
     for (int q0 = 0; q0 < m; q0 += YW) {
       const int qi = q0 + lane;          // slot N-1-qi, walking down from the top
@@ -224,18 +236,12 @@ __global__ __launch_bounds__(YW) void k_particle_y2(
       const unsigned long long mk = __ballot(nonhole);
       const int nrank = done + lanes_below(mk);
       if (nonhole && A + nrank < H) {
-        const int dst = hl[A + nrank];   // the nrank-th remaining hole (< L)
-        const size_t sd = base + src, dd = base + dst;
-        x[dd] = x[sd];
-        y[dd] = y[sd];
-        ux[dd] = ux[sd];
-        uy[dd] = uy[sd];
-        uz[dd] = uz[sd];
-        id[dd] = id[sd];
+        const int dst = hl[A + nrank];   // the nrank-th remaining hole (below N-m)
+        B.store(base + dst, B.load(base + src));
+        id[base + dst] = id[base + src];
       }
       done += __popcll(mk);
     }
-    (void)L;
   }
   if (lane == 0) np2[seg] = newn;
 }
@@ -266,14 +272,24 @@ int wm_k_particle_x(wm_ctx *ctx, int bank) {
 int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x) {
   BP P = make_bp(ctx);
   const int nseg = ctx->g.nsp * ctx->g.ny;
-  const PBank &B = ctx->bank[bank];
   int64_t *id = ctx->id[ctx->idcur];
-  hipLaunchKernelGGL(k_particle_y1, dim3(nseg), dim3(YW), 0, ctx->stream, P, fuse_x, ctx->np2,
-                     B.x, B.y, B.ux, B.uy, B.uz, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn,
-                     ctx->mv_up, ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
-  hipLaunchKernelGGL(k_particle_y2, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2, B.x, B.y,
-                     B.ux, B.uy, B.uz, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
-                     ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
+  if (ctx->form == WM_FORM_FP32) {
+    const Rec32 &B = ctx->b32[bank];
+    hipLaunchKernelGGL(k_particle_y1<Rec32>, dim3(nseg), dim3(YW), 0, ctx->stream, P, false,
+                       ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv32_up,
+                       ctx->mvid_up, ctx->mv32_dn, ctx->mvid_dn, ctx->derr);
+    hipLaunchKernelGGL(k_particle_y2<Rec32>, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2,
+                       B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv32_up,
+                       ctx->mvid_up, ctx->mv32_dn, ctx->mvid_dn, ctx->derr);
+  } else {
+    const Rec64 &B = ctx->bank[bank];
+    hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, fuse_x,
+                       ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
+                       ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
+    hipLaunchKernelGGL(k_particle_y2<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2,
+                       B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
+                       ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
+  }
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

@@ -35,7 +35,30 @@ struct IP {
   double vth, ud, xhi;   // xhi = largest double below nxge+1
 };
 
-__global__ void k_init(IP P, PBank B, int64_t *id, int *np2) {
+// Stores one generated particle in the bank's form.
+__device__ inline void put(const Rec64 &B, size_t p, double x, double y, double ux, double uy,
+                           double uz, const IP &P, int j) {
+  B.x[p] = x;
+  B.y[p] = y;
+  B.ux[p] = ux;
+  B.uy[p] = uy;
+  B.uz[p] = uz;
+  (void)P;
+  (void)j;
+}
+__device__ inline void put(const Rec32 &B, size_t p, double x, double y, double ux, double uy,
+                           double uz, const IP &P, int j) {
+  const int ix = (int)x;
+  B.ci[p] = ix - P.nxgs;
+  B.ox[p] = wm_below_one((float)(x - ix));
+  B.oy[p] = wm_below_one((float)(y - j));
+  B.ux[p] = (float)ux;
+  B.uy[p] = (float)uy;
+  B.uz[p] = (float)uz;
+}
+
+template <class R>
+__global__ void k_init(IP P, R B, int64_t *id, int *np2) {
   const int seg = blockIdx.y;
   const int s = seg / P.ny, jl = seg % P.ny;
   const int j = P.nys + jl;
@@ -49,11 +72,8 @@ __global__ void k_init(IP P, PBank B, int64_t *id, int *np2) {
     y = fmin(y, yhi);
     const uint64_t vs = P.seed ^ 0x5851F42D4C957F2DULL;
     const size_t p = (size_t)seg * P.cap + k;
-    B.x[p] = x;
-    B.y[p] = y;
-    B.ux[p] = P.vth * gauss(vs, 3 * gidx);
-    B.uy[p] = P.vth * gauss(vs, 3 * gidx + 1);
-    B.uz[p] = ((k & 1) ? -P.ud : P.ud) + P.vth * gauss(vs, 3 * gidx + 2);
+    put(B, p, x, y, P.vth * gauss(vs, 3 * gidx), P.vth * gauss(vs, 3 * gidx + 1),
+        ((k & 1) ? -P.ud : P.ud) + P.vth * gauss(vs, 3 * gidx + 2), P, j);
     id[p] = -(int64_t)(gidx + 1);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) np2[seg] = P.n;
@@ -78,8 +98,12 @@ int wm_k_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udr
   P.xhi = nextafter((double)(g.nxge + 1), 0.0);
   const int nseg = g.nsp * g.ny;
   const int nb = std::max(1, std::min(64, wm_cdiv(n, 256)));
-  hipLaunchKernelGGL(k_init, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->bank[1],
-                     ctx->id[ctx->idcur], ctx->np2);
+  if (ctx->form == WM_FORM_FP32)
+    hipLaunchKernelGGL(k_init<Rec32>, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->b32[1],
+                       ctx->id[ctx->idcur], ctx->np2);
+  else
+    hipLaunchKernelGGL(k_init<Rec64>, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->bank[1],
+                       ctx->id[ctx->idcur], ctx->np2);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

@@ -19,6 +19,7 @@
 #include <string>
 
 #include "../../include/wmpic.h"
+#include "wm_records.h"
 
 #define WM_MAXSP 8
 
@@ -74,9 +75,7 @@ struct CgScal {
   int active, ite, err, pad;
 };
 
-struct PBank {
-  double *x, *y, *ux, *uy, *uz;
-};
+using PBank = Rec64;
 
 struct wm_ctx {
   int device = 0;
@@ -96,8 +95,13 @@ struct wm_ctx {
   int mcap = 0;            // migrating particles per row and direction
   bool alloc_done = false;
 
-  // particles
+  // particles: FP64 form (bank) or FP32 form (b32), chosen before allocation
+  int form = WM_FORM_FP64;
   PBank bank[2]{};
+  Rec32 b32[2]{};
+  Rec32 mv32_up{}, mv32_dn{};
+  float *tmp32 = nullptr;  // cell-centred fields of the FP32 push
+  Rec64 b64(int k) const { return bank[k]; }
   int64_t *id[2]{};
   int idcur = 0;           // which id array goes with bank 0 / bank 1 data
   int *np2 = nullptr;      // [nseg]
@@ -162,6 +166,12 @@ int wm_k_particle_x(wm_ctx *ctx, int bank);
 int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x);
 // Sort (wm_sort.hip)
 int wm_k_sort(wm_ctx *ctx, int src, int dst);
+// FP32 form (wm_particle32.hip)
+int wm_k_cell_fields32(wm_ctx *ctx);
+int wm_k_push_deposit32(wm_ctx *ctx, int src, int dst);
+int wm_k_deposit_gather32(wm_ctx *ctx);
+int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);
+int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk);
 // Synthetic initial state (wm_init.hip)
 int wm_k_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udrift);
 

@@ -241,7 +241,7 @@ __device__ inline double group_sum(double v) {
 // (bit m of mask = particle kb+sub+GROUP*m; particles beyond the mask are
 // re-examined), combined over the group and added by lane 0 to cellbuf.
 template <int C>
-__device__ inline void crossing_pass(double *cb, int sub, unsigned long long mask, int kb,
+__device__ inline void crossing_pass(double *cb, int sub, unsigned long long mask, int ka,
                                      int ke, size_t base, const double *xa, const double *ya,
                                      const double *xb, const double *yb, const double *uxb,
                                      const double *uyb, const double *uzb, int i, int j,
@@ -268,9 +268,9 @@ __device__ inline void crossing_pass(double *cb, int sub, unsigned long long mas
   while (mask) {
     const int m = __ffsll((long long)mask) - 1;
     mask &= mask - 1;
-    one(kb + sub + GROUP * m, true);
+    one(ka + sub + GROUP * m, true);
   }
-  for (int k = kb + sub + GROUP * 64; k < ke; k += GROUP) one(k, false);
+  for (int k = ka + sub + GROUP * 64; k < ke; k += GROUP) one(k, false);
   const int off = C == 0 ? 0 : (C == 1 ? 20 : 40);
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -324,9 +324,13 @@ __global__ __launch_bounds__(256, 3) void k_particle(
       unsigned long long mask = 0ULL;
       // the next particle's state is loaded while the current one is
       // computed (one step of software pipelining per lane)
-      int k = kb + sub;
+      // windows of GROUP particles aligned to 64-byte sectors (segments start
+      // on a sector): a cell's first and last windows are shared with its
+      // neighbours and the lanes outside [kb, ke) idle there
+      const int ka = kb & ~(GROUP - 1);
+      int k = ka + sub;
       double nx0 = 0, ny0 = 0, nux = 0, nuy = 0, nuz = 0;
-      if (k < ke) {
+      if (k >= kb && k < ke) {
         const size_t p = base + (size_t)k;
         nx0 = xa[p]; ny0 = ya[p];
         if (PUSH) { nux = uxa[p]; nuy = uya[p]; nuz = uza[p]; }
@@ -339,6 +343,7 @@ __global__ __launch_bounds__(256, 3) void k_particle(
           nx0 = xa[pn]; ny0 = ya[pn];
           if (PUSH) { nux = uxa[pn]; nuy = uya[pn]; nuz = uza[pn]; }
         }
+        if (k < kb) continue;
         double x1, y1, ux, uy, uz;
         // keep the 54 tile values in LDS instead of hoisting them into
         // registers for the whole cell loop (occupancy)
@@ -370,12 +375,12 @@ __global__ __launch_bounds__(256, 3) void k_particle(
           if (sub == 0) cb[core_slot(c)] += v;
         }
       }
-      if (DEPOSIT && __ballot(mask != 0ULL || ke - kb > GROUP * 64)) {
+      if (DEPOSIT && __ballot(mask != 0ULL || ke - ka > GROUP * 64)) {
         // cell-crossing particles: full 5x5 stencils, one component at a time
         // (stream-ordered after this lane's own writes of the new state)
-        crossing_pass<0>(cb, sub, mask, kb, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
-        crossing_pass<1>(cb, sub, mask, kb, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
-        crossing_pass<2>(cb, sub, mask, kb, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
+        crossing_pass<0>(cb, sub, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
+        crossing_pass<1>(cb, sub, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
+        crossing_pass<2>(cb, sub, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
       }
     }
   }

@@ -1,0 +1,53 @@
+// wm_records.h -- the two particle record forms kept in HBM, as
+// structure-of-arrays banks, and the per-record operations the migration and
+// sort kernels share.
+//
+//  Rec64 (FP64 form, the reference's arithmetic): absolute x, y and
+//        u = gamma v in FP64 -- 40 B per particle (+ 8 B ID).
+//  Rec32 (FP32 form): cell index ci (int32, x-cell relative to nxgs), the
+//        in-cell offsets ox in [0, 1) and oy (relative to the row; in [0, 1)
+//        except between the push and the row migration), u in FP32 --
+//        24 B per particle (+ 8 B ID).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct Rec64 {
+  double *x, *y, *ux, *uy, *uz;
+  struct Val {
+    double x, y, ux, uy, uz;
+  };
+  __device__ inline Val load(size_t p) const { return Val{x[p], y[p], ux[p], uy[p], uz[p]}; }
+  __device__ inline void store(size_t p, const Val &v) const {
+    x[p] = v.x;
+    y[p] = v.y;
+    ux[p] = v.ux;
+    uy[p] = v.uy;
+    uz[p] = v.uz;
+  }
+  // x-cell relative to nxgs, as sort.f90:60 bins it (int(x), delx = 1)
+  __device__ inline int key(const Val &v, int nxgs) const { return (int)v.x - nxgs; }
+};
+
+struct Rec32 {
+  int *ci;
+  float *ox, *oy, *ux, *uy, *uz;
+  struct Val {
+    int ci;
+    float ox, oy, ux, uy, uz;
+  };
+  __device__ inline Val load(size_t p) const { return Val{ci[p], ox[p], oy[p], ux[p], uy[p], uz[p]}; }
+  __device__ inline void store(size_t p, const Val &v) const {
+    ci[p] = v.ci;
+    ox[p] = v.ox;
+    oy[p] = v.oy;
+    ux[p] = v.ux;
+    uy[p] = v.uy;
+    uz[p] = v.uz;
+  }
+  __device__ inline int key(const Val &v, int) const { return v.ci; }
+};
+
+// Largest float below 1: offsets are kept in [0, 1) after a cell change.
+__device__ inline float wm_below_one(float v) { return v < 1.0f ? v : 0x1.fffffep-1f; }

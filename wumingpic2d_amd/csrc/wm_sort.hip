@@ -46,18 +46,13 @@ __device__ inline WaveMatch wave_match(int key, bool active) {
   return w;
 }
 
-__global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2,
-                                             const double *__restrict__ xs,
-                                             const double *__restrict__ ys,
-                                             const double *__restrict__ uxs,
-                                             const double *__restrict__ uys,
-                                             const double *__restrict__ uzs,
-                                             const int64_t *__restrict__ ids,
-                                             double *__restrict__ xd, double *__restrict__ yd,
-                                             double *__restrict__ uxd, double *__restrict__ uyd,
-                                             double *__restrict__ uzd, int64_t *__restrict__ idd,
+template <class R>
+__global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, R src,
+                                             const int64_t *__restrict__ ids, R dst,
+                                             int64_t *__restrict__ idd,
                                              int *__restrict__ cumcnt, int *derr) {
   extern __shared__ int cnt[];             // nx counters, then running offsets
+  using V = typename R::Val;
   const int seg = blockIdx.x;
   const int lane = threadIdx.x;
   const int n = np2[seg];
@@ -71,7 +66,7 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2,
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
       const int k = k0 + b * SW + lane;
-      key[b] = k < n ? (int)xs[base + k] - P.nxgs : -1;
+      key[b] = k < n ? src.key(src.load(base + k), P.nxgs) : -1;
     }
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
@@ -112,7 +107,7 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2,
   // stable scatter (sort.f90:71-75), in index order
   for (int k0 = 0; k0 < n; k0 += SW * BT) {
     int key[BT];
-    double vx[BT], vy[BT], vux[BT], vuy[BT], vuz[BT];
+    V v[BT];
     int64_t vid[BT];
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
@@ -120,13 +115,9 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2,
       key[b] = -1;
       if (k < n) {
         const size_t s = base + k;
-        vx[b] = xs[s];
-        vy[b] = ys[s];
-        vux[b] = uxs[s];
-        vuy[b] = uys[s];
-        vuz[b] = uzs[s];
+        v[b] = src.load(s);
         vid[b] = ids[s];
-        key[b] = (int)vx[b] - P.nxgs;
+        key[b] = src.key(v[b], P.nxgs);
       }
     }
 #pragma unroll
@@ -135,11 +126,7 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2,
       const WaveMatch m = wave_match(key[b], act);
       if (act) {
         const size_t d = base + cnt[key[b]] + m.rank;
-        xd[d] = vx[b];
-        yd[d] = vy[b];
-        uxd[d] = vux[b];
-        uyd[d] = vuy[b];
-        uzd[d] = vuz[b];
+        dst.store(d, v[b]);
         idd[d] = vid[b];
       }
       // every lane has read its offset before the leaders advance them
@@ -164,11 +151,15 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst) {
     wm_set_error(ctx, WM_ERR_ARG, "sort: nx=%d exceeds the 16384 cells one LDS histogram holds", P.nx);
     return WM_ERR_ARG;
   }
-  const PBank &S = ctx->bank[src];
-  const PBank &D = ctx->bank[dst];
-  hipLaunchKernelGGL(k_sort, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2, S.x, S.y, S.ux,
-                     S.uy, S.uz, ctx->id[ctx->idcur], D.x, D.y, D.ux, D.uy, D.uz,
-                     ctx->id[1 - ctx->idcur], ctx->cumcnt, ctx->derr);
+  if (ctx->form == WM_FORM_FP32) {
+    hipLaunchKernelGGL(k_sort<Rec32>, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2,
+                       ctx->b32[src], ctx->id[ctx->idcur], ctx->b32[dst], ctx->id[1 - ctx->idcur],
+                       ctx->cumcnt, ctx->derr);
+  } else {
+    hipLaunchKernelGGL(k_sort<Rec64>, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2,
+                       ctx->b64(src), ctx->id[ctx->idcur], ctx->b64(dst), ctx->id[1 - ctx->idcur],
+                       ctx->cumcnt, ctx->derr);
+  }
   WM_HIP(hipGetLastError());
   ctx->idcur = 1 - ctx->idcur;
   return WM_OK;
