@@ -80,6 +80,89 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
                       f"{steps} steps after 1 warm-up, FP64 C restatement with {threads} OpenMP threads"}
 
 
+def load_traffic(workload: str, form: str):
+    """HBM bytes per push+deposit launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["workloads"][workload][form]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def make_context(wm, weibel, workload: str, form: str):
+    nx, ny, ppc, capf = WORKLOADS[workload]
+    p = weibel.grid_params(nx, ny, ppc)
+    cap = int(capf * ppc * nx)
+    ctx = wm.Context(0)
+    ctx.set_form(wm.FORM_FP32 if form == "fp32" else wm.FORM_FP64)
+    ctx.set_capacity(cap, max(1024, cap // 16))
+    g = (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"], p["nye"])
+    ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
+    ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
+    ctx.field__init(*g, 0, 0, 0, 0, p["delx"], p["delt"], p["c"], p["q"], p["r"], p["gfac"])
+    ctx.sort__init(*g)
+    log(f"workload {workload} ({form}): {nx}x{ny}, {ppc} ppc/species, capacity {cap}/row")
+    return ctx, p, cap
+
+
+PHASES = ["cell_fields", "push_deposit", "current_gather_fold", "field_solve", "particle_bc",
+          "sort"]
+
+
+def run_leg(wm, weibel, args, form: str, solver: str, steps: int, warmup: int) -> dict:
+    """Warm-up (plain solves; with the learned solver they record pairs and
+    the net is trained after them), then `steps` timed steps."""
+    import numpy as np
+
+    ctx, p, cap = make_context(wm, weibel, args.workload, form)
+    ppc = WORKLOADS[args.workload][2]
+    ctx.init_weibel(args.seed, ppc, 0.05, 0.5)
+    npart = ctx.particle_count()
+    log(f"{npart} particles generated on the device")
+    train = {}
+    if solver == "fcg":
+        ctx.pc_collect(16, 4)
+    for w in range(warmup):
+        ctx.step(1)
+        log(f"warm-up step {w + 1}/{warmup}")
+    if solver == "fcg":
+        t0 = time.perf_counter()
+        loss = ctx.pc_train(args.pc_steps, 3e-3, args.seed)
+        train = {"pairs": ctx.pc_pair_count(), "adam_steps": args.pc_steps,
+                 "seconds": round(time.perf_counter() - t0, 3),
+                 "loss_first": float(loss[0]), "loss_last10": float(loss[1])}
+        log(f"learned preconditioner trained: {train}")
+        ctx.pc_collect(0, 0)
+        ctx.set_solver(wm.SOLVER_FCG)
+    phase = np.zeros(6)
+    iters = fcg_iters = plain_iters = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.step(1)
+        phase += ctx.phase_times()[:6]
+        iters += int(ctx.cg_iters().sum())
+        a, b = ctx.solve_stats()
+        fcg_iters += int(a.sum())
+        plain_iters += int(b.sum())
+        if (k + 1) % 5 == 0 or k + 1 == steps:
+            log(f"timed step {k + 1}/{steps} ({form}, {solver})")
+    elapsed = time.perf_counter() - t0
+    energy = ctx.energy()
+    ctx.close()
+    return {"form": form, "solver": solver, "particles": npart, "steps": steps,
+            "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "value": npart * steps / elapsed,
+            "phase_ms_per_step": {n: round(float(v) / steps, 4) for n, v in zip(PHASES, phase)},
+            "push_deposit_ms": float(phase[1]) / steps,
+            "solve_ms_per_step": round(float(phase[3]) / steps, 4),
+            "cg_iters_per_step": iters / steps,
+            "fcg_iters_per_step": fcg_iters / steps, "plain_iters_per_step": plain_iters / steps,
+            "energy": [float(e) for e in energy], "pc_training": train, "p": p}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,8 +170,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="weibel_2048", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain FP64 comparison leg")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--form", default="fp32", choices=["fp64", "fp32"])
+    ap.add_argument("--solver", default=None, choices=["cg", "fcg"],
+                    help="implicit solve: plain CG or flexible CG with the learned "
+                         "preconditioner (default: fcg for fp32, cg for fp64)")
+    ap.add_argument("--pc-steps", type=int, default=200, help="Adam steps of the preconditioner")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -105,81 +194,67 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
-              f"{args.gpus}); the domain would split into y-slabs, see DESIGN.md", file=sys.stderr)
+              f"{args.gpus}); the domain would split into y-slabs of rows, one per GPU",
+              file=sys.stderr)
         return 2
-
-    import numpy as np
+    solver = args.solver or ("fcg" if args.form == "fp32" else "cg")
 
     import wumingpic2d_amd as wm
     from wumingpic2d_amd import weibel
 
-    nx, ny, ppc, capf = WORKLOADS[args.workload]
-    p = weibel.grid_params(nx, ny, ppc)
-    cap = int(capf * ppc * nx)
-    ctx = wm.Context(0)
-    ctx.set_form(wm.FORM_FP32 if args.form == "fp32" else wm.FORM_FP64)
-    ctx.set_capacity(cap, max(1024, cap // 16))
-    g = (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"], p["nye"])
-    ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
-    ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
-    ctx.field__init(*g, 0, 0, 0, 0, p["delx"], p["delt"], p["c"], p["q"], p["r"], p["gfac"])
-    ctx.sort__init(*g)
-    log(f"workload {args.workload}: {nx}x{ny}, {ppc} ppc/species, capacity {cap}/row")
-    ctx.init_weibel(args.seed, ppc, 0.05, 0.5)
-    npart = ctx.particle_count()
-    log(f"{npart} particles generated on the device")
-
-    for w in range(args.warmup):
-        ctx.step(1)
-        log(f"warm-up step {w + 1}/{args.warmup}")
-    phase = np.zeros(6)
-    iters = 0
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ctx.step(1)
-        phase += ctx.phase_times()[:6]
-        iters += int(ctx.cg_iters().sum())
-        if (k + 1) % 5 == 0 or k + 1 == args.steps:
-            log(f"timed step {k + 1}/{args.steps}")
-    elapsed = time.perf_counter() - t0
-    steps = args.steps
-    ms_per_step = 1e3 * elapsed / steps
-    value = npart * steps / elapsed
-    energy = ctx.energy()
-    names = ["cell_fields", "push_deposit", "current_gather_fold", "field_solve",
-             "particle_bc", "sort"]
-    phase_ms = {n: round(float(v) / steps, 4) for n, v in zip(names, phase)}
-    pd_ms = float(phase[1]) / steps
+    head = run_leg(wm, weibel, args, args.form, solver, args.steps, args.warmup)
+    nx, ny, ppc, _ = WORKLOADS[args.workload]
+    npart = head["particles"]
     bpp = PUSHDEP_BYTES_PER_PARTICLE if args.form == "fp64" else PUSHDEP_BYTES_PER_PARTICLE_FP32
-    achieved = bpp * npart / (pd_ms * 1e-3) / 1e9
+    achieved = bpp * npart / (head["push_deposit_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": load_traffic(args.workload, args.form),
                 "kernel": ("k_particle<PUSH,DEPOSIT>" if args.form == "fp64" else "k_push_deposit32")
                 + " (fused push+deposit)",
-                "bytes_per_particle": bpp}
+                "bytes_per_particle": bpp, "launch_ms": round(head["push_deposit_ms"], 4)}
+    p = head["p"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32 particles, f64 fields",
+        "dtype": "f64" if args.form == "fp64" else "f32 particles, f64 current/solve/diagnostics",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload} Weibel pair plasma, {args.form.upper()} form", "nx": nx, "ny": ny,
-                   "ppc_per_species": ppc, "species": 2, "particles": npart, "dt": p["delt"],
-                   "form": args.form, "parallelism": "single GPU"},
-        "solve_ms_per_step": phase_ms["field_solve"],
-        "cg_iters_per_step": iters / steps,
-        "phase_ms_per_step": phase_ms,
-        "energy_last": [float(e) for e in energy],
+        "config": {"workload": f"{args.workload}: Weibel pair plasma {nx}x{ny}, {ppc} ppc/species, "
+                               f"{args.form.upper()} form, {solver.upper()} solve",
+                   "nx": nx, "ny": ny, "ppc_per_species": ppc, "species": 2, "particles": npart,
+                   "dt": p["delt"], "form": args.form, "solver": solver,
+                   "parallelism": "single GPU"},
+        "solve_ms_per_step": head["solve_ms_per_step"],
+        "cg_iters_per_step": head["cg_iters_per_step"],
+        "fcg_iters_per_step": head["fcg_iters_per_step"],
+        "plain_iters_per_step": head["plain_iters_per_step"],
+        "pc_training": head["pc_training"],
+        "phase_ms_per_step": head["phase_ms_per_step"],
+        "energy_last": head["energy"],
         "roofline": roofline,
     }
-    ctx.close()
+    if not args.no_compare and not (args.form == "fp64" and solver == "cg"):
+        ref = run_leg(wm, weibel, args, "fp64", "cg", args.steps, args.warmup)
+        e1, e0 = head["energy"], ref["energy"]
+        tot0 = sum(e0)
+        out["comparison_fp64_cg"] = {
+            "ms_per_step": ref["ms_per_step"], "value": ref["value"],
+            "solve_ms_per_step": ref["solve_ms_per_step"],
+            "cg_iters_per_step": ref["cg_iters_per_step"],
+            "phase_ms_per_step": ref["phase_ms_per_step"],
+            "step_rate_ratio": round(ref["ms_per_step"] / head["ms_per_step"], 3),
+            "solve_iters_ratio": round(ref["cg_iters_per_step"] / max(head["cg_iters_per_step"], 1e-9), 3),
+            "energy_rel_diff_total": abs(sum(e1) - tot0) / tot0,
+            "energy_rel_diff": [abs(a - b) / max(abs(b), 1e-300) for a, b in zip(e1, e0)],
+        }
     if not args.no_cpu_baseline:
         log("timing the CPU restatement")
         out["cpu_baseline"] = cpu_baseline()

@@ -52,6 +52,10 @@ extern "C" {
 #define WM_FORM_FP64 0      /* absolute x, y and u in FP64, the reference's arithmetic */
 #define WM_FORM_FP32 1      /* int32 cell index + FP32 in-cell offsets + FP32 u */
 
+/* Implicit-solve methods (wm_set_solver). */
+#define WM_SOLVER_CG 0      /* the reference's plain CG (field.f90:319-461) */
+#define WM_SOLVER_FCG 1     /* flexible CG with the learned preconditioner */
+
 /* Boundary families for wm_field_fdtd_i's set_boundary_* callbacks. */
 #define WM_BC_PERIODIC 0    /* boundary_periodic__dfield/__curre/__phi */
 
@@ -154,10 +158,45 @@ int wm_cg_iters(wm_ctx *ctx, int *iters3);
 /* GPU time (ms) of each phase of the last wm_step call, summed over its
  * steps, from HIP events on the context's stream: [0] cell-centred fields,
  * [1] push+deposit, [2] current gather+fold, [3] field solve, [4] particle
- * boundaries, [5] sort.  Returns the number written. */
+ * boundaries, [5] sort.  In the FP32 form the row migration and the sort
+ * are one pass, reported in [5] ([4] is empty).  Returns the number written. */
 int wm_phase_times(wm_ctx *ctx, double *ms, int n);
 /* Total active particles (sum of np2). */
 int wm_particle_count(wm_ctx *ctx, long long *count);
+
+/* ---- learned preconditioner of the implicit solve (no reference
+ *      counterpart: the reference runs unpreconditioned CG, field.f90:319-461).
+ *      A four-level linear convolutional net M(r) ~ (f4 I - N4)^-1 r with BF16
+ *      MFMA convolutions, trained on the device on (residual, correction)
+ *      pairs that the plain solver records.  Needs nx, ny multiples of 8 and
+ *      field__init (the operator depends on delx, delt, c, gfac). ---- */
+/* WM_SOLVER_FCG: flexible CG with M, stopping at |r| <= err |b|, followed
+ * by the plain solver's own residual test (and plain iterations from the
+ * result if that test fails or a component went bad).  Needs parameters. */
+int wm_set_solver(wm_ctx *ctx, int solver);
+/* Record pairs: from now on each plain solve stores (r_k, x_final - x_k)
+ * of its first per_solve iterations (all three components, one slot each)
+ * in a ring of nslot slots (nslot <= 64). */
+int wm_pc_collect(wm_ctx *ctx, int nslot, int per_solve);
+/* Number of recorded pairs (slots filled). */
+int wm_pc_pair_count(wm_ctx *ctx);
+/* Default initial parameters from seed (also done by the first wm_pc_train). */
+int wm_pc_init(wm_ctx *ctx, uint64_t seed);
+/* Adam on the recorded pairs for `steps` steps, learning rate decaying
+ * from lr by a cosine; loss2 (may be NULL) gets the first and the mean of
+ * the last ten losses, L = mean((M(r)/s - e/s)^2), s = rms(r). */
+int wm_pc_train(wm_ctx *ctx, int steps, double lr, uint64_t seed, double *loss2);
+/* Number of FP32 parameters. */
+int wm_pc_num_params(void);
+/* Copy the FP32 parameters out (set = 0) or in (set = 1, resets Adam). */
+int wm_pc_params(wm_ctx *ctx, float *theta, int n, int set);
+/* z = M(r) for three host images r[3][ny][nx] (x fastest). */
+int wm_pc_apply(wm_ctx *ctx, const double *r, double *z);
+/* Loss and dL/dtheta for host images r, e [3][ny][nx]. */
+int wm_pc_grad(wm_ctx *ctx, const double *r, const double *e, float *grad,
+               double *loss);
+/* Iterations of the last solve per component: flexible CG, then plain. */
+int wm_solve_stats(wm_ctx *ctx, int *fcg3, int *plain3);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,8 @@ WM_ERR_NODEV = 8
 BC_PERIODIC = 0
 FORM_FP64 = 0
 FORM_FP32 = 1
+SOLVER_CG = 0
+SOLVER_FCG = 1
 
 # Symbols declared in include/wmpic.h (checked by the CPU test suite).
 EXPORTS = (
@@ -64,6 +66,8 @@ EXPORTS = (
     "wm_sort_bucket", "wm_bcp_dfield", "wm_bcp_curre", "wm_bcp_phi", "wm_state_upload",
     "wm_state_download", "wm_state_init_weibel", "wm_step", "wm_energy", "wm_cg_iters",
     "wm_phase_times", "wm_particle_count",
+    "wm_set_solver", "wm_pc_collect", "wm_pc_pair_count", "wm_pc_init", "wm_pc_train",
+    "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_grad", "wm_solve_stats",
 )
 
 
@@ -126,6 +130,16 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_cg_iters": [_P, _P],
         "wm_phase_times": [_P, _P, _I],
         "wm_particle_count": [_P, ctypes.POINTER(ctypes.c_longlong)],
+        "wm_set_solver": [_P, _I],
+        "wm_pc_collect": [_P, _I, _I],
+        "wm_pc_pair_count": [_P],
+        "wm_pc_init": [_P, ctypes.c_uint64],
+        "wm_pc_train": [_P, _I, _D, ctypes.c_uint64, _P],
+        "wm_pc_num_params": [],
+        "wm_pc_params": [_P, _P, _I, _I],
+        "wm_pc_apply": [_P, _P, _P],
+        "wm_pc_grad": [_P, _P, _P, _P, _P],
+        "wm_solve_stats": [_P, _P, _P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -323,3 +337,60 @@ class Context:
         c = ctypes.c_longlong()
         self._check(self._lib.wm_particle_count(self._h, ctypes.byref(c)))
         return c.value
+
+    # ------------------------------------------ learned preconditioner
+    def set_solver(self, solver: int) -> None:
+        """SOLVER_CG (the reference's plain CG) or SOLVER_FCG (flexible CG
+        with the learned preconditioner; plain CG judges and falls back)."""
+        self._check(self._lib.wm_set_solver(self._h, int(solver)))
+
+    def pc_collect(self, nslot: int, per_solve: int) -> None:
+        """Record (residual, correction) pairs of the next plain solves."""
+        self._check(self._lib.wm_pc_collect(self._h, int(nslot), int(per_solve)))
+
+    def pc_pair_count(self) -> int:
+        return int(self._lib.wm_pc_pair_count(self._h))
+
+    def pc_init(self, seed: int = 0) -> None:
+        self._check(self._lib.wm_pc_init(self._h, int(seed)))
+
+    def pc_train(self, steps: int, lr: float = 3e-3, seed: int = 0) -> np.ndarray:
+        """Adam on the recorded pairs; returns [first loss, mean of last ten]."""
+        out = np.zeros(2)
+        self._check(self._lib.wm_pc_train(self._h, int(steps), float(lr), int(seed),
+                                          out.ctypes.data))
+        return out
+
+    def pc_params(self) -> np.ndarray:
+        n = self._lib.wm_pc_num_params()
+        th = np.zeros(n, dtype=np.float32)
+        self._check(self._lib.wm_pc_params(self._h, th.ctypes.data, n, 0))
+        return th
+
+    def pc_set_params(self, theta: np.ndarray) -> None:
+        th = np.ascontiguousarray(theta, dtype=np.float32)
+        self._check(self._lib.wm_pc_params(self._h, th.ctypes.data, th.size, 1))
+
+    def pc_apply(self, r: np.ndarray) -> np.ndarray:
+        """z = M(r) for three images r[3, ny, nx]."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        self._check(self._lib.wm_pc_apply(self._h, r.ctypes.data, z.ctypes.data))
+        return z
+
+    def pc_grad(self, r: np.ndarray, e: np.ndarray):
+        """(loss, dL/dtheta) for images r, e [3, ny, nx]."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        g = np.zeros(self._lib.wm_pc_num_params(), dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(self._lib.wm_pc_grad(self._h, r.ctypes.data, e.ctypes.data, g.ctypes.data,
+                                         ctypes.byref(loss)))
+        return loss.value, g
+
+    def solve_stats(self):
+        """(flexible-CG iterations, plain-CG iterations) of the last solve."""
+        a = np.zeros(3, dtype=np.int32)
+        b = np.zeros(3, dtype=np.int32)
+        self._check(self._lib.wm_solve_stats(self._h, a.ctypes.data, b.ctypes.data))
+        return a, b

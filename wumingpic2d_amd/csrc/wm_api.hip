@@ -413,6 +413,7 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->diag, (size_t)nseg * 8 + 2 * (size_t)g.ny + 64));
   WM_TRY(dmalloc(ctx, &ctx->derr, 1));
   WM_TRY(dmalloc(ctx, &ctx->cnt_scratch, nseg));
+  if (ctx->form == WM_FORM_FP32) WM_TRY(dmalloc(ctx, &ctx->bcnt, (size_t)nseg * 3 * g.nx));
   WM_TRY(dmalloc(ctx, &ctx->fscratch, 6 * fg.ps));
 
   WM_HIP(hipMemsetAsync(ctx->red_cnt, 0, 8 * sizeof(unsigned), ctx->stream));
@@ -483,9 +484,11 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
-                  ctx->cnt_scratch, ctx->fscratch};
+                  ctx->cnt_scratch, ctx->fscratch, ctx->bcnt, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  wm_pc_free(ctx);
   for (int k = 0; k < 16; ++k) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -795,6 +798,7 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     WM_HIP(hipEventRecord(e[0], ctx->stream));
     // particle__solv + ele_cur (proj/weibel/app.f90:102, field.f90:121)
     const bool f32 = ctx->form == WM_FORM_FP32;
+    if (f32) WM_TRY(wm_k_bin_clear(ctx));
     WM_TRY(f32 ? wm_k_cell_fields32(ctx) : wm_k_cell_fields(ctx));
     WM_HIP(hipEventRecord(e[1], ctx->stream));
     WM_TRY(f32 ? wm_k_push_deposit32(ctx, 0, 1) : wm_k_push_deposit(ctx, 0, 1));
@@ -804,9 +808,16 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     WM_HIP(hipEventRecord(e[3], ctx->stream));
     WM_TRY(wm_k_field_solve(ctx));                // field.f90:125-184
     WM_HIP(hipEventRecord(e[4], ctx->stream));
-    WM_TRY(wm_k_particle_y(ctx, 1, true));        // app.f90:104-105 (x fused into y pass 1)
-    WM_HIP(hipEventRecord(e[5], ctx->stream));
-    WM_TRY(wm_k_sort(ctx, 1, 0));                 // app.f90:106
+    if (f32) {
+      // app.f90:104-106 in one pass: x wrap done by the push, rows and
+      // cells from the push's destination counts
+      WM_HIP(hipEventRecord(e[5], ctx->stream));
+      WM_TRY(wm_k_bin32(ctx, 1, 0));
+    } else {
+      WM_TRY(wm_k_particle_y(ctx, 1, true));      // app.f90:104-105 (x fused into y pass 1)
+      WM_HIP(hipEventRecord(e[5], ctx->stream));
+      WM_TRY(wm_k_sort(ctx, 1, 0));               // app.f90:106
+    }
     WM_HIP(hipEventRecord(e[6], ctx->stream));
     WM_HIP(hipEventSynchronize(e[6]));
     for (int k = 0; k < WM_NPHASE; ++k) {
@@ -849,6 +860,78 @@ int wm_particle_count(wm_ctx *ctx, long long *count) {
   long long c = 0;
   for (int v : n2) c += v;
   *count = c;
+  return WM_OK;
+}
+
+// ------------------------------------------- learned preconditioner
+namespace {
+int require_grid(wm_ctx *ctx) {
+  WM_TRY(require(ctx, ctx->finit, "field__init"));
+  WM_TRY(require(ctx, ctx->pinit, "particle__init"));
+  return wm_alloc_device(ctx);
+}
+}  // namespace
+
+int wm_set_solver(wm_ctx *ctx, int solver) {
+  if (!ctx || (solver != WM_SOLVER_CG && solver != WM_SOLVER_FCG)) return WM_ERR_ARG;
+  if (solver == WM_SOLVER_FCG && !wm_pc_ready(ctx)) {
+    wm_set_error(ctx, WM_ERR_ARG, "flexible CG needs preconditioner parameters (train or set them)");
+    return WM_ERR_ARG;
+  }
+  ctx->solver = solver;
+  return WM_OK;
+}
+
+int wm_pc_collect(wm_ctx *ctx, int nslot, int per_solve) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require_grid(ctx));
+  return wm_pc_collect_cfg(ctx, nslot, per_solve);
+}
+
+int wm_pc_pair_count(wm_ctx *ctx) { return ctx ? wm_pc_pairs(ctx) : 0; }
+
+int wm_pc_init(wm_ctx *ctx, uint64_t seed) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require_grid(ctx));
+  return wm_pc_init_params(ctx, seed);
+}
+
+int wm_pc_train(wm_ctx *ctx, int steps, double lr, uint64_t seed, double *loss2) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require_grid(ctx));
+  return wm_pc_train_dev(ctx, steps, lr, seed, loss2);
+}
+
+int wm_pc_num_params(void) { return wm_pc_nparam(); }
+
+int wm_pc_params(wm_ctx *ctx, float *theta, int n, int set) {
+  if (!ctx || !theta) return WM_ERR_ARG;
+  WM_TRY(require_grid(ctx));
+  return set ? wm_pc_set_params_dev(ctx, theta, n) : wm_pc_get_params_dev(ctx, theta, n);
+}
+
+int wm_pc_apply(wm_ctx *ctx, const double *r, double *z) {
+  if (!ctx || !r || !z) return WM_ERR_ARG;
+  WM_TRY(require_grid(ctx));
+  return wm_pc_apply_host(ctx, r, z);
+}
+
+int wm_pc_grad(wm_ctx *ctx, const double *r, const double *e, float *grad, double *loss) {
+  if (!ctx || !r || !e || !grad || !loss) return WM_ERR_ARG;
+  WM_TRY(require_grid(ctx));
+  return wm_pc_grad_host(ctx, r, e, grad, loss);
+}
+
+int wm_solve_stats(wm_ctx *ctx, int *fcg3, int *plain3) {
+  if (!ctx || !fcg3 || !plain3) return WM_ERR_ARG;
+  if (ctx->solver == WM_SOLVER_FCG) {
+    wm_pc_get_stats(ctx, fcg3, plain3);
+  } else {
+    for (int l = 0; l < 3; ++l) {
+      fcg3[l] = 0;
+      plain3[l] = ctx->last_iters[l];
+    }
+  }
   return WM_OK;
 }
 

@@ -402,6 +402,158 @@ __global__ __launch_bounds__(CGB) void k_cg_fin(CgArgs A, int stage, const doubl
   }
 }
 
+// ------------------------------------------------------- flexible CG
+// Flexible CG (Notay 2000; Polak-Ribiere beta) with the learned
+// preconditioner z = M(r).  Per component:
+//   z = M(r); rz = r.z; beta = -alpha_prev (z.q_prev) / rz_prev
+//   [= z.(r - r_prev) / rz_prev]; p = z + beta p; q = A p; alpha = rz / p.q;
+//   phi += alpha p; r -= alpha q; stop when |r| <= err |b|.
+// A non-finite or non-positive curvature marks the component bad: its phi is
+// restored from the warm start and the plain solve takes over.
+struct FcgScal {
+  double eps, sumr, rz, rz_prev, zq, pq, alpha, beta;
+  int active, it, bad, itmax;
+};
+
+__global__ __launch_bounds__(CGB) void k_fcg_save(CgArgs A, const double *__restrict__ df,
+                                                  double *__restrict__ x0) {
+  const int l = blockIdx.y;
+  const CgCell C = cg_cell(A);
+  if (!C.in) return;
+  x0[(size_t)l * A.n + C.t] = df[(size_t)l * A.ps + (size_t)(C.ii + 2) + (size_t)A.gx * (C.jj + 2)];
+}
+
+__global__ __launch_bounds__(CGB) void k_fcg_restore(CgArgs A, double *__restrict__ df,
+                                                     const double *__restrict__ x0,
+                                                     const FcgScal *S) {
+  const int l = blockIdx.y;
+  if (!S[l].bad) return;
+  const CgCell C = cg_cell(A);
+  if (!C.in) return;
+  df[(size_t)l * A.ps + (size_t)(C.ii + 2) + (size_t)A.gx * (C.jj + 2)] = x0[(size_t)l * A.n + C.t];
+}
+
+// partial r.z and z.q_prev
+__global__ __launch_bounds__(CGB) void k_fcg_dots(CgArgs A, const double *__restrict__ z,
+                                                  const double *__restrict__ r,
+                                                  const double *__restrict__ q, double *partials,
+                                                  const FcgScal *S) {
+  __shared__ double sh[2][CGB];
+  const int l = blockIdx.y;
+  if (!S[l].active) return;
+  const bool first = S[l].it == 0;
+  const CgCell C = cg_cell(A);
+  double v[2] = {0.0, 0.0};
+  if (C.in) {
+    const size_t o = (size_t)l * A.n + C.t;
+    v[0] = r[o] * z[o];
+    v[1] = first ? 0.0 : z[o] * q[o];
+  }
+  block_partials<2>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
+}
+
+// p = z + beta p_old at the five stencil points, q = A p, partial p.q
+__global__ __launch_bounds__(CGB) void k_fcg_dir(CgArgs A, const double *__restrict__ z,
+                                                 const double *__restrict__ pold,
+                                                 double *__restrict__ pnew, double *__restrict__ q,
+                                                 double *partials, const FcgScal *S) {
+  __shared__ double sh[1][CGB];
+  const int l = blockIdx.y;
+  if (!S[l].active) return;
+  const bool first = S[l].it == 0;
+  const double bv = S[l].beta;
+  const CgCell C = cg_cell(A);
+  const int ii = C.ii, jj = C.jj, t = C.t;
+  double v[1] = {0.0};
+  if (C.in) {
+    const double *Z = z + (size_t)l * A.n, *P = pold + (size_t)l * A.n;
+    auto PN = [&](int a, int b) {
+      const int k = wrapm(a, A.nx) + A.nx * wrapm(b, A.ny);
+      return first ? Z[k] : __fma_rn(bv, P[k], Z[k]);
+    };
+    const double pc = PN(ii, jj);
+    const double ap = apply_a(A.f4, pc, PN(ii, jj - 1), PN(ii - 1, jj), PN(ii + 1, jj), PN(ii, jj + 1));
+    pnew[(size_t)l * A.n + t] = pc;
+    q[(size_t)l * A.n + t] = ap;
+    v[0] = pc * ap;
+  }
+  block_partials<1>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
+}
+
+// phi += alpha p, r -= alpha q, partial |r|^2
+__global__ __launch_bounds__(CGB) void k_fcg_update(CgArgs A, double *__restrict__ df,
+                                                    double *__restrict__ r,
+                                                    const double *__restrict__ p,
+                                                    const double *__restrict__ q,
+                                                    double *partials, const FcgScal *S) {
+  __shared__ double sh[1][CGB];
+  const int l = blockIdx.y;
+  if (!S[l].active) return;
+  const double al = S[l].alpha;
+  const CgCell C = cg_cell(A);
+  double v[1] = {0.0};
+  if (C.in) {
+    const size_t o = (size_t)l * A.n + C.t;
+    double *phi = df + (size_t)l * A.ps + (size_t)(C.ii + 2) + (size_t)A.gx * (size_t)(C.jj + 2);
+    *phi = *phi + al * p[o];
+    const double rn = r[o] - al * q[o];
+    r[o] = rn;
+    v[0] = rn * rn;
+  }
+  block_partials<1>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
+}
+
+// Scalars of flexible CG; sc gets the residual scales s = |r|/sqrt(n), 1/s
+// that normalise the preconditioner's input.
+// stage 0 after k_cg_init, 1 after k_fcg_dots, 2 after k_fcg_dir,
+// 3 after k_fcg_update.
+__global__ __launch_bounds__(CGB) void k_fcg_fin(CgArgs A, int stage, const double *partials,
+                                                 FcgScal *S, double *sc, int itmax) {
+  __shared__ double sh[2][CGB];
+  const int l = blockIdx.x;
+  if (stage != 0 && !S[l].active) return;
+  const double *pl = partials + (size_t)l * 2 * A.nblk;
+  double tot[2] = {0.0, 0.0};
+  if (stage <= 1) {
+    partial_total<2>(pl, A.nblk, sh, tot);
+  } else {
+    double t1[1];
+    partial_total<1>(pl, A.nblk, sh, t1);
+    tot[0] = t1[0];
+  }
+  if (threadIdx.x != 0) return;
+  FcgScal s = stage == 0 ? FcgScal{} : S[l];
+  if (stage == 0) {
+    s.eps = sqrt(tot[0]) * A.err;
+    s.sumr = tot[1];
+    s.it = 0;
+    s.bad = 0;
+    s.itmax = itmax;
+    s.active = sqrt(s.sumr) > s.eps;
+  } else if (stage == 1) {
+    s.rz = tot[0];
+    s.zq = tot[1];
+    s.beta = s.it == 0 ? 0.0 : -s.alpha * s.zq / s.rz_prev;
+    if (!isfinite(s.rz) || !isfinite(s.beta)) s.bad = 1;
+  } else if (stage == 2) {
+    s.pq = tot[0];
+    s.alpha = s.rz / s.pq;
+    if (!(s.pq > 0.0) || !isfinite(s.alpha)) s.bad = 1;
+  } else {
+    s.sumr = tot[0];
+    s.rz_prev = s.rz;
+    s.it += 1;
+    if (!isfinite(s.sumr)) s.bad = 1;
+    s.active = sqrt(s.sumr) > s.eps && s.it < s.itmax;
+  }
+  if (s.bad) s.active = 0;
+  double sd = sqrt(s.sumr / (double)A.n);
+  if (!(sd > 0.0) || !isfinite(sd)) sd = 1.0;
+  sc[l] = sd;
+  sc[3 + l] = 1.0 / sd;
+  S[l] = s;
+}
+
 // Energy sums (proj/weibel/app.f90:499-535): per-workgroup partials.
 __global__ __launch_bounds__(256) void k_energy_kin(int ny, int cap, int nsp_used,
                                                     const int *np2, const double *ux,
@@ -473,6 +625,59 @@ size_t wm_cg_partials(int nx, int ny) {
   return 3 * 2 * (size_t)wm_cdiv((long long)n, CGB);
 }
 
+// Flexible CG with the learned preconditioner from the warm start in df;
+// components that go bad get their warm start back.  its = iterations done.
+static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
+  if (!wm_pc_ready(ctx)) {
+    wm_set_error(ctx, WM_ERR_ARG, "flexible CG: the learned preconditioner has no parameters");
+    return WM_ERR_ARG;
+  }
+  const size_t n = A.n;
+  if (!ctx->cg_z) {
+    WM_HIP(hipMalloc((void **)&ctx->cg_z, sizeof(double) * 3 * n));
+    WM_HIP(hipMalloc((void **)&ctx->cg_q, sizeof(double) * 3 * n));
+    WM_HIP(hipMalloc((void **)&ctx->cg_x0, sizeof(double) * 3 * n));
+    WM_HIP(hipMalloc((void **)&ctx->pc_sc, sizeof(double) * 6));
+    WM_HIP(hipMalloc(&ctx->fcg_s, sizeof(FcgScal) * 3));
+  }
+  FcgScal *S = (FcgScal *)ctx->fcg_s;
+  const dim3 grid(A.nblk, 3);
+  hipStream_t st = ctx->stream;
+  double *pbuf[2] = {ctx->cg_p[0], ctx->cg_p[1]};
+  hipLaunchKernelGGL(k_fcg_save, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_x0);
+  hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, st, A, ctx->gkl, ctx->df, ctx->cg_r, pbuf[0],
+                     ctx->red);
+  hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 0, ctx->red, S, ctx->pc_sc,
+                     ctx->fcg_max);
+  WM_HIP(hipGetLastError());
+  FcgScal hs[3];
+  int cur = 0;
+  for (;;) {
+    WM_HIP(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+    WM_HIP(hipStreamSynchronize(st));
+    if (!hs[0].active && !hs[1].active && !hs[2].active) break;
+    WM_TRY(wm_pc_apply_dev(ctx, ctx->cg_r, ctx->pc_sc, ctx->cg_z));
+    hipLaunchKernelGGL(k_fcg_dots, grid, dim3(CGB), 0, st, A, ctx->cg_z, ctx->cg_r, ctx->cg_q,
+                       ctx->red, S);
+    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 1, ctx->red, S, ctx->pc_sc,
+                       ctx->fcg_max);
+    hipLaunchKernelGGL(k_fcg_dir, grid, dim3(CGB), 0, st, A, ctx->cg_z, pbuf[cur], pbuf[1 - cur],
+                       ctx->cg_q, ctx->red, S);
+    cur = 1 - cur;
+    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 2, ctx->red, S, ctx->pc_sc,
+                       ctx->fcg_max);
+    hipLaunchKernelGGL(k_fcg_update, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_r, pbuf[cur],
+                       ctx->cg_q, ctx->red, S);
+    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 3, ctx->red, S, ctx->pc_sc,
+                       ctx->fcg_max);
+    WM_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_fcg_restore, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_x0, S);
+  WM_HIP(hipGetLastError());
+  for (int l = 0; l < 3; ++l) its[l] = hs[l].it;
+  return WM_OK;
+}
+
 // RHS -> CG -> bc_dfield -> dE -> bc_dfield -> uf += df (field.f90:125-184).
 // The current uj must already be folded.
 int wm_k_field_solve(wm_ctx *ctx) {
@@ -490,29 +695,37 @@ int wm_k_field_solve(wm_ctx *ctx) {
   A.nblk = A.nbx * g.ny;
   A.f4 = f.f4; A.f5 = f.f5; A.err = ctx->cg_err; A.ite_max = ctx->ite_max;
   const dim3 grid(A.nblk, 3);
+  int fcg_it[3] = {0, 0, 0};
+  if (ctx->solver == WM_SOLVER_FCG) WM_TRY(fcg_solve(ctx, A, fcg_it));
+  // the reference's plain CG; after flexible CG it recomputes the true
+  // residual and only iterates if that misses the tolerance
   double *pbuf[2] = {ctx->cg_p[0], ctx->cg_p[1]};
   hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, ctx->stream, A, ctx->gkl, ctx->df,
                      ctx->cg_r, pbuf[0], ctx->red);
   hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 0, ctx->red, ctx->cg_s);
   WM_HIP(hipGetLastError());
+  const bool collect = ctx->solver == WM_SOLVER_CG && wm_pc_collecting(ctx);
   CgScal hs[3];
   int iters = 0, cur = 0;
   for (;;) {
     WM_HIP(hipMemcpyAsync(hs, ctx->cg_s, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
     WM_HIP(hipStreamSynchronize(ctx->stream));
-    bool any = false;
+    bool any = false, all = true;
     for (int l = 0; l < 3; ++l) {
       if (hs[l].err) {
         wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
         return WM_ERR_CG_ITEMAX;
       }
       any = any || hs[l].active;
+      all = all && hs[l].active;
     }
     if (!any) break;
-    if (++iters > ctx->ite_max + 1) {
+    if (iters >= ctx->ite_max + 1) {
       wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
       return WM_ERR_CG_ITEMAX;
     }
+    if (collect && all) WM_TRY(wm_pc_snapshot(ctx, iters));
+    ++iters;
     hipLaunchKernelGGL(k_cg_pap, grid, dim3(CGB), 0, ctx->stream, A, ctx->cg_r, pbuf[cur],
                        pbuf[1 - cur], ctx->red, ctx->cg_s);
     cur = 1 - cur;
@@ -522,8 +735,13 @@ int wm_k_field_solve(wm_ctx *ctx) {
     hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 2, ctx->red, ctx->cg_s);
     WM_HIP(hipGetLastError());
   }
-  for (int l = 0; l < 3; ++l) ctx->last_iters[l] = hs[l].ite;
-  ctx->cg_iters_total += hs[0].ite + hs[1].ite + hs[2].ite;
+  if (collect) WM_TRY(wm_pc_snapshot_fin(ctx));
+  for (int l = 0; l < 3; ++l) ctx->last_iters[l] = fcg_it[l] + hs[l].ite;
+  if (ctx->solver == WM_SOLVER_FCG) {
+    int fb[3] = {hs[0].ite, hs[1].ite, hs[2].ite};
+    wm_pc_set_stats(ctx, fcg_it, fb);
+  }
+  ctx->cg_iters_total += ctx->last_iters[0] + ctx->last_iters[1] + ctx->last_iters[2];
 
   WM_TRY(wm_k_bc_dfield(ctx, ctx->df));
   hipLaunchKernelGGL(k_delta_e, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, ctx->uf,

@@ -77,6 +77,8 @@ struct CgScal {
 
 using PBank = Rec64;
 
+struct wm_pc;   // learned preconditioner (wm_pc.hip)
+
 struct wm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -140,8 +142,18 @@ struct wm_ctx {
   int last_iters[3] = {0, 0, 0};
   double phase_ms[8] = {0};
   int *cnt_scratch = nullptr;    // [nseg] counts for layout conversion
+  int *bcnt = nullptr;           // FP32 form: [nseg][3][nx] destination counts, then offsets
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
   long long cg_iters_total = 0;
+
+  // implicit solve: plain CG (the reference's) or flexible CG with the
+  // learned preconditioner, plain CG as fallback and judge
+  int solver = WM_SOLVER_CG;
+  int fcg_max = 50;              // flexible-CG iterations before the fallback
+  wm_pc *pc = nullptr;
+  double *cg_z = nullptr, *cg_q = nullptr, *cg_x0 = nullptr;   // 3 planes nx*ny each
+  double *pc_sc = nullptr;       // s[3], 1/s[3] of the current residuals
+  void *fcg_s = nullptr;         // flexible-CG scalars [3]
 };
 
 void wm_set_error(wm_ctx *ctx, int code, const char *fmt, ...);
@@ -166,12 +178,35 @@ int wm_k_particle_x(wm_ctx *ctx, int bank);
 int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x);
 // Sort (wm_sort.hip)
 int wm_k_sort(wm_ctx *ctx, int src, int dst);
+// FP32 form: row migration + stable cell sort in one counting scatter, from
+// the destination counts the push records in bcnt
+int wm_k_bin_clear(wm_ctx *ctx);
+int wm_k_bin32(wm_ctx *ctx, int src, int dst);
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);
 int wm_k_push_deposit32(wm_ctx *ctx, int src, int dst);
 int wm_k_deposit_gather32(wm_ctx *ctx);
 int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);
 int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk);
+// Learned preconditioner (wm_pc.hip)
+int wm_pc_nparam();
+int wm_pc_alloc(wm_ctx *ctx);
+void wm_pc_free(wm_ctx *ctx);
+int wm_pc_init_params(wm_ctx *ctx, uint64_t seed);
+int wm_pc_set_params_dev(wm_ctx *ctx, const float *theta_host, int n);
+int wm_pc_get_params_dev(wm_ctx *ctx, float *theta_host, int n);
+int wm_pc_apply_host(wm_ctx *ctx, const double *r, double *z);
+int wm_pc_grad_host(wm_ctx *ctx, const double *r, const double *e, float *grad, double *loss);
+int wm_pc_collect_cfg(wm_ctx *ctx, int nslot, int per_solve);
+bool wm_pc_collecting(const wm_ctx *ctx);
+int wm_pc_snapshot(wm_ctx *ctx, int k);
+int wm_pc_snapshot_fin(wm_ctx *ctx);
+int wm_pc_pairs(const wm_ctx *ctx);
+int wm_pc_train_dev(wm_ctx *ctx, int steps, double lr, uint64_t seed, double *loss_out);
+int wm_pc_apply_dev(wm_ctx *ctx, const double *r, const double *sc, double *z);
+bool wm_pc_ready(const wm_ctx *ctx);
+void wm_pc_set_stats(wm_ctx *ctx, const int *fcg, const int *fb);
+void wm_pc_get_stats(const wm_ctx *ctx, int *fcg, int *fb);
 // Synthetic initial state (wm_init.hip)
 int wm_k_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udrift);
 

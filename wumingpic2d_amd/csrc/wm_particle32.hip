@@ -241,10 +241,12 @@ constexpr int XCAP = 64;        // listed cell-crossing particles per cell and s
 
 __global__ __launch_bounds__(256) void k_push_deposit32(
     PP32 P, const int *__restrict__ cumcnt, const float *__restrict__ tmp, int TX, size_t tps,
-    Rec32 A, Rec32 B, double *__restrict__ part, int *__restrict__ derr) {
+    Rec32 A, Rec32 B, double *__restrict__ part, int *__restrict__ bcnt, int *__restrict__ derr) {
   __shared__ float T[6 * 3 * TW];
   __shared__ double cellbuf[CHUNK * NACC];
   __shared__ int xl[CHUNK * XCAP];
+  // destination counts of this workgroup's particles: [species][dy + 1][cell i0 - 1 + c]
+  __shared__ int lc[WM_MAXSP * 3 * (CHUNK + 2)];
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
@@ -254,6 +256,7 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
     T[t] = col < ncell + 2 ? tmp[(size_t)k * tps + (size_t)(i0 + col) + (size_t)TX * (jl + b)] : 0.0f;
   }
   for (int e = threadIdx.x; e < CHUNK * NACC; e += blockDim.x) cellbuf[e] = 0.0;
+  for (int e = threadIdx.x; e < P.nsp * 3 * (CHUNK + 2); e += blockDim.x) lc[e] = 0;
   __syncthreads();
   const int g = threadIdx.x / GROUP;
   const int sub = threadIdx.x % GROUP;
@@ -311,6 +314,7 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
           B.uz[p] = o.uz;
           ok = ok && fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f;
           cross = (dx | dy) != 0;
+          if (bcnt) atomicAdd(&lc[(s * 3 + dy + 1) * (CHUNK + 2) + g + dx + 1], 1);
           if (!cross) {
             const float qv =
                 q * (o.uz / sqrtf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) / P.c2));
@@ -371,6 +375,18 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
   }
   if (!ok) atomicOr(derr, DERR_JUMP);
   __syncthreads();
+  if (bcnt) {
+    // shared edge cells are also counted by the neighbouring chunks: integer
+    // atomics keep the totals exact
+    for (int e = threadIdx.x; e < P.nsp * 3 * (CHUNK + 2); e += blockDim.x) {
+      const int v = lc[e];
+      if (!v) continue;
+      const int c = e % (CHUNK + 2), k = (e / (CHUNK + 2)) % 3, s = e / (3 * (CHUNK + 2));
+      int cell = i0 + c - 1;
+      cell = cell < 0 ? cell + P.nx : (cell >= P.nx ? cell - P.nx : cell);
+      atomicAdd(&bcnt[(((size_t)s * P.ny + jl) * 3 + k) * P.nx + cell], v);
+    }
+  }
   double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;
   for (int e = threadIdx.x; e < TILE; e += blockDim.x) {
     const int comp = e % 3;
@@ -528,7 +544,7 @@ int wm_k_push_deposit32(wm_ctx *ctx, int src, int dst) {
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
   hipLaunchKernelGGL(k_push_deposit32, dim3(P.nchunk, P.ny), dim3(256), 0, ctx->stream, P,
                      ctx->cumcnt, ctx->tmp32, TX, tps, ctx->b32[src], ctx->b32[dst],
-                     ctx->dep_part, ctx->derr);
+                     ctx->dep_part, ctx->bcnt, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

@@ -1,0 +1,1037 @@
+// wm_pc.hip -- learned preconditioner for the implicit field solve.
+//
+// The reference solves (f4 I - N4) dB_l = f5 gkl_l with plain CG
+// (field.f90:319-461).  This file holds a four-level linear convolutional
+// net M(r) ~ (f4 I - N4)^-1 r, applied to the three components as a batch of
+// three images, and its training on (residual, correction) pairs that the
+// plain solver records.  The solver side (flexible CG, fallback) lives with
+// the CG kernels in wm_field.hip.
+//
+// Net (every convolution 3x3, periodic; level l has (nx >> l) x (ny >> l)
+// pixels, channels C0 = C1 = 16, C2 = 32):
+//   x0 = r / s                  s = rms(r) of the image
+//   a0 = lift(x0)               1 -> C0, FP32, never stored
+//   d1 = pool(a0)               2x2 mean                         BF16
+//   a1 = D1(d1), d2 = pool(a1), a2 = D2(d2), d3 = pool(a2), a3 = D3(d3)
+//   u2 = a2 + up(a3), b2 = U2(u2), u1 = a1 + up(b2), b1 = U1(u1)
+//   u0 = a0 + up(b1)            FP32
+//   z  = s * proj(u0)           C0 -> 1, FP32
+// D1..U1 run on MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+// accumulation) with BF16 activations in NHWC layout.  The fine level never
+// touches HBM except for r and z: lift+pool is one kernel, and up+add+proj
+// recomputes a0 from r in LDS.
+//
+// Training: L = mean((y - e/s)^2), gradients by hand-written backward
+// kernels (adjoint convolutions reuse the MFMA kernel with flipped weights,
+// weight gradients are per-workgroup partial sums added in a fixed order),
+// Adam on FP32 master weights.  Everything is deterministic.
+#include <math.h>
+
+#include "wm_internal.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int C0 = 16, C1 = 16, C2 = 32;
+constexpr int T = 16;     // tile edge (pixels) of every kernel
+
+// Layers in the flat parameter vector, each [tap][cout][cin],
+// tap = (dy+1)*3 + (dx+1).
+enum { L_LIFT, L_D1, L_D2, L_D3, L_U2, L_U1, L_PROJ, NLAYER };
+constexpr int LCIN[NLAYER] = {1, C0, C1, C2, C2, C1, C0};
+constexpr int LCOUT[NLAYER] = {C0, C1, C2, C2, C1, C0, 1};
+__host__ __device__ constexpr int loff(int l) {
+  return l == 0 ? 0 : loff(l - 1) + 9 * LCIN[l - 1] * LCOUT[l - 1];
+}
+constexpr int NPARAM = loff(NLAYER);
+constexpr int MAXW = 9 * 32 * 32;   // largest layer
+
+__device__ inline int pmod(int a, int n) {
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+
+// ------------------------------------------------------------- MFMA conv
+// out[b, y, x, co] = sum_tap sum_ci in[b, y+dy, x+dx, ci] * w[tap, co, ci].
+// One workgroup per 16x16 output tile; wave w computes tile rows 4w..4w+3.
+// MFMA operands: A = weights (M = 16 output channels), B = input pixels
+// (N = 16 pixels of a row), K = 32 (tap, input channel) pairs.  Lane
+// (n = lane & 15, g = lane >> 4) feeds A row n and B column n with the 8 K
+// values of slot g; the accumulator holds D[4g + j][n], j = 0..3.
+template <int CIN, int COUT, bool OUTF32>
+__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
+                                              const bf16 *__restrict__ w, void *outp, int H,
+                                              int W) {
+  static_assert(CIN == 16 || CIN == 32, "CIN");
+  static_assert(COUT % 16 == 0, "COUT");
+  __shared__ __attribute__((aligned(16))) bf16 sin[(T + 2) * (T + 2) * CIN];
+  __shared__ __attribute__((aligned(16))) bf16 sw[9 * COUT * CIN];
+  const int b = blockIdx.z;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  const bf16 *inb = in + (size_t)b * H * W * CIN;
+  constexpr int CH = CIN / 8;
+  for (int e = threadIdx.x; e < (T + 2) * (T + 2) * CH; e += 256) {
+    const int c = e % CH, px = e / CH;
+    const int gy = pmod(y0 + px / (T + 2) - 1, H), gx = pmod(x0 + px % (T + 2) - 1, W);
+    *(uint4 *)&sin[px * CIN + c * 8] = *(const uint4 *)&inb[((size_t)gy * W + gx) * CIN + c * 8];
+  }
+  for (int e = threadIdx.x; e < 9 * COUT * CH; e += 256)
+    *(uint4 *)&sw[e * 8] = *(const uint4 *)&w[e * 8];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = lane & 15, g = lane >> 4;
+  constexpr int NCH = CIN == 16 ? 5 : 9;     // K chunks of 32
+  bf16x8 zero;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) zero[k] = (bf16)0.0f;
+#pragma unroll
+  for (int mb = 0; mb < COUT / 16; ++mb) {
+    f32x4 acc[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) acc[rr] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int tap = CIN == 16 ? 2 * c + (g >> 1) : c;
+      const int cin0 = CIN == 16 ? 8 * (g & 1) : 8 * g;
+      const bool live = tap < 9;
+      const int tp = live ? tap : 8;
+      const int dy = tp / 3 - 1, dx = tp % 3 - 1;
+      const bf16x8 A = live ? *(const bf16x8 *)&sw[(tp * COUT + mb * 16 + n) * CIN + cin0] : zero;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int ty = wave * 4 + rr;
+        const bf16x8 Bv =
+            live ? *(const bf16x8 *)&sin[((ty + dy + 1) * (T + 2) + (n + dx + 1)) * CIN + cin0] : zero;
+        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, Bv, acc[rr], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int gy = y0 + wave * 4 + rr, gx = x0 + n;
+      if (gy < H && gx < W) {
+        const size_t o = (((size_t)b * H + gy) * W + gx) * COUT + mb * 16 + 4 * g;
+        if (OUTF32) {
+          float *out = (float *)outp;
+          *(float4 *)&out[o] = make_float4(acc[rr][0], acc[rr][1], acc[rr][2], acc[rr][3]);
+        } else {
+          bf16 *out = (bf16 *)outp;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) out[o + j] = (bf16)acc[rr][j];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------- fine-level kernels
+// x0 tile with a halo of HALO pixels, (float)(r * inv_s), periodic.
+template <int HALO>
+__device__ inline void load_x0(float *xs, const double *__restrict__ r, double is, int x0,
+                               int y0, int H, int W) {
+  constexpr int E = T + 2 * HALO;
+  for (int e = threadIdx.x; e < E * E; e += 256) {
+    const int gy = pmod(y0 + e / E - HALO, H), gx = pmod(x0 + e % E - HALO, W);
+    xs[e] = (float)(r[(size_t)gy * W + gx] * is);
+  }
+}
+
+// a0 (C0 channels) at tile-local pixel (py, px) of an x0 tile of edge E whose
+// origin sits `o` pixels before the pixel's own frame.
+template <int E>
+__device__ inline void lift_at(float *a, const float *xs, const float *__restrict__ wl, int py,
+                               int px) {
+#pragma unroll
+  for (int co = 0; co < C0; ++co) a[co] = 0.0f;
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const float v = xs[(py + tap / 3 - 1) * E + (px + tap % 3 - 1)];
+#pragma unroll
+    for (int co = 0; co < C0; ++co) a[co] = fmaf(v, wl[tap * C0 + co], a[co]);
+  }
+}
+
+// d1 = pool(lift(r / s)), one 16x16 fine tile -> 8x8 level-1 pixels.
+__global__ __launch_bounds__(256) void k_lift_pool(const double *__restrict__ r, size_t rstride,
+                                                   const double *__restrict__ sc,
+                                                   const float *__restrict__ wl,
+                                                   bf16 *__restrict__ d1, int H, int W, int B) {
+  __shared__ float xs[(T + 2) * (T + 2)];
+  __shared__ float as[T * T * C0];
+  const int b = blockIdx.z;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  load_x0<1>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
+  __syncthreads();
+  {
+    const int ty = threadIdx.x / T, tx = threadIdx.x % T;
+    float a[C0];
+    lift_at<T + 2>(a, xs, wl, ty + 1, tx + 1);
+#pragma unroll
+    for (int co = 0; co < C0; ++co) as[threadIdx.x * C0 + co] = a[co];
+  }
+  __syncthreads();
+  const int H1 = H / 2, W1 = W / 2;
+  for (int e = threadIdx.x; e < (T / 2) * (T / 2) * C0; e += 256) {
+    const int co = e % C0, q = e / C0;
+    const int qy = q / (T / 2), qx = q % (T / 2);
+    const int Y = y0 / 2 + qy, X = x0 / 2 + qx;
+    if (Y >= H1 || X >= W1) continue;
+    const int p = (2 * qy) * T + 2 * qx;
+    const float v = ((as[p * C0 + co] + as[(p + 1) * C0 + co]) +
+                     (as[(p + T) * C0 + co] + as[(p + T + 1) * C0 + co])) * 0.25f;
+    d1[(((size_t)b * H1 + Y) * W1 + X) * C0 + co] = (bf16)v;
+  }
+}
+
+// u0 = lift(x0) + up(b1) on the tile plus a 1-pixel halo, in LDS.
+__device__ inline void build_u0(float *us, const float *xs, const float *__restrict__ wl,
+                                const bf16 *__restrict__ b1b, int x0, int y0, int H, int W) {
+  const int W1 = W / 2;
+  for (int e = threadIdx.x; e < (T + 2) * (T + 2); e += 256) {
+    const int uy = e / (T + 2), ux = e % (T + 2);
+    float a[C0];
+    lift_at<T + 4>(a, xs, wl, uy + 1, ux + 1);
+    const int gy = pmod(y0 + uy - 1, H), gx = pmod(x0 + ux - 1, W);
+    const bf16 *bp = b1b + ((size_t)(gy / 2) * W1 + gx / 2) * C0;
+#pragma unroll
+    for (int co = 0; co < C0; ++co) us[e * C0 + co] = a[co] + (float)bp[co];
+  }
+}
+
+// z = s * proj(lift(r / s) + up(b1)); y = proj(...) as well when y != null.
+__global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, size_t rstride,
+                                                 const double *__restrict__ sc,
+                                                 const float *__restrict__ wl,
+                                                 const float *__restrict__ wp,
+                                                 const bf16 *__restrict__ b1,
+                                                 double *__restrict__ z, size_t zstride,
+                                                 float *__restrict__ y, int H, int W, int B) {
+  __shared__ float xs[(T + 4) * (T + 4)];
+  __shared__ float us[(T + 2) * (T + 2) * C0];
+  const int b = blockIdx.z;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  load_x0<2>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
+  __syncthreads();
+  build_u0(us, xs, wl, b1 + (size_t)b * (H / 2) * (W / 2) * C0, x0, y0, H, W);
+  __syncthreads();
+  const int ty = threadIdx.x / T, tx = threadIdx.x % T;
+  const int gy = y0 + ty, gx = x0 + tx;
+  if (gy >= H || gx >= W) return;
+  float v = 0.0f;
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap) {
+    const float *u = us + ((ty + tap / 3) * (T + 2) + (tx + tap % 3)) * C0;
+#pragma unroll
+    for (int ci = 0; ci < C0; ++ci) v = fmaf(u[ci], wp[tap * C0 + ci], v);
+  }
+  const size_t p = (size_t)gy * W + gx;
+  z[(size_t)b * zstride + p] = (double)v * sc[b];
+  if (y) y[(size_t)b * H * W + p] = v;
+}
+
+// ------------------------------------------------------------ coarse levels
+// 2x2 mean, ((x00 + x01) + (x10 + x11)) * 0.25, rounded to BF16.
+__global__ void k_pool(const bf16 *__restrict__ in, bf16 *__restrict__ out, int B, int H, int W,
+                       int C) {
+  const int Ho = H / 2, Wo = W / 2;
+  const size_t n = (size_t)B * Ho * Wo * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = e % C;
+    const size_t q = e / C;
+    const int X = q % Wo, Y = (q / Wo) % Ho, b = q / ((size_t)Wo * Ho);
+    const bf16 *p = in + (((size_t)b * H + 2 * Y) * W + 2 * X) * C + c;
+    const float v = (((float)p[0] + (float)p[C]) + ((float)p[(size_t)W * C] + (float)p[(size_t)W * C + C])) * 0.25f;
+    out[e] = (bf16)v;
+  }
+}
+
+// u = a + up(bc) (nearest), rounded to BF16.
+__global__ void k_upadd(const bf16 *__restrict__ a, const bf16 *__restrict__ bc,
+                        bf16 *__restrict__ u, int B, int H, int W, int C) {
+  const size_t n = (size_t)B * H * W * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = e % C;
+    const size_t q = e / C;
+    const int X = q % W, Y = (q / W) % H, b = q / ((size_t)W * H);
+    const float v = (float)a[e] + (float)bc[(((size_t)b * (H / 2) + Y / 2) * (W / 2) + X / 2) * C + c];
+    u[e] = (bf16)v;
+  }
+}
+
+// ---------------------------------------------------------------- backward
+// Adjoint of up: out[q] = BF16((g00 + g01) + (g10 + g11)).
+__global__ void k_up_t(const float *__restrict__ g, bf16 *__restrict__ out, int B, int H, int W,
+                       int C) {
+  const int Ho = H / 2, Wo = W / 2;
+  const size_t n = (size_t)B * Ho * Wo * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = e % C;
+    const size_t q = e / C;
+    const int X = q % Wo, Y = (q / Wo) % Ho, b = q / ((size_t)Wo * Ho);
+    const float *p = g + (((size_t)b * H + 2 * Y) * W + 2 * X) * C + c;
+    out[e] = (bf16)((p[0] + p[C]) + (p[(size_t)W * C] + p[(size_t)W * C + C]));
+  }
+}
+
+// out = BF16(du + 0.25 * up(dd)): gradient reaching a level's activation
+// through the up+add path (du) and through the pool feeding the next level.
+__global__ void k_add_unpool(const float *__restrict__ du, const float *__restrict__ dd,
+                             bf16 *__restrict__ out, int B, int H, int W, int C) {
+  const size_t n = (size_t)B * H * W * C;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = e % C;
+    const size_t q = e / C;
+    const int X = q % W, Y = (q / W) % H, b = q / ((size_t)W * H);
+    const float v = du[e] + 0.25f * dd[(((size_t)b * (H / 2) + Y / 2) * (W / 2) + X / 2) * C + c];
+    out[e] = (bf16)v;
+  }
+}
+
+// Fixed-order workgroup sum of one double; result in thread 0.
+__device__ inline double block_sum1(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Backward of y = proj(u0) for the loss mean((y - e/s)^2):
+//   dy = 2 (y - e/s) / N, dW_proj = sum u0(p+off) dy(p), du0 = proj^T dy,
+// plus the loss partial.  One partial row per tile.
+__global__ __launch_bounds__(256) void k_proj_bwd(
+    const double *__restrict__ r, size_t rstride, const double *__restrict__ e, size_t estride,
+    const double *__restrict__ sc, const float *__restrict__ y, const float *__restrict__ wl,
+    const float *__restrict__ wp, const bf16 *__restrict__ b1, float *__restrict__ du0,
+    float *__restrict__ part, double *__restrict__ lossp, double gs, int H, int W, int B) {
+  __shared__ float xs[(T + 4) * (T + 4)];
+  __shared__ float us[(T + 2) * (T + 2) * C0];
+  __shared__ float dys[(T + 2) * (T + 2)];
+  __shared__ double sh[256];
+  const int b = blockIdx.z;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  const double is = sc[B + b];
+  load_x0<2>(xs, r + (size_t)b * rstride, is, x0, y0, H, W);
+  for (int q = threadIdx.x; q < (T + 2) * (T + 2); q += 256) {
+    const int gy = pmod(y0 + q / (T + 2) - 1, H), gx = pmod(x0 + q % (T + 2) - 1, W);
+    const size_t p = (size_t)gy * W + gx;
+    const double d = (double)y[(size_t)b * H * W + p] - e[(size_t)b * estride + p] * is;
+    dys[q] = (float)(gs * d);
+  }
+  __syncthreads();
+  build_u0(us, xs, wl, b1 + (size_t)b * (H / 2) * (W / 2) * C0, x0, y0, H, W);
+  __syncthreads();
+  const int ty = threadIdx.x / T, tx = threadIdx.x % T;
+  const bool in = y0 + ty < H && x0 + tx < W;
+  double l2 = 0.0;
+  if (in) {
+    const size_t p = (size_t)(y0 + ty) * W + (x0 + tx);
+    const double d = (double)y[(size_t)b * H * W + p] - e[(size_t)b * estride + p] * is;
+    l2 = d * d;
+    float g[C0];
+#pragma unroll
+    for (int ci = 0; ci < C0; ++ci) g[ci] = 0.0f;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      // dy at p - off(tap)
+      const float v = dys[(ty + 1 - (tap / 3 - 1)) * (T + 2) + (tx + 1 - (tap % 3 - 1))];
+#pragma unroll
+      for (int ci = 0; ci < C0; ++ci) g[ci] = fmaf(v, wp[tap * C0 + ci], g[ci]);
+    }
+    float *o = du0 + (((size_t)b * H + y0 + ty) * W + x0 + tx) * C0;
+#pragma unroll
+    for (int ci = 0; ci < C0; ci += 4) *(float4 *)&o[ci] = make_float4(g[ci], g[ci + 1], g[ci + 2], g[ci + 3]);
+  }
+  const size_t blk = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const double tot = block_sum1(l2, sh);
+  if (threadIdx.x == 0) lossp[blk] = tot;
+  if (threadIdx.x < 9 * C0) {
+    const int tap = threadIdx.x / C0, ci = threadIdx.x % C0;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    float acc = 0.0f;
+    for (int q = 0; q < T * T; ++q) {
+      const int qy = q / T, qx = q % T;
+      if (y0 + qy >= H || x0 + qx >= W) continue;
+      acc = fmaf(us[((qy + 1 + dy) * (T + 2) + (qx + 1 + dx)) * C0 + ci],
+                 dys[(qy + 1) * (T + 2) + qx + 1], acc);
+    }
+    part[blk * (9 * C0) + threadIdx.x] = acc;
+  }
+}
+
+// dW_lift[tap][co] = sum_p x0(p+off) * da0(p, co), da0 = du0 + 0.25 up(dd1).
+__global__ __launch_bounds__(256) void k_lift_bwd(const double *__restrict__ r, size_t rstride,
+                                                  const double *__restrict__ sc,
+                                                  const float *__restrict__ du0,
+                                                  const float *__restrict__ dd1,
+                                                  float *__restrict__ part, int H, int W, int B) {
+  __shared__ float xs[(T + 2) * (T + 2)];
+  __shared__ float das[T * T * C0];
+  const int b = blockIdx.z;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  load_x0<1>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
+  const int W1 = W / 2;
+  for (int e = threadIdx.x; e < T * T * C0; e += 256) {
+    const int co = e % C0, q = e / C0;
+    const int gy = y0 + q / T, gx = x0 + q % T;
+    float v = 0.0f;
+    if (gy < H && gx < W)
+      v = du0[(((size_t)b * H + gy) * W + gx) * C0 + co] +
+          0.25f * dd1[(((size_t)b * (H / 2) + gy / 2) * W1 + gx / 2) * C0 + co];
+    das[e] = v;
+  }
+  __syncthreads();
+  const size_t blk = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  if (threadIdx.x < 9 * C0) {
+    const int tap = threadIdx.x / C0, co = threadIdx.x % C0;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    float acc = 0.0f;
+    for (int q = 0; q < T * T; ++q) {
+      const int qy = q / T, qx = q % T;
+      acc = fmaf(xs[(qy + 1 + dy) * (T + 2) + qx + 1 + dx], das[q * C0 + co], acc);
+    }
+    part[blk * (9 * C0) + threadIdx.x] = acc;
+  }
+}
+
+// Weight gradient of an MFMA layer: dW[tap][co][ci] = sum_p X(p+off, ci) G(p, co),
+// BF16 inputs, FP32 sums.  Workgroup k sums tiles k, k + grid, ... in order.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X,
+                                                 const bf16 *__restrict__ G,
+                                                 float *__restrict__ part, int H, int W,
+                                                 int ntx, int nty, int ntiles) {
+  __shared__ float xs[(T + 2) * (T + 2) * CIN];
+  __shared__ bf16 gsh[T * T * COUT];
+  constexpr int NO = 9 * CIN * COUT;
+  constexpr int PER = (NO + 255) / 256;
+  float acc[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) acc[k] = 0.0f;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int b = t / (ntx * nty), tt = t % (ntx * nty);
+    const int x0 = (tt % ntx) * T, y0 = (tt / ntx) * T;
+    __syncthreads();
+    for (int e = threadIdx.x; e < (T + 2) * (T + 2) * CIN; e += 256) {
+      const int c = e % CIN, px = e / CIN;
+      const int gy = pmod(y0 + px / (T + 2) - 1, H), gx = pmod(x0 + px % (T + 2) - 1, W);
+      xs[e] = (float)X[(((size_t)b * H + gy) * W + gx) * CIN + c];
+    }
+    for (int e = threadIdx.x; e < T * T * COUT; e += 256) {
+      const int c = e % COUT, q = e / COUT;
+      const int gy = y0 + q / T, gx = x0 + q % T;
+      gsh[e] = (gy < H && gx < W) ? G[(((size_t)b * H + gy) * W + gx) * COUT + c] : (bf16)0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int o = threadIdx.x + 256 * k;
+      if (o >= NO) break;
+      const int ci = o % CIN, co = (o / CIN) % COUT, tap = o / (CIN * COUT);
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      float a = acc[k];
+      for (int q = 0; q < T * T; ++q) {
+        const int qy = q / T, qx = q % T;
+        a = fmaf(xs[((qy + 1 + dy) * (T + 2) + qx + 1 + dx) * CIN + ci], (float)gsh[q * COUT + co], a);
+      }
+      acc[k] = a;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int o = threadIdx.x + 256 * k;
+    if (o < NO) part[(size_t)blockIdx.x * NO + o] = acc[k];
+  }
+}
+
+// grad[o] = sum over nblk partial rows, in row order.
+__global__ void k_reduce_rows(const float *__restrict__ part, int nblk, int n,
+                              float *__restrict__ grad) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  float s = 0.0f;
+  for (int k = 0; k < nblk; ++k) s += part[(size_t)k * n + o];
+  grad[o] = s;
+}
+
+__global__ void k_sum_loss(const double *__restrict__ lossp, int nblk, double inv_n,
+                           double *__restrict__ out) {
+  __shared__ double sh[256];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 256) v += lossp[k];
+  const double t = block_sum1(v, sh);
+  if (threadIdx.x == 0) *out = t * inv_n;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights; c1 = 1/(1 - b1^t),
+// c2 = 1/(1 - b2^t).
+__global__ void k_adam(float *__restrict__ th, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, int n, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  const float gg = g[o];
+  const float mm = b1 * m[o] + (1.0f - b1) * gg;
+  const float vv = b2 * v[o] + (1.0f - b2) * gg * gg;
+  m[o] = mm;
+  v[o] = vv;
+  th[o] = th[o] - lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+}
+
+// BF16 copies of the MFMA layers: wb in the [tap][cout][cin] layout, wt the
+// adjoint's weights wt[tap'][ci][co] = w[8 - tap'][co][ci].
+__global__ void k_weights_bf16(const float *__restrict__ th, bf16 *__restrict__ wb,
+                               bf16 *__restrict__ wt) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= NPARAM) return;
+  int l = 0;
+  while (l + 1 < NLAYER && o >= loff(l + 1)) ++l;
+  if (l == L_LIFT || l == L_PROJ) return;
+  const int ci_n = LCIN[l], co_n = LCOUT[l];
+  const int k = o - loff(l);
+  const int ci = k % ci_n, co = (k / ci_n) % co_n, tap = k / (ci_n * co_n);
+  const bf16 v = (bf16)th[o];
+  wb[o] = v;
+  wt[loff(l) + ((8 - tap) * ci_n + ci) * co_n + co] = v;
+}
+
+// Pair snapshot of the plain solver (iteration k): r_k and x_k of the three
+// components, and their scales s = |r_k| / sqrt(n) from the solver's own sum.
+__global__ void k_snap(const double *__restrict__ r, const double *__restrict__ df, int nx,
+                       int ny, int gx, size_t ps, const CgScal *__restrict__ S,
+                       double *__restrict__ sr, double *__restrict__ sx, double *__restrict__ ssc) {
+  const size_t n = (size_t)nx * ny;
+  const int l = blockIdx.y;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int ii = t % nx, jj = t / nx;
+    sr[l * n + t] = r[l * n + t];
+    sx[l * n + t] = df[l * ps + (size_t)(ii + 2) + (size_t)gx * (jj + 2)];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = sqrt(S[l].sumr / (double)n);
+    if (!(s > 0.0)) s = 1.0;
+    ssc[l] = s;
+    ssc[3 + l] = 1.0 / s;
+  }
+}
+
+// e = x_final - x_k, in place over the snapshot's x.
+__global__ void k_snap_fin(const double *__restrict__ df, int nx, int ny, int gx, size_t ps,
+                           double *__restrict__ sx) {
+  const size_t n = (size_t)nx * ny;
+  const int l = blockIdx.y;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int ii = t % nx, jj = t / nx;
+    sx[l * n + t] = df[l * ps + (size_t)(ii + 2) + (size_t)gx * (jj + 2)] - sx[l * n + t];
+  }
+}
+
+// Scales of three host-supplied images (rms over the image, fixed order).
+__global__ void k_img_scale(const double *__restrict__ r, size_t n, double *__restrict__ sc) {
+  __shared__ double sh[256];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  for (size_t t = threadIdx.x; t < n; t += 256) v += r[b * n + t] * r[b * n + t];
+  const double tot = block_sum1(v, sh);
+  if (threadIdx.x == 0) {
+    double s = sqrt(tot / (double)n);
+    if (!(s > 0.0)) s = 1.0;
+    sc[b] = s;
+    sc[3 + b] = 1.0 / s;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ state
+struct wm_pc {
+  int H = 0, W = 0;
+  static constexpr int B = 3;
+  float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  bf16 *wb = nullptr, *wt = nullptr;
+  int adam_t = 0;
+  bool trained = false;
+  // activations (BF16, NHWC)
+  bf16 *d1 = nullptr, *a1 = nullptr, *d2 = nullptr, *a2 = nullptr, *d3 = nullptr, *a3 = nullptr;
+  bf16 *u2 = nullptr, *b2 = nullptr, *u1 = nullptr, *b1 = nullptr;
+  // training
+  float *y = nullptr, *du0 = nullptr, *du1 = nullptr, *du2 = nullptr;
+  float *dd1 = nullptr, *dd2 = nullptr, *dd3 = nullptr;
+  bf16 *db1 = nullptr, *db2 = nullptr, *da3 = nullptr, *da2 = nullptr, *da1 = nullptr;
+  double *zt = nullptr;          // training forward output
+  float *part = nullptr;         // weight-gradient partial rows
+  size_t part_elems = 0;
+  double *lossp = nullptr, *losses = nullptr;
+  int nlossp = 0, losses_cap = 0;
+  double *isc = nullptr;         // scales of host-supplied images
+  double *himg = nullptr;        // host-supplied images [2][3][n]
+  // pairs recorded from the plain solver
+  int nslot = 0, per_solve = 0, filled = 0, next = 0;
+  double *pr = nullptr, *pe = nullptr, *psc = nullptr;
+  int snap_slot[64];
+  int nsnap = 0;
+  int last_fcg[3] = {0, 0, 0}, last_fallback[3] = {0, 0, 0};
+};
+
+namespace {
+
+template <class Tp>
+int dalloc(wm_ctx *ctx, Tp **p, size_t n) {
+  WM_HIP(hipMalloc((void **)p, sizeof(Tp) * (n ? n : 1)));
+  return WM_OK;
+}
+
+size_t lvl(const wm_pc *pc, int l, int c) {
+  return (size_t)wm_pc::B * (pc->H >> l) * (pc->W >> l) * c;
+}
+
+dim3 tiles(int H, int W, int B) { return dim3(wm_cdiv(W, T), wm_cdiv(H, T), B); }
+
+int ew_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
+
+template <int CIN, int COUT, bool F32>
+int conv(wm_ctx *ctx, const bf16 *in, const bf16 *w, void *out, int H, int W) {
+  hipLaunchKernelGGL((k_conv<CIN, COUT, F32>), tiles(H, W, wm_pc::B), dim3(256), 0, ctx->stream,
+                     in, w, out, H, W);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+// Weight gradient of layer l into grad (partials then fixed-order rows).
+template <int CIN, int COUT>
+int conv_dw(wm_ctx *ctx, wm_pc *pc, int l, const bf16 *X, const bf16 *G, int H, int W) {
+  const int ntx = wm_cdiv(W, T), nty = wm_cdiv(H, T), ntiles = ntx * nty * wm_pc::B;
+  const int nblk = std::min(ntiles, 512);
+  constexpr int NO = 9 * CIN * COUT;
+  hipLaunchKernelGGL((k_conv_dw<CIN, COUT>), dim3(nblk), dim3(256), 0, ctx->stream, X, G, pc->part,
+                     H, W, ntx, nty, ntiles);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(wm_cdiv(NO, 256)), dim3(256), 0, ctx->stream, pc->part,
+                     nblk, NO, pc->grad + loff(l));
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+// Forward pass over three images r (stride rstride) with scales sc (device,
+// s[3] then 1/s[3]); z gets s * y (stride zstride), y (optional) the net output.
+int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const double *sc, double *z,
+            size_t zstride, float *y) {
+  const int H = pc->H, W = pc->W, B = wm_pc::B;
+  const float *th = pc->theta;
+  const bf16 *wb = pc->wb;
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_lift_pool, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc,
+                     th + loff(L_LIFT), pc->d1, H, W, B);
+  WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1)));
+  hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 2, C1))), dim3(256), 0, st, pc->a1, pc->d2, B,
+                     H >> 1, W >> 1, C1);
+  WM_TRY((conv<C1, C2, false>(ctx, pc->d2, wb + loff(L_D2), pc->a2, H >> 2, W >> 2)));
+  hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 3, C2))), dim3(256), 0, st, pc->a2, pc->d3, B,
+                     H >> 2, W >> 2, C2);
+  WM_TRY((conv<C2, C2, false>(ctx, pc->d3, wb + loff(L_D3), pc->a3, H >> 3, W >> 3)));
+  hipLaunchKernelGGL(k_upadd, dim3(ew_blocks(lvl(pc, 2, C2))), dim3(256), 0, st, pc->a2, pc->a3,
+                     pc->u2, B, H >> 2, W >> 2, C2);
+  WM_TRY((conv<C2, C1, false>(ctx, pc->u2, wb + loff(L_U2), pc->b2, H >> 2, W >> 2)));
+  hipLaunchKernelGGL(k_upadd, dim3(ew_blocks(lvl(pc, 1, C1))), dim3(256), 0, st, pc->a1, pc->b2,
+                     pc->u1, B, H >> 1, W >> 1, C1);
+  WM_TRY((conv<C1, C0, false>(ctx, pc->u1, wb + loff(L_U1), pc->b1, H >> 1, W >> 1)));
+  hipLaunchKernelGGL(k_up_proj, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc,
+                     th + loff(L_LIFT), th + loff(L_PROJ), pc->b1, z, zstride, y, H, W, B);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+// Loss and gradient (into pc->grad) of the pair (r, e) with scales sc;
+// the loss lands in *loss_dev.
+int backward(wm_ctx *ctx, wm_pc *pc, const double *r, const double *e, size_t stride,
+             const double *sc, double *loss_dev) {
+  const int H = pc->H, W = pc->W, B = wm_pc::B;
+  const float *th = pc->theta;
+  const bf16 *wt = pc->wt;
+  hipStream_t st = ctx->stream;
+  WM_TRY(forward(ctx, pc, r, stride, sc, pc->zt, (size_t)H * W, pc->y));
+  const double ntot = (double)B * H * W;
+  const dim3 t0 = tiles(H, W, B);
+  const int nb0 = (int)(t0.x * t0.y * t0.z);
+  hipLaunchKernelGGL(k_proj_bwd, t0, dim3(256), 0, st, r, stride, e, stride, sc, pc->y,
+                     th + loff(L_LIFT), th + loff(L_PROJ), pc->b1, pc->du0, pc->part, pc->lossp,
+                     2.0 / ntot, H, W, B);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, st, pc->part, nb0, 9 * C0,
+                     pc->grad + loff(L_PROJ));
+  hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, st, pc->lossp, nb0, 1.0 / ntot, loss_dev);
+  WM_HIP(hipGetLastError());
+  // U1: b1 = U1(u1); u0 = a0 + up(b1)
+  hipLaunchKernelGGL(k_up_t, dim3(ew_blocks(lvl(pc, 1, C0))), dim3(256), 0, st, pc->du0, pc->db1,
+                     B, H, W, C0);
+  WM_TRY((conv_dw<C1, C0>(ctx, pc, L_U1, pc->u1, pc->db1, H >> 1, W >> 1)));
+  WM_TRY((conv<C0, C1, true>(ctx, pc->db1, wt + loff(L_U1), pc->du1, H >> 1, W >> 1)));
+  // U2: b2 = U2(u2); u1 = a1 + up(b2)
+  hipLaunchKernelGGL(k_up_t, dim3(ew_blocks(lvl(pc, 2, C1))), dim3(256), 0, st, pc->du1, pc->db2,
+                     B, H >> 1, W >> 1, C1);
+  WM_TRY((conv_dw<C2, C1>(ctx, pc, L_U2, pc->u2, pc->db2, H >> 2, W >> 2)));
+  WM_TRY((conv<C1, C2, true>(ctx, pc->db2, wt + loff(L_U2), pc->du2, H >> 2, W >> 2)));
+  // D3: a3 = D3(d3); u2 = a2 + up(a3)
+  hipLaunchKernelGGL(k_up_t, dim3(ew_blocks(lvl(pc, 3, C2))), dim3(256), 0, st, pc->du2, pc->da3,
+                     B, H >> 2, W >> 2, C2);
+  WM_TRY((conv_dw<C2, C2>(ctx, pc, L_D3, pc->d3, pc->da3, H >> 3, W >> 3)));
+  WM_TRY((conv<C2, C2, true>(ctx, pc->da3, wt + loff(L_D3), pc->dd3, H >> 3, W >> 3)));
+  // D2: a2 = D2(d2); d3 = pool(a2)
+  hipLaunchKernelGGL(k_add_unpool, dim3(ew_blocks(lvl(pc, 2, C2))), dim3(256), 0, st, pc->du2,
+                     pc->dd3, pc->da2, B, H >> 2, W >> 2, C2);
+  WM_TRY((conv_dw<C1, C2>(ctx, pc, L_D2, pc->d2, pc->da2, H >> 2, W >> 2)));
+  WM_TRY((conv<C2, C1, true>(ctx, pc->da2, wt + loff(L_D2), pc->dd2, H >> 2, W >> 2)));
+  // D1: a1 = D1(d1); d2 = pool(a1)
+  hipLaunchKernelGGL(k_add_unpool, dim3(ew_blocks(lvl(pc, 1, C1))), dim3(256), 0, st, pc->du1,
+                     pc->dd2, pc->da1, B, H >> 1, W >> 1, C1);
+  WM_TRY((conv_dw<C0, C1>(ctx, pc, L_D1, pc->d1, pc->da1, H >> 1, W >> 1)));
+  WM_TRY((conv<C1, C0, true>(ctx, pc->da1, wt + loff(L_D1), pc->dd1, H >> 1, W >> 1)));
+  // lift: d1 = pool(a0), u0 = a0 + ...
+  hipLaunchKernelGGL(k_lift_bwd, t0, dim3(256), 0, st, r, stride, sc, pc->du0, pc->dd1, pc->part,
+                     H, W, B);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, st, pc->part, nb0, 9 * C0,
+                     pc->grad + loff(L_LIFT));
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int refresh_weights(wm_ctx *ctx, wm_pc *pc) {
+  hipLaunchKernelGGL(k_weights_bf16, dim3(wm_cdiv(NPARAM, 256)), dim3(256), 0, ctx->stream,
+                     pc->theta, pc->wb, pc->wt);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+uint64_t splitmix(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- launchers
+int wm_pc_nparam() { return NPARAM; }
+
+void wm_pc_free(wm_ctx *ctx) {
+  wm_pc *pc = ctx->pc;
+  if (!pc) return;
+  void *ptrs[] = {pc->theta, pc->m,   pc->v,   pc->grad, pc->wb,  pc->wt,  pc->d1,  pc->a1,
+                  pc->d2,    pc->a2,  pc->d3,  pc->a3,   pc->u2,  pc->b2,  pc->u1,  pc->b1,
+                  pc->y,     pc->du0, pc->du1, pc->du2,  pc->dd1, pc->dd2, pc->dd3, pc->db1,
+                  pc->db2,   pc->da3, pc->da2, pc->da1,  pc->zt,  pc->part, pc->lossp,
+                  pc->losses, pc->isc, pc->himg, pc->pr, pc->pe, pc->psc};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete pc;
+  ctx->pc = nullptr;
+}
+
+// Allocates the net for the context's grid (nx, ny multiples of 8).
+int wm_pc_alloc(wm_ctx *ctx) {
+  if (ctx->pc) return WM_OK;
+  const int nx = ctx->g.nx, ny = ctx->g.ny;
+  if (nx % 8 || ny % 8) {
+    wm_set_error(ctx, WM_ERR_ARG,
+                 "learned preconditioner: nx=%d and ny=%d must be multiples of 8 (three 2x2 "
+                 "coarsenings)", nx, ny);
+    return WM_ERR_ARG;
+  }
+  wm_pc *pc = new wm_pc();
+  ctx->pc = pc;
+  pc->H = ny;
+  pc->W = nx;
+  const size_t n0 = (size_t)wm_pc::B * nx * ny;
+  WM_TRY(dalloc(ctx, &pc->theta, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->m, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->v, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->grad, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->wb, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->wt, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->d1, lvl(pc, 1, C0)));
+  WM_TRY(dalloc(ctx, &pc->a1, lvl(pc, 1, C1)));
+  WM_TRY(dalloc(ctx, &pc->d2, lvl(pc, 2, C1)));
+  WM_TRY(dalloc(ctx, &pc->a2, lvl(pc, 2, C2)));
+  WM_TRY(dalloc(ctx, &pc->d3, lvl(pc, 3, C2)));
+  WM_TRY(dalloc(ctx, &pc->a3, lvl(pc, 3, C2)));
+  WM_TRY(dalloc(ctx, &pc->u2, lvl(pc, 2, C2)));
+  WM_TRY(dalloc(ctx, &pc->b2, lvl(pc, 2, C1)));
+  WM_TRY(dalloc(ctx, &pc->u1, lvl(pc, 1, C1)));
+  WM_TRY(dalloc(ctx, &pc->b1, lvl(pc, 1, C0)));
+  WM_TRY(dalloc(ctx, &pc->isc, 6));
+  WM_HIP(hipMemsetAsync(pc->m, 0, sizeof(float) * NPARAM, ctx->stream));
+  WM_HIP(hipMemsetAsync(pc->v, 0, sizeof(float) * NPARAM, ctx->stream));
+  (void)n0;
+  return WM_OK;
+}
+
+static int alloc_training(wm_ctx *ctx, wm_pc *pc) {
+  if (pc->y) return WM_OK;
+  const size_t n0 = (size_t)wm_pc::B * pc->W * pc->H;
+  WM_TRY(dalloc(ctx, &pc->y, n0));
+  WM_TRY(dalloc(ctx, &pc->zt, n0));
+  WM_TRY(dalloc(ctx, &pc->du0, lvl(pc, 0, C0)));
+  WM_TRY(dalloc(ctx, &pc->du1, lvl(pc, 1, C1)));
+  WM_TRY(dalloc(ctx, &pc->du2, lvl(pc, 2, C2)));
+  WM_TRY(dalloc(ctx, &pc->dd1, lvl(pc, 1, C0)));
+  WM_TRY(dalloc(ctx, &pc->dd2, lvl(pc, 2, C1)));
+  WM_TRY(dalloc(ctx, &pc->dd3, lvl(pc, 3, C2)));
+  WM_TRY(dalloc(ctx, &pc->db1, lvl(pc, 1, C0)));
+  WM_TRY(dalloc(ctx, &pc->db2, lvl(pc, 2, C1)));
+  WM_TRY(dalloc(ctx, &pc->da3, lvl(pc, 3, C2)));
+  WM_TRY(dalloc(ctx, &pc->da2, lvl(pc, 2, C2)));
+  WM_TRY(dalloc(ctx, &pc->da1, lvl(pc, 1, C1)));
+  const dim3 t0 = tiles(pc->H, pc->W, wm_pc::B);
+  pc->nlossp = (int)(t0.x * t0.y * t0.z);
+  pc->part_elems = std::max<size_t>((size_t)512 * MAXW, (size_t)pc->nlossp * 9 * C0);
+  WM_TRY(dalloc(ctx, &pc->part, pc->part_elems));
+  WM_TRY(dalloc(ctx, &pc->lossp, pc->nlossp));
+  WM_TRY(dalloc(ctx, &pc->himg, 2 * n0));
+  return WM_OK;
+}
+
+int wm_pc_set_params_dev(wm_ctx *ctx, const float *theta_host, int n) {
+  if (n != NPARAM) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: %d parameters given, %d expected", n,
+                 NPARAM);
+    return WM_ERR_ARG;
+  }
+  WM_TRY(wm_pc_alloc(ctx));
+  wm_pc *pc = ctx->pc;
+  WM_HIP(hipMemcpyAsync(pc->theta, theta_host, sizeof(float) * NPARAM, hipMemcpyHostToDevice,
+                        ctx->stream));
+  WM_HIP(hipMemsetAsync(pc->m, 0, sizeof(float) * NPARAM, ctx->stream));
+  WM_HIP(hipMemsetAsync(pc->v, 0, sizeof(float) * NPARAM, ctx->stream));
+  pc->adam_t = 0;
+  WM_TRY(refresh_weights(ctx, pc));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  pc->trained = true;
+  return WM_OK;
+}
+
+int wm_pc_get_params_dev(wm_ctx *ctx, float *theta_host, int n) {
+  if (!ctx->pc || n != NPARAM) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: no parameters (or n != %d)", NPARAM);
+    return WM_ERR_ARG;
+  }
+  WM_HIP(hipMemcpyAsync(theta_host, ctx->pc->theta, sizeof(float) * NPARAM, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+// Net applied to three host images r[3][ny][nx] -> z.
+int wm_pc_apply_host(wm_ctx *ctx, const double *r, double *z) {
+  wm_pc *pc = ctx->pc;
+  if (!pc || !pc->trained) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: set or train the parameters first");
+    return WM_ERR_ARG;
+  }
+  WM_TRY(alloc_training(ctx, pc));
+  const size_t n = (size_t)pc->H * pc->W;
+  WM_HIP(hipMemcpyAsync(pc->himg, r, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_img_scale, dim3(3), dim3(256), 0, ctx->stream, pc->himg, n, pc->isc);
+  WM_TRY(forward(ctx, pc, pc->himg, n, pc->isc, pc->zt, n, nullptr));
+  WM_HIP(hipMemcpyAsync(z, pc->zt, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+// Loss and gradient for host pair images (r, e)[3][ny][nx].
+int wm_pc_grad_host(wm_ctx *ctx, const double *r, const double *e, float *grad, double *loss) {
+  wm_pc *pc = ctx->pc;
+  if (!pc || !pc->trained) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: set or train the parameters first");
+    return WM_ERR_ARG;
+  }
+  WM_TRY(alloc_training(ctx, pc));
+  const size_t n = (size_t)pc->H * pc->W;
+  WM_HIP(hipMemcpyAsync(pc->himg, r, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+  WM_HIP(hipMemcpyAsync(pc->himg + 3 * n, e, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                        ctx->stream));
+  hipLaunchKernelGGL(k_img_scale, dim3(3), dim3(256), 0, ctx->stream, pc->himg, n, pc->isc);
+  double *ld = pc->lossp;   // reused: k_sum_loss reads lossp before writing here? no: separate slot
+  if (pc->losses_cap < 1) {
+    if (pc->losses) (void)hipFree(pc->losses);
+    WM_TRY(dalloc(ctx, &pc->losses, 1024));
+    pc->losses_cap = 1024;
+  }
+  ld = pc->losses;
+  WM_TRY(backward(ctx, pc, pc->himg, pc->himg + 3 * n, n, pc->isc, ld));
+  WM_HIP(hipMemcpyAsync(grad, pc->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipMemcpyAsync(loss, ld, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int wm_pc_collect_cfg(wm_ctx *ctx, int nslot, int per_solve) {
+  if (nslot < 0 || nslot > 64 || per_solve < 0) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: 0 <= slots <= 64, per_solve >= 0");
+    return WM_ERR_ARG;
+  }
+  WM_TRY(wm_pc_alloc(ctx));
+  wm_pc *pc = ctx->pc;
+  const size_t n = (size_t)pc->H * pc->W;
+  if (nslot > pc->nslot) {
+    if (pc->pr) (void)hipFree(pc->pr);
+    if (pc->pe) (void)hipFree(pc->pe);
+    if (pc->psc) (void)hipFree(pc->psc);
+    pc->pr = pc->pe = pc->psc = nullptr;
+    WM_TRY(dalloc(ctx, &pc->pr, (size_t)nslot * 3 * n));
+    WM_TRY(dalloc(ctx, &pc->pe, (size_t)nslot * 3 * n));
+    WM_TRY(dalloc(ctx, &pc->psc, (size_t)nslot * 6));
+    pc->filled = pc->next = 0;
+  }
+  pc->nslot = nslot;
+  pc->per_solve = nslot ? per_solve : 0;
+  if (pc->filled > nslot) pc->filled = nslot;
+  if (pc->next >= std::max(nslot, 1)) pc->next = 0;
+  return WM_OK;
+}
+
+bool wm_pc_collecting(const wm_ctx *ctx) { return ctx->pc && ctx->pc->per_solve > 0; }
+
+// Called by the plain solver at the start of iteration k (all components
+// still active).
+int wm_pc_snapshot(wm_ctx *ctx, int k) {
+  wm_pc *pc = ctx->pc;
+  if (!pc || k >= pc->per_solve || pc->nsnap >= 64) return WM_OK;
+  const int s = pc->next;
+  pc->next = (pc->next + 1) % pc->nslot;
+  if (pc->filled < pc->nslot) ++pc->filled;
+  const size_t n = (size_t)pc->H * pc->W;
+  hipLaunchKernelGGL(k_snap, dim3(std::min(wm_cdiv((long long)n, 256), 4096), 3), dim3(256), 0,
+                     ctx->stream, ctx->cg_r, ctx->df, ctx->g.nx, ctx->g.ny, ctx->fg.gx, ctx->fg.ps,
+                     ctx->cg_s, pc->pr + (size_t)s * 3 * n, pc->pe + (size_t)s * 3 * n,
+                     pc->psc + (size_t)s * 6);
+  WM_HIP(hipGetLastError());
+  pc->snap_slot[pc->nsnap++] = s;
+  return WM_OK;
+}
+
+// Called by the plain solver after convergence: corrections of the snapshots.
+int wm_pc_snapshot_fin(wm_ctx *ctx) {
+  wm_pc *pc = ctx->pc;
+  if (!pc) return WM_OK;
+  const size_t n = (size_t)pc->H * pc->W;
+  for (int i = 0; i < pc->nsnap; ++i) {
+    const int s = pc->snap_slot[i];
+    hipLaunchKernelGGL(k_snap_fin, dim3(std::min(wm_cdiv((long long)n, 256), 4096), 3), dim3(256),
+                       0, ctx->stream, ctx->df, ctx->g.nx, ctx->g.ny, ctx->fg.gx, ctx->fg.ps,
+                       pc->pe + (size_t)s * 3 * n);
+  }
+  pc->nsnap = 0;
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_pc_pairs(const wm_ctx *ctx) { return ctx->pc ? ctx->pc->filled : 0; }
+
+// Default initial weights: the 1/f4 identity stencil through channel 0 of
+// lift/proj plus small fan-in-scaled values (host counter-based generator).
+static void init_theta(std::vector<float> &th, double f4, uint64_t seed) {
+  th.assign(NPARAM, 0.0f);
+  uint64_t s = seed ^ 0x243F6A8885A308D3ULL;
+  for (int l = 0; l < NLAYER; ++l) {
+    const int ci = LCIN[l], co = LCOUT[l];
+    const double sd = 0.1 / sqrt(9.0 * ci);
+    for (int k = 0; k < 9 * ci * co; ++k) {
+      const double u1 = ((splitmix(s) >> 11) + 1) * 0x1.0p-53, u2 = (splitmix(s) >> 11) * 0x1.0p-53;
+      th[loff(l) + k] = (float)(sd * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+    }
+  }
+  th[loff(L_LIFT) + 4 * C0 + 0] = 1.0f;
+  th[loff(L_PROJ) + 4 * C0 + 0] = (float)(1.0 / f4);
+}
+
+int wm_pc_init_params(wm_ctx *ctx, uint64_t seed) {
+  WM_TRY(wm_pc_alloc(ctx));
+  const double f5 = pow(ctx->g.delx / (ctx->g.c * ctx->g.delt * ctx->g.gfac), 2);
+  std::vector<float> th;
+  init_theta(th, 4.0 + f5, seed);
+  return wm_pc_set_params_dev(ctx, th.data(), NPARAM);
+}
+
+// Adam training on the recorded pairs, cosine learning-rate decay from lr.
+int wm_pc_train_dev(wm_ctx *ctx, int steps, double lr, uint64_t seed, double *loss_out) {
+  wm_pc *pc = ctx->pc;
+  if (!pc || pc->filled == 0) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: no (residual, correction) pairs recorded");
+    return WM_ERR_ARG;
+  }
+  if (steps <= 0 || !(lr > 0)) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: steps > 0 and lr > 0 required");
+    return WM_ERR_ARG;
+  }
+  if (!pc->trained) WM_TRY(wm_pc_init_params(ctx, seed));
+  WM_TRY(alloc_training(ctx, pc));
+  if (pc->losses_cap < steps) {
+    if (pc->losses) (void)hipFree(pc->losses);
+    pc->losses = nullptr;
+    WM_TRY(dalloc(ctx, &pc->losses, steps));
+    pc->losses_cap = steps;
+  }
+  const size_t n = (size_t)pc->H * pc->W;
+  uint64_t rs = seed;
+  const double b1 = 0.9, b2 = 0.999;
+  for (int it = 0; it < steps; ++it) {
+    const int s = (int)(splitmix(rs) % (uint64_t)pc->filled);
+    WM_TRY(backward(ctx, pc, pc->pr + (size_t)s * 3 * n, pc->pe + (size_t)s * 3 * n, n,
+                    pc->psc + (size_t)s * 6, pc->losses + it));
+    ++pc->adam_t;
+    const double lrt = lr * 0.5 * (1.0 + cos(M_PI * it / steps));
+    const float c1 = (float)(1.0 / (1.0 - pow(b1, pc->adam_t)));
+    const float c2 = (float)(1.0 / (1.0 - pow(b2, pc->adam_t)));
+    hipLaunchKernelGGL(k_adam, dim3(wm_cdiv(NPARAM, 256)), dim3(256), 0, ctx->stream, pc->theta,
+                       pc->m, pc->v, pc->grad, NPARAM, (float)lrt, (float)b1, (float)b2, c1, c2,
+                       1e-8f);
+    WM_TRY(refresh_weights(ctx, pc));
+  }
+  std::vector<double> ls(steps);
+  WM_HIP(hipMemcpyAsync(ls.data(), pc->losses, sizeof(double) * steps, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (loss_out) {
+    const int k = std::min(steps, 10);
+    double a = 0.0;
+    for (int i = steps - k; i < steps; ++i) a += ls[i];
+    loss_out[0] = ls[0];
+    loss_out[1] = a / k;
+  }
+  pc->trained = true;
+  return WM_OK;
+}
+
+// Preconditioner application inside the solver: z[3][n] = M(r[3][n]) with
+// device scales sc (s[3], 1/s[3]).
+int wm_pc_apply_dev(wm_ctx *ctx, const double *r, const double *sc, double *z) {
+  wm_pc *pc = ctx->pc;
+  const size_t n = (size_t)pc->H * pc->W;
+  return forward(ctx, pc, r, n, sc, z, n, nullptr);
+}
+
+bool wm_pc_ready(const wm_ctx *ctx) { return ctx->pc && ctx->pc->trained; }
+
+void wm_pc_set_stats(wm_ctx *ctx, const int *fcg, const int *fb) {
+  if (!ctx->pc) return;
+  for (int l = 0; l < 3; ++l) {
+    ctx->pc->last_fcg[l] = fcg[l];
+    ctx->pc->last_fallback[l] = fb[l];
+  }
+}
+
+void wm_pc_get_stats(const wm_ctx *ctx, int *fcg, int *fb) {
+  for (int l = 0; l < 3; ++l) {
+    fcg[l] = ctx->pc ? ctx->pc->last_fcg[l] : 0;
+    fb[l] = ctx->pc ? ctx->pc->last_fallback[l] : 0;
+  }
+}

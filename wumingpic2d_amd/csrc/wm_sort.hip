@@ -11,6 +11,9 @@
 //   start[key] + (equal keys already written) + rank,
 // exactly the reference's stable scatter.  Keys are int(x) as in sort.f90:60
 // (the reference assumes delx = 1).
+#include <algorithm>
+#include <utility>
+
 #include "wm_internal.h"
 
 namespace {
@@ -138,7 +141,156 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, 
   if (bad) atomicOr(derr, DERR_RANGE);
 }
 
+// ------------------------------------------------------------------------
+// FP32 form: row migration and cell sort as one stable counting scatter.
+//
+// The push counts, per source segment, where its particles go: bcnt[src
+// seg][dy + 1][new cell] (dy = row change).  Destination row jd takes, cell
+// by cell, the arrivals from row jd - 1 (moving up), then the particles that
+// stayed, then the arrivals from row jd + 1 (moving down), each group in its
+// source index order.  This is a fixed permutation (the same every run) but
+// not the reference's hole-filling order of boundary_periodic.f90:193-226,
+// which the FP64 form keeps; cumcnt and np2 mean exactly what they mean
+// after sort.f90:36-82.
+
+__global__ void k_fill_int(int *p, size_t n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    p[e] = 0;
+}
+
+// One workgroup per destination segment: cumcnt, np2 and, in place of the
+// three source counts that land here, their first destination slot.
+__global__ __launch_bounds__(256) void k_bin_prefix(int nx, int ny, int cap, int *__restrict__ bcnt,
+                                                    int *__restrict__ cumcnt,
+                                                    int *__restrict__ np2new,
+                                                    int *__restrict__ derr) {
+  __shared__ int sh[256];
+  const int dseg = blockIdx.x;
+  const int s = dseg / ny, jd = dseg % ny;
+  const int jlo = jd == 0 ? ny - 1 : jd - 1, jhi = jd == ny - 1 ? 0 : jd + 1;
+  int *lo = bcnt + (((size_t)s * ny + jlo) * 3 + 2) * nx;   // row jd-1 moving up
+  int *mid = bcnt + (((size_t)s * ny + jd) * 3 + 1) * nx;   // staying
+  int *hi = bcnt + (((size_t)s * ny + jhi) * 3 + 0) * nx;   // row jd+1 moving down
+  const int per = (nx + 255) / 256;
+  const int c0 = min(nx, (int)threadIdx.x * per), c1 = min(nx, c0 + per);
+  int tot = 0;
+  for (int c = c0; c < c1; ++c) tot += lo[c] + mid[c] + hi[c];
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - tot;
+  int *cc = cumcnt + (size_t)dseg * (nx + 1);
+  for (int c = c0; c < c1; ++c) {
+    const int a = lo[c], b = mid[c], d = hi[c];
+    cc[c] = run;
+    lo[c] = run;
+    mid[c] = run + a;
+    hi[c] = run + a + b;
+    run += a + b + d;
+  }
+  if (threadIdx.x == 255) {
+    cc[nx] = sh[255];
+    np2new[dseg] = sh[255];
+    if (sh[255] > cap) atomicOr(derr, DERR_NP_OVER);
+  }
+}
+
+// One wavefront per source segment, walking it in index order (as k_sort).
+__global__ __launch_bounds__(SW) void k_bin_scatter(SP P, int ny, const int *__restrict__ np2,
+                                                    Rec32 src, const int64_t *__restrict__ ids,
+                                                    Rec32 dst, int64_t *__restrict__ idd,
+                                                    const int *__restrict__ bcnt,
+                                                    int *__restrict__ derr) {
+  extern __shared__ int offs[];            // [3][nx] next destination slot
+  const int seg = blockIdx.x;
+  const int s = seg / ny, jl = seg % ny;
+  const int lane = threadIdx.x;
+  const int n = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  for (int i = lane; i < 3 * P.nx; i += SW) offs[i] = bcnt[(size_t)seg * 3 * P.nx + i];
+  __syncthreads();
+  int bad = 0;
+  for (int k0 = 0; k0 < n; k0 += SW * BT) {
+    int key[BT], dy[BT];
+    Rec32::Val v[BT];
+    int64_t vid[BT];
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const int k = k0 + b * SW + lane;
+      key[b] = -1;
+      dy[b] = 0;
+      if (k < n) {
+        const size_t p = base + k;
+        v[b] = src.load(p);
+        vid[b] = ids[p];
+        dy[b] = v[b].oy < 0.0f ? -1 : (v[b].oy >= 1.0f ? 1 : 0);
+        key[b] = (v[b].ci >= 0 && v[b].ci < P.nx) ? (dy[b] + 1) * P.nx + v[b].ci : -1;
+        if (key[b] < 0) bad = 1;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < BT; ++b) {
+      const bool act = key[b] >= 0;
+      const WaveMatch m = wave_match(key[b], act);
+      if (act) {
+        const int slot = offs[key[b]] + m.rank;
+        const int jd = jl + dy[b] < 0 ? ny - 1 : (jl + dy[b] >= ny ? 0 : jl + dy[b]);
+        if (slot < P.cap) {
+          Rec32::Val w = v[b];
+          w.oy = dy[b] > 0 ? w.oy - 1.0f : (dy[b] < 0 ? wm_below_one(w.oy + 1.0f) : w.oy);
+          const size_t d = ((size_t)s * ny + jd) * P.cap + slot;
+          dst.store(d, w);
+          idd[d] = vid[b];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (act && m.leader) offs[key[b]] += m.count;
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (bad) atomicOr(derr, DERR_RANGE);
+}
+
 }  // namespace
+
+int wm_k_bin_clear(wm_ctx *ctx) {
+  const size_t n = (size_t)ctx->g.nsp * ctx->g.ny * 3 * ctx->g.nx;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)),
+                     dim3(256), 0, ctx->stream, ctx->bcnt, n);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_bin32(wm_ctx *ctx, int src, int dst) {
+  SP P;
+  P.nx = ctx->g.nx;
+  P.nxgs = ctx->g.nxgs;
+  P.cap = ctx->cap;
+  const int ny = ctx->g.ny;
+  const int nseg = ctx->g.nsp * ny;
+  const size_t lds = sizeof(int) * 3 * (size_t)P.nx;
+  if (lds > 65536) {
+    wm_set_error(ctx, WM_ERR_ARG, "row migration: nx=%d exceeds the 5461 cells one LDS offset table holds",
+                 P.nx);
+    return WM_ERR_ARG;
+  }
+  // new np2 goes to the spare per-segment array, which then becomes np2
+  hipLaunchKernelGGL(k_bin_prefix, dim3(nseg), dim3(256), 0, ctx->stream, P.nx, ny, P.cap,
+                     ctx->bcnt, ctx->cumcnt, ctx->nup, ctx->derr);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(nseg), dim3(SW), lds, ctx->stream, P, ny, ctx->np2,
+                     ctx->b32[src], ctx->id[ctx->idcur], ctx->b32[dst], ctx->id[1 - ctx->idcur],
+                     ctx->bcnt, ctx->derr);
+  WM_HIP(hipGetLastError());
+  std::swap(ctx->np2, ctx->nup);
+  ctx->idcur = 1 - ctx->idcur;
+  return WM_OK;
+}
 
 int wm_k_sort(wm_ctx *ctx, int src, int dst) {
   SP P;
