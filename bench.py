@@ -188,7 +188,8 @@ def main() -> int:
         a = torch.ones(n, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            b.copy_(a)                                # runtime copy kernel
+            torch.add(a, 1.0, out=b)                  # elementwise kernel: 1 GiB in, 1 GiB out
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n}))
         return 0

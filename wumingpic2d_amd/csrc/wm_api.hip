@@ -413,7 +413,10 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->diag, (size_t)nseg * 8 + 2 * (size_t)g.ny + 64));
   WM_TRY(dmalloc(ctx, &ctx->derr, 1));
   WM_TRY(dmalloc(ctx, &ctx->cnt_scratch, nseg));
-  if (ctx->form == WM_FORM_FP32) WM_TRY(dmalloc(ctx, &ctx->bcnt, (size_t)nseg * 3 * g.nx));
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(dmalloc(ctx, &ctx->bcnt, (size_t)nseg * 3 * g.nx));
+    WM_TRY(dmalloc(ctx, &ctx->cumcnt2, (size_t)nseg * (g.nx + 1)));
+  }
   WM_TRY(dmalloc(ctx, &ctx->fscratch, 6 * fg.ps));
 
   WM_HIP(hipMemsetAsync(ctx->red_cnt, 0, 8 * sizeof(unsigned), ctx->stream));
@@ -484,7 +487,7 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
-                  ctx->cnt_scratch, ctx->fscratch, ctx->bcnt, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->cnt_scratch, ctx->fscratch, ctx->bcnt, ctx->cumcnt2, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);

@@ -143,6 +143,7 @@ struct wm_ctx {
   double phase_ms[8] = {0};
   int *cnt_scratch = nullptr;    // [nseg] counts for layout conversion
   int *bcnt = nullptr;           // FP32 form: [nseg][3][nx] destination counts, then offsets
+  int *cumcnt2 = nullptr;        // FP32 form: the next step's cumcnt while the old is read
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
   long long cg_iters_total = 0;
 

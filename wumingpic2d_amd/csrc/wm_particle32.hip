@@ -27,7 +27,7 @@ constexpr int NACC = 65;
 
 struct PP32 {
   int nx, ny, nxgs, nys, nsp, cap, nchunk;
-  float delt, c, c2, qc_scale;        // qc_scale = delx/delt
+  float delt, c, c2, inv_c2, qc_scale;   // qc_scale = delx/delt
   float q[WM_MAXSP], r[WM_MAXSP];
 };
 
@@ -61,10 +61,13 @@ __device__ inline Push32 boris32(const float *T, int gcol, float ox, float oy, f
   float uvm1 = ux + fac1 * epx;
   float uvm2 = uy + fac1 * epy;
   float uvm3 = uz + fac1 * epz;
-  const float gam = sqrtf(P.c2 + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
-  const float igam = 1.0f / gam;
+  // hardware reciprocal square root and reciprocal (1 ulp): the FP32 form's
+  // error budget is set by FP32 storage, not by these
+  const float g2 = P.c2 + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3;
+  const float igam = __builtin_amdgcn_rsqf(g2);
+  const float gam = g2 * igam;
   const float fac1r = fac1 * igam;
-  const float fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+  const float fac2r = fac2 * __builtin_amdgcn_rcpf(gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
   const float uvm4 = uvm1 + fac1r * (uvm2 * bpz - uvm3 * bpy);
   const float uvm5 = uvm2 + fac1r * (uvm3 * bpx - uvm1 * bpz);
   const float uvm6 = uvm3 + fac1r * (uvm1 * bpy - uvm2 * bpx);
@@ -75,7 +78,7 @@ __device__ inline Push32 boris32(const float *T, int gcol, float ox, float oy, f
   o.ux = uvm1 + fac1 * epx;
   o.uy = uvm2 + fac1 * epy;
   o.uz = uvm3 + fac1 * epz;
-  const float ig = 1.0f / sqrtf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) / P.c2);
+  const float ig = __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2);
   o.ox = ox + o.ux * P.delt * ig;
   o.oy = oy + o.uy * P.delt * ig;
   return o;
@@ -230,7 +233,7 @@ __device__ inline void crossing_stencils(float *acc, size_t p, int il, const Rec
   Move32 m;
   make_move32(m, A.ox[p], A.oy[p], B.ox[p], oy1, dx, dy);
   const float ux = B.ux[p], uy = B.uy[p], uz = B.uz[p];
-  const float qv = q * (uz / sqrtf(1.0f + (ux * ux + uy * uy + uz * uz) / P.c2));
+  const float qv = q * (uz * __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz) * P.inv_c2));
   const float qc = q * P.qc_scale;
   stencil32<0>(acc, m, qc, qv);
   stencil32<1>(acc + 20, m, qc, qv);
@@ -317,7 +320,7 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
           if (bcnt) atomicAdd(&lc[(s * 3 + dy + 1) * (CHUNK + 2) + g + dx + 1], 1);
           if (!cross) {
             const float qv =
-                q * (o.uz / sqrtf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) / P.c2));
+                q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
             core32(core, ox0, oy0, ox1, oy1, qc, qv);
           }
         }
@@ -515,6 +518,7 @@ PP32 make_pp32(const wm_ctx *ctx) {
   P.delt = (float)g.delt;
   P.c = (float)g.c;
   P.c2 = (float)(g.c * g.c);
+  P.inv_c2 = (float)(1.0 / (g.c * g.c));
   P.qc_scale = (float)(g.delx / g.delt);
   for (int s = 0; s < WM_MAXSP; ++s) {
     P.q[s] = s < g.nsp ? (float)g.q[s] : 0.0f;

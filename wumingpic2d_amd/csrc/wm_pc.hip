@@ -48,6 +48,7 @@ __host__ __device__ constexpr int loff(int l) {
 }
 constexpr int NPARAM = loff(NLAYER);
 constexpr int MAXW = 9 * 32 * 32;   // largest layer
+constexpr int FINE_BLOCKS = 1024;   // workgroups of the fine-level weight gradients
 
 __device__ inline int pmod(int a, int n) {
   a %= n;
@@ -306,100 +307,108 @@ __device__ inline double block_sum1(double v, double *sh) {
 
 // Backward of y = proj(u0) for the loss mean((y - e/s)^2):
 //   dy = 2 (y - e/s) / N, dW_proj = sum u0(p+off) dy(p), du0 = proj^T dy,
-// plus the loss partial.  One partial row per tile.
+// plus the loss.  Workgroup k handles tiles k, k + grid, ... in order and
+// leaves one partial row (and one loss partial).
 __global__ __launch_bounds__(256) void k_proj_bwd(
     const double *__restrict__ r, size_t rstride, const double *__restrict__ e, size_t estride,
     const double *__restrict__ sc, const float *__restrict__ y, const float *__restrict__ wl,
     const float *__restrict__ wp, const bf16 *__restrict__ b1, float *__restrict__ du0,
-    float *__restrict__ part, double *__restrict__ lossp, double gs, int H, int W, int B) {
+    float *__restrict__ part, double *__restrict__ lossp, double gs, int H, int W, int B,
+    int ntx, int nty, int ntiles) {
   __shared__ float xs[(T + 4) * (T + 4)];
   __shared__ float us[(T + 2) * (T + 2) * C0];
   __shared__ float dys[(T + 2) * (T + 2)];
   __shared__ double sh[256];
-  const int b = blockIdx.z;
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
-  const double is = sc[B + b];
-  load_x0<2>(xs, r + (size_t)b * rstride, is, x0, y0, H, W);
-  for (int q = threadIdx.x; q < (T + 2) * (T + 2); q += 256) {
-    const int gy = pmod(y0 + q / (T + 2) - 1, H), gx = pmod(x0 + q % (T + 2) - 1, W);
-    const size_t p = (size_t)gy * W + gx;
-    const double d = (double)y[(size_t)b * H * W + p] - e[(size_t)b * estride + p] * is;
-    dys[q] = (float)(gs * d);
-  }
-  __syncthreads();
-  build_u0(us, xs, wl, b1 + (size_t)b * (H / 2) * (W / 2) * C0, x0, y0, H, W);
-  __syncthreads();
   const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-  const bool in = y0 + ty < H && x0 + tx < W;
   double l2 = 0.0;
-  if (in) {
-    const size_t p = (size_t)(y0 + ty) * W + (x0 + tx);
-    const double d = (double)y[(size_t)b * H * W + p] - e[(size_t)b * estride + p] * is;
-    l2 = d * d;
-    float g[C0];
-#pragma unroll
-    for (int ci = 0; ci < C0; ++ci) g[ci] = 0.0f;
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      // dy at p - off(tap)
-      const float v = dys[(ty + 1 - (tap / 3 - 1)) * (T + 2) + (tx + 1 - (tap % 3 - 1))];
-#pragma unroll
-      for (int ci = 0; ci < C0; ++ci) g[ci] = fmaf(v, wp[tap * C0 + ci], g[ci]);
+  float acc = 0.0f;                                  // dW_proj[threadIdx.x] (< 9*C0)
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int b = t / (ntx * nty), tt = t % (ntx * nty);
+    const int x0 = (tt % ntx) * T, y0 = (tt / ntx) * T;
+    const double is = sc[B + b];
+    __syncthreads();
+    load_x0<2>(xs, r + (size_t)b * rstride, is, x0, y0, H, W);
+    for (int q = threadIdx.x; q < (T + 2) * (T + 2); q += 256) {
+      const int gy = pmod(y0 + q / (T + 2) - 1, H), gx = pmod(x0 + q % (T + 2) - 1, W);
+      const size_t p = (size_t)gy * W + gx;
+      const double d = (double)y[(size_t)b * H * W + p] - e[(size_t)b * estride + p] * is;
+      dys[q] = (float)(gs * d);
     }
-    float *o = du0 + (((size_t)b * H + y0 + ty) * W + x0 + tx) * C0;
+    __syncthreads();
+    build_u0(us, xs, wl, b1 + (size_t)b * (H / 2) * (W / 2) * C0, x0, y0, H, W);
+    __syncthreads();
+    if (y0 + ty < H && x0 + tx < W) {
+      const size_t p = (size_t)(y0 + ty) * W + (x0 + tx);
+      const double d = (double)y[(size_t)b * H * W + p] - e[(size_t)b * estride + p] * is;
+      l2 += d * d;
+      float g[C0];
 #pragma unroll
-    for (int ci = 0; ci < C0; ci += 4) *(float4 *)&o[ci] = make_float4(g[ci], g[ci + 1], g[ci + 2], g[ci + 3]);
+      for (int ci = 0; ci < C0; ++ci) g[ci] = 0.0f;
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        // dy at p - off(tap)
+        const float v = dys[(ty + 1 - (tap / 3 - 1)) * (T + 2) + (tx + 1 - (tap % 3 - 1))];
+#pragma unroll
+        for (int ci = 0; ci < C0; ++ci) g[ci] = fmaf(v, wp[tap * C0 + ci], g[ci]);
+      }
+      float *o = du0 + (((size_t)b * H + y0 + ty) * W + x0 + tx) * C0;
+#pragma unroll
+      for (int ci = 0; ci < C0; ci += 4) *(float4 *)&o[ci] = make_float4(g[ci], g[ci + 1], g[ci + 2], g[ci + 3]);
+    }
+    if (threadIdx.x < 9 * C0) {
+      const int tap = threadIdx.x / C0, ci = threadIdx.x % C0;
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      for (int q = 0; q < T * T; ++q) {
+        const int qy = q / T, qx = q % T;
+        if (y0 + qy >= H || x0 + qx >= W) continue;
+        acc = fmaf(us[((qy + 1 + dy) * (T + 2) + (qx + 1 + dx)) * C0 + ci],
+                   dys[(qy + 1) * (T + 2) + qx + 1], acc);
+      }
+    }
   }
-  const size_t blk = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  __syncthreads();
   const double tot = block_sum1(l2, sh);
-  if (threadIdx.x == 0) lossp[blk] = tot;
-  if (threadIdx.x < 9 * C0) {
-    const int tap = threadIdx.x / C0, ci = threadIdx.x % C0;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    float acc = 0.0f;
-    for (int q = 0; q < T * T; ++q) {
-      const int qy = q / T, qx = q % T;
-      if (y0 + qy >= H || x0 + qx >= W) continue;
-      acc = fmaf(us[((qy + 1 + dy) * (T + 2) + (qx + 1 + dx)) * C0 + ci],
-                 dys[(qy + 1) * (T + 2) + qx + 1], acc);
-    }
-    part[blk * (9 * C0) + threadIdx.x] = acc;
-  }
+  if (threadIdx.x == 0) lossp[blockIdx.x] = tot;
+  if (threadIdx.x < 9 * C0) part[(size_t)blockIdx.x * (9 * C0) + threadIdx.x] = acc;
 }
 
-// dW_lift[tap][co] = sum_p x0(p+off) * da0(p, co), da0 = du0 + 0.25 up(dd1).
+// dW_lift[tap][co] = sum_p x0(p+off) * da0(p, co), da0 = du0 + 0.25 up(dd1);
+// tiles as in k_proj_bwd.
 __global__ __launch_bounds__(256) void k_lift_bwd(const double *__restrict__ r, size_t rstride,
                                                   const double *__restrict__ sc,
                                                   const float *__restrict__ du0,
                                                   const float *__restrict__ dd1,
-                                                  float *__restrict__ part, int H, int W, int B) {
+                                                  float *__restrict__ part, int H, int W, int B,
+                                                  int ntx, int nty, int ntiles) {
   __shared__ float xs[(T + 2) * (T + 2)];
   __shared__ float das[T * T * C0];
-  const int b = blockIdx.z;
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
-  load_x0<1>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
   const int W1 = W / 2;
-  for (int e = threadIdx.x; e < T * T * C0; e += 256) {
-    const int co = e % C0, q = e / C0;
-    const int gy = y0 + q / T, gx = x0 + q % T;
-    float v = 0.0f;
-    if (gy < H && gx < W)
-      v = du0[(((size_t)b * H + gy) * W + gx) * C0 + co] +
-          0.25f * dd1[(((size_t)b * (H / 2) + gy / 2) * W1 + gx / 2) * C0 + co];
-    das[e] = v;
-  }
-  __syncthreads();
-  const size_t blk = ((size_t)b * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  if (threadIdx.x < 9 * C0) {
-    const int tap = threadIdx.x / C0, co = threadIdx.x % C0;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    float acc = 0.0f;
-    for (int q = 0; q < T * T; ++q) {
-      const int qy = q / T, qx = q % T;
-      acc = fmaf(xs[(qy + 1 + dy) * (T + 2) + qx + 1 + dx], das[q * C0 + co], acc);
+  float acc = 0.0f;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int b = t / (ntx * nty), tt = t % (ntx * nty);
+    const int x0 = (tt % ntx) * T, y0 = (tt / ntx) * T;
+    __syncthreads();
+    load_x0<1>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
+    for (int e = threadIdx.x; e < T * T * C0; e += 256) {
+      const int co = e % C0, q = e / C0;
+      const int gy = y0 + q / T, gx = x0 + q % T;
+      float v = 0.0f;
+      if (gy < H && gx < W)
+        v = du0[(((size_t)b * H + gy) * W + gx) * C0 + co] +
+            0.25f * dd1[(((size_t)b * (H / 2) + gy / 2) * W1 + gx / 2) * C0 + co];
+      das[e] = v;
     }
-    part[blk * (9 * C0) + threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 9 * C0) {
+      const int tap = threadIdx.x / C0, co = threadIdx.x % C0;
+      const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+      for (int q = 0; q < T * T; ++q) {
+        const int qy = q / T, qx = q % T;
+        acc = fmaf(xs[(qy + 1 + dy) * (T + 2) + qx + 1 + dx], das[q * C0 + co], acc);
+      }
+    }
   }
+  if (threadIdx.x < 9 * C0) part[(size_t)blockIdx.x * (9 * C0) + threadIdx.x] = acc;
 }
 
 // Weight gradient of an MFMA layer: dW[tap][co][ci] = sum_p X(p+off, ci) G(p, co),
@@ -452,14 +461,24 @@ __global__ __launch_bounds__(256) void k_conv_dw(const bf16 *__restrict__ X,
   }
 }
 
-// grad[o] = sum over nblk partial rows, in row order.
-__global__ void k_reduce_rows(const float *__restrict__ part, int nblk, int n,
-                              float *__restrict__ grad) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= n) return;
+// grad[o] = sum over nblk partial rows: 32 outputs per workgroup, eight
+// row groups (rows g, g + 8, ...) added in a fixed order.
+__global__ __launch_bounds__(256) void k_reduce_rows(const float *__restrict__ part, int nblk,
+                                                     int n, float *__restrict__ grad) {
+  __shared__ float sh[8][32];
+  const int ol = threadIdx.x % 32, g = threadIdx.x / 32;
+  const int o = blockIdx.x * 32 + ol;
   float s = 0.0f;
-  for (int k = 0; k < nblk; ++k) s += part[(size_t)k * n + o];
-  grad[o] = s;
+  if (o < n)
+    for (int k = g; k < nblk; k += 8) s += part[(size_t)k * n + o];
+  sh[g][ol] = s;
+  __syncthreads();
+  if (g == 0 && o < n) {
+    float t = sh[0][ol];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) t += sh[k][ol];
+    grad[o] = t;
+  }
 }
 
 __global__ void k_sum_loss(const double *__restrict__ lossp, int nblk, double inv_n,
@@ -615,7 +634,7 @@ int conv_dw(wm_ctx *ctx, wm_pc *pc, int l, const bf16 *X, const bf16 *G, int H, 
   constexpr int NO = 9 * CIN * COUT;
   hipLaunchKernelGGL((k_conv_dw<CIN, COUT>), dim3(nblk), dim3(256), 0, ctx->stream, X, G, pc->part,
                      H, W, ntx, nty, ntiles);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(wm_cdiv(NO, 256)), dim3(256), 0, ctx->stream, pc->part,
+  hipLaunchKernelGGL(k_reduce_rows, dim3(wm_cdiv(NO, 32)), dim3(256), 0, ctx->stream, pc->part,
                      nblk, NO, pc->grad + loff(l));
   WM_HIP(hipGetLastError());
   return WM_OK;
@@ -661,12 +680,13 @@ int backward(wm_ctx *ctx, wm_pc *pc, const double *r, const double *e, size_t st
   WM_TRY(forward(ctx, pc, r, stride, sc, pc->zt, (size_t)H * W, pc->y));
   const double ntot = (double)B * H * W;
   const dim3 t0 = tiles(H, W, B);
-  const int nb0 = (int)(t0.x * t0.y * t0.z);
-  hipLaunchKernelGGL(k_proj_bwd, t0, dim3(256), 0, st, r, stride, e, stride, sc, pc->y,
+  const int ntiles0 = (int)(t0.x * t0.y * t0.z);
+  const int nb0 = std::min(ntiles0, FINE_BLOCKS);
+  hipLaunchKernelGGL(k_proj_bwd, dim3(nb0), dim3(256), 0, st, r, stride, e, stride, sc, pc->y,
                      th + loff(L_LIFT), th + loff(L_PROJ), pc->b1, pc->du0, pc->part, pc->lossp,
-                     2.0 / ntot, H, W, B);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, st, pc->part, nb0, 9 * C0,
-                     pc->grad + loff(L_PROJ));
+                     2.0 / ntot, H, W, B, (int)t0.x, (int)t0.y, ntiles0);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(wm_cdiv(9 * C0, 32)), dim3(256), 0, st, pc->part, nb0,
+                     9 * C0, pc->grad + loff(L_PROJ));
   hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, st, pc->lossp, nb0, 1.0 / ntot, loss_dev);
   WM_HIP(hipGetLastError());
   // U1: b1 = U1(u1); u0 = a0 + up(b1)
@@ -695,10 +715,10 @@ int backward(wm_ctx *ctx, wm_pc *pc, const double *r, const double *e, size_t st
   WM_TRY((conv_dw<C0, C1>(ctx, pc, L_D1, pc->d1, pc->da1, H >> 1, W >> 1)));
   WM_TRY((conv<C1, C0, true>(ctx, pc->da1, wt + loff(L_D1), pc->dd1, H >> 1, W >> 1)));
   // lift: d1 = pool(a0), u0 = a0 + ...
-  hipLaunchKernelGGL(k_lift_bwd, t0, dim3(256), 0, st, r, stride, sc, pc->du0, pc->dd1, pc->part,
-                     H, W, B);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, st, pc->part, nb0, 9 * C0,
-                     pc->grad + loff(L_LIFT));
+  hipLaunchKernelGGL(k_lift_bwd, dim3(nb0), dim3(256), 0, st, r, stride, sc, pc->du0, pc->dd1,
+                     pc->part, H, W, B, (int)t0.x, (int)t0.y, ntiles0);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(wm_cdiv(9 * C0, 32)), dim3(256), 0, st, pc->part, nb0,
+                     9 * C0, pc->grad + loff(L_LIFT));
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -790,9 +810,8 @@ static int alloc_training(wm_ctx *ctx, wm_pc *pc) {
   WM_TRY(dalloc(ctx, &pc->da3, lvl(pc, 3, C2)));
   WM_TRY(dalloc(ctx, &pc->da2, lvl(pc, 2, C2)));
   WM_TRY(dalloc(ctx, &pc->da1, lvl(pc, 1, C1)));
-  const dim3 t0 = tiles(pc->H, pc->W, wm_pc::B);
-  pc->nlossp = (int)(t0.x * t0.y * t0.z);
-  pc->part_elems = std::max<size_t>((size_t)512 * MAXW, (size_t)pc->nlossp * 9 * C0);
+  pc->nlossp = FINE_BLOCKS;
+  pc->part_elems = std::max<size_t>((size_t)512 * MAXW, (size_t)FINE_BLOCKS * 9 * C0);
   WM_TRY(dalloc(ctx, &pc->part, pc->part_elems));
   WM_TRY(dalloc(ctx, &pc->lossp, pc->nlossp));
   WM_TRY(dalloc(ctx, &pc->himg, 2 * n0));

@@ -201,22 +201,96 @@ __global__ __launch_bounds__(256) void k_bin_prefix(int nx, int ny, int cap, int
   }
 }
 
-// One wavefront per source segment, walking it in index order (as k_sort).
-__global__ __launch_bounds__(SW) void k_bin_scatter(SP P, int ny, const int *__restrict__ np2,
-                                                    Rec32 src, const int64_t *__restrict__ ids,
-                                                    Rec32 dst, int64_t *__restrict__ idd,
-                                                    const int *__restrict__ bcnt,
-                                                    int *__restrict__ derr) {
-  extern __shared__ int offs[];            // [3][nx] next destination slot
-  const int seg = blockIdx.x;
+// Stable scatter, one wavefront per (source segment, range of SUBC old
+// cells).  The source is sorted by old cell, and a particle lands at most one
+// cell away, so a range's particles can only share destination keys with the
+// old cells just outside it.  Those that come earlier in the source order
+// (old cells below c0, including the periodic wrap of the last range) are
+// counted first; the range then writes its particles in order behind them.
+constexpr int SUBC = 32;
+
+// Local key of landing cell k (-1 if outside the range's key set): the
+// range's own cells, then the periodic neighbours c0 - 1 and c1.
+__device__ inline int local_key(int k, int c0, int c1, int nx, int nk) {
+  if (k >= c0 && k < c1) return k - c0 + 1;
+  if (k == (c0 == 0 ? nx - 1 : c0 - 1)) return 0;
+  if (k == (c1 == nx ? 0 : c1)) return nk - 1;
+  return -1;
+}
+
+__device__ inline int row_move(float oy) { return oy < 0.0f ? -1 : (oy >= 1.0f ? 1 : 0); }
+
+constexpr int SCW = 4;     // wavefronts (ranges) per workgroup
+
+__global__ __launch_bounds__(SW * SCW) void k_bin_scatter(SP P, int ny, int nsub,
+                                                          const int *__restrict__ ccold,
+                                                          Rec32 src, const int64_t *__restrict__ ids,
+                                                          Rec32 dst, int64_t *__restrict__ idd,
+                                                          const int *__restrict__ bcnt,
+                                                          int *__restrict__ derr) {
+  __shared__ int offs[SCW][3 * (SUBC + 2)];
+  __shared__ int preds[SCW][8];
+  __shared__ int npreds[SCW];
+  const int wv = threadIdx.x / SW;
+  const int range = blockIdx.x * SCW + wv;
+  if (range >= nsub) return;                       // whole wavefronts only: no barrier below
+  int *off = offs[wv], *pred = preds[wv];
+  int &npred = npreds[wv];
+  const int seg = blockIdx.y;
   const int s = seg / ny, jl = seg % ny;
-  const int lane = threadIdx.x;
-  const int n = np2[seg];
+  const int lane = threadIdx.x % SW;
+  const int nx = P.nx;
+  const int c0 = range * SUBC, c1 = min(nx, c0 + SUBC);
+  const bool whole = c0 == 0 && c1 == nx;           // one range: no neighbour keys
+  const int nk = c1 - c0 + 2;
+  const int *cc = ccold + (size_t)seg * (nx + 1);
   const size_t base = (size_t)seg * P.cap;
-  for (int i = lane; i < 3 * P.nx; i += SW) offs[i] = bcnt[(size_t)seg * 3 * P.nx + i];
-  __syncthreads();
+  // first destination slots of the range's keys
+  for (int e = lane; e < 3 * nk; e += SW) {
+    const int d = e / nk, kl = e % nk;
+    int cell = c0 - 1 + kl;
+    if (kl == 0) cell = c0 == 0 ? nx - 1 : c0 - 1;
+    if (kl == nk - 1) cell = c1 == nx ? 0 : c1;
+    off[e] = (whole && (kl == 0 || kl == nk - 1)) ? 0
+                                                   : bcnt[((size_t)seg * 3 + d) * nx + cell];
+  }
+  // old cells before c0 (in the source order) that neighbour one of the
+  // range's keys; only the edge keys c0 - 1, c0, c1 - 1, c1 (periodic) have
+  // neighbours outside [c0, c1), and these are among c0 - 2, c0 - 1, 0, 1
+  if (lane == 0) {
+    int np = 0;
+    if (!whole) {
+      const int edge[4] = {c0 == 0 ? nx - 1 : c0 - 1, c0, c1 - 1, c1 == nx ? 0 : c1};
+      for (int q = 0; q < 4; ++q) {
+        for (int d = -1; d <= 1; ++d) {
+          int o = edge[q] + d;
+          o = o < 0 ? o + nx : (o >= nx ? o - nx : o);
+          if (o >= c0) continue;
+          bool dup = false;
+          for (int t = 0; t < np; ++t) dup = dup || pred[t] == o;
+          if (!dup && np < 8) pred[np++] = o;
+        }
+      }
+    }
+    npred = np;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int q = 0; q < npred; ++q) {
+    const int o = pred[q];
+    for (int k = cc[o] + lane; k < cc[o + 1]; k += SW) {
+      const size_t p = base + k;
+      const int kl = local_key(src.ci[p], c0, c1, nx, nk);
+      if (kl >= 0) atomicAdd(&off[(row_move(src.oy[p]) + 1) * nk + kl], 1);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int bad = 0;
-  for (int k0 = 0; k0 < n; k0 += SW * BT) {
+  const int kb = cc[c0], ke = cc[c1];
+  for (int k0 = kb; k0 < ke; k0 += SW * BT) {
     int key[BT], dy[BT];
     Rec32::Val v[BT];
     int64_t vid[BT];
@@ -225,12 +299,13 @@ __global__ __launch_bounds__(SW) void k_bin_scatter(SP P, int ny, const int *__r
       const int k = k0 + b * SW + lane;
       key[b] = -1;
       dy[b] = 0;
-      if (k < n) {
+      if (k < ke) {
         const size_t p = base + k;
         v[b] = src.load(p);
         vid[b] = ids[p];
-        dy[b] = v[b].oy < 0.0f ? -1 : (v[b].oy >= 1.0f ? 1 : 0);
-        key[b] = (v[b].ci >= 0 && v[b].ci < P.nx) ? (dy[b] + 1) * P.nx + v[b].ci : -1;
+        dy[b] = row_move(v[b].oy);
+        const int kl = (v[b].ci >= 0 && v[b].ci < nx) ? local_key(v[b].ci, c0, c1, nx, nk) : -1;
+        key[b] = kl >= 0 ? (dy[b] + 1) * nk + kl : -1;
         if (key[b] < 0) bad = 1;
       }
     }
@@ -239,7 +314,7 @@ __global__ __launch_bounds__(SW) void k_bin_scatter(SP P, int ny, const int *__r
       const bool act = key[b] >= 0;
       const WaveMatch m = wave_match(key[b], act);
       if (act) {
-        const int slot = offs[key[b]] + m.rank;
+        const int slot = off[key[b]] + m.rank;
         const int jd = jl + dy[b] < 0 ? ny - 1 : (jl + dy[b] >= ny ? 0 : jl + dy[b]);
         if (slot < P.cap) {
           Rec32::Val w = v[b];
@@ -250,7 +325,7 @@ __global__ __launch_bounds__(SW) void k_bin_scatter(SP P, int ny, const int *__r
         }
       }
       __builtin_amdgcn_wave_barrier();
-      if (act && m.leader) offs[key[b]] += m.count;
+      if (act && m.leader) off[key[b]] += m.count;
       __builtin_amdgcn_wave_barrier();
     }
   }
@@ -274,20 +349,17 @@ int wm_k_bin32(wm_ctx *ctx, int src, int dst) {
   P.cap = ctx->cap;
   const int ny = ctx->g.ny;
   const int nseg = ctx->g.nsp * ny;
-  const size_t lds = sizeof(int) * 3 * (size_t)P.nx;
-  if (lds > 65536) {
-    wm_set_error(ctx, WM_ERR_ARG, "row migration: nx=%d exceeds the 5461 cells one LDS offset table holds",
-                 P.nx);
-    return WM_ERR_ARG;
-  }
-  // new np2 goes to the spare per-segment array, which then becomes np2
+  // the new np2 and cumcnt go to spare arrays, which then take their place
+  // (the scatter reads the old cumcnt)
   hipLaunchKernelGGL(k_bin_prefix, dim3(nseg), dim3(256), 0, ctx->stream, P.nx, ny, P.cap,
-                     ctx->bcnt, ctx->cumcnt, ctx->nup, ctx->derr);
-  hipLaunchKernelGGL(k_bin_scatter, dim3(nseg), dim3(SW), lds, ctx->stream, P, ny, ctx->np2,
-                     ctx->b32[src], ctx->id[ctx->idcur], ctx->b32[dst], ctx->id[1 - ctx->idcur],
-                     ctx->bcnt, ctx->derr);
+                     ctx->bcnt, ctx->cumcnt2, ctx->nup, ctx->derr);
+  const int nsub = wm_cdiv(P.nx, SUBC);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(wm_cdiv(nsub, SCW), nseg), dim3(SW * SCW), 0, ctx->stream,
+                     P, ny, nsub, ctx->cumcnt, ctx->b32[src], ctx->id[ctx->idcur], ctx->b32[dst],
+                     ctx->id[1 - ctx->idcur], ctx->bcnt, ctx->derr);
   WM_HIP(hipGetLastError());
   std::swap(ctx->np2, ctx->nup);
+  std::swap(ctx->cumcnt, ctx->cumcnt2);
   ctx->idcur = 1 - ctx->idcur;
   return WM_OK;
 }
