@@ -410,10 +410,6 @@ __global__ __launch_bounds__(CGB) void k_cg_fin(CgArgs A, int stage, const doubl
 //   phi += alpha p; r -= alpha q; stop when |r| <= err |b|.
 // A non-finite or non-positive curvature marks the component bad: its phi is
 // restored from the warm start and the plain solve takes over.
-struct FcgScal {
-  double eps, sumr, rz, rz_prev, zq, pq, alpha, beta;
-  int active, it, bad, itmax;
-};
 
 __global__ __launch_bounds__(CGB) void k_fcg_save(CgArgs A, const double *__restrict__ df,
                                                   double *__restrict__ x0) {
@@ -431,25 +427,6 @@ __global__ __launch_bounds__(CGB) void k_fcg_restore(CgArgs A, double *__restric
   const CgCell C = cg_cell(A);
   if (!C.in) return;
   df[(size_t)l * A.ps + (size_t)(C.ii + 2) + (size_t)A.gx * (C.jj + 2)] = x0[(size_t)l * A.n + C.t];
-}
-
-// partial r.z and z.q_prev
-__global__ __launch_bounds__(CGB) void k_fcg_dots(CgArgs A, const double *__restrict__ z,
-                                                  const double *__restrict__ r,
-                                                  const double *__restrict__ q, double *partials,
-                                                  const FcgScal *S) {
-  __shared__ double sh[2][CGB];
-  const int l = blockIdx.y;
-  if (!S[l].active) return;
-  const bool first = S[l].it == 0;
-  const CgCell C = cg_cell(A);
-  double v[2] = {0.0, 0.0};
-  if (C.in) {
-    const size_t o = (size_t)l * A.n + C.t;
-    v[0] = r[o] * z[o];
-    v[1] = first ? 0.0 : z[o] * q[o];
-  }
-  block_partials<2>(v, sh, partials + (size_t)l * 2 * A.nblk, A.nblk, C.b);
 }
 
 // p = z + beta p_old at the five stencil points, q = A p, partial p.q
@@ -622,7 +599,7 @@ int wm_k_bc_phi(wm_ctx *ctx, double *phi) {
 
 size_t wm_cg_partials(int nx, int ny) {
   const size_t n = (size_t)nx * ny;
-  return 3 * 2 * (size_t)wm_cdiv((long long)n, CGB);
+  return 3 * 2 * std::max((size_t)wm_cdiv((long long)n, CGB), (size_t)wm_pc_dot_tiles(nx, ny));
 }
 
 // Flexible CG with the learned preconditioner from the warm start in df;
@@ -650,16 +627,18 @@ static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
   hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 0, ctx->red, S, ctx->pc_sc,
                      ctx->fcg_max);
   WM_HIP(hipGetLastError());
+  // r.z and z.q_prev come out of the preconditioner's last kernel, one
+  // partial pair per 16x16 tile
+  CgArgs AT = A;
+  AT.nblk = wm_pc_dot_tiles(A.nx, A.ny);
   FcgScal hs[3];
   int cur = 0;
   for (;;) {
     WM_HIP(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
     WM_HIP(hipStreamSynchronize(st));
     if (!hs[0].active && !hs[1].active && !hs[2].active) break;
-    WM_TRY(wm_pc_apply_dev(ctx, ctx->cg_r, ctx->pc_sc, ctx->cg_z));
-    hipLaunchKernelGGL(k_fcg_dots, grid, dim3(CGB), 0, st, A, ctx->cg_z, ctx->cg_r, ctx->cg_q,
-                       ctx->red, S);
-    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 1, ctx->red, S, ctx->pc_sc,
+    WM_TRY(wm_pc_apply_dev(ctx, ctx->cg_r, ctx->pc_sc, ctx->cg_z, ctx->cg_q, ctx->red, S));
+    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, AT, 1, ctx->red, S, ctx->pc_sc,
                        ctx->fcg_max);
     hipLaunchKernelGGL(k_fcg_dir, grid, dim3(CGB), 0, st, A, ctx->cg_z, pbuf[cur], pbuf[1 - cur],
                        ctx->cg_q, ctx->red, S);
@@ -724,16 +703,22 @@ int wm_k_field_solve(wm_ctx *ctx) {
       wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
       return WM_ERR_CG_ITEMAX;
     }
-    if (collect && all) WM_TRY(wm_pc_snapshot(ctx, iters));
-    ++iters;
-    hipLaunchKernelGGL(k_cg_pap, grid, dim3(CGB), 0, ctx->stream, A, ctx->cg_r, pbuf[cur],
-                       pbuf[1 - cur], ctx->red, ctx->cg_s);
-    cur = 1 - cur;
-    hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 1, ctx->red, ctx->cg_s);
-    hipLaunchKernelGGL(k_cg_update, grid, dim3(CGB), 0, ctx->stream, A, ctx->df, ctx->cg_r,
-                       pbuf[cur], ctx->red, ctx->cg_s);
-    hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 2, ctx->red, ctx->cg_s);
-    WM_HIP(hipGetLastError());
+    // several iterations per host check: a component that has stopped skips
+    // every kernel (its scalars say inactive), so the extra launches change
+    // nothing; while pairs are recorded, one iteration at a time
+    const int batch = collect ? 1 : 3;
+    for (int bi = 0; bi < batch; ++bi) {
+      if (collect && all) WM_TRY(wm_pc_snapshot(ctx, iters));
+      ++iters;
+      hipLaunchKernelGGL(k_cg_pap, grid, dim3(CGB), 0, ctx->stream, A, ctx->cg_r, pbuf[cur],
+                         pbuf[1 - cur], ctx->red, ctx->cg_s);
+      cur = 1 - cur;
+      hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 1, ctx->red, ctx->cg_s);
+      hipLaunchKernelGGL(k_cg_update, grid, dim3(CGB), 0, ctx->stream, A, ctx->df, ctx->cg_r,
+                         pbuf[cur], ctx->red, ctx->cg_s);
+      hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 2, ctx->red, ctx->cg_s);
+      WM_HIP(hipGetLastError());
+    }
   }
   if (collect) WM_TRY(wm_pc_snapshot_fin(ctx));
   for (int l = 0; l < 3; ++l) ctx->last_iters[l] = fcg_it[l] + hs[l].ite;

@@ -75,6 +75,12 @@ struct CgScal {
   int active, ite, err, pad;
 };
 
+// Flexible-CG per-component scalar state (device).
+struct FcgScal {
+  double eps, sumr, rz, rz_prev, zq, pq, alpha, beta;
+  int active, it, bad, itmax;
+};
+
 using PBank = Rec64;
 
 struct wm_pc;   // learned preconditioner (wm_pc.hip)
@@ -204,7 +210,11 @@ int wm_pc_snapshot(wm_ctx *ctx, int k);
 int wm_pc_snapshot_fin(wm_ctx *ctx);
 int wm_pc_pairs(const wm_ctx *ctx);
 int wm_pc_train_dev(wm_ctx *ctx, int steps, double lr, uint64_t seed, double *loss_out);
-int wm_pc_apply_dev(wm_ctx *ctx, const double *r, const double *sc, double *z);
+// z = M(r) inside flexible CG; also leaves per-tile partials of r.z and
+// z.q (q ignored on a component's first iteration) in dots[l][2][tiles]
+int wm_pc_apply_dev(wm_ctx *ctx, const double *r, const double *sc, double *z, const double *q,
+                    double *dots, const FcgScal *S);
+int wm_pc_dot_tiles(int nx, int ny);
 bool wm_pc_ready(const wm_ctx *ctx);
 void wm_pc_set_stats(wm_ctx *ctx, const int *fcg, const int *fb);
 void wm_pc_get_stats(const wm_ctx *ctx, int *fcg, int *fb);

@@ -3,7 +3,7 @@
 //
 // Same decomposition as the FP64 kernel: one workgroup per (row, chunk of
 // CHUNK cells), a group of GROUP lanes per cell striding through the cell's
-// particles in windows aligned to 64-byte sectors (16 floats).  The push and
+// particles in windows of GROUP consecutive particles.  The push and
 // the per-lane deposit run in FP32; the group's stencils are summed in FP32
 // and then converted, so the workgroup tile, the cross-tile gather and the
 // field solve accumulate in FP64.
@@ -41,20 +41,41 @@ struct Push32 {
   float ox, oy, ux, uy, uz;   // ox unnormalised (relative to the old cell)
 };
 
+// The TSC gather of particle.f90:107-129 inside one cell is a biquadratic
+// in the offsets: with w-(o) = 1/2 - o + o^2/2, w0(o) = 1/2 + o - o^2,
+// w+(o) = o^2/2, sum_ab T(a,b) w_a(ox) w_b(oy) = sum_ij C_ij ox^i oy^j.
+// The workgroup computes C (in FP64, rounded once) for each of its cells and
+// fields; a particle then needs 8 FMAs per field.
+constexpr int CFS = 56;               // floats per cell: 6 fields x 9, padded to 16 B
+__device__ inline void cell_coeffs(float *cf, const float *T, int c, int k) {
+  const double M[3][3] = {{0.5, -1.0, 0.5}, {0.5, 1.0, -1.0}, {0.0, 0.0, 0.5}};
+  const float *Tk = T + k * 3 * TW + c;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double v = 0.0;
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) v += (double)Tk[b * TW + a] * M[a][i] * M[b][j];
+      cf[k * 9 + i * 3 + j] = (float)v;
+    }
+}
+
 // particle.f90:96-161 in FP32 on the cell frame.
-__device__ inline Push32 boris32(const float *T, int gcol, float ox, float oy, float ux,
-                                 float uy, float uz, const PP32 &P, float fac1, float txxx,
-                                 float fac2) {
-  float sx0, sx1, sx2, sy0, sy1, sy2;
-  shapef(ox - 0.5f, sx0, sx1, sx2);
-  shapef(oy - 0.5f, sy0, sy1, sy2);
+__device__ inline Push32 boris32(const float4 *cf4, float ox, float oy, float ux, float uy,
+                                 float uz, const PP32 &P, float fac1, float txxx, float fac2) {
+  float c[CFS];
+#pragma unroll
+  for (int q = 0; q < CFS / 4; ++q) {
+    const float4 v = cf4[q];
+    c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w;
+  }
   float f[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float *Tk = T + k * 3 * TW + gcol;
-    f[k] = (Tk[0] * sx0 + Tk[1] * sx1 + Tk[2] * sx2) * sy0 +
-           (Tk[TW] * sx0 + Tk[TW + 1] * sx1 + Tk[TW + 2] * sx2) * sy1 +
-           (Tk[2 * TW] * sx0 + Tk[2 * TW + 1] * sx1 + Tk[2 * TW + 2] * sx2) * sy2;
+    const float *C = c + 9 * k;
+    const float r0 = C[0] + oy * (C[1] + oy * C[2]);
+    const float r1 = C[3] + oy * (C[4] + oy * C[5]);
+    const float r2 = C[6] + oy * (C[7] + oy * C[8]);
+    f[k] = r0 + ox * (r1 + ox * r2);
   }
   const float bpx = f[0], bpy = f[1], bpz = f[2];
   const float epx = f[3], epy = f[4], epz = f[5];
@@ -246,6 +267,7 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
     PP32 P, const int *__restrict__ cumcnt, const float *__restrict__ tmp, int TX, size_t tps,
     Rec32 A, Rec32 B, double *__restrict__ part, int *__restrict__ bcnt, int *__restrict__ derr) {
   __shared__ float T[6 * 3 * TW];
+  __shared__ float4 CF4[CHUNK * CFS / 4];
   __shared__ double cellbuf[CHUNK * NACC];
   __shared__ int xl[CHUNK * XCAP];
   // destination counts of this workgroup's particles: [species][dy + 1][cell i0 - 1 + c]
@@ -260,6 +282,13 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
   }
   for (int e = threadIdx.x; e < CHUNK * NACC; e += blockDim.x) cellbuf[e] = 0.0;
   for (int e = threadIdx.x; e < P.nsp * 3 * (CHUNK + 2); e += blockDim.x) lc[e] = 0;
+  __syncthreads();
+  if (threadIdx.x < CHUNK * 6) {
+    const int c = threadIdx.x / 6, k = threadIdx.x % 6;
+    float *cf = (float *)CF4 + c * CFS;
+    cell_coeffs(cf, T, c, k);
+    if (k == 0) cf[54] = cf[55] = 0.0f;
+  }
   __syncthreads();
   const int g = threadIdx.x / GROUP;
   const int sub = threadIdx.x % GROUP;
@@ -277,44 +306,49 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
       const int *cc = cumcnt + seg * (size_t)(P.nx + 1);
       const int kb = max(0, cc[il]), ke = min(P.cap, cc[il + 1]);
       const size_t base = seg * (size_t)P.cap;
+      // segment bases: the same for the whole workgroup, 32-bit indices below
+      const float *Aox = A.ox + base, *Aoy = A.oy + base, *Aux = A.ux + base;
+      const float *Auy = A.uy + base, *Auz = A.uz + base;
+      int *Bci = B.ci + base;
+      float *Box = B.ox + base, *Boy = B.oy + base, *Bux = B.ux + base, *Buy = B.uy + base;
+      float *Buz = B.uz + base;
       const float q = P.q[s], r = P.r[s];
       const float fac1 = q / r * 0.5f * P.delt;
       const float txxx = fac1 * fac1;
       const float fac2 = q * P.delt / r;
       const float qc = q * P.qc_scale;
       int xcnt = 0;                                  // cell-crossing particles seen
-      const int ka = kb & ~(GROUP - 1);
+      const int ka = kb;                             // windows start at the cell's first particle
       float nox = 0, noy = 0, nux = 0, nuy = 0, nuz = 0;
       if (ka + sub >= kb && ka + sub < ke) {
-        const size_t p = base + (size_t)(ka + sub);
-        nox = A.ox[p]; noy = A.oy[p]; nux = A.ux[p]; nuy = A.uy[p]; nuz = A.uz[p];
+        const int p = ka + sub;
+        nox = Aox[p]; noy = Aoy[p]; nux = Aux[p]; nuy = Auy[p]; nuz = Auz[p];
       }
       // the window loop is uniform over the group, so every lane keeps the
       // same crossing count
       for (int w = ka; w < ke; w += GROUP) {
         const int k = w + sub;
-        const size_t p = base + (size_t)k;
         const float ox0 = nox, oy0 = noy, cux = nux, cuy = nuy, cuz = nuz;
         if (k + GROUP < ke) {
-          const size_t pn = p + GROUP;
-          nox = A.ox[pn]; noy = A.oy[pn]; nux = A.ux[pn]; nuy = A.uy[pn]; nuz = A.uz[pn];
+          const int pn = k + GROUP;
+          nox = Aox[pn]; noy = Aoy[pn]; nux = Aux[pn]; nuy = Auy[pn]; nuz = Auz[pn];
         }
         const bool on = k >= kb && k < ke;
         bool cross = false;
         if (on) {
           __asm__ volatile("" ::: "memory");
-          const Push32 o = boris32(T, g, ox0, oy0, cux, cuy, cuz, P, fac1, txxx, fac2);
+          const Push32 o = boris32(CF4 + g * (CFS / 4), ox0, oy0, cux, cuy, cuz, P, fac1, txxx, fac2);
           float ox1, oy1;
           const int dx = rebase(o.ox, ox1);
           const int dy = rebase(o.oy, oy1);
           int cn = il + dx;
           cn = cn < 0 ? cn + P.nx : (cn >= P.nx ? cn - P.nx : cn);   // periodic x
-          B.ci[p] = cn;
-          B.ox[p] = ox1;
-          B.oy[p] = o.oy;                                 // rebased by the row migration
-          B.ux[p] = o.ux;
-          B.uy[p] = o.uy;
-          B.uz[p] = o.uz;
+          Bci[k] = cn;
+          Box[k] = ox1;
+          Boy[k] = o.oy;                                  // rebased by the row migration
+          Bux[k] = o.ux;
+          Buy[k] = o.uy;
+          Buz[k] = o.uz;
           ok = ok && fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f;
           cross = (dx | dy) != 0;
           if (bcnt) atomicAdd(&lc[(s * 3 + dy + 1) * (CHUNK + 2) + g + dx + 1], 1);

@@ -62,10 +62,14 @@ __device__ inline int pmod(int a, int n) {
 // (N = 16 pixels of a row), K = 32 (tap, input channel) pairs.  Lane
 // (n = lane & 15, g = lane >> 4) feeds A row n and B column n with the 8 K
 // values of slot g; the accumulator holds D[4g + j][n], j = 0..3.
-template <int CIN, int COUT, bool OUTF32>
+//
+// UP: the input is u = BF16(a + up(bc)) (nearest upsampling of the coarser
+// level bc), formed while the tile is staged; uout (optional) receives u.
+template <int CIN, int COUT, bool OUTF32, bool UP = false>
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
                                               const bf16 *__restrict__ w, void *outp, int H,
-                                              int W) {
+                                              int W, const bf16 *__restrict__ inc = nullptr,
+                                              bf16 *__restrict__ uout = nullptr) {
   static_assert(CIN == 16 || CIN == 32, "CIN");
   static_assert(COUT % 16 == 0, "COUT");
   __shared__ __attribute__((aligned(16))) bf16 sin[(T + 2) * (T + 2) * CIN];
@@ -77,7 +81,21 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
   for (int e = threadIdx.x; e < (T + 2) * (T + 2) * CH; e += 256) {
     const int c = e % CH, px = e / CH;
     const int gy = pmod(y0 + px / (T + 2) - 1, H), gx = pmod(x0 + px % (T + 2) - 1, W);
-    *(uint4 *)&sin[px * CIN + c * 8] = *(const uint4 *)&inb[((size_t)gy * W + gx) * CIN + c * 8];
+    const size_t src = ((size_t)gy * W + gx) * CIN + c * 8;
+    if (UP) {
+      const bf16 *cb = inc + (size_t)b * (H / 2) * (W / 2) * CIN;
+      bf16x8 va, vb, vu;
+      __builtin_memcpy(&va, &inb[src], 16);
+      __builtin_memcpy(&vb, &cb[((size_t)(gy >> 1) * (W / 2) + (gx >> 1)) * CIN + c * 8], 16);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) vu[k] = (bf16)((float)va[k] + (float)vb[k]);
+      __builtin_memcpy(&sin[px * CIN + c * 8], &vu, 16);
+      const int ty = px / (T + 2) - 1, tx = px % (T + 2) - 1;
+      if (uout && ty >= 0 && ty < T && tx >= 0 && tx < T && y0 + ty < H && x0 + tx < W)
+        __builtin_memcpy(&uout[(size_t)b * H * W * CIN + src], &vu, 16);
+    } else {
+      *(uint4 *)&sin[px * CIN + c * 8] = *(const uint4 *)&inb[src];
+    }
   }
   for (int e = threadIdx.x; e < 9 * COUT * CH; e += 256)
     *(uint4 *)&sw[e * 8] = *(const uint4 *)&w[e * 8];
@@ -127,6 +145,17 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
   }
 }
 
+// Fixed-order workgroup sum of one double; result in thread 0.
+__device__ inline double block_sum1(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
 // ------------------------------------------------------- fine-level kernels
 // x0 tile with a halo of HALO pixels, (float)(r * inv_s), periodic.
 template <int HALO>
@@ -154,24 +183,61 @@ __device__ inline void lift_at(float *a, const float *xs, const float *__restric
   }
 }
 
-// d1 = pool(lift(r / s)), one 16x16 fine tile -> 8x8 level-1 pixels.
+// Folded fine-level weights (k_fold), FP32:
+//   FK[5][5]            lift then proj as one 5x5 stencil on x0
+//   FP[py][px][a][b][c] proj applied to up(b1): fine pixels of parity
+//                       (py, px) see coarse pixels at offsets a - 1 + py,
+//                       b - 1 + px (a, b in {0, 1})
+//   FL[r][c][co]        pool(lift(x0)) as one 4x4 stencil per channel on the
+//                       x0 rows 2Y-1..2Y+2, columns 2X-1..2X+2
+constexpr int FK_OFF = 0, FP_OFF = 32, FL_OFF = 32 + 256, NFOLD = 32 + 256 + 256;
+
+__global__ void k_fold(const float *__restrict__ wl, const float *__restrict__ wp,
+                       float *__restrict__ f) {
+  const int t = threadIdx.x;
+  if (t < 25) {
+    const int ky = t / 5 - 2, kx = t % 5 - 2;
+    double v = 0.0;
+    for (int tp = 0; tp < 9; ++tp)
+      for (int tl = 0; tl < 9; ++tl)
+        if (tp / 3 - 1 + tl / 3 - 1 == ky && tp % 3 - 1 + tl % 3 - 1 == kx)
+          for (int ci = 0; ci < C0; ++ci) v += (double)wp[tp * C0 + ci] * wl[tl * C0 + ci];
+    f[FK_OFF + t] = (float)v;
+  }
+  for (int e = t; e < 256; e += blockDim.x) {
+    // FP: e = (((py * 2 + px) * 2 + a) * 2 + b) * C0 + ci
+    const int ci = e % C0, bb = (e / C0) % 2, a = (e / (2 * C0)) % 2, px = (e / (4 * C0)) % 2,
+              py = e / (8 * C0);
+    float v = 0.0f;
+    for (int tp = 0; tp < 9; ++tp) {
+      const int dy = tp / 3 - 1, dx = tp % 3 - 1;
+      const int cy = (py + dy) >> 1, cx = (px + dx) >> 1;   // floor
+      if (cy == a - 1 + py && cx == bb - 1 + px) v += wp[tp * C0 + ci];
+    }
+    f[FP_OFF + e] = v;
+    // FL: e = (r * 4 + c) * C0 + co
+    const int co = e % C0, c = (e / C0) % 4, r = e / (4 * C0);
+    double w = 0.0;
+    for (int chy = 0; chy < 2; ++chy)
+      for (int chx = 0; chx < 2; ++chx)
+        for (int tl = 0; tl < 9; ++tl)
+          if (chy + tl / 3 == r && chx + tl % 3 == c) w += wl[tl * C0 + co];
+    f[FL_OFF + e] = (float)(0.25 * w);
+  }
+}
+
+// d1 = pool(lift(r / s)) with the folded 4x4 stencils, one 16x16 fine tile
+// -> 8x8 level-1 pixels.
 __global__ __launch_bounds__(256) void k_lift_pool(const double *__restrict__ r, size_t rstride,
                                                    const double *__restrict__ sc,
-                                                   const float *__restrict__ wl,
+                                                   const float *__restrict__ fold,
                                                    bf16 *__restrict__ d1, int H, int W, int B) {
   __shared__ float xs[(T + 2) * (T + 2)];
-  __shared__ float as[T * T * C0];
+  __shared__ float fl[16 * C0];
   const int b = blockIdx.z;
   const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
   load_x0<1>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
-  __syncthreads();
-  {
-    const int ty = threadIdx.x / T, tx = threadIdx.x % T;
-    float a[C0];
-    lift_at<T + 2>(a, xs, wl, ty + 1, tx + 1);
-#pragma unroll
-    for (int co = 0; co < C0; ++co) as[threadIdx.x * C0 + co] = a[co];
-  }
+  for (int e = threadIdx.x; e < 16 * C0; e += 256) fl[e] = fold[FL_OFF + e];
   __syncthreads();
   const int H1 = H / 2, W1 = W / 2;
   for (int e = threadIdx.x; e < (T / 2) * (T / 2) * C0; e += 256) {
@@ -179,9 +245,12 @@ __global__ __launch_bounds__(256) void k_lift_pool(const double *__restrict__ r,
     const int qy = q / (T / 2), qx = q % (T / 2);
     const int Y = y0 / 2 + qy, X = x0 / 2 + qx;
     if (Y >= H1 || X >= W1) continue;
-    const int p = (2 * qy) * T + 2 * qx;
-    const float v = ((as[p * C0 + co] + as[(p + 1) * C0 + co]) +
-                     (as[(p + T) * C0 + co] + as[(p + T + 1) * C0 + co])) * 0.25f;
+    float v = 0.0f;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+        v = fmaf(fl[(rr * 4 + cc) * C0 + co], xs[(2 * qy + rr) * (T + 2) + 2 * qx + cc], v);
     d1[(((size_t)b * H1 + Y) * W1 + X) * C0 + co] = (bf16)v;
   }
 }
@@ -201,35 +270,75 @@ __device__ inline void build_u0(float *us, const float *xs, const float *__restr
   }
 }
 
-// z = s * proj(lift(r / s) + up(b1)); y = proj(...) as well when y != null.
+// z = s * proj(lift(r / s) + up(b1)) with the folded weights:
+// y = FK * x0 + sum over the 2x2 coarse neighbours of FP[parity] . b1;
+// y itself is stored as well when y != null.
+//
+// With dots != null (inside flexible CG) the kernel also leaves this tile's
+// partial sums of r.z and z.q in dots[(b * 2 + k) * tiles + tile], k = 0, 1
+// (z.q = 0 on the component's first iteration), fixed-order tree sums.
 __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, size_t rstride,
                                                  const double *__restrict__ sc,
-                                                 const float *__restrict__ wl,
-                                                 const float *__restrict__ wp,
+                                                 const float *__restrict__ fold,
                                                  const bf16 *__restrict__ b1,
                                                  double *__restrict__ z, size_t zstride,
-                                                 float *__restrict__ y, int H, int W, int B) {
+                                                 float *__restrict__ y, int H, int W, int B,
+                                                 const double *__restrict__ q,
+                                                 double *__restrict__ dots,
+                                                 const FcgScal *__restrict__ S) {
+  constexpr int CE = T / 2 + 2;                    // coarse tile edge (halo 1)
   __shared__ float xs[(T + 4) * (T + 4)];
-  __shared__ float us[(T + 2) * (T + 2) * C0];
+  __shared__ float bs[CE * CE * C0];
+  __shared__ float fk[25], fp[256];
+  __shared__ double sh[256];
   const int b = blockIdx.z;
   const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
   load_x0<2>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
-  __syncthreads();
-  build_u0(us, xs, wl, b1 + (size_t)b * (H / 2) * (W / 2) * C0, x0, y0, H, W);
+  const int H1 = H / 2, W1 = W / 2;
+  const bf16 *b1b = b1 + (size_t)b * H1 * W1 * C0;
+  for (int e = threadIdx.x; e < CE * CE * C0; e += 256) {
+    const int c = e % C0, q = e / C0;
+    const int Y = pmod(y0 / 2 + q / CE - 1, H1), X = pmod(x0 / 2 + q % CE - 1, W1);
+    bs[e] = (float)b1b[((size_t)Y * W1 + X) * C0 + c];
+  }
+  for (int e = threadIdx.x; e < 256; e += 256) fp[e] = fold[FP_OFF + e];
+  if (threadIdx.x < 25) fk[threadIdx.x] = fold[FK_OFF + threadIdx.x];
   __syncthreads();
   const int ty = threadIdx.x / T, tx = threadIdx.x % T;
   const int gy = y0 + ty, gx = x0 + tx;
-  if (gy >= H || gx >= W) return;
+  const bool in = gy < H && gx < W;
   float v = 0.0f;
 #pragma unroll
-  for (int tap = 0; tap < 9; ++tap) {
-    const float *u = us + ((ty + tap / 3) * (T + 2) + (tx + tap % 3)) * C0;
+  for (int k = 0; k < 25; ++k) v = fmaf(fk[k], xs[(ty + k / 5) * (T + 4) + tx + k % 5], v);
+  const int py = ty & 1, px = tx & 1;
+  const float *fpp = fp + (py * 2 + px) * 4 * C0;
 #pragma unroll
-    for (int ci = 0; ci < C0; ++ci) v = fmaf(u[ci], wp[tap * C0 + ci], v);
-  }
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) {
+      // coarse pixel (ty>>1) + a - 1 + py in the tile frame, +1 for the halo
+      const float *bp = bs + ((ty / 2 + a + py) * CE + (tx / 2 + bb + px)) * C0;
+      const float *wv = fpp + (a * 2 + bb) * C0;
+#pragma unroll
+      for (int ci = 0; ci < C0; ++ci) v = fmaf(wv[ci], bp[ci], v);
+    }
   const size_t p = (size_t)gy * W + gx;
-  z[(size_t)b * zstride + p] = (double)v * sc[b];
-  if (y) y[(size_t)b * H * W + p] = v;
+  const double zv = (double)v * sc[b];
+  if (in) {
+    z[(size_t)b * zstride + p] = zv;
+    if (y) y[(size_t)b * H * W + p] = v;
+  }
+  if (dots) {
+    const bool first = S[b].it == 0;
+    const double zr = in ? zv * r[(size_t)b * rstride + p] : 0.0;
+    const double zq = (in && !first) ? zv * q[(size_t)b * rstride + p] : 0.0;
+    const int tiles = gridDim.x * gridDim.y, tile = blockIdx.y * gridDim.x + blockIdx.x;
+    const double t0 = block_sum1(zr, sh);
+    if (threadIdx.x == 0) dots[(size_t)(b * 2 + 0) * tiles + tile] = t0;
+    __syncthreads();
+    const double t1 = block_sum1(zq, sh);
+    if (threadIdx.x == 0) dots[(size_t)(b * 2 + 1) * tiles + tile] = t1;
+  }
 }
 
 // ------------------------------------------------------------ coarse levels
@@ -246,20 +355,6 @@ __global__ void k_pool(const bf16 *__restrict__ in, bf16 *__restrict__ out, int 
     const bf16 *p = in + (((size_t)b * H + 2 * Y) * W + 2 * X) * C + c;
     const float v = (((float)p[0] + (float)p[C]) + ((float)p[(size_t)W * C] + (float)p[(size_t)W * C + C])) * 0.25f;
     out[e] = (bf16)v;
-  }
-}
-
-// u = a + up(bc) (nearest), rounded to BF16.
-__global__ void k_upadd(const bf16 *__restrict__ a, const bf16 *__restrict__ bc,
-                        bf16 *__restrict__ u, int B, int H, int W, int C) {
-  const size_t n = (size_t)B * H * W * C;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int c = e % C;
-    const size_t q = e / C;
-    const int X = q % W, Y = (q / W) % H, b = q / ((size_t)W * H);
-    const float v = (float)a[e] + (float)bc[(((size_t)b * (H / 2) + Y / 2) * (W / 2) + X / 2) * C + c];
-    u[e] = (bf16)v;
   }
 }
 
@@ -292,17 +387,6 @@ __global__ void k_add_unpool(const float *__restrict__ du, const float *__restri
     const float v = du[e] + 0.25f * dd[(((size_t)b * (H / 2) + Y / 2) * (W / 2) + X / 2) * C + c];
     out[e] = (bf16)v;
   }
-}
-
-// Fixed-order workgroup sum of one double; result in thread 0.
-__device__ inline double block_sum1(double v, double *sh) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  return sh[0];
 }
 
 // Backward of y = proj(u0) for the loss mean((y - e/s)^2):
@@ -578,6 +662,7 @@ struct wm_pc {
   static constexpr int B = 3;
   float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
   bf16 *wb = nullptr, *wt = nullptr;
+  float *fold = nullptr;         // folded fine-level weights (k_fold)
   int adam_t = 0;
   bool trained = false;
   // activations (BF16, NHWC)
@@ -626,6 +711,15 @@ int conv(wm_ctx *ctx, const bf16 *in, const bf16 *w, void *out, int H, int W) {
   return WM_OK;
 }
 
+template <int CIN, int COUT>
+int conv_up(wm_ctx *ctx, const bf16 *a, const bf16 *bc, bf16 *uout, const bf16 *w, bf16 *out,
+            int H, int W) {
+  hipLaunchKernelGGL((k_conv<CIN, COUT, false, true>), tiles(H, W, wm_pc::B), dim3(256), 0,
+                     ctx->stream, a, w, (void *)out, H, W, bc, uout);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
 // Weight gradient of layer l into grad (partials then fixed-order rows).
 template <int CIN, int COUT>
 int conv_dw(wm_ctx *ctx, wm_pc *pc, int l, const bf16 *X, const bf16 *G, int H, int W) {
@@ -643,13 +737,14 @@ int conv_dw(wm_ctx *ctx, wm_pc *pc, int l, const bf16 *X, const bf16 *G, int H, 
 // Forward pass over three images r (stride rstride) with scales sc (device,
 // s[3] then 1/s[3]); z gets s * y (stride zstride), y (optional) the net output.
 int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const double *sc, double *z,
-            size_t zstride, float *y) {
+            size_t zstride, float *y, const double *q = nullptr, double *dots = nullptr,
+            const FcgScal *S = nullptr) {
   const int H = pc->H, W = pc->W, B = wm_pc::B;
   const float *th = pc->theta;
   const bf16 *wb = pc->wb;
   hipStream_t st = ctx->stream;
-  hipLaunchKernelGGL(k_lift_pool, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc,
-                     th + loff(L_LIFT), pc->d1, H, W, B);
+  hipLaunchKernelGGL(k_lift_pool, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
+                     pc->d1, H, W, B);
   WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1)));
   hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 2, C1))), dim3(256), 0, st, pc->a1, pc->d2, B,
                      H >> 1, W >> 1, C1);
@@ -657,14 +752,16 @@ int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const doubl
   hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 3, C2))), dim3(256), 0, st, pc->a2, pc->d3, B,
                      H >> 2, W >> 2, C2);
   WM_TRY((conv<C2, C2, false>(ctx, pc->d3, wb + loff(L_D3), pc->a3, H >> 3, W >> 3)));
-  hipLaunchKernelGGL(k_upadd, dim3(ew_blocks(lvl(pc, 2, C2))), dim3(256), 0, st, pc->a2, pc->a3,
-                     pc->u2, B, H >> 2, W >> 2, C2);
-  WM_TRY((conv<C2, C1, false>(ctx, pc->u2, wb + loff(L_U2), pc->b2, H >> 2, W >> 2)));
-  hipLaunchKernelGGL(k_upadd, dim3(ew_blocks(lvl(pc, 1, C1))), dim3(256), 0, st, pc->a1, pc->b2,
-                     pc->u1, B, H >> 1, W >> 1, C1);
-  WM_TRY((conv<C1, C0, false>(ctx, pc->u1, wb + loff(L_U1), pc->b1, H >> 1, W >> 1)));
-  hipLaunchKernelGGL(k_up_proj, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc,
-                     th + loff(L_LIFT), th + loff(L_PROJ), pc->b1, z, zstride, y, H, W, B);
+  // u2 = a2 + up(a3) and u1 = a1 + up(b2) are formed inside the U2/U1
+  // convolutions; training keeps them for the weight gradients
+  const bool keep = y != nullptr;
+  WM_TRY((conv_up<C2, C1>(ctx, pc->a2, pc->a3, keep ? pc->u2 : nullptr, wb + loff(L_U2), pc->b2,
+                          H >> 2, W >> 2)));
+  WM_TRY((conv_up<C1, C0>(ctx, pc->a1, pc->b2, keep ? pc->u1 : nullptr, wb + loff(L_U1), pc->b1,
+                          H >> 1, W >> 1)));
+  hipLaunchKernelGGL(k_up_proj, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
+                     pc->b1, z, zstride, y, H, W, B, q, dots, S);
+  (void)th;
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -726,6 +823,8 @@ int backward(wm_ctx *ctx, wm_pc *pc, const double *r, const double *e, size_t st
 int refresh_weights(wm_ctx *ctx, wm_pc *pc) {
   hipLaunchKernelGGL(k_weights_bf16, dim3(wm_cdiv(NPARAM, 256)), dim3(256), 0, ctx->stream,
                      pc->theta, pc->wb, pc->wt);
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(256), 0, ctx->stream, pc->theta + loff(L_LIFT),
+                     pc->theta + loff(L_PROJ), pc->fold);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -745,7 +844,7 @@ int wm_pc_nparam() { return NPARAM; }
 void wm_pc_free(wm_ctx *ctx) {
   wm_pc *pc = ctx->pc;
   if (!pc) return;
-  void *ptrs[] = {pc->theta, pc->m,   pc->v,   pc->grad, pc->wb,  pc->wt,  pc->d1,  pc->a1,
+  void *ptrs[] = {pc->theta, pc->m,   pc->v,   pc->grad, pc->wb,  pc->wt,  pc->fold, pc->d1, pc->a1,
                   pc->d2,    pc->a2,  pc->d3,  pc->a3,   pc->u2,  pc->b2,  pc->u1,  pc->b1,
                   pc->y,     pc->du0, pc->du1, pc->du2,  pc->dd1, pc->dd2, pc->dd3, pc->db1,
                   pc->db2,   pc->da3, pc->da2, pc->da1,  pc->zt,  pc->part, pc->lossp,
@@ -777,6 +876,7 @@ int wm_pc_alloc(wm_ctx *ctx) {
   WM_TRY(dalloc(ctx, &pc->grad, NPARAM));
   WM_TRY(dalloc(ctx, &pc->wb, NPARAM));
   WM_TRY(dalloc(ctx, &pc->wt, NPARAM));
+  WM_TRY(dalloc(ctx, &pc->fold, NFOLD));
   WM_TRY(dalloc(ctx, &pc->d1, lvl(pc, 1, C0)));
   WM_TRY(dalloc(ctx, &pc->a1, lvl(pc, 1, C1)));
   WM_TRY(dalloc(ctx, &pc->d2, lvl(pc, 2, C1)));
@@ -1032,11 +1132,14 @@ int wm_pc_train_dev(wm_ctx *ctx, int steps, double lr, uint64_t seed, double *lo
 
 // Preconditioner application inside the solver: z[3][n] = M(r[3][n]) with
 // device scales sc (s[3], 1/s[3]).
-int wm_pc_apply_dev(wm_ctx *ctx, const double *r, const double *sc, double *z) {
+int wm_pc_apply_dev(wm_ctx *ctx, const double *r, const double *sc, double *z, const double *q,
+                    double *dots, const FcgScal *S) {
   wm_pc *pc = ctx->pc;
   const size_t n = (size_t)pc->H * pc->W;
-  return forward(ctx, pc, r, n, sc, z, n, nullptr);
+  return forward(ctx, pc, r, n, sc, z, n, nullptr, q, dots, S);
 }
+
+int wm_pc_dot_tiles(int nx, int ny) { return wm_cdiv(nx, T) * wm_cdiv(ny, T); }
 
 bool wm_pc_ready(const wm_ctx *ctx) { return ctx->pc && ctx->pc->trained; }
 
