@@ -461,7 +461,8 @@ int wm_create(wm_ctx **out, int device) {
     return WM_ERR_NODEV;
   wm_ctx *ctx = new wm_ctx();
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipStreamCreate(&ctx->stream2) != hipSuccess) {
     delete ctx;
     return WM_ERR_NODEV;
   }
@@ -494,6 +495,7 @@ int wm_destroy(wm_ctx *ctx) {
   wm_pc_free(ctx);
   for (int k = 0; k < 16; ++k) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
   return WM_OK;
 }
@@ -788,6 +790,16 @@ int wm_state_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double
   return WM_OK;
 }
 
+namespace {
+// Routes the launches of a scope to another stream of the context.
+struct StreamSwap {
+  wm_ctx *c;
+  hipStream_t saved;
+  StreamSwap(wm_ctx *ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { ctx->stream = s; }
+  ~StreamSwap() { c->stream = saved; }
+};
+}  // namespace
+
 int wm_step(wm_ctx *ctx, int nsteps) {
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->pinit, "particle__init"));
@@ -806,26 +818,40 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     WM_HIP(hipEventRecord(e[1], ctx->stream));
     WM_TRY(f32 ? wm_k_push_deposit32(ctx, 0, 1) : wm_k_push_deposit(ctx, 0, 1));
     WM_HIP(hipEventRecord(e[2], ctx->stream));
+    // The particle boundaries and the sort (app.f90:104-106) read only the
+    // pushed particles, and the field solve (app.f90:103) only the current
+    // and the fields: they run concurrently, the particles on the second
+    // stream, and the next step waits for both.
+    WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
+    {
+      StreamSwap on2(ctx, ctx->stream2);
+      WM_HIP(hipEventRecord(e[5], ctx->stream));
+      if (f32) {
+        // one pass: x wrap done by the push, rows and cells from the push's
+        // destination counts
+        WM_HIP(hipEventRecord(e[6], ctx->stream));
+        WM_TRY(wm_k_bin32(ctx, 1, 0));
+      } else {
+        WM_TRY(wm_k_particle_y(ctx, 1, true));    // app.f90:104-105 (x fused into y pass 1)
+        WM_HIP(hipEventRecord(e[6], ctx->stream));
+        WM_TRY(wm_k_sort(ctx, 1, 0));             // app.f90:106
+      }
+      WM_HIP(hipEventRecord(e[7], ctx->stream));
+    }
     WM_TRY(f32 ? wm_k_deposit_gather32(ctx) : wm_k_deposit_gather(ctx));
     WM_TRY(wm_k_bc_curre(ctx, ctx->uj));          // field.f90:122
     WM_HIP(hipEventRecord(e[3], ctx->stream));
     WM_TRY(wm_k_field_solve(ctx));                // field.f90:125-184
     WM_HIP(hipEventRecord(e[4], ctx->stream));
-    if (f32) {
-      // app.f90:104-106 in one pass: x wrap done by the push, rows and
-      // cells from the push's destination counts
-      WM_HIP(hipEventRecord(e[5], ctx->stream));
-      WM_TRY(wm_k_bin32(ctx, 1, 0));
-    } else {
-      WM_TRY(wm_k_particle_y(ctx, 1, true));      // app.f90:104-105 (x fused into y pass 1)
-      WM_HIP(hipEventRecord(e[5], ctx->stream));
-      WM_TRY(wm_k_sort(ctx, 1, 0));               // app.f90:106
-    }
-    WM_HIP(hipEventRecord(e[6], ctx->stream));
-    WM_HIP(hipEventSynchronize(e[6]));
+    WM_HIP(hipStreamWaitEvent(ctx->stream, e[7], 0));
+    WM_HIP(hipEventRecord(e[8], ctx->stream));
+    WM_HIP(hipEventSynchronize(e[8]));
+    // phases: [0..3] on the first stream (e0..e4), [4] particle boundaries
+    // (e5..e6) and [5] sort (e6..e7) on the second
+    const int from[WM_NPHASE] = {0, 1, 2, 3, 5, 6}, to[WM_NPHASE] = {1, 2, 3, 4, 6, 7};
     for (int k = 0; k < WM_NPHASE; ++k) {
       float ms = 0.f;
-      WM_HIP(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+      WM_HIP(hipEventElapsedTime(&ms, e[from[k]], e[to[k]]));
       acc[k] += ms;
     }
     WM_TRY(wm_check_derr(ctx, "step"));

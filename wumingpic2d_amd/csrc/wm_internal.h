@@ -88,6 +88,7 @@ struct wm_pc;   // learned preconditioner (wm_pc.hip)
 struct wm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // particle boundaries + sort, beside the field solve
   int status = WM_OK;
   char msg[512] = {0};
 

@@ -323,27 +323,26 @@ __global__ __launch_bounds__(256, 3) void k_particle(
       const double fac2 = q * P.delt / r;
       unsigned long long mask = 0ULL;
       // the next particle's state is loaded while the current one is
-      // computed (one step of software pipelining per lane)
-      // windows of GROUP particles aligned to 64-byte sectors (segments start
-      // on a sector): a cell's first and last windows are shared with its
-      // neighbours and the lanes outside [kb, ke) idle there
-      const int ka = kb & ~(GROUP - 1);
+      // computed (one step of software pipelining per lane); windows of
+      // GROUP consecutive particles start at the cell's first particle
+      const int ka = kb;
+      // segment bases: the same for the whole workgroup, 32-bit indices below
+      const double *Xa = xa + base, *Ya = ya + base;
+      const double *Uxa = uxa + base, *Uya = uya + base, *Uza = uza + base;
       int k = ka + sub;
       double nx0 = 0, ny0 = 0, nux = 0, nuy = 0, nuz = 0;
-      if (k >= kb && k < ke) {
-        const size_t p = base + (size_t)k;
-        nx0 = xa[p]; ny0 = ya[p];
-        if (PUSH) { nux = uxa[p]; nuy = uya[p]; nuz = uza[p]; }
+      if (k < ke) {
+        nx0 = Xa[k]; ny0 = Ya[k];
+        if (PUSH) { nux = Uxa[k]; nuy = Uya[k]; nuz = Uza[k]; }
       }
       for (int m = 0; k < ke; k += GROUP, ++m) {
         const size_t p = base + (size_t)k;
         const double x0 = nx0, y0 = ny0, cux = nux, cuy = nuy, cuz = nuz;
         if (k + GROUP < ke) {
-          const size_t pn = p + GROUP;
-          nx0 = xa[pn]; ny0 = ya[pn];
-          if (PUSH) { nux = uxa[pn]; nuy = uya[pn]; nuz = uza[pn]; }
+          const int kn = k + GROUP;
+          nx0 = Xa[kn]; ny0 = Ya[kn];
+          if (PUSH) { nux = Uxa[kn]; nuy = Uya[kn]; nuz = Uza[kn]; }
         }
-        if (k < kb) continue;
         double x1, y1, ux, uy, uz;
         // keep the 54 tile values in LDS instead of hoisting them into
         // registers for the whole cell loop (occupancy)
