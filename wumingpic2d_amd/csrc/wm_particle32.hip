@@ -261,7 +261,38 @@ __device__ inline void crossing_stencils(float *acc, size_t p, int il, const Rec
   stencil32<2>(acc + 40, m, qc, qv);
 }
 
-constexpr int XCAP = 64;        // listed cell-crossing particles per cell and species
+// One component of the stencil of a cell-crossing particle p (cell il):
+// comp 0 -> out[0..19] (Jx), 1 -> out[20..39] (Jy), 2 -> out[40..64] (Jz).
+__device__ inline void crossing_comp(float *out, int comp, size_t p, int il, const Rec32 &A,
+                                     const Rec32 &B, float q, const PP32 &P) {
+  int dx, dy;
+  float oy1;
+  new_frame(B.ci[p], il, P.nx, B.oy[p], dx, dy, oy1);
+  Move32 m;
+  make_move32(m, A.ox[p], A.oy[p], B.ox[p], oy1, dx, dy);
+  const float qc = q * P.qc_scale;
+  float acc[25];
+#pragma unroll
+  for (int v = 0; v < 25; ++v) acc[v] = 0.0f;
+  if (comp == 0) {
+    stencil32<0>(acc, m, qc, 0.0f);
+#pragma unroll
+    for (int v = 0; v < 20; ++v) out[v] = acc[v];
+  } else if (comp == 1) {
+    stencil32<1>(acc, m, qc, 0.0f);
+#pragma unroll
+    for (int v = 0; v < 20; ++v) out[20 + v] = acc[v];
+  } else {
+    const float ux = B.ux[p], uy = B.uy[p], uz = B.uz[p];
+    const float qv = q * (uz * __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz) * P.inv_c2));
+    stencil32<2>(acc, m, qc, qv);
+#pragma unroll
+    for (int v = 0; v < 25; ++v) out[40 + v] = acc[v];
+  }
+}
+
+constexpr int XCAP = 64;        // listed cell-crossing particles per cell (all species)
+constexpr int XR = 64;          // listed crossers staged per round of the workgroup pass
 
 __global__ __launch_bounds__(256) void k_push_deposit32(
     PP32 P, const int *__restrict__ cumcnt, const float *__restrict__ tmp, int TX, size_t tps,
@@ -269,7 +300,9 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
   __shared__ float T[6 * 3 * TW];
   __shared__ float4 CF4[CHUNK * CFS / 4];
   __shared__ double cellbuf[CHUNK * NACC];
-  __shared__ int xl[CHUNK * XCAP];
+  __shared__ int xl[CHUNK * XCAP];                   // (species << 28) | index, per cell
+  __shared__ int pre[CHUNK + 1];                     // prefix of the listed counts
+  __shared__ float stg[XR * NACC];                   // stencils of one round of crossers
   // destination counts of this workgroup's particles: [species][dy + 1][cell i0 - 1 + c]
   __shared__ int lc[WM_MAXSP * 3 * (CHUNK + 2)];
   const int chunk = blockIdx.x;
@@ -300,6 +333,7 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
 #pragma unroll
   for (int k = 0; k < 27; ++k) core[k] = 0.0f;
   bool ok = true;
+  int xcnt = 0;                                      // cell-crossing particles seen (all species)
   if (g < ncell) {
     for (int s = 0; s < P.nsp; ++s) {
       const size_t seg = (size_t)s * P.ny + jl;
@@ -317,7 +351,6 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
       const float txxx = fac1 * fac1;
       const float fac2 = q * P.delt / r;
       const float qc = q * P.qc_scale;
-      int xcnt = 0;                                  // cell-crossing particles seen
       const int ka = kb;                             // windows start at the cell's first particle
       float nox = 0, noy = 0, nux = 0, nuy = 0, nuz = 0;
       if (ka + sub >= kb && ka + sub < ke) {
@@ -351,45 +384,41 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
           Buz[k] = o.uz;
           ok = ok && fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f;
           cross = (dx | dy) != 0;
-          if (bcnt) atomicAdd(&lc[(s * 3 + dy + 1) * (CHUNK + 2) + g + dx + 1], 1);
+          // movers count one by one; the stayers once per group (below)
+          if (bcnt && cross) atomicAdd(&lc[(s * 3 + dy + 1) * (CHUNK + 2) + g + dx + 1], 1);
           if (!cross) {
             const float qv =
                 q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
             core32(core, ox0, oy0, ox1, oy1, qc, qv);
           }
         }
-        // list the group's cell-crossing particles in index order
+        const unsigned sm = (unsigned)(__ballot(on && !cross) >> gshift) & 0xFFFFu;
+        if (bcnt && sub == 0 && sm) atomicAdd(&lc[(s * 3 + 1) * (CHUNK + 2) + g + 1], __popc(sm));
+        // list the group's cell-crossing particles in (species, index) order
         const unsigned gm = (unsigned)(__ballot(cross) >> gshift) & 0xFFFFu;
         if (cross) {
           const int pos = xcnt + __popc(gm & ((1u << sub) - 1u));
-          if (pos < XCAP) xg[pos] = k;
+          if (pos < XCAP) xg[pos] = (s << 28) | k;
         }
         xcnt += __popc(gm);
       }
+    }
+    // stay-in-cell stencils of every species, summed over the group once
 #pragma unroll
-      for (int c = 0; c < 27; ++c) {
-        const float v = row_sum16(core[c]);
-        core[c] = 0.0f;
-        if (sub == 0) cb[core_slot(c)] += (double)v;
-      }
-      // cell-crossing particles: full 5x5 stencils, one particle per lane,
-      // reading back the group's own stores of the new state
+    for (int c = 0; c < 27; ++c) {
+      const float v = row_sum16(core[c]);
+      if (sub == 0) cb[core_slot(c)] += (double)v;
+    }
+    if (xcnt > XCAP) {
+      // more crossers than the list holds: this group does all of its own,
+      // one particle per lane, reading back the group's stores
       __threadfence_block();
-      const int nl = min(xcnt, XCAP);
-      for (int t0 = 0; t0 < nl; t0 += GROUP) {
-        float acc[NACC];
-#pragma unroll
-        for (int v = 0; v < NACC; ++v) acc[v] = 0.0f;
-        if (t0 + sub < nl) crossing_stencils(acc, base + (size_t)xg[t0 + sub], il, A, B, q, P);
-#pragma unroll
-        for (int v = 0; v < NACC; ++v) {
-          const float w = row_sum16(acc[v]);
-          if (sub == 0) cb[v] += (double)w;
-        }
-      }
-      if (xcnt > XCAP) {
-        // more crossers than the list holds: the rest follow the last listed one
-        for (int k0 = xg[XCAP - 1] + 1; k0 < ke; k0 += GROUP) {
+      for (int s = 0; s < P.nsp; ++s) {
+        const size_t seg = (size_t)s * P.ny + jl;
+        const int *cc = cumcnt + seg * (size_t)(P.nx + 1);
+        const int kb = max(0, cc[il]), ke = min(P.cap, cc[il + 1]);
+        const size_t base = seg * (size_t)P.cap;
+        for (int k0 = kb; k0 < ke; k0 += GROUP) {
           float acc[NACC];
 #pragma unroll
           for (int v = 0; v < NACC; ++v) acc[v] = 0.0f;
@@ -399,7 +428,7 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
             int dx, dy;
             float oy1;
             new_frame(B.ci[p], il, P.nx, B.oy[p], dx, dy, oy1);
-            if ((dx | dy) != 0) crossing_stencils(acc, p, il, A, B, q, P);
+            if ((dx | dy) != 0) crossing_stencils(acc, p, il, A, B, P.q[s], P);
           }
 #pragma unroll
           for (int v = 0; v < NACC; ++v) {
@@ -411,7 +440,38 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
     }
   }
   if (!ok) atomicOr(derr, DERR_JUMP);
+  __shared__ int gcnt[CHUNK];
+  if (sub == 0 && g < CHUNK) gcnt[g] = (g < ncell && xcnt <= XCAP) ? xcnt : 0;
+  __syncthreads();                                   // also orders the new state's stores
+  if (threadIdx.x == 0) {
+    pre[0] = 0;
+    for (int c = 0; c < CHUNK; ++c) pre[c + 1] = pre[c] + gcnt[c];
+  }
   __syncthreads();
+  // Listed crossers of the whole workgroup: wave w computes stencil
+  // component w of XR crossers at a time (one crosser per lane), then each
+  // (cell, value) pair adds its cell's crossers in list order.
+  const int nxs = pre[CHUNK];
+  for (int r0 = 0; r0 < nxs; r0 += XR) {
+    const int comp = threadIdx.x / XR, e = r0 + threadIdx.x % XR;
+    if (comp < 3 && e < nxs) {
+      int gg = 0;
+      while (pre[gg + 1] <= e) ++gg;
+      const int ent = xl[gg * XCAP + (e - pre[gg])];
+      const int s = ent >> 28, k = ent & ((1 << 28) - 1);
+      const size_t p = ((size_t)s * P.ny + jl) * P.cap + (size_t)k;
+      crossing_comp(stg + (e - r0) * NACC, comp, p, i0 + gg, A, B, P.q[s], P);
+    }
+    __syncthreads();
+    for (int pr = threadIdx.x; pr < ncell * NACC; pr += blockDim.x) {
+      const int gg = pr / NACC, v = pr % NACC;
+      const int lo = max(pre[gg], r0), hi = min(pre[gg + 1], r0 + XR);
+      double a = cellbuf[pr];
+      for (int e2 = lo; e2 < hi; ++e2) a += (double)stg[(e2 - r0) * NACC + v];
+      cellbuf[pr] = a;
+    }
+    __syncthreads();
+  }
   if (bcnt) {
     // shared edge cells are also counted by the neighbouring chunks: integer
     // atomics keep the totals exact
