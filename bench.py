@@ -36,11 +36,12 @@ WORKLOADS = {
 }
 
 # Algorithmic bytes per particle of the fused push+deposit kernel.  FP64 form:
-# read x, y, ux, uy, uz (40 B) and write the same 40 B.  FP32 form: read the
-# offsets and momenta (20 B; the cell index is implied by the sorted order)
-# and write them with the new cell index (24 B).
+# read x, y, ux, uy, uz (40 B) and write the same 40 B.  FP32 form (cell
+# bins, in place): read the offsets and momenta (20 B; the cell is the bin's)
+# and write them back (20 B); the few particles that change cell go to the
+# mover list instead.
 PUSHDEP_BYTES_PER_PARTICLE = 80.0
-PUSHDEP_BYTES_PER_PARTICLE_FP32 = 44.0
+PUSHDEP_BYTES_PER_PARTICLE_FP32 = 40.0
 
 
 def log(msg: str) -> None:

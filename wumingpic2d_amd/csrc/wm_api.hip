@@ -361,7 +361,32 @@ int wm_alloc_device(wm_ctx *ctx) {
   const int nseg = g.nsp * g.ny;
   const size_t slots = (size_t)nseg * ctx->cap;
   if (ctx->form == WM_FORM_FP32) {
-    WM_TRY(alloc_rec32(ctx, ctx->b32[0], slots));
+    // cell bins with room for 1.5x the row capacity's mean per cell, in
+    // 16-slot (64-byte) units; the dense row bank converts state in and out
+    ctx->CB = std::max(16, (int)((1.5 * ctx->cap / g.nx + 15) / 16) * 16);
+    if (ctx->CB >= (1 << 20)) {
+      wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %d particles per cell bin exceed 2^20", ctx->CB);
+      return WM_ERR_ARG;
+    }
+    const size_t nbin = (size_t)nseg * g.nx, bslots = nbin * ctx->CB;
+    Rec32 &b = ctx->b32[0];
+    WM_TRY(dmalloc(ctx, &b.ox, bslots));
+    WM_TRY(dmalloc(ctx, &b.oy, bslots));
+    WM_TRY(dmalloc(ctx, &b.ux, bslots));
+    WM_TRY(dmalloc(ctx, &b.uy, bslots));
+    WM_TRY(dmalloc(ctx, &b.uz, bslots));
+    WM_TRY(dmalloc(ctx, &ctx->id_bins, bslots));
+    WM_TRY(dmalloc(ctx, &ctx->bin_cnt, nbin));
+    WM_TRY(dmalloc(ctx, &ctx->bin_ncnt, nbin));
+    WM_TRY(dmalloc(ctx, &ctx->bin_mcnt, nbin * 8));
+    WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
+    ctx->nregion = wm_push32_regions(g.nx, g.ny);
+    const size_t ms = (size_t)ctx->nregion * WM_MG;
+    Movers &m = ctx->mv;
+    WM_TRY(dmalloc(ctx, &m.rec, ms));
+    WM_TRY(dmalloc(ctx, &m.n, (size_t)ctx->nregion));
+    WM_TRY(dmalloc(ctx, &ctx->bin_aoff, nbin * 8));
+    WM_HIP(hipMemsetAsync(m.n, 0, sizeof(int) * ctx->nregion, ctx->stream));
     WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
   } else {
     WM_TRY(alloc_bank(ctx, ctx->bank[0], slots));
@@ -376,10 +401,7 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->nhole, nseg));
   WM_TRY(dmalloc(ctx, &ctx->nup, nseg));
   WM_TRY(dmalloc(ctx, &ctx->ndn, nseg));
-  if (ctx->form == WM_FORM_FP32) {
-    WM_TRY(alloc_rec32(ctx, ctx->mv32_up, ms));
-    WM_TRY(alloc_rec32(ctx, ctx->mv32_dn, ms));
-  } else {
+  if (ctx->form == WM_FORM_FP64) {
     WM_TRY(alloc_bank(ctx, ctx->mv_up, ms));
     WM_TRY(alloc_bank(ctx, ctx->mv_dn, ms));
   }
@@ -413,10 +435,6 @@ int wm_alloc_device(wm_ctx *ctx) {
   WM_TRY(dmalloc(ctx, &ctx->diag, (size_t)nseg * 8 + 2 * (size_t)g.ny + 64));
   WM_TRY(dmalloc(ctx, &ctx->derr, 1));
   WM_TRY(dmalloc(ctx, &ctx->cnt_scratch, nseg));
-  if (ctx->form == WM_FORM_FP32) {
-    WM_TRY(dmalloc(ctx, &ctx->bcnt, (size_t)nseg * 3 * g.nx));
-    WM_TRY(dmalloc(ctx, &ctx->cumcnt2, (size_t)nseg * (g.nx + 1)));
-  }
   WM_TRY(dmalloc(ctx, &ctx->fscratch, 6 * fg.ps));
 
   WM_HIP(hipMemsetAsync(ctx->red_cnt, 0, 8 * sizeof(unsigned), ctx->stream));
@@ -481,14 +499,13 @@ int wm_destroy(wm_ctx *ctx) {
   free_bank(ctx->mv_dn);
   free_rec32(ctx->b32[0]);
   free_rec32(ctx->b32[1]);
-  free_rec32(ctx->mv32_up);
-  free_rec32(ctx->mv32_dn);
   if (ctx->tmp32) (void)hipFree(ctx->tmp32);
   void *ptrs[] = {ctx->id[0], ctx->id[1], ctx->np2, ctx->cumcnt, ctx->hole, ctx->nhole,
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
-                  ctx->cnt_scratch, ctx->fscratch, ctx->bcnt, ctx->cumcnt2, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_ncnt, ctx->bin_mcnt,
+                  ctx->bin_aoff, ctx->mv.rec, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -754,11 +771,14 @@ int wm_state_upload(wm_ctx *ctx, const double *up, const int *np2, const int *cu
   WM_TRY(require(ctx, ctx->pinit, "particle__init"));
   WM_TRY(wm_alloc_device(ctx));
   const int nseg = nseg_of(ctx);
-  WM_TRY(upload_particles(ctx, up, np2, 0));
+  const bool f32 = ctx->form == WM_FORM_FP32;
+  WM_TRY(upload_particles(ctx, up, np2, f32 ? 1 : 0));
   WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
   WM_TRY(upload_cumcnt(ctx, cumcnt));
+  if (f32) WM_TRY(wm_k_dense_to_bins(ctx));
   WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
   WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (f32) WM_TRY(wm_check_derr(ctx, "state upload"));
   return WM_OK;
 }
 
@@ -766,6 +786,12 @@ int wm_state_download(wm_ctx *ctx, double *up, int *np2, int *cumcnt, double *uf
   if (!ctx) return WM_ERR_ARG;
   if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
   const int nseg = nseg_of(ctx);
+  const bool f32 = ctx->form == WM_FORM_FP32;
+  // FP32: the bins become dense rows (sorted by cell, as sort.f90 leaves them)
+  if (f32 && (up || np2 || cumcnt)) {
+    WM_TRY(wm_k_bins_to_dense(ctx));
+    WM_TRY(wm_check_derr(ctx, "state download"));
+  }
   std::vector<int> n2(nseg);
   WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
   if (cumcnt)
@@ -773,7 +799,7 @@ int wm_state_download(wm_ctx *ctx, double *up, int *np2, int *cumcnt, double *uf
                           hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
   if (np2) memcpy(np2, n2.data(), sizeof(int) * nseg);
-  if (up) WM_TRY(download_particles(ctx, up, n2.data(), 0));
+  if (up) WM_TRY(download_particles(ctx, up, n2.data(), f32 ? 1 : 0));
   if (uf) WM_TRY(download_field(ctx, uf, 6, ctx->uf));
   return WM_OK;
 }
@@ -784,7 +810,8 @@ int wm_state_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double
   WM_TRY(require(ctx, ctx->sinit, "sort__init"));
   WM_TRY(wm_alloc_device(ctx));
   WM_TRY(wm_k_init_weibel(ctx, seed, ppc, vth, udrift));
-  WM_TRY(wm_k_sort(ctx, 1, 0));
+  // stable by cell within each row: the order of sort__bucket
+  WM_TRY(ctx->form == WM_FORM_FP32 ? wm_k_dense_to_bins(ctx) : wm_k_sort(ctx, 1, 0));
   WM_TRY(fill_zero(ctx, ctx->uf, 6 * ctx->fg.ps));
   WM_TRY(wm_check_derr(ctx, "initial state"));
   return WM_OK;
@@ -813,10 +840,9 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     WM_HIP(hipEventRecord(e[0], ctx->stream));
     // particle__solv + ele_cur (proj/weibel/app.f90:102, field.f90:121)
     const bool f32 = ctx->form == WM_FORM_FP32;
-    if (f32) WM_TRY(wm_k_bin_clear(ctx));
     WM_TRY(f32 ? wm_k_cell_fields32(ctx) : wm_k_cell_fields(ctx));
     WM_HIP(hipEventRecord(e[1], ctx->stream));
-    WM_TRY(f32 ? wm_k_push_deposit32(ctx, 0, 1) : wm_k_push_deposit(ctx, 0, 1));
+    WM_TRY(f32 ? wm_k_push_deposit32(ctx) : wm_k_push_deposit(ctx, 0, 1));
     WM_HIP(hipEventRecord(e[2], ctx->stream));
     // The particle boundaries and the sort (app.f90:104-106) read only the
     // pushed particles, and the field solve (app.f90:103) only the current
@@ -827,10 +853,10 @@ int wm_step(wm_ctx *ctx, int nsteps) {
       StreamSwap on2(ctx, ctx->stream2);
       WM_HIP(hipEventRecord(e[5], ctx->stream));
       if (f32) {
-        // one pass: x wrap done by the push, rows and cells from the push's
-        // destination counts
+        // cell bins: the push already wrapped x and closed the holes; the
+        // particles that changed cell join their destination bins
         WM_HIP(hipEventRecord(e[6], ctx->stream));
-        WM_TRY(wm_k_bin32(ctx, 1, 0));
+        WM_TRY(wm_k_insert_movers(ctx));
       } else {
         WM_TRY(wm_k_particle_y(ctx, 1, true));    // app.f90:104-105 (x fused into y pass 1)
         WM_HIP(hipEventRecord(e[6], ctx->stream));
@@ -882,13 +908,16 @@ int wm_phase_times(wm_ctx *ctx, double *ms, int n) {
 int wm_particle_count(wm_ctx *ctx, long long *count) {
   if (!ctx || !count) return WM_ERR_ARG;
   if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
-  const int nseg = nseg_of(ctx);
-  std::vector<int> n2(nseg);
-  WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
+  // FP32: the bin counts; FP64: np2
+  const bool f32 = ctx->form == WM_FORM_FP32;
+  const size_t n = f32 ? (size_t)nseg_of(ctx) * ctx->g.nx : (size_t)nseg_of(ctx);
+  std::vector<int> c(n);
+  WM_HIP(hipMemcpyAsync(c.data(), f32 ? ctx->bin_cnt : ctx->np2, sizeof(int) * n,
+                        hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
-  long long c = 0;
-  for (int v : n2) c += v;
-  *count = c;
+  long long t = 0;
+  for (int v : c) t += v;
+  *count = t;
   return WM_OK;
 }
 

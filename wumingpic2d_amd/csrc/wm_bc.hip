@@ -89,25 +89,14 @@ __device__ inline int row_change(const Rec64 &R, size_t p, int j, bool fuse_x, c
   const int d = jpos - j;
   return (d >= -1 && d <= 1) ? d : 2;
 }
-// FP32: the row offset oy leaves [0, 1) (x wraps in the push).
-__device__ inline int row_change(const Rec32 &R, size_t p, int, bool, const BP &, int &) {
-  const float oy = R.oy[p];
-  return oy < 0.0f ? (oy < -1.0f ? 2 : -1) : (oy >= 1.0f ? (oy >= 2.0f ? 2 : 1) : 0);
-}
 
-// The record of a leaver as it enters the receiving row: FP64 wraps y across
-// the periodic boundary in round-down arithmetic (:150-154); FP32 rebases oy
-// on the new row.
+// The record of a leaver as it enters the receiving row: y wraps across the
+// periodic boundary in round-down arithmetic (:150-154).
 __device__ inline Rec64::Val arriving(const Rec64 &R, size_t p, int j, int d, const BP &P) {
   Rec64::Val v = R.load(p);
   const int jpos = j + d;
   if (jpos <= P.nygs - 1) v.y = add_rd(v.y, P.spany);
   else if (jpos >= P.nyge + 1) v.y = add_rd(v.y, -P.spany);
-  return v;
-}
-__device__ inline Rec32::Val arriving(const Rec32 &R, size_t p, int, int d, const BP &) {
-  Rec32::Val v = R.load(p);
-  v.oy = d > 0 ? v.oy - 1.0f : wm_below_one(v.oy + 1.0f);
   return v;
 }
 
@@ -274,14 +263,11 @@ int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x) {
   const int nseg = ctx->g.nsp * ctx->g.ny;
   int64_t *id = ctx->id[ctx->idcur];
   if (ctx->form == WM_FORM_FP32) {
-    const Rec32 &B = ctx->b32[bank];
-    hipLaunchKernelGGL(k_particle_y1<Rec32>, dim3(nseg), dim3(YW), 0, ctx->stream, P, false,
-                       ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv32_up,
-                       ctx->mvid_up, ctx->mv32_dn, ctx->mvid_dn, ctx->derr);
-    hipLaunchKernelGGL(k_particle_y2<Rec32>, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2,
-                       B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv32_up,
-                       ctx->mvid_up, ctx->mv32_dn, ctx->mvid_dn, ctx->derr);
-  } else {
+    // the FP32 form migrates particles between its cell bins (wm_sort.hip)
+    wm_set_error(ctx, WM_ERR_ARG, "particle_y works on the FP64 particle form");
+    return WM_ERR_ARG;
+  }
+  {
     const Rec64 &B = ctx->bank[bank];
     hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, fuse_x,
                        ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,

@@ -637,19 +637,22 @@ static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
     WM_HIP(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
     WM_HIP(hipStreamSynchronize(st));
     if (!hs[0].active && !hs[1].active && !hs[2].active) break;
-    WM_TRY(wm_pc_apply_dev(ctx, ctx->cg_r, ctx->pc_sc, ctx->cg_z, ctx->cg_q, ctx->red, S));
-    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, AT, 1, ctx->red, S, ctx->pc_sc,
-                       ctx->fcg_max);
-    hipLaunchKernelGGL(k_fcg_dir, grid, dim3(CGB), 0, st, A, ctx->cg_z, pbuf[cur], pbuf[1 - cur],
-                       ctx->cg_q, ctx->red, S);
-    cur = 1 - cur;
-    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 2, ctx->red, S, ctx->pc_sc,
-                       ctx->fcg_max);
-    hipLaunchKernelGGL(k_fcg_update, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_r, pbuf[cur],
-                       ctx->cg_q, ctx->red, S);
-    hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 3, ctx->red, S, ctx->pc_sc,
-                       ctx->fcg_max);
-    WM_HIP(hipGetLastError());
+    // two iterations per host check; launches after convergence return at once
+    for (int bi = 0; bi < 2; ++bi) {
+      WM_TRY(wm_pc_apply_dev(ctx, ctx->cg_r, ctx->pc_sc, ctx->cg_z, ctx->cg_q, ctx->red, S));
+      hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, AT, 1, ctx->red, S, ctx->pc_sc,
+                         ctx->fcg_max);
+      hipLaunchKernelGGL(k_fcg_dir, grid, dim3(CGB), 0, st, A, ctx->cg_z, pbuf[cur],
+                         pbuf[1 - cur], ctx->cg_q, ctx->red, S);
+      cur = 1 - cur;
+      hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 2, ctx->red, S, ctx->pc_sc,
+                         ctx->fcg_max);
+      hipLaunchKernelGGL(k_fcg_update, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_r, pbuf[cur],
+                         ctx->cg_q, ctx->red, S);
+      hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 3, ctx->red, S, ctx->pc_sc,
+                         ctx->fcg_max);
+      WM_HIP(hipGetLastError());
+    }
   }
   hipLaunchKernelGGL(k_fcg_restore, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_x0, S);
   WM_HIP(hipGetLastError());
@@ -706,7 +709,7 @@ int wm_k_field_solve(wm_ctx *ctx) {
     // several iterations per host check: a component that has stopped skips
     // every kernel (its scalars say inactive), so the extra launches change
     // nothing; while pairs are recorded, one iteration at a time
-    const int batch = collect ? 1 : 3;
+    const int batch = collect ? 1 : 4;
     for (int bi = 0; bi < batch; ++bi) {
       if (collect && all) WM_TRY(wm_pc_snapshot(ctx, iters));
       ++iters;

@@ -81,6 +81,23 @@ struct FcgScal {
   int active, it, bad, itmax;
 };
 
+// FP32 form, cell bins: the movers of one step, structure of arrays, one
+// region of WM_MG slots per cell of every push workgroup.
+#define WM_MG 64
+struct __align__(16) MoverRec {   // 48 B: three 16-byte stores
+  int dst;                     // destination bin
+  int dir;                     // (dy + 1) * 3 + (dx + 1)
+  float ox0, oy0;              // offsets before the push (deposit)
+  float ox1, oy1;              // offsets in the destination cell
+  float ux, uy, uz;
+  int pad;
+  int64_t id;
+};
+struct Movers {
+  MoverRec *rec = nullptr;
+  int *n = nullptr;            // movers per region
+};
+
 using PBank = Rec64;
 
 struct wm_pc;   // learned preconditioner (wm_pc.hip)
@@ -108,7 +125,6 @@ struct wm_ctx {
   int form = WM_FORM_FP64;
   PBank bank[2]{};
   Rec32 b32[2]{};
-  Rec32 mv32_up{}, mv32_dn{};
   float *tmp32 = nullptr;  // cell-centred fields of the FP32 push
   Rec64 b64(int k) const { return bank[k]; }
   int64_t *id[2]{};
@@ -149,8 +165,17 @@ struct wm_ctx {
   int last_iters[3] = {0, 0, 0};
   double phase_ms[8] = {0};
   int *cnt_scratch = nullptr;    // [nseg] counts for layout conversion
-  int *bcnt = nullptr;           // FP32 form: [nseg][3][nx] destination counts, then offsets
-  int *cumcnt2 = nullptr;        // FP32 form: the next step's cumcnt while the old is read
+  // FP32 form: particles live in cell bins of CB slots, bin = seg * nx + i
+  // (bank b32[0], IDs id_bins); b32[1] with id[] is the dense row layout of
+  // the reference, used to convert state in and out
+  int CB = 0;
+  int *bin_cnt = nullptr;        // particles per bin
+  int *bin_ncnt = nullptr;       // per bin after the push: particles that stayed
+  int *bin_mcnt = nullptr;       // per bin and direction (8): particles that left
+  int *bin_aoff = nullptr;       // per bin and direction (8): first slot of the arrivals
+  int64_t *id_bins = nullptr;
+  Movers mv{};
+  int nregion = 0;
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
   long long cg_iters_total = 0;
 
@@ -186,13 +211,15 @@ int wm_k_particle_x(wm_ctx *ctx, int bank);
 int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x);
 // Sort (wm_sort.hip)
 int wm_k_sort(wm_ctx *ctx, int src, int dst);
-// FP32 form: row migration + stable cell sort in one counting scatter, from
-// the destination counts the push records in bcnt
-int wm_k_bin_clear(wm_ctx *ctx);
-int wm_k_bin32(wm_ctx *ctx, int src, int dst);
+// FP32 form, cell bins (wm_sort.hip): dense rows <-> bins, movers into
+// their destination bins, bin counts, row counts
+int wm_k_dense_to_bins(wm_ctx *ctx);
+int wm_k_bins_to_dense(wm_ctx *ctx);
+int wm_k_insert_movers(wm_ctx *ctx);
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);
-int wm_k_push_deposit32(wm_ctx *ctx, int src, int dst);
+int wm_k_push_deposit32(wm_ctx *ctx);   // in place on the bins
+int wm_push32_regions(int nx, int ny);   // mover regions of the push launch
 int wm_k_deposit_gather32(wm_ctx *ctx);
 int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);
 int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk);

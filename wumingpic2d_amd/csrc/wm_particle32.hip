@@ -236,40 +236,25 @@ __device__ inline float row_sum16(float v) {
   return v;
 }
 
-// Cell change of a pushed particle read back from the new bank.
-__device__ inline void new_frame(int ci_new, int il, int nx, float oy_raw, int &dx, int &dy,
-                                 float &oy1) {
-  dx = ci_new - il;
-  if (dx > 1) dx -= nx;          // periodic wrap of the cell index
-  if (dx < -1) dx += nx;
-  dy = rebase(oy_raw, oy1);
+
+
+
+constexpr int XR = 64;          // listed movers staged per round of the workgroup pass
+
+// Bin of (species s, row jl, cell i).
+__device__ inline size_t bin_of(int s, int jl, int i, int ny, int nx) {
+  return ((size_t)s * ny + jl) * nx + i;
 }
 
-// Full stencils (all three components) of one cell-crossing particle p.
-__device__ inline void crossing_stencils(float *acc, size_t p, int il, const Rec32 &A,
-                                         const Rec32 &B, float q, const PP32 &P) {
-  int dx, dy;
-  float oy1;
-  new_frame(B.ci[p], il, P.nx, B.oy[p], dx, dy, oy1);
+// One component of the stencil of a cell-crossing particle from its mover
+// record R: comp 0 -> out[0..19] (Jx), 1 -> out[20..39] (Jy), 2 -> out[40..64]
+// (Jz).
+__device__ inline void mover_comp(float *out, int comp, const Movers &M, size_t R, float q,
+                                  const PP32 &P) {
+  const MoverRec mr = M.rec[R];
+  const int dy = mr.dir / 3 - 1, dx = mr.dir % 3 - 1;
   Move32 m;
-  make_move32(m, A.ox[p], A.oy[p], B.ox[p], oy1, dx, dy);
-  const float ux = B.ux[p], uy = B.uy[p], uz = B.uz[p];
-  const float qv = q * (uz * __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz) * P.inv_c2));
-  const float qc = q * P.qc_scale;
-  stencil32<0>(acc, m, qc, qv);
-  stencil32<1>(acc + 20, m, qc, qv);
-  stencil32<2>(acc + 40, m, qc, qv);
-}
-
-// One component of the stencil of a cell-crossing particle p (cell il):
-// comp 0 -> out[0..19] (Jx), 1 -> out[20..39] (Jy), 2 -> out[40..64] (Jz).
-__device__ inline void crossing_comp(float *out, int comp, size_t p, int il, const Rec32 &A,
-                                     const Rec32 &B, float q, const PP32 &P) {
-  int dx, dy;
-  float oy1;
-  new_frame(B.ci[p], il, P.nx, B.oy[p], dx, dy, oy1);
-  Move32 m;
-  make_move32(m, A.ox[p], A.oy[p], B.ox[p], oy1, dx, dy);
+  make_move32(m, mr.ox0, mr.oy0, mr.ox1, mr.oy1, dx, dy);
   const float qc = q * P.qc_scale;
   float acc[25];
 #pragma unroll
@@ -283,7 +268,7 @@ __device__ inline void crossing_comp(float *out, int comp, size_t p, int il, con
 #pragma unroll
     for (int v = 0; v < 20; ++v) out[20 + v] = acc[v];
   } else {
-    const float ux = B.ux[p], uy = B.uy[p], uz = B.uz[p];
+    const float ux = mr.ux, uy = mr.uy, uz = mr.uz;
     const float qv = q * (uz * __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz) * P.inv_c2));
     stencil32<2>(acc, m, qc, qv);
 #pragma unroll
@@ -291,30 +276,35 @@ __device__ inline void crossing_comp(float *out, int comp, size_t p, int il, con
   }
 }
 
-constexpr int XCAP = 64;        // listed cell-crossing particles per cell (all species)
-constexpr int XR = 64;          // listed crossers staged per round of the workgroup pass
-
-__global__ __launch_bounds__(256) void k_push_deposit32(
-    PP32 P, const int *__restrict__ cumcnt, const float *__restrict__ tmp, int TX, size_t tps,
-    Rec32 A, Rec32 B, double *__restrict__ part, int *__restrict__ bcnt, int *__restrict__ derr) {
+// Push + deposit in place on the cell bins.  A particle that stays in its
+// cell is written back to its slot; one that leaves goes to the cell's mover
+// region (new state, old offsets for the deposit, ID) and leaves a hole.
+// After the deposit of the movers, each bin closes its holes with its last
+// particles (the k-th hole below the new count takes the k-th remaining
+// particle above it), and records how many stayed and how many left in each
+// of the 8 directions; wm_k_insert_movers then appends the movers to their
+// destination bins.
+__global__ __launch_bounds__(256, 4) void k_push_deposit32(
+    PP32 P, int CB, const int *__restrict__ cnt, const float *__restrict__ tmp, int TX,
+    size_t tps, Rec32 A, int64_t *__restrict__ ids, Movers M, int *__restrict__ ncnt,
+    int *__restrict__ mcnt, double *__restrict__ part, int *__restrict__ derr) {
   __shared__ float T[6 * 3 * TW];
   __shared__ float4 CF4[CHUNK * CFS / 4];
   __shared__ double cellbuf[CHUNK * NACC];
-  __shared__ int xl[CHUNK * XCAP];                   // (species << 28) | index, per cell
-  __shared__ int pre[CHUNK + 1];                     // prefix of the listed counts
-  __shared__ float stg[XR * NACC];                   // stencils of one round of crossers
-  // destination counts of this workgroup's particles: [species][dy + 1][cell i0 - 1 + c]
-  __shared__ int lc[WM_MAXSP * 3 * (CHUNK + 2)];
+  __shared__ int xl[CHUNK * WM_MG];                  // species << 28 | dir << 20 | slot
+  __shared__ int pre[CHUNK + 1];                     // prefix of the mover counts
+  __shared__ int gcnt[CHUNK];
+  __shared__ float stg[XR * NACC];                   // stencils of one round of movers
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
   const int ncell = min(CHUNK, P.nx - i0);
+  const size_t rbase = ((size_t)jl * P.nchunk + chunk) * CHUNK;   // first region
   for (int t = threadIdx.x; t < 6 * 3 * TW; t += blockDim.x) {
     const int col = t % TW, b = (t / TW) % 3, k = t / (3 * TW);
     T[t] = col < ncell + 2 ? tmp[(size_t)k * tps + (size_t)(i0 + col) + (size_t)TX * (jl + b)] : 0.0f;
   }
   for (int e = threadIdx.x; e < CHUNK * NACC; e += blockDim.x) cellbuf[e] = 0.0;
-  for (int e = threadIdx.x; e < P.nsp * 3 * (CHUNK + 2); e += blockDim.x) lc[e] = 0;
   __syncthreads();
   if (threadIdx.x < CHUNK * 6) {
     const int c = threadIdx.x / 6, k = threadIdx.x % 6;
@@ -328,77 +318,87 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
   const int gshift = GROUP * (g & 3);                // the group's lanes in the wave ballot
   const int il = i0 + g;                             // this group's cell
   double *cb = cellbuf + g * NACC;
-  int *xg = xl + g * XCAP;
+  int *xg = xl + g * WM_MG;
+  const size_t reg = (rbase + g) * WM_MG;            // this cell's mover region
   float core[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) core[k] = 0.0f;
   bool ok = true;
-  int xcnt = 0;                                      // cell-crossing particles seen (all species)
+  int xcnt = 0;                                      // movers of the cell (all species)
   if (g < ncell) {
     for (int s = 0; s < P.nsp; ++s) {
-      const size_t seg = (size_t)s * P.ny + jl;
-      const int *cc = cumcnt + seg * (size_t)(P.nx + 1);
-      const int kb = max(0, cc[il]), ke = min(P.cap, cc[il + 1]);
-      const size_t base = seg * (size_t)P.cap;
-      // segment bases: the same for the whole workgroup, 32-bit indices below
-      const float *Aox = A.ox + base, *Aoy = A.oy + base, *Aux = A.ux + base;
-      const float *Auy = A.uy + base, *Auz = A.uz + base;
-      int *Bci = B.ci + base;
-      float *Box = B.ox + base, *Boy = B.oy + base, *Bux = B.ux + base, *Buy = B.uy + base;
-      float *Buz = B.uz + base;
+      const size_t bin = bin_of(s, jl, il, P.ny, P.nx);
+      const int n = min(CB, cnt[bin]);
+      const size_t base = bin * (size_t)CB;
+      // bin bases: 32-bit indices below
+      float *Aox = A.ox + base, *Aoy = A.oy + base, *Aux = A.ux + base;
+      float *Auy = A.uy + base, *Auz = A.uz + base;
       const float q = P.q[s], r = P.r[s];
       const float fac1 = q / r * 0.5f * P.delt;
       const float txxx = fac1 * fac1;
       const float fac2 = q * P.delt / r;
       const float qc = q * P.qc_scale;
-      const int ka = kb;                             // windows start at the cell's first particle
       float nox = 0, noy = 0, nux = 0, nuy = 0, nuz = 0;
-      if (ka + sub >= kb && ka + sub < ke) {
-        const int p = ka + sub;
-        nox = Aox[p]; noy = Aoy[p]; nux = Aux[p]; nuy = Auy[p]; nuz = Auz[p];
+      if (sub < n) {
+        nox = Aox[sub]; noy = Aoy[sub]; nux = Aux[sub]; nuy = Auy[sub]; nuz = Auz[sub];
       }
       // the window loop is uniform over the group, so every lane keeps the
-      // same crossing count
-      for (int w = ka; w < ke; w += GROUP) {
+      // same mover count
+      for (int w = 0; w < n; w += GROUP) {
         const int k = w + sub;
         const float ox0 = nox, oy0 = noy, cux = nux, cuy = nuy, cuz = nuz;
-        if (k + GROUP < ke) {
-          const int pn = k + GROUP;
-          nox = Aox[pn]; noy = Aoy[pn]; nux = Aux[pn]; nuy = Auy[pn]; nuz = Auz[pn];
+        if (k + GROUP < n) {
+          const int kn = k + GROUP;
+          nox = Aox[kn]; noy = Aoy[kn]; nux = Aux[kn]; nuy = Auy[kn]; nuz = Auz[kn];
         }
-        const bool on = k >= kb && k < ke;
+        const bool on = k < n;
         bool cross = false;
+        int dx = 0, dy = 0;
+        float ox1 = 0.0f, oy1 = 0.0f;
+        Push32 o{};
         if (on) {
           __asm__ volatile("" ::: "memory");
-          const Push32 o = boris32(CF4 + g * (CFS / 4), ox0, oy0, cux, cuy, cuz, P, fac1, txxx, fac2);
-          float ox1, oy1;
-          const int dx = rebase(o.ox, ox1);
-          const int dy = rebase(o.oy, oy1);
-          int cn = il + dx;
-          cn = cn < 0 ? cn + P.nx : (cn >= P.nx ? cn - P.nx : cn);   // periodic x
-          Bci[k] = cn;
-          Box[k] = ox1;
-          Boy[k] = o.oy;                                  // rebased by the row migration
-          Bux[k] = o.ux;
-          Buy[k] = o.uy;
-          Buz[k] = o.uz;
+          o = boris32(CF4 + g * (CFS / 4), ox0, oy0, cux, cuy, cuz, P, fac1, txxx, fac2);
+          dx = rebase(o.ox, ox1);
+          dy = rebase(o.oy, oy1);
           ok = ok && fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f;
           cross = (dx | dy) != 0;
-          // movers count one by one; the stayers once per group (below)
-          if (bcnt && cross) atomicAdd(&lc[(s * 3 + dy + 1) * (CHUNK + 2) + g + dx + 1], 1);
           if (!cross) {
+            Aox[k] = ox1;
+            Aoy[k] = oy1;
+            Aux[k] = o.ux;
+            Auy[k] = o.uy;
+            Auz[k] = o.uz;
             const float qv =
                 q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
             core32(core, ox0, oy0, ox1, oy1, qc, qv);
           }
         }
-        const unsigned sm = (unsigned)(__ballot(on && !cross) >> gshift) & 0xFFFFu;
-        if (bcnt && sub == 0 && sm) atomicAdd(&lc[(s * 3 + 1) * (CHUNK + 2) + g + 1], __popc(sm));
-        // list the group's cell-crossing particles in (species, index) order
+        // movers in (species, slot) order
         const unsigned gm = (unsigned)(__ballot(cross) >> gshift) & 0xFFFFu;
         if (cross) {
           const int pos = xcnt + __popc(gm & ((1u << sub) - 1u));
-          if (pos < XCAP) xg[pos] = (s << 28) | k;
+          if (pos < WM_MG) {
+            const size_t R = reg + pos;
+            int ci = il + dx;
+            ci = ci < 0 ? ci + P.nx : (ci >= P.nx ? ci - P.nx : ci);   // periodic x
+            int cj = jl + dy;
+            cj = cj < 0 ? cj + P.ny : (cj >= P.ny ? cj - P.ny : cj);   // periodic y
+            MoverRec mr;
+            mr.dst = (int)bin_of(s, cj, ci, P.ny, P.nx);
+            mr.dir = (dy + 1) * 3 + (dx + 1);
+            mr.ox0 = ox0;
+            mr.oy0 = oy0;
+            mr.ox1 = ox1;
+            mr.oy1 = oy1;
+            mr.ux = o.ux;
+            mr.uy = o.uy;
+            mr.uz = o.uz;
+            mr.pad = 0;
+            mr.id = ids[base + k];
+            M.rec[R] = mr;
+            xg[pos] = (s << 28) | (((dy + 1) * 3 + (dx + 1)) << 20) | k;
+          }
         }
         xcnt += __popc(gm);
       }
@@ -409,58 +409,31 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
       const float v = row_sum16(core[c]);
       if (sub == 0) cb[core_slot(c)] += (double)v;
     }
-    if (xcnt > XCAP) {
-      // more crossers than the list holds: this group does all of its own,
-      // one particle per lane, reading back the group's stores
-      __threadfence_block();
-      for (int s = 0; s < P.nsp; ++s) {
-        const size_t seg = (size_t)s * P.ny + jl;
-        const int *cc = cumcnt + seg * (size_t)(P.nx + 1);
-        const int kb = max(0, cc[il]), ke = min(P.cap, cc[il + 1]);
-        const size_t base = seg * (size_t)P.cap;
-        for (int k0 = kb; k0 < ke; k0 += GROUP) {
-          float acc[NACC];
-#pragma unroll
-          for (int v = 0; v < NACC; ++v) acc[v] = 0.0f;
-          const int kk = k0 + sub;
-          if (kk < ke) {
-            const size_t p = base + (size_t)kk;
-            int dx, dy;
-            float oy1;
-            new_frame(B.ci[p], il, P.nx, B.oy[p], dx, dy, oy1);
-            if ((dx | dy) != 0) crossing_stencils(acc, p, il, A, B, P.q[s], P);
-          }
-#pragma unroll
-          for (int v = 0; v < NACC; ++v) {
-            const float w = row_sum16(acc[v]);
-            if (sub == 0) cb[v] += (double)w;
-          }
-        }
-      }
-    }
   }
   if (!ok) atomicOr(derr, DERR_JUMP);
-  __shared__ int gcnt[CHUNK];
-  if (sub == 0 && g < CHUNK) gcnt[g] = (g < ncell && xcnt <= XCAP) ? xcnt : 0;
-  __syncthreads();                                   // also orders the new state's stores
+  if (xcnt > WM_MG && sub == 0) atomicOr(derr, DERR_MOVER_OVER);
+  if (sub == 0 && g < CHUNK) {
+    gcnt[g] = g < ncell ? min(xcnt, WM_MG) : 0;
+    M.n[rbase + g] = gcnt[g];
+  }
+  __syncthreads();                                   // also orders the movers' stores
   if (threadIdx.x == 0) {
     pre[0] = 0;
     for (int c = 0; c < CHUNK; ++c) pre[c + 1] = pre[c] + gcnt[c];
   }
   __syncthreads();
-  // Listed crossers of the whole workgroup: wave w computes stencil
-  // component w of XR crossers at a time (one crosser per lane), then each
-  // (cell, value) pair adds its cell's crossers in list order.
+  // Movers of the whole workgroup: wave w computes stencil component w of
+  // XR movers at a time (one mover per lane), then each (cell, value) pair
+  // adds its cell's movers in list order.
   const int nxs = pre[CHUNK];
   for (int r0 = 0; r0 < nxs; r0 += XR) {
     const int comp = threadIdx.x / XR, e = r0 + threadIdx.x % XR;
     if (comp < 3 && e < nxs) {
       int gg = 0;
       while (pre[gg + 1] <= e) ++gg;
-      const int ent = xl[gg * XCAP + (e - pre[gg])];
-      const int s = ent >> 28, k = ent & ((1 << 28) - 1);
-      const size_t p = ((size_t)s * P.ny + jl) * P.cap + (size_t)k;
-      crossing_comp(stg + (e - r0) * NACC, comp, p, i0 + gg, A, B, P.q[s], P);
+      const int j = e - pre[gg];
+      const int s = xl[gg * WM_MG + j] >> 28;
+      mover_comp(stg + (e - r0) * NACC, comp, M, (rbase + gg) * WM_MG + j, P.q[s], P);
     }
     __syncthreads();
     for (int pr = threadIdx.x; pr < ncell * NACC; pr += blockDim.x) {
@@ -472,16 +445,47 @@ __global__ __launch_bounds__(256) void k_push_deposit32(
     }
     __syncthreads();
   }
-  if (bcnt) {
-    // shared edge cells are also counted by the neighbouring chunks: integer
-    // atomics keep the totals exact
-    for (int e = threadIdx.x; e < P.nsp * 3 * (CHUNK + 2); e += blockDim.x) {
-      const int v = lc[e];
-      if (!v) continue;
-      const int c = e % (CHUNK + 2), k = (e / (CHUNK + 2)) % 3, s = e / (3 * (CHUNK + 2));
-      int cell = i0 + c - 1;
-      cell = cell < 0 ? cell + P.nx : (cell >= P.nx ? cell - P.nx : cell);
-      atomicAdd(&bcnt[(((size_t)s * P.ny + jl) * 3 + k) * P.nx + cell], v);
+  // close the holes and record the counts, one bin per (group, species)
+  if (g < ncell) {
+    const int m_all = gcnt[g];
+    int j0 = 0;
+    for (int s = 0; s < P.nsp; ++s) {
+      int m = 0;
+      while (j0 + m < m_all && (xg[j0 + m] >> 28) == s) ++m;
+      const size_t bin = bin_of(s, jl, il, P.ny, P.nx);
+      const int n = min(CB, cnt[bin]);
+      const int n1 = n - m;
+      const size_t base = bin * (size_t)CB;
+      int a = 0;                                     // holes below n1
+      while (a < m && (xg[j0 + a] & 0xFFFFF) < n1) ++a;
+      for (int t = sub; t < a; t += GROUP) {
+        // the t-th particle of [n1, n) that is not a hole
+        int idx = a, c = 0, from = n1;
+        for (; from < n; ++from) {
+          if (idx < m && (xg[j0 + idx] & 0xFFFFF) == from) {
+            ++idx;
+            continue;
+          }
+          if (c == t) break;
+          ++c;
+        }
+        if (from >= n) break;                          // (cannot happen: a tail particles)
+        const size_t h = base + (xg[j0 + t] & 0xFFFFF), f = base + from;
+        A.ox[h] = A.ox[f];
+        A.oy[h] = A.oy[f];
+        A.ux[h] = A.ux[f];
+        A.uy[h] = A.uy[f];
+        A.uz[h] = A.uz[f];
+        ids[h] = ids[f];
+      }
+      if (sub < 8) {
+        const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
+        int c = 0;
+        for (int q = 0; q < m; ++q) c += ((xg[j0 + q] >> 20) & 0xF) == d;
+        mcnt[bin * 8 + sub] = c;
+      }
+      if (sub == 0) ncnt[bin] = n1;
+      j0 += m;
     }
   }
   double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;
@@ -582,17 +586,22 @@ __global__ void k_rec32_to_aos(int ndim, int w, int cap, int ny, int nxgs, int n
 }
 
 // FP64 energy sums of FP32 momenta: per-workgroup partials.
-__global__ __launch_bounds__(256) void k_energy_kin32(int cap, const int *np2, const float *ux,
-                                                      const float *uy, const float *uz, double c,
-                                                      double *part, int nchunk) {
+// Workgroup (x, seg) sums the bins of cells x, x + nchunk, ... of segment seg.
+__global__ __launch_bounds__(256) void k_energy_kin32(int nx, int CB, const int *cnt,
+                                                      const float *ux, const float *uy,
+                                                      const float *uz, double c, double *part,
+                                                      int nchunk) {
   __shared__ double sh[256];
   const int seg = blockIdx.y;
-  const int n = np2[seg];
   double v = 0.0;
-  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += 256 * nchunk) {
-    const size_t p = (size_t)seg * cap + k;
-    const double a = ux[p], b = uy[p], d = uz[p];
-    v += sqrt(1 + (a * a + b * b + d * d) / (c * c)) - 1;
+  for (int i = blockIdx.x; i < nx; i += nchunk) {
+    const size_t bin = (size_t)seg * nx + i;
+    const int n = min(CB, cnt[bin]);
+    for (int k = threadIdx.x; k < n; k += 256) {
+      const size_t p = bin * CB + k;
+      const double a = ux[p], b = uy[p], d = uz[p];
+      v += sqrt(1 + (a * a + b * b + d * d) / (c * c)) - 1;
+    }
   }
   sh[threadIdx.x] = v;
   __syncthreads();
@@ -636,13 +645,15 @@ int wm_k_cell_fields32(wm_ctx *ctx) {
   return WM_OK;
 }
 
-int wm_k_push_deposit32(wm_ctx *ctx, int src, int dst) {
+int wm_push32_regions(int nx, int ny) { return ny * wm_cdiv(nx, CHUNK) * CHUNK; }
+
+int wm_k_push_deposit32(wm_ctx *ctx) {
   PP32 P = make_pp32(ctx);
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
   hipLaunchKernelGGL(k_push_deposit32, dim3(P.nchunk, P.ny), dim3(256), 0, ctx->stream, P,
-                     ctx->cumcnt, ctx->tmp32, TX, tps, ctx->b32[src], ctx->b32[dst],
-                     ctx->dep_part, ctx->bcnt, ctx->derr);
+                     ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->b32[0], ctx->id_bins,
+                     ctx->mv, ctx->bin_ncnt, ctx->bin_mcnt, ctx->dep_part, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -672,9 +683,10 @@ int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *sta
 
 int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk) {
   const int nseg = ctx->g.nsp * ctx->g.ny;
-  const Rec32 &B = ctx->b32[bank];
-  hipLaunchKernelGGL(k_energy_kin32, dim3(nchunk, nseg), dim3(256), 0, ctx->stream, ctx->cap,
-                     ctx->np2, B.ux, B.uy, B.uz, ctx->g.c, part, nchunk);
+  (void)bank;                                      // the bins are the state
+  const Rec32 &B = ctx->b32[0];
+  hipLaunchKernelGGL(k_energy_kin32, dim3(nchunk, nseg), dim3(256), 0, ctx->stream, ctx->g.nx,
+                     ctx->CB, ctx->bin_cnt, B.ux, B.uy, B.uz, ctx->g.c, part, nchunk);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

@@ -55,6 +55,13 @@ __device__ inline int pmod(int a, int n) {
   return a < 0 ? a + n : a;
 }
 
+// Inside flexible CG the preconditioner's launches are queued ahead of the
+// host's convergence check; once every component has stopped they return at
+// once.
+__device__ inline bool gated_off(const FcgScal *gate) {
+  return gate && !(gate[0].active || gate[1].active || gate[2].active);
+}
+
 // ------------------------------------------------------------- MFMA conv
 // out[b, y, x, co] = sum_tap sum_ci in[b, y+dy, x+dx, ci] * w[tap, co, ci].
 // One workgroup per 16x16 output tile; wave w computes tile rows 4w..4w+3.
@@ -69,9 +76,11 @@ template <int CIN, int COUT, bool OUTF32, bool UP = false>
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
                                               const bf16 *__restrict__ w, void *outp, int H,
                                               int W, const bf16 *__restrict__ inc = nullptr,
-                                              bf16 *__restrict__ uout = nullptr) {
+                                              bf16 *__restrict__ uout = nullptr,
+                                              const FcgScal *__restrict__ gate = nullptr) {
   static_assert(CIN == 16 || CIN == 32, "CIN");
   static_assert(COUT % 16 == 0, "COUT");
+  if (gated_off(gate)) return;
   __shared__ __attribute__((aligned(16))) bf16 sin[(T + 2) * (T + 2) * CIN];
   __shared__ __attribute__((aligned(16))) bf16 sw[9 * COUT * CIN];
   const int b = blockIdx.z;
@@ -231,7 +240,9 @@ __global__ void k_fold(const float *__restrict__ wl, const float *__restrict__ w
 __global__ __launch_bounds__(256) void k_lift_pool(const double *__restrict__ r, size_t rstride,
                                                    const double *__restrict__ sc,
                                                    const float *__restrict__ fold,
-                                                   bf16 *__restrict__ d1, int H, int W, int B) {
+                                                   bf16 *__restrict__ d1, int H, int W, int B,
+                                                   const FcgScal *__restrict__ gate) {
+  if (gated_off(gate)) return;
   __shared__ float xs[(T + 2) * (T + 2)];
   __shared__ float fl[16 * C0];
   const int b = blockIdx.z;
@@ -291,6 +302,7 @@ __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, s
   __shared__ float bs[CE * CE * C0];
   __shared__ float fk[25], fp[256];
   __shared__ double sh[256];
+  if (dots && gated_off(S)) return;
   const int b = blockIdx.z;
   const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
   load_x0<2>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
@@ -344,7 +356,8 @@ __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, s
 // ------------------------------------------------------------ coarse levels
 // 2x2 mean, ((x00 + x01) + (x10 + x11)) * 0.25, rounded to BF16.
 __global__ void k_pool(const bf16 *__restrict__ in, bf16 *__restrict__ out, int B, int H, int W,
-                       int C) {
+                       int C, const FcgScal *__restrict__ gate) {
+  if (gated_off(gate)) return;
   const int Ho = H / 2, Wo = W / 2;
   const size_t n = (size_t)B * Ho * Wo * C;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
@@ -704,18 +717,19 @@ dim3 tiles(int H, int W, int B) { return dim3(wm_cdiv(W, T), wm_cdiv(H, T), B); 
 int ew_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
 
 template <int CIN, int COUT, bool F32>
-int conv(wm_ctx *ctx, const bf16 *in, const bf16 *w, void *out, int H, int W) {
+int conv(wm_ctx *ctx, const bf16 *in, const bf16 *w, void *out, int H, int W,
+         const FcgScal *gate = nullptr) {
   hipLaunchKernelGGL((k_conv<CIN, COUT, F32>), tiles(H, W, wm_pc::B), dim3(256), 0, ctx->stream,
-                     in, w, out, H, W);
+                     in, w, out, H, W, nullptr, nullptr, gate);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
 template <int CIN, int COUT>
 int conv_up(wm_ctx *ctx, const bf16 *a, const bf16 *bc, bf16 *uout, const bf16 *w, bf16 *out,
-            int H, int W) {
+            int H, int W, const FcgScal *gate) {
   hipLaunchKernelGGL((k_conv<CIN, COUT, false, true>), tiles(H, W, wm_pc::B), dim3(256), 0,
-                     ctx->stream, a, w, (void *)out, H, W, bc, uout);
+                     ctx->stream, a, w, (void *)out, H, W, bc, uout, gate);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -743,22 +757,23 @@ int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const doubl
   const float *th = pc->theta;
   const bf16 *wb = pc->wb;
   hipStream_t st = ctx->stream;
+  const FcgScal *gate = dots ? S : nullptr;
   hipLaunchKernelGGL(k_lift_pool, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
-                     pc->d1, H, W, B);
-  WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1)));
+                     pc->d1, H, W, B, gate);
+  WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1, gate)));
   hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 2, C1))), dim3(256), 0, st, pc->a1, pc->d2, B,
-                     H >> 1, W >> 1, C1);
-  WM_TRY((conv<C1, C2, false>(ctx, pc->d2, wb + loff(L_D2), pc->a2, H >> 2, W >> 2)));
+                     H >> 1, W >> 1, C1, gate);
+  WM_TRY((conv<C1, C2, false>(ctx, pc->d2, wb + loff(L_D2), pc->a2, H >> 2, W >> 2, gate)));
   hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 3, C2))), dim3(256), 0, st, pc->a2, pc->d3, B,
-                     H >> 2, W >> 2, C2);
-  WM_TRY((conv<C2, C2, false>(ctx, pc->d3, wb + loff(L_D3), pc->a3, H >> 3, W >> 3)));
+                     H >> 2, W >> 2, C2, gate);
+  WM_TRY((conv<C2, C2, false>(ctx, pc->d3, wb + loff(L_D3), pc->a3, H >> 3, W >> 3, gate)));
   // u2 = a2 + up(a3) and u1 = a1 + up(b2) are formed inside the U2/U1
   // convolutions; training keeps them for the weight gradients
   const bool keep = y != nullptr;
   WM_TRY((conv_up<C2, C1>(ctx, pc->a2, pc->a3, keep ? pc->u2 : nullptr, wb + loff(L_U2), pc->b2,
-                          H >> 2, W >> 2)));
+                          H >> 2, W >> 2, gate)));
   WM_TRY((conv_up<C1, C0>(ctx, pc->a1, pc->b2, keep ? pc->u1 : nullptr, wb + loff(L_U1), pc->b1,
-                          H >> 1, W >> 1)));
+                          H >> 1, W >> 1, gate)));
   hipLaunchKernelGGL(k_up_proj, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
                      pc->b1, z, zstride, y, H, W, B, q, dots, S);
   (void)th;

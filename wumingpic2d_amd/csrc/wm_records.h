@@ -5,9 +5,9 @@
 //  Rec64 (FP64 form, the reference's arithmetic): absolute x, y and
 //        u = gamma v in FP64 -- 40 B per particle (+ 8 B ID).
 //  Rec32 (FP32 form): cell index ci (int32, x-cell relative to nxgs), the
-//        in-cell offsets ox in [0, 1) and oy (relative to the row; in [0, 1)
-//        except between the push and the row migration), u in FP32 --
-//        24 B per particle (+ 8 B ID).
+//        in-cell offsets ox, oy in [0, 1), u in FP32 -- 24 B per particle
+//        (+ 8 B ID).  In the cell bins of the resident state the cell index
+//        is the bin's and ci is not stored (20 B).
 #pragma once
 
 #include <hip/hip_runtime.h>

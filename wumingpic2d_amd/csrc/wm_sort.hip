@@ -142,40 +142,78 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, 
 }
 
 // ------------------------------------------------------------------------
-// FP32 form: row migration and cell sort as one stable counting scatter.
-//
-// The push counts, per source segment, where its particles go: bcnt[src
-// seg][dy + 1][new cell] (dy = row change).  Destination row jd takes, cell
-// by cell, the arrivals from row jd - 1 (moving up), then the particles that
-// stayed, then the arrivals from row jd + 1 (moving down), each group in its
-// source index order.  This is a fixed permutation (the same every run) but
-// not the reference's hole-filling order of boundary_periodic.f90:193-226,
-// which the FP64 form keeps; cumcnt and np2 mean exactly what they mean
-// after sort.f90:36-82.
+// FP32 form: cell bins.  Bin (s, jl, i) = seg * nx + i holds up to CB
+// particles of cell i of row segment seg = s * ny + jl in slots
+// [bin * CB, bin * CB + cnt[bin]).
 
-__global__ void k_fill_int(int *p, size_t n) {
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * blockDim.x)
-    p[e] = 0;
+// Dense rows (any order, np2 per segment) -> bins, stable: within a bin the
+// particles keep their row order.  One wavefront per segment, as k_sort.
+__global__ __launch_bounds__(SW) void k_dense_to_bins(SP P, int CB, const int *__restrict__ np2,
+                                                      Rec32 src, const int64_t *__restrict__ ids,
+                                                      Rec32 dst, int64_t *__restrict__ idd,
+                                                      int *__restrict__ cnt, int *derr) {
+  extern __shared__ int run[];             // particles written per cell so far
+  const int seg = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int n = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  int bad = 0;
+  for (int i = lane; i < P.nx; i += SW) run[i] = 0;
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += SW) {
+    const int k = k0 + lane;
+    Rec32::Val v{};
+    int key = -1;
+    int64_t vid = 0;
+    if (k < n) {
+      v = src.load(base + k);
+      vid = ids[base + k];
+      key = v.ci;
+      if (key < 0 || key >= P.nx) {
+        bad |= DERR_RANGE;
+        key = -1;
+      }
+    }
+    const bool act = key >= 0;
+    const WaveMatch m = wave_match(key, act);
+    if (act) {
+      const int slot = run[key] + m.rank;
+      if (slot < CB) {
+        const size_t d = ((size_t)seg * P.nx + key) * CB + slot;
+        dst.ox[d] = v.ox;
+        dst.oy[d] = v.oy;
+        dst.ux[d] = v.ux;
+        dst.uy[d] = v.uy;
+        dst.uz[d] = v.uz;
+        idd[d] = vid;
+      } else {
+        bad |= DERR_NP_OVER;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (act && m.leader) run[key] += m.count;
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  for (int i = lane; i < P.nx; i += SW) cnt[(size_t)seg * P.nx + i] = min(run[i], CB);
+  if (bad) atomicOr(derr, bad);
 }
 
-// One workgroup per destination segment: cumcnt, np2 and, in place of the
-// three source counts that land here, their first destination slot.
-__global__ __launch_bounds__(256) void k_bin_prefix(int nx, int ny, int cap, int *__restrict__ bcnt,
-                                                    int *__restrict__ cumcnt,
-                                                    int *__restrict__ np2new,
-                                                    int *__restrict__ derr) {
+// Bins -> dense rows sorted by cell, with cumcnt and np2 as sort.f90:64-69
+// leaves them.  One workgroup per segment.
+__global__ __launch_bounds__(256) void k_bins_to_dense(SP P, int CB, const int *__restrict__ cnt,
+                                                       Rec32 src, const int64_t *__restrict__ ids,
+                                                       Rec32 dst, int64_t *__restrict__ idd,
+                                                       int *__restrict__ cumcnt,
+                                                       int *__restrict__ np2, int *derr) {
+  extern __shared__ int cc[];              // nx + 1 prefix
   __shared__ int sh[256];
-  const int dseg = blockIdx.x;
-  const int s = dseg / ny, jd = dseg % ny;
-  const int jlo = jd == 0 ? ny - 1 : jd - 1, jhi = jd == ny - 1 ? 0 : jd + 1;
-  int *lo = bcnt + (((size_t)s * ny + jlo) * 3 + 2) * nx;   // row jd-1 moving up
-  int *mid = bcnt + (((size_t)s * ny + jd) * 3 + 1) * nx;   // staying
-  int *hi = bcnt + (((size_t)s * ny + jhi) * 3 + 0) * nx;   // row jd+1 moving down
-  const int per = (nx + 255) / 256;
-  const int c0 = min(nx, (int)threadIdx.x * per), c1 = min(nx, c0 + per);
+  const int seg = blockIdx.x;
+  const int *c = cnt + (size_t)seg * P.nx;
+  const int per = (P.nx + 255) / 256;
+  const int c0 = min(P.nx, (int)threadIdx.x * per), c1 = min(P.nx, c0 + per);
   int tot = 0;
-  for (int c = c0; c < c1; ++c) tot += lo[c] + mid[c] + hi[c];
+  for (int i = c0; i < c1; ++i) tot += c[i];
   sh[threadIdx.x] = tot;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
@@ -185,182 +223,142 @@ __global__ __launch_bounds__(256) void k_bin_prefix(int nx, int ny, int cap, int
     __syncthreads();
   }
   int run = sh[threadIdx.x] - tot;
-  int *cc = cumcnt + (size_t)dseg * (nx + 1);
-  for (int c = c0; c < c1; ++c) {
-    const int a = lo[c], b = mid[c], d = hi[c];
-    cc[c] = run;
-    lo[c] = run;
-    mid[c] = run + a;
-    hi[c] = run + a + b;
-    run += a + b + d;
+  for (int i = c0; i < c1; ++i) {
+    cc[i] = run;
+    run += c[i];
   }
-  if (threadIdx.x == 255) {
-    cc[nx] = sh[255];
-    np2new[dseg] = sh[255];
-    if (sh[255] > cap) atomicOr(derr, DERR_NP_OVER);
+  if (threadIdx.x == 255) cc[P.nx] = sh[255];
+  __syncthreads();
+  const int n = cc[P.nx];
+  int *cco = cumcnt + (size_t)seg * (P.nx + 1);
+  for (int i = threadIdx.x; i <= P.nx; i += 256) cco[i] = cc[i];
+  if (threadIdx.x == 0) {
+    np2[seg] = n;
+    if (n > P.cap) atomicOr(derr, DERR_NP_OVER);
   }
-}
-
-// Stable scatter, one wavefront per (source segment, range of SUBC old
-// cells).  The source is sorted by old cell, and a particle lands at most one
-// cell away, so a range's particles can only share destination keys with the
-// old cells just outside it.  Those that come earlier in the source order
-// (old cells below c0, including the periodic wrap of the last range) are
-// counted first; the range then writes its particles in order behind them.
-constexpr int SUBC = 32;
-
-// Local key of landing cell k (-1 if outside the range's key set): the
-// range's own cells, then the periodic neighbours c0 - 1 and c1.
-__device__ inline int local_key(int k, int c0, int c1, int nx, int nk) {
-  if (k >= c0 && k < c1) return k - c0 + 1;
-  if (k == (c0 == 0 ? nx - 1 : c0 - 1)) return 0;
-  if (k == (c1 == nx ? 0 : c1)) return nk - 1;
-  return -1;
-}
-
-__device__ inline int row_move(float oy) { return oy < 0.0f ? -1 : (oy >= 1.0f ? 1 : 0); }
-
-constexpr int SCW = 4;     // wavefronts (ranges) per workgroup
-
-__global__ __launch_bounds__(SW * SCW) void k_bin_scatter(SP P, int ny, int nsub,
-                                                          const int *__restrict__ ccold,
-                                                          Rec32 src, const int64_t *__restrict__ ids,
-                                                          Rec32 dst, int64_t *__restrict__ idd,
-                                                          const int *__restrict__ bcnt,
-                                                          int *__restrict__ derr) {
-  __shared__ int offs[SCW][3 * (SUBC + 2)];
-  __shared__ int preds[SCW][8];
-  __shared__ int npreds[SCW];
-  const int wv = threadIdx.x / SW;
-  const int range = blockIdx.x * SCW + wv;
-  if (range >= nsub) return;                       // whole wavefronts only: no barrier below
-  int *off = offs[wv], *pred = preds[wv];
-  int &npred = npreds[wv];
-  const int seg = blockIdx.y;
-  const int s = seg / ny, jl = seg % ny;
-  const int lane = threadIdx.x % SW;
-  const int nx = P.nx;
-  const int c0 = range * SUBC, c1 = min(nx, c0 + SUBC);
-  const bool whole = c0 == 0 && c1 == nx;           // one range: no neighbour keys
-  const int nk = c1 - c0 + 2;
-  const int *cc = ccold + (size_t)seg * (nx + 1);
   const size_t base = (size_t)seg * P.cap;
-  // first destination slots of the range's keys
-  for (int e = lane; e < 3 * nk; e += SW) {
-    const int d = e / nk, kl = e % nk;
-    int cell = c0 - 1 + kl;
-    if (kl == 0) cell = c0 == 0 ? nx - 1 : c0 - 1;
-    if (kl == nk - 1) cell = c1 == nx ? 0 : c1;
-    off[e] = (whole && (kl == 0 || kl == nk - 1)) ? 0
-                                                   : bcnt[((size_t)seg * 3 + d) * nx + cell];
+  for (int k = threadIdx.x; k < min(n, P.cap); k += 256) {
+    int lo = 0, hi = P.nx - 1;                     // cell i with cc[i] <= k < cc[i + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (cc[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    const size_t f = ((size_t)seg * P.nx + lo) * CB + (k - cc[lo]);
+    const size_t d = base + k;
+    dst.ci[d] = lo;
+    dst.ox[d] = src.ox[f];
+    dst.oy[d] = src.oy[f];
+    dst.ux[d] = src.ux[f];
+    dst.uy[d] = src.uy[f];
+    dst.uz[d] = src.uz[f];
+    idd[d] = ids[f];
   }
-  // old cells before c0 (in the source order) that neighbour one of the
-  // range's keys; only the edge keys c0 - 1, c0, c1 - 1, c1 (periodic) have
-  // neighbours outside [c0, c1), and these are among c0 - 2, c0 - 1, 0, 1
-  if (lane == 0) {
-    int np = 0;
-    if (!whole) {
-      const int edge[4] = {c0 == 0 ? nx - 1 : c0 - 1, c0, c1 - 1, c1 == nx ? 0 : c1};
-      for (int q = 0; q < 4; ++q) {
-        for (int d = -1; d <= 1; ++d) {
-          int o = edge[q] + d;
-          o = o < 0 ? o + nx : (o >= nx ? o - nx : o);
-          if (o >= c0) continue;
-          bool dup = false;
-          for (int t = 0; t < np; ++t) dup = dup || pred[t] == o;
-          if (!dup && np < 8) pred[np++] = o;
-        }
+}
+
+// Source bin of destination bin d for direction dir = (dy+1)*3 + (dx+1).
+__device__ inline size_t source_bin(size_t d, int dir, int nx, int ny) {
+  const int i = (int)(d % nx);
+  const size_t sj = d / nx;
+  const int jl = (int)(sj % ny);
+  const size_t s = sj / ny;
+  int si = i - (dir % 3 - 1), sj2 = jl - (dir / 3 - 1);
+  si = si < 0 ? si + nx : (si >= nx ? si - nx : si);
+  sj2 = sj2 < 0 ? sj2 + ny : (sj2 >= ny ? sj2 - ny : sj2);
+  return (s * ny + sj2) * nx + si;
+}
+
+// Bin counts after the migration, and where each direction's arrivals
+// start: bin d keeps its ncnt[d] stayers, then receives the arrivals of
+// directions 0, 1, 2, 3, 5, 6, 7, 8 in turn (aoff[d * 8 + k] = first slot of
+// the k-th direction), each in the order its source bin listed them.
+__global__ void k_bin_counts(int nx, int ny, int CB, size_t nbin, const int *__restrict__ ncnt,
+                             const int *__restrict__ mcnt, int *__restrict__ aoff,
+                             int *__restrict__ cnt, int *derr) {
+  for (size_t d = blockIdx.x * (size_t)blockDim.x + threadIdx.x; d < nbin;
+       d += (size_t)gridDim.x * blockDim.x) {
+    int c = ncnt[d];
+    for (int dd = 0; dd < 8; ++dd) {
+      const int dirp = dd < 4 ? dd : dd + 1;
+      aoff[d * 8 + dd] = c;
+      c += mcnt[source_bin(d, dirp, nx, ny) * 8 + dd];
+    }
+    if (c > CB) {
+      atomicOr(derr, DERR_NP_OVER);
+      c = CB;
+    }
+    cnt[d] = c;
+  }
+}
+
+// Movers -> destination bins, one thread per mover region (source cell).
+__global__ void k_insert_movers(int CB, size_t nregion, Movers M, const int *__restrict__ aoff,
+                                Rec32 A, int64_t *__restrict__ ids, int *derr) {
+  for (size_t region = blockIdx.x * (size_t)blockDim.x + threadIdx.x; region < nregion;
+       region += (size_t)gridDim.x * blockDim.x) {
+    const int nm = min(M.n[region], WM_MG);
+    int dprev[16];                                 // destinations of the first movers
+    for (int j = 0; j < nm; ++j) {
+      const MoverRec mr = M.rec[region * WM_MG + j];
+      int rank = 0;                                // earlier movers to the same bin
+      for (int q = 0; q < j; ++q)
+        rank += (q < 16 ? dprev[q] : M.rec[region * WM_MG + q].dst) == mr.dst;
+      if (j < 16) dprev[j] = mr.dst;
+      const int d8 = mr.dir < 4 ? mr.dir : mr.dir - 1;
+      const int off = aoff[(size_t)mr.dst * 8 + d8] + rank;
+      if (off >= CB) {
+        atomicOr(derr, DERR_NP_OVER);
+        continue;
       }
-    }
-    npred = np;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int q = 0; q < npred; ++q) {
-    const int o = pred[q];
-    for (int k = cc[o] + lane; k < cc[o + 1]; k += SW) {
-      const size_t p = base + k;
-      const int kl = local_key(src.ci[p], c0, c1, nx, nk);
-      if (kl >= 0) atomicAdd(&off[(row_move(src.oy[p]) + 1) * nk + kl], 1);
+      const size_t p = (size_t)mr.dst * CB + off;
+      A.ox[p] = mr.ox1;
+      A.oy[p] = mr.oy1;
+      A.ux[p] = mr.ux;
+      A.uy[p] = mr.uy;
+      A.uz[p] = mr.uz;
+      ids[p] = mr.id;
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int bad = 0;
-  const int kb = cc[c0], ke = cc[c1];
-  for (int k0 = kb; k0 < ke; k0 += SW * BT) {
-    int key[BT], dy[BT];
-    Rec32::Val v[BT];
-    int64_t vid[BT];
-#pragma unroll
-    for (int b = 0; b < BT; ++b) {
-      const int k = k0 + b * SW + lane;
-      key[b] = -1;
-      dy[b] = 0;
-      if (k < ke) {
-        const size_t p = base + k;
-        v[b] = src.load(p);
-        vid[b] = ids[p];
-        dy[b] = row_move(v[b].oy);
-        const int kl = (v[b].ci >= 0 && v[b].ci < nx) ? local_key(v[b].ci, c0, c1, nx, nk) : -1;
-        key[b] = kl >= 0 ? (dy[b] + 1) * nk + kl : -1;
-        if (key[b] < 0) bad = 1;
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < BT; ++b) {
-      const bool act = key[b] >= 0;
-      const WaveMatch m = wave_match(key[b], act);
-      if (act) {
-        const int slot = off[key[b]] + m.rank;
-        const int jd = jl + dy[b] < 0 ? ny - 1 : (jl + dy[b] >= ny ? 0 : jl + dy[b]);
-        if (slot < P.cap) {
-          Rec32::Val w = v[b];
-          w.oy = dy[b] > 0 ? w.oy - 1.0f : (dy[b] < 0 ? wm_below_one(w.oy + 1.0f) : w.oy);
-          const size_t d = ((size_t)s * ny + jd) * P.cap + slot;
-          dst.store(d, w);
-          idd[d] = vid[b];
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      if (act && m.leader) off[key[b]] += m.count;
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  if (bad) atomicOr(derr, DERR_RANGE);
 }
 
 }  // namespace
 
-int wm_k_bin_clear(wm_ctx *ctx) {
-  const size_t n = (size_t)ctx->g.nsp * ctx->g.ny * 3 * ctx->g.nx;
-  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)),
-                     dim3(256), 0, ctx->stream, ctx->bcnt, n);
-  WM_HIP(hipGetLastError());
-  return WM_OK;
-}
-
-int wm_k_bin32(wm_ctx *ctx, int src, int dst) {
+int wm_k_dense_to_bins(wm_ctx *ctx) {
   SP P;
   P.nx = ctx->g.nx;
   P.nxgs = ctx->g.nxgs;
   P.cap = ctx->cap;
-  const int ny = ctx->g.ny;
-  const int nseg = ctx->g.nsp * ny;
-  // the new np2 and cumcnt go to spare arrays, which then take their place
-  // (the scatter reads the old cumcnt)
-  hipLaunchKernelGGL(k_bin_prefix, dim3(nseg), dim3(256), 0, ctx->stream, P.nx, ny, P.cap,
-                     ctx->bcnt, ctx->cumcnt2, ctx->nup, ctx->derr);
-  const int nsub = wm_cdiv(P.nx, SUBC);
-  hipLaunchKernelGGL(k_bin_scatter, dim3(wm_cdiv(nsub, SCW), nseg), dim3(SW * SCW), 0, ctx->stream,
-                     P, ny, nsub, ctx->cumcnt, ctx->b32[src], ctx->id[ctx->idcur], ctx->b32[dst],
-                     ctx->id[1 - ctx->idcur], ctx->bcnt, ctx->derr);
+  const int nseg = ctx->g.nsp * ctx->g.ny;
+  hipLaunchKernelGGL(k_dense_to_bins, dim3(nseg), dim3(SW), sizeof(int) * P.nx, ctx->stream, P,
+                     ctx->CB, ctx->np2, ctx->b32[1], ctx->id[ctx->idcur], ctx->b32[0],
+                     ctx->id_bins, ctx->bin_cnt, ctx->derr);
   WM_HIP(hipGetLastError());
-  std::swap(ctx->np2, ctx->nup);
-  std::swap(ctx->cumcnt, ctx->cumcnt2);
-  ctx->idcur = 1 - ctx->idcur;
+  return WM_OK;
+}
+
+int wm_k_bins_to_dense(wm_ctx *ctx) {
+  SP P;
+  P.nx = ctx->g.nx;
+  P.nxgs = ctx->g.nxgs;
+  P.cap = ctx->cap;
+  const int nseg = ctx->g.nsp * ctx->g.ny;
+  hipLaunchKernelGGL(k_bins_to_dense, dim3(nseg), dim3(256), sizeof(int) * (P.nx + 1),
+                     ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->b32[0], ctx->id_bins,
+                     ctx->b32[1], ctx->id[ctx->idcur], ctx->cumcnt, ctx->np2, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_insert_movers(wm_ctx *ctx) {
+  const int nx = ctx->g.nx, ny = ctx->g.ny;
+  const size_t nreg = (size_t)ctx->nregion;
+  const size_t nbin = (size_t)ctx->g.nsp * ny * nx;
+  hipLaunchKernelGGL(k_bin_counts, dim3((unsigned)std::min<size_t>((nbin + 255) / 256, 65536)),
+                     dim3(256), 0, ctx->stream, nx, ny, ctx->CB, nbin, ctx->bin_ncnt, ctx->bin_mcnt,
+                     ctx->bin_aoff, ctx->bin_cnt, ctx->derr);
+  hipLaunchKernelGGL(k_insert_movers, dim3((unsigned)std::min<size_t>((nreg + 255) / 256, 65536)),
+                     dim3(256), 0, ctx->stream, ctx->CB, nreg, ctx->mv, ctx->bin_aoff,
+                     ctx->b32[0], ctx->id_bins, ctx->derr);
+  WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
@@ -376,10 +374,11 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst) {
     return WM_ERR_ARG;
   }
   if (ctx->form == WM_FORM_FP32) {
-    hipLaunchKernelGGL(k_sort<Rec32>, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2,
-                       ctx->b32[src], ctx->id[ctx->idcur], ctx->b32[dst], ctx->id[1 - ctx->idcur],
-                       ctx->cumcnt, ctx->derr);
-  } else {
+    // the FP32 form keeps its particles in cell bins (wm_k_dense_to_bins)
+    wm_set_error(ctx, WM_ERR_ARG, "sort__bucket works on the FP64 particle form");
+    return WM_ERR_ARG;
+  }
+  {
     hipLaunchKernelGGL(k_sort<Rec64>, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2,
                        ctx->b64(src), ctx->id[ctx->idcur], ctx->b64(dst), ctx->id[1 - ctx->idcur],
                        ctx->cumcnt, ctx->derr);
