@@ -54,9 +54,14 @@ def test_fp32_initialiser_matches_host_state():
             assert np.allclose(b[2:5], a[2:5], rtol=2e-7, atol=1e-9)
 
 
-@pytest.mark.parametrize("nx,ny,ppc,seed,steps", [(32, 16, 16, 42, 6), (48, 12, 8, 43, 4)])
-def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps):
-    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=0.1)
+@pytest.mark.parametrize("nx,ny,ppc,seed,steps,nsp", [(32, 16, 16, 42, 6, 2), (48, 12, 8, 43, 4, 2),
+                                                     (8, 4, 96, 44, 3, 1), (16, 16, 8, 45, 4, 3)])
+def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp):
+    """Two species run the push's straight-line pair path; one and three
+    species run its species loop.  96 particles per cell put bins above the
+    80 the push holds in registers, so its direct-load tail runs too (and
+    an 8-cell row is narrower than one workgroup's 16 cells)."""
+    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=0.1, nsp=nsp)
     ctx = _ctx32(p)
     ctx.state_upload(up, np2, cc, uf)
     orc = Oracle(p)

@@ -28,7 +28,7 @@ def make_ctx(p, cap=0, mcap=0, init=("particle", "field", "sort", "bc")):
 
 
 def state(nx, ny, ppc, seed, **kw):
-    p = weibel.grid_params(nx, ny, ppc, **{k: v for k, v in kw.items() if k in ("np_cap", "delt", "wpe")})
+    p = weibel.grid_params(nx, ny, ppc, **{k: v for k, v in kw.items() if k in ("np_cap", "delt", "wpe", "nsp")})
     up, np2, cc, uf = weibel.init_state(p, seed, **{k: v for k, v in kw.items() if k in ("vth", "udrift")})
     return p, up, np2, cc, uf
 

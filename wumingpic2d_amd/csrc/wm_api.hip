@@ -377,7 +377,7 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(dmalloc(ctx, &b.uz, bslots));
     WM_TRY(dmalloc(ctx, &ctx->id_bins, bslots));
     WM_TRY(dmalloc(ctx, &ctx->bin_cnt, nbin));
-    WM_TRY(dmalloc(ctx, &ctx->bin_ncnt, nbin));
+    WM_TRY(dmalloc(ctx, &ctx->bin_mout, nbin));
     WM_TRY(dmalloc(ctx, &ctx->bin_mcnt, nbin * 8));
     WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
     ctx->nregion = wm_push32_regions(g.nx, g.ny);
@@ -386,6 +386,7 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(dmalloc(ctx, &m.rec, ms));
     WM_TRY(dmalloc(ctx, &m.n, (size_t)ctx->nregion));
     WM_TRY(dmalloc(ctx, &ctx->bin_aoff, nbin * 8));
+    WM_TRY(dmalloc(ctx, &ctx->bin_nold, nbin));
     WM_HIP(hipMemsetAsync(m.n, 0, sizeof(int) * ctx->nregion, ctx->stream));
     WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
   } else {
@@ -504,8 +505,8 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
-                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_ncnt, ctx->bin_mcnt,
-                  ctx->bin_aoff, ctx->mv.rec, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_mout, ctx->bin_mcnt,
+                  ctx->bin_aoff, ctx->bin_nold, ctx->mv.rec, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);

@@ -90,7 +90,7 @@ struct __align__(16) MoverRec {   // 48 B: three 16-byte stores
   float ox0, oy0;              // offsets before the push (deposit)
   float ox1, oy1;              // offsets in the destination cell
   float ux, uy, uz;
-  int pad;
+  int slot;                    // slot the particle left in its source bin
   int64_t id;
 };
 struct Movers {
@@ -170,9 +170,10 @@ struct wm_ctx {
   // the reference, used to convert state in and out
   int CB = 0;
   int *bin_cnt = nullptr;        // particles per bin
-  int *bin_ncnt = nullptr;       // per bin after the push: particles that stayed
+  int *bin_mout = nullptr;       // per bin after the push: particles that left
   int *bin_mcnt = nullptr;       // per bin and direction (8): particles that left
-  int *bin_aoff = nullptr;       // per bin and direction (8): first slot of the arrivals
+  int *bin_aoff = nullptr;       // per bin and direction (8): index of the first arrival
+  int *bin_nold = nullptr;       // per bin: particles before the migration
   int64_t *id_bins = nullptr;
   Movers mv{};
   int nregion = 0;

@@ -24,6 +24,8 @@ constexpr int TW = CHUNK + 2;         // push tile width (halo 1)
 constexpr int DW = CHUNK + 4;         // deposit tile width (halo 2)
 constexpr int TILE = 5 * DW * 3;      // doubles per deposit tile
 constexpr int NACC = 65;
+constexpr int NT = 256;               // push workgroup size (a constant: blockDim.x is a
+                                      // memory load, whose wait would include the stores)
 
 struct PP32 {
   int nx, ny, nxgs, nys, nsp, cap, nchunk;
@@ -46,7 +48,10 @@ struct Push32 {
 // w+(o) = o^2/2, sum_ab T(a,b) w_a(ox) w_b(oy) = sum_ij C_ij ox^i oy^j.
 // The workgroup computes C (in FP64, rounded once) for each of its cells and
 // fields; a particle then needs 8 FMAs per field.
-constexpr int CFS = 56;               // floats per cell: 6 fields x 9, padded to 16 B
+// Per cell: field k's 9 coefficients at floats 12 k .. 12 k + 8 (padded to
+// 16 B), so a particle reads one field at a time.
+constexpr int CFS = 72;
+__device__ inline int coeff_at(int k) { return 12 * k; }
 __device__ inline void cell_coeffs(float *cf, const float *T, int c, int k) {
   const double M[3][3] = {{0.5, -1.0, 0.5}, {0.5, 1.0, -1.0}, {0.0, 0.0, 0.5}};
   const float *Tk = T + k * 3 * TW + c;
@@ -55,27 +60,30 @@ __device__ inline void cell_coeffs(float *cf, const float *T, int c, int k) {
       double v = 0.0;
       for (int b = 0; b < 3; ++b)
         for (int a = 0; a < 3; ++a) v += (double)Tk[b * TW + a] * M[a][i] * M[b][j];
-      cf[k * 9 + i * 3 + j] = (float)v;
+      cf[coeff_at(k) + i * 3 + j] = (float)v;
     }
 }
 
 // particle.f90:96-161 in FP32 on the cell frame.
 __device__ inline Push32 boris32(const float4 *cf4, float ox, float oy, float ux, float uy,
                                  float uz, const PP32 &P, float fac1, float txxx, float fac2) {
-  float c[CFS];
-#pragma unroll
-  for (int q = 0; q < CFS / 4; ++q) {
-    const float4 v = cf4[q];
-    c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w;
-  }
   float f[6];
+  // one field at a time, E first: the memory clobber keeps the next field's
+  // LDS reads after this one's use, so only 12 coefficients are live
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const float *C = c + 9 * k;
+  for (int kk = 0; kk < 6; ++kk) {
+    const int k = (kk + 3) % 6;
+    float C[12];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float4 v = cf4[3 * k + q];
+      C[4 * q] = v.x; C[4 * q + 1] = v.y; C[4 * q + 2] = v.z; C[4 * q + 3] = v.w;
+    }
     const float r0 = C[0] + oy * (C[1] + oy * C[2]);
     const float r1 = C[3] + oy * (C[4] + oy * C[5]);
     const float r2 = C[6] + oy * (C[7] + oy * C[8]);
     f[k] = r0 + ox * (r1 + ox * r2);
+    __asm__ volatile("" ::: "memory");
   }
   const float bpx = f[0], bpy = f[1], bpz = f[2];
   const float epx = f[3], epy = f[4], epz = f[5];
@@ -219,10 +227,13 @@ __device__ inline int core_slot(int c) {
   return 40 + a * 5 + b;
 }
 
+// Value (a, b) of component comp of a cell's stencil sums (Jx has no a = 0
+// column and Jy no b = 0 row); branch-free index arithmetic.
 __device__ inline double acc_at(const double *cb, int comp, int a, int b) {
-  if (comp == 0) return a >= 1 ? cb[(a - 1) * 5 + b] : 0.0;
-  if (comp == 1) return b >= 1 ? cb[20 + a * 4 + (b - 1)] : 0.0;
-  return cb[40 + a * 5 + b];
+  const int idx = comp == 0 ? (a - 1) * 5 + b : (comp == 1 ? 20 + a * 4 + (b - 1) : 40 + a * 5 + b);
+  const bool none = (comp == 0 && a == 0) || (comp == 1 && b == 0);
+  const double v = cb[none ? 0 : idx];
+  return none ? 0.0 : v;
 }
 
 // Sum over the 16 lanes of one DPP row (a group): quad butterflies, then
@@ -239,169 +250,312 @@ __device__ inline float row_sum16(float v) {
 
 
 
-constexpr int XR = 64;          // listed movers staged per round of the workgroup pass
-
 // Bin of (species s, row jl, cell i).
 __device__ inline size_t bin_of(int s, int jl, int i, int ny, int nx) {
   return ((size_t)s * ny + jl) * nx + i;
 }
 
-// One component of the stencil of a cell-crossing particle from its mover
-// record R: comp 0 -> out[0..19] (Jx), 1 -> out[20..39] (Jy), 2 -> out[40..64]
-// (Jz).
-__device__ inline void mover_comp(float *out, int comp, const Movers &M, size_t R, float q,
-                                  const PP32 &P) {
-  const MoverRec mr = M.rec[R];
-  const int dy = mr.dir / 3 - 1, dx = mr.dir % 3 - 1;
-  Move32 m;
-  make_move32(m, mr.ox0, mr.oy0, mr.ox1, mr.oy1, dx, dy);
-  const float qc = q * P.qc_scale;
-  float acc[25];
+constexpr int XC = WM_MG;       // movers per cell whose deposit inputs stay in LDS (all)
+
+// The push kernel's LDS (file scope, so device functions address it directly).
+__shared__ float T[6 * 3 * TW];                   // cell-centred fields of the chunk
+__shared__ float4 CF4[CHUNK * CFS / 4];           // per-cell gather coefficients
+__shared__ double cellbuf[CHUNK * NACC];          // per-cell stencil sums
+__shared__ int xl[CHUNK * WM_MG];                 // movers: species << 28 | dir << 20 | slot
+__shared__ float4 xpl[CHUNK * XC];                // movers' offsets before and after
+__shared__ float xql[CHUNK * XC];                 // movers' charge times v_z
+__shared__ int gcnt[CHUNK];                       // movers per cell
+__shared__ int scnt[WM_MAXSP * CHUNK];            // particles per (species, cell)
+__shared__ float sqc[WM_MAXSP];                   // q delx / delt per species (an LDS read:
+                                                  // a per-lane index into the kernel arguments
+                                                  // is a memory load, and its wait would also
+                                                  // wait for the stores in flight)
+
+// Shape weight at index a (0..4) of the 5-point stencil for offset o whose
+// 3-point support starts at index 1 + inc (inc = cell change, -1..1): the
+// values make_move32 places.
+__device__ inline float shape5(float sm, float s0, float sp, int inc, int a) {
+  const int r = a - 1 - inc;
+  return r == 0 ? sm : (r == 1 ? s0 : (r == 2 ? sp : 0.0f));
+}
+
+// The deposit of a cell's movers, one (component, row) of the 5x5 stencil per
+// thread: task 0..4 = Jx row b (values a = 0..3), 5..8 = Jy column b
+// (a = 0..4), 9..13 = Jz row b (a = 0..4), each value summed over the movers
+// in list order in FP64, as the FP32 stencil of stencil32 gives it.
+__device__ inline void mover_row(double *cb, int task, int nm, const float4 *xp,
+                                 const float *xq, const int *xg, const PP32 &P) {
+  const int comp = task < 5 ? 0 : (task < 9 ? 1 : 2);
+  const int b = task - (comp == 0 ? 0 : (comp == 1 ? 5 : 9));
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < nm; ++j) {
+    const float4 v = xp[j];
+    const float ox0 = v.x, oy0 = v.y, ox1 = v.z, oy1 = v.w, qv = xq[j];
+    const int dir = (xg[j] >> 20) & 0xF;
+    const int dy = dir / 3 - 1, dx = dir % 3 - 1;
+    const float qc = sqc[xg[j] >> 28];
+    float xm, x0, xp_, ym, y0, yp, nxm, nx0, nxp, nym, ny0, nyp;
+    shapef(ox0 - 0.5f, xm, x0, xp_);
+    shapef(oy0 - 0.5f, ym, y0, yp);
+    shapef(ox1 - 0.5f, nxm, nx0, nxp);
+    shapef(oy1 - 0.5f, nym, ny0, nyp);
+    float s0x[5], dsx[5];
 #pragma unroll
-  for (int v = 0; v < 25; ++v) acc[v] = 0.0f;
+    for (int k = 0; k < 5; ++k) {
+      s0x[k] = shape5(xm, x0, xp_, 0, k);
+      dsx[k] = shape5(nxm, nx0, nxp, dx, k) - s0x[k];
+    }
+    if (comp == 1) {
+      // column b of Jy: w_b = -qc * sum_{b' <= b} dsy[b']
+      float w = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k <= b) w -= qc * (shape5(nym, ny0, nyp, dy, k) - shape5(ym, y0, yp, 0, k));
+#pragma unroll
+      for (int a = 0; a < 5; ++a) acc[a] += (double)((s0x[a] + 0.5f * dsx[a]) * w);
+    } else {
+      const float s0y = shape5(ym, y0, yp, 0, b);
+      const float dsy = shape5(nym, ny0, nyp, dy, b) - s0y;
+      if (comp == 0) {
+        const float Y = s0y + 0.5f * dsy;
+        float w = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          w -= qc * dsx[a];
+          acc[a] += (double)(w * Y);
+        }
+      } else {
+        const float third = 1.0f / 3.0f;
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+          const float za = qv * (s0x[a] + 0.5f * dsx[a]);
+          const float zb = qv * (0.5f * s0x[a] + third * dsx[a]);
+          acc[a] += (double)(za * s0y + zb * dsy);
+        }
+      }
+    }
+  }
   if (comp == 0) {
-    stencil32<0>(acc, m, qc, 0.0f);
 #pragma unroll
-    for (int v = 0; v < 20; ++v) out[v] = acc[v];
+    for (int a = 0; a < 4; ++a) cb[a * 5 + b] += acc[a];
   } else if (comp == 1) {
-    stencil32<1>(acc, m, qc, 0.0f);
 #pragma unroll
-    for (int v = 0; v < 20; ++v) out[20 + v] = acc[v];
+    for (int a = 0; a < 5; ++a) cb[20 + a * 4 + b] += acc[a];
   } else {
-    const float ux = mr.ux, uy = mr.uy, uz = mr.uz;
-    const float qv = q * (uz * __builtin_amdgcn_rsqf(1.0f + (ux * ux + uy * uy + uz * uz) * P.inv_c2));
-    stencil32<2>(acc, m, qc, qv);
 #pragma unroll
-    for (int v = 0; v < 25; ++v) out[40 + v] = acc[v];
+    for (int a = 0; a < 5; ++a) cb[40 + a * 5 + b] += acc[a];
   }
 }
 
-// Push + deposit in place on the cell bins.  A particle that stays in its
-// cell is written back to its slot; one that leaves goes to the cell's mover
-// region (new state, old offsets for the deposit, ID) and leaves a hole.
-// After the deposit of the movers, each bin closes its holes with its last
-// particles (the k-th hole below the new count takes the k-th remaining
-// particle above it), and records how many stayed and how many left in each
-// of the 8 directions; wm_k_insert_movers then appends the movers to their
-// destination bins.
+constexpr int PD = 5;           // windows of a bin held in registers (80 particles)
+
+// The first PD windows of one bin, loaded in one burst.
+struct Win {
+  float ox[PD], oy[PD], ux[PD], uy[PD], uz[PD];
+};
+
+struct Spc {                    // one species of one cell
+  int s, n, CB;
+  size_t base0;                 // first slot of the species' bin of the chunk's first cell
+  float q, fac1, txxx, fac2, qc;
+};
+// First slot of the group's bin: from the workgroup-uniform base0, so that no
+// 64-bit per-lane base stays live through the windows.
+__device__ inline size_t spc_base(const Spc &S) {
+  return S.base0 + (size_t)((threadIdx.x / GROUP) * S.CB);
+}
+
+__device__ inline void make_spc(Spc &S, int s, int n, int jl, int il, int CB, const PP32 &P) {
+  S.s = s;
+  S.n = n;
+  S.CB = CB;
+  S.base0 = bin_of(s, jl, il - (int)(threadIdx.x / GROUP), P.ny, P.nx) * (size_t)CB;
+  S.q = P.q[s];
+  S.fac1 = P.q[s] / P.r[s] * 0.5f * P.delt;
+  S.txxx = S.fac1 * S.fac1;
+  S.fac2 = P.q[s] * P.delt / P.r[s];
+  S.qc = P.q[s] * P.qc_scale;
+}
+
+// Window w of a bin into the register slot w.  The load is unconditional
+// (a lane past the bin's count re-reads its last particle, or slot 0 of an
+// empty bin): a conditional load would first clear the registers, and that
+// write makes the compiler wait for every load still in flight.
+__device__ inline void load_slot(Win &W, int w, const Rec32 &A, const Spc &S, int sub) {
+  const size_t p = spc_base(S) + min(w * GROUP + sub, max(S.n - 1, 0));
+  W.ox[w] = A.ox[p];
+  W.oy[w] = A.oy[p];
+  W.ux[w] = A.ux[p];
+  W.uy[w] = A.uy[p];
+  W.uz[w] = A.uz[p];
+}
+
+// Per-lane state of one group's pass over its cell; the group's indices
+// follow from threadIdx and blockIdx.
+struct Lane {
+  int xcnt;                     // movers of the cell so far (all species)
+  bool ok;
+};
+__device__ inline int lane_g() { return threadIdx.x / GROUP; }
+__device__ inline int lane_sub() { return threadIdx.x % GROUP; }
+__device__ inline size_t lane_reg(const PP32 &P) {   // first record of the cell's mover region
+  return (((size_t)blockIdx.y * P.nchunk + blockIdx.x) * CHUNK + lane_g()) * WM_MG;
+}
+
+// One window of GROUP particles (slot k = w * GROUP + sub of the bin); the
+// group executes it together.  Every particle's new state is written back to
+// its slot (a mover's slot becomes a hole, so what it holds does not matter;
+// the window's stores stay contiguous).  A stayer's core stencil is
+// accumulated; a mover goes to the cell's mover region (all but its ID) and
+// to the LDS list, with the inputs of its deposit.
+__device__ inline void push_window(Lane &L, float *core, int k, float ox0, float oy0, float cux,
+                                   float cuy, float cuz, const Spc &S, const PP32 &P, const Rec32 &A,
+                                   const Movers &M) {
+  // Every lane pushes and stores: a lane past the bin's count holds its last
+  // particle (or slot 0 of an empty bin) and writes back the same values as
+  // that particle's lane, to the same slot.  Unconditional stores keep the
+  // compiler's count of memory operations in flight exact, so its waits for
+  // the register windows do not also wait for recent stores.
+  const bool on = k < S.n;
+  __asm__ volatile("" ::: "memory");               // gather coefficients re-read from LDS
+  const Push32 o = boris32(CF4 + lane_g() * (CFS / 4), ox0, oy0, cux, cuy, cuz, P, S.fac1, S.txxx,
+                           S.fac2);
+  float ox1, oy1;
+  const int dx = rebase(o.ox, ox1);
+  const int dy = rebase(o.oy, oy1);
+  L.ok = L.ok && (!on || (fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f));
+  const bool cross = on && (dx | dy) != 0;
+  int kk = min(k, max(S.n - 1, 0));
+  __asm__ volatile("" : "+v"(kk));                 // store addresses formed here, not earlier
+  const size_t p = spc_base(S) + kk;
+  A.ox[p] = ox1;
+  A.oy[p] = oy1;
+  A.ux[p] = o.ux;
+  A.uy[p] = o.uy;
+  A.uz[p] = o.uz;
+  if (on && !cross) {
+    const float qv =
+        S.q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
+    core32(core, ox0, oy0, ox1, oy1, S.qc, qv);
+  }
+  // movers in (species, slot) order
+  const unsigned gm = (unsigned)(__ballot(cross) >> (GROUP * (lane_g() & 3))) & 0xFFFFu;
+  if (cross) {
+    const int pos = L.xcnt + __popc(gm & ((1u << lane_sub()) - 1u));
+    if (pos < WM_MG) {
+      int ci = (int)blockIdx.x * CHUNK + lane_g() + dx;
+      ci = ci < 0 ? ci + P.nx : (ci >= P.nx ? ci - P.nx : ci);   // periodic x
+      int cj = (int)blockIdx.y + dy;
+      cj = cj < 0 ? cj + P.ny : (cj >= P.ny ? cj - P.ny : cj);   // periodic y
+      const int dir = (dy + 1) * 3 + (dx + 1);
+      float4 *rp = (float4 *)(M.rec + lane_reg(P) + pos);
+      rp[0] = make_float4(__int_as_float((int)bin_of(S.s, cj, ci, P.ny, P.nx)), __int_as_float(dir),
+                          ox0, oy0);
+      rp[1] = make_float4(ox1, oy1, o.ux, o.uy);
+      ((float2 *)rp)[4] = make_float2(o.uz, __int_as_float(k));
+      xl[lane_g() * WM_MG + pos] = (S.s << 28) | (dir << 20) | k;
+      {
+        xpl[lane_g() * XC + pos] = make_float4(ox0, oy0, ox1, oy1);
+        xql[lane_g() * XC + pos] = S.q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
+      }
+    }
+  }
+  L.xcnt += __popc(gm);
+  __builtin_amdgcn_sched_barrier(0);               // windows are not interleaved (registers)
+}
+
+// All windows of one species of the group's cell: the PD register slots
+// (all of them, whatever the count), each refilled with the same window of
+// the next species (when there is one) as soon as it is pushed; then the rest
+// of bins above PD * GROUP particles, loaded directly.
+template <bool MORE>
+__device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, const Spc &Sn,
+                                    const PP32 &P, const Rec32 &A, const Movers &M) {
+#pragma unroll
+  for (int w = 0; w < PD; ++w) {
+    push_window(L, core, w * GROUP + lane_sub(), W.ox[w], W.oy[w], W.ux[w], W.uy[w], W.uz[w], S, P, A,
+                M);
+    if (MORE) load_slot(W, w, A, Sn, lane_sub());
+  }
+  for (int w = PD * GROUP; w < S.n; w += GROUP) {
+    const int k = w + lane_sub();
+    const size_t p = spc_base(S) + min(k, S.n - 1);
+    push_window(L, core, k, A.ox[p], A.oy[p], A.ux[p], A.uy[p], A.uz[p], S, P, A, M);
+    // nothing of this rare path stays in flight: otherwise the compiler, not
+    // knowing whether the loop ran, would drain the register windows after it
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+}
+
+// Push + deposit in place on the cell bins.  Every particle's new state is
+// written to its slot; one that leaves its cell also goes to the cell's mover
+// region (new state, old offsets, source slot; its ID follows in
+// wm_k_insert_movers) and its slot becomes a hole.  The movers' deposit is
+// summed per cell in list order.  Each bin records how many of its particles
+// left, in total and in each of the 8 directions; wm_k_insert_movers then
+// fills the holes and appends the movers to their destination bins.
+// PAIR: exactly two species (straight-line code, so the compiler keeps the
+// second species' windows in flight through the first without spilling).
+template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     PP32 P, int CB, const int *__restrict__ cnt, const float *__restrict__ tmp, int TX,
-    size_t tps, Rec32 A, int64_t *__restrict__ ids, Movers M, int *__restrict__ ncnt,
+    size_t tps, Rec32 A, Movers M, int *__restrict__ mout,
     int *__restrict__ mcnt, double *__restrict__ part, int *__restrict__ derr) {
-  __shared__ float T[6 * 3 * TW];
-  __shared__ float4 CF4[CHUNK * CFS / 4];
-  __shared__ double cellbuf[CHUNK * NACC];
-  __shared__ int xl[CHUNK * WM_MG];                  // species << 28 | dir << 20 | slot
-  __shared__ int pre[CHUNK + 1];                     // prefix of the mover counts
-  __shared__ int gcnt[CHUNK];
-  __shared__ float stg[XR * NACC];                   // stencils of one round of movers
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
   const int ncell = min(CHUNK, P.nx - i0);
   const size_t rbase = ((size_t)jl * P.nchunk + chunk) * CHUNK;   // first region
-  for (int t = threadIdx.x; t < 6 * 3 * TW; t += blockDim.x) {
+  if (threadIdx.x < WM_MAXSP) sqc[threadIdx.x] = P.q[threadIdx.x] * P.qc_scale;
+  for (int t = threadIdx.x; t < P.nsp * CHUNK; t += NT) {
+    const int c = t % CHUNK, s = t / CHUNK;
+    scnt[t] = c < ncell ? min(CB, cnt[bin_of(s, jl, i0 + c, P.ny, P.nx)]) : 0;
+  }
+  for (int t = threadIdx.x; t < 6 * 3 * TW; t += NT) {
     const int col = t % TW, b = (t / TW) % 3, k = t / (3 * TW);
     T[t] = col < ncell + 2 ? tmp[(size_t)k * tps + (size_t)(i0 + col) + (size_t)TX * (jl + b)] : 0.0f;
   }
-  for (int e = threadIdx.x; e < CHUNK * NACC; e += blockDim.x) cellbuf[e] = 0.0;
+  for (int e = threadIdx.x; e < CHUNK * NACC; e += NT) cellbuf[e] = 0.0;
   __syncthreads();
+  const int g = threadIdx.x / GROUP;
+  const int sub = threadIdx.x % GROUP;
+  const int il = i0 + g;                             // this group's cell
+  // the first species' windows are in flight during the coefficient set-up
+  Win win;
+  Spc sc{}, sn{};
+  if (g < ncell) {
+    make_spc(sc, 0, scnt[g], jl, il, CB, P);
+#pragma unroll
+    for (int w = 0; w < PD; ++w) load_slot(win, w, A, sc, sub);
+  }
   if (threadIdx.x < CHUNK * 6) {
     const int c = threadIdx.x / 6, k = threadIdx.x % 6;
     float *cf = (float *)CF4 + c * CFS;
     cell_coeffs(cf, T, c, k);
-    if (k == 0) cf[54] = cf[55] = 0.0f;
+    cf[12 * k + 9] = cf[12 * k + 10] = cf[12 * k + 11] = 0.0f;
   }
   __syncthreads();
-  const int g = threadIdx.x / GROUP;
-  const int sub = threadIdx.x % GROUP;
-  const int gshift = GROUP * (g & 3);                // the group's lanes in the wave ballot
-  const int il = i0 + g;                             // this group's cell
   double *cb = cellbuf + g * NACC;
   int *xg = xl + g * WM_MG;
-  const size_t reg = (rbase + g) * WM_MG;            // this cell's mover region
+  Lane L;
+  L.xcnt = 0;
+  L.ok = true;
   float core[27];
 #pragma unroll
   for (int k = 0; k < 27; ++k) core[k] = 0.0f;
-  bool ok = true;
-  int xcnt = 0;                                      // movers of the cell (all species)
   if (g < ncell) {
-    for (int s = 0; s < P.nsp; ++s) {
-      const size_t bin = bin_of(s, jl, il, P.ny, P.nx);
-      const int n = min(CB, cnt[bin]);
-      const size_t base = bin * (size_t)CB;
-      // bin bases: 32-bit indices below
-      float *Aox = A.ox + base, *Aoy = A.oy + base, *Aux = A.ux + base;
-      float *Auy = A.uy + base, *Auz = A.uz + base;
-      const float q = P.q[s], r = P.r[s];
-      const float fac1 = q / r * 0.5f * P.delt;
-      const float txxx = fac1 * fac1;
-      const float fac2 = q * P.delt / r;
-      const float qc = q * P.qc_scale;
-      float nox = 0, noy = 0, nux = 0, nuy = 0, nuz = 0;
-      if (sub < n) {
-        nox = Aox[sub]; noy = Aoy[sub]; nux = Aux[sub]; nuy = Auy[sub]; nuz = Auz[sub];
+    // the next species' windows load while this one is pushed
+    if (PAIR) {
+      make_spc(sn, 1, scnt[CHUNK + g], jl, il, CB, P);
+      push_species<true>(L, core, win, sc, sn, P, A, M);
+      push_species<false>(L, core, win, sn, sn, P, A, M);
+    } else {
+      for (int s = 0; s + 1 < P.nsp; ++s) {
+        make_spc(sn, s + 1, scnt[(s + 1) * CHUNK + g], jl, il, CB, P);
+        push_species<true>(L, core, win, sc, sn, P, A, M);
+        sc = sn;
       }
-      // the window loop is uniform over the group, so every lane keeps the
-      // same mover count
-      for (int w = 0; w < n; w += GROUP) {
-        const int k = w + sub;
-        const float ox0 = nox, oy0 = noy, cux = nux, cuy = nuy, cuz = nuz;
-        if (k + GROUP < n) {
-          const int kn = k + GROUP;
-          nox = Aox[kn]; noy = Aoy[kn]; nux = Aux[kn]; nuy = Auy[kn]; nuz = Auz[kn];
-        }
-        const bool on = k < n;
-        bool cross = false;
-        int dx = 0, dy = 0;
-        float ox1 = 0.0f, oy1 = 0.0f;
-        Push32 o{};
-        if (on) {
-          __asm__ volatile("" ::: "memory");
-          o = boris32(CF4 + g * (CFS / 4), ox0, oy0, cux, cuy, cuz, P, fac1, txxx, fac2);
-          dx = rebase(o.ox, ox1);
-          dy = rebase(o.oy, oy1);
-          ok = ok && fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f;
-          cross = (dx | dy) != 0;
-          if (!cross) {
-            Aox[k] = ox1;
-            Aoy[k] = oy1;
-            Aux[k] = o.ux;
-            Auy[k] = o.uy;
-            Auz[k] = o.uz;
-            const float qv =
-                q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
-            core32(core, ox0, oy0, ox1, oy1, qc, qv);
-          }
-        }
-        // movers in (species, slot) order
-        const unsigned gm = (unsigned)(__ballot(cross) >> gshift) & 0xFFFFu;
-        if (cross) {
-          const int pos = xcnt + __popc(gm & ((1u << sub) - 1u));
-          if (pos < WM_MG) {
-            const size_t R = reg + pos;
-            int ci = il + dx;
-            ci = ci < 0 ? ci + P.nx : (ci >= P.nx ? ci - P.nx : ci);   // periodic x
-            int cj = jl + dy;
-            cj = cj < 0 ? cj + P.ny : (cj >= P.ny ? cj - P.ny : cj);   // periodic y
-            MoverRec mr;
-            mr.dst = (int)bin_of(s, cj, ci, P.ny, P.nx);
-            mr.dir = (dy + 1) * 3 + (dx + 1);
-            mr.ox0 = ox0;
-            mr.oy0 = oy0;
-            mr.ox1 = ox1;
-            mr.oy1 = oy1;
-            mr.ux = o.ux;
-            mr.uy = o.uy;
-            mr.uz = o.uz;
-            mr.pad = 0;
-            mr.id = ids[base + k];
-            M.rec[R] = mr;
-            xg[pos] = (s << 28) | (((dy + 1) * 3 + (dx + 1)) << 20) | k;
-          }
-        }
-        xcnt += __popc(gm);
-      }
+      push_species<false>(L, core, win, sc, sn, P, A, M);
     }
     // stay-in-cell stencils of every species, summed over the group once
 #pragma unroll
@@ -410,6 +564,8 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
       if (sub == 0) cb[core_slot(c)] += (double)v;
     }
   }
+  const bool ok = L.ok;
+  const int xcnt = L.xcnt;
   if (!ok) atomicOr(derr, DERR_JUMP);
   if (xcnt > WM_MG && sub == 0) atomicOr(derr, DERR_MOVER_OVER);
   if (sub == 0 && g < CHUNK) {
@@ -417,79 +573,34 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     M.n[rbase + g] = gcnt[g];
   }
   __syncthreads();                                   // also orders the movers' stores
-  if (threadIdx.x == 0) {
-    pre[0] = 0;
-    for (int c = 0; c < CHUNK; ++c) pre[c + 1] = pre[c] + gcnt[c];
+  const int gm = g < ncell ? gcnt[g] : 0;
+  // the movers' deposit: thread t takes (cell t / 14, row t % 14)
+  if (threadIdx.x < CHUNK * 14) {
+    const int cg = threadIdx.x / 14;
+    if (cg < ncell)
+      mover_row(cellbuf + cg * NACC, threadIdx.x % 14, gcnt[cg], xpl + cg * XC, xql + cg * XC,
+                xl + cg * WM_MG, P);
   }
-  __syncthreads();
-  // Movers of the whole workgroup: wave w computes stencil component w of
-  // XR movers at a time (one mover per lane), then each (cell, value) pair
-  // adds its cell's movers in list order.
-  const int nxs = pre[CHUNK];
-  for (int r0 = 0; r0 < nxs; r0 += XR) {
-    const int comp = threadIdx.x / XR, e = r0 + threadIdx.x % XR;
-    if (comp < 3 && e < nxs) {
-      int gg = 0;
-      while (pre[gg + 1] <= e) ++gg;
-      const int j = e - pre[gg];
-      const int s = xl[gg * WM_MG + j] >> 28;
-      mover_comp(stg + (e - r0) * NACC, comp, M, (rbase + gg) * WM_MG + j, P.q[s], P);
-    }
-    __syncthreads();
-    for (int pr = threadIdx.x; pr < ncell * NACC; pr += blockDim.x) {
-      const int gg = pr / NACC, v = pr % NACC;
-      const int lo = max(pre[gg], r0), hi = min(pre[gg + 1], r0 + XR);
-      double a = cellbuf[pr];
-      for (int e2 = lo; e2 < hi; ++e2) a += (double)stg[(e2 - r0) * NACC + v];
-      cellbuf[pr] = a;
-    }
-    __syncthreads();
-  }
-  // close the holes and record the counts, one bin per (group, species)
+  // per bin: movers in total and per direction
   if (g < ncell) {
-    const int m_all = gcnt[g];
     int j0 = 0;
     for (int s = 0; s < P.nsp; ++s) {
       int m = 0;
-      while (j0 + m < m_all && (xg[j0 + m] >> 28) == s) ++m;
+      while (j0 + m < gm && (xg[j0 + m] >> 28) == s) ++m;
       const size_t bin = bin_of(s, jl, il, P.ny, P.nx);
-      const int n = min(CB, cnt[bin]);
-      const int n1 = n - m;
-      const size_t base = bin * (size_t)CB;
-      int a = 0;                                     // holes below n1
-      while (a < m && (xg[j0 + a] & 0xFFFFF) < n1) ++a;
-      for (int t = sub; t < a; t += GROUP) {
-        // the t-th particle of [n1, n) that is not a hole
-        int idx = a, c = 0, from = n1;
-        for (; from < n; ++from) {
-          if (idx < m && (xg[j0 + idx] & 0xFFFFF) == from) {
-            ++idx;
-            continue;
-          }
-          if (c == t) break;
-          ++c;
-        }
-        if (from >= n) break;                          // (cannot happen: a tail particles)
-        const size_t h = base + (xg[j0 + t] & 0xFFFFF), f = base + from;
-        A.ox[h] = A.ox[f];
-        A.oy[h] = A.oy[f];
-        A.ux[h] = A.ux[f];
-        A.uy[h] = A.uy[f];
-        A.uz[h] = A.uz[f];
-        ids[h] = ids[f];
-      }
       if (sub < 8) {
         const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
         int c = 0;
         for (int q = 0; q < m; ++q) c += ((xg[j0 + q] >> 20) & 0xF) == d;
         mcnt[bin * 8 + sub] = c;
       }
-      if (sub == 0) ncnt[bin] = n1;
+      if (sub == 0) mout[bin] = m;
       j0 += m;
     }
   }
+  __syncthreads();
   double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;
-  for (int e = threadIdx.x; e < TILE; e += blockDim.x) {
+  for (int e = threadIdx.x; e < TILE; e += NT) {
     const int comp = e % 3;
     const int t = (e / 3) % DW;
     const int b = e / (3 * DW);
@@ -651,9 +762,9 @@ int wm_k_push_deposit32(wm_ctx *ctx) {
   PP32 P = make_pp32(ctx);
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
-  hipLaunchKernelGGL(k_push_deposit32, dim3(P.nchunk, P.ny), dim3(256), 0, ctx->stream, P,
-                     ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->b32[0], ctx->id_bins,
-                     ctx->mv, ctx->bin_ncnt, ctx->bin_mcnt, ctx->dep_part, ctx->derr);
+  hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
+                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->b32[0], ctx->mv,
+                     ctx->bin_mout, ctx->bin_mcnt, ctx->dep_part, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

@@ -267,55 +267,112 @@ __device__ inline size_t source_bin(size_t d, int dir, int nx, int ny) {
   return (s * ny + sj2) * nx + si;
 }
 
-// Bin counts after the migration, and where each direction's arrivals
-// start: bin d keeps its ncnt[d] stayers, then receives the arrivals of
-// directions 0, 1, 2, 3, 5, 6, 7, 8 in turn (aoff[d * 8 + k] = first slot of
-// the k-th direction), each in the order its source bin listed them.
-__global__ void k_bin_counts(int nx, int ny, int CB, size_t nbin, const int *__restrict__ ncnt,
-                             const int *__restrict__ mcnt, int *__restrict__ aoff,
-                             int *__restrict__ cnt, int *derr) {
-  for (size_t d = blockIdx.x * (size_t)blockDim.x + threadIdx.x; d < nbin;
+struct Mig {                    // bins and mover regions of one migration
+  int nx, ny, nxr, nsp, CB;     // nxr: regions per row (cells rounded up to the push chunk)
+  size_t nregion, nbin;
+  const int *mout, *mcnt;       // per bin: movers out in total, and per direction (8)
+  int *aoff, *nold, *cnt;
+};
+
+// First record of bin d's movers in its cell's region (the region lists the
+// cell's movers by species, then slot).
+__device__ inline size_t first_record(const Mig &G, size_t d) {
+  const int i = (int)(d % G.nx);
+  const size_t sj = d / G.nx;
+  const int jl = (int)(sj % G.ny), s = (int)(sj / G.ny);
+  size_t r = ((size_t)jl * G.nxr + i) * WM_MG;
+  for (int q = 0; q < s; ++q) r += G.mout[((size_t)q * G.ny + jl) * G.nx + i];
+  return r;
+}
+
+// Bin counts after the migration, one thread per bin d: it held nold[d]
+// particles and lost mout[d]; its arrivals come from directions 0, 1, 2, 3,
+// 5, 6, 7, 8 in turn, each in the order its source bin listed them
+// (aoff[d * 8 + k] = index, among all of d's arrivals, of the k-th
+// direction's first one).  It also copies the IDs of d's movers into their
+// records, before any arrival can take their slots.
+__global__ void k_bin_counts(Mig G, Movers M, const int64_t *__restrict__ ids, int *derr) {
+  for (size_t d = blockIdx.x * (size_t)blockDim.x + threadIdx.x; d < G.nbin;
        d += (size_t)gridDim.x * blockDim.x) {
-    int c = ncnt[d];
+    int a = 0;
     for (int dd = 0; dd < 8; ++dd) {
       const int dirp = dd < 4 ? dd : dd + 1;
-      aoff[d * 8 + dd] = c;
-      c += mcnt[source_bin(d, dirp, nx, ny) * 8 + dd];
+      G.aoff[d * 8 + dd] = a;
+      a += G.mcnt[source_bin(d, dirp, G.nx, G.ny) * 8 + dd];
     }
-    if (c > CB) {
+    const int n = min(G.cnt[d], G.CB), m = G.mout[d];
+    G.nold[d] = n;
+    int c = n - m + a;
+    if (c > G.CB) {
       atomicOr(derr, DERR_NP_OVER);
-      c = CB;
+      c = G.CB;
     }
-    cnt[d] = c;
+    G.cnt[d] = c;
+    if (m > 0) {
+      const size_t r0 = first_record(G, d);
+      for (int j = 0; j < m; ++j) M.rec[r0 + j].id = ids[d * G.CB + M.rec[r0 + j].slot];
+    }
   }
 }
 
-// Movers -> destination bins, one thread per mover region (source cell).
-__global__ void k_insert_movers(int CB, size_t nregion, Movers M, const int *__restrict__ aoff,
-                                Rec32 A, int64_t *__restrict__ ids, int *derr) {
-  for (size_t region = blockIdx.x * (size_t)blockDim.x + threadIdx.x; region < nregion;
-       region += (size_t)gridDim.x * blockDim.x) {
+// Movers -> destination bins, 8 lanes per mover region (source cell), lane l
+// taking the cell's movers l, l + 8, ...  Bin d's k-th arrival takes d's k-th
+// hole (the slots its own movers left, in ascending order) while there are
+// holes, then the slots after its old count.  When d receives fewer arrivals
+// than it lost, the holes left below its new count take, in order, the
+// particles above it; lane s of d's own region moves them for species s
+// (arrivals only fill lower holes, so nothing overlaps).
+constexpr int ILANES = 8;
+__global__ void k_insert_movers(Mig G, Movers M, Rec32 A, int64_t *__restrict__ ids, int *derr) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < G.nregion * ILANES;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t region = t / ILANES;
+    const int lane = (int)(t % ILANES);
+    const MoverRec *rec = M.rec + region * WM_MG;
     const int nm = min(M.n[region], WM_MG);
-    int dprev[16];                                 // destinations of the first movers
-    for (int j = 0; j < nm; ++j) {
-      const MoverRec mr = M.rec[region * WM_MG + j];
+    for (int j = lane; j < nm; j += ILANES) {
+      const MoverRec mr = rec[j];
       int rank = 0;                                // earlier movers to the same bin
-      for (int q = 0; q < j; ++q)
-        rank += (q < 16 ? dprev[q] : M.rec[region * WM_MG + q].dst) == mr.dst;
-      if (j < 16) dprev[j] = mr.dst;
+      for (int q = 0; q < j; ++q) rank += rec[q].dst == mr.dst;
       const int d8 = mr.dir < 4 ? mr.dir : mr.dir - 1;
-      const int off = aoff[(size_t)mr.dst * 8 + d8] + rank;
-      if (off >= CB) {
+      const int a = G.aoff[(size_t)mr.dst * 8 + d8] + rank;
+      const int md = G.mout[mr.dst];
+      const int slot = a < md ? M.rec[first_record(G, mr.dst) + a].slot : G.nold[mr.dst] + (a - md);
+      if (slot >= G.CB) {
         atomicOr(derr, DERR_NP_OVER);
         continue;
       }
-      const size_t p = (size_t)mr.dst * CB + off;
+      const size_t p = (size_t)mr.dst * G.CB + slot;
       A.ox[p] = mr.ox1;
       A.oy[p] = mr.oy1;
       A.ux[p] = mr.ux;
       A.uy[p] = mr.uy;
       A.uz[p] = mr.uz;
       ids[p] = mr.id;
+    }
+    const int i = (int)(region % G.nxr), jl = (int)(region / G.nxr);
+    if (lane >= G.nsp || i >= G.nx) continue;
+    const size_t b = ((size_t)lane * G.ny + jl) * G.nx + i;
+    const int n = G.nold[b], m = G.mout[b], n1 = n - m, arr = G.cnt[b] - n1;
+    if (arr >= m) continue;
+    const size_t r = first_record(G, b);
+    const int nnew = n1 + arr;
+    int hb = arr;                                  // holes below the new count
+    while (hb < m && M.rec[r + hb].slot < nnew) ++hb;
+    int k = arr, h = hb;
+    for (int from = nnew; from < n && k < hb; ++from) {
+      if (h < m && M.rec[r + h].slot == from) {    // a hole above the new count
+        ++h;
+        continue;
+      }
+      const size_t dst = b * G.CB + M.rec[r + k].slot, src = b * G.CB + from;
+      A.ox[dst] = A.ox[src];
+      A.oy[dst] = A.oy[src];
+      A.ux[dst] = A.ux[src];
+      A.uy[dst] = A.uy[src];
+      A.uz[dst] = A.uz[src];
+      ids[dst] = ids[src];
+      ++k;
     }
   }
 }
@@ -350,14 +407,24 @@ int wm_k_bins_to_dense(wm_ctx *ctx) {
 
 int wm_k_insert_movers(wm_ctx *ctx) {
   const int nx = ctx->g.nx, ny = ctx->g.ny;
-  const size_t nreg = (size_t)ctx->nregion;
-  const size_t nbin = (size_t)ctx->g.nsp * ny * nx;
-  hipLaunchKernelGGL(k_bin_counts, dim3((unsigned)std::min<size_t>((nbin + 255) / 256, 65536)),
-                     dim3(256), 0, ctx->stream, nx, ny, ctx->CB, nbin, ctx->bin_ncnt, ctx->bin_mcnt,
-                     ctx->bin_aoff, ctx->bin_cnt, ctx->derr);
-  hipLaunchKernelGGL(k_insert_movers, dim3((unsigned)std::min<size_t>((nreg + 255) / 256, 65536)),
-                     dim3(256), 0, ctx->stream, ctx->CB, nreg, ctx->mv, ctx->bin_aoff,
-                     ctx->b32[0], ctx->id_bins, ctx->derr);
+  Mig G;
+  G.nx = nx;
+  G.ny = ny;
+  G.nregion = (size_t)ctx->nregion;
+  G.nxr = (int)(G.nregion / (size_t)ny);
+  G.nsp = ctx->g.nsp;
+  G.CB = ctx->CB;
+  G.nbin = (size_t)ctx->g.nsp * ny * nx;
+  G.mout = ctx->bin_mout;
+  G.mcnt = ctx->bin_mcnt;
+  G.aoff = ctx->bin_aoff;
+  G.nold = ctx->bin_nold;
+  G.cnt = ctx->bin_cnt;
+  hipLaunchKernelGGL(k_bin_counts, dim3((unsigned)std::min<size_t>((G.nbin + 255) / 256, 65536)),
+                     dim3(256), 0, ctx->stream, G, ctx->mv, ctx->id_bins, ctx->derr);
+  hipLaunchKernelGGL(k_insert_movers,
+                     dim3((unsigned)std::min<size_t>((G.nregion * ILANES + 255) / 256, 65536)),
+                     dim3(256), 0, ctx->stream, G, ctx->mv, ctx->b32[0], ctx->id_bins, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
