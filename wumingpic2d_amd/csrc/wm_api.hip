@@ -384,9 +384,8 @@ int wm_alloc_device(wm_ctx *ctx) {
     const size_t ms = (size_t)ctx->nregion * WM_MG;
     Movers &m = ctx->mv;
     WM_TRY(dmalloc(ctx, &m.rec, ms));
+    WM_TRY(dmalloc(ctx, &m.dir, ms));
     WM_TRY(dmalloc(ctx, &m.n, (size_t)ctx->nregion));
-    WM_TRY(dmalloc(ctx, &ctx->bin_aoff, nbin * 8));
-    WM_TRY(dmalloc(ctx, &ctx->bin_nold, nbin));
     WM_HIP(hipMemsetAsync(m.n, 0, sizeof(int) * ctx->nregion, ctx->stream));
     WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
   } else {
@@ -453,9 +452,10 @@ int wm_alloc_device(wm_ctx *ctx) {
 }
 
 int wm_check_derr(wm_ctx *ctx, const char *where) {
-  int h = 0;
-  WM_HIP(hipMemcpyAsync(&h, ctx->derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  int *hp = (int *)ctx->h_pin;
+  WM_HIP(hipMemcpyAsync(hp, ctx->derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
+  const int h = *hp;
   if (!h) return WM_OK;
   WM_HIP(hipMemsetAsync(ctx->derr, 0, sizeof(int), ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
@@ -485,6 +485,14 @@ int wm_create(wm_ctx **out, int device) {
     delete ctx;
     return WM_ERR_NODEV;
   }
+  // small device-to-host read-backs (solver scalars, error flags) land in
+  // pinned memory: a copy to pageable memory is staged by the runtime
+  if (hipHostMalloc(&ctx->h_pin, WM_PIN_BYTES, hipHostMallocDefault) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream2);
+    delete ctx;
+    return WM_ERR_NODEV;
+  }
   for (int k = 0; k < 16; ++k) (void)hipEventCreate(&ctx->ev[k]);
   *out = ctx;
   return WM_OK;
@@ -506,7 +514,7 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
                   ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_mout, ctx->bin_mcnt,
-                  ctx->bin_aoff, ctx->bin_nold, ctx->mv.rec, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->mv.rec, ctx->mv.dir, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -514,6 +522,7 @@ int wm_destroy(wm_ctx *ctx) {
   for (int k = 0; k < 16; ++k) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
   (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
   delete ctx;
   return WM_OK;
 }

@@ -187,21 +187,25 @@ struct CgArgs {
   int ite_max;
 };
 
-// Fixed-order workgroup tree reduction of NV values; result in thread 0.
+// Fixed-order workgroup sum of NV values: a butterfly within each wave, then
+// the CGB / 64 wave totals in order; the totals land in every thread.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double (*sh)[CGB]) {
+  constexpr int NW = CGB / 64;
+  const int wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int s = CGB / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-#pragma unroll
-      for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
-    }
-    __syncthreads();
+  for (int k = 0; k < NV; ++k) {
+    const double w = wm_wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) sh[k][wave] = w;
   }
+  __syncthreads();
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = sh[k][0];
+  for (int k = 0; k < NV; ++k) {
+    double t = sh[k][0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) t += sh[k][q];
+    v[k] = t;
+  }
 }
 
 // Publish this workgroup's NV partial sums (fixed-order tree).
@@ -631,10 +635,11 @@ static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
   // partial pair per 16x16 tile
   CgArgs AT = A;
   AT.nblk = wm_pc_dot_tiles(A.nx, A.ny);
-  FcgScal hs[3];
+  static_assert(3 * sizeof(FcgScal) <= WM_PIN_BYTES, "pinned scratch too small");
+  FcgScal *hs = (FcgScal *)ctx->h_pin;
   int cur = 0;
   for (;;) {
-    WM_HIP(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+    WM_HIP(hipMemcpyAsync(hs, S, 3 * sizeof(FcgScal), hipMemcpyDeviceToHost, st));
     WM_HIP(hipStreamSynchronize(st));
     if (!hs[0].active && !hs[1].active && !hs[2].active) break;
     // two iterations per host check; launches after convergence return at once
@@ -687,10 +692,11 @@ int wm_k_field_solve(wm_ctx *ctx) {
   hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 0, ctx->red, ctx->cg_s);
   WM_HIP(hipGetLastError());
   const bool collect = ctx->solver == WM_SOLVER_CG && wm_pc_collecting(ctx);
-  CgScal hs[3];
+  static_assert(3 * sizeof(CgScal) <= WM_PIN_BYTES, "pinned scratch too small");
+  CgScal *hs = (CgScal *)ctx->h_pin;
   int iters = 0, cur = 0;
   for (;;) {
-    WM_HIP(hipMemcpyAsync(hs, ctx->cg_s, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    WM_HIP(hipMemcpyAsync(hs, ctx->cg_s, 3 * sizeof(CgScal), hipMemcpyDeviceToHost, ctx->stream));
     WM_HIP(hipStreamSynchronize(ctx->stream));
     bool any = false, all = true;
     for (int l = 0; l < 3; ++l) {

@@ -70,6 +70,8 @@ struct FieldGeom {
 };
 
 // CG per-component scalar state (device).
+#define WM_PIN_BYTES 4096
+
 struct CgScal {
   double sum_g, eps, sumr, sum2, av, bv;
   int active, ite, err, pad;
@@ -95,6 +97,7 @@ struct __align__(16) MoverRec {   // 48 B: three 16-byte stores
 };
 struct Movers {
   MoverRec *rec = nullptr;
+  uint8_t *dir = nullptr;      // each record's direction, packed for the migration's scan
   int *n = nullptr;            // movers per region
 };
 
@@ -153,7 +156,7 @@ struct wm_ctx {
   size_t dep_part_elems = 0;
   double *diag = nullptr;  // diagnostics scratch
   int *derr = nullptr;     // device error bits
-  int *h_derr = nullptr;   // pinned mirror
+  void *h_pin = nullptr;   // pinned host scratch for small read-backs (WM_PIN_BYTES)
 
   // host staging (drop-in host-array entry points)
   double *stage = nullptr; size_t stage_elems = 0;
@@ -170,10 +173,8 @@ struct wm_ctx {
   // the reference, used to convert state in and out
   int CB = 0;
   int *bin_cnt = nullptr;        // particles per bin
-  int *bin_mout = nullptr;       // per bin after the push: particles that left
-  int *bin_mcnt = nullptr;       // per bin and direction (8): particles that left
-  int *bin_aoff = nullptr;       // per bin and direction (8): index of the first arrival
-  int *bin_nold = nullptr;       // per bin: particles before the migration
+  int *bin_mout = nullptr;       // per bin after the push: (first record << 8) | movers out
+  uint8_t *bin_mcnt = nullptr;   // per bin and direction (8): particles that left (<= WM_MG)
   int64_t *id_bins = nullptr;
   Movers mv{};
   int nregion = 0;

@@ -453,6 +453,7 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
                           ox0, oy0);
       rp[1] = make_float4(ox1, oy1, o.ux, o.uy);
       ((float2 *)rp)[4] = make_float2(o.uz, __int_as_float(k));
+      M.dir[lane_reg(P) + pos] = (uint8_t)dir;
       xl[lane_g() * WM_MG + pos] = (S.s << 28) | (dir << 20) | k;
       {
         xpl[lane_g() * XC + pos] = make_float4(ox0, oy0, ox1, oy1);
@@ -489,18 +490,19 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
 
 // Push + deposit in place on the cell bins.  Every particle's new state is
 // written to its slot; one that leaves its cell also goes to the cell's mover
-// region (new state, old offsets, source slot; its ID follows in
-// wm_k_insert_movers) and its slot becomes a hole.  The movers' deposit is
-// summed per cell in list order.  Each bin records how many of its particles
-// left, in total and in each of the 8 directions; wm_k_insert_movers then
-// fills the holes and appends the movers to their destination bins.
+// region (new state, old offsets, source slot; its ID at the kernel's end)
+// and its slot becomes a hole.  The movers' deposit is summed per cell in
+// list order.  Each bin records how many of its particles left, in total
+// (with the offset of its first record in the region: (offset << 8) | count)
+// and in each of the 8 directions; wm_k_insert_movers then moves the movers
+// into their destination bins.
 // PAIR: exactly two species (straight-line code, so the compiler keeps the
 // second species' windows in flight through the first without spilling).
 template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     PP32 P, int CB, const int *__restrict__ cnt, const float *__restrict__ tmp, int TX,
-    size_t tps, Rec32 A, Movers M, int *__restrict__ mout,
-    int *__restrict__ mcnt, double *__restrict__ part, int *__restrict__ derr) {
+    size_t tps, Rec32 A, const int64_t *__restrict__ ids, Movers M, int *__restrict__ mout,
+    uint8_t *__restrict__ mcnt, double *__restrict__ part, int *__restrict__ derr) {
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
@@ -592,9 +594,9 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
         const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
         int c = 0;
         for (int q = 0; q < m; ++q) c += ((xg[j0 + q] >> 20) & 0xF) == d;
-        mcnt[bin * 8 + sub] = c;
+        mcnt[bin * 8 + sub] = (uint8_t)c;
       }
-      if (sub == 0) mout[bin] = m;
+      if (sub == 0) mout[bin] = (j0 << 8) | m;       // first record in the region, count
       j0 += m;
     }
   }
@@ -608,6 +610,12 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     const int glo = max(0, t - 4), ghi = min(ncell - 1, t);
     for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(cellbuf + gg * NACC, comp, t - gg, b);
     out[e] = sum;
+  }
+  // last, the movers' IDs into their records (the wait for these loads also
+  // waits for this wave's earlier stores, which are done by now)
+  for (int j = sub; j < gm; j += GROUP) {
+    const int xv = xg[j];
+    M.rec[(rbase + g) * WM_MG + j].id = ids[bin_of(xv >> 28, jl, il, P.ny, P.nx) * CB + (xv & 0xFFFFF)];
   }
 }
 
@@ -763,7 +771,7 @@ int wm_k_push_deposit32(wm_ctx *ctx) {
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
   hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
-                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->b32[0], ctx->mv,
+                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->b32[0], ctx->id_bins, ctx->mv,
                      ctx->bin_mout, ctx->bin_mcnt, ctx->dep_part, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;

@@ -72,12 +72,17 @@ __device__ inline bool gated_off(const FcgScal *gate) {
 //
 // UP: the input is u = BF16(a + up(bc)) (nearest upsampling of the coarser
 // level bc), formed while the tile is staged; uout (optional) receives u.
+//
+// pout (optional, BF16 output only): also the 2x2 mean of the BF16 output,
+// ((x00 + x01) + (x10 + x11)) * 0.25 (x01: the next column, x10: the next
+// row), rounded to BF16, for the next level.
 template <int CIN, int COUT, bool OUTF32, bool UP = false>
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
                                               const bf16 *__restrict__ w, void *outp, int H,
                                               int W, const bf16 *__restrict__ inc = nullptr,
                                               bf16 *__restrict__ uout = nullptr,
-                                              const FcgScal *__restrict__ gate = nullptr) {
+                                              const FcgScal *__restrict__ gate = nullptr,
+                                              bf16 *__restrict__ pout = nullptr) {
   static_assert(CIN == 16 || CIN == 32, "CIN");
   static_assert(COUT % 16 == 0, "COUT");
   if (gated_off(gate)) return;
@@ -89,7 +94,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
   constexpr int CH = CIN / 8;
   for (int e = threadIdx.x; e < (T + 2) * (T + 2) * CH; e += 256) {
     const int c = e % CH, px = e / CH;
-    const int gy = pmod(y0 + px / (T + 2) - 1, H), gx = pmod(x0 + px % (T + 2) - 1, W);
+    const int gy = wm_wrap(y0 + px / (T + 2) - 1, H), gx = wm_wrap(x0 + px % (T + 2) - 1, W);
     const size_t src = ((size_t)gy * W + gx) * CIN + c * 8;
     if (UP) {
       const bf16 *cb = inc + (size_t)b * (H / 2) * (W / 2) * CIN;
@@ -151,6 +156,23 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
         }
       }
     }
+    if (!OUTF32 && pout) {
+      // rows (rr, rr + 1) of this lane, columns (n, n ^ 1) of lanes (lane, lane ^ 1)
+      const int Ho = H / 2, Wo = W / 2;
+#pragma unroll
+      for (int rr = 0; rr < 4; rr += 2) {
+        const int Y = (y0 + wave * 4 + rr) >> 1, X = (x0 + n) >> 1;
+        const bool st = (n & 1) == 0 && Y < Ho && X < Wo;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float a00 = (float)(bf16)acc[rr][j], a10 = (float)(bf16)acc[rr + 1][j];
+          const float a01 = __shfl_xor(a00, 1, 64), a11 = __shfl_xor(a10, 1, 64);
+          if (st)
+            pout[(((size_t)b * Ho + Y) * Wo + X) * COUT + mb * 16 + 4 * g + j] =
+                (bf16)(((a00 + a01) + (a10 + a11)) * 0.25f);
+        }
+      }
+    }
   }
 }
 
@@ -172,7 +194,7 @@ __device__ inline void load_x0(float *xs, const double *__restrict__ r, double i
                                int y0, int H, int W) {
   constexpr int E = T + 2 * HALO;
   for (int e = threadIdx.x; e < E * E; e += 256) {
-    const int gy = pmod(y0 + e / E - HALO, H), gx = pmod(x0 + e % E - HALO, W);
+    const int gy = wm_wrap(y0 + e / E - HALO, H), gx = wm_wrap(x0 + e % E - HALO, W);
     xs[e] = (float)(r[(size_t)gy * W + gx] * is);
   }
 }
@@ -301,17 +323,20 @@ __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, s
   __shared__ float xs[(T + 4) * (T + 4)];
   __shared__ float bs[CE * CE * C0];
   __shared__ float fk[25], fp[256];
-  __shared__ double sh[256];
+  __shared__ double sh[8];
   if (dots && gated_off(S)) return;
   const int b = blockIdx.z;
   const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
   load_x0<2>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
   const int H1 = H / 2, W1 = W / 2;
   const bf16 *b1b = b1 + (size_t)b * H1 * W1 * C0;
-  for (int e = threadIdx.x; e < CE * CE * C0; e += 256) {
-    const int c = e % C0, q = e / C0;
-    const int Y = pmod(y0 / 2 + q / CE - 1, H1), X = pmod(x0 / 2 + q % CE - 1, W1);
-    bs[e] = (float)b1b[((size_t)Y * W1 + X) * C0 + c];
+  for (int e = threadIdx.x; e < CE * CE * (C0 / 8); e += 256) {
+    const int c8 = e % (C0 / 8), q = e / (C0 / 8);
+    const int Y = wm_wrap(y0 / 2 + q / CE - 1, H1), X = wm_wrap(x0 / 2 + q % CE - 1, W1);
+    bf16x8 v;
+    __builtin_memcpy(&v, &b1b[((size_t)Y * W1 + X) * C0 + 8 * c8], 16);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) bs[q * C0 + 8 * c8 + k] = (float)v[k];
   }
   for (int e = threadIdx.x; e < 256; e += 256) fp[e] = fold[FP_OFF + e];
   if (threadIdx.x < 25) fk[threadIdx.x] = fold[FK_OFF + threadIdx.x];
@@ -345,29 +370,17 @@ __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, s
     const double zr = in ? zv * r[(size_t)b * rstride + p] : 0.0;
     const double zq = (in && !first) ? zv * q[(size_t)b * rstride + p] : 0.0;
     const int tiles = gridDim.x * gridDim.y, tile = blockIdx.y * gridDim.x + blockIdx.x;
-    const double t0 = block_sum1(zr, sh);
-    if (threadIdx.x == 0) dots[(size_t)(b * 2 + 0) * tiles + tile] = t0;
+    // fixed order: a butterfly within each wave, then the 4 wave totals
+    const double w0 = wm_wave_sum(zr), w1 = wm_wave_sum(zq);
+    if ((threadIdx.x & 63) == 0) {
+      sh[threadIdx.x >> 6] = w0;
+      sh[4 + (threadIdx.x >> 6)] = w1;
+    }
     __syncthreads();
-    const double t1 = block_sum1(zq, sh);
-    if (threadIdx.x == 0) dots[(size_t)(b * 2 + 1) * tiles + tile] = t1;
-  }
-}
-
-// ------------------------------------------------------------ coarse levels
-// 2x2 mean, ((x00 + x01) + (x10 + x11)) * 0.25, rounded to BF16.
-__global__ void k_pool(const bf16 *__restrict__ in, bf16 *__restrict__ out, int B, int H, int W,
-                       int C, const FcgScal *__restrict__ gate) {
-  if (gated_off(gate)) return;
-  const int Ho = H / 2, Wo = W / 2;
-  const size_t n = (size_t)B * Ho * Wo * C;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int c = e % C;
-    const size_t q = e / C;
-    const int X = q % Wo, Y = (q / Wo) % Ho, b = q / ((size_t)Wo * Ho);
-    const bf16 *p = in + (((size_t)b * H + 2 * Y) * W + 2 * X) * C + c;
-    const float v = (((float)p[0] + (float)p[C]) + ((float)p[(size_t)W * C] + (float)p[(size_t)W * C + C])) * 0.25f;
-    out[e] = (bf16)v;
+    if (threadIdx.x == 0) {
+      dots[(size_t)(b * 2 + 0) * tiles + tile] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+      dots[(size_t)(b * 2 + 1) * tiles + tile] = ((sh[4] + sh[5]) + sh[6]) + sh[7];
+    }
   }
 }
 
@@ -718,9 +731,9 @@ int ew_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
 
 template <int CIN, int COUT, bool F32>
 int conv(wm_ctx *ctx, const bf16 *in, const bf16 *w, void *out, int H, int W,
-         const FcgScal *gate = nullptr) {
+         const FcgScal *gate = nullptr, bf16 *pool = nullptr) {
   hipLaunchKernelGGL((k_conv<CIN, COUT, F32>), tiles(H, W, wm_pc::B), dim3(256), 0, ctx->stream,
-                     in, w, out, H, W, nullptr, nullptr, gate);
+                     in, w, out, H, W, nullptr, nullptr, gate, pool);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -760,12 +773,9 @@ int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const doubl
   const FcgScal *gate = dots ? S : nullptr;
   hipLaunchKernelGGL(k_lift_pool, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
                      pc->d1, H, W, B, gate);
-  WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1, gate)));
-  hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 2, C1))), dim3(256), 0, st, pc->a1, pc->d2, B,
-                     H >> 1, W >> 1, C1, gate);
-  WM_TRY((conv<C1, C2, false>(ctx, pc->d2, wb + loff(L_D2), pc->a2, H >> 2, W >> 2, gate)));
-  hipLaunchKernelGGL(k_pool, dim3(ew_blocks(lvl(pc, 3, C2))), dim3(256), 0, st, pc->a2, pc->d3, B,
-                     H >> 2, W >> 2, C2, gate);
+  // d2 = pool(a1) and d3 = pool(a2) leave the D1 and D2 convolutions with a1, a2
+  WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1, gate, pc->d2)));
+  WM_TRY((conv<C1, C2, false>(ctx, pc->d2, wb + loff(L_D2), pc->a2, H >> 2, W >> 2, gate, pc->d3)));
   WM_TRY((conv<C2, C2, false>(ctx, pc->d3, wb + loff(L_D3), pc->a3, H >> 3, W >> 3, gate)));
   // u2 = a2 + up(a3) and u1 = a1 + up(b2) are formed inside the U2/U1
   // convolutions; training keeps them for the weight gradients

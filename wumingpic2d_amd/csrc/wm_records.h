@@ -51,3 +51,22 @@ struct Rec32 {
 
 // Largest float below 1: offsets are kept in [0, 1) after a cell change.
 __device__ inline float wm_below_one(float v) { return v < 1.0f ? v : 0x1.fffffep-1f; }
+
+// Fixed-order sum over a 64-lane wave: a butterfly on lane xor 32, 16, ...,
+// 1.  Partners add the same two values (IEEE addition commutes), so every
+// lane ends with the bit-identical total.
+__device__ inline double wm_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Periodic index a mod n for a within a few periods of [0, n): no division
+// on the common in-range path.
+__device__ inline int wm_wrap(int a, int n) {
+  if ((unsigned)a < (unsigned)n) return a;
+  if (a < 0 && a + n >= 0) return a + n;
+  if (a >= n && a - n < n) return a - n;
+  a %= n;
+  return a < 0 ? a + n : a;
+}

@@ -268,112 +268,103 @@ __device__ inline size_t source_bin(size_t d, int dir, int nx, int ny) {
 }
 
 struct Mig {                    // bins and mover regions of one migration
-  int nx, ny, nxr, nsp, CB;     // nxr: regions per row (cells rounded up to the push chunk)
-  size_t nregion, nbin;
-  const int *mout, *mcnt;       // per bin: movers out in total, and per direction (8)
-  int *aoff, *nold, *cnt;
+  int nx, ny, nxr, CB;          // nxr: regions per row (cells rounded up to the push chunk)
+  size_t nbin;
+  const int *mout;              // per bin: (first record in its region << 8) | movers out
+  const uint8_t *mcnt;          // per bin and direction (8): movers out (8 bytes a bin, so
+                                // neighbouring bins' counts share a memory sector)
+  int *cnt;
 };
 
-// First record of bin d's movers in its cell's region (the region lists the
-// cell's movers by species, then slot).
-__device__ inline size_t first_record(const Mig &G, size_t d) {
+// First record of bin d's movers, and their number.
+__device__ inline size_t bin_records(const Mig &G, size_t d, int &m) {
   const int i = (int)(d % G.nx);
-  const size_t sj = d / G.nx;
-  const int jl = (int)(sj % G.ny), s = (int)(sj / G.ny);
-  size_t r = ((size_t)jl * G.nxr + i) * WM_MG;
-  for (int q = 0; q < s; ++q) r += G.mout[((size_t)q * G.ny + jl) * G.nx + i];
-  return r;
+  const int jl = (int)((d / G.nx) % G.ny);
+  const int mo = G.mout[d];
+  m = mo & 0xFF;
+  return ((size_t)jl * G.nxr + i) * WM_MG + (mo >> 8);
 }
 
-// Bin counts after the migration, one thread per bin d: it held nold[d]
-// particles and lost mout[d]; its arrivals come from directions 0, 1, 2, 3,
-// 5, 6, 7, 8 in turn, each in the order its source bin listed them
-// (aoff[d * 8 + k] = index, among all of d's arrivals, of the k-th
-// direction's first one).  It also copies the IDs of d's movers into their
-// records, before any arrival can take their slots.
-__global__ void k_bin_counts(Mig G, Movers M, const int64_t *__restrict__ ids, int *derr) {
-  for (size_t d = blockIdx.x * (size_t)blockDim.x + threadIdx.x; d < G.nbin;
-       d += (size_t)gridDim.x * blockDim.x) {
-    int a = 0;
-    for (int dd = 0; dd < 8; ++dd) {
-      const int dirp = dd < 4 ? dd : dd + 1;
-      G.aoff[d * 8 + dd] = a;
-      a += G.mcnt[source_bin(d, dirp, G.nx, G.ny) * 8 + dd];
-    }
-    const int n = min(G.cnt[d], G.CB), m = G.mout[d];
-    G.nold[d] = n;
-    int c = n - m + a;
-    if (c > G.CB) {
-      atomicOr(derr, DERR_NP_OVER);
-      c = G.CB;
-    }
-    G.cnt[d] = c;
-    if (m > 0) {
-      const size_t r0 = first_record(G, d);
-      for (int j = 0; j < m; ++j) M.rec[r0 + j].id = ids[d * G.CB + M.rec[r0 + j].slot];
-    }
-  }
-}
-
-// Movers -> destination bins, 8 lanes per mover region (source cell), lane l
-// taking the cell's movers l, l + 8, ...  Bin d's k-th arrival takes d's k-th
-// hole (the slots its own movers left, in ascending order) while there are
-// holes, then the slots after its old count.  When d receives fewer arrivals
-// than it lost, the holes left below its new count take, in order, the
-// particles above it; lane s of d's own region moves them for species s
-// (arrivals only fill lower holes, so nothing overlaps).
-constexpr int ILANES = 8;
-__global__ void k_insert_movers(Mig G, Movers M, Rec32 A, int64_t *__restrict__ ids, int *derr) {
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < G.nregion * ILANES;
+// The migration of bin d, by a group of 8 lanes that alone writes d (no
+// lane reads what another group writes).  Lane k takes the arrivals from the
+// source bin in direction k (0, 1, 2, 3, 5, 6, 7, 8), in the order the source
+// listed them; directions follow one another, so arrival index = the earlier
+// directions' count (a prefix over the group's lanes) + the rank within
+// direction k.  d held n particles and lost m (its movers' slots, ascending,
+// are its holes): the a-th arrival takes the a-th hole while there are holes,
+// then slot n + (a - m).  With fewer arrivals than holes, lane 0 fills the
+// holes left below the new count, in order, with the particles above it.
+constexpr int DL = 8;
+__global__ void k_migrate(Mig G, Movers M, Rec32 A, int64_t *__restrict__ ids, int *derr) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < G.nbin;
        t += (size_t)gridDim.x * blockDim.x) {
-    const size_t region = t / ILANES;
-    const int lane = (int)(t % ILANES);
-    const MoverRec *rec = M.rec + region * WM_MG;
-    const int nm = min(M.n[region], WM_MG);
-    for (int j = lane; j < nm; j += ILANES) {
-      const MoverRec mr = rec[j];
-      int rank = 0;                                // earlier movers to the same bin
-      for (int q = 0; q < j; ++q) rank += rec[q].dst == mr.dst;
-      const int d8 = mr.dir < 4 ? mr.dir : mr.dir - 1;
-      const int a = G.aoff[(size_t)mr.dst * 8 + d8] + rank;
-      const int md = G.mout[mr.dst];
-      const int slot = a < md ? M.rec[first_record(G, mr.dst) + a].slot : G.nold[mr.dst] + (a - md);
-      if (slot >= G.CB) {
-        atomicOr(derr, DERR_NP_OVER);
-        continue;
-      }
-      const size_t p = (size_t)mr.dst * G.CB + slot;
-      A.ox[p] = mr.ox1;
-      A.oy[p] = mr.oy1;
-      A.ux[p] = mr.ux;
-      A.uy[p] = mr.uy;
-      A.uz[p] = mr.uz;
-      ids[p] = mr.id;
+    const size_t d = t / DL;
+    const int k = (int)(t % DL);
+    const int dir = k < 4 ? k : k + 1;
+    const size_t src = source_bin(d, dir, G.nx, G.ny);
+    const int c = G.mcnt[src * 8 + k];
+    int m;
+    const size_t r0 = bin_records(G, d, m);
+    const int n = min(G.cnt[d], G.CB);
+    int inc = c;                                   // inclusive prefix over the group
+#pragma unroll
+    for (int o = 1; o < DL; o <<= 1) {
+      const int v = __shfl_up(inc, o, DL);
+      if (k >= o) inc += v;
     }
-    const int i = (int)(region % G.nxr), jl = (int)(region / G.nxr);
-    if (lane >= G.nsp || i >= G.nx) continue;
-    const size_t b = ((size_t)lane * G.ny + jl) * G.nx + i;
-    const int n = G.nold[b], m = G.mout[b], n1 = n - m, arr = G.cnt[b] - n1;
-    if (arr >= m) continue;
-    const size_t r = first_record(G, b);
-    const int nnew = n1 + arr;
-    int hb = arr;                                  // holes below the new count
-    while (hb < m && M.rec[r + hb].slot < nnew) ++hb;
-    int k = arr, h = hb;
-    for (int from = nnew; from < n && k < hb; ++from) {
-      if (h < m && M.rec[r + h].slot == from) {    // a hole above the new count
-        ++h;
-        continue;
+    const int total = __shfl(inc, DL - 1, DL);
+    if (c > 0) {
+      int ms;
+      const size_t rs = bin_records(G, src, ms);
+      int a = inc - c, left = c;
+      for (int j = 0; j < ms && left > 0; ++j) {
+        if (M.dir[rs + j] != dir) continue;
+        const MoverRec mr = M.rec[rs + j];
+        --left;
+        const int slot = a < m ? M.rec[r0 + a].slot : n + (a - m);
+        ++a;
+        if (slot >= G.CB) {
+          atomicOr(derr, DERR_NP_OVER);
+          continue;
+        }
+        const size_t p = d * G.CB + slot;
+        A.ox[p] = mr.ox1;
+        A.oy[p] = mr.oy1;
+        A.ux[p] = mr.ux;
+        A.uy[p] = mr.uy;
+        A.uz[p] = mr.uz;
+        ids[p] = mr.id;
       }
-      const size_t dst = b * G.CB + M.rec[r + k].slot, src = b * G.CB + from;
-      A.ox[dst] = A.ox[src];
-      A.oy[dst] = A.oy[src];
-      A.ux[dst] = A.ux[src];
-      A.uy[dst] = A.uy[src];
-      A.uz[dst] = A.uz[src];
-      ids[dst] = ids[src];
-      ++k;
     }
+    if (k != 0) continue;
+    const int n1 = n - m;
+    if (total < m) {
+      const size_t base = d * G.CB;
+      const int nnew = n1 + total;
+      int hb = total;                              // holes below the new count
+      while (hb < m && M.rec[r0 + hb].slot < nnew) ++hb;
+      int q = total, h = hb;
+      for (int from = nnew; from < n && q < hb; ++from) {
+        if (h < m && M.rec[r0 + h].slot == from) { // a hole above the new count
+          ++h;
+          continue;
+        }
+        const size_t dst = base + M.rec[r0 + q].slot, sp = base + from;
+        A.ox[dst] = A.ox[sp];
+        A.oy[dst] = A.oy[sp];
+        A.ux[dst] = A.ux[sp];
+        A.uy[dst] = A.uy[sp];
+        A.uz[dst] = A.uz[sp];
+        ids[dst] = ids[sp];
+        ++q;
+      }
+    }
+    int cn = n1 + total;
+    if (cn > G.CB) {
+      atomicOr(derr, DERR_NP_OVER);
+      cn = G.CB;
+    }
+    G.cnt[d] = cn;
   }
 }
 
@@ -406,25 +397,20 @@ int wm_k_bins_to_dense(wm_ctx *ctx) {
 }
 
 int wm_k_insert_movers(wm_ctx *ctx) {
-  const int nx = ctx->g.nx, ny = ctx->g.ny;
   Mig G;
-  G.nx = nx;
-  G.ny = ny;
-  G.nregion = (size_t)ctx->nregion;
-  G.nxr = (int)(G.nregion / (size_t)ny);
-  G.nsp = ctx->g.nsp;
+  G.nx = ctx->g.nx;
+  G.ny = ctx->g.ny;
+  G.nxr = (int)((size_t)ctx->nregion / (size_t)ctx->g.ny);
   G.CB = ctx->CB;
-  G.nbin = (size_t)ctx->g.nsp * ny * nx;
+  G.nbin = (size_t)ctx->g.nsp * G.ny * G.nx;
   G.mout = ctx->bin_mout;
   G.mcnt = ctx->bin_mcnt;
-  G.aoff = ctx->bin_aoff;
-  G.nold = ctx->bin_nold;
   G.cnt = ctx->bin_cnt;
-  hipLaunchKernelGGL(k_bin_counts, dim3((unsigned)std::min<size_t>((G.nbin + 255) / 256, 65536)),
-                     dim3(256), 0, ctx->stream, G, ctx->mv, ctx->id_bins, ctx->derr);
-  hipLaunchKernelGGL(k_insert_movers,
-                     dim3((unsigned)std::min<size_t>((G.nregion * ILANES + 255) / 256, 65536)),
-                     dim3(256), 0, ctx->stream, G, ctx->mv, ctx->b32[0], ctx->id_bins, ctx->derr);
+  // a group of DL lanes never straddles a wave (the stride is a whole number
+  // of waves), and the loop bound is per group, so the shuffles see all lanes
+  hipLaunchKernelGGL(k_migrate,
+                     dim3((unsigned)std::min<size_t>((G.nbin * DL + 255) / 256, 65536)), dim3(256),
+                     0, ctx->stream, G, ctx->mv, ctx->b32[0], ctx->id_bins, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
