@@ -369,12 +369,8 @@ int wm_alloc_device(wm_ctx *ctx) {
       return WM_ERR_ARG;
     }
     const size_t nbin = (size_t)nseg * g.nx, bslots = nbin * ctx->CB;
-    Rec32 &b = ctx->b32[0];
-    WM_TRY(dmalloc(ctx, &b.ox, bslots));
-    WM_TRY(dmalloc(ctx, &b.oy, bslots));
-    WM_TRY(dmalloc(ctx, &b.ux, bslots));
-    WM_TRY(dmalloc(ctx, &b.uy, bslots));
-    WM_TRY(dmalloc(ctx, &b.uz, bslots));
+    WM_TRY(dmalloc(ctx, &ctx->bins.q, bslots));
+    WM_TRY(dmalloc(ctx, &ctx->bins.uz, bslots));
     WM_TRY(dmalloc(ctx, &ctx->id_bins, bslots));
     WM_TRY(dmalloc(ctx, &ctx->bin_cnt, nbin));
     WM_TRY(dmalloc(ctx, &ctx->bin_mout, nbin));
@@ -507,6 +503,8 @@ int wm_destroy(wm_ctx *ctx) {
   free_bank(ctx->mv_up);
   free_bank(ctx->mv_dn);
   free_rec32(ctx->b32[0]);
+  if (ctx->bins.q) (void)hipFree(ctx->bins.q);
+  if (ctx->bins.uz) (void)hipFree(ctx->bins.uz);
   free_rec32(ctx->b32[1]);
   if (ctx->tmp32) (void)hipFree(ctx->tmp32);
   void *ptrs[] = {ctx->id[0], ctx->id[1], ctx->np2, ctx->cumcnt, ctx->hole, ctx->nhole,

@@ -169,8 +169,9 @@ struct wm_ctx {
   double phase_ms[8] = {0};
   int *cnt_scratch = nullptr;    // [nseg] counts for layout conversion
   // FP32 form: particles live in cell bins of CB slots, bin = seg * nx + i
-  // (bank b32[0], IDs id_bins); b32[1] with id[] is the dense row layout of
-  // the reference, used to convert state in and out
+  // (bins, IDs id_bins); b32[1] with id[] is the dense row layout of the
+  // reference, used to convert state in and out
+  Bins32 bins;
   int CB = 0;
   int *bin_cnt = nullptr;        // particles per bin
   int *bin_mout = nullptr;       // per bin after the push: (first record << 8) | movers out

@@ -381,12 +381,13 @@ __device__ inline void make_spc(Spc &S, int s, int n, int jl, int il, int CB, co
 // (a lane past the bin's count re-reads its last particle, or slot 0 of an
 // empty bin): a conditional load would first clear the registers, and that
 // write makes the compiler wait for every load still in flight.
-__device__ inline void load_slot(Win &W, int w, const Rec32 &A, const Spc &S, int sub) {
+__device__ inline void load_slot(Win &W, int w, const Bins32 &A, const Spc &S, int sub) {
   const size_t p = spc_base(S) + min(w * GROUP + sub, max(S.n - 1, 0));
-  W.ox[w] = A.ox[p];
-  W.oy[w] = A.oy[p];
-  W.ux[w] = A.ux[p];
-  W.uy[w] = A.uy[p];
+  const float4 v = A.q[p];
+  W.ox[w] = v.x;
+  W.oy[w] = v.y;
+  W.ux[w] = v.z;
+  W.uy[w] = v.w;
   W.uz[w] = A.uz[p];
 }
 
@@ -409,7 +410,7 @@ __device__ inline size_t lane_reg(const PP32 &P) {   // first record of the cell
 // accumulated; a mover goes to the cell's mover region (all but its ID) and
 // to the LDS list, with the inputs of its deposit.
 __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float oy0, float cux,
-                                   float cuy, float cuz, const Spc &S, const PP32 &P, const Rec32 &A,
+                                   float cuy, float cuz, const Spc &S, const PP32 &P, const Bins32 &A,
                                    const Movers &M) {
   // Every lane pushes and stores: a lane past the bin's count holds its last
   // particle (or slot 0 of an empty bin) and writes back the same values as
@@ -428,10 +429,7 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
   int kk = min(k, max(S.n - 1, 0));
   __asm__ volatile("" : "+v"(kk));                 // store addresses formed here, not earlier
   const size_t p = spc_base(S) + kk;
-  A.ox[p] = ox1;
-  A.oy[p] = oy1;
-  A.ux[p] = o.ux;
-  A.uy[p] = o.uy;
+  A.q[p] = make_float4(ox1, oy1, o.ux, o.uy);
   A.uz[p] = o.uz;
   if (on && !cross) {
     const float qv =
@@ -471,7 +469,7 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
 // of bins above PD * GROUP particles, loaded directly.
 template <bool MORE>
 __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, const Spc &Sn,
-                                    const PP32 &P, const Rec32 &A, const Movers &M) {
+                                    const PP32 &P, const Bins32 &A, const Movers &M) {
 #pragma unroll
   for (int w = 0; w < PD; ++w) {
     push_window(L, core, w * GROUP + lane_sub(), W.ox[w], W.oy[w], W.ux[w], W.uy[w], W.uz[w], S, P, A,
@@ -481,7 +479,8 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
   for (int w = PD * GROUP; w < S.n; w += GROUP) {
     const int k = w + lane_sub();
     const size_t p = spc_base(S) + min(k, S.n - 1);
-    push_window(L, core, k, A.ox[p], A.oy[p], A.ux[p], A.uy[p], A.uz[p], S, P, A, M);
+    const float4 v = A.q[p];
+    push_window(L, core, k, v.x, v.y, v.z, v.w, A.uz[p], S, P, A, M);
     // nothing of this rare path stays in flight: otherwise the compiler, not
     // knowing whether the loop ran, would drain the register windows after it
     __builtin_amdgcn_s_waitcnt(0);
@@ -501,7 +500,7 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
 template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     PP32 P, int CB, const int *__restrict__ cnt, const float *__restrict__ tmp, int TX,
-    size_t tps, Rec32 A, const int64_t *__restrict__ ids, Movers M, int *__restrict__ mout,
+    size_t tps, Bins32 A, const int64_t *__restrict__ ids, Movers M, int *__restrict__ mout,
     uint8_t *__restrict__ mcnt, double *__restrict__ part, int *__restrict__ derr) {
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
@@ -707,9 +706,8 @@ __global__ void k_rec32_to_aos(int ndim, int w, int cap, int ny, int nxgs, int n
 // FP64 energy sums of FP32 momenta: per-workgroup partials.
 // Workgroup (x, seg) sums the bins of cells x, x + nchunk, ... of segment seg.
 __global__ __launch_bounds__(256) void k_energy_kin32(int nx, int CB, const int *cnt,
-                                                      const float *ux, const float *uy,
-                                                      const float *uz, double c, double *part,
-                                                      int nchunk) {
+                                                      const float4 *q, const float *uz, double c,
+                                                      double *part, int nchunk) {
   __shared__ double sh[256];
   const int seg = blockIdx.y;
   double v = 0.0;
@@ -718,7 +716,7 @@ __global__ __launch_bounds__(256) void k_energy_kin32(int nx, int CB, const int 
     const int n = min(CB, cnt[bin]);
     for (int k = threadIdx.x; k < n; k += 256) {
       const size_t p = bin * CB + k;
-      const double a = ux[p], b = uy[p], d = uz[p];
+      const double a = q[p].z, b = q[p].w, d = uz[p];
       v += sqrt(1 + (a * a + b * b + d * d) / (c * c)) - 1;
     }
   }
@@ -771,7 +769,7 @@ int wm_k_push_deposit32(wm_ctx *ctx) {
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
   hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
-                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->b32[0], ctx->id_bins, ctx->mv,
+                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->bins, ctx->id_bins, ctx->mv,
                      ctx->bin_mout, ctx->bin_mcnt, ctx->dep_part, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
@@ -803,9 +801,8 @@ int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *sta
 int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk) {
   const int nseg = ctx->g.nsp * ctx->g.ny;
   (void)bank;                                      // the bins are the state
-  const Rec32 &B = ctx->b32[0];
   hipLaunchKernelGGL(k_energy_kin32, dim3(nchunk, nseg), dim3(256), 0, ctx->stream, ctx->g.nx,
-                     ctx->CB, ctx->bin_cnt, B.ux, B.uy, B.uz, ctx->g.c, part, nchunk);
+                     ctx->CB, ctx->bin_cnt, ctx->bins.q, ctx->bins.uz, ctx->g.c, part, nchunk);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

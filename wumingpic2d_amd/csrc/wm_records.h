@@ -52,6 +52,13 @@ struct Rec32 {
 // Largest float below 1: offsets are kept in [0, 1) after a cell change.
 __device__ inline float wm_below_one(float v) { return v < 1.0f ? v : 0x1.fffffep-1f; }
 
+// FP32 form, cell bins: per slot a float4 (ox, oy, ux, uy) and uz (the IDs
+// apart).  A particle placed by the migration touches three arrays.
+struct Bins32 {
+  float4 *q = nullptr;          // offsets in the cell, in-plane momenta
+  float *uz = nullptr;
+};
+
 // Fixed-order sum over a 64-lane wave: a butterfly on lane xor 32, 16, ...,
 // 1.  Partners add the same two values (IEEE addition commutes), so every
 // lane ends with the bit-identical total.

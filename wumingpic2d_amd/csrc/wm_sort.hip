@@ -150,7 +150,7 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, 
 // particles keep their row order.  One wavefront per segment, as k_sort.
 __global__ __launch_bounds__(SW) void k_dense_to_bins(SP P, int CB, const int *__restrict__ np2,
                                                       Rec32 src, const int64_t *__restrict__ ids,
-                                                      Rec32 dst, int64_t *__restrict__ idd,
+                                                      Bins32 dst, int64_t *__restrict__ idd,
                                                       int *__restrict__ cnt, int *derr) {
   extern __shared__ int run[];             // particles written per cell so far
   const int seg = blockIdx.x;
@@ -180,10 +180,7 @@ __global__ __launch_bounds__(SW) void k_dense_to_bins(SP P, int CB, const int *_
       const int slot = run[key] + m.rank;
       if (slot < CB) {
         const size_t d = ((size_t)seg * P.nx + key) * CB + slot;
-        dst.ox[d] = v.ox;
-        dst.oy[d] = v.oy;
-        dst.ux[d] = v.ux;
-        dst.uy[d] = v.uy;
+        dst.q[d] = make_float4(v.ox, v.oy, v.ux, v.uy);
         dst.uz[d] = v.uz;
         idd[d] = vid;
       } else {
@@ -202,7 +199,7 @@ __global__ __launch_bounds__(SW) void k_dense_to_bins(SP P, int CB, const int *_
 // Bins -> dense rows sorted by cell, with cumcnt and np2 as sort.f90:64-69
 // leaves them.  One workgroup per segment.
 __global__ __launch_bounds__(256) void k_bins_to_dense(SP P, int CB, const int *__restrict__ cnt,
-                                                       Rec32 src, const int64_t *__restrict__ ids,
+                                                       Bins32 src, const int64_t *__restrict__ ids,
                                                        Rec32 dst, int64_t *__restrict__ idd,
                                                        int *__restrict__ cumcnt,
                                                        int *__restrict__ np2, int *derr) {
@@ -245,11 +242,12 @@ __global__ __launch_bounds__(256) void k_bins_to_dense(SP P, int CB, const int *
     }
     const size_t f = ((size_t)seg * P.nx + lo) * CB + (k - cc[lo]);
     const size_t d = base + k;
+    const float4 v = src.q[f];
     dst.ci[d] = lo;
-    dst.ox[d] = src.ox[f];
-    dst.oy[d] = src.oy[f];
-    dst.ux[d] = src.ux[f];
-    dst.uy[d] = src.uy[f];
+    dst.ox[d] = v.x;
+    dst.oy[d] = v.y;
+    dst.ux[d] = v.z;
+    dst.uy[d] = v.w;
     dst.uz[d] = src.uz[f];
     idd[d] = ids[f];
   }
@@ -295,7 +293,7 @@ __device__ inline size_t bin_records(const Mig &G, size_t d, int &m) {
 // then slot n + (a - m).  With fewer arrivals than holes, lane 0 fills the
 // holes left below the new count, in order, with the particles above it.
 constexpr int DL = 8;
-__global__ void k_migrate(Mig G, Movers M, Rec32 A, int64_t *__restrict__ ids, int *derr) {
+__global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids, int *derr) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < G.nbin;
        t += (size_t)gridDim.x * blockDim.x) {
     const size_t d = t / DL;
@@ -328,10 +326,7 @@ __global__ void k_migrate(Mig G, Movers M, Rec32 A, int64_t *__restrict__ ids, i
           continue;
         }
         const size_t p = d * G.CB + slot;
-        A.ox[p] = mr.ox1;
-        A.oy[p] = mr.oy1;
-        A.ux[p] = mr.ux;
-        A.uy[p] = mr.uy;
+        A.q[p] = make_float4(mr.ox1, mr.oy1, mr.ux, mr.uy);
         A.uz[p] = mr.uz;
         ids[p] = mr.id;
       }
@@ -350,10 +345,7 @@ __global__ void k_migrate(Mig G, Movers M, Rec32 A, int64_t *__restrict__ ids, i
           continue;
         }
         const size_t dst = base + M.rec[r0 + q].slot, sp = base + from;
-        A.ox[dst] = A.ox[sp];
-        A.oy[dst] = A.oy[sp];
-        A.ux[dst] = A.ux[sp];
-        A.uy[dst] = A.uy[sp];
+        A.q[dst] = A.q[sp];
         A.uz[dst] = A.uz[sp];
         ids[dst] = ids[sp];
         ++q;
@@ -377,7 +369,7 @@ int wm_k_dense_to_bins(wm_ctx *ctx) {
   P.cap = ctx->cap;
   const int nseg = ctx->g.nsp * ctx->g.ny;
   hipLaunchKernelGGL(k_dense_to_bins, dim3(nseg), dim3(SW), sizeof(int) * P.nx, ctx->stream, P,
-                     ctx->CB, ctx->np2, ctx->b32[1], ctx->id[ctx->idcur], ctx->b32[0],
+                     ctx->CB, ctx->np2, ctx->b32[1], ctx->id[ctx->idcur], ctx->bins,
                      ctx->id_bins, ctx->bin_cnt, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
@@ -390,7 +382,7 @@ int wm_k_bins_to_dense(wm_ctx *ctx) {
   P.cap = ctx->cap;
   const int nseg = ctx->g.nsp * ctx->g.ny;
   hipLaunchKernelGGL(k_bins_to_dense, dim3(nseg), dim3(256), sizeof(int) * (P.nx + 1),
-                     ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->b32[0], ctx->id_bins,
+                     ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->bins, ctx->id_bins,
                      ctx->b32[1], ctx->id[ctx->idcur], ctx->cumcnt, ctx->np2, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
@@ -410,7 +402,7 @@ int wm_k_insert_movers(wm_ctx *ctx) {
   // of waves), and the loop bound is per group, so the shuffles see all lanes
   hipLaunchKernelGGL(k_migrate,
                      dim3((unsigned)std::min<size_t>((G.nbin * DL + 255) / 256, 65536)), dim3(256),
-                     0, ctx->stream, G, ctx->mv, ctx->b32[0], ctx->id_bins, ctx->derr);
+                     0, ctx->stream, G, ctx->mv, ctx->bins, ctx->id_bins, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
