@@ -226,6 +226,8 @@ __device__ inline void partial_total(const double *partials, int nblk, double (*
   double a[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) a[k] = 0.0;
+  // unrolled so that the loads go out together; the additions keep their order
+#pragma unroll 8
   for (int b = threadIdx.x; b < nblk; b += CGB) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) a[k] += partials[(size_t)k * nblk + b];
@@ -638,12 +640,14 @@ static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
   static_assert(3 * sizeof(FcgScal) <= WM_PIN_BYTES, "pinned scratch too small");
   FcgScal *hs = (FcgScal *)ctx->h_pin;
   int cur = 0;
+  int batch = std::max(1, ctx->fcg_hint);
   for (;;) {
     WM_HIP(hipMemcpyAsync(hs, S, 3 * sizeof(FcgScal), hipMemcpyDeviceToHost, st));
     WM_HIP(hipStreamSynchronize(st));
     if (!hs[0].active && !hs[1].active && !hs[2].active) break;
-    // two iterations per host check; launches after convergence return at once
-    for (int bi = 0; bi < 2; ++bi) {
+    // first the last solve's iteration count, then one per host check;
+    // launches after convergence return at once
+    for (int bi = 0; bi < batch; ++bi) {
       WM_TRY(wm_pc_apply_dev(ctx, ctx->cg_r, ctx->pc_sc, ctx->cg_z, ctx->cg_q, ctx->red, S));
       hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, AT, 1, ctx->red, S, ctx->pc_sc,
                          ctx->fcg_max);
@@ -658,10 +662,12 @@ static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
                          ctx->fcg_max);
       WM_HIP(hipGetLastError());
     }
+    batch = 1;
   }
   hipLaunchKernelGGL(k_fcg_restore, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_x0, S);
   WM_HIP(hipGetLastError());
   for (int l = 0; l < 3; ++l) its[l] = hs[l].it;
+  ctx->fcg_hint = std::max(its[0], std::max(its[1], its[2]));
   return WM_OK;
 }
 
@@ -695,6 +701,7 @@ int wm_k_field_solve(wm_ctx *ctx) {
   static_assert(3 * sizeof(CgScal) <= WM_PIN_BYTES, "pinned scratch too small");
   CgScal *hs = (CgScal *)ctx->h_pin;
   int iters = 0, cur = 0;
+  int batch = collect ? 1 : std::max(1, ctx->cg_hint);
   for (;;) {
     WM_HIP(hipMemcpyAsync(hs, ctx->cg_s, 3 * sizeof(CgScal), hipMemcpyDeviceToHost, ctx->stream));
     WM_HIP(hipStreamSynchronize(ctx->stream));
@@ -712,10 +719,10 @@ int wm_k_field_solve(wm_ctx *ctx) {
       wm_set_error(ctx, WM_ERR_CG_ITEMAX, "********** stop at cgm after ite_max **********");
       return WM_ERR_CG_ITEMAX;
     }
-    // several iterations per host check: a component that has stopped skips
-    // every kernel (its scalars say inactive), so the extra launches change
-    // nothing; while pairs are recorded, one iteration at a time
-    const int batch = collect ? 1 : 4;
+    // several iterations per host check (first the last plain solve's count,
+    // then two): a component that has stopped skips every kernel (its scalars
+    // say inactive), so the extra launches change nothing; while pairs are
+    // recorded, one iteration at a time
     for (int bi = 0; bi < batch; ++bi) {
       if (collect && all) WM_TRY(wm_pc_snapshot(ctx, iters));
       ++iters;
@@ -728,8 +735,10 @@ int wm_k_field_solve(wm_ctx *ctx) {
       hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 2, ctx->red, ctx->cg_s);
       WM_HIP(hipGetLastError());
     }
+    batch = collect ? 1 : 2;
   }
   if (collect) WM_TRY(wm_pc_snapshot_fin(ctx));
+  if (iters > 0) ctx->cg_hint = std::max(hs[0].ite, std::max(hs[1].ite, hs[2].ite));
   for (int l = 0; l < 3; ++l) ctx->last_iters[l] = fcg_it[l] + hs[l].ite;
   if (ctx->solver == WM_SOLVER_FCG) {
     int fb[3] = {hs[0].ite, hs[1].ite, hs[2].ite};

@@ -186,6 +186,9 @@ struct wm_ctx {
   // learned preconditioner, plain CG as fallback and judge
   int solver = WM_SOLVER_CG;
   int fcg_max = 50;              // flexible-CG iterations before the fallback
+  // iterations queued before the first host check: the previous solve's
+  // count, so a solve that converges as the last one did needs one check
+  int fcg_hint = 2, cg_hint = 4;
   wm_pc *pc = nullptr;
   double *cg_z = nullptr, *cg_q = nullptr, *cg_x0 = nullptr;   // 3 planes nx*ny each
   double *pc_sc = nullptr;       // s[3], 1/s[3] of the current residuals

@@ -55,6 +55,23 @@ __device__ inline int pmod(int a, int n) {
   return a < 0 ? a + n : a;
 }
 
+// Tile of this workgroup in a (tiles x, tiles y, images) grid, renumbered so
+// that each XCD sweeps a contiguous band of tiles.  The dispatcher deals
+// workgroups round-robin to the 8 XCDs, each with its own L2; without the
+// renumbering neighbouring tiles (which share halo lines) sit in different
+// L2s.
+struct Tile {
+  int x, y, b;
+};
+__device__ inline Tile xcd_tile() {
+  const int nb = gridDim.x * gridDim.y * gridDim.z;
+  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = bid & 7, s = bid >> 3;
+  const int t = x * q + min(x, r) + s;
+  return Tile{(int)(t % gridDim.x), (int)((t / gridDim.x) % gridDim.y),
+              (int)(t / (gridDim.x * gridDim.y))};
+}
+
 // Inside flexible CG the preconditioner's launches are queued ahead of the
 // host's convergence check; once every component has stopped they return at
 // once.
@@ -88,8 +105,9 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
   if (gated_off(gate)) return;
   __shared__ __attribute__((aligned(16))) bf16 sin[(T + 2) * (T + 2) * CIN];
   __shared__ __attribute__((aligned(16))) bf16 sw[9 * COUT * CIN];
-  const int b = blockIdx.z;
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
+  const Tile tl = xcd_tile();
+  const int b = tl.b;
+  const int x0 = tl.x * T, y0 = tl.y * T;
   const bf16 *inb = in + (size_t)b * H * W * CIN;
   constexpr int CH = CIN / 8;
   for (int e = threadIdx.x; e < (T + 2) * (T + 2) * CH; e += 256) {
@@ -188,15 +206,42 @@ __device__ inline double block_sum1(double v, double *sh) {
 }
 
 // ------------------------------------------------------- fine-level kernels
-// x0 tile with a halo of HALO pixels, (float)(r * inv_s), periodic.
+// x0 tile with a halo of HALO pixels, (float)(r * inv_s), periodic.  Staged
+// in two passes, every thread's loads issued before any is used, so the
+// tile costs one memory latency.
+template <int HALO>
+struct X0Tile {
+  static constexpr int E = T + 2 * HALO, N = E * E, NIT = (N + 255) / 256;
+  double v[NIT];
+};
+template <int HALO>
+__device__ inline void x0_issue(X0Tile<HALO> &X, const double *__restrict__ r, int x0, int y0,
+                                int H, int W) {
+  constexpr int E = X0Tile<HALO>::E;
+#pragma unroll
+  for (int i = 0; i < X0Tile<HALO>::NIT; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    X.v[i] = 0.0;
+    if (e < X0Tile<HALO>::N) {
+      const int gy = wm_wrap(y0 + e / E - HALO, H), gx = wm_wrap(x0 + e % E - HALO, W);
+      X.v[i] = r[(size_t)gy * W + gx];
+    }
+  }
+}
+template <int HALO>
+__device__ inline void x0_put(float *xs, const X0Tile<HALO> &X, double is) {
+#pragma unroll
+  for (int i = 0; i < X0Tile<HALO>::NIT; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < X0Tile<HALO>::N) xs[e] = (float)(X.v[i] * is);
+  }
+}
 template <int HALO>
 __device__ inline void load_x0(float *xs, const double *__restrict__ r, double is, int x0,
                                int y0, int H, int W) {
-  constexpr int E = T + 2 * HALO;
-  for (int e = threadIdx.x; e < E * E; e += 256) {
-    const int gy = wm_wrap(y0 + e / E - HALO, H), gx = wm_wrap(x0 + e % E - HALO, W);
-    xs[e] = (float)(r[(size_t)gy * W + gx] * is);
-  }
+  X0Tile<HALO> X;
+  x0_issue<HALO>(X, r, x0, y0, H, W);
+  x0_put<HALO>(xs, X, is);
 }
 
 // a0 (C0 channels) at tile-local pixel (py, px) of an x0 tile of edge E whose
@@ -266,26 +311,41 @@ __global__ __launch_bounds__(256) void k_lift_pool(const double *__restrict__ r,
                                                    const FcgScal *__restrict__ gate) {
   if (gated_off(gate)) return;
   __shared__ float xs[(T + 2) * (T + 2)];
-  __shared__ float fl[16 * C0];
-  const int b = blockIdx.z;
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
-  load_x0<1>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
-  for (int e = threadIdx.x; e < 16 * C0; e += 256) fl[e] = fold[FL_OFF + e];
+  __shared__ __attribute__((aligned(16))) float fl[16 * C0];
+  const Tile tl = xcd_tile();
+  const int b = __builtin_amdgcn_readfirstlane(tl.b);
+  const int x0 = tl.x * T, y0 = tl.y * T;
+  static_assert(16 * C0 == 256, "one weight per thread");
+  X0Tile<1> xt;
+  x0_issue<1>(xt, r + (size_t)b * rstride, x0, y0, H, W);
+  const float flv = fold[FL_OFF + threadIdx.x];
+  x0_put<1>(xs, xt, sc[B + b]);
+  fl[threadIdx.x] = flv;
   __syncthreads();
   const int H1 = H / 2, W1 = W / 2;
-  for (int e = threadIdx.x; e < (T / 2) * (T / 2) * C0; e += 256) {
-    const int co = e % C0, q = e / C0;
-    const int qy = q / (T / 2), qx = q % (T / 2);
-    const int Y = y0 / 2 + qy, X = x0 / 2 + qx;
-    if (Y >= H1 || X >= W1) continue;
-    float v = 0.0f;
+  // thread: channels 4 c4 .. 4 c4 + 3 of level-1 pixel q (64 pixels x 4)
+  static_assert((T / 2) * (T / 2) * (C0 / 4) == 256, "one pass");
+  const int c4 = threadIdx.x % (C0 / 4), q = threadIdx.x / (C0 / 4);
+  const int qy = q / (T / 2), qx = q % (T / 2);
+  const int Y = y0 / 2 + qy, X = x0 / 2 + qx;
+  if (Y >= H1 || X >= W1) return;
+  float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr)
+  for (int rr = 0; rr < 4; ++rr)
 #pragma unroll
-      for (int cc = 0; cc < 4; ++cc)
-        v = fmaf(fl[(rr * 4 + cc) * C0 + co], xs[(2 * qy + rr) * (T + 2) + 2 * qx + cc], v);
-    d1[(((size_t)b * H1 + Y) * W1 + X) * C0 + co] = (bf16)v;
-  }
+    for (int cc = 0; cc < 4; ++cc) {
+      const float xv = xs[(2 * qy + rr) * (T + 2) + 2 * qx + cc];
+      const float4 w = *(const float4 *)&fl[(rr * 4 + cc) * C0 + 4 * c4];
+      v[0] = fmaf(w.x, xv, v[0]);
+      v[1] = fmaf(w.y, xv, v[1]);
+      v[2] = fmaf(w.z, xv, v[2]);
+      v[3] = fmaf(w.w, xv, v[3]);
+    }
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  bf16x4 o;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = (bf16)v[k];
+  __builtin_memcpy(&d1[(((size_t)b * H1 + Y) * W1 + X) * C0 + 4 * c4], &o, 8);
 }
 
 // u0 = lift(x0) + up(b1) on the tile plus a 1-pixel halo, in LDS.
@@ -320,56 +380,83 @@ __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, s
                                                  double *__restrict__ dots,
                                                  const FcgScal *__restrict__ S) {
   constexpr int CE = T / 2 + 2;                    // coarse tile edge (halo 1)
+  // coarse pixels 20 floats apart (16 channels + 4): the lanes' 16-byte reads
+  // of neighbouring pixels fall in different LDS banks
+  constexpr int CS = C0 + 4;
   __shared__ float xs[(T + 4) * (T + 4)];
-  __shared__ float bs[CE * CE * C0];
-  __shared__ float fk[25], fp[256];
+  __shared__ __attribute__((aligned(16))) float bs[CE * CE * CS];
+  // parity groups 68 floats apart, so a wave's four parities use different banks
+  constexpr int PS = 4 * C0 + 4;
+  __shared__ __attribute__((aligned(16))) float fp[4 * PS];
+  __shared__ float fk[25];
   __shared__ double sh[8];
   if (dots && gated_off(S)) return;
-  const int b = blockIdx.z;
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T;
-  load_x0<2>(xs, r + (size_t)b * rstride, sc[B + b], x0, y0, H, W);
-  const int H1 = H / 2, W1 = W / 2;
-  const bf16 *b1b = b1 + (size_t)b * H1 * W1 * C0;
-  for (int e = threadIdx.x; e < CE * CE * (C0 / 8); e += 256) {
-    const int c8 = e % (C0 / 8), q = e / (C0 / 8);
-    const int Y = wm_wrap(y0 / 2 + q / CE - 1, H1), X = wm_wrap(x0 / 2 + q % CE - 1, W1);
-    bf16x8 v;
-    __builtin_memcpy(&v, &b1b[((size_t)Y * W1 + X) * C0 + 8 * c8], 16);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) bs[q * C0 + 8 * c8 + k] = (float)v[k];
-  }
-  for (int e = threadIdx.x; e < 256; e += 256) fp[e] = fold[FP_OFF + e];
-  if (threadIdx.x < 25) fk[threadIdx.x] = fold[FK_OFF + threadIdx.x];
-  __syncthreads();
+  const Tile tl = xcd_tile();
+  const int b = __builtin_amdgcn_readfirstlane(tl.b);
+  const int x0 = tl.x * T, y0 = tl.y * T;
   const int ty = threadIdx.x / T, tx = threadIdx.x % T;
   const int gy = y0 + ty, gx = x0 + tx;
   const bool in = gy < H && gx < W;
+  const size_t p = (size_t)gy * W + gx;
+  // every load of the tile goes out before any is used: one memory latency
+  const bool first = dots && S[b].it == 0;
+  const double rv = (dots && in) ? r[(size_t)b * rstride + p] : 0.0;
+  const double qv = (dots && in && !first) ? q[(size_t)b * rstride + p] : 0.0;
+  X0Tile<2> xt;
+  x0_issue<2>(xt, r + (size_t)b * rstride, x0, y0, H, W);
+  const int H1 = H / 2, W1 = W / 2;
+  const bf16 *b1b = b1 + (size_t)b * H1 * W1 * C0;
+  static_assert(CE * CE * (C0 / 8) <= 256, "one coarse load per thread");
+  const int c8 = threadIdx.x % (C0 / 8), cq = threadIdx.x / (C0 / 8);
+  const bool cin = threadIdx.x < CE * CE * (C0 / 8);
+  bf16x8 cv;
+  if (cin) {
+    const int Y = wm_wrap(y0 / 2 + cq / CE - 1, H1), X = wm_wrap(x0 / 2 + cq % CE - 1, W1);
+    __builtin_memcpy(&cv, &b1b[((size_t)Y * W1 + X) * C0 + 8 * c8], 16);
+  }
+  const float fpv = fold[FP_OFF + threadIdx.x];
+  const float fkv = threadIdx.x < 25 ? fold[FK_OFF + threadIdx.x] : 0.0f;
+  const double scale = sc[b];
+  x0_put<2>(xs, xt, sc[B + b]);
+  if (cin) {
+    float4 *dst = (float4 *)&bs[cq * CS + 8 * c8];
+    dst[0] = make_float4((float)cv[0], (float)cv[1], (float)cv[2], (float)cv[3]);
+    dst[1] = make_float4((float)cv[4], (float)cv[5], (float)cv[6], (float)cv[7]);
+  }
+  fp[(threadIdx.x / (4 * C0)) * PS + threadIdx.x % (4 * C0)] = fpv;
+  if (threadIdx.x < 25) fk[threadIdx.x] = fkv;
+  __syncthreads();
   float v = 0.0f;
 #pragma unroll
   for (int k = 0; k < 25; ++k) v = fmaf(fk[k], xs[(ty + k / 5) * (T + 4) + tx + k % 5], v);
   const int py = ty & 1, px = tx & 1;
-  const float *fpp = fp + (py * 2 + px) * 4 * C0;
+  const float *fpp = fp + (py * 2 + px) * PS;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb) {
-      // coarse pixel (ty>>1) + a - 1 + py in the tile frame, +1 for the halo
-      const float *bp = bs + ((ty / 2 + a + py) * CE + (tx / 2 + bb + px)) * C0;
-      const float *wv = fpp + (a * 2 + bb) * C0;
+      // coarse pixel (ty>>1) + a - 1 + py in the tile frame, +1 for the halo;
+      // 16 channels as four 16-byte LDS reads of features and of weights
+      const float4 *bp = (const float4 *)(bs + ((ty / 2 + a + py) * CE + (tx / 2 + bb + px)) * CS);
+      const float4 *wv = (const float4 *)(fpp + (a * 2 + bb) * C0);
 #pragma unroll
-      for (int ci = 0; ci < C0; ++ci) v = fmaf(wv[ci], bp[ci], v);
+      for (int c4 = 0; c4 < C0 / 4; ++c4) {
+        const float4 f = bp[c4], w = wv[c4];
+        v = fmaf(w.x, f.x, v);
+        v = fmaf(w.y, f.y, v);
+        v = fmaf(w.z, f.z, v);
+        v = fmaf(w.w, f.w, v);
+      }
     }
-  const size_t p = (size_t)gy * W + gx;
-  const double zv = (double)v * sc[b];
+  const double zv = (double)v * scale;
   if (in) {
     z[(size_t)b * zstride + p] = zv;
     if (y) y[(size_t)b * H * W + p] = v;
   }
   if (dots) {
-    const bool first = S[b].it == 0;
-    const double zr = in ? zv * r[(size_t)b * rstride + p] : 0.0;
-    const double zq = (in && !first) ? zv * q[(size_t)b * rstride + p] : 0.0;
-    const int tiles = gridDim.x * gridDim.y, tile = blockIdx.y * gridDim.x + blockIdx.x;
+    const double zr = in ? zv * rv : 0.0;
+    const double zq = (in && !first) ? zv * qv : 0.0;
+    const int tiles = gridDim.x * gridDim.y, tile = tl.y * gridDim.x + tl.x;
     // fixed order: a butterfly within each wave, then the 4 wave totals
     const double w0 = wm_wave_sum(zr), w1 = wm_wave_sum(zq);
     if ((threadIdx.x & 63) == 0) {
