@@ -55,13 +55,19 @@ __device__ inline int coeff_at(int k) { return 12 * k; }
 __device__ inline void cell_coeffs(float *cf, const float *T, int c, int k) {
   const double M[3][3] = {{0.5, -1.0, 0.5}, {0.5, 1.0, -1.0}, {0.0, 0.0, 0.5}};
   const float *Tk = T + k * 3 * TW + c;
+  // C_ij = sum_b (sum_a T(b,a) M[a][i]) M[b][j], in FP64, rounded once
+  double R[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      R[b][i] = (double)Tk[b * TW] * M[0][i] + (double)Tk[b * TW + 1] * M[1][i] +
+                (double)Tk[b * TW + 2] * M[2][i];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      double v = 0.0;
-      for (int b = 0; b < 3; ++b)
-        for (int a = 0; a < 3; ++a) v += (double)Tk[b * TW + a] * M[a][i] * M[b][j];
-      cf[coeff_at(k) + i * 3 + j] = (float)v;
-    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      cf[coeff_at(k) + i * 3 + j] = (float)(R[0][i] * M[0][j] + R[1][i] * M[1][j] + R[2][i] * M[2][j]);
 }
 
 // particle.f90:96-161 in FP32 on the cell frame.
@@ -181,8 +187,12 @@ __device__ inline void stencil32(float *acc, const Move32 &m, float qc, float qv
   }
 }
 
-// Stay-in-cell particle: the 27 possibly non-zero stencil values (same
-// layout as the FP64 kernel's core set).
+// Stay-in-cell particle: the 21 possibly non-zero stencil values.  Of the
+// 3x3 core, Jx's last row and Jy's last column vanish: their weights are
+// -qc times the sum of all three shape differences, which is zero because
+// the quadratic shape functions sum to one.  Layout: Jx (a < 2, b < 3) at
+// 0..5, Jy (a < 3, b < 2) at 6..11, Jz (3 x 3) at 12..20.
+constexpr int NCORE = 21;
 __device__ inline void core32(float *core, float ox0, float oy0, float ox1, float oy1, float qc,
                               float qv) {
   float sx[3], sy[3], dx[3], dy[3];
@@ -195,16 +205,23 @@ __device__ inline void core32(float *core, float ox0, float oy0, float ox1, floa
     dx[k] -= sx[k];
     dy[k] -= sy[k];
   }
-  float X[3], Y[3], wx[3], wy[3];
-  float w1 = 0.0f, w2 = 0.0f;
+  float X[3], Y[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     X[k] = sx[k] + 0.5f * dx[k];
     Y[k] = sy[k] + 0.5f * dy[k];
-    w1 -= qc * dx[k];
-    w2 -= qc * dy[k];
-    wx[k] = w1;
-    wy[k] = w2;
+  }
+  const float wx0 = -(qc * dx[0]), wx1 = wx0 - qc * dx[1];
+  const float wy0 = -(qc * dy[0]), wy1 = wy0 - qc * dy[1];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    core[b] += wx0 * Y[b];
+    core[3 + b] += wx1 * Y[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    core[6 + 2 * a] += X[a] * wy0;
+    core[7 + 2 * a] += X[a] * wy1;
   }
   const float third = 1.0f / 3.0f;
 #pragma unroll
@@ -212,23 +229,19 @@ __device__ inline void core32(float *core, float ox0, float oy0, float ox1, floa
     const float za = qv * X[a];
     const float zb = qv * (0.5f * sx[a] + third * dx[a]);
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      core[a * 3 + b] += wx[a] * Y[b];
-      core[9 + a * 3 + b] += X[a] * wy[b];
-      core[18 + a * 3 + b] += za * sy[b] + zb * dy[b];
-    }
+    for (int b = 0; b < 3; ++b) core[12 + a * 3 + b] += za * sy[b] + zb * dy[b];
   }
 }
 
+// Slot of core value c in a cell's 65 stencil sums (Jx 0..19 as a*5 + b with
+// a < 4, Jy 20..39 as 20 + a*4 + b with b < 4, Jz 40..64 as 40 + a*5 + b); the
+// core sits at 5-point indices 1..3.
 __device__ inline int core_slot(int c) {
-  const int comp = c / 9, a = (c % 9) / 3 + 1, b = c % 3 + 1;
-  if (comp == 0) return a * 5 + b;
-  if (comp == 1) return 20 + a * 4 + b;
-  return 40 + a * 5 + b;
+  if (c < 6) return (c / 3 + 1) * 5 + (c % 3 + 1);
+  if (c < 12) return 20 + ((c - 6) / 2 + 1) * 4 + ((c - 6) % 2 + 1);
+  return 40 + ((c - 12) / 3 + 1) * 5 + ((c - 12) % 3 + 1);
 }
 
-// Value (a, b) of component comp of a cell's stencil sums (Jx has no a = 0
-// column and Jy no b = 0 row); branch-free index arithmetic.
 __device__ inline double acc_at(const double *cb, int comp, int a, int b) {
   const int idx = comp == 0 ? (a - 1) * 5 + b : (comp == 1 ? 20 + a * 4 + (b - 1) : 40 + a * 5 + b);
   const bool none = (comp == 0 && a == 0) || (comp == 1 && b == 0);
@@ -541,9 +554,9 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   Lane L;
   L.xcnt = 0;
   L.ok = true;
-  float core[27];
+  float core[NCORE];
 #pragma unroll
-  for (int k = 0; k < 27; ++k) core[k] = 0.0f;
+  for (int k = 0; k < NCORE; ++k) core[k] = 0.0f;
   if (g < ncell) {
     // the next species' windows load while this one is pushed
     if (PAIR) {
@@ -560,7 +573,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     }
     // stay-in-cell stencils of every species, summed over the group once
 #pragma unroll
-    for (int c = 0; c < 27; ++c) {
+    for (int c = 0; c < NCORE; ++c) {
       const float v = row_sum16(core[c]);
       if (sub == 0) cb[core_slot(c)] += (double)v;
     }
