@@ -440,6 +440,7 @@ int wm_alloc_device(wm_ctx *ctx) {
   // field.f90:105-119: df, gkl, uj start at zero
   WM_TRY(fill_zero(ctx, ctx->uf, 6 * fg.ps));
   WM_TRY(fill_zero(ctx, ctx->df, 6 * fg.ps));
+  ctx->x0_fresh = false;
   WM_TRY(fill_zero(ctx, ctx->uj, 3 * fg.ps));
   WM_TRY(fill_zero(ctx, ctx->gkl, 3 * n));
   WM_HIP(hipStreamSynchronize(ctx->stream));

@@ -270,7 +270,8 @@ __device__ inline CgCell cg_cell(const CgArgs &A) {
 __global__ __launch_bounds__(CGB) void k_cg_init(CgArgs A, const double *__restrict__ gkl,
                                                  const double *__restrict__ df,
                                                  double *__restrict__ r, double *__restrict__ p,
-                                                 double *partials) {
+                                                 double *partials,
+                                                 double *__restrict__ x0 = nullptr) {
   __shared__ double sh[2][CGB];
   const int l = blockIdx.y;
   const CgCell C = cg_cell(A);
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(CGB) void k_cg_init(CgArgs A, const double *__restr
                       PH(ii + 1, jj) + PH(ii, jj + 1);
     r[(size_t)l * A.n + t] = rr;
     p[(size_t)l * A.n + t] = rr;
+    if (x0) x0[(size_t)l * A.n + t] = PH(ii, jj);   // the next flexible solve's warm start
     v[0] = b * b;
     v[1] = rr * rr;
   }
@@ -627,7 +629,9 @@ static int fcg_solve(wm_ctx *ctx, const CgArgs &A, int *its) {
   const dim3 grid(A.nblk, 3);
   hipStream_t st = ctx->stream;
   double *pbuf[2] = {ctx->cg_p[0], ctx->cg_p[1]};
-  hipLaunchKernelGGL(k_fcg_save, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_x0);
+  // the warm start, unless the last solve's judge already left it in cg_x0
+  if (!ctx->x0_fresh) hipLaunchKernelGGL(k_fcg_save, grid, dim3(CGB), 0, st, A, ctx->df, ctx->cg_x0);
+  ctx->x0_fresh = false;
   hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, st, A, ctx->gkl, ctx->df, ctx->cg_r, pbuf[0],
                      ctx->red);
   hipLaunchKernelGGL(k_fcg_fin, dim3(3), dim3(CGB), 0, st, A, 0, ctx->red, S, ctx->pc_sc,
@@ -693,8 +697,11 @@ int wm_k_field_solve(wm_ctx *ctx) {
   // the reference's plain CG; after flexible CG it recomputes the true
   // residual and only iterates if that misses the tolerance
   double *pbuf[2] = {ctx->cg_p[0], ctx->cg_p[1]};
+  // after flexible CG the judge also copies phi to cg_x0: if no plain
+  // iteration follows, that is the next flexible solve's warm start
+  double *x0 = ctx->solver == WM_SOLVER_FCG ? ctx->cg_x0 : nullptr;
   hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, ctx->stream, A, ctx->gkl, ctx->df,
-                     ctx->cg_r, pbuf[0], ctx->red);
+                     ctx->cg_r, pbuf[0], ctx->red, x0);
   hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 0, ctx->red, ctx->cg_s);
   WM_HIP(hipGetLastError());
   const bool collect = ctx->solver == WM_SOLVER_CG && wm_pc_collecting(ctx);
@@ -738,6 +745,7 @@ int wm_k_field_solve(wm_ctx *ctx) {
     batch = collect ? 1 : 2;
   }
   if (collect) WM_TRY(wm_pc_snapshot_fin(ctx));
+  ctx->x0_fresh = x0 != nullptr && iters == 0;
   if (iters > 0) ctx->cg_hint = std::max(hs[0].ite, std::max(hs[1].ite, hs[2].ite));
   for (int l = 0; l < 3; ++l) ctx->last_iters[l] = fcg_it[l] + hs[l].ite;
   if (ctx->solver == WM_SOLVER_FCG) {

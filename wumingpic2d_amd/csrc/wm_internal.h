@@ -189,6 +189,7 @@ struct wm_ctx {
   // iterations queued before the first host check: the previous solve's
   // count, so a solve that converges as the last one did needs one check
   int fcg_hint = 2, cg_hint = 4;
+  bool x0_fresh = false;         // cg_x0 holds df's solution planes (the warm start)
   wm_pc *pc = nullptr;
   double *cg_z = nullptr, *cg_q = nullptr, *cg_x0 = nullptr;   // 3 planes nx*ny each
   double *pc_sc = nullptr;       // s[3], 1/s[3] of the current residuals
