@@ -126,67 +126,6 @@ __device__ inline int rebase(float o, float &onew) {
   return d;
 }
 
-struct Move32 {
-  float s0x[5], s0y[5], dsx[5], dsy[5];
-};
-
-__device__ inline void place_s1(float a, float b, float c, int inc, float *ds) {
-  ds[0] = inc < 0 ? a : 0.0f;
-  ds[1] = inc < 0 ? b : (inc == 0 ? a : 0.0f);
-  ds[2] = inc < 0 ? c : (inc == 0 ? b : a);
-  ds[3] = inc > 0 ? b : (inc == 0 ? c : 0.0f);
-  ds[4] = inc > 0 ? c : 0.0f;
-}
-
-// field.f90:224-268 on offsets: old offsets (ox0, oy0) in cell (0, 0), new
-// rebased offsets (ox1, oy1) in cell (dx, dy).
-__device__ inline void make_move32(Move32 &m, float ox0, float oy0, float ox1, float oy1, int dx,
-                                   int dy) {
-  m.s0x[0] = m.s0x[4] = m.s0y[0] = m.s0y[4] = 0.0f;
-  shapef(ox0 - 0.5f, m.s0x[1], m.s0x[2], m.s0x[3]);
-  shapef(oy0 - 0.5f, m.s0y[1], m.s0y[2], m.s0y[3]);
-  float a, b, c;
-  shapef(ox1 - 0.5f, a, b, c);
-  place_s1(a, b, c, dx, m.dsx);
-  shapef(oy1 - 0.5f, a, b, c);
-  place_s1(a, b, c, dy, m.dsy);
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    m.dsx[k] -= m.s0x[k];
-    m.dsy[k] -= m.s0y[k];
-  }
-}
-
-template <int C>
-__device__ inline void stencil32(float *acc, const Move32 &m, float qc, float qv) {
-  if (C == 0) {
-    float w = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      w -= qc * m.dsx[a];
-#pragma unroll
-      for (int b = 0; b < 5; ++b) acc[a * 5 + b] += w * (m.s0y[b] + 0.5f * m.dsy[b]);
-    }
-  } else if (C == 1) {
-    float w = 0.0f;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      w -= qc * m.dsy[b];
-#pragma unroll
-      for (int a = 0; a < 5; ++a) acc[a * 4 + b] += (m.s0x[a] + 0.5f * m.dsx[a]) * w;
-    }
-  } else {
-    const float third = 1.0f / 3.0f;
-#pragma unroll
-    for (int a = 0; a < 5; ++a) {
-      const float za = qv * (m.s0x[a] + 0.5f * m.dsx[a]);
-      const float zb = qv * (0.5f * m.s0x[a] + third * m.dsx[a]);
-#pragma unroll
-      for (int b = 0; b < 5; ++b) acc[a * 5 + b] += za * m.s0y[b] + zb * m.dsy[b];
-    }
-  }
-}
-
 // Stay-in-cell particle: the 21 possibly non-zero stencil values.  Of the
 // 3x3 core, Jx's last row and Jy's last column vanish: their weights are
 // -qc times the sum of all three shape differences, which is zero because
@@ -285,79 +224,82 @@ __shared__ float sqc[WM_MAXSP];                   // q delx / delt per species (
                                                   // wait for the stores in flight)
 
 // Shape weight at index a (0..4) of the 5-point stencil for offset o whose
-// 3-point support starts at index 1 + inc (inc = cell change, -1..1): the
-// values make_move32 places.
+// 3-point support starts at index 1 + inc (inc = cell change, -1..1; the old
+// position's support starts at index 1).
 __device__ inline float shape5(float sm, float s0, float sp, int inc, int a) {
   const int r = a - 1 - inc;
   return r == 0 ? sm : (r == 1 ? s0 : (r == 2 ? sp : 0.0f));
 }
 
-// The deposit of a cell's movers, one (component, row) of the 5x5 stencil per
-// thread: task 0..4 = Jx row b (values a = 0..3), 5..8 = Jy column b
-// (a = 0..4), 9..13 = Jz row b (a = 0..4), each value summed over the movers
-// in list order in FP64, as the FP32 stencil of stencil32 gives it.
-__device__ inline void mover_row(double *cb, int task, int nm, const float4 *xp,
-                                 const float *xq, const int *xg, const PP32 &P) {
-  const int comp = task < 5 ? 0 : (task < 9 ? 1 : 2);
-  const int b = task - (comp == 0 ? 0 : (comp == 1 ? 5 : 9));
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int j = 0; j < nm; ++j) {
+// The deposit of one cell's movers j = p, p + MQ, ... (part p of MQ) into
+// the 65 stencil sums of the cell (Jx a < 4 at a*5 + b, Jy b < 4 at
+// 20 + a*4 + b, Jz at 40 + a*5 + b): field.f90:224-268's Esirkepov weights
+// with s0 the old and ds the new minus the old 5-point shapes, in FP32,
+// summed over the part's movers in FP32.  Jz uses
+// s0x s0y + ds_x s0y / 2 + s0x ds_y / 2 + ds_x ds_y / 3 = X Y + ds_x ds_y / 12
+// with X = s0x + ds_x / 2, Y = s0y + ds_y / 2.
+constexpr int MQ = 4;
+// The 5-point old shape (support at 1..3) and new shape (support at
+// 1 + inc .. 3 + inc): ds = new - old and X = old + ds / 2.  Selects on
+// three flags rather than on inc's value, which the compiler would turn into
+// a branching switch.
+__device__ inline void place5(float sm, float s0, float sp, float nm, float n0, float np, int inc,
+                              float *X, float *ds) {
+  const bool lo = inc < 0, mid = inc == 0, hi = inc > 0;
+  const float n[5] = {lo ? nm : 0.0f, lo ? n0 : (mid ? nm : 0.0f), lo ? np : (mid ? n0 : nm),
+                      hi ? n0 : (mid ? np : 0.0f), hi ? np : 0.0f};
+  const float o[5] = {0.0f, sm, s0, sp, 0.0f};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    ds[k] = n[k] - o[k];
+    X[k] = o[k] + 0.5f * ds[k];
+  }
+}
+__device__ inline void mover_part(float *acc, int p, int nm, const float4 *xp, const float *xq,
+                                  const int *xg) {
+  const float twelfth = 1.0f / 12.0f;
+  for (int j = p; j < nm; j += MQ) {
     const float4 v = xp[j];
-    const float ox0 = v.x, oy0 = v.y, ox1 = v.z, oy1 = v.w, qv = xq[j];
-    const int dir = (xg[j] >> 20) & 0xF;
-    const int dy = dir / 3 - 1, dx = dir % 3 - 1;
-    const float qc = sqc[xg[j] >> 28];
+    const float qv = xq[j];
+    const int xv = xg[j];
+    const int dir = (xv >> 20) & 0xF;
+    const int dy = dir / 3 - 1, dx = dir - 3 * (dy + 1) - 1;
+    const float qc = sqc[xv >> 28];
     float xm, x0, xp_, ym, y0, yp, nxm, nx0, nxp, nym, ny0, nyp;
-    shapef(ox0 - 0.5f, xm, x0, xp_);
-    shapef(oy0 - 0.5f, ym, y0, yp);
-    shapef(ox1 - 0.5f, nxm, nx0, nxp);
-    shapef(oy1 - 0.5f, nym, ny0, nyp);
-    float s0x[5], dsx[5];
+    shapef(v.x - 0.5f, xm, x0, xp_);
+    shapef(v.y - 0.5f, ym, y0, yp);
+    shapef(v.z - 0.5f, nxm, nx0, nxp);
+    shapef(v.w - 0.5f, nym, ny0, nyp);
+    float X[5], dX[5], Y[5], dY[5];
+    place5(xm, x0, xp_, nxm, nx0, nxp, dx, X, dX);
+    place5(ym, y0, yp, nym, ny0, nyp, dy, Y, dY);
+    // Jx (a, b) = w_a Y_b, w_a = -qc sum_{a' <= a} ds_x[a']; Jy likewise in y
+    float wx = 0.0f, wy[4];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      s0x[k] = shape5(xm, x0, xp_, 0, k);
-      dsx[k] = shape5(nxm, nx0, nxp, dx, k) - s0x[k];
+    for (int k = 0; k < 4; ++k) wy[k] = (k ? wy[k - 1] : 0.0f) - qc * dY[k];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      wx -= qc * dX[a];
+#pragma unroll
+      for (int b = 0; b < 5; ++b) acc[a * 5 + b] += wx * Y[b];
     }
-    if (comp == 1) {
-      // column b of Jy: w_b = -qc * sum_{b' <= b} dsy[b']
-      float w = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (k <= b) w -= qc * (shape5(nym, ny0, nyp, dy, k) - shape5(ym, y0, yp, 0, k));
+    for (int a = 0; a < 5; ++a) {
 #pragma unroll
-      for (int a = 0; a < 5; ++a) acc[a] += (double)((s0x[a] + 0.5f * dsx[a]) * w);
-    } else {
-      const float s0y = shape5(ym, y0, yp, 0, b);
-      const float dsy = shape5(nym, ny0, nyp, dy, b) - s0y;
-      if (comp == 0) {
-        const float Y = s0y + 0.5f * dsy;
-        float w = 0.0f;
+      for (int b = 0; b < 4; ++b) acc[20 + a * 4 + b] += X[a] * wy[b];
+      const float za = qv * X[a], zb = (qv * twelfth) * dX[a];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          w -= qc * dsx[a];
-          acc[a] += (double)(w * Y);
-        }
-      } else {
-        const float third = 1.0f / 3.0f;
-#pragma unroll
-        for (int a = 0; a < 5; ++a) {
-          const float za = qv * (s0x[a] + 0.5f * dsx[a]);
-          const float zb = qv * (0.5f * s0x[a] + third * dsx[a]);
-          acc[a] += (double)(za * s0y + zb * dsy);
-        }
-      }
+      for (int b = 0; b < 5; ++b) acc[40 + a * 5 + b] += za * Y[b] + zb * dY[b];
     }
   }
-  if (comp == 0) {
-#pragma unroll
-    for (int a = 0; a < 4; ++a) cb[a * 5 + b] += acc[a];
-  } else if (comp == 1) {
-#pragma unroll
-    for (int a = 0; a < 5; ++a) cb[20 + a * 4 + b] += acc[a];
-  } else {
-#pragma unroll
-    for (int a = 0; a < 5; ++a) cb[40 + a * 5 + b] += acc[a];
-  }
+}
+
+// Sum over the four lanes of a DPP quad; lane 0 of the quad holds
+// (v0 + v1) + (v2 + v3).
+__device__ inline float quad_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+  return v;
 }
 
 constexpr int PD = 5;           // windows of a bin held in registers (80 particles)
@@ -588,12 +530,23 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   }
   __syncthreads();                                   // also orders the movers' stores
   const int gm = g < ncell ? gcnt[g] : 0;
-  // the movers' deposit: thread t takes (cell t / 14, row t % 14)
-  if (threadIdx.x < CHUNK * 14) {
-    const int cg = threadIdx.x / 14;
-    if (cg < ncell)
-      mover_row(cellbuf + cg * NACC, threadIdx.x % 14, gcnt[cg], xpl + cg * XC, xql + cg * XC,
-                xl + cg * WM_MG, P);
+  // the movers' deposit, on the first wave alone: lane t takes the cell's
+  // movers j = t % MQ, t % MQ + MQ, ... of cell t / MQ
+  static_assert(CHUNK * MQ == 64, "the movers' deposit runs on one wave");
+  if (threadIdx.x < CHUNK * MQ) {
+    const int mc = threadIdx.x / MQ;
+    float acc[NACC];
+#pragma unroll
+    for (int e = 0; e < NACC; ++e) acc[e] = 0.0f;
+    mover_part(acc, threadIdx.x % MQ, mc < ncell ? gcnt[mc] : 0, xpl + mc * XC, xql + mc * XC,
+               xl + mc * WM_MG);
+    double *mcb = cellbuf + mc * NACC;
+    const bool lead = threadIdx.x % MQ == 0 && mc < ncell;
+#pragma unroll
+    for (int e = 0; e < NACC; ++e) {
+      const float v = quad_sum(acc[e]);
+      if (lead) mcb[e] += (double)v;
+    }
   }
   // per bin: movers in total and per direction
   if (g < ncell) {
