@@ -41,6 +41,7 @@ __device__ inline void shapef(float dh, float &sm, float &s0, float &sp) {
 
 struct Push32 {
   float ox, oy, ux, uy, uz;   // ox unnormalised (relative to the old cell)
+  float ig;                   // 1 / gamma of the new momentum
 };
 
 // The TSC gather of particle.f90:107-129 inside one cell is a biquadratic
@@ -114,61 +115,79 @@ __device__ inline Push32 boris32(const float4 *cf4, float ox, float oy, float ux
   o.uy = uvm2 + fac1 * epy;
   o.uz = uvm3 + fac1 * epz;
   const float ig = __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2);
+  o.ig = ig;
   o.ox = ox + o.ux * P.delt * ig;
   o.oy = oy + o.uy * P.delt * ig;
   return o;
 }
 
 // Cell change and the offset rebased on the new cell, kept in [0, 1).
+// Branch-free: o - d is exact for d = 1 (o in [1, 2)) and o itself for d = 0;
+// for d = -1 the sum o + 1 may round up to 1, which the min takes below one.
 __device__ inline int rebase(float o, float &onew) {
-  const int d = o < 0.0f ? -1 : (o >= 1.0f ? 1 : 0);
-  onew = d > 0 ? o - 1.0f : (d < 0 ? wm_below_one(o + 1.0f) : o);
+  const int d = (int)(o >= 1.0f) - (int)(o < 0.0f);
+  onew = fminf(o - (float)d, 0x1.fffffep-1f);
   return d;
+}
+
+// Quadratic spline weights at the cell's -1, 0, +1 neighbours for the old
+// offset o0, s- = (1 - o0)^2 / 2, s+ = o0^2 / 2, s0 = 1 - s- - s+, and their
+// change to the new offset o1 of the same cell, taken directly rather than as
+// a difference: d+ = (o1 - o0)(o1 + o0) / 2, d- = (o1 - o0)(o1 + o0 - 2) / 2,
+// d0 = -(d- + d+).
+__device__ inline void shape_delta(float o0, float o1, float *s, float *d) {
+  const float t = 1.0f - o0;
+  s[0] = (0.5f * t) * t;
+  s[2] = (0.5f * o0) * o0;
+  s[1] = (1.0f - s[0]) - s[2];
+  const float hd = 0.5f * (o1 - o0), sg = o1 + o0;
+  d[2] = hd * sg;
+  d[0] = hd * (sg - 2.0f);
+  d[1] = -(d[0] + d[2]);
 }
 
 // Stay-in-cell particle: the 21 possibly non-zero stencil values.  Of the
 // 3x3 core, Jx's last row and Jy's last column vanish: their weights are
-// -qc times the sum of all three shape differences, which is zero because
-// the quadratic shape functions sum to one.  Layout: Jx (a < 2, b < 3) at
-// 0..5, Jy (a < 3, b < 2) at 6..11, Jz (3 x 3) at 12..20.
+// -qc times the sum of all three shape changes, which is zero because the
+// quadratic shape functions sum to one; for the same reason the middle row's
+// weight -qc (d- + d0) is qc d+.  Jz's weight
+// s0x s0y + dx s0y / 2 + s0x dy / 2 + dx dy / 3 is X Y + dx dy / 12 with
+// X = s0x + dx / 2, Y = s0y + dy / 2.  Layout: Jx (a < 2, b < 3) at 0..5,
+// Jy (a < 3, b < 2) at 6..11, Jz (3 x 3) at 12..20.
 constexpr int NCORE = 21;
 __device__ inline void core32(float *core, float ox0, float oy0, float ox1, float oy1, float qc,
                               float qv) {
   float sx[3], sy[3], dx[3], dy[3];
-  shapef(ox0 - 0.5f, sx[0], sx[1], sx[2]);
-  shapef(oy0 - 0.5f, sy[0], sy[1], sy[2]);
-  shapef(ox1 - 0.5f, dx[0], dx[1], dx[2]);
-  shapef(oy1 - 0.5f, dy[0], dy[1], dy[2]);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    dx[k] -= sx[k];
-    dy[k] -= sy[k];
-  }
+  shape_delta(ox0, ox1, sx, dx);
+  shape_delta(oy0, oy1, sy, dy);
   float X[3], Y[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    X[k] = sx[k] + 0.5f * dx[k];
-    Y[k] = sy[k] + 0.5f * dy[k];
+    X[k] = fmaf(0.5f, dx[k], sx[k]);
+    Y[k] = fmaf(0.5f, dy[k], sy[k]);
   }
-  const float wx0 = -(qc * dx[0]), wx1 = wx0 - qc * dx[1];
-  const float wy0 = -(qc * dy[0]), wy1 = wy0 - qc * dy[1];
+  const float wx0 = -(qc * dx[0]), wx1 = qc * dx[2];
+  const float wy0 = -(qc * dy[0]), wy1 = qc * dy[2];
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    core[b] += wx0 * Y[b];
-    core[3 + b] += wx1 * Y[b];
+    core[b] = fmaf(wx0, Y[b], core[b]);
+    core[3 + b] = fmaf(wx1, Y[b], core[3 + b]);
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    core[6 + 2 * a] += X[a] * wy0;
-    core[7 + 2 * a] += X[a] * wy1;
+    core[6 + 2 * a] = fmaf(X[a], wy0, core[6 + 2 * a]);
+    core[7 + 2 * a] = fmaf(X[a], wy1, core[7 + 2 * a]);
   }
-  const float third = 1.0f / 3.0f;
+  const float qv12 = qv * (1.0f / 12.0f);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float za = qv * X[a];
-    const float zb = qv * (0.5f * sx[a] + third * dx[a]);
+    const float za = qv * X[a], zb = qv12 * dx[a];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) core[12 + a * 3 + b] += za * sy[b] + zb * dy[b];
+    for (int b = 0; b < 3; ++b) {
+      float &c = core[12 + a * 3 + b];
+      c = fmaf(za, Y[b], c);
+      c = fmaf(zb, dy[b], c);
+    }
   }
 }
 
@@ -379,16 +398,15 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
   float ox1, oy1;
   const int dx = rebase(o.ox, ox1);
   const int dy = rebase(o.oy, oy1);
-  L.ok = L.ok && (!on || (fabsf(o.ox - 0.5f) < 1.5f && fabsf(o.oy - 0.5f) < 1.5f));
+  L.ok = L.ok & (!on | (fmaxf(fabsf(o.ox - 0.5f), fabsf(o.oy - 0.5f)) < 1.5f));
   const bool cross = on && (dx | dy) != 0;
   int kk = min(k, max(S.n - 1, 0));
   __asm__ volatile("" : "+v"(kk));                 // store addresses formed here, not earlier
   const size_t p = spc_base(S) + kk;
   A.q[p] = make_float4(ox1, oy1, o.ux, o.uy);
   A.uz[p] = o.uz;
+  const float qv = S.q * (o.uz * o.ig);            // charge times v_z
   if (on && !cross) {
-    const float qv =
-        S.q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
     core32(core, ox0, oy0, ox1, oy1, S.qc, qv);
   }
   // movers in (species, slot) order
@@ -408,10 +426,8 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
       ((float2 *)rp)[4] = make_float2(o.uz, __int_as_float(k));
       M.dir[lane_reg(P) + pos] = (uint8_t)dir;
       xl[lane_g() * WM_MG + pos] = (S.s << 28) | (dir << 20) | k;
-      {
-        xpl[lane_g() * XC + pos] = make_float4(ox0, oy0, ox1, oy1);
-        xql[lane_g() * XC + pos] = S.q * (o.uz * __builtin_amdgcn_rsqf(1.0f + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) * P.inv_c2));
-      }
+      xpl[lane_g() * XC + pos] = make_float4(ox0, oy0, ox1, oy1);
+      xql[lane_g() * XC + pos] = qv;
     }
   }
   L.xcnt += __popc(gm);
