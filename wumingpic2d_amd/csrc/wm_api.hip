@@ -364,23 +364,26 @@ int wm_alloc_device(wm_ctx *ctx) {
     // cell bins with room for 1.5x the row capacity's mean per cell, in
     // 16-slot (64-byte) units; the dense row bank converts state in and out
     ctx->CB = std::max(16, (int)((1.5 * ctx->cap / g.nx + 15) / 16) * 16);
-    if (ctx->CB >= (1 << 20)) {
-      wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %d particles per cell bin exceed 2^20", ctx->CB);
+    if (ctx->CB > (1 << WM_SLOT_BITS)) {
+      wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %d particles per cell bin exceed %d", ctx->CB,
+                   1 << WM_SLOT_BITS);
       return WM_ERR_ARG;
     }
     const size_t nbin = (size_t)nseg * g.nx, bslots = nbin * ctx->CB;
+    if (nbin > 0xFFFFFFFFu) {
+      wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %zu cell bins exceed 2^32", nbin);
+      return WM_ERR_ARG;
+    }
     WM_TRY(dmalloc(ctx, &ctx->bins.q, bslots));
     WM_TRY(dmalloc(ctx, &ctx->bins.uz, bslots));
     WM_TRY(dmalloc(ctx, &ctx->id_bins, bslots));
     WM_TRY(dmalloc(ctx, &ctx->bin_cnt, nbin));
-    WM_TRY(dmalloc(ctx, &ctx->bin_mout, nbin));
-    WM_TRY(dmalloc(ctx, &ctx->bin_mcnt, nbin * 8));
+    WM_TRY(dmalloc(ctx, &ctx->bin_sum, nbin));
     WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
     ctx->nregion = wm_push32_regions(g.nx, g.ny);
     const size_t ms = (size_t)ctx->nregion * WM_MG;
     Movers &m = ctx->mv;
     WM_TRY(dmalloc(ctx, &m.rec, ms));
-    WM_TRY(dmalloc(ctx, &m.dir, ms));
     WM_TRY(dmalloc(ctx, &m.n, (size_t)ctx->nregion));
     WM_HIP(hipMemsetAsync(m.n, 0, sizeof(int) * ctx->nregion, ctx->stream));
     WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
@@ -490,7 +493,7 @@ int wm_create(wm_ctx **out, int device) {
     delete ctx;
     return WM_ERR_NODEV;
   }
-  for (int k = 0; k < 16; ++k) (void)hipEventCreate(&ctx->ev[k]);
+  for (int k = 0; k < 32; ++k) (void)hipEventCreate(&ctx->ev[k]);
   *out = ctx;
   return WM_OK;
 }
@@ -512,13 +515,13 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
-                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_mout, ctx->bin_mcnt,
-                  ctx->mv.rec, ctx->mv.dir, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum,
+                  ctx->mv.rec, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   wm_pc_free(ctx);
-  for (int k = 0; k < 16; ++k) (void)hipEventDestroy(ctx->ev[k]);
+  for (int k = 0; k < 32; ++k) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
   (void)hipStreamDestroy(ctx->stream2);
   if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
@@ -851,22 +854,50 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     const bool f32 = ctx->form == WM_FORM_FP32;
     WM_TRY(f32 ? wm_k_cell_fields32(ctx) : wm_k_cell_fields(ctx));
     WM_HIP(hipEventRecord(e[1], ctx->stream));
-    WM_TRY(f32 ? wm_k_push_deposit32(ctx) : wm_k_push_deposit(ctx, 0, 1));
-    WM_HIP(hipEventRecord(e[2], ctx->stream));
     // The particle boundaries and the sort (app.f90:104-106) read only the
     // pushed particles, and the field solve (app.f90:103) only the current
     // and the fields: they run concurrently, the particles on the second
     // stream, and the next step waits for both.
-    WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
-    {
+    if (f32) {
+      // Cell bins: the push wraps x itself; the particles that changed cell
+      // join their destination bins in the migration.  The push runs in row
+      // bands, and band b's migration (which reads the push results of rows
+      // b's first - 1 .. b's last + 1) starts on the second stream as soon as
+      // band b + 1 is pushed, beside the later bands' push; the last band's
+      // and (its rows wrapping to the last band) the first band's follow the
+      // last push, beside the field solve.
+      const int nb = std::max(1, std::min(WM_PUSH_BANDS, ctx->g.ny / 2));
+      auto row = [&](int b) { return (int)((long long)b * ctx->g.ny / nb); };
+      for (int b = 0; b < nb; ++b) {
+        WM_TRY(wm_k_push_deposit32(ctx, row(b), row(b + 1) - row(b)));
+        if (b >= 2) {
+          WM_HIP(hipEventRecord(e[16 + b], ctx->stream));
+          WM_HIP(hipStreamWaitEvent(ctx->stream2, e[16 + b], 0));
+          if (b == 2) {
+            WM_HIP(hipEventRecord(e[5], ctx->stream2));
+            WM_HIP(hipEventRecord(e[6], ctx->stream2));
+          }
+          StreamSwap on2(ctx, ctx->stream2);
+          WM_TRY(wm_k_insert_movers(ctx, row(b - 1), row(b) - row(b - 1)));
+        }
+      }
+      WM_HIP(hipEventRecord(e[2], ctx->stream));
+      WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
+      StreamSwap on2(ctx, ctx->stream2);
+      if (nb < 3) {
+        WM_HIP(hipEventRecord(e[5], ctx->stream));
+        WM_HIP(hipEventRecord(e[6], ctx->stream));
+      }
+      if (nb > 1) WM_TRY(wm_k_insert_movers(ctx, row(nb - 1), row(nb) - row(nb - 1)));
+      WM_TRY(wm_k_insert_movers(ctx, row(0), row(1) - row(0)));
+      WM_HIP(hipEventRecord(e[7], ctx->stream));
+    } else {
+      WM_TRY(wm_k_push_deposit(ctx, 0, 1));
+      WM_HIP(hipEventRecord(e[2], ctx->stream));
+      WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
       StreamSwap on2(ctx, ctx->stream2);
       WM_HIP(hipEventRecord(e[5], ctx->stream));
-      if (f32) {
-        // cell bins: the push already wrapped x and closed the holes; the
-        // particles that changed cell join their destination bins
-        WM_HIP(hipEventRecord(e[6], ctx->stream));
-        WM_TRY(wm_k_insert_movers(ctx));
-      } else {
+      {
         WM_TRY(wm_k_particle_y(ctx, 1, true));    // app.f90:104-105 (x fused into y pass 1)
         WM_HIP(hipEventRecord(e[6], ctx->stream));
         WM_TRY(wm_k_sort(ctx, 1, 0));             // app.f90:106

@@ -86,19 +86,27 @@ struct FcgScal {
 // FP32 form, cell bins: the movers of one step, structure of arrays, one
 // region of WM_MG slots per cell of every push workgroup.
 #define WM_MG 64
-struct __align__(16) MoverRec {   // 48 B: three 16-byte stores
-  int dst;                     // destination bin
-  int dir;                     // (dy + 1) * 3 + (dx + 1)
-  float ox0, oy0;              // offsets before the push (deposit)
+#define WM_PUSH_BANDS 4        // row bands of the FP32 push (the migration follows band by band)
+struct __align__(16) MoverRec {   // 32 B: two 16-byte halves
   float ox1, oy1;              // offsets in the destination cell
   float ux, uy, uz;
-  int slot;                    // slot the particle left in its source bin
+  int sd;                      // source slot | direction index << 24 (index k of
+                               // direction (dy + 1) * 3 + (dx + 1): 0..3, then 5..8 as 4..7)
   int64_t id;
 };
 struct Movers {
   MoverRec *rec = nullptr;
-  uint8_t *dir = nullptr;      // each record's direction, packed for the migration's scan
   int *n = nullptr;            // movers per region
+};
+// Per bin after the push: what the migration needs of the movers that left
+// it, in one 32-byte load.
+#define WM_SUM_MV 8            // movers of a bin listed in its summary
+#define WM_SLOT_BITS 13        // source slots below 2^13 (bins of at most 8192 slots)
+struct __align__(32) BinSum {
+  uint8_t cnt[8];              // movers out per direction index
+  uint32_t out;                // (first record in the cell's region << 8) | movers out
+  uint32_t pad;
+  uint16_t mv[WM_SUM_MV];      // the first movers in list order: slot | index << 13
 };
 
 using PBank = Rec64;
@@ -163,7 +171,7 @@ struct wm_ctx {
   int *stage_i = nullptr; size_t stage_ielems = 0;
 
   // timing (per-phase HIP events of the last wm_step call)
-  hipEvent_t ev[16]{};
+  hipEvent_t ev[32]{};
   bool ev_ready = false;
   int last_iters[3] = {0, 0, 0};
   double phase_ms[8] = {0};
@@ -174,8 +182,7 @@ struct wm_ctx {
   Bins32 bins;
   int CB = 0;
   int *bin_cnt = nullptr;        // particles per bin
-  int *bin_mout = nullptr;       // per bin after the push: (first record << 8) | movers out
-  uint8_t *bin_mcnt = nullptr;   // per bin and direction (8): particles that left (<= WM_MG)
+  BinSum *bin_sum = nullptr;     // per bin after the push: its movers' summary
   int64_t *id_bins = nullptr;
   Movers mv{};
   int nregion = 0;
@@ -222,10 +229,10 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst);
 // their destination bins, bin counts, row counts
 int wm_k_dense_to_bins(wm_ctx *ctx);
 int wm_k_bins_to_dense(wm_ctx *ctx);
-int wm_k_insert_movers(wm_ctx *ctx);
+int wm_k_insert_movers(wm_ctx *ctx, int r0 = 0, int nrows = -1);   // rows r0 .. r0 + nrows - 1
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);
-int wm_k_push_deposit32(wm_ctx *ctx);   // in place on the bins
+int wm_k_push_deposit32(wm_ctx *ctx, int r0 = 0, int nrows = -1);   // in place on the bins
 int wm_push32_regions(int nx, int ny);   // mover regions of the push launch
 int wm_k_deposit_gather32(wm_ctx *ctx);
 int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);

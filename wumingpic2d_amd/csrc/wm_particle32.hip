@@ -29,6 +29,7 @@ constexpr int NT = 256;               // push workgroup size (a constant: blockD
 
 struct PP32 {
   int nx, ny, nxgs, nys, nsp, cap, nchunk;
+  int j0;                                // first row of the launch (blockIdx.y = row - j0)
   float delt, c, c2, inv_c2, qc_scale;   // qc_scale = delx/delt
   float q[WM_MAXSP], r[WM_MAXSP];
 };
@@ -374,7 +375,7 @@ struct Lane {
 __device__ inline int lane_g() { return threadIdx.x / GROUP; }
 __device__ inline int lane_sub() { return threadIdx.x % GROUP; }
 __device__ inline size_t lane_reg(const PP32 &P) {   // first record of the cell's mover region
-  return (((size_t)blockIdx.y * P.nchunk + blockIdx.x) * CHUNK + lane_g()) * WM_MG;
+  return (((size_t)(P.j0 + blockIdx.y) * P.nchunk + blockIdx.x) * CHUNK + lane_g()) * WM_MG;
 }
 
 // One window of GROUP particles (slot k = w * GROUP + sub of the bin); the
@@ -414,17 +415,11 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
   if (cross) {
     const int pos = L.xcnt + __popc(gm & ((1u << lane_sub()) - 1u));
     if (pos < WM_MG) {
-      int ci = (int)blockIdx.x * CHUNK + lane_g() + dx;
-      ci = ci < 0 ? ci + P.nx : (ci >= P.nx ? ci - P.nx : ci);   // periodic x
-      int cj = (int)blockIdx.y + dy;
-      cj = cj < 0 ? cj + P.ny : (cj >= P.ny ? cj - P.ny : cj);   // periodic y
       const int dir = (dy + 1) * 3 + (dx + 1);
+      const int kd = dir - (dir > 4);              // direction index 0..7
       float4 *rp = (float4 *)(M.rec + lane_reg(P) + pos);
-      rp[0] = make_float4(__int_as_float((int)bin_of(S.s, cj, ci, P.ny, P.nx)), __int_as_float(dir),
-                          ox0, oy0);
-      rp[1] = make_float4(ox1, oy1, o.ux, o.uy);
-      ((float2 *)rp)[4] = make_float2(o.uz, __int_as_float(k));
-      M.dir[lane_reg(P) + pos] = (uint8_t)dir;
+      rp[0] = make_float4(ox1, oy1, o.ux, o.uy);
+      ((float2 *)rp)[2] = make_float2(o.uz, __int_as_float(k | (kd << 24)));
       xl[lane_g() * WM_MG + pos] = (S.s << 28) | (dir << 20) | k;
       xpl[lane_g() * XC + pos] = make_float4(ox0, oy0, ox1, oy1);
       xql[lane_g() * XC + pos] = qv;
@@ -460,21 +455,22 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
 
 // Push + deposit in place on the cell bins.  Every particle's new state is
 // written to its slot; one that leaves its cell also goes to the cell's mover
-// region (new state, old offsets, source slot; its ID at the kernel's end)
-// and its slot becomes a hole.  The movers' deposit is summed per cell in
-// list order.  Each bin records how many of its particles left, in total
-// (with the offset of its first record in the region: (offset << 8) | count)
-// and in each of the 8 directions; wm_k_insert_movers then moves the movers
-// into their destination bins.
+// region (new state, source slot and direction; its ID at the kernel's end)
+// and its slot becomes a hole.  The movers' deposit is summed per cell from
+// the list in LDS.  Each bin's summary records how many of its particles
+// left, in total (with the offset of its first record in the region) and in
+// each of the 8 directions, and the slots and directions of the first
+// WM_SUM_MV of them; wm_k_insert_movers then moves the movers into their
+// destination bins.
 // PAIR: exactly two species (straight-line code, so the compiler keeps the
 // second species' windows in flight through the first without spilling).
 template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     PP32 P, int CB, const int *__restrict__ cnt, const float *__restrict__ tmp, int TX,
-    size_t tps, Bins32 A, const int64_t *__restrict__ ids, Movers M, int *__restrict__ mout,
-    uint8_t *__restrict__ mcnt, double *__restrict__ part, int *__restrict__ derr) {
+    size_t tps, Bins32 A, const int64_t *__restrict__ ids, Movers M,
+    BinSum *__restrict__ msum, double *__restrict__ part, int *__restrict__ derr) {
   const int chunk = blockIdx.x;
-  const int jl = blockIdx.y;
+  const int jl = P.j0 + blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
   const int ncell = min(CHUNK, P.nx - i0);
   const size_t rbase = ((size_t)jl * P.nchunk + chunk) * CHUNK;   // first region
@@ -570,14 +566,19 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     for (int s = 0; s < P.nsp; ++s) {
       int m = 0;
       while (j0 + m < gm && (xg[j0 + m] >> 28) == s) ++m;
-      const size_t bin = bin_of(s, jl, il, P.ny, P.nx);
+      BinSum *bs = msum + bin_of(s, jl, il, P.ny, P.nx);
       if (sub < 8) {
         const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
         int c = 0;
         for (int q = 0; q < m; ++q) c += ((xg[j0 + q] >> 20) & 0xF) == d;
-        mcnt[bin * 8 + sub] = (uint8_t)c;
+        bs->cnt[sub] = (uint8_t)c;
       }
-      if (sub == 0) mout[bin] = (j0 << 8) | m;       // first record in the region, count
+      if (sub < min(m, WM_SUM_MV)) {
+        const int xv = xg[j0 + sub];
+        const int dir = (xv >> 20) & 0xF;
+        bs->mv[sub] = (uint16_t)((xv & 0xFFFFF) | ((dir - (dir > 4)) << WM_SLOT_BITS));
+      }
+      if (sub == 0) bs->out = (uint32_t)((j0 << 8) | m);   // first record in the region, count
       j0 += m;
     }
   }
@@ -717,6 +718,7 @@ PP32 make_pp32(const wm_ctx *ctx) {
   P.nx = g.nx; P.ny = g.ny; P.nxgs = g.nxgs; P.nys = g.nys; P.nsp = g.nsp;
   P.cap = ctx->cap;
   P.nchunk = wm_cdiv(g.nx, CHUNK);
+  P.j0 = 0;
   P.delt = (float)g.delt;
   P.c = (float)g.c;
   P.c2 = (float)(g.c * g.c);
@@ -746,13 +748,19 @@ int wm_k_cell_fields32(wm_ctx *ctx) {
 
 int wm_push32_regions(int nx, int ny) { return ny * wm_cdiv(nx, CHUNK) * CHUNK; }
 
-int wm_k_push_deposit32(wm_ctx *ctx) {
+int wm_k_push_deposit32(wm_ctx *ctx, int r0, int nrows) {
   PP32 P = make_pp32(ctx);
+  if (nrows < 0) nrows = ctx->g.ny - r0;
+  if (r0 < 0 || nrows < 1 || r0 + nrows > ctx->g.ny) {
+    wm_set_error(ctx, WM_ERR_ARG, "push: rows %d..%d outside 0..%d", r0, r0 + nrows, ctx->g.ny);
+    return WM_ERR_ARG;
+  }
+  P.j0 = r0;
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
   hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
-                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->bins, ctx->id_bins, ctx->mv,
-                     ctx->bin_mout, ctx->bin_mcnt, ctx->dep_part, ctx->derr);
+                     dim3(P.nchunk, nrows), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->bins, ctx->id_bins, ctx->mv,
+                     ctx->bin_sum, ctx->dep_part, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
