@@ -73,11 +73,9 @@ __device__ inline Tile xcd_tile() {
 }
 
 // Inside flexible CG the preconditioner's launches are queued ahead of the
-// host's convergence check; once every component has stopped they return at
-// once.
-__device__ inline bool gated_off(const FcgScal *gate) {
-  return gate && !(gate[0].active || gate[1].active || gate[2].active);
-}
+// host's convergence check; a workgroup of a component (batch index b) that
+// has stopped returns at once.
+__device__ inline bool gated_off(const FcgScal *gate, int b) { return gate && !gate[b].active; }
 
 // ------------------------------------------------------------- MFMA conv
 // out[b, y, x, co] = sum_tap sum_ci in[b, y+dy, x+dx, ci] * w[tap, co, ci].
@@ -102,11 +100,11 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in,
                                               bf16 *__restrict__ pout = nullptr) {
   static_assert(CIN == 16 || CIN == 32, "CIN");
   static_assert(COUT % 16 == 0, "COUT");
-  if (gated_off(gate)) return;
   __shared__ __attribute__((aligned(16))) bf16 sin[(T + 2) * (T + 2) * CIN];
   __shared__ __attribute__((aligned(16))) bf16 sw[9 * COUT * CIN];
   const Tile tl = xcd_tile();
   const int b = tl.b;
+  if (gated_off(gate, b)) return;
   const int x0 = tl.x * T, y0 = tl.y * T;
   const bf16 *inb = in + (size_t)b * H * W * CIN;
   constexpr int CH = CIN / 8;
@@ -309,11 +307,11 @@ __global__ __launch_bounds__(256) void k_lift_pool(const double *__restrict__ r,
                                                    const float *__restrict__ fold,
                                                    bf16 *__restrict__ d1, int H, int W, int B,
                                                    const FcgScal *__restrict__ gate) {
-  if (gated_off(gate)) return;
   __shared__ float xs[(T + 2) * (T + 2)];
   __shared__ __attribute__((aligned(16))) float fl[16 * C0];
   const Tile tl = xcd_tile();
   const int b = __builtin_amdgcn_readfirstlane(tl.b);
+  if (gated_off(gate, b)) return;
   const int x0 = tl.x * T, y0 = tl.y * T;
   static_assert(16 * C0 == 256, "one weight per thread");
   X0Tile<1> xt;
@@ -390,9 +388,9 @@ __global__ __launch_bounds__(256) void k_up_proj(const double *__restrict__ r, s
   __shared__ __attribute__((aligned(16))) float fp[4 * PS];
   __shared__ float fk[25];
   __shared__ double sh[8];
-  if (dots && gated_off(S)) return;
   const Tile tl = xcd_tile();
   const int b = __builtin_amdgcn_readfirstlane(tl.b);
+  if (dots && gated_off(S, b)) return;
   const int x0 = tl.x * T, y0 = tl.y * T;
   const int ty = threadIdx.x / T, tx = threadIdx.x % T;
   const int gy = y0 + ty, gx = x0 + tx;
