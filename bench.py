@@ -82,15 +82,17 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
 
 
 def load_traffic(workload: str, form: str):
-    """HBM bytes per push+deposit launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or None."""
+    """(HBM bytes per push+deposit launch, launches per step) from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE,
+    copy-calibrated), or (None, None)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["workloads"][workload][form]["bytes_per_launch"]
+        w = t["workloads"][workload][form]
+        return w["bytes_per_launch"], w.get("launches_per_step", 1)
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def make_context(wm, weibel, workload: str, form: str):
@@ -208,13 +210,20 @@ def main() -> int:
     nx, ny, ppc, _ = WORKLOADS[args.workload]
     npart = head["particles"]
     bpp = PUSHDEP_BYTES_PER_PARTICLE if args.form == "fp64" else PUSHDEP_BYTES_PER_PARTICLE_FP32
-    achieved = bpp * npart / (head["push_deposit_ms"] * 1e-3) / 1e9
+    # The FP32 push runs as one launch per row band (the profile records how
+    # many per step); each launch covers its share of the particles, so the
+    # per-launch rate is the step's rate.
+    traffic, launches = load_traffic(args.workload, args.form)
+    launches = launches or 1
+    launch_ms = head["push_deposit_ms"] / launches
+    achieved = bpp * npart / launches / (launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(args.workload, args.form),
+                "traffic": traffic,
                 "kernel": ("k_particle<PUSH,DEPOSIT>" if args.form == "fp64" else "k_push_deposit32")
                 + " (fused push+deposit)",
-                "bytes_per_particle": bpp, "launch_ms": round(head["push_deposit_ms"], 4)}
+                "bytes_per_particle": bpp, "launches_per_step": launches,
+                "launch_ms": round(launch_ms, 4)}
     p = head["p"]
     out = {
         "metric": METRIC,
