@@ -54,14 +54,21 @@ def test_fp32_initialiser_matches_host_state():
             assert np.allclose(b[2:5], a[2:5], rtol=2e-7, atol=1e-9)
 
 
-@pytest.mark.parametrize("nx,ny,ppc,seed,steps,nsp", [(32, 16, 16, 42, 6, 2), (48, 12, 8, 43, 4, 2),
-                                                     (8, 4, 96, 44, 3, 1), (16, 16, 8, 45, 4, 3)])
-def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp):
+@pytest.mark.parametrize("nx,ny,ppc,seed,steps,nsp,vth", [(32, 16, 16, 42, 6, 2, 0.1),
+                                                         (48, 12, 8, 43, 4, 2, 0.1),
+                                                         (8, 4, 96, 44, 3, 1, 0.1),
+                                                         (16, 16, 8, 45, 4, 3, 0.1),
+                                                         (16, 8, 64, 46, 3, 2, 0.4)])
+def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp, vth):
     """Two species run the push's straight-line pair path; one and three
     species run its species loop.  96 particles per cell put bins above the
     80 the push holds in registers, so its direct-load tail runs too (and
-    an 8-cell row is narrower than one workgroup's 16 cells)."""
-    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=0.1, nsp=nsp)
+    an 8-cell row is narrower than one workgroup's 16 cells).  The hot plasma
+    (u_th = 0.4 c) moves about a quarter of the particles per step, about 15
+    per bin: bins with more movers than their summary lists, several arrivals
+    from one direction, and holes past the listed ones all occur, so the
+    migration's record-reading paths run."""
+    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth, nsp=nsp)
     ctx = _ctx32(p)
     ctx.state_upload(up, np2, cc, uf)
     orc = Oracle(p)
