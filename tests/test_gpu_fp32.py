@@ -58,7 +58,8 @@ def test_fp32_initialiser_matches_host_state():
                                                          (48, 12, 8, 43, 4, 2, 0.1),
                                                          (8, 4, 96, 44, 3, 1, 0.1),
                                                          (16, 16, 8, 45, 4, 3, 0.1),
-                                                         (16, 8, 64, 46, 3, 2, 0.4)])
+                                                         (16, 8, 64, 46, 3, 2, 0.4),
+                                                         (24, 6, 16, 47, 4, 2, 0.2)])
 def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp, vth):
     """Two species run the push's straight-line pair path; one and three
     species run its species loop.  96 particles per cell put bins above the
@@ -67,7 +68,9 @@ def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp, vth):
     (u_th = 0.4 c) moves about a quarter of the particles per step, about 15
     per bin: bins with more movers than their summary lists, several arrivals
     from one direction, and holes past the listed ones all occur, so the
-    migration's record-reading paths run."""
+    migration's record-reading paths run.  The push runs in min(4, ny / 2)
+    row bands, each band's migration once its neighbours are pushed: 4 rows
+    give 2 bands, 6 rows 3 and the others 4."""
     p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth, nsp=nsp)
     ctx = _ctx32(p)
     ctx.state_upload(up, np2, cc, uf)
