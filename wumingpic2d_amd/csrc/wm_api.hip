@@ -378,14 +378,14 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(dmalloc(ctx, &ctx->bins.uz, bslots));
     WM_TRY(dmalloc(ctx, &ctx->id_bins, bslots));
     WM_TRY(dmalloc(ctx, &ctx->bin_cnt, nbin));
-    WM_TRY(dmalloc(ctx, &ctx->bin_sum, nbin));
     WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
     ctx->nregion = wm_push32_regions(g.nx, g.ny);
     const size_t ms = (size_t)ctx->nregion * WM_MG;
-    Movers &m = ctx->mv;
-    WM_TRY(dmalloc(ctx, &m.rec, ms));
-    WM_TRY(dmalloc(ctx, &m.n, (size_t)ctx->nregion));
-    WM_HIP(hipMemsetAsync(m.n, 0, sizeof(int) * ctx->nregion, ctx->stream));
+    for (int b = 0; b < 2; ++b) {
+      WM_TRY(dmalloc(ctx, &ctx->bin_sum[b], nbin));
+      WM_TRY(dmalloc(ctx, &ctx->mv[b].rec, ms));
+    }
+    ctx->mv_pending = false;
     WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
   } else {
     WM_TRY(alloc_bank(ctx, ctx->bank[0], slots));
@@ -515,8 +515,8 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
-                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum,
-                  ctx->mv.rec, ctx->mv.n, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum[0],
+                  ctx->bin_sum[1], ctx->mv[0].rec, ctx->mv[1].rec, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -854,44 +854,21 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     const bool f32 = ctx->form == WM_FORM_FP32;
     WM_TRY(f32 ? wm_k_cell_fields32(ctx) : wm_k_cell_fields(ctx));
     WM_HIP(hipEventRecord(e[1], ctx->stream));
-    // The particle boundaries and the sort (app.f90:104-106) read only the
-    // pushed particles, and the field solve (app.f90:103) only the current
-    // and the fields: they run concurrently, the particles on the second
-    // stream, and the next step waits for both.
     if (f32) {
-      // Cell bins: the push wraps x itself; the particles that changed cell
-      // join their destination bins in the migration.  The push runs in row
-      // bands, and band b's migration (which reads the push results of rows
-      // b's first - 1 .. b's last + 1) starts on the second stream as soon as
-      // band b + 1 is pushed, beside the later bands' push; the last band's
-      // and (its rows wrapping to the last band) the first band's follow the
-      // last push, beside the field solve.
-      const int nb = std::max(1, std::min(WM_PUSH_BANDS, ctx->g.ny / 2));
-      auto row = [&](int b) { return (int)((long long)b * ctx->g.ny / nb); };
-      for (int b = 0; b < nb; ++b) {
-        WM_TRY(wm_k_push_deposit32(ctx, row(b), row(b + 1) - row(b)));
-        if (b >= 2) {
-          WM_HIP(hipEventRecord(e[16 + b], ctx->stream));
-          WM_HIP(hipStreamWaitEvent(ctx->stream2, e[16 + b], 0));
-          if (b == 2) {
-            WM_HIP(hipEventRecord(e[5], ctx->stream2));
-            WM_HIP(hipEventRecord(e[6], ctx->stream2));
-          }
-          StreamSwap on2(ctx, ctx->stream2);
-          WM_TRY(wm_k_insert_movers(ctx, row(b - 1), row(b) - row(b - 1)));
-        }
-      }
+      // Cell bins: the push wraps x itself, and moves the last push's movers
+      // into their bins before it pushes (the particle boundaries and the
+      // sort of app.f90:104-106 reduce to that), so nothing runs beside the
+      // solve.
+      WM_TRY(wm_k_push_deposit32(ctx));
       WM_HIP(hipEventRecord(e[2], ctx->stream));
-      WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
-      StreamSwap on2(ctx, ctx->stream2);
-      if (nb < 3) {
-        WM_HIP(hipEventRecord(e[5], ctx->stream));
-        WM_HIP(hipEventRecord(e[6], ctx->stream));
-      }
-      if (nb > 1) WM_TRY(wm_k_insert_movers(ctx, row(nb - 1), row(nb) - row(nb - 1)));
-      WM_TRY(wm_k_insert_movers(ctx, row(0), row(1) - row(0)));
+      WM_HIP(hipEventRecord(e[5], ctx->stream));
+      WM_HIP(hipEventRecord(e[6], ctx->stream));
       WM_HIP(hipEventRecord(e[7], ctx->stream));
     } else {
+      // The particle boundaries and the sort (app.f90:104-106) read only the
+      // pushed particles, and the field solve (app.f90:103) only the current
+      // and the fields: they run concurrently, the particles on the second
+      // stream, and the next step waits for both.
       WM_TRY(wm_k_push_deposit(ctx, 0, 1));
       WM_HIP(hipEventRecord(e[2], ctx->stream));
       WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
@@ -913,7 +890,8 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     WM_HIP(hipEventRecord(e[8], ctx->stream));
     WM_HIP(hipEventSynchronize(e[8]));
     // phases: [0..3] on the first stream (e0..e4), [4] particle boundaries
-    // (e5..e6) and [5] sort (e6..e7) on the second
+    // (e5..e6) and [5] sort (e6..e7) on the second (FP64; in the FP32 form
+    // the push holds them)
     const int from[WM_NPHASE] = {0, 1, 2, 3, 5, 6}, to[WM_NPHASE] = {1, 2, 3, 4, 6, 7};
     for (int k = 0; k < WM_NPHASE; ++k) {
       float ms = 0.f;

@@ -86,7 +86,6 @@ struct FcgScal {
 // FP32 form, cell bins: the movers of one step, structure of arrays, one
 // region of WM_MG slots per cell of every push workgroup.
 #define WM_MG 64
-#define WM_PUSH_BANDS 4        // row bands of the FP32 push (the migration follows band by band)
 struct __align__(16) MoverRec {   // 32 B: two 16-byte halves
   float ox1, oy1;              // offsets in the destination cell
   float ux, uy, uz;
@@ -96,7 +95,6 @@ struct __align__(16) MoverRec {   // 32 B: two 16-byte halves
 };
 struct Movers {
   MoverRec *rec = nullptr;
-  int *n = nullptr;            // movers per region
 };
 // Per bin after the push: what the migration needs of the movers that left
 // it, in one 32-byte load.
@@ -182,9 +180,15 @@ struct wm_ctx {
   Bins32 bins;
   int CB = 0;
   int *bin_cnt = nullptr;        // particles per bin
-  BinSum *bin_sum = nullptr;     // per bin after the push: its movers' summary
   int64_t *id_bins = nullptr;
-  Movers mv{};
+  // The movers of a push (records per cell, summary per bin) join their
+  // destination bins in the next push's prologue, or in the settle kernel
+  // before the bins are read otherwise; two buffers, because a push reads
+  // the last push's movers while it writes its own.
+  Movers mv[2]{};
+  BinSum *bin_sum[2] = {nullptr, nullptr};
+  int mv_last = 0;               // the buffer the last push wrote
+  bool mv_pending = false;       // its movers are not in their bins yet
   int nregion = 0;
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
   long long cg_iters_total = 0;
@@ -229,10 +233,10 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst);
 // their destination bins, bin counts, row counts
 int wm_k_dense_to_bins(wm_ctx *ctx);
 int wm_k_bins_to_dense(wm_ctx *ctx);
-int wm_k_insert_movers(wm_ctx *ctx, int r0 = 0, int nrows = -1);   // rows r0 .. r0 + nrows - 1
+int wm_settle_movers(wm_ctx *ctx);   // the last push's movers into their bins, if pending
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);
-int wm_k_push_deposit32(wm_ctx *ctx, int r0 = 0, int nrows = -1);   // in place on the bins
+int wm_k_push_deposit32(wm_ctx *ctx);   // in place on the bins (pending movers first)
 int wm_push32_regions(int nx, int ny);   // mover regions of the push launch
 int wm_k_deposit_gather32(wm_ctx *ctx);
 int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);

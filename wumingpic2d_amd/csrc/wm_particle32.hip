@@ -15,6 +15,7 @@
 // directly (dh = offset - 1/2), which keeps their accuracy independent of the
 // grid size.
 #include "wm_internal.h"
+#include "wm_migrate.h"
 
 namespace {
 
@@ -29,7 +30,6 @@ constexpr int NT = 256;               // push workgroup size (a constant: blockD
 
 struct PP32 {
   int nx, ny, nxgs, nys, nsp, cap, nchunk;
-  int j0;                                // first row of the launch (blockIdx.y = row - j0)
   float delt, c, c2, inv_c2, qc_scale;   // qc_scale = delx/delt
   float q[WM_MAXSP], r[WM_MAXSP];
 };
@@ -375,7 +375,7 @@ struct Lane {
 __device__ inline int lane_g() { return threadIdx.x / GROUP; }
 __device__ inline int lane_sub() { return threadIdx.x % GROUP; }
 __device__ inline size_t lane_reg(const PP32 &P) {   // first record of the cell's mover region
-  return (((size_t)(P.j0 + blockIdx.y) * P.nchunk + blockIdx.x) * CHUNK + lane_g()) * WM_MG;
+  return (((size_t)blockIdx.y * P.nchunk + blockIdx.x) * CHUNK + lane_g()) * WM_MG;
 }
 
 // One window of GROUP particles (slot k = w * GROUP + sub of the bin); the
@@ -466,23 +466,50 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
 // second species' windows in flight through the first without spilling).
 template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
-    PP32 P, int CB, const int *__restrict__ cnt, const float *__restrict__ tmp, int TX,
-    size_t tps, Bins32 A, const int64_t *__restrict__ ids, Movers M,
-    BinSum *__restrict__ msum, double *__restrict__ part, int *__restrict__ derr) {
+    PP32 P, int CB, const float *__restrict__ tmp, int TX, size_t tps, Bins32 A,
+    int64_t *ids, wm_mig::Mig G, Movers Mp, int pending, Movers M, BinSum *__restrict__ msum,
+    double *__restrict__ part, int *__restrict__ derr) {
   const int chunk = blockIdx.x;
-  const int jl = P.j0 + blockIdx.y;
+  const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
   const int ncell = min(CHUNK, P.nx - i0);
+  // the chunk's cell-centred fields first: they do not depend on the
+  // migration, so their loads overlap its
+  constexpr int NTV = (6 * 3 * TW + NT - 1) / NT;
+  float tv[NTV];
+#pragma unroll
+  for (int q = 0; q < NTV; ++q) {
+    const int t = threadIdx.x + q * NT;
+    const int col = t % TW, b = (t / TW) % 3, k = t / (3 * TW);
+    tv[q] = t < 6 * 3 * TW && col < ncell + 2
+                ? tmp[(size_t)k * tps + (size_t)(i0 + col) + (size_t)TX * (jl + b)]
+                : 0.0f;
+  }
+  // The last push's movers into this workgroup's bins (8 lanes per bin);
+  // they read only what the last push wrote, and write only these bins, so
+  // workgroups need not wait for one another.  Each bin's new count goes
+  // straight to LDS; the barrier below also makes the bins' new slots
+  // visible to the window loads.
+  if (pending) {
+    for (int u = threadIdx.x; u < P.nsp * CHUNK * wm_mig::DL; u += NT) {
+      const int k = u % wm_mig::DL, c = (u / wm_mig::DL) % CHUNK, s = u / (wm_mig::DL * CHUNK);
+      int cn = 0;
+      if (c < ncell)
+        cn = wm_mig::migrate_bin(G, Mp, A, ids, derr, bin_of(s, jl, i0 + c, P.ny, P.nx), i0 + c,
+                                 jl, s, k);
+      if (k == 0) scnt[s * CHUNK + c] = cn;
+    }
+  } else {
+    for (int t = threadIdx.x; t < P.nsp * CHUNK; t += NT) {
+      const int c = t % CHUNK, s = t / CHUNK;
+      scnt[t] = c < ncell ? min(CB, G.cnt[bin_of(s, jl, i0 + c, P.ny, P.nx)]) : 0;
+    }
+  }
   const size_t rbase = ((size_t)jl * P.nchunk + chunk) * CHUNK;   // first region
   if (threadIdx.x < WM_MAXSP) sqc[threadIdx.x] = P.q[threadIdx.x] * P.qc_scale;
-  for (int t = threadIdx.x; t < P.nsp * CHUNK; t += NT) {
-    const int c = t % CHUNK, s = t / CHUNK;
-    scnt[t] = c < ncell ? min(CB, cnt[bin_of(s, jl, i0 + c, P.ny, P.nx)]) : 0;
-  }
-  for (int t = threadIdx.x; t < 6 * 3 * TW; t += NT) {
-    const int col = t % TW, b = (t / TW) % 3, k = t / (3 * TW);
-    T[t] = col < ncell + 2 ? tmp[(size_t)k * tps + (size_t)(i0 + col) + (size_t)TX * (jl + b)] : 0.0f;
-  }
+#pragma unroll
+  for (int q = 0; q < NTV; ++q)
+    if (threadIdx.x + q * NT < 6 * 3 * TW) T[threadIdx.x + q * NT] = tv[q];
   for (int e = threadIdx.x; e < CHUNK * NACC; e += NT) cellbuf[e] = 0.0;
   __syncthreads();
   const int g = threadIdx.x / GROUP;
@@ -536,10 +563,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   const int xcnt = L.xcnt;
   if (!ok) atomicOr(derr, DERR_JUMP);
   if (xcnt > WM_MG && sub == 0) atomicOr(derr, DERR_MOVER_OVER);
-  if (sub == 0 && g < CHUNK) {
-    gcnt[g] = g < ncell ? min(xcnt, WM_MG) : 0;
-    M.n[rbase + g] = gcnt[g];
-  }
+  if (sub == 0 && g < CHUNK) gcnt[g] = g < ncell ? min(xcnt, WM_MG) : 0;
   __syncthreads();                                   // also orders the movers' stores
   const int gm = g < ncell ? gcnt[g] : 0;
   // the movers' deposit, on the first wave alone: lane t takes the cell's
@@ -718,7 +742,6 @@ PP32 make_pp32(const wm_ctx *ctx) {
   P.nx = g.nx; P.ny = g.ny; P.nxgs = g.nxgs; P.nys = g.nys; P.nsp = g.nsp;
   P.cap = ctx->cap;
   P.nchunk = wm_cdiv(g.nx, CHUNK);
-  P.j0 = 0;
   P.delt = (float)g.delt;
   P.c = (float)g.c;
   P.c2 = (float)(g.c * g.c);
@@ -748,20 +771,30 @@ int wm_k_cell_fields32(wm_ctx *ctx) {
 
 int wm_push32_regions(int nx, int ny) { return ny * wm_cdiv(nx, CHUNK) * CHUNK; }
 
-int wm_k_push_deposit32(wm_ctx *ctx, int r0, int nrows) {
+int wm_k_push_deposit32(wm_ctx *ctx) {
   PP32 P = make_pp32(ctx);
-  if (nrows < 0) nrows = ctx->g.ny - r0;
-  if (r0 < 0 || nrows < 1 || r0 + nrows > ctx->g.ny) {
-    wm_set_error(ctx, WM_ERR_ARG, "push: rows %d..%d outside 0..%d", r0, r0 + nrows, ctx->g.ny);
-    return WM_ERR_ARG;
-  }
-  P.j0 = r0;
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
+  // the last push's movers (if not yet settled) are read from its buffer,
+  // this push's go to the other
+  const int in = ctx->mv_last, out = ctx->mv_pending ? 1 - in : in;
+  wm_mig::Mig G;
+  G.nx = ctx->g.nx;
+  G.ny = ctx->g.ny;
+  G.nxr = (int)((size_t)ctx->nregion / (size_t)ctx->g.ny);
+  G.CB = ctx->CB;
+  G.r0 = 0;
+  G.nr = ctx->g.ny;
+  G.nbin = (size_t)ctx->g.nsp * G.ny * G.nx;
+  G.sum = ctx->bin_sum[in];
+  G.cnt = ctx->bin_cnt;
   hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
-                     dim3(P.nchunk, nrows), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->tmp32, TX, tps, ctx->bins, ctx->id_bins, ctx->mv,
-                     ctx->bin_sum, ctx->dep_part, ctx->derr);
+                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->tmp32, TX,
+                     tps, ctx->bins, ctx->id_bins, G, ctx->mv[in], ctx->mv_pending ? 1 : 0,
+                     ctx->mv[out], ctx->bin_sum[out], ctx->dep_part, ctx->derr);
   WM_HIP(hipGetLastError());
+  ctx->mv_last = out;
+  ctx->mv_pending = true;
   return WM_OK;
 }
 
