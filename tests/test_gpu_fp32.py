@@ -115,6 +115,33 @@ def test_fp32_run_is_deterministic():
                                   outs[1][0][:, :k, jl, s].view(np.int64))
 
 
+def test_fp32_settling_between_steps_changes_nothing():
+    """A push leaves its movers pending; the next push places them in its
+    prologue, or the settle kernel does when the bins are read first (energy
+    sums, download).  Both use the same code, so energy sums between steps
+    leave the trajectory bit-identical."""
+    p, up, np2, cc, uf = state(32, 12, 16, 48, vth=0.3)
+    outs = []
+    for settle_between in (False, True):
+        ctx = _ctx32(p)
+        ctx.state_upload(up, np2, cc, uf)
+        for _ in range(4):
+            ctx.step(1)
+            if settle_between:
+                ctx.energy()
+        o, n, f = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+        ctx.state_download(up=o, np2=n, uf=f)
+        outs.append((o, n, f))
+        ctx.close()
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+    for s in range(2):
+        for jl in range(p["ny"]):
+            k = outs[0][1][jl, s]
+            assert np.array_equal(outs[0][0][:, :k, jl, s].view(np.int64),
+                                  outs[1][0][:, :k, jl, s].view(np.int64))
+
+
 def test_fp32_host_array_entry_points_refuse():
     p, up, np2, cc, uf = state(16, 4, 2, 45)
     ctx = _ctx32(p)
