@@ -77,38 +77,111 @@ __device__ inline int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A
     if (k >= o) inc += v;
   }
   const int total = __shfl(inc, DL - 1, DL);
-  if (c > 0) {
-    const int ms = (int)(fh.z & 0xFFu);
-    const size_t rs = bin_records(G, si, sjl, fh.z);
-    int a = inc - c, left = c;
-    for (int j = 0; j < ms && left > 0; ++j) {
-      const int kj = j < WM_SUM_MV ? sum_mv(fl.x, fl.y, fl.z, fl.w, j) >> WM_SLOT_BITS
-                                   : M.rec[rs + j].sd >> 24;
-      if (kj != k) continue;
-      const MoverRec mr = M.rec[rs + j];
-      --left;
-      const int slot = a < m ? hole_slot(M, r0, ol, a) : n + (a - m);
-      ++a;
-      if (slot >= G.CB) {
-        atomicOr(derr, DERR_NP_OVER);
-        continue;
+  const int n1 = n - m, nnew = n1 + total;
+  const size_t base = d * G.CB;
+  // Lane 0's compaction (fewer arrivals than holes): the particles above the
+  // new count that are not holes fill the holes left below it, in order.
+  // Its sources lie at or above the new count and the arrivals' holes below
+  // it, so the first NMV moves load together with the arrivals' records
+  // (the rest follow one by one).
+  constexpr int NMV = 4, NAR = 2;
+  const bool compact = k == 0 && total < m;
+  int hb = total, q = total, h = 0, from_slot = nnew;
+  if (compact) {
+    while (hb < m && hole_slot(M, r0, ol, hb) < nnew) ++hb;   // holes below the new count
+    h = hb;
+  }
+  int mdst[NMV], msrc[NMV];
+  bool mok[NMV];
+  float4 mq[NMV];
+  float muz[NMV];
+  int64_t mid[NMV];
+#pragma unroll
+  for (int t = 0; t < NMV; ++t) {
+    mok[t] = false;
+    mdst[t] = msrc[t] = 0;
+    if (compact) {
+      while (from_slot < n && h < m && hole_slot(M, r0, ol, h) == from_slot) {   // holes above
+        ++h;
+        ++from_slot;
       }
-      const size_t p = d * G.CB + slot;
-      A.q[p] = make_float4(mr.ox1, mr.oy1, mr.ux, mr.uy);
-      A.uz[p] = mr.uz;
-      ids[p] = mr.id;
+      if (from_slot < n && q < hb) {
+        mdst[t] = hole_slot(M, r0, ol, q);
+        msrc[t] = from_slot;
+        mok[t] = true;
+        ++q;
+        ++from_slot;
+      }
+    }
+    if (mok[t]) {
+      mq[t] = A.q[base + msrc[t]];
+      muz[t] = A.uz[base + msrc[t]];
+      mid[t] = ids[base + msrc[t]];
+    }
+  }
+  // Arrivals from direction k, in the source's list order: the first NAR
+  // records load together, the rest one by one.
+  int a = inc - c, left = c, j = 0;
+  int ms = 0;
+  size_t rs = 0;
+  if (c > 0) {
+    ms = (int)(fh.z & 0xFFu);
+    rs = bin_records(G, si, sjl, fh.z);
+  }
+  auto dir_of = [&](int jj) {
+    return jj < WM_SUM_MV ? sum_mv(fl.x, fl.y, fl.z, fl.w, jj) >> WM_SLOT_BITS
+                          : M.rec[rs + jj].sd >> 24;
+  };
+  auto slot_of = [&](int aa) { return aa < m ? hole_slot(M, r0, ol, aa) : n + (aa - m); };
+  auto put = [&](const MoverRec &mr, int slot) {
+    if (slot >= G.CB) {
+      atomicOr(derr, DERR_NP_OVER);
+      return;
+    }
+    const size_t p = base + slot;
+    A.q[p] = make_float4(mr.ox1, mr.oy1, mr.ux, mr.uy);
+    A.uz[p] = mr.uz;
+    ids[p] = mr.id;
+  };
+  MoverRec mr[NAR];
+  int aslot[NAR];
+  bool aok[NAR];
+#pragma unroll
+  for (int t = 0; t < NAR; ++t) {
+    aok[t] = false;
+    aslot[t] = 0;
+    while (left > 0 && j < ms && dir_of(j) != k) ++j;
+    if (left > 0 && j < ms) {
+      mr[t] = M.rec[rs + j];
+      aslot[t] = slot_of(a);
+      aok[t] = true;
+      --left;
+      ++a;
+      ++j;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NAR; ++t)
+    if (aok[t]) put(mr[t], aslot[t]);
+  for (; left > 0 && j < ms; ++j) {
+    if (dir_of(j) != k) continue;
+    put(M.rec[rs + j], slot_of(a));
+    --left;
+    ++a;
+  }
+#pragma unroll
+  for (int t = 0; t < NMV; ++t) {
+    if (mok[t]) {
+      const size_t dst = base + mdst[t];
+      A.q[dst] = mq[t];
+      A.uz[dst] = muz[t];
+      ids[dst] = mid[t];
     }
   }
   if (k != 0) return -1;
-  const int n1 = n - m;
-  if (total < m) {
-    const size_t base = d * G.CB;
-    const int nnew = n1 + total;
-    int hb = total;                              // holes below the new count
-    while (hb < m && hole_slot(M, r0, ol, hb) < nnew) ++hb;
-    int q = total, h = hb;
-    for (int from_slot = nnew; from_slot < n && q < hb; ++from_slot) {
-      if (h < m && hole_slot(M, r0, ol, h) == from_slot) {       // a hole above the new count
+  if (compact) {
+    for (; from_slot < n && q < hb; ++from_slot) {
+      if (h < m && hole_slot(M, r0, ol, h) == from_slot) {     // a hole above the new count
         ++h;
         continue;
       }
