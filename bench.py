@@ -212,9 +212,9 @@ def main() -> int:
     nx, ny, ppc, _ = WORKLOADS[args.workload]
     npart = head["particles"]
     bpp = PUSHDEP_BYTES_PER_PARTICLE if args.form == "fp64" else PUSHDEP_BYTES_PER_PARTICLE_FP32
-    # The FP32 push runs as one launch per row band (the profile records how
-    # many per step); each launch covers its share of the particles, so the
-    # per-launch rate is the step's rate.
+    # The profile records the push's launches per step (one; a push split in
+    # several launches would cover its share of the particles in each, so the
+    # per-launch rate is the step's rate).
     traffic, launches = load_traffic(args.workload, args.form)
     launches = launches or 1
     launch_ms = head["push_deposit_ms"] / launches

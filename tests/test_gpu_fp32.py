@@ -68,9 +68,9 @@ def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp, vth):
     (u_th = 0.4 c) moves about a quarter of the particles per step, about 15
     per bin: bins with more movers than their summary lists, several arrivals
     from one direction, and holes past the listed ones all occur, so the
-    migration's record-reading paths run.  The push runs in min(4, ny / 2)
-    row bands, each band's migration once its neighbours are pushed: 4 rows
-    give 2 bands, 6 rows 3 and the others 4."""
+    migration's record-reading paths run.  A 6-row grid is the smallest
+    here where a cell's two row neighbours are distinct from each other and
+    from the rows two away."""
     p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth, nsp=nsp)
     ctx = _ctx32(p)
     ctx.state_upload(up, np2, cc, uf)

@@ -460,8 +460,8 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
 // the list in LDS.  Each bin's summary records how many of its particles
 // left, in total (with the offset of its first record in the region) and in
 // each of the 8 directions, and the slots and directions of the first
-// WM_SUM_MV of them; wm_k_insert_movers then moves the movers into their
-// destination bins.
+// WM_SUM_MV of them.  The next push's prologue (or the settle kernel) then
+// moves the movers into their destination bins.
 // PAIR: exactly two species (straight-line code, so the compiler keeps the
 // second species' windows in flight through the first without spilling).
 template <bool PAIR>
