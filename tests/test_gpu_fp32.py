@@ -68,9 +68,8 @@ def test_fp32_steps_track_fp64_oracle(nx, ny, ppc, seed, steps, nsp, vth):
     (u_th = 0.4 c) moves about a quarter of the particles per step, about 15
     per bin: bins with more movers than their summary lists, several arrivals
     from one direction, and holes past the listed ones all occur, so the
-    migration's record-reading paths run.  A 6-row grid is the smallest
-    here where a cell's two row neighbours are distinct from each other and
-    from the rows two away."""
+    migration's record-reading paths run.  The 6-row grid adds a row count
+    that is neither a power of two nor a multiple of four."""
     p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth, nsp=nsp)
     ctx = _ctx32(p)
     ctx.state_upload(up, np2, cc, uf)
