@@ -16,26 +16,120 @@ struct Mig {                    // bins and mover regions of one migration
 
 // First record of the movers of the bin in cell (i, jl) in the cell's region
 // (out: the bin summary's (offset << 8) | count).
-__device__ inline size_t bin_records(const Mig &G, int i, int jl, uint32_t out) {
+__device__ __attribute__((always_inline)) inline size_t bin_records(const Mig &G, int i, int jl, uint32_t out) {
   return ((size_t)jl * G.nxr + i) * WM_MG + (out >> 8);
 }
 
 // Byte k of a summary's direction counts (words lo, hi), and entry j of its
 // mover list (words w0..w3).  The words are values, not a dynamically indexed
 // array, which the compiler would place in memory.
-__device__ inline int sum_cnt(uint32_t lo, uint32_t hi, int k) {
+__device__ __attribute__((always_inline)) inline int sum_cnt(uint32_t lo, uint32_t hi, int k) {
   const uint32_t v = k < 4 ? lo : hi;
   return (int)((v >> (8 * (k & 3))) & 0xFFu);
 }
-__device__ inline int sum_mv(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) {
+__device__ __attribute__((always_inline)) inline int sum_mv(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) {
   const uint32_t v = j < 4 ? (j < 2 ? w0 : w1) : (j < 6 ? w2 : w3);
   return (int)((v >> (16 * (j & 1))) & 0xFFFFu);
 }
 // Slot of bin d's h-th hole: from its summary for the first WM_SUM_MV, else
 // from the record.
-__device__ inline int hole_slot(const Movers &M, size_t r0, uint4 l, int h) {
+__device__ __attribute__((always_inline)) inline int hole_slot(const Movers &M, size_t r0, uint4 l, int h) {
   return h < WM_SUM_MV ? sum_mv(l.x, l.y, l.z, l.w, h) & ((1 << WM_SLOT_BITS) - 1)
                        : M.rec[r0 + h].sd & 0xFFFFFF;
+}
+
+// The compaction's running state: next hole to fill (q), end of the holes
+// below the new count (hb), next hole above it (h), next source slot (from).
+struct MoveState {
+  int q, hb, h, from;
+};
+struct Move {                    // one compaction move: particle at src -> slot dst
+  int dst, src;
+  bool ok;
+  float qx, qy, qz, qw, uz;      // components, not a float4: the vector type's copy
+  int64_t id;                    // would keep the struct in memory
+};
+// Plans the next move (skipping holes above the new count) and issues its
+// source's loads.
+__device__ __attribute__((always_inline)) inline void plan_move(Move &v, MoveState &st, bool compact, const Movers &M, size_t r0,
+                                 uint4 ol, int m, int n, size_t base, const Bins32 &A,
+                                 const int64_t *ids) {
+  v.ok = false;
+  v.dst = v.src = 0;
+  if (compact) {
+    while (st.from < n && st.h < m && hole_slot(M, r0, ol, st.h) == st.from) {
+      ++st.h;
+      ++st.from;
+    }
+    if (st.from < n && st.q < st.hb) {
+      v.dst = hole_slot(M, r0, ol, st.q);
+      v.src = st.from;
+      v.ok = true;
+      ++st.q;
+      ++st.from;
+    }
+  }
+  if (v.ok) {
+    const float4 t = A.q[base + v.src];
+    v.qx = t.x;
+    v.qy = t.y;
+    v.qz = t.z;
+    v.qw = t.w;
+    v.uz = A.uz[base + v.src];
+    v.id = ids[base + v.src];
+  }
+}
+__device__ __attribute__((always_inline)) inline void store_move(const Move &v, const Bins32 &A, int64_t *ids, size_t base) {
+  if (!v.ok) return;
+  A.q[base + v.dst] = make_float4(v.qx, v.qy, v.qz, v.qw);
+  A.uz[base + v.dst] = v.uz;
+  ids[base + v.dst] = v.id;
+}
+
+// Direction index of entry j of a source's mover list.
+__device__ __attribute__((always_inline)) inline int list_dir(const Movers &M, uint4 fl, size_t rs, int j) {
+  return j < WM_SUM_MV ? sum_mv(fl.x, fl.y, fl.z, fl.w, j) >> WM_SLOT_BITS
+                       : M.rec[rs + j].sd >> 24;
+}
+// Arrivals from one direction: next arrival index (a), how many are still to
+// come (left), next position in the source's list (j), its length (ms) and
+// first record (rs).
+struct ArrState {
+  int a, left, j, ms;
+  size_t rs;
+};
+struct Arrival {
+  MoverRec rec;
+  int slot;
+  bool ok;
+};
+// Finds the next arrival of direction k in the source's list and issues its
+// record's load.
+__device__ __attribute__((always_inline)) inline void plan_arrival(Arrival &w, ArrState &as, int k, const Movers &M, uint4 fl,
+                                    uint4 ol, size_t r0, int m, int n) {
+  w.ok = false;
+  w.slot = 0;
+  while (as.left > 0 && as.j < as.ms && list_dir(M, fl, as.rs, as.j) != k) ++as.j;
+  if (as.left > 0 && as.j < as.ms) {
+    w.rec = M.rec[as.rs + as.j];
+    w.slot = as.a < m ? hole_slot(M, r0, ol, as.a) : n + (as.a - m);
+    w.ok = true;
+    --as.left;
+    ++as.a;
+    ++as.j;
+  }
+}
+__device__ __attribute__((always_inline)) inline void put_arrival(const Arrival &w, const Mig &G, const Bins32 &A, int64_t *ids,
+                                   int *derr, size_t base) {
+  if (!w.ok) return;
+  if (w.slot >= G.CB) {
+    atomicOr(derr, DERR_NP_OVER);
+    return;
+  }
+  const size_t p = base + w.slot;
+  A.q[p] = make_float4(w.rec.ox1, w.rec.oy1, w.rec.ux, w.rec.uy);
+  A.uz[p] = w.rec.uz;
+  ids[p] = w.rec.id;
 }
 
 // The migration of bin d (cell i, row jl, species sp), by a group of 8
@@ -54,7 +148,7 @@ __device__ inline int hole_slot(const Movers &M, size_t r0, uint4 l, int h) {
 // records.
 constexpr int DL = 8;
 // Returns d's new count to lane 0 of the group (-1 to the others).
-__device__ inline int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids,
+__device__ __attribute__((always_inline)) inline int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids,
                                   int *derr, size_t d, int i, int jl, int sp, int k) {
   const int dir = k < 4 ? k : k + 1;
   const int si = wm_wrap(i - (dir % 3 - 1), G.nx), sjl = wm_wrap(jl - (dir / 3 - 1), G.ny);
@@ -82,102 +176,45 @@ __device__ inline int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A
   // Lane 0's compaction (fewer arrivals than holes): the particles above the
   // new count that are not holes fill the holes left below it, in order.
   // Its sources lie at or above the new count and the arrivals' holes below
-  // it, so the first NMV moves load together with the arrivals' records
-  // (the rest follow one by one).
-  constexpr int NMV = 4, NAR = 2;
+  // it, so the first four moves load together with the arrivals' records
+  // (the rest follow one by one).  Each batched move and arrival is a named
+  // variable, not an array element: an indexed array would go to memory.
   const bool compact = k == 0 && total < m;
-  int hb = total, q = total, h = 0, from_slot = nnew;
+  MoveState st{total, total, 0, nnew};
   if (compact) {
-    while (hb < m && hole_slot(M, r0, ol, hb) < nnew) ++hb;   // holes below the new count
-    h = hb;
+    while (st.hb < m && hole_slot(M, r0, ol, st.hb) < nnew) ++st.hb;   // holes below the new count
+    st.h = st.hb;
   }
-  int mdst[NMV], msrc[NMV];
-  bool mok[NMV];
-  float4 mq[NMV];
-  float muz[NMV];
-  int64_t mid[NMV];
-#pragma unroll
-  for (int t = 0; t < NMV; ++t) {
-    mok[t] = false;
-    mdst[t] = msrc[t] = 0;
-    if (compact) {
-      while (from_slot < n && h < m && hole_slot(M, r0, ol, h) == from_slot) {   // holes above
-        ++h;
-        ++from_slot;
-      }
-      if (from_slot < n && q < hb) {
-        mdst[t] = hole_slot(M, r0, ol, q);
-        msrc[t] = from_slot;
-        mok[t] = true;
-        ++q;
-        ++from_slot;
-      }
-    }
-    if (mok[t]) {
-      mq[t] = A.q[base + msrc[t]];
-      muz[t] = A.uz[base + msrc[t]];
-      mid[t] = ids[base + msrc[t]];
-    }
-  }
-  // Arrivals from direction k, in the source's list order: the first NAR
+  Move v0, v1, v2, v3;
+  plan_move(v0, st, compact, M, r0, ol, m, n, base, A, ids);
+  plan_move(v1, st, compact, M, r0, ol, m, n, base, A, ids);
+  plan_move(v2, st, compact, M, r0, ol, m, n, base, A, ids);
+  plan_move(v3, st, compact, M, r0, ol, m, n, base, A, ids);
+  // Arrivals from direction k, in the source's list order: the first two
   // records load together, the rest one by one.
-  int a = inc - c, left = c, j = 0;
-  int ms = 0;
-  size_t rs = 0;
-  if (c > 0) {
-    ms = (int)(fh.z & 0xFFu);
-    rs = bin_records(G, si, sjl, fh.z);
+  ArrState as{inc - c, c, 0, c > 0 ? (int)(fh.z & 0xFFu) : 0,
+              c > 0 ? bin_records(G, si, sjl, fh.z) : 0};
+  Arrival w0, w1;
+  plan_arrival(w0, as, k, M, fl, ol, r0, m, n);
+  plan_arrival(w1, as, k, M, fl, ol, r0, m, n);
+  put_arrival(w0, G, A, ids, derr, base);
+  put_arrival(w1, G, A, ids, derr, base);
+  for (; as.left > 0 && as.j < as.ms; ++as.j) {
+    if (list_dir(M, fl, as.rs, as.j) != k) continue;
+    Arrival w;
+    w.rec = M.rec[as.rs + as.j];
+    w.slot = as.a < m ? hole_slot(M, r0, ol, as.a) : n + (as.a - m);
+    w.ok = true;
+    put_arrival(w, G, A, ids, derr, base);
+    --as.left;
+    ++as.a;
   }
-  auto dir_of = [&](int jj) {
-    return jj < WM_SUM_MV ? sum_mv(fl.x, fl.y, fl.z, fl.w, jj) >> WM_SLOT_BITS
-                          : M.rec[rs + jj].sd >> 24;
-  };
-  auto slot_of = [&](int aa) { return aa < m ? hole_slot(M, r0, ol, aa) : n + (aa - m); };
-  auto put = [&](const MoverRec &mr, int slot) {
-    if (slot >= G.CB) {
-      atomicOr(derr, DERR_NP_OVER);
-      return;
-    }
-    const size_t p = base + slot;
-    A.q[p] = make_float4(mr.ox1, mr.oy1, mr.ux, mr.uy);
-    A.uz[p] = mr.uz;
-    ids[p] = mr.id;
-  };
-  MoverRec mr[NAR];
-  int aslot[NAR];
-  bool aok[NAR];
-#pragma unroll
-  for (int t = 0; t < NAR; ++t) {
-    aok[t] = false;
-    aslot[t] = 0;
-    while (left > 0 && j < ms && dir_of(j) != k) ++j;
-    if (left > 0 && j < ms) {
-      mr[t] = M.rec[rs + j];
-      aslot[t] = slot_of(a);
-      aok[t] = true;
-      --left;
-      ++a;
-      ++j;
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < NAR; ++t)
-    if (aok[t]) put(mr[t], aslot[t]);
-  for (; left > 0 && j < ms; ++j) {
-    if (dir_of(j) != k) continue;
-    put(M.rec[rs + j], slot_of(a));
-    --left;
-    ++a;
-  }
-#pragma unroll
-  for (int t = 0; t < NMV; ++t) {
-    if (mok[t]) {
-      const size_t dst = base + mdst[t];
-      A.q[dst] = mq[t];
-      A.uz[dst] = muz[t];
-      ids[dst] = mid[t];
-    }
-  }
+  store_move(v0, A, ids, base);
+  store_move(v1, A, ids, base);
+  store_move(v2, A, ids, base);
+  store_move(v3, A, ids, base);
+  int from_slot = st.from, h = st.h, q = st.q;
+  const int hb = st.hb;
   if (k != 0) return -1;
   if (compact) {
     for (; from_slot < n && q < hb; ++from_slot) {
