@@ -141,6 +141,37 @@ def test_fp32_settling_between_steps_changes_nothing():
                                   outs[1][0][:, :k, jl, s].view(np.int64))
 
 
+def test_fp32_bin_overflow_is_reported():
+    """A bin at 30 of its 32 slots receives 10 fast particles from its left
+    neighbour: the migration (in the next push's prologue) stops at the
+    bin's capacity, writes nothing past it, and the step reports the
+    overflow with the reference's 'memory over' error."""
+    from wumingpic2d_amd import weibel
+    p = weibel.grid_params(16, 8, 4)                 # bins of 32 slots
+    ndim, npc, ny, nsp = p["ndim"], p["np"], p["ny"], p["nsp"]
+    up = np.zeros((ndim, npc, ny, nsp), order="F")
+    np2 = np.zeros((ny, nsp), dtype=np.int32, order="F")
+    cc = np.zeros((p["nx"] + 1, ny, nsp), dtype=np.int32, order="F")
+    jl, s = 3, 0
+    x = np.concatenate([np.full(10, p["nxgs"] + 4 + 0.95), p["nxgs"] + 5 + np.linspace(0.1, 0.6, 30)])
+    ux = np.concatenate([np.full(10, 0.9), np.zeros(30)])
+    n = x.size
+    up[0, :n, jl, s] = x
+    up[1, :n, jl, s] = p["nys"] + jl + 0.5
+    up[2, :n, jl, s] = ux
+    up[5, :n, jl, s] = (-(np.arange(n, dtype=np.int64) + 1)).view(np.float64)
+    np2[jl, s] = n
+    cnt = np.bincount((x - p["nxgs"]).astype(np.int64), minlength=p["nx"])
+    cc[1:, jl, s] = np.cumsum(cnt)
+    uf = np.zeros((6, p["nx"] + 4, ny + 4), order="F")
+    ctx = _ctx32(p)
+    ctx.state_upload(up, np2, cc, uf)
+    ctx.step(1)                                      # the 10 leave their cell
+    with pytest.raises(wm.WmError) as e:
+        ctx.step(1)                                  # and arrive: 40 > 32
+    assert e.value.code == wm.WM_ERR_NP_OVER
+
+
 def test_fp32_host_array_entry_points_refuse():
     p, up, np2, cc, uf = state(16, 4, 2, 45)
     ctx = _ctx32(p)
