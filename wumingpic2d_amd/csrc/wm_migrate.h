@@ -6,6 +6,9 @@
 
 namespace wm_mig {
 
+// Inlined into the caller, so that the structs below stay in registers.
+#define WM_MIG_FN __device__ __attribute__((always_inline)) inline
+
 struct Mig {                    // bins and mover regions of one migration
   int nx, ny, nxr, CB;          // nxr: regions per row (cells rounded up to the push chunk)
   int r0, nr;                   // the migrated rows (every species; settle kernel)
@@ -16,24 +19,24 @@ struct Mig {                    // bins and mover regions of one migration
 
 // First record of the movers of the bin in cell (i, jl) in the cell's region
 // (out: the bin summary's (offset << 8) | count).
-__device__ __attribute__((always_inline)) inline size_t bin_records(const Mig &G, int i, int jl, uint32_t out) {
+WM_MIG_FN size_t bin_records(const Mig &G, int i, int jl, uint32_t out) {
   return ((size_t)jl * G.nxr + i) * WM_MG + (out >> 8);
 }
 
 // Byte k of a summary's direction counts (words lo, hi), and entry j of its
 // mover list (words w0..w3).  The words are values, not a dynamically indexed
 // array, which the compiler would place in memory.
-__device__ __attribute__((always_inline)) inline int sum_cnt(uint32_t lo, uint32_t hi, int k) {
+WM_MIG_FN int sum_cnt(uint32_t lo, uint32_t hi, int k) {
   const uint32_t v = k < 4 ? lo : hi;
   return (int)((v >> (8 * (k & 3))) & 0xFFu);
 }
-__device__ __attribute__((always_inline)) inline int sum_mv(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) {
+WM_MIG_FN int sum_mv(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) {
   const uint32_t v = j < 4 ? (j < 2 ? w0 : w1) : (j < 6 ? w2 : w3);
   return (int)((v >> (16 * (j & 1))) & 0xFFFFu);
 }
 // Slot of bin d's h-th hole: from its summary for the first WM_SUM_MV, else
 // from the record.
-__device__ __attribute__((always_inline)) inline int hole_slot(const Movers &M, size_t r0, uint4 l, int h) {
+WM_MIG_FN int hole_slot(const Movers &M, size_t r0, uint4 l, int h) {
   return h < WM_SUM_MV ? sum_mv(l.x, l.y, l.z, l.w, h) & ((1 << WM_SLOT_BITS) - 1)
                        : M.rec[r0 + h].sd & 0xFFFFFF;
 }
@@ -51,9 +54,8 @@ struct Move {                    // one compaction move: particle at src -> slot
 };
 // Plans the next move (skipping holes above the new count) and issues its
 // source's loads.
-__device__ __attribute__((always_inline)) inline void plan_move(Move &v, MoveState &st, bool compact, const Movers &M, size_t r0,
-                                 uint4 ol, int m, int n, size_t base, const Bins32 &A,
-                                 const int64_t *ids) {
+WM_MIG_FN void plan_move(Move &v, MoveState &st, bool compact, const Movers &M, size_t r0, uint4 ol,
+                         int m, int n, size_t base, const Bins32 &A, const int64_t *ids) {
   v.ok = false;
   v.dst = v.src = 0;
   if (compact) {
@@ -79,7 +81,7 @@ __device__ __attribute__((always_inline)) inline void plan_move(Move &v, MoveSta
     v.id = ids[base + v.src];
   }
 }
-__device__ __attribute__((always_inline)) inline void store_move(const Move &v, const Bins32 &A, int64_t *ids, size_t base) {
+WM_MIG_FN void store_move(const Move &v, const Bins32 &A, int64_t *ids, size_t base) {
   if (!v.ok) return;
   A.q[base + v.dst] = make_float4(v.qx, v.qy, v.qz, v.qw);
   A.uz[base + v.dst] = v.uz;
@@ -87,7 +89,7 @@ __device__ __attribute__((always_inline)) inline void store_move(const Move &v, 
 }
 
 // Direction index of entry j of a source's mover list.
-__device__ __attribute__((always_inline)) inline int list_dir(const Movers &M, uint4 fl, size_t rs, int j) {
+WM_MIG_FN int list_dir(const Movers &M, uint4 fl, size_t rs, int j) {
   return j < WM_SUM_MV ? sum_mv(fl.x, fl.y, fl.z, fl.w, j) >> WM_SLOT_BITS
                        : M.rec[rs + j].sd >> 24;
 }
@@ -105,8 +107,8 @@ struct Arrival {
 };
 // Finds the next arrival of direction k in the source's list and issues its
 // record's load.
-__device__ __attribute__((always_inline)) inline void plan_arrival(Arrival &w, ArrState &as, int k, const Movers &M, uint4 fl,
-                                    uint4 ol, size_t r0, int m, int n) {
+WM_MIG_FN void plan_arrival(Arrival &w, ArrState &as, int k, const Movers &M, uint4 fl, uint4 ol,
+                            size_t r0, int m, int n) {
   w.ok = false;
   w.slot = 0;
   while (as.left > 0 && as.j < as.ms && list_dir(M, fl, as.rs, as.j) != k) ++as.j;
@@ -119,8 +121,8 @@ __device__ __attribute__((always_inline)) inline void plan_arrival(Arrival &w, A
     ++as.j;
   }
 }
-__device__ __attribute__((always_inline)) inline void put_arrival(const Arrival &w, const Mig &G, const Bins32 &A, int64_t *ids,
-                                   int *derr, size_t base) {
+WM_MIG_FN void put_arrival(const Arrival &w, const Mig &G, const Bins32 &A, int64_t *ids, int *derr,
+                           size_t base) {
   if (!w.ok) return;
   if (w.slot >= G.CB) {
     atomicOr(derr, DERR_NP_OVER);
@@ -148,8 +150,8 @@ __device__ __attribute__((always_inline)) inline void put_arrival(const Arrival 
 // records.
 constexpr int DL = 8;
 // Returns d's new count to lane 0 of the group (-1 to the others).
-__device__ __attribute__((always_inline)) inline int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids,
-                                  int *derr, size_t d, int i, int jl, int sp, int k) {
+WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids, int *derr,
+                          size_t d, int i, int jl, int sp, int k) {
   const int dir = k < 4 ? k : k + 1;
   const int si = wm_wrap(i - (dir % 3 - 1), G.nx), sjl = wm_wrap(jl - (dir / 3 - 1), G.ny);
   const size_t src = ((size_t)sp * G.ny + sjl) * G.nx + si;
