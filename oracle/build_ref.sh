@@ -1,5 +1,5 @@
 #!/bin/bash
-# Compiles the reference's own particle and sort modules, exactly as they lie in
+# Compiles the reference's own particle, sort and moment modules, exactly as they lie in
 # the reference tree, into oracle/_ref/libwmref.so.  Nothing else is compiled:
 # field.f90 and boundary_periodic.f90 `use mpi`, and this image has no MPI
 # module in the compiler's format, so those two files are not buildable here.
@@ -14,7 +14,7 @@ if [ ! -f "$REF/common/particle.f90" ]; then
 fi
 mkdir -p "$OUT"
 amdflang -O2 -fPIC -shared -module-dir "$OUT" \
-  "$REF/common/particle.f90" "$REF/common/sort.f90" \
+  "$REF/common/particle.f90" "$REF/common/sort.f90" "$REF/common/mom_calc.f90" \
   -o "$OUT/libwmref.so.tmp"
 mv "$OUT/libwmref.so.tmp" "$OUT/libwmref.so"
 echo "build_ref: built $OUT/libwmref.so"

@@ -71,7 +71,8 @@ class Oracle:
         for n, k in (("or_particle_solv", 5), ("or_ele_cur", 5), ("or_bc_curre", 2),
                      ("or_bc_dfield", 2), ("or_bc_phi", 2), ("or_field_rhs", 4),
                      ("or_field_fdtd_i", 6), ("or_bc_particle_x", 3), ("or_bc_particle_y", 4),
-                     ("or_sort_bucket", 5), ("or_energy", 5), ("or_step", 7)):
+                     ("or_sort_bucket", 5), ("or_energy", 5), ("or_step", 7),
+                     ("or_mom_calc_accl", 5), ("or_mom_calc_nvt", 4), ("or_bc_mom", 2)):
             getattr(L, n).argtypes = [vp] * k
         self.p = p
         self.par = OrParams()
@@ -138,6 +139,16 @@ class Oracle:
     def sort__bucket(self, gp, up, cumcnt, np2):
         self._ck(self.lib.or_sort_bucket(self.pp, _p(gp), _p(up), _p(cumcnt), _p(np2)), "sort__bucket")
 
+    def mom_calc_accl(self, gp, up, uf, cumcnt):
+        self._ck(self.lib.or_mom_calc_accl(self.pp, _p(gp), _p(up), _p(uf), _p(cumcnt)),
+                 "mom_calc__accl")
+
+    def mom_calc_nvt(self, mom, up, np2):
+        self._ck(self.lib.or_mom_calc_nvt(self.pp, _p(mom), _p(up), _p(np2)), "mom_calc__nvt")
+
+    def bc_mom(self, mom):
+        self.lib.or_bc_mom(self.pp, _p(mom))
+
     def energy(self, up, uf, np2):
         out = np.zeros(self.p["nsp"] + 2)
         self.lib.or_energy(self.pp, _p(up), _p(uf), _p(np2), _p(out))
@@ -179,7 +190,7 @@ def ref_available() -> bool:
 
 
 class RefBuild:
-    """The reference's particle.f90 / sort.f90, compiled unchanged.
+    """The reference's particle.f90 / sort.f90 / mom_calc.f90, compiled unchanged.
 
     Module procedures are called through their compiler symbols with every
     argument by reference.  The reference's *_init routines allocate module
@@ -198,6 +209,9 @@ class RefBuild:
         self.lib._QMparticlePparticle__solv.argtypes = [vp] * 6
         self.lib._QMsortPsort__init.argtypes = [vp] * 9
         self.lib._QMsortPsort__bucket.argtypes = [vp] * 6
+        self.lib._QMmom_calcPmom_calc__init.argtypes = [vp] * 14
+        self.lib._QMmom_calcPmom_calc__accl.argtypes = [vp] * 6
+        self.lib._QMmom_calcPmom_calc__nvt.argtypes = [vp] * 3
         self.p = p
         self._keep = []
         self._q = np.ascontiguousarray(p["q"], dtype=np.float64)
@@ -206,6 +220,8 @@ class RefBuild:
         self.lib._QMparticlePparticle__init(*g, self.D(p["delx"]), self.D(p["delt"]),
                                             self.D(p["c"]), _p(self._q), _p(self._r))
         self.lib._QMsortPsort__init(*g)
+        self.lib._QMmom_calcPmom_calc__init(*g, self.D(p["delx"]), self.D(p["delt"]),
+                                            self.D(p["c"]), _p(self._q), _p(self._r))
 
     def I(self, v):
         o = ctypes.c_int(int(v))
@@ -227,6 +243,14 @@ class RefBuild:
         I = self.I
         self.lib._QMparticlePparticle__solv(_p(gp), _p(up), _p(uf), _p(cumcnt),
                                             I(self.p["nxgs"]), I(self.p["nxge"]))
+
+    def mom_calc__accl(self, gp, up, uf, cumcnt):
+        I = self.I
+        self.lib._QMmom_calcPmom_calc__accl(_p(gp), _p(up), _p(uf), _p(cumcnt),
+                                            I(self.p["nxgs"]), I(self.p["nxge"]))
+
+    def mom_calc__nvt(self, mom, up, np2):
+        self.lib._QMmom_calcPmom_calc__nvt(_p(mom), _p(up), _p(np2))
 
     def sort__bucket(self, gp, up, cumcnt, np2):
         I = self.I

@@ -855,6 +855,167 @@ int or_sort_bucket(const or_params *p, double *gp, const double *up,
 }
 
 /* --------------------------------------------------------- energy sums */
+/* --------------------------------------------------- moment diagnostics */
+static inline size_t imom(const or_params *p, int k, int i, int j, int isp) {
+  /* mom(7, nxgs-1:nxge+1, nys-1:nye+1, nsp) */
+  return (size_t)(k - 1) +
+         7u * ((size_t)(i - (p->nxgs - 1)) +
+               (size_t)(NXg(p) + 2) * ((size_t)(j - (p->nys - 1)) +
+                                       (size_t)(NYl(p) + 2) * (size_t)(isp - 1)));
+}
+
+int or_mom_calc_accl(const or_params *p, double *gp, const double *up,
+                     const double *uf, const int *cumcnt) {
+  /* mom_calc.f90:48-164: particle.f90's velocity update with the step that
+   * mom_calc__init keeps, delt*0.5 (mom_calc.f90:34), and no move: the
+   * positions are copied.  gp(6) is not written. */
+  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxt = nxe - nxs + 3, nyt = nye - nys + 3;
+  const double d_delx = 1. / p->delx;           /* mom_calc.f90:41 */
+  const double delt = p->delt * 0.5, c = p->c;  /* :34 */
+  double *tmp = (double *)malloc(sizeof(double) * 6u * (size_t)nxt * (size_t)nyt);
+  if (!tmp) return OR_ERR_ARG;
+#define TMP(k, i, j) tmp[(size_t)((k) - 1) + 6u * ((size_t)((i) - (nxs - 1)) + (size_t)nxt * (size_t)((j) - (nys - 1)))]
+#define UF(k, i, j) uf[iuf(p, k, i, j)]
+  /* fields at (i+1/2, j+1/2): :68-81 */
+  for (int j = nys - 1; j <= nye + 1; ++j)
+    for (int i = nxs - 1; i <= nxe + 1; ++i) {
+      TMP(1, i, j) = 0.5 * (UF(1, i, j) + UF(1, i, j + 1));
+      TMP(2, i, j) = 0.5 * (UF(2, i, j) + UF(2, i + 1, j));
+      TMP(3, i, j) = 0.25 * (UF(3, i, j) + UF(3, i + 1, j) + UF(3, i, j + 1) +
+                             UF(3, i + 1, j + 1));
+      TMP(4, i, j) = 0.5 * (UF(4, i, j) + UF(4, i + 1, j));
+      TMP(5, i, j) = 0.5 * (UF(5, i, j) + UF(5, i, j + 1));
+      TMP(6, i, j) = UF(6, i, j);
+    }
+  int bad = 0;
+  for (int j = nys; j <= nye; ++j)
+    for (int i = nxs; i <= nxe; ++i)
+      for (int isp = 1; isp <= p->nsp; ++isp) {
+        const double q = p->q[isp - 1], r = p->r[isp - 1];
+        const double fac1 = q / r * 0.5 * delt;  /* :90 */
+        const double txxx = fac1 * fac1;         /* :91 */
+        const double fac2 = q * delt / r;        /* :92 */
+        const int i0 = cumcnt[icc(p, i, j, isp)], i1 = cumcnt[icc(p, i + 1, j, isp)];
+        if (i0 < 0 || i1 > p->np || i1 < i0) { bad = 1; continue; }
+        for (int ii = i0 + 1; ii <= i1; ++ii) {
+          const double *u = up + iup(p, 1, ii, j, isp);
+          double *g = gp + iup(p, 1, ii, j, isp);
+          double sh[3][2], dh;
+          /* :97-105 */
+          dh = u[0] * d_delx - 0.5 - i;
+          sh[0][0] = 0.5 * (0.5 - dh) * (0.5 - dh);
+          sh[1][0] = 0.75 - dh * dh;
+          sh[2][0] = 0.5 * (0.5 + dh) * (0.5 + dh);
+          dh = u[1] * d_delx - 0.5 - j;
+          sh[0][1] = 0.5 * (0.5 - dh) * (0.5 - dh);
+          sh[1][1] = 0.75 - dh * dh;
+          sh[2][1] = 0.5 * (0.5 + dh) * (0.5 + dh);
+          /* :107-129 */
+          double f[6];
+          for (int k = 1; k <= 6; ++k)
+            f[k - 1] =
+                (TMP(k, i - 1, j - 1) * sh[0][0] + TMP(k, i, j - 1) * sh[1][0] +
+                 TMP(k, i + 1, j - 1) * sh[2][0]) * sh[0][1] +
+                (TMP(k, i - 1, j) * sh[0][0] + TMP(k, i, j) * sh[1][0] +
+                 TMP(k, i + 1, j) * sh[2][0]) * sh[1][1] +
+                (TMP(k, i - 1, j + 1) * sh[0][0] + TMP(k, i, j + 1) * sh[1][0] +
+                 TMP(k, i + 1, j + 1) * sh[2][0]) * sh[2][1];
+          const double bpx = f[0], bpy = f[1], bpz = f[2];
+          const double epx = f[3], epy = f[4], epz = f[5];
+          /* :132-148 */
+          double uvm1 = u[2] + fac1 * epx;
+          double uvm2 = u[3] + fac1 * epy;
+          double uvm3 = u[4] + fac1 * epz;
+          const double gam = sqrt(c * c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+          const double igam = 1. / gam;
+          const double fac1r = fac1 * igam;
+          const double fac2r =
+              fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+          const double uvm4 = uvm1 + fac1r * (uvm2 * bpz - uvm3 * bpy);
+          const double uvm5 = uvm2 + fac1r * (uvm3 * bpx - uvm1 * bpz);
+          const double uvm6 = uvm3 + fac1r * (uvm1 * bpy - uvm2 * bpx);
+          uvm1 = uvm1 + fac2r * (uvm5 * bpz - uvm6 * bpy);
+          uvm2 = uvm2 + fac2r * (uvm6 * bpx - uvm4 * bpz);
+          uvm3 = uvm3 + fac2r * (uvm4 * bpy - uvm5 * bpx);
+          /* :151-155 */
+          g[0] = u[0];
+          g[1] = u[1];
+          g[2] = uvm1 + fac1 * epx;
+          g[3] = uvm2 + fac1 * epy;
+          g[4] = uvm3 + fac1 * epz;
+        }
+      }
+  free(tmp);
+#undef TMP
+#undef UF
+  return bad ? OR_ERR_RANGE : OR_OK;
+}
+
+int or_mom_calc_nvt(const or_params *p, double *mom, const double *up,
+                    const int *np2) {
+  /* mom_calc.f90:167-249, in its serial order (species, row, particle).
+   * ih = int(x/delx - 0.5) truncates toward zero, as Fortran's int() does:
+   * below x = delx/2 of cell 0 it is 0, not -1, and the weights extrapolate.
+   * A particle whose nodes fall outside mom's bounds is skipped and reported
+   * (the reference would write out of bounds). */
+  const double d_delx = 1. / p->delx, c = p->c;
+  const size_t n = 7u * (size_t)(NXg(p) + 2) * (size_t)(NYl(p) + 2) * (size_t)p->nsp;
+  for (size_t t = 0; t < n; ++t) mom[t] = 0.0;   /* :182 */
+  int bad = 0;
+  for (int isp = 1; isp <= p->nsp; ++isp)
+    for (int j = p->nys; j <= p->nye; ++j)
+      for (int ii = 1; ii <= np2[inp2(p, j, isp)]; ++ii) {
+        const double *u = up + iup(p, 1, ii, j, isp);
+        const int ih = (int)(u[0] * d_delx - 0.5);   /* :190-191 */
+        const int jh = (int)(u[1] * d_delx - 0.5);
+        if (ih < p->nxgs - 1 || ih > p->nxge || jh < p->nys - 1 || jh > p->nye) {
+          bad = 1;
+          continue;
+        }
+        const double dx = u[0] * d_delx - 0.5 - ih;  /* :193-196 */
+        const double dxm = 1. - dx;
+        const double dy = u[1] * d_delx - 0.5 - jh;
+        const double dym = 1. - dy;
+        const double gam =
+            1. / sqrt(1.0 + (u[2] * u[2] + u[3] * u[3] + u[4] * u[4]) / (c * c));
+        /* per node (ih, jh), (ih+1, jh), (ih, jh+1), (ih+1, jh+1): :203-243 */
+        const double wx[2] = {dxm, dx}, wy[2] = {dym, dy};
+        const double v[7] = {1.0, u[2] * gam, u[3] * gam, u[4] * gam,
+                             u[2] * u[2] * gam, u[3] * u[3] * gam, u[4] * u[4] * gam};
+        for (int k = 1; k <= 7; ++k)
+          for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) {
+              /* N: dx*dy; the others: ((u gam) dx) dy, as written */
+              const double w = k == 1 ? wx[a] * wy[b] : v[k - 1] * wx[a] * wy[b];
+              mom[imom(p, k, ih + a, jh + b, isp)] += w;
+            }
+      }
+  return bad ? OR_ERR_RANGE : OR_OK;
+}
+
+void or_bc_mom(const or_params *p, double *mom) {
+  /* boundary_periodic.f90:571-636 with one rank (nup = ndown = itself): the
+   * x ghosts fold into the opposite edge over every row, ghost rows
+   * included (:579-584); then, per species, row nys-1 into row nye (:586-608)
+   * and row nye+1 into row nys (:611-633), ghost columns included. */
+  const int nxgs = p->nxgs, nxge = p->nxge, nys = p->nys, nye = p->nye;
+  for (int isp = 1; isp <= p->nsp; ++isp)
+    for (int j = nys - 1; j <= nye + 1; ++j)
+      for (int k = 1; k <= 7; ++k) {
+        mom[imom(p, k, nxgs, j, isp)] += mom[imom(p, k, nxge + 1, j, isp)];
+        mom[imom(p, k, nxge, j, isp)] += mom[imom(p, k, nxgs - 1, j, isp)];
+      }
+  for (int isp = 1; isp <= p->nsp; ++isp) {
+    for (int i = nxgs - 1; i <= nxge + 1; ++i)
+      for (int k = 1; k <= 7; ++k)
+        mom[imom(p, k, i, nye, isp)] += mom[imom(p, k, i, nys - 1, isp)];
+    for (int i = nxgs - 1; i <= nxge + 1; ++i)
+      for (int k = 1; k <= 7; ++k)
+        mom[imom(p, k, i, nys, isp)] += mom[imom(p, k, i, nye + 1, isp)];
+  }
+}
+
 void or_energy(const or_params *p, const double *up, const double *uf,
                const int *np2, double *out) {
   /* proj/weibel/app.f90:499-535 */

@@ -39,6 +39,12 @@
  *                      boundary_periodic.f90; the order follows a reading of
  *                      boundary_periodic.f90:143-236 only).
  *   or_energy          exact sums of closed-form states.
+ *   or_mom_calc_accl   bit-exact vs the amdflang build of common/mom_calc.f90;
+ *   or_mom_calc_nvt    accl = particle__solv's velocities at delt/2; zero
+ *                      fields leave u unchanged; a single particle's four node
+ *                      weights (truncating int() below x = delx/2); partition
+ *                      of unity of N and sum Vx = sum u gamma^-1.
+ *   or_bc_mom          every ghost entry lands on its periodic image once.
  */
 #ifndef WM_ORACLE_H
 #define WM_ORACLE_H
@@ -108,6 +114,16 @@ int or_bc_particle_y(const or_params *p, or_state *s, double *up, int *np2);
 /* sort__bucket (sort.f90:36-82) */
 int or_sort_bucket(const or_params *p, double *gp, const double *up,
                    int *cumcnt, const int *np2);
+/* mom_calc__accl (mom_calc.f90:48-164); p->delt is the run's step (the
+ * module halves it, :34).  Writes gp(1:5) of the particles in cumcnt. */
+int or_mom_calc_accl(const or_params *p, double *gp, const double *up,
+                     const double *uf, const int *cumcnt);
+/* mom_calc__nvt (mom_calc.f90:167-249): mom (7, nxgs-1:nxge+1, nys-1:nye+1,
+ * nsp) = N, Vx, Vy, Vz, Txx, Tyy, Tzz (CIC sums) of up's np2 particles. */
+int or_mom_calc_nvt(const or_params *p, double *mom, const double *up,
+                    const int *np2);
+/* boundary_periodic__mom (boundary_periodic.f90:571-636), one rank */
+void or_bc_mom(const or_params *p, double *mom);
 /* energy_history sums (proj/weibel/app.f90:479-545): out[0..nsp-1] kinetic
  * per species, out[nsp] E energy, out[nsp+1] B energy. */
 void or_energy(const or_params *p, const double *up, const double *uf,
