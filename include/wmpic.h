@@ -164,6 +164,30 @@ int wm_phase_times(wm_ctx *ctx, double *ms, int n);
 /* Total active particles (sum of np2). */
 int wm_particle_count(wm_ctx *ctx, long long *count);
 
+/* ---- moment diagnostics of an output step (proj/weibel/app.f90:119-121)
+ *      mom (7, nxgs-1:nxge+1, nys-1:nye+1, nsp) real(8): N, Vx, Vy, Vz, Txx,
+ *      Tyy, Tzz per node and species ---- */
+/* mom_calc__init (mom_calc.f90:18-45); delt is the run's step, the module
+ * advances u over delt/2 (mom_calc.f90:34). */
+int wm_mom_calc_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs,
+                     int nxge, int nygs, int nyge, int nys, int nye,
+                     double delx, double delt, double c, const double *q,
+                     const double *r);
+/* mom_calc__accl(gp,up,uf,cumcnt,nxs,nxe) (mom_calc.f90:48-164): gp(1:5) of
+ * the particles in cumcnt; gp(6) is not written. */
+int wm_mom_calc_accl(wm_ctx *ctx, double *gp, const double *up,
+                     const double *uf, const int *cumcnt, int nxs, int nxe);
+/* mom_calc__nvt(mom,up,np2) (mom_calc.f90:167-249).  Positions must lie in
+ * the cells nxgs..nxge of their rows (as after sort__bucket). */
+int wm_mom_calc_nvt(wm_ctx *ctx, double *mom, const double *up,
+                    const int *np2);
+/* boundary_periodic__mom(mom) (boundary_periodic.f90:571-636) */
+int wm_bcp_mom(wm_ctx *ctx, double *mom);
+/* The three calls above on the device-resident state, fused (u over delt/2
+ * and the moments in one pass over the particles, then the fold); only mom
+ * is copied to the host.  Needs mom_calc__init. */
+int wm_moments(wm_ctx *ctx, double *mom);
+
 /* ---- learned preconditioner of the implicit solve (no reference
  *      counterpart: the reference runs unpreconditioned CG, field.f90:319-461).
  *      A four-level linear convolutional net M(r) ~ (f4 I - N4)^-1 r with BF16

@@ -68,6 +68,7 @@ EXPORTS = (
     "wm_phase_times", "wm_particle_count",
     "wm_set_solver", "wm_pc_collect", "wm_pc_pair_count", "wm_pc_init", "wm_pc_train",
     "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_grad", "wm_solve_stats",
+    "wm_mom_calc_init", "wm_mom_calc_accl", "wm_mom_calc_nvt", "wm_bcp_mom", "wm_moments",
 )
 
 
@@ -140,6 +141,11 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_pc_apply": [_P, _P, _P],
         "wm_pc_grad": [_P, _P, _P, _P, _P],
         "wm_solve_stats": [_P, _P, _P],
+        "wm_mom_calc_init": [_P] + [_I] * 9 + [_D, _D, _D, _P, _P],
+        "wm_mom_calc_accl": [_P, _P, _P, _P, _P, _I, _I],
+        "wm_mom_calc_nvt": [_P, _P, _P, _P],
+        "wm_bcp_mom": [_P, _P],
+        "wm_moments": [_P, _P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -257,6 +263,15 @@ class Context:
                                           ns.ctypes.data, ns.size, delx, delt, c))
         self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
 
+    def mom_calc__init(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c,
+                       q: Sequence[float], r: Sequence[float]) -> None:
+        qa = np.ascontiguousarray(q, dtype=np.float64)
+        ra = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.wm_mom_calc_init(self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge,
+                                               nys, nye, delx, delt, c, qa.ctypes.data,
+                                               ra.ctypes.data))
+        self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+
     # ------------------------------------------- host-array entry points
     def particle__solv(self, gp, up, uf, cumcnt, nxs, nxe) -> None:
         self._check(self._lib.wm_particle_solv(self._h, _ptr(gp, np.float64, "gp"),
@@ -297,6 +312,20 @@ class Context:
         self._check(self._lib.wm_bcp_phi(self._h, _ptr(phi, np.float64, "phi"), nxs, nxe, nys,
                                          nye, l))
 
+    def mom_calc__accl(self, gp, up, uf, cumcnt, nxs, nxe) -> None:
+        self._check(self._lib.wm_mom_calc_accl(self._h, _ptr(gp, np.float64, "gp"),
+                                               _ptr(up, np.float64, "up"),
+                                               _ptr(uf, np.float64, "uf"),
+                                               _ptr(cumcnt, np.int32, "cumcnt"), nxs, nxe))
+
+    def mom_calc__nvt(self, mom, up, np2) -> None:
+        self._check(self._lib.wm_mom_calc_nvt(self._h, _ptr(mom, np.float64, "mom"),
+                                              _ptr(up, np.float64, "up"),
+                                              _ptr(np2, np.int32, "np2")))
+
+    def boundary_periodic__mom(self, mom) -> None:
+        self._check(self._lib.wm_bcp_mom(self._h, _ptr(mom, np.float64, "mom")))
+
     # ------------------------------------------------- resident run
     def state_upload(self, up, np2, cumcnt, uf) -> None:
         self._check(self._lib.wm_state_upload(self._h, _ptr(up, np.float64, "up"),
@@ -322,6 +351,14 @@ class Context:
         out = np.zeros(nsp + 2)
         self._check(self._lib.wm_energy(self._h, out.ctypes.data))
         return out
+
+    def moments(self) -> np.ndarray:
+        """mom (7, nx+2, ny+2, nsp) of the resident state: mom_calc__accl,
+        mom_calc__nvt and boundary_periodic__mom on the device."""
+        g = self.grid
+        mom = np.zeros((7, g["nx"] + 2, g["ny"] + 2, g["nsp"]), order="F")
+        self._check(self._lib.wm_moments(self._h, mom.ctypes.data))
+        return mom
 
     def cg_iters(self) -> np.ndarray:
         out = np.zeros(3, dtype=np.int32)

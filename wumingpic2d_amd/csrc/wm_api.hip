@@ -243,7 +243,7 @@ int upload_particles(wm_ctx *ctx, const double *up, const int *cnt_host, int b) 
 
 // Write the active records of bank `b` (cnt_host per segment) into a
 // reference-layout host array; other slots are left untouched.
-int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b) {
+int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b, int ncomp = 0) {
   const wm_grid &g = ctx->g;
   const int nseg = nseg_of(ctx);
   int w = 1;
@@ -268,6 +268,13 @@ int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b) {
   WM_HIP(hipMemcpyAsync(h.data(), ctx->stage, sizeof(double) * elems, hipMemcpyDeviceToHost,
                         ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (ncomp > 0 && ncomp < g.ndim) {      // the first ncomp components of each record only
+    for (int s = 0; s < nseg; ++s)
+      for (int k = 0; k < cnt_host[s]; ++k)
+        memcpy(up + ((size_t)s * g.np + k) * g.ndim, &h[((size_t)s * w + k) * g.ndim],
+               sizeof(double) * ncomp);
+    return WM_OK;
+  }
   for (int s = 0; s < nseg; ++s)
     memcpy(up + (size_t)s * g.np * g.ndim, &h[(size_t)s * w * g.ndim],
            sizeof(double) * (size_t)cnt_host[s] * g.ndim);
@@ -517,7 +524,7 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
                   ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum[0],
                   ctx->bin_sum[1], ctx->mv[0].rec, ctx->mv[1].rec, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
-                  ctx->pc_sc, ctx->fcg_s};
+                  ctx->pc_sc, ctx->fcg_s, ctx->mom, ctx->mom_part};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   wm_pc_free(ctx);
@@ -908,6 +915,84 @@ int wm_energy(wm_ctx *ctx, double *out) {
   if (!ctx || !out) return WM_ERR_ARG;
   if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
   return wm_k_energy(ctx, 0, out);
+}
+
+// ------------------------------------------------------ moment diagnostics
+int wm_mom_calc_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs,
+                     int nyge, int nys, int nye, double delx, double delt, double c,
+                     const double *q, const double *r) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(check_grid(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, delx, delt, c, q, r, true));
+  if (ctx->g.nx > 16384) {     // mom_calc__nvt's host-array form bins rows with the sort
+    wm_set_error(ctx, WM_ERR_ARG, "mom_calc: nx=%d exceeds the 16384 cells one LDS histogram holds",
+                 ctx->g.nx);
+    return WM_ERR_ARG;
+  }
+  ctx->minit = true;
+  return WM_OK;
+}
+
+int wm_mom_calc_accl(wm_ctx *ctx, double *gp, const double *up, const double *uf,
+                     const int *cumcnt, int nxs, int nxe) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(require_fp64(ctx, "mom_calc__accl"));
+  WM_TRY(wm_alloc_device(ctx));
+  const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
+  WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
+  WM_TRY(upload_cumcnt(ctx, cumcnt));
+  WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
+  WM_TRY(wm_k_accl(ctx, 0, 1));
+  WM_TRY(wm_check_derr(ctx, "mom_calc__accl"));
+  // gp(1:5) of the particles in cumcnt; gp(6) is not written (mom_calc.f90:151-155)
+  return download_particles(ctx, gp, cnt.data(), 1, 5);
+}
+
+int wm_mom_calc_nvt(wm_ctx *ctx, double *mom, const double *up, const int *np2) {
+  if (!ctx || !mom) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
+  WM_TRY(require_fp64(ctx, "mom_calc__nvt"));
+  WM_TRY(wm_alloc_device(ctx));
+  // the rows' particles, in any order, are binned by cell first (the sums
+  // then run per cell; the reference sums in row order)
+  const int nseg = nseg_of(ctx);
+  WM_TRY(upload_particles(ctx, up, np2, 1));
+  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(wm_k_sort(ctx, 1, 0));
+  WM_TRY(wm_k_moments(ctx, false));
+  WM_TRY(wm_check_derr(ctx, "mom_calc__nvt"));
+  WM_HIP(hipMemcpyAsync(mom, ctx->mom, sizeof(double) * wm_mom_elems(ctx), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int wm_bcp_mom(wm_ctx *ctx, double *mom) {
+  if (!ctx || !mom) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(wm_mom_buffers(ctx));
+  const size_t n = wm_mom_elems(ctx);
+  WM_HIP(hipMemcpyAsync(ctx->mom, mom, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(wm_k_bc_mom(ctx, ctx->mom));
+  WM_HIP(hipMemcpyAsync(mom, ctx->mom, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int wm_moments(wm_ctx *ctx, double *mom) {
+  if (!ctx || !mom) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
+  if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
+  if (ctx->form == WM_FORM_FP32) WM_TRY(wm_settle_movers(ctx));
+  WM_TRY(wm_k_moments(ctx, true));
+  WM_TRY(wm_k_bc_mom(ctx, ctx->mom));
+  WM_TRY(wm_check_derr(ctx, "mom_calc__nvt"));
+  WM_HIP(hipMemcpyAsync(mom, ctx->mom, sizeof(double) * wm_mom_elems(ctx), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
 }
 
 int wm_cg_iters(wm_ctx *ctx, int *iters3) {

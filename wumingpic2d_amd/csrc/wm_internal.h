@@ -119,7 +119,7 @@ struct wm_ctx {
   char msg[512] = {0};
 
   // init flags and parameters (one grid per context)
-  bool pinit = false, finit = false, sinit = false, binit = false;
+  bool pinit = false, finit = false, sinit = false, binit = false, minit = false;
   bool have_grid = false;
   wm_grid g{};
   double cg_err = 1e-6;    // field.f90:339
@@ -191,6 +191,8 @@ struct wm_ctx {
   bool mv_pending = false;       // its movers are not in their bins yet
   int nregion = 0;
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
+  // moment diagnostics: mom in the reference layout, per-tile node sums
+  double *mom = nullptr, *mom_part = nullptr;
   long long cg_iters_total = 0;
 
   // implicit solve: plain CG (the reference's) or flexible CG with the
@@ -241,6 +243,15 @@ int wm_push32_regions(int nx, int ny);   // mover regions of the push launch
 int wm_k_deposit_gather32(wm_ctx *ctx);
 int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);
 int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk);
+// Moment diagnostics (wm_mom.hip): CIC moments of bank 0 with cumcnt (FP64)
+// or of the settled bins (FP32) into ctx->mom (allocated by wm_mom_buffers on
+// first use), u first advanced over half a step if accl; the half-step
+// update alone into bank dst; the periodic fold
+size_t wm_mom_elems(const wm_ctx *ctx);
+int wm_mom_buffers(wm_ctx *ctx);
+int wm_k_moments(wm_ctx *ctx, bool accl);
+int wm_k_accl(wm_ctx *ctx, int src, int dst);
+int wm_k_bc_mom(wm_ctx *ctx, double *mom);
 // Learned preconditioner (wm_pc.hip)
 int wm_pc_nparam();
 int wm_pc_alloc(wm_ctx *ctx);

@@ -1,0 +1,372 @@
+// wm_mom.hip -- the moment diagnostics of an output step on the device:
+// mom_calc__accl's velocity update over half a step (mom_calc.f90:48-164),
+// mom_calc__nvt's cloud-in-cell moments N, V, T (:167-249), fused per tile of
+// cells, and boundary_periodic__mom's periodic fold
+// (boundary_periodic.f90:571-636).
+//
+// One workgroup takes MC cells x MR rows of one species.  Eight lanes share a
+// cell; each lane sums its particles' contributions to the 3 x 3 nodes around
+// the cell (x - 1/2 truncates to the cell or the one before it), the eight
+// lanes add theirs in a fixed butterfly, and the cell sums go into the
+// tile's node sums in LDS in ascending cell order.  The tiles overlap by two
+// node columns and two node rows; a gather adds the (at most four) tiles of a
+// node in a fixed order.  Every sum has one order: the result does not depend
+// on scheduling.
+#include "wm_internal.h"
+
+namespace {
+
+constexpr int MC = 32;           // cells per workgroup along x
+constexpr int MR = 8;            // rows per workgroup
+constexpr int MG = 8;            // lanes per cell
+constexpr int MW = MC + 2;       // tile columns: cells / nodes i0-1 .. i0+MC
+constexpr int MH = MR + 2;       // tile rows: j0-1 .. j0+MR
+constexpr int NMOM = 7;          // N, Vx, Vy, Vz, Txx, Tyy, Tzz
+constexpr int NCELL = 9 * NMOM;  // one cell's sums: 3 x 3 nodes x 7 moments
+constexpr int CSTR = NCELL + 1;  // LDS stride of one cell's sums
+constexpr int TSZ = NMOM * MH * MW;
+
+struct MP {
+  int nx, ny, nxgs, nys, nsp, nchunk, nrb, cap, CB;
+  double d_delx, c, delt;        // delt: the moment step, half the run's (mom_calc.f90:34)
+  double q[WM_MAXSP], r[WM_MAXSP];
+};
+
+struct Pt {
+  double x, y, ux, uy, uz;
+};
+
+// FP64 form: the records of cell i (local index) of a row, in cumcnt order.
+struct Src64 {
+  const double *x, *y, *ux, *uy, *uz;
+  const int *cumcnt;
+  __device__ void range(const MP &P, int s, int jl, int i, size_t &base, int &kb, int &ke) const {
+    const size_t seg = (size_t)s * P.ny + jl;
+    const int *cc = cumcnt + seg * (size_t)(P.nx + 1);
+    kb = max(0, cc[i]);
+    ke = min(P.cap, cc[i + 1]);
+    base = seg * (size_t)P.cap;
+  }
+  __device__ Pt load(size_t p, int, int) const { return Pt{x[p], y[p], ux[p], uy[p], uz[p]}; }
+};
+
+// FP32 form: the cell's bin.  The position is the cell's index plus the
+// offset, in FP64, as the layout conversion to the reference arrays forms it.
+struct Src32 {
+  const float4 *q;
+  const float *uz;
+  const int *cnt;
+  __device__ void range(const MP &P, int s, int jl, int i, size_t &base, int &kb, int &ke) const {
+    const size_t bin = ((size_t)s * P.ny + jl) * P.nx + i;
+    kb = 0;
+    ke = min(P.CB, cnt[bin]);
+    base = bin * (size_t)P.CB;
+  }
+  __device__ Pt load(size_t p, int iabs, int jabs) const {
+    const float4 v = q[p];
+    return Pt{(double)iabs + (double)v.x, (double)jabs + (double)v.y, (double)v.z, (double)v.w,
+              (double)uz[p]};
+  }
+};
+
+__device__ inline void tsc(double dh, double &sm, double &s0, double &sp) {
+  // mom_calc.f90:97-105
+  sm = 0.5 * (0.5 - dh) * (0.5 - dh);
+  s0 = 0.75 - dh * dh;
+  sp = 0.5 * (0.5 + dh) * (0.5 + dh);
+}
+
+// mom_calc.f90:107-155: the Boris update of u over the moment step from the
+// cell-centred fields F (rows of cells j-1..j+1 from Frow, columns of cells
+// i-1..i+1 from col); the position is not advanced.
+__device__ inline void half_push(const double *F, int Frow, int col, const Pt &v, int iabs,
+                                 int jabs, const MP &P, double fac1, double txxx, double fac2,
+                                 double &ux, double &uy, double &uz) {
+  double sx0, sx1, sx2, sy0, sy1, sy2;
+  tsc(v.x * P.d_delx - 0.5 - iabs, sx0, sx1, sx2);
+  tsc(v.y * P.d_delx - 0.5 - jabs, sy0, sy1, sy2);
+  double f[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double *T = F + (k * MH + Frow) * MW + col;
+    f[k] = (T[0] * sx0 + T[1] * sx1 + T[2] * sx2) * sy0 +
+           (T[MW] * sx0 + T[MW + 1] * sx1 + T[MW + 2] * sx2) * sy1 +
+           (T[2 * MW] * sx0 + T[2 * MW + 1] * sx1 + T[2 * MW + 2] * sx2) * sy2;
+  }
+  const double bpx = f[0], bpy = f[1], bpz = f[2], epx = f[3], epy = f[4], epz = f[5];
+  double uvm1 = v.ux + fac1 * epx;
+  double uvm2 = v.uy + fac1 * epy;
+  double uvm3 = v.uz + fac1 * epz;
+  const double gam = sqrt(P.c * P.c + uvm1 * uvm1 + uvm2 * uvm2 + uvm3 * uvm3);
+  const double igam = 1.0 / gam;
+  const double fac1r = fac1 * igam;
+  const double fac2r = fac2 / (gam + txxx * (bpx * bpx + bpy * bpy + bpz * bpz) * igam);
+  const double uvm4 = uvm1 + fac1r * (uvm2 * bpz - uvm3 * bpy);
+  const double uvm5 = uvm2 + fac1r * (uvm3 * bpx - uvm1 * bpz);
+  const double uvm6 = uvm3 + fac1r * (uvm1 * bpy - uvm2 * bpx);
+  uvm1 = uvm1 + fac2r * (uvm5 * bpz - uvm6 * bpy);
+  uvm2 = uvm2 + fac2r * (uvm6 * bpx - uvm4 * bpz);
+  uvm3 = uvm3 + fac2r * (uvm4 * bpy - uvm5 * bpx);
+  ux = uvm1 + fac1 * epx;
+  uy = uvm2 + fac1 * epy;
+  uz = uvm3 + fac1 * epz;
+}
+
+// Workgroup (chunk, row block, species).  ACCL: update u over the moment step
+// first.  GP: write the updated records to the gp bank instead of summing
+// moments (mom_calc__accl on its own).
+template <class S, bool ACCL, bool GP>
+__global__ __launch_bounds__(256) void k_mom(MP P, FieldGeom fg, const double *__restrict__ uf,
+                                             S src, double *__restrict__ part, Rec64 gp,
+                                             int *__restrict__ derr) {
+  __shared__ double F[ACCL ? 6 * MH * MW : 1];
+  __shared__ double Nt[GP ? 1 : TSZ];
+  __shared__ double Cs[GP ? 1 : MC * CSTR];
+  const int chunk = blockIdx.x, rb = blockIdx.y, s = blockIdx.z;
+  const int c0 = chunk * MC, r0 = rb * MR;       // first cell, first row (local indices)
+  if (ACCL) {
+    // cell-centred fields, mom_calc.f90:68-81, of cells c0-1 .. c0+MC and
+    // rows r0-1 .. r0+MR; cell (a, b) is at (a+2, b+2) of a ghost-inclusive plane
+    for (int t = threadIdx.x; t < 6 * MH * MW; t += blockDim.x) {
+      const int col = t % MW, row = (t / MW) % MH, k = t / (MW * MH);
+      const int a = c0 - 1 + col, b = r0 - 1 + row;
+      double v = 0.0;
+      if (a <= P.nx && b <= P.ny) {
+        const size_t gx = fg.gx, o = (size_t)(a + 2) + gx * (size_t)(b + 2);
+        const double *U = uf + (size_t)k * fg.ps;
+        switch (k) {
+          case 0: v = 0.5 * (U[o] + U[o + gx]); break;
+          case 1: v = 0.5 * (U[o] + U[o + 1]); break;
+          case 2: v = 0.25 * (U[o] + U[o + 1] + U[o + gx] + U[o + gx + 1]); break;
+          case 3: v = 0.5 * (U[o] + U[o + 1]); break;
+          case 4: v = 0.5 * (U[o] + U[o + gx]); break;
+          default: v = U[o]; break;
+        }
+      }
+      F[t] = v;
+    }
+  }
+  if (!GP)
+    for (int t = threadIdx.x; t < TSZ; t += blockDim.x) Nt[t] = 0.0;
+  __syncthreads();
+  const int g = threadIdx.x / MG, sub = threadIdx.x % MG;
+  const int i = c0 + g;
+  const double q = P.q[s], rr = P.r[s];
+  const double fac1 = q / rr * 0.5 * P.delt;     // mom_calc.f90:90-92
+  const double txxx = fac1 * fac1;
+  const double fac2 = q * P.delt / rr;
+  const double ic2 = 1.0 / (P.c * P.c);
+  bool bad = false;
+  for (int rl = 0; rl < MR; ++rl) {
+    const int jl = r0 + rl;
+    if (jl >= P.ny) break;                       // the same for the whole workgroup
+    const int iabs = P.nxgs + i, jabs = P.nys + jl;
+    double acc[GP ? 1 : NCELL];
+    if (!GP) {
+#pragma unroll
+      for (int e = 0; e < NCELL; ++e) acc[e] = 0.0;
+    }
+    if (i < P.nx) {
+      size_t base;
+      int kb, ke;
+      src.range(P, s, jl, i, base, kb, ke);
+      for (int k = kb + sub; k < ke; k += MG) {
+        const size_t p = base + (size_t)k;
+        const Pt v = src.load(p, iabs, jabs);
+        double ux = v.ux, uy = v.uy, uz = v.uz;
+        if (ACCL) half_push(F, rl, g, v, iabs, jabs, P, fac1, txxx, fac2, ux, uy, uz);
+        if (GP) {
+          gp.x[p] = v.x;
+          gp.y[p] = v.y;
+          gp.ux[p] = ux;
+          gp.uy[p] = uy;
+          gp.uz[p] = uz;
+          continue;
+        }
+        // mom_calc.f90:190-243: ih = int(x - 1/2) (truncation), the cell or
+        // the one before it for a particle of this cell
+        const double xs = v.x * P.d_delx - 0.5, ys = v.y * P.d_delx - 0.5;
+        const int ih = (int)xs, jh = (int)ys;
+        const int di = ih - iabs, dj = jh - jabs;
+        if (di < -1 || di > 0 || dj < -1 || dj > 0) {
+          bad = true;
+          continue;
+        }
+        const double dx = xs - ih, dxm = 1.0 - dx, dy = ys - jh, dym = 1.0 - dy;
+        const double wx[3] = {di < 0 ? dxm : 0.0, di < 0 ? dx : dxm, di < 0 ? 0.0 : dx};
+        const double wy[3] = {dj < 0 ? dym : 0.0, dj < 0 ? dy : dym, dj < 0 ? 0.0 : dy};
+        const double ig = 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) * ic2);
+        const double vm[NMOM] = {1.0, ux * ig, uy * ig, uz * ig, ux * ux * ig, uy * uy * ig,
+                                 uz * uz * ig};
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int m = 0; m < NMOM; ++m)
+              acc[(b * 3 + a) * NMOM + m] += m == 0 ? wx[a] * wy[b] : vm[m] * wx[a] * wy[b];
+      }
+    }
+    if (GP) continue;
+#pragma unroll
+    for (int e = 0; e < NCELL; ++e) {
+#pragma unroll
+      for (int o = MG / 2; o > 0; o >>= 1) acc[e] += __shfl_xor(acc[e], o, MG);
+    }
+    if (sub == 0) {
+#pragma unroll
+      for (int e = 0; e < NCELL; ++e) Cs[g * CSTR + e] = acc[e];
+    }
+    __syncthreads();
+    // node (tile column c, node row j-1+b) takes cells c-2, c-1, c in order
+    for (int t = threadIdx.x; t < NMOM * 3 * MW; t += blockDim.x) {
+      const int c = t % MW, b = (t / MW) % 3, m = t / (3 * MW);
+      double *d = Nt + (m * MH + rl + b) * MW + c;
+      double sum = *d;
+      for (int gg = max(0, c - 2); gg <= min(MC - 1, c); ++gg)
+        sum += Cs[gg * CSTR + (b * 3 + (c - gg)) * NMOM + m];
+      *d = sum;
+    }
+    __syncthreads();
+  }
+  if (!GP) {
+    double *T = part + (((size_t)s * P.nrb + rb) * P.nchunk + chunk) * TSZ;
+    for (int t = threadIdx.x; t < TSZ; t += blockDim.x) T[t] = Nt[t];
+  }
+  if (bad) atomicOr(derr, DERR_RANGE);
+}
+
+// mom (7, nxgs-1:nxge+1, nys-1:nye+1, nsp) from the tiles: node (a, b)
+// (offsets from nxgs-1, nys-1) is column a - ch*MC of chunk ch and row
+// b - rb*MR of row block rb; the tiles add in ascending (rb, ch).
+__global__ void k_mom_gather(MP P, const double *__restrict__ part, double *__restrict__ mom) {
+  const int NXm = P.nx + 2, NYm = P.ny + 2;
+  const size_t n = (size_t)NMOM * NXm * NYm * P.nsp;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int m = (int)(t % NMOM);
+  size_t rest = t / NMOM;
+  const int a = (int)(rest % NXm);
+  rest /= NXm;
+  const int b = (int)(rest % NYm);
+  const int s = (int)(rest / NYm);
+  double sum = 0.0;
+  for (int rb = max(0, b / MR - 1); rb <= min(P.nrb - 1, b / MR); ++rb) {
+    const int tr = b - rb * MR;
+    if (tr >= MH) continue;
+    for (int ch = max(0, a / MC - 1); ch <= min(P.nchunk - 1, a / MC); ++ch) {
+      const int tc = a - ch * MC;
+      if (tc >= MW) continue;
+      sum += part[(((size_t)s * P.nrb + rb) * P.nchunk + ch) * TSZ + ((size_t)m * MH + tr) * MW + tc];
+    }
+  }
+  mom[t] = sum;
+}
+
+// boundary_periodic.f90:579-584: the x ghosts into the opposite edge, every row.
+__global__ void k_mom_fold_x(int nx, int ny, int nsp, double *mom) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= NMOM * (ny + 2) * nsp) return;
+  const int m = t % NMOM, row = t / NMOM;        // row = b + (ny+2) * s
+  double *r = mom + m + (size_t)NMOM * (nx + 2) * row;
+  r[NMOM * 1] += r[NMOM * (nx + 1)];
+  r[NMOM * nx] += r[0];
+}
+
+// boundary_periodic.f90:586-633 on one rank: row nys-1 into row nye, then
+// row nye+1 into row nys, every column.
+__global__ void k_mom_fold_y(int nx, int ny, int nsp, double *mom) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= NMOM * (nx + 2) * nsp) return;
+  const int e = t % (NMOM * (nx + 2)), s = t / (NMOM * (nx + 2));
+  const size_t rs = (size_t)NMOM * (nx + 2);
+  double *c = mom + e + rs * (ny + 2) * s;
+  c[rs * ny] += c[0];
+  c[rs * 1] += c[rs * (ny + 1)];
+}
+
+MP make_mp(const wm_ctx *ctx) {
+  const wm_grid &g = ctx->g;
+  MP P;
+  P.nx = g.nx;
+  P.ny = g.ny;
+  P.nxgs = g.nxgs;
+  P.nys = g.nys;
+  P.nsp = g.nsp;
+  P.nchunk = wm_cdiv(g.nx, MC);
+  P.nrb = wm_cdiv(g.ny, MR);
+  P.cap = ctx->cap;
+  P.CB = ctx->CB;
+  P.d_delx = 1. / g.delx;
+  P.c = g.c;
+  P.delt = g.delt * 0.5;
+  for (int s = 0; s < WM_MAXSP; ++s) {
+    P.q[s] = s < g.nsp ? g.q[s] : 0.0;
+    P.r[s] = s < g.nsp ? g.r[s] : 1.0;
+  }
+  return P;
+}
+
+template <class S, bool ACCL, bool GP>
+int launch_mom(wm_ctx *ctx, const MP &P, const S &src, Rec64 gp) {
+  hipLaunchKernelGGL((k_mom<S, ACCL, GP>), dim3(P.nchunk, P.nrb, P.nsp), dim3(MC * MG), 0,
+                     ctx->stream, P, ctx->fg, ctx->uf, src, ctx->mom_part, gp, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+}  // namespace
+
+size_t wm_mom_elems(const wm_ctx *ctx) {
+  return (size_t)NMOM * (ctx->g.nx + 2) * (ctx->g.ny + 2) * ctx->g.nsp;
+}
+
+int wm_mom_buffers(wm_ctx *ctx) {
+  const MP P = make_mp(ctx);
+  const size_t np = (size_t)P.nsp * P.nrb * P.nchunk * TSZ;
+  if (!ctx->mom) WM_HIP(hipMalloc((void **)&ctx->mom, sizeof(double) * wm_mom_elems(ctx)));
+  if (!ctx->mom_part) WM_HIP(hipMalloc((void **)&ctx->mom_part, sizeof(double) * np));
+  return WM_OK;
+}
+
+// Into ctx->mom, allocated here on first use (read the pointer only after).
+int wm_k_moments(wm_ctx *ctx, bool accl) {
+  WM_TRY(wm_mom_buffers(ctx));
+  const MP P = make_mp(ctx);
+  if (ctx->form == WM_FORM_FP32) {
+    if (!accl) return WM_ERR_ARG;
+    const Src32 src{ctx->bins.q, ctx->bins.uz, ctx->bin_cnt};
+    WM_TRY((launch_mom<Src32, true, false>(ctx, P, src, Rec64{})));
+  } else {
+    const PBank &A = ctx->bank[0];
+    const Src64 src{A.x, A.y, A.ux, A.uy, A.uz, ctx->cumcnt};
+    if (accl)
+      WM_TRY((launch_mom<Src64, true, false>(ctx, P, src, Rec64{})));
+    else
+      WM_TRY((launch_mom<Src64, false, false>(ctx, P, src, Rec64{})));
+  }
+  const size_t n = wm_mom_elems(ctx);
+  hipLaunchKernelGGL(k_mom_gather, dim3(wm_cdiv((long long)n, 256)), dim3(256), 0, ctx->stream, P,
+                     ctx->mom_part, ctx->mom);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_accl(wm_ctx *ctx, int src_bank, int dst_bank) {
+  const MP P = make_mp(ctx);
+  const PBank &A = ctx->bank[src_bank];
+  const Src64 src{A.x, A.y, A.ux, A.uy, A.uz, ctx->cumcnt};
+  return launch_mom<Src64, true, true>(ctx, P, src, ctx->bank[dst_bank]);
+}
+
+int wm_k_bc_mom(wm_ctx *ctx, double *mom) {
+  const wm_grid &g = ctx->g;
+  const int nxr = NMOM * (g.ny + 2) * g.nsp, nyr = NMOM * (g.nx + 2) * g.nsp;
+  hipLaunchKernelGGL(k_mom_fold_x, dim3(wm_cdiv(nxr, 256)), dim3(256), 0, ctx->stream, g.nx, g.ny,
+                     g.nsp, mom);
+  WM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mom_fold_y, dim3(wm_cdiv(nyr, 256)), dim3(256), 0, ctx->stream, g.nx, g.ny,
+                     g.nsp, mom);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
