@@ -159,7 +159,9 @@ int wm_cg_iters(wm_ctx *ctx, int *iters3);
  * steps, from HIP events on the context's stream: [0] cell-centred fields,
  * [1] push+deposit, [2] current gather+fold, [3] field solve, [4] particle
  * boundaries, [5] sort.  In the FP32 form the row migration and the sort
- * are one pass, reported in [5] ([4] is empty).  Returns the number written. */
+ * are one pass, reported in [5] ([4] is empty).  [6] and [7] hold the last
+ * wm_moments call's fused moment pass and its fold.  Returns the number
+ * written. */
 int wm_phase_times(wm_ctx *ctx, double *ms, int n);
 /* Total active particles (sum of np2). */
 int wm_particle_count(wm_ctx *ctx, long long *count);

@@ -1,7 +1,8 @@
 """The reference's time loop, written in Fortran against the drop-in modules
-(particle, field, sort, boundary_periodic, mpi_set with the reference's
-module and procedure names and argument lists), run on the device and
-compared with the oracle over the same steps."""
+(particle, field, sort, boundary_periodic, mom_calc, mpi_set with the
+reference's module and procedure names and argument lists), run on the device
+and compared with the oracle over the same steps and the moments of an output
+step."""
 import os
 import subprocess
 
@@ -28,10 +29,10 @@ def _write(d, p, up, np2, cc, uf, nsteps):
             f.write(np.asfortranarray(a).tobytes(order="F"))
 
 
-def _read(d, up, np2, cc, uf):
+def _read(d, up, np2, cc, uf, mom):
     raw = open(os.path.join(d, "out.bin"), "rb").read()
     out, off = [], 0
-    for a in (up, np2, cc, uf):
+    for a in (up, np2, cc, uf, mom):
         n = a.nbytes
         out.append(np.frombuffer(raw[off:off + n], dtype=a.dtype).reshape(a.shape, order="F"))
         off += n
@@ -45,11 +46,19 @@ def test_fortran_weibel_loop_matches_oracle(tmp_path):
     _write(str(tmp_path), p, up, np2, cc, uf, nsteps)
     r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    fup, fnp2, fcc, fuf = _read(str(tmp_path), up, np2, cc, uf)
+    mom = np.zeros((7, p["nx"] + 2, p["ny"] + 2, p["nsp"]), order="F")
+    fup, fnp2, fcc, fuf, fmom = _read(str(tmp_path), up, np2, cc, uf, mom)
     orc = Oracle(p)
     gp = up.copy(order="F")
     for _ in range(nsteps):
         orc.step(up, gp, uf, cc, np2)
+    orc.mom_calc_accl(gp, up, uf, cc)            # proj/weibel/app.f90:119-121
+    orc.mom_calc_nvt(mom, gp, np2)
+    orc.bc_mom(mom)
+    for s in range(p["nsp"]):
+        for k in range(7):
+            scale = np.abs(mom[k, ..., s]).max()
+            assert np.abs(fmom[k, ..., s] - mom[k, ..., s]).max() <= 1e-11 * scale
     assert np.array_equal(fnp2, np2)
     assert np.array_equal(fcc, cc)
     for s in range(p["nsp"]):

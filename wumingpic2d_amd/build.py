@@ -54,7 +54,7 @@ FORTRAN = os.path.join(HERE, "fortran")
 FMOD = os.path.join(FORTRAN, "mod")
 FLIB = os.path.join(HERE, "libwmpic_f.so")
 FSRC = ["wmpic_capi.f90", "wm_mpi_set.f90", "wm_particle.f90", "wm_field.f90", "wm_sort.f90",
-        "wm_boundary_periodic.f90"]
+        "wm_boundary_periodic.f90", "wm_mom_calc.f90"]
 DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "weibel_loop.f90")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "weibel_loop.x")
 

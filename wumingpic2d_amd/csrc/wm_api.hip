@@ -986,9 +986,18 @@ int wm_moments(wm_ctx *ctx, double *mom) {
   WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
   if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
   if (ctx->form == WM_FORM_FP32) WM_TRY(wm_settle_movers(ctx));
+  hipEvent_t *e = ctx->ev + 20;
+  WM_HIP(hipEventRecord(e[0], ctx->stream));
   WM_TRY(wm_k_moments(ctx, true));
+  WM_HIP(hipEventRecord(e[1], ctx->stream));
   WM_TRY(wm_k_bc_mom(ctx, ctx->mom));
+  WM_HIP(hipEventRecord(e[2], ctx->stream));
   WM_TRY(wm_check_derr(ctx, "mom_calc__nvt"));
+  float ms = 0.f;
+  WM_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+  ctx->phase_ms[WM_NPHASE] = ms;
+  WM_HIP(hipEventElapsedTime(&ms, e[1], e[2]));
+  ctx->phase_ms[WM_NPHASE + 1] = ms;
   WM_HIP(hipMemcpyAsync(mom, ctx->mom, sizeof(double) * wm_mom_elems(ctx), hipMemcpyDeviceToHost,
                         ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
@@ -1003,7 +1012,7 @@ int wm_cg_iters(wm_ctx *ctx, int *iters3) {
 
 int wm_phase_times(wm_ctx *ctx, double *ms, int n) {
   if (!ctx || !ms) return WM_ERR_ARG;
-  const int m = std::min(n, (int)WM_NPHASE);
+  const int m = std::min(n, (int)WM_NPHASE + 2);
   for (int k = 0; k < m; ++k) ms[k] = ctx->phase_ms[k];
   return m;
 }

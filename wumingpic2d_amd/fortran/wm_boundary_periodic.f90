@@ -1,5 +1,5 @@
 ! Drop-in module `boundary_periodic` (the public interface of
-! common/boundary_periodic.f90:26-32, 61, 99, 251, 357, 511) for one rank
+! common/boundary_periodic.f90:26-32, 61, 99, 251, 357, 511, 571) for one rank
 ! holding the whole domain: the halo exchanges are device kernels.
 module boundary_periodic
   use iso_c_binding
@@ -12,6 +12,7 @@ module boundary_periodic
   public :: boundary_periodic__particle_y
   public :: boundary_periodic__curre
   public :: boundary_periodic__phi
+  public :: boundary_periodic__mom
 
   integer, save :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye
 
@@ -69,5 +70,11 @@ contains
     call wmpic_context()
     call wmpic_check(wm_bcp_phi(wm_ctx, phi, nxs, nxe, nys, nye, l))
   end subroutine boundary_periodic__phi
+
+  subroutine boundary_periodic__mom(mom)
+    real(8), intent(inout) :: mom(7,nxgs-1:nxge+1,nys-1:nye+1,nsp)
+    call wmpic_context()
+    call wmpic_check(wm_bcp_mom(wm_ctx, mom))
+  end subroutine boundary_periodic__mom
 
 end module boundary_periodic

@@ -15,6 +15,7 @@ module wmpic_capi
   public :: wm_sort_init, wm_sort_bucket, wm_bcp_init, wm_bcp_particle_x
   public :: wm_bcp_particle_y, wm_bcp_dfield, wm_bcp_curre, wm_bcp_phi
   public :: wm_mpi_set_init, wm_set_capacity, WM_BC_PERIODIC
+  public :: wm_mom_calc_init, wm_mom_calc_accl, wm_mom_calc_nvt, wm_bcp_mom
 
   integer(c_int), parameter :: WM_BC_PERIODIC = 0
   type(c_ptr), save :: wm_ctx = c_null_ptr
@@ -133,6 +134,35 @@ module wmpic_capi
        real(c_double) :: phi(*)
        integer(c_int), value :: nxs, nxe, nys, nye, l
      end function wm_bcp_phi
+     integer(c_int) function wm_mom_calc_init(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, &
+          nye, delx, delt, c, q, r) bind(C, name="wm_mom_calc_init")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye
+       real(c_double), value :: delx, delt, c
+       real(c_double), intent(in) :: q(*), r(*)
+     end function wm_mom_calc_init
+     integer(c_int) function wm_mom_calc_accl(ctx, gp, up, uf, cumcnt, nxs, nxe) &
+          bind(C, name="wm_mom_calc_accl")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: gp(*)
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: cumcnt(*)
+       integer(c_int), value :: nxs, nxe
+     end function wm_mom_calc_accl
+     integer(c_int) function wm_mom_calc_nvt(ctx, mom, up, np2) bind(C, name="wm_mom_calc_nvt")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: mom(*)
+       real(c_double), intent(in) :: up(*)
+       integer(c_int), intent(in) :: np2(*)
+     end function wm_mom_calc_nvt
+     integer(c_int) function wm_bcp_mom(ctx, mom) bind(C, name="wm_bcp_mom")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: mom(*)
+     end function wm_bcp_mom
   end interface
 
 contains
