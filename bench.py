@@ -42,6 +42,10 @@ WORKLOADS = {
 # mover list instead.
 PUSHDEP_BYTES_PER_PARTICLE = 80.0
 PUSHDEP_BYTES_PER_PARTICLE_FP32 = 40.0
+# The moment pass reads each particle once: FP64 x, y, u (40 B) or the FP32
+# bin slot (offsets and u, 20 B).
+MOM_BYTES_PER_PARTICLE = {"fp64": 40.0, "fp32": 20.0}
+MOM_REPEATS = 3
 
 
 def log(msg: str) -> None:
@@ -107,6 +111,7 @@ def make_context(wm, weibel, workload: str, form: str):
     ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
     ctx.field__init(*g, 0, 0, 0, 0, p["delx"], p["delt"], p["c"], p["q"], p["r"], p["gfac"])
     ctx.sort__init(*g)
+    ctx.mom_calc__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
     log(f"workload {workload} ({form}): {nx}x{ny}, {ppc} ppc/species, capacity {cap}/row")
     return ctx, p, cap
 
@@ -154,6 +159,14 @@ def run_leg(wm, weibel, args, form: str, solver: str, steps: int, warmup: int) -
             log(f"timed step {k + 1}/{steps} ({form}, {solver})")
     elapsed = time.perf_counter() - t0
     energy = ctx.energy()
+    # output-step moment diagnostics (mom_calc__accl + mom_calc__nvt +
+    # boundary_periodic__mom) on the final state, outside the timed steps
+    mom_ms = np.zeros(2)
+    for _ in range(MOM_REPEATS):
+        ctx.moments()
+        mom_ms += ctx.phase_times()[6:8]
+    mom_ms /= MOM_REPEATS
+    log(f"moments: {mom_ms[0]:.3f} ms fused pass + gather, {mom_ms[1]:.3f} ms fold")
     ctx.close()
     return {"form": form, "solver": solver, "particles": npart, "steps": steps,
             "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / steps,
@@ -163,7 +176,26 @@ def run_leg(wm, weibel, args, form: str, solver: str, steps: int, warmup: int) -
             "solve_ms_per_step": round(float(phase[3]) / steps, 4),
             "cg_iters_per_step": iters / steps,
             "fcg_iters_per_step": fcg_iters / steps, "plain_iters_per_step": plain_iters / steps,
-            "energy": [float(e) for e in energy], "pc_training": train, "p": p}
+            "energy": [float(e) for e in energy], "pc_training": train, "p": p,
+            "moments_ms": [float(v) for v in mom_ms]}
+
+
+def moments_line(leg: dict, form: str, nx: int, ny: int) -> dict:
+    """Device time and HBM rate of the output-step moment pass (k_mom over
+    the particles, then the tile gather; the fold apart).  Algorithmic bytes:
+    the particles once, the six field planes once, the per-tile node sums
+    written and read back, mom written."""
+    nsp = 2
+    tiles = nsp * -(-ny // 4) * -(-nx // 32) * 7 * 6 * 34 * 8   # 4 rows x 32 cells per tile
+    fields = 6 * (nx + 4) * (ny + 4) * 8
+    mom = 7 * (nx + 2) * (ny + 2) * nsp * 8
+    nbytes = leg["particles"] * MOM_BYTES_PER_PARTICLE[form] + fields + 2 * tiles + mom
+    ms, fold_ms = leg["moments_ms"]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "k_mom (fused half-step push + CIC moments) + k_mom_gather",
+            "ms": round(ms, 4), "fold_ms": round(fold_ms, 4), "bytes": nbytes,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
 def main() -> int:
@@ -253,6 +285,7 @@ def main() -> int:
         "phase_ms_per_step": head["phase_ms_per_step"],
         "energy_last": head["energy"],
         "roofline": roofline,
+        "moments": moments_line(head, args.form, nx, ny),
     }
     if not args.no_compare and not (args.form == "fp64" and solver == "cg"):
         ref = run_leg(wm, weibel, args, "fp64", "cg", args.steps, args.warmup)
@@ -263,6 +296,7 @@ def main() -> int:
             "solve_ms_per_step": ref["solve_ms_per_step"],
             "cg_iters_per_step": ref["cg_iters_per_step"],
             "phase_ms_per_step": ref["phase_ms_per_step"],
+            "moments": moments_line(ref, "fp64", nx, ny),
             "step_rate_ratio": round(ref["ms_per_step"] / head["ms_per_step"], 3),
             "solve_iters_ratio": round(ref["cg_iters_per_step"] / max(head["cg_iters_per_step"], 1e-9), 3),
             "energy_rel_diff_total": abs(sum(e1) - tot0) / tot0,

@@ -17,14 +17,17 @@
 namespace {
 
 constexpr int MC = 32;           // cells per workgroup along x
-constexpr int MR = 8;            // rows per workgroup
+constexpr int MR = 4;            // rows per workgroup
 constexpr int MG = 8;            // lanes per cell
 constexpr int MW = MC + 2;       // tile columns: cells / nodes i0-1 .. i0+MC
 constexpr int MH = MR + 2;       // tile rows: j0-1 .. j0+MR
 constexpr int NMOM = 7;          // N, Vx, Vy, Vz, Txx, Tyy, Tzz
-constexpr int NCELL = 9 * NMOM;  // one cell's sums: 3 x 3 nodes x 7 moments
-constexpr int CSTR = NCELL + 1;  // LDS stride of one cell's sums
+constexpr int NST = 6 + NMOM;    // staged per particle: 3 + 3 weights, 7 moment values
 constexpr int TSZ = NMOM * MH * MW;
+static_assert(9 * NMOM <= MG * NST, "a group's slots hold its cell's sums");
+
+// The first LDS slot of group g.
+__device__ inline double *grp_w(double *St, int g) { return St + g * MG * NST; }
 
 struct MP {
   int nx, ny, nxgs, nys, nsp, nchunk, nrb, cap, CB;
@@ -121,7 +124,9 @@ __global__ __launch_bounds__(256) void k_mom(MP P, FieldGeom fg, const double *_
                                              int *__restrict__ derr) {
   __shared__ double F[ACCL ? 6 * MH * MW : 1];
   __shared__ double Nt[GP ? 1 : TSZ];
-  __shared__ double Cs[GP ? 1 : MC * CSTR];
+  // per lane, one particle of a round: x weights, y weights, the 7 moment
+  // values; after a row, each group's slots hold its cell's 63 sums
+  __shared__ double St[GP ? 1 : MC * MG * NST];
   const int chunk = blockIdx.x, rb = blockIdx.y, s = blockIdx.z;
   const int c0 = chunk * MC, r0 = rb * MR;       // first cell, first row (local indices)
   if (ACCL) {
@@ -157,65 +162,98 @@ __global__ __launch_bounds__(256) void k_mom(MP P, FieldGeom fg, const double *_
   const double fac2 = q * P.delt / rr;
   const double ic2 = 1.0 / (P.c * P.c);
   bool bad = false;
+  double *my = St + (g * MG + sub) * NST;        // this lane's staging slot
+  const double *grp = St + g * MG * NST;         // the group's eight slots
   for (int rl = 0; rl < MR; ++rl) {
     const int jl = r0 + rl;
     if (jl >= P.ny) break;                       // the same for the whole workgroup
     const int iabs = P.nxgs + i, jabs = P.nys + jl;
-    double acc[GP ? 1 : NCELL];
-    if (!GP) {
-#pragma unroll
-      for (int e = 0; e < NCELL; ++e) acc[e] = 0.0;
-    }
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; // moment `sub` at the 3 x 3 nodes
     if (i < P.nx) {
       size_t base;
       int kb, ke;
       src.range(P, s, jl, i, base, kb, ke);
-      for (int k = kb + sub; k < ke; k += MG) {
-        const size_t p = base + (size_t)k;
-        const Pt v = src.load(p, iabs, jabs);
+      // rounds of eight particles, one per lane; the next round's particle
+      // is loaded while this one is computed
+      int k = kb + sub;
+      Pt nxt{0, 0, 0, 0, 0};
+      if (k < ke) nxt = src.load(base + (size_t)k, iabs, jabs);
+      for (int k0 = kb; k0 < ke; k0 += MG, k += MG) {
+        const Pt v = nxt;
+        const bool live = k < ke;
+        if (k + MG < ke) nxt = src.load(base + (size_t)(k + MG), iabs, jabs);
         double ux = v.ux, uy = v.uy, uz = v.uz;
-        if (ACCL) half_push(F, rl, g, v, iabs, jabs, P, fac1, txxx, fac2, ux, uy, uz);
+        if (ACCL && live) half_push(F, rl, g, v, iabs, jabs, P, fac1, txxx, fac2, ux, uy, uz);
         if (GP) {
-          gp.x[p] = v.x;
-          gp.y[p] = v.y;
-          gp.ux[p] = ux;
-          gp.uy[p] = uy;
-          gp.uz[p] = uz;
+          if (live) {
+            const size_t p = base + (size_t)k;
+            gp.x[p] = v.x;
+            gp.y[p] = v.y;
+            gp.ux[p] = ux;
+            gp.uy[p] = uy;
+            gp.uz[p] = uz;
+          }
           continue;
         }
-        // mom_calc.f90:190-243: ih = int(x - 1/2) (truncation), the cell or
+        // mom_calc.f90:190-201: ih = int(x - 1/2) (truncation), the cell or
         // the one before it for a particle of this cell
+        double w[NST];
+#pragma unroll
+        for (int e = 0; e < NST; ++e) w[e] = 0.0;
         const double xs = v.x * P.d_delx - 0.5, ys = v.y * P.d_delx - 0.5;
         const int ih = (int)xs, jh = (int)ys;
         const int di = ih - iabs, dj = jh - jabs;
-        if (di < -1 || di > 0 || dj < -1 || dj > 0) {
-          bad = true;
-          continue;
+        const bool inside = di >= -1 && di <= 0 && dj >= -1 && dj <= 0;
+        bad |= live && !inside;
+        if (live && inside) {
+          const double dx = xs - ih, dxm = 1.0 - dx, dy = ys - jh, dym = 1.0 - dy;
+          w[0] = di < 0 ? dxm : 0.0;             // x weights of nodes i-1, i, i+1
+          w[1] = di < 0 ? dx : dxm;
+          w[2] = di < 0 ? 0.0 : dx;
+          w[3] = dj < 0 ? dym : 0.0;             // y weights of nodes j-1, j, j+1
+          w[4] = dj < 0 ? dy : dym;
+          w[5] = dj < 0 ? 0.0 : dy;
+          const double ig = 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) * ic2);
+          w[6] = 1.0;                            // N, V, T: mom_calc.f90:203-243
+          w[7] = ux * ig;
+          w[8] = uy * ig;
+          w[9] = uz * ig;
+          w[10] = ux * ux * ig;
+          w[11] = uy * uy * ig;
+          w[12] = uz * uz * ig;
         }
-        const double dx = xs - ih, dxm = 1.0 - dx, dy = ys - jh, dym = 1.0 - dy;
-        const double wx[3] = {di < 0 ? dxm : 0.0, di < 0 ? dx : dxm, di < 0 ? 0.0 : dx};
-        const double wy[3] = {dj < 0 ? dym : 0.0, dj < 0 ? dy : dym, dj < 0 ? 0.0 : dy};
-        const double ig = 1.0 / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) * ic2);
-        const double vm[NMOM] = {1.0, ux * ig, uy * ig, uz * ig, ux * ux * ig, uy * uy * ig,
-                                 uz * uz * ig};
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
+        for (int e = 0; e < NST; ++e) my[e] = w[e];
+        // the group's eight lanes are in one wavefront, whose LDS accesses
+        // complete in order: the slots are written before any lane reads them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (sub < NMOM) {
+          // lane m adds moment m of the round's particles, in lane order:
+          // ((v_m x-weight) y-weight), the reference's rounding per term
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int m = 0; m < NMOM; ++m)
-              acc[(b * 3 + a) * NMOM + m] += m == 0 ? wx[a] * wy[b] : vm[m] * wx[a] * wy[b];
+          for (int j = 0; j < MG; ++j) {
+            const double *o = grp + j * NST;
+            const double vm = o[6 + sub];
+            const double t0 = vm * o[0], t1 = vm * o[1], t2 = vm * o[2];
+            const double y0 = o[3], y1 = o[4], y2 = o[5];
+            acc[0] += t0 * y0; acc[1] += t1 * y0; acc[2] += t2 * y0;
+            acc[3] += t0 * y1; acc[4] += t1 * y1; acc[5] += t2 * y1;
+            acc[6] += t0 * y2; acc[7] += t1 * y2; acc[8] += t2 * y2;
+          }
+        }
+        // every read of this round is issued before the next round writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
     if (GP) continue;
+    // the cell's sums into the group's own slots: [(b*3 + a)*NMOM + m]
+    if (sub < NMOM) {
 #pragma unroll
-    for (int e = 0; e < NCELL; ++e) {
-#pragma unroll
-      for (int o = MG / 2; o > 0; o >>= 1) acc[e] += __shfl_xor(acc[e], o, MG);
-    }
-    if (sub == 0) {
-#pragma unroll
-      for (int e = 0; e < NCELL; ++e) Cs[g * CSTR + e] = acc[e];
+      for (int e = 0; e < 9; ++e) grp_w(St, g)[e * NMOM + sub] = acc[e];
     }
     __syncthreads();
     // node (tile column c, node row j-1+b) takes cells c-2, c-1, c in order
@@ -224,7 +262,7 @@ __global__ __launch_bounds__(256) void k_mom(MP P, FieldGeom fg, const double *_
       double *d = Nt + (m * MH + rl + b) * MW + c;
       double sum = *d;
       for (int gg = max(0, c - 2); gg <= min(MC - 1, c); ++gg)
-        sum += Cs[gg * CSTR + (b * 3 + (c - gg)) * NMOM + m];
+        sum += St[gg * MG * NST + (b * 3 + (c - gg)) * NMOM + m];
       *d = sum;
     }
     __syncthreads();
