@@ -55,6 +55,7 @@ def log(msg: str) -> None:
 def cpu_baseline(seconds: float = 12.0) -> dict:
     """FP64 CPU restatement (the oracle) on a 256x256, 64 ppc/species sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
     from oracle import Oracle
     from wumingpic2d_amd import weibel
 
@@ -79,10 +80,24 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
+    # the output-step moment diagnostics on the same sample (serial C port)
+    mom = np.zeros((7, nx + 2, ny + 2, p["nsp"]), order="F")
+    reps, t1 = 0, time.perf_counter()
+    while reps < 5:
+        orc.mom_calc_accl(gp, up, uf, cc)
+        orc.mom_calc_nvt(mom, gp, np2)
+        orc.bc_mom(mom)
+        reps += 1
+        if time.perf_counter() - t1 > 4.0:
+            break
+    dm = time.perf_counter() - t1
     return {"value": npart * steps / dt, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"Weibel {nx}x{ny}, {ppc} ppc/species, 2 species ({npart} particles), "
-                      f"{steps} steps after 1 warm-up, FP64 C restatement with {threads} OpenMP threads"}
+                      f"{steps} steps after 1 warm-up, FP64 C restatement with {threads} OpenMP threads",
+            "moments": {"value": npart * reps / dm, "unit": "particles/s", "cores": 1,
+                        "sample": "mom_calc__accl + mom_calc__nvt + boundary_periodic__mom of the "
+                                  f"same state, {reps} times, serial C restatement"}}
 
 
 def load_traffic(workload: str, form: str):
