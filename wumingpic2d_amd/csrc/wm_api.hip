@@ -984,7 +984,11 @@ int wm_bcp_mom(wm_ctx *ctx, double *mom) {
 int wm_moments(wm_ctx *ctx, double *mom) {
   if (!ctx || !mom) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
-  if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
+  if (!ctx->alloc_done) {
+    wm_set_error(ctx, WM_ERR_NOT_INIT, "moments: no device-resident state (wm_state_upload or "
+                                       "wm_state_init_weibel first)");
+    return WM_ERR_NOT_INIT;
+  }
   if (ctx->form == WM_FORM_FP32) WM_TRY(wm_settle_movers(ctx));
   hipEvent_t *e = ctx->ev + 20;
   WM_HIP(hipEventRecord(e[0], ctx->stream));
