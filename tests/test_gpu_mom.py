@@ -81,6 +81,23 @@ def test_mom_calc_nvt_matches_oracle(nx, ny, ppc, seed):
     _mom_close(m2, m1)
 
 
+def test_mom_calc_nvt_with_first_cell_at_zero():
+    """int(x - 1/2) truncates to 0 below x = 1/2: the weights extrapolate."""
+    p, up, np2, cc, uf = state(24, 6, 9, 14, vth=0.3)
+    p = dict(p, nxgs=0, nxge=p["nx"] - 1, nygs=0, nyge=p["ny"] - 1, nys=0, nye=p["ny"] - 1)
+    up = up.copy(order="F")
+    up[0] -= 2.0
+    up[1] -= 2.0
+    assert (up[0, :np2[0, 0], :, 0] < 0.5).any()
+    m1 = np.zeros((7, p["nx"] + 2, p["ny"] + 2, p["nsp"]), order="F")
+    Oracle(p).mom_calc_nvt(m1, up, np2)
+    ctx = make_ctx(p, init=("sort",))
+    _mom_init(ctx, p)
+    m2 = np.full_like(m1, np.nan)
+    ctx.mom_calc__nvt(m2, up, np2)
+    _mom_close(m2, m1)
+
+
 @pytest.mark.parametrize("nx,ny", [(10, 7), (4, 1), (33, 2)])
 def test_bcp_mom_bitexact(nx, ny):
     p, up, np2, cc, uf = state(nx, ny, 1, 7)
@@ -107,6 +124,8 @@ def test_resident_moments_fp64_match_oracle(nx, ny, ppc, seed, steps):
     want = _oracle_moments(p, gup, guf, gcc, gnp2)
     got = ctx.moments()
     _mom_close(got, want)
+    # one order for every sum: a second call gives the same bits
+    assert np.array_equal(bits(ctx.moments()), bits(got))
     # N sums to the particle count over the interior after the fold
     for s in range(p["nsp"]):
         assert abs(got[0, 1:-1, 1:-1, s].sum() - gnp2[:, s].sum()) < 1e-9 * gnp2[:, s].sum()
