@@ -157,6 +157,22 @@ def test_resident_moments_fp32_match_oracle(nx, ny, ppc, seed, steps):
     _mom_close(got, want)
 
 
+@pytest.mark.parametrize("nsp", [1, 3])
+def test_moments_other_species_counts(nsp):
+    """One and three species: the species index of the launch grid and of mom."""
+    p, up, np2, cc, uf = state(24, 7, 6, 15 + nsp, vth=0.2, nsp=nsp)
+    random_fields(uf, nsp, 0.1)
+    ctx = make_ctx(p)
+    _mom_init(ctx, p)
+    ctx.state_upload(up, np2, cc, uf)
+    _mom_close(ctx.moments(), _oracle_moments(p, up, uf, cc, np2))
+    m1 = np.zeros((7, p["nx"] + 2, p["ny"] + 2, nsp), order="F")
+    Oracle(p).mom_calc_nvt(m1, up, np2)
+    m2 = np.full_like(m1, np.nan)
+    ctx.mom_calc__nvt(m2, up, np2)
+    _mom_close(m2, m1)
+
+
 def test_nvt_particle_outside_its_row_is_reported():
     p, up, np2, cc, uf = state(16, 4, 3, 12)
     up[1, 0, 1, 0] += 2.0                       # two rows above the row that holds it
