@@ -5,13 +5,14 @@
 // (boundary_periodic.f90:571-636).
 //
 // One workgroup takes MC cells x MR rows of one species.  Eight lanes share a
-// cell; each lane sums its particles' contributions to the 3 x 3 nodes around
-// the cell (x - 1/2 truncates to the cell or the one before it), the eight
-// lanes add theirs in a fixed butterfly, and the cell sums go into the
-// tile's node sums in LDS in ascending cell order.  The tiles overlap by two
-// node columns and two node rows; a gather adds the (at most four) tiles of a
-// node in a fixed order.  Every sum has one order: the result does not depend
-// on scheduling.
+// cell and take its particles in rounds of eight, one per lane: each lane
+// stages its particle's weights at the 3 x 3 nodes around the cell (x - 1/2
+// truncates to the cell or the one before it) and its 7 moment values in
+// LDS, then lane m adds moment m of the round's particles, in lane order, to
+// its 9 node sums.  The cell sums go into the tile's node sums in LDS in
+// ascending cell order.  The tiles overlap by two node columns and two node
+// rows; a gather adds the (at most four) tiles of a node in a fixed order.
+// Every sum has one order: the result does not depend on scheduling.
 #include "wm_internal.h"
 
 namespace {
