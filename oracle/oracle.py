@@ -50,10 +50,11 @@ class OracleError(RuntimeError):
 class Oracle:
     """FP64 restatement; one instance = one grid and its SAVE state."""
 
-    def __init__(self, p: dict, threads: int = 1):
-        if not os.path.exists(ORACLE_LIB):
-            raise OracleError(f"{ORACLE_LIB} not built (run make -C oracle)")
-        self.lib = ctypes.CDLL(ORACLE_LIB)
+    def __init__(self, p: dict, threads: int = 1, lib: str | None = None):
+        path = lib or ORACLE_LIB
+        if not os.path.exists(path):
+            raise OracleError(f"{path} not built (run make -C oracle)")
+        self.lib = ctypes.CDLL(path)
         L = self.lib
         vp = ctypes.c_void_p
         L.or_state_new.restype = vp
@@ -70,7 +71,8 @@ class Oracle:
         L.or_set_threads.argtypes = [ctypes.c_int]
         for n, k in (("or_particle_solv", 5), ("or_ele_cur", 5), ("or_bc_curre", 2),
                      ("or_bc_dfield", 2), ("or_bc_phi", 2), ("or_field_rhs", 4),
-                     ("or_field_fdtd_i", 6), ("or_bc_particle_x", 3), ("or_bc_particle_y", 4),
+                     ("or_field_fdtd_i", 6), ("or_field_advance", 4), ("or_bc_particle_x", 3),
+                     ("or_bc_particle_y", 4),
                      ("or_sort_bucket", 5), ("or_energy", 5), ("or_step", 7),
                      ("or_mom_calc_accl", 5), ("or_mom_calc_nvt", 4), ("or_bc_mom", 2)):
             getattr(L, n).argtypes = [vp] * k
@@ -129,6 +131,11 @@ class Oracle:
     def field__fdtd_i(self, uf, up, gp, cumcnt):
         self._ck(self.lib.or_field_fdtd_i(self.pp, ctypes.c_void_p(self.state), _p(uf), _p(up),
                                           _p(gp), _p(cumcnt)), "field__fdtd_i")
+
+    def field_advance(self, uf, uj):
+        """field.f90:125-184 from a folded current uj (ghosts filled)."""
+        self._ck(self.lib.or_field_advance(self.pp, ctypes.c_void_p(self.state), _p(uf), _p(uj)),
+                 "field_advance")
 
     def particle_x(self, up, np2):
         self._ck(self.lib.or_bc_particle_x(self.pp, _p(up), _p(np2)), "particle_x")

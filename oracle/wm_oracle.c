@@ -647,19 +647,22 @@ int or_cgm(const or_params *p, double *db, const double *gkl, double err,
 /* -------------------------------------------------------- field__fdtd_i */
 int or_field_fdtd_i(const or_params *p, or_state *s, double *uf,
                     const double *up, const double *gp, const int *cumcnt) {
+  int st = or_ele_cur(p, s->uj, up, gp, cumcnt);   /* :121 */
+  if (st) return st;
+  or_bc_curre(p, s->uj);                           /* :122 */
+  return or_field_advance(p, s, uf, s->uj);        /* :125-184 */
+}
+
+int or_field_advance(const or_params *p, or_state *s, double *uf, const double *uj) {
   const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
   double f1, f2, f3, f4, f5;
   field_consts(p, &f1, &f2, &f3, &f4, &f5);
   const double gfac = p->gfac, delt = p->delt;
-  int st = or_ele_cur(p, s->uj, up, gp, cumcnt);   /* :121 */
-  if (st) return st;
-  or_bc_curre(p, s->uj);                           /* :122 */
-  or_field_rhs(p, s->gkl, uf, s->uj);              /* :125-146 */
-  st = or_cgm(p, s->df, s->gkl, s->cg_err, s->iters); /* :149 */
+  or_field_rhs(p, s->gkl, uf, uj);                 /* :125-146 */
+  int st = or_cgm(p, s->df, s->gkl, s->cg_err, s->iters); /* :149 */
   if (st) return st;
   or_bc_dfield(p, s->df);                          /* :151 */
   double *df = s->df;
-  const double *uj = s->uj;
 #define UF(k, i, j) uf[iuf(p, k, i, j)]
 #define DF(k, i, j) df[iuf(p, k, i, j)]
 #define UJ(k, i, j) uj[iuj(p, k, i, j)]

@@ -23,15 +23,23 @@
  *   or_sort_bucket     bit-exact vs the amdflang build of common/sort.f90 and
  *                      golden vectors; brute-force stable argsort.
  *   or_ele_cur         discrete charge continuity (rho from the shape at old and
- *                      new positions), sum Jx = q dx/dt * displacement, Jz closed
- *                      form for a single particle, partition of unity.
+ *                      new positions), sum Jx = q dx/dt * displacement, Jz of a
+ *                      resting particle, and Jz of moving (also cell-crossing)
+ *                      particles against the time integral of the linearly
+ *                      interpolated TSC shapes, by Gauss-Legendre quadrature
+ *                      (test_oracle.py, test_oracle_field.py).
  *   or_cgm             converged solution solves the 5-point Helmholtz system;
  *                      Fourier-mode exact solution; eigenvector RHS takes exactly
  *                      two iterations (stop test on the pre-update residual);
  *                      |b| < 1e-6 skips the loop (squared initial sum_g).
- *   or_field_fdtd_i    Gauss law stays at roundoff (div of the curl terms
- *                      vanishes, deposit continuity); vacuum plane wave speed;
- *                      div B = 0 preserved.
+ *   or_field_advance   (and or_field_fdtd_i) single Fourier modes of fields and
+ *                      current against the implicit Maxwell scheme solved in
+ *                      Fourier space; TE and TM vacuum eigenmodes against the
+ *                      closed-form amplification factor of the theta scheme;
+ *                      uniform J gives dE = -4 pi dt J and no dB; div B stays at
+ *                      roundoff over 20 whole steps; Gauss law stays at
+ *                      roundoff over 6 whole steps (test_oracle_field.py,
+ *                      test_oracle.py).
  *   bc_* (halo/fold)   brute-force periodic index maps; fold conserves the sum.
  *   or_bc_particle_x   round-down wrap never lands on the upper bound.
  *   or_bc_particle_y   multiset and row membership invariants; hole-filling
@@ -104,9 +112,13 @@ void or_field_rhs(const or_params *p, double *gkl, const double *uf,
  * component; returns OR_ERR_CG_ITEMAX when ite reaches ite_max. */
 int or_cgm(const or_params *p, double *db, const double *gkl, double err,
            int *iters3);
-/* field__fdtd_i (field.f90:66-186) using the periodic BC family */
+/* field__fdtd_i (field.f90:66-186) using the periodic BC family:
+ * ele_cur, bc_curre, then or_field_advance. */
 int or_field_fdtd_i(const or_params *p, or_state *s, double *uf,
                     const double *up, const double *gp, const int *cumcnt);
+/* field.f90:125-184 given the folded current uj (ghosts filled as after
+ * boundary_periodic__curre): gkl, cgm, dE, both dfield halos, uf += df. */
+int or_field_advance(const or_params *p, or_state *s, double *uf, const double *uj);
 /* boundary_periodic__particle_x (boundary_periodic.f90:61-96) */
 int or_bc_particle_x(const or_params *p, double *up, const int *np2);
 /* boundary_periodic__particle_y (boundary_periodic.f90:99-248) */
