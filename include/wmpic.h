@@ -24,6 +24,11 @@
  *    written to host memory, persistent state (the CG warm start of
  *    field__fdtd_i) stays on the device.  The device-resident entry points
  *    (wm_state_*, wm_step) keep particles and fields in HBM across steps.
+ *    The host-array entry points that stage particles or fields
+ *    (particle_solv, field_fdtd_i, bcp_particle_x/y, sort_bucket,
+ *    mom_calc_accl/nvt) use the buffers a device-resident state lives in: on
+ *    a context that holds one (after wm_state_upload or wm_state_init_weibel)
+ *    they return WM_ERR_ARG and change nothing.
  *
  * Citations name the reference routine each entry point replaces
  * (files of WumingCode/WumingPIC2D).

@@ -108,3 +108,15 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_undersized_arrays_are_refused_before_any_device_call():
+    """The C side reads and writes the reference's full shapes: an array
+    smaller than its grid shape is a ValueError on the host."""
+    from wumingpic2d_amd import _ptr
+    a = np.zeros((6, 10, 4, 2), order="F")
+    assert _ptr(a, np.float64, "up", a.size) == a.ctypes.data
+    with pytest.raises(ValueError):
+        _ptr(a, np.float64, "up", a.size + 1)
+    with pytest.raises(TypeError):
+        _ptr(np.zeros((6, 10), order="C"), np.float64, "up", 60)

@@ -178,3 +178,43 @@ def test_fp32_host_array_entry_points_refuse():
     with pytest.raises(wm.WmError) as e:
         ctx.particle__solv(np.zeros_like(up), up, uf, cc, p["nxgs"], p["nxge"])
     assert e.value.code == wm.WM_ERR_ARG
+
+
+def test_fp32_learned_pc_flexible_cg_tracks_fp64_oracle():
+    """The headline form: FP32 particles, and from the fourth step on the
+    field solve by flexible CG with the learned preconditioner (trained on
+    the device on pairs the first three plain solves recorded), against the
+    FP64 oracle's plain CG at the FP32 tolerances above."""
+    p, up, np2, cc, uf = state(64, 32, 16, 49, vth=0.1)
+    ctx = _ctx32(p)
+    ctx.state_upload(up, np2, cc, uf)
+    ctx.pc_collect(8, 2)
+    orc = Oracle(p)
+    gp = up.copy(order="F")
+    for _ in range(3):
+        orc.step(up, gp, uf, cc, np2)
+        ctx.step(1)
+    assert ctx.pc_pair_count() > 0
+    loss = ctx.pc_train(200, 3e-3, 1)
+    assert loss[1] < loss[0]
+    ctx.pc_collect(0, 0)
+    ctx.set_solver(wm.SOLVER_FCG)
+    fcg = plain = 0
+    for _ in range(5):
+        orc.step(up, gp, uf, cc, np2)
+        ctx.step(1)
+        a, b = ctx.solve_stats()
+        fcg += int(a.sum())
+        plain += int(b.sum())
+    assert fcg > 0, "flexible CG did not run"
+    gup, gnp2, guf = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+    ctx.state_download(up=gup, np2=gnp2, uf=guf)
+    assert gnp2.sum() == np2.sum()
+    a, b = _by_id(up, np2), _by_id(gup, gnp2)
+    assert a.keys() == b.keys()
+    assert max(np.abs(a[k][:2] - b[k][:2]).max() for k in a) < 2e-5
+    assert max(np.abs(a[k][2:] - b[k][2:]).max() for k in a) < 2e-5
+    assert np.abs(guf - uf).max() < 1e-3 * np.abs(uf).max()
+    e1, e2 = orc.energy(up, uf, np2), ctx.energy()
+    assert np.allclose(e1[:2], e2[:2], rtol=1e-5)
+    assert np.allclose(e1[2:], e2[2:], rtol=1e-2, atol=1e-9 * e1[:2].sum())

@@ -170,8 +170,53 @@ def test_moments_other_species_counts(nsp):
     m1 = np.zeros((7, p["nx"] + 2, p["ny"] + 2, nsp), order="F")
     Oracle(p).mom_calc_nvt(m1, up, np2)
     m2 = np.full_like(m1, np.nan)
-    ctx.mom_calc__nvt(m2, up, np2)
+    host = make_ctx(p)                   # the host-array form on a context of its own
+    _mom_init(host, p)
+    host.mom_calc__nvt(m2, up, np2)
     _mom_close(m2, m1)
+
+
+def test_host_array_calls_refuse_on_a_resident_context():
+    """The host-array entry points stage their arguments through the buffers
+    of the device-resident state: on a context holding one they refuse and
+    change nothing, so the steps after the refused calls are bit-identical
+    to a run without them."""
+    p, up, np2, cc, uf = state(32, 8, 8, 17, vth=0.2)
+    outs = []
+    for mix in (False, True):
+        ctx = make_ctx(p)
+        _mom_init(ctx, p)
+        ctx.state_upload(up, np2, cc, uf)
+        ctx.step(2)
+        if mix:
+            mom = np.zeros((7, p["nx"] + 2, p["ny"] + 2, p["nsp"]), order="F")
+            calls = [lambda: ctx.mom_calc__nvt(mom, up, np2),
+                     lambda: ctx.mom_calc__accl(np.zeros_like(up), up, uf, cc, p["nxgs"], p["nxge"]),
+                     lambda: ctx.particle__solv(np.zeros_like(up), up, uf, cc, p["nxgs"], p["nxge"]),
+                     lambda: ctx.field__fdtd_i(uf.copy(order="F"), up, up, cc, p["nxgs"], p["nxge"]),
+                     lambda: ctx.sort__bucket(np.zeros_like(up), up, cc.copy(order="F"), np2,
+                                              p["nxgs"], p["nxge"]),
+                     lambda: ctx.boundary_periodic__particle_x(up.copy(order="F"), np2),
+                     lambda: ctx.boundary_periodic__particle_y(up.copy(order="F"),
+                                                               np2.copy(order="F"))]
+            for call in calls:
+                with pytest.raises(wm.WmError) as e:
+                    call()
+                assert e.value.code == wm.WM_ERR_ARG
+                assert "device-resident state" in str(e.value)
+        ctx.step(3)
+        o, n, f = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+        ctx.state_download(up=o, np2=n, uf=f)
+        outs.append((o, n, f, ctx.moments()))
+        ctx.close()
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][2], outs[1][2])
+    assert np.array_equal(outs[0][3], outs[1][3])
+    for s in range(p["nsp"]):
+        for jl in range(p["ny"]):
+            k = outs[0][1][jl, s]
+            assert np.array_equal(outs[0][0][:, :k, jl, s].view(np.int64),
+                                  outs[1][0][:, :k, jl, s].view(np.int64))
 
 
 def test_nvt_particle_outside_its_row_is_reported():

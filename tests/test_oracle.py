@@ -20,6 +20,7 @@ import pytest
 from oracle import OR_ERR_CG_ITEMAX, Oracle, RefBuild, ref_available
 from wumingpic2d_amd import weibel
 from wmhelpers import bits, random_fields, state
+from wmhelpers import rho_tsc as _rho, tsc_weights as _w
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ref_particle_sort.npz")
 
@@ -199,27 +200,6 @@ def test_magnetic_gather_reproduces_linear_fields(comp):
 
 
 # ------------------------------------------------ deposit: invariants
-def _rho(p, up, np2):
-    """Charge on cells from the TSC shape of ele_cur (field.f90:224-236):
-    q * s0x * s0y around the particle's cell, periodic wrap."""
-    nx, ny = p["nx"], p["ny"]
-    rho = np.zeros((nx, ny))
-    for s in range(p["nsp"]):
-        for jl in range(ny):
-            n = np2[jl, s]
-            x, y = up[0, :n, jl, s], up[1, :n, jl, s]
-            i, j = x.astype(int), y.astype(int)
-            for sx, dxo in zip(_w(x - 0.5 - i), (-1, 0, 1)):
-                for sy, dyo in zip(_w(y - 0.5 - j), (-1, 0, 1)):
-                    np.add.at(rho, ((i + dxo - p["nxgs"]) % nx, (j + dyo - p["nys"]) % ny),
-                              p["q"][s] * sx * sy)
-    return rho
-
-
-def _w(dh):
-    return (0.5 * (0.5 - dh) ** 2, 0.75 - dh * dh, 0.5 * (0.5 + dh) ** 2)
-
-
 def _fold(p, uj3):
     """Periodic sum of a ghost-inclusive (nx+4, ny+4) array onto the interior."""
     nx, ny = p["nx"], p["ny"]

@@ -11,8 +11,10 @@ def grid_args(p):
             p["nye"])
 
 
-def make_ctx(p, cap=0, mcap=0, init=("particle", "field", "sort", "bc")):
+def make_ctx(p, cap=0, mcap=0, init=("particle", "field", "sort", "bc"), form=None):
     ctx = wm.Context(0)
+    if form is not None:
+        ctx.set_form(form)
     if cap or mcap:
         ctx.set_capacity(cap, mcap)
     g = grid_args(p)
@@ -45,3 +47,32 @@ def bits(a):
 
 def active(up, np2, jl, s, ndim=5):
     return up[:ndim, :np2[jl, s], jl, s]
+
+
+def rho_tsc(p, up, np2):
+    """Charge on cells from the TSC shape of ele_cur (field.f90:224-236):
+    q * s0x * s0y around the particle's cell, periodic wrap."""
+    nx, ny = p["nx"], p["ny"]
+    rho = np.zeros((nx, ny))
+    for s in range(p["nsp"]):
+        for jl in range(ny):
+            n = np2[jl, s]
+            x, y = up[0, :n, jl, s], up[1, :n, jl, s]
+            i, j = x.astype(int), y.astype(int)
+            for sx, dxo in zip(tsc_weights(x - 0.5 - i), (-1, 0, 1)):
+                for sy, dyo in zip(tsc_weights(y - 0.5 - j), (-1, 0, 1)):
+                    np.add.at(rho, ((i + dxo - p["nxgs"]) % nx, (j + dyo - p["nys"]) % ny),
+                              p["q"][s] * sx * sy)
+    return rho
+
+
+def tsc_weights(dh):
+    return (0.5 * (0.5 - dh) ** 2, 0.75 - dh * dh, 0.5 * (0.5 + dh) ** 2)
+
+
+def gauss_residual(p, uf, up, np2):
+    """div+ E - 4 pi rho at the cell centres (interior)."""
+    ex = uf[3, 2:-2, 2:-2]
+    ey = uf[4, 2:-2, 2:-2]
+    div = (np.roll(ex, -1, 0) - ex) + (np.roll(ey, -1, 1) - ey)
+    return div - 4 * np.pi * rho_tsc(p, up, np2)

@@ -155,13 +155,17 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.wm_last_error.restype = ctypes.c_char_p
 
 
-def _ptr(a: np.ndarray | None, dtype, name: str):
+def _ptr(a: np.ndarray | None, dtype, name: str, n: int | None = None):
+    """Address of a Fortran-ordered array of at least n elements (the C side
+    reads and writes the reference's full shapes)."""
     if a is None:
         return None
     if not isinstance(a, np.ndarray) or a.dtype != np.dtype(dtype):
         raise TypeError(f"{name} must be a numpy array of {np.dtype(dtype)}")
     if not a.flags.f_contiguous:
         raise TypeError(f"{name} must be Fortran-contiguous (the reference's column-major layout)")
+    if n is not None and a.size < n:
+        raise ValueError(f"{name} holds {a.size} elements; the grid needs {n}")
     return a.ctypes.data
 
 
@@ -227,6 +231,20 @@ class Context:
     def set_cg_tolerance(self, err: float) -> None:
         self._check(self._lib.wm_set_cg_tolerance(self._h, float(err)))
 
+    def _n(self, kind: str) -> int | None:
+        """Elements of the reference array `kind` on this context's grid."""
+        g = self.grid
+        if g is None:
+            return None
+        nx, ny, nsp = g["nx"], g["ny"], g["nsp"]
+        return {"up": g["ndim"] * g["np"] * ny * nsp, "uf": 6 * (nx + 4) * (ny + 4),
+                "uj": 3 * (nx + 4) * (ny + 4), "cumcnt": (nx + 1) * ny * nsp, "np2": ny * nsp,
+                "phi": (nx + 2) * (ny + 2), "mom": 7 * (nx + 2) * (ny + 2) * nsp}[kind]
+
+    def _a(self, a, kind: str, name: str | None = None):
+        dtype = np.int32 if kind in ("cumcnt", "np2") else np.float64
+        return _ptr(a, dtype, name or kind, self._n(kind))
+
     def _remember(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye):
         self.grid = dict(ndim=ndim, np=np_, nsp=nsp, nxgs=nxgs, nxge=nxge, nygs=nygs,
                          nyge=nyge, nys=nys, nye=nye, nx=nxge - nxgs + 1, ny=nye - nys + 1)
@@ -274,70 +292,70 @@ class Context:
 
     # ------------------------------------------- host-array entry points
     def particle__solv(self, gp, up, uf, cumcnt, nxs, nxe) -> None:
-        self._check(self._lib.wm_particle_solv(self._h, _ptr(gp, np.float64, "gp"),
-                                               _ptr(up, np.float64, "up"),
-                                               _ptr(uf, np.float64, "uf"),
-                                               _ptr(cumcnt, np.int32, "cumcnt"), nxs, nxe))
+        self._check(self._lib.wm_particle_solv(self._h, self._a(gp, "up", "gp"),
+                                               self._a(up, "up"),
+                                               self._a(uf, "uf"),
+                                               self._a(cumcnt, "cumcnt"), nxs, nxe))
 
     def field__fdtd_i(self, uf, up, gp, cumcnt, nxs, nxe, bc_family: int = BC_PERIODIC) -> None:
-        self._check(self._lib.wm_field_fdtd_i(self._h, _ptr(uf, np.float64, "uf"),
-                                              _ptr(up, np.float64, "up"),
-                                              _ptr(gp, np.float64, "gp"),
-                                              _ptr(cumcnt, np.int32, "cumcnt"), nxs, nxe,
+        self._check(self._lib.wm_field_fdtd_i(self._h, self._a(uf, "uf"),
+                                              self._a(up, "up"),
+                                              self._a(gp, "up", "gp"),
+                                              self._a(cumcnt, "cumcnt"), nxs, nxe,
                                               bc_family))
 
     def boundary_periodic__particle_x(self, up, np2) -> None:
-        self._check(self._lib.wm_bcp_particle_x(self._h, _ptr(up, np.float64, "up"),
-                                                _ptr(np2, np.int32, "np2")))
+        self._check(self._lib.wm_bcp_particle_x(self._h, self._a(up, "up"),
+                                                self._a(np2, "np2")))
 
     def boundary_periodic__particle_y(self, up, np2) -> None:
-        self._check(self._lib.wm_bcp_particle_y(self._h, _ptr(up, np.float64, "up"),
-                                                _ptr(np2, np.int32, "np2")))
+        self._check(self._lib.wm_bcp_particle_y(self._h, self._a(up, "up"),
+                                                self._a(np2, "np2")))
 
     def sort__bucket(self, gp, up, cumcnt, np2, nxs, nxe) -> None:
-        self._check(self._lib.wm_sort_bucket(self._h, _ptr(gp, np.float64, "gp"),
-                                             _ptr(up, np.float64, "up"),
-                                             _ptr(cumcnt, np.int32, "cumcnt"),
-                                             _ptr(np2, np.int32, "np2"), nxs, nxe))
+        self._check(self._lib.wm_sort_bucket(self._h, self._a(gp, "up", "gp"),
+                                             self._a(up, "up"),
+                                             self._a(cumcnt, "cumcnt"),
+                                             self._a(np2, "np2"), nxs, nxe))
 
     def boundary_periodic__dfield(self, df, nxs, nxe, nys, nye, nxgs, nxge) -> None:
-        self._check(self._lib.wm_bcp_dfield(self._h, _ptr(df, np.float64, "df"), nxs, nxe, nys,
+        self._check(self._lib.wm_bcp_dfield(self._h, self._a(df, "uf", "df"), nxs, nxe, nys,
                                             nye, nxgs, nxge))
 
     def boundary_periodic__curre(self, uj, nxs, nxe, nys, nye, nxgs, nxge) -> None:
-        self._check(self._lib.wm_bcp_curre(self._h, _ptr(uj, np.float64, "uj"), nxs, nxe, nys,
+        self._check(self._lib.wm_bcp_curre(self._h, self._a(uj, "uj"), nxs, nxe, nys,
                                            nye, nxgs, nxge))
 
     def boundary_periodic__phi(self, phi, nxs, nxe, nys, nye, l) -> None:
-        self._check(self._lib.wm_bcp_phi(self._h, _ptr(phi, np.float64, "phi"), nxs, nxe, nys,
+        self._check(self._lib.wm_bcp_phi(self._h, self._a(phi, "phi"), nxs, nxe, nys,
                                          nye, l))
 
     def mom_calc__accl(self, gp, up, uf, cumcnt, nxs, nxe) -> None:
-        self._check(self._lib.wm_mom_calc_accl(self._h, _ptr(gp, np.float64, "gp"),
-                                               _ptr(up, np.float64, "up"),
-                                               _ptr(uf, np.float64, "uf"),
-                                               _ptr(cumcnt, np.int32, "cumcnt"), nxs, nxe))
+        self._check(self._lib.wm_mom_calc_accl(self._h, self._a(gp, "up", "gp"),
+                                               self._a(up, "up"),
+                                               self._a(uf, "uf"),
+                                               self._a(cumcnt, "cumcnt"), nxs, nxe))
 
     def mom_calc__nvt(self, mom, up, np2) -> None:
-        self._check(self._lib.wm_mom_calc_nvt(self._h, _ptr(mom, np.float64, "mom"),
-                                              _ptr(up, np.float64, "up"),
-                                              _ptr(np2, np.int32, "np2")))
+        self._check(self._lib.wm_mom_calc_nvt(self._h, self._a(mom, "mom"),
+                                              self._a(up, "up"),
+                                              self._a(np2, "np2")))
 
     def boundary_periodic__mom(self, mom) -> None:
-        self._check(self._lib.wm_bcp_mom(self._h, _ptr(mom, np.float64, "mom")))
+        self._check(self._lib.wm_bcp_mom(self._h, self._a(mom, "mom")))
 
     # ------------------------------------------------- resident run
     def state_upload(self, up, np2, cumcnt, uf) -> None:
-        self._check(self._lib.wm_state_upload(self._h, _ptr(up, np.float64, "up"),
-                                              _ptr(np2, np.int32, "np2"),
-                                              _ptr(cumcnt, np.int32, "cumcnt"),
-                                              _ptr(uf, np.float64, "uf")))
+        self._check(self._lib.wm_state_upload(self._h, self._a(up, "up"),
+                                              self._a(np2, "np2"),
+                                              self._a(cumcnt, "cumcnt"),
+                                              self._a(uf, "uf")))
 
     def state_download(self, up=None, np2=None, cumcnt=None, uf=None) -> None:
-        self._check(self._lib.wm_state_download(self._h, _ptr(up, np.float64, "up"),
-                                                _ptr(np2, np.int32, "np2"),
-                                                _ptr(cumcnt, np.int32, "cumcnt"),
-                                                _ptr(uf, np.float64, "uf")))
+        self._check(self._lib.wm_state_download(self._h, self._a(up, "up"),
+                                                self._a(np2, "np2"),
+                                                self._a(cumcnt, "cumcnt"),
+                                                self._a(uf, "uf")))
 
     def init_weibel(self, seed: int, ppc: int, vth: float, udrift: float) -> None:
         self._check(self._lib.wm_state_init_weibel(self._h, int(seed), int(ppc), float(vth),

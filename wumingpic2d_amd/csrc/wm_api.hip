@@ -343,6 +343,19 @@ int require_fp64(wm_ctx *ctx, const char *what) {
   return WM_OK;
 }
 
+// The host-array entry points stage their arguments through the particle
+// banks, fields, np2 and cumcnt that a device-resident state lives in.
+int require_no_resident(wm_ctx *ctx, const char *what) {
+  if (ctx->resident) {
+    wm_set_error(ctx, WM_ERR_ARG,
+                 "%s (host arrays) would overwrite this context's device-resident state; call it "
+                 "on a context without one",
+                 what);
+    return WM_ERR_ARG;
+  }
+  return WM_OK;
+}
+
 int check_bounds(wm_ctx *ctx, int nxs, int nxe) {
   if (nxs != ctx->g.nxgs || nxe != ctx->g.nxge) {
     wm_set_error(ctx, WM_ERR_ARG, "nxs:nxe = %d:%d must equal nxgs:nxge = %d:%d on one rank", nxs,
@@ -643,6 +656,7 @@ int wm_particle_solv(wm_ctx *ctx, double *gp, const double *up, const double *uf
   WM_TRY(require(ctx, ctx->pinit, "particle__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   WM_TRY(require_fp64(ctx, "particle__solv"));
+  WM_TRY(require_no_resident(ctx, "particle__solv"));
   WM_TRY(wm_alloc_device(ctx));
   const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
   WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
@@ -671,6 +685,7 @@ int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   WM_TRY(require_fp64(ctx, "field__fdtd_i"));
+  WM_TRY(require_no_resident(ctx, "field__fdtd_i"));
   WM_TRY(wm_alloc_device(ctx));
   const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
   WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
@@ -690,6 +705,7 @@ int wm_bcp_particle_x(wm_ctx *ctx, double *up, const int *np2) {
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
   WM_TRY(require_fp64(ctx, "boundary_periodic__particle_x"));
+  WM_TRY(require_no_resident(ctx, "boundary_periodic__particle_x"));
   WM_TRY(wm_alloc_device(ctx));
   WM_TRY(upload_particles(ctx, up, np2, 0));
   WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg_of(ctx), hipMemcpyHostToDevice,
@@ -703,6 +719,7 @@ int wm_bcp_particle_y(wm_ctx *ctx, double *up, int *np2) {
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
   WM_TRY(require_fp64(ctx, "boundary_periodic__particle_y"));
+  WM_TRY(require_no_resident(ctx, "boundary_periodic__particle_y"));
   WM_TRY(wm_alloc_device(ctx));
   WM_TRY(upload_particles(ctx, up, np2, 0));
   const int nseg = nseg_of(ctx);
@@ -729,6 +746,7 @@ int wm_sort_bucket(wm_ctx *ctx, double *gp, const double *up, int *cumcnt, const
   WM_TRY(require(ctx, ctx->sinit, "sort__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   WM_TRY(require_fp64(ctx, "sort__bucket"));
+  WM_TRY(require_no_resident(ctx, "sort__bucket"));
   WM_TRY(wm_alloc_device(ctx));
   const int nseg = nseg_of(ctx);
   WM_TRY(upload_particles(ctx, up, np2, 1));
@@ -798,6 +816,7 @@ int wm_state_upload(wm_ctx *ctx, const double *up, const int *np2, const int *cu
   WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
   WM_HIP(hipStreamSynchronize(ctx->stream));
   if (f32) WM_TRY(wm_check_derr(ctx, "state upload"));
+  ctx->resident = true;
   return WM_OK;
 }
 
@@ -833,6 +852,7 @@ int wm_state_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double
   WM_TRY(ctx->form == WM_FORM_FP32 ? wm_k_dense_to_bins(ctx) : wm_k_sort(ctx, 1, 0));
   WM_TRY(fill_zero(ctx, ctx->uf, 6 * ctx->fg.ps));
   WM_TRY(wm_check_derr(ctx, "initial state"));
+  ctx->resident = true;
   return WM_OK;
 }
 
@@ -938,6 +958,7 @@ int wm_mom_calc_accl(wm_ctx *ctx, double *gp, const double *up, const double *uf
   WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   WM_TRY(require_fp64(ctx, "mom_calc__accl"));
+  WM_TRY(require_no_resident(ctx, "mom_calc__accl"));
   WM_TRY(wm_alloc_device(ctx));
   const std::vector<int> cnt = counts_from_cumcnt(ctx, cumcnt);
   WM_TRY(upload_particles(ctx, up, cnt.data(), 0));
@@ -953,6 +974,7 @@ int wm_mom_calc_nvt(wm_ctx *ctx, double *mom, const double *up, const int *np2) 
   if (!ctx || !mom) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->minit, "mom_calc__init"));
   WM_TRY(require_fp64(ctx, "mom_calc__nvt"));
+  WM_TRY(require_no_resident(ctx, "mom_calc__nvt"));
   WM_TRY(wm_alloc_device(ctx));
   // the rows' particles, in any order, are binned by cell first (the sums
   // then run per cell; the reference sums in row order)

@@ -129,6 +129,10 @@ struct wm_ctx {
   int cap = 0;             // device slots per (species, row)
   int mcap = 0;            // migrating particles per row and direction
   bool alloc_done = false;
+  // a device-resident state exists (wm_state_upload / wm_state_init_weibel):
+  // the host-array entry points, which stage their arguments through the
+  // same particle banks, fields and counts, refuse from then on
+  bool resident = false;
 
   // particles: FP64 form (bank) or FP32 form (b32), chosen before allocation
   int form = WM_FORM_FP64;
