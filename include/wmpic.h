@@ -71,9 +71,11 @@ typedef struct wm_ctx wm_ctx;
 int wm_create(wm_ctx **ctx, int device);
 int wm_destroy(wm_ctx *ctx);
 const char *wm_last_error(const wm_ctx *ctx);
-/* Device capacity per (species, row) and migration buffer per row and
- * direction.  The reference sizes rows by np (proj/weibel/app.f90:248);
- * the device may use less.  0 selects defaults (np, np/8). */
+/* Device capacity per (species, row) and the migration buffer.  The
+ * reference sizes rows by np (proj/weibel/app.f90:248); the device may use
+ * less.  mover_cap: FP64 form, migrating particles per row and direction;
+ * FP32 form, mover records per row (all species) per push.  0 selects
+ * defaults (np; np/8 or a quarter of a row's slots). */
 int wm_set_capacity(wm_ctx *ctx, int cap_per_row, int mover_cap);
 /* CG relative tolerance; the reference fixes 1e-6 (field.f90:339). */
 int wm_set_cg_tolerance(wm_ctx *ctx, double err);
@@ -163,13 +165,23 @@ int wm_cg_iters(wm_ctx *ctx, int *iters3);
 /* GPU time (ms) of each phase of the last wm_step call, summed over its
  * steps, from HIP events on the context's stream: [0] cell-centred fields,
  * [1] push+deposit, [2] current gather+fold, [3] field solve, [4] particle
- * boundaries, [5] sort.  In the FP32 form the row migration and the sort
- * are one pass, reported in [5] ([4] is empty).  [6] and [7] hold the last
- * wm_moments call's fused moment pass and its fold.  Returns the number
- * written. */
+ * boundaries, [5] sort.  In the FP32 form the push places the last push's
+ * movers (the particle boundaries and the sort reduce to that); [4] is the
+ * bins' fill check beside the solve and [5] the re-layouts of rows whose
+ * bins would overflow.  [6] and [7] hold the last wm_moments call's fused
+ * moment pass and its fold.  Returns the number written. */
 int wm_phase_times(wm_ctx *ctx, double *ms, int n);
 /* Total active particles (sum of np2). */
 int wm_particle_count(wm_ctx *ctx, long long *count);
+/* FP32 form: the particle store's capacities and their use since the state
+ * was made (no reference counterpart; the reference stops when a row
+ * overflows np, boundary_periodic.f90:228-236).  out[0] most particles in
+ * one cell bin, [1] most particles leaving one bin in a push, [2] most
+ * movers of one push workgroup, [3] most mover records of one row in a
+ * push, [4] rows laid out anew (a bin would have overflowed), [5] slots per
+ * (species, row) segment, [6] mover records per row, [7] movers one push
+ * workgroup can list.  Returns the number written. */
+int wm_capacity_stats(wm_ctx *ctx, long long *out, int n);
 
 /* ---- moment diagnostics of an output step (proj/weibel/app.f90:119-121)
  *      mom (7, nxgs-1:nxge+1, nys-1:nye+1, nsp) real(8): N, Vx, Vy, Vz, Txx,

@@ -141,35 +141,128 @@ def test_fp32_settling_between_steps_changes_nothing():
                                   outs[1][0][:, :k, jl, s].view(np.int64))
 
 
-def test_fp32_bin_overflow_is_reported():
-    """A bin at 30 of its 32 slots receives 10 fast particles from its left
-    neighbour: the migration (in the next push's prologue) stops at the
-    bin's capacity, writes nothing past it, and the step reports the
-    overflow with the reference's 'memory over' error."""
-    from wumingpic2d_amd import weibel
-    p = weibel.grid_params(16, 8, 4)                 # bins of 32 slots
+def _one_row_state(p, rows):
+    """Particles of species 0 only: rows = {jl: [(x, y, ux, uy), ...]}."""
     ndim, npc, ny, nsp = p["ndim"], p["np"], p["ny"], p["nsp"]
     up = np.zeros((ndim, npc, ny, nsp), order="F")
     np2 = np.zeros((ny, nsp), dtype=np.int32, order="F")
     cc = np.zeros((p["nx"] + 1, ny, nsp), dtype=np.int32, order="F")
-    jl, s = 3, 0
-    x = np.concatenate([np.full(10, p["nxgs"] + 4 + 0.95), p["nxgs"] + 5 + np.linspace(0.1, 0.6, 30)])
-    ux = np.concatenate([np.full(10, 0.9), np.zeros(30)])
-    n = x.size
-    up[0, :n, jl, s] = x
-    up[1, :n, jl, s] = p["nys"] + jl + 0.5
-    up[2, :n, jl, s] = ux
-    up[5, :n, jl, s] = (-(np.arange(n, dtype=np.int64) + 1)).view(np.float64)
-    np2[jl, s] = n
-    cnt = np.bincount((x - p["nxgs"]).astype(np.int64), minlength=p["nx"])
-    cc[1:, jl, s] = np.cumsum(cnt)
-    uf = np.zeros((6, p["nx"] + 4, ny + 4), order="F")
+    gid = 0
+    for jl, parts in rows.items():
+        parts = sorted(parts, key=lambda t: int(t[0]))
+        n = len(parts)
+        a = np.array(parts, dtype=np.float64).reshape(n, 4)
+        up[0, :n, jl, 0], up[1, :n, jl, 0], up[2, :n, jl, 0], up[3, :n, jl, 0] = a.T
+        up[5, :n, jl, 0] = (-(np.arange(gid, gid + n, dtype=np.int64) + 1)).view(np.float64)
+        gid += n
+        np2[jl, 0] = n
+        cnt = np.bincount((a[:, 0] - p["nxgs"]).astype(np.int64), minlength=p["nx"])
+        cc[1:, jl, 0] = np.cumsum(cnt)
+    return up, np2, cc, np.zeros((6, p["nx"] + 4, ny + 4), order="F")
+
+
+def test_fp32_full_bin_is_laid_out_anew():
+    """Row segments of 160 slots leave no room beyond each bin's need (16
+    cells of at least 8 slots; 30 particles in cell 5 get 32).  Ten fast
+    particles of cell 4 arrive in cell 5: the fill check after the push
+    sees 40 > 32, and the row is laid out anew (placing the movers) before
+    the next push.  The state then matches the FP64 oracle's."""
+    from wumingpic2d_amd import weibel
+    p = weibel.grid_params(16, 8, 4)
+    jl, y = 3, p["nys"] + 3 + 0.5
+    row = [(p["nxgs"] + 4.95, y, 0.9, 0.0)] * 10 + \
+          [(p["nxgs"] + 5 + f, y, 0.0, 0.0) for f in np.linspace(0.1, 0.6, 30)]
+    up, np2, cc, uf = _one_row_state(p, {jl: row})
     ctx = _ctx32(p)
+    ctx.set_capacity(160)
     ctx.state_upload(up, np2, cc, uf)
-    ctx.step(1)                                      # the 10 leave their cell
+    orc = Oracle(p)
+    gp = up.copy(order="F")
+    for _ in range(3):
+        ctx.step(1)
+        orc.step(up, gp, uf, cc, np2)
+    st = ctx.capacity_stats()
+    assert st["relayouts"] >= 1 and st["max_bin_fill"] >= 40, st
+    gup, gnp2, guf = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+    ctx.state_download(up=gup, np2=gnp2, uf=guf)
+    assert gnp2.sum() == np2.sum() == 40
+    a, b = _by_id(up, np2), _by_id(gup, gnp2)
+    assert a.keys() == b.keys()
+    assert max(np.abs(a[k] - b[k]).max() for k in a) < 2e-5
+    assert np.abs(guf - uf).max() < 1e-3 * np.abs(uf).max()
+
+
+def test_fp32_row_overflow_is_reported():
+    """A row of 160 slots holding 30 particles in cell 5 and 5 in cell 6
+    (need 14 x 8 + 32 + 8 = 152) receives 12 particles from the row below
+    into cell 5: it would need 168 slots, more than its segment holds, and
+    the step stops with the reference's 'memory over' (its rows overflow np
+    the same way, boundary_periodic.f90:228-236).  A row too full for its
+    segment is refused at upload with the same error."""
+    from wumingpic2d_amd import weibel
+    p = weibel.grid_params(16, 8, 4)
+    x5, x6 = p["nxgs"] + 5, p["nxgs"] + 6
+    y3, y2 = p["nys"] + 3, p["nys"] + 2
+    rows = {3: [(x5 + f, y3 + 0.5, 0.0, 0.0) for f in np.linspace(0.1, 0.6, 30)] +
+               [(x6 + 0.5, y3 + 0.5, 0.0, 0.0)] * 5,
+            2: [(x5 + 0.5, y2 + 0.95, 0.0, 0.9)] * 12}
+    up, np2, cc, uf = _one_row_state(p, rows)
+    ctx = _ctx32(p)
+    ctx.set_capacity(160)
+    ctx.state_upload(up, np2, cc, uf)
     with pytest.raises(wm.WmError) as e:
-        ctx.step(1)                                  # and arrive: 40 > 32
+        ctx.step(1)
     assert e.value.code == wm.WM_ERR_NP_OVER
+    ctx.close()
+    full = {3: [(x5 + f, y3 + 0.5, 0.0, 0.0) for f in np.linspace(0.1, 0.6, 42)] +
+               [(x6 + 0.5, y3 + 0.5, 0.0, 0.0)] * 5}
+    up, np2, cc, uf = _one_row_state(p, full)
+    ctx = _ctx32(p)
+    ctx.set_capacity(160)
+    with pytest.raises(wm.WmError) as e:
+        ctx.state_upload(up, np2, cc, uf)
+    assert e.value.code == wm.WM_ERR_NP_OVER
+
+
+def test_fp32_tight_rows_relayout_and_track_oracle():
+    """Row segments of 1.375 times a row's 512 particles: bins of about 16
+    particles take 20 slots on average once rounded up to 8-slot blocks, so
+    each bin has a few slots of room, a hot plasma (u_th = 0.2 c) fills bins
+    every step and rows are laid out anew again and again.  The trajectory
+    still tracks the FP64 oracle at the FP32 bars, and two runs are
+    bit-identical."""
+    p, up0, np20, cc0, uf0 = state(32, 16, 16, 50, vth=0.2)
+    cap = 704
+    outs = []
+    for rep in range(2):
+        up, np2, cc, uf = (a.copy(order="F") for a in (up0, np20, cc0, uf0))
+        ctx = _ctx32(p)
+        ctx.set_capacity(cap)
+        ctx.state_upload(up, np2, cc, uf)
+        ctx.step(6)
+        st = ctx.capacity_stats()
+        assert st["relayouts"] > 0, st
+        o, n, f = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+        ctx.state_download(up=o, np2=n, uf=f)
+        outs.append((o, n, f))
+        ctx.close()
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+    for s in range(2):
+        for jl in range(p["ny"]):
+            k = outs[0][1][jl, s]
+            assert np.array_equal(outs[0][0][:, :k, jl, s].view(np.int64),
+                                  outs[1][0][:, :k, jl, s].view(np.int64))
+    up, np2, cc, uf = (a.copy(order="F") for a in (up0, np20, cc0, uf0))
+    orc = Oracle(p)
+    gp = up.copy(order="F")
+    for _ in range(6):
+        orc.step(up, gp, uf, cc, np2)
+    gup, gnp2, guf = outs[0]
+    assert gnp2.sum() == np2.sum()
+    a, b = _by_id(up, np2), _by_id(gup, gnp2)
+    assert a.keys() == b.keys()
+    assert max(np.abs(a[k] - b[k]).max() for k in a) < 2e-5
+    assert np.abs(guf - uf).max() < 1e-3 * np.abs(uf).max()
 
 
 def test_fp32_host_array_entry_points_refuse():

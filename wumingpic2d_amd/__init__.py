@@ -65,7 +65,7 @@ EXPORTS = (
     "wm_particle_solv", "wm_field_fdtd_i", "wm_bcp_particle_x", "wm_bcp_particle_y",
     "wm_sort_bucket", "wm_bcp_dfield", "wm_bcp_curre", "wm_bcp_phi", "wm_state_upload",
     "wm_state_download", "wm_state_init_weibel", "wm_step", "wm_energy", "wm_cg_iters",
-    "wm_phase_times", "wm_particle_count",
+    "wm_phase_times", "wm_particle_count", "wm_capacity_stats",
     "wm_set_solver", "wm_pc_collect", "wm_pc_pair_count", "wm_pc_init", "wm_pc_train",
     "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_grad", "wm_solve_stats",
     "wm_mom_calc_init", "wm_mom_calc_accl", "wm_mom_calc_nvt", "wm_bcp_mom", "wm_moments",
@@ -131,6 +131,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_cg_iters": [_P, _P],
         "wm_phase_times": [_P, _P, _I],
         "wm_particle_count": [_P, ctypes.POINTER(ctypes.c_longlong)],
+        "wm_capacity_stats": [_P, _P, _I],
         "wm_set_solver": [_P, _I],
         "wm_pc_collect": [_P, _I, _I],
         "wm_pc_pair_count": [_P],
@@ -392,6 +393,17 @@ class Context:
         c = ctypes.c_longlong()
         self._check(self._lib.wm_particle_count(self._h, ctypes.byref(c)))
         return c.value
+
+    def capacity_stats(self) -> dict:
+        """FP32 form: the particle store's capacities and their use since the
+        state was made (maxima over the pushes)."""
+        out = np.zeros(8, dtype=np.int64)
+        n = self._lib.wm_capacity_stats(self._h, out.ctypes.data, 8)
+        if n != 8:                      # an error status (8 entries on success)
+            self._check(n)
+        keys = ("max_bin_fill", "max_bin_movers", "max_workgroup_movers", "max_row_records",
+                "relayouts", "slots_per_row", "records_per_row", "workgroup_mover_list")
+        return {k: int(v) for k, v in zip(keys, out[:n])}
 
     # ------------------------------------------ learned preconditioner
     def set_solver(self, solver: int) -> None:

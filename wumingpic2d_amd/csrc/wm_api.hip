@@ -155,24 +155,6 @@ int alloc_bank(wm_ctx *ctx, PBank &b, size_t n) {
   WM_TRY(dmalloc(ctx, &b.uz, n));
   return WM_OK;
 }
-int alloc_rec32(wm_ctx *ctx, Rec32 &b, size_t n) {
-  WM_TRY(dmalloc(ctx, &b.ci, n));
-  WM_TRY(dmalloc(ctx, &b.ox, n));
-  WM_TRY(dmalloc(ctx, &b.oy, n));
-  WM_TRY(dmalloc(ctx, &b.ux, n));
-  WM_TRY(dmalloc(ctx, &b.uy, n));
-  WM_TRY(dmalloc(ctx, &b.uz, n));
-  return WM_OK;
-}
-void free_rec32(Rec32 &b) {
-  (void)hipFree(b.ci);
-  (void)hipFree(b.ox);
-  (void)hipFree(b.oy);
-  (void)hipFree(b.ux);
-  (void)hipFree(b.uy);
-  (void)hipFree(b.uz);
-  b = Rec32{};
-}
 void free_bank(PBank &b) {
   (void)hipFree(b.x);
   (void)hipFree(b.y);
@@ -229,13 +211,9 @@ int upload_particles(wm_ctx *ctx, const double *up, const int *cnt_host, int b) 
   WM_HIP(hipMemcpyAsync(ctx->stage, h.data(), sizeof(double) * elems, hipMemcpyHostToDevice,
                         ctx->stream));
   WM_HIP(hipMemcpyAsync(dcnt, cnt_host, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  if (ctx->form == WM_FORM_FP32) {
-    WM_TRY(wm_k_aos_rec32(ctx, true, w, dcnt, ctx->stage, b));
-  } else {
-    hipLaunchKernelGGL(k_aos_to_soa, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
-                       ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
-                       ctx->id[ctx->idcur]);
-  }
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
+                     ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
+                     ctx->id[ctx->idcur]);
   WM_HIP(hipGetLastError());
   WM_HIP(hipStreamSynchronize(ctx->stream));
   return WM_OK;
@@ -256,13 +234,9 @@ int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b, int 
   WM_TRY(ensure_stage(ctx, elems));
   int *dcnt = ctx->cnt_scratch;
   WM_HIP(hipMemcpyAsync(dcnt, cnt_host, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  if (ctx->form == WM_FORM_FP32) {
-    WM_TRY(wm_k_aos_rec32(ctx, false, w, dcnt, ctx->stage, b));
-  } else {
-    hipLaunchKernelGGL(k_soa_to_aos, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
-                       ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
-                       ctx->id[ctx->idcur]);
-  }
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(std::min(64, wm_cdiv(w, 256)), nseg), dim3(256), 0,
+                     ctx->stream, g.ndim, w, ctx->cap, dcnt, ctx->stage, ctx->bank[b],
+                     ctx->id[ctx->idcur]);
   WM_HIP(hipGetLastError());
   std::vector<double> h(elems);
   WM_HIP(hipMemcpyAsync(h.data(), ctx->stage, sizeof(double) * elems, hipMemcpyDeviceToHost,
@@ -278,6 +252,92 @@ int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b, int 
   for (int s = 0; s < nseg; ++s)
     memcpy(up + (size_t)s * g.np * g.ndim, &h[(size_t)s * w * g.ndim],
            sizeof(double) * (size_t)cnt_host[s] * g.ndim);
+  return WM_OK;
+}
+
+// FP32 form: the reference's rows <-> cell bins, through the staging buffer
+// in batches of segments of at most WM_STAGE_ELEMS doubles (at least one
+// segment), so that no dense copy of the state is kept on the device.
+#define WM_STAGE_ELEMS ((size_t)1 << 26)
+int batch_end(const wm_ctx *ctx, const int *cnt, int s0, int &w) {
+  const int nseg = nseg_of(ctx);
+  w = std::max(1, cnt[s0]);
+  int s1 = s0 + 1;
+  while (s1 < nseg) {
+    const int w1 = std::max(w, cnt[s1]);
+    if ((size_t)(s1 + 1 - s0) * w1 * ctx->g.ndim > WM_STAGE_ELEMS) break;
+    w = w1;
+    ++s1;
+  }
+  return s1;
+}
+
+int upload_bins(wm_ctx *ctx, const double *up, const int *cnt_host) {
+  const wm_grid &g = ctx->g;
+  const int nseg = nseg_of(ctx);
+  for (int s = 0; s < nseg; ++s)
+    if (cnt_host[s] < 0 || cnt_host[s] > g.np) {
+      wm_set_error(ctx, WM_ERR_RANGE, "particle count %d of row segment %d outside 0..np=%d",
+                   cnt_host[s], s, g.np);
+      return WM_ERR_RANGE;
+    }
+  for (int s0 = 0; s0 < nseg;) {
+    int w = 1;
+    const int s1 = batch_end(ctx, cnt_host, s0, w);
+    const size_t elems = (size_t)(s1 - s0) * w * g.ndim;
+    WM_TRY(ensure_stage(ctx, elems));
+    std::vector<double> h(elems);
+    for (int s = s0; s < s1; ++s)
+      memcpy(&h[(size_t)(s - s0) * w * g.ndim], up + (size_t)s * g.np * g.ndim,
+             sizeof(double) * (size_t)cnt_host[s] * g.ndim);
+    WM_HIP(hipMemcpyAsync(ctx->stage, h.data(), sizeof(double) * elems, hipMemcpyHostToDevice,
+                          ctx->stream));
+    WM_HIP(hipMemcpyAsync(ctx->cnt_scratch, cnt_host + s0, sizeof(int) * (s1 - s0),
+                          hipMemcpyHostToDevice, ctx->stream));
+    WM_TRY(wm_k_stage_to_bins(ctx, s0, s1 - s0, w));
+    WM_HIP(hipStreamSynchronize(ctx->stream));
+    s0 = s1;
+  }
+  return WM_OK;
+}
+
+// The bins' particles (row counts cnt_host, as k_bin_rows left them) into
+// the reference-layout host array up, sorted by cell within each row.
+int download_bins(wm_ctx *ctx, double *up, const int *cnt_host) {
+  const wm_grid &g = ctx->g;
+  const int nseg = nseg_of(ctx);
+  for (int s = 0; s < nseg; ++s)
+    if (cnt_host[s] > g.np) {
+      wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np): %d > %d", cnt_host[s], g.np);
+      return WM_ERR_NP_OVER;
+    }
+  for (int s0 = 0; s0 < nseg;) {
+    int w = 1;
+    const int s1 = batch_end(ctx, cnt_host, s0, w);
+    const size_t elems = (size_t)(s1 - s0) * w * g.ndim;
+    WM_TRY(ensure_stage(ctx, elems));
+    WM_TRY(wm_k_bins_to_stage(ctx, s0, s1 - s0, w));
+    std::vector<double> h(elems);
+    WM_HIP(hipMemcpyAsync(h.data(), ctx->stage, sizeof(double) * elems, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    WM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = s0; s < s1; ++s)
+      memcpy(up + (size_t)s * g.np * g.ndim, &h[(size_t)(s - s0) * w * g.ndim],
+             sizeof(double) * (size_t)cnt_host[s] * g.ndim);
+    s0 = s1;
+  }
+  return WM_OK;
+}
+
+// FP32 form: a fresh state's bookkeeping (no pending movers, statistics
+// cleared).
+int fresh_bins(wm_ctx *ctx) {
+  hipLaunchKernelGGL(k_fill_int, dim3(wm_cdiv(ctx->g.ny, 256)), dim3(256), 0, ctx->stream,
+                     ctx->rowpend, (size_t)ctx->g.ny, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, ctx->stream, ctx->fstat, (size_t)WM_NSTAT, 0);
+  WM_HIP(hipGetLastError());
+  ctx->mv_pending = false;
+  ctx->relayouts = 0;
   return WM_OK;
 }
 
@@ -377,55 +437,73 @@ int wm_alloc_device(wm_ctx *ctx) {
   const wm_grid &g = ctx->g;
   if (ctx->cap <= 0) ctx->cap = g.np;
   ctx->cap = (ctx->cap + 15) / 16 * 16;   // segments start on 64-byte sectors
-  if (ctx->mcap <= 0) ctx->mcap = std::max(1024, ctx->cap / 8);
+  if (ctx->form == WM_FORM_FP64 && ctx->mcap <= 0) ctx->mcap = std::max(1024, ctx->cap / 8);
   const int nseg = g.nsp * g.ny;
   const size_t slots = (size_t)nseg * ctx->cap;
   if (ctx->form == WM_FORM_FP32) {
-    // cell bins with room for 1.5x the row capacity's mean per cell, in
-    // 16-slot (64-byte) units; the dense row bank converts state in and out
-    ctx->CB = std::max(16, (int)((1.5 * ctx->cap / g.nx + 15) / 16) * 16);
-    if (ctx->CB > (1 << WM_SLOT_BITS)) {
-      wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %d particles per cell bin exceed %d", ctx->CB,
-                   1 << WM_SLOT_BITS);
-      return WM_ERR_ARG;
-    }
-    const size_t nbin = (size_t)nseg * g.nx, bslots = nbin * ctx->CB;
+    // cell bins inside the row segments (wm_internal.h); the row pools hold a
+    // quarter of a row's slots by default
+    const size_t nbin = (size_t)nseg * g.nx;
     if (nbin > 0xFFFFFFFFu) {
       wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %zu cell bins exceed 2^32", nbin);
       return WM_ERR_ARG;
     }
-    WM_TRY(dmalloc(ctx, &ctx->bins.q, bslots));
-    WM_TRY(dmalloc(ctx, &ctx->bins.uz, bslots));
-    WM_TRY(dmalloc(ctx, &ctx->id_bins, bslots));
+    const long long mrow = ctx->mcap > 0 ? ctx->mcap : std::max(1024LL, (long long)g.nsp * ctx->cap / 4);
+    if (mrow >= (1LL << 24)) {
+      wm_set_error(ctx, WM_ERR_ARG, "FP32 form: %lld mover records per row exceed 2^24", mrow);
+      return WM_ERR_ARG;
+    }
+    ctx->mrow = (int)((mrow + 15) / 16 * 16);
+    WM_TRY(dmalloc(ctx, &ctx->bins.q, slots));
+    WM_TRY(dmalloc(ctx, &ctx->bins.uz, slots));
+    if (g.ndim >= 6) WM_TRY(dmalloc(ctx, &ctx->id_bins, slots));
+    WM_TRY(dmalloc(ctx, &ctx->bstart, (size_t)nseg * (g.nx + 1)));
     WM_TRY(dmalloc(ctx, &ctx->bin_cnt, nbin));
-    WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
-    ctx->nregion = wm_push32_regions(g.nx, g.ny);
-    const size_t ms = (size_t)ctx->nregion * WM_MG;
+    WM_TRY(dmalloc(ctx, &ctx->pcnt, nbin));
     for (int b = 0; b < 2; ++b) {
       WM_TRY(dmalloc(ctx, &ctx->bin_sum[b], nbin));
-      WM_TRY(dmalloc(ctx, &ctx->mv[b].rec, ms));
+      WM_TRY(dmalloc(ctx, &ctx->mv[b].rec, (size_t)g.ny * ctx->mrow));
+      WM_TRY(dmalloc(ctx, &ctx->mfill[b], g.ny));
+      WM_HIP(hipMemsetAsync(ctx->mfill[b], 0, sizeof(int) * g.ny, ctx->stream));
     }
+    WM_TRY(dmalloc(ctx, &ctx->rowpend, g.ny));
+    WM_TRY(dmalloc(ctx, &ctx->rowflag, g.ny));
+    WM_TRY(dmalloc(ctx, &ctx->fstat, WM_NSTAT));
+    // the re-layout's workspace: rows of at most 512 MB together
+    const size_t row_bytes = (size_t)g.nsp * ctx->cap * (20 + (g.ndim >= 6 ? 8 : 0));
+    ctx->relay_rows = (int)std::max<size_t>(1, std::min<size_t>(g.ny, ((size_t)512 << 20) / row_bytes));
+    const size_t rslots = (size_t)ctx->relay_rows * g.nsp * ctx->cap;
+    WM_TRY(dmalloc(ctx, &ctx->rtmp.q, rslots));
+    WM_TRY(dmalloc(ctx, &ctx->rtmp.uz, rslots));
+    if (g.ndim >= 6) WM_TRY(dmalloc(ctx, &ctx->rtmp_id, rslots));
+    WM_TRY(dmalloc(ctx, &ctx->rtmp_bstart, (size_t)ctx->relay_rows * g.nsp * (g.nx + 1)));
+    WM_TRY(dmalloc(ctx, &ctx->relay_list, ctx->relay_rows));
+    WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
+    WM_HIP(hipMemsetAsync(ctx->bstart, 0, sizeof(int) * (size_t)nseg * (g.nx + 1), ctx->stream));
+    WM_HIP(hipMemsetAsync(ctx->rowpend, 0, sizeof(int) * g.ny, ctx->stream));
+    WM_HIP(hipMemsetAsync(ctx->rowflag, 0, sizeof(int) * g.ny, ctx->stream));
+    WM_HIP(hipMemsetAsync(ctx->fstat, 0, sizeof(int) * WM_NSTAT, ctx->stream));
     ctx->mv_pending = false;
-    WM_TRY(alloc_rec32(ctx, ctx->b32[1], slots));
   } else {
     WM_TRY(alloc_bank(ctx, ctx->bank[0], slots));
     WM_TRY(alloc_bank(ctx, ctx->bank[1], slots));
+    WM_TRY(dmalloc(ctx, &ctx->id[0], slots));
+    WM_TRY(dmalloc(ctx, &ctx->id[1], slots));
   }
-  WM_TRY(dmalloc(ctx, &ctx->id[0], slots));
-  WM_TRY(dmalloc(ctx, &ctx->id[1], slots));
   WM_TRY(dmalloc(ctx, &ctx->np2, nseg));
   WM_TRY(dmalloc(ctx, &ctx->cumcnt, (size_t)nseg * (g.nx + 1)));
-  const size_t ms = (size_t)nseg * ctx->mcap;
-  WM_TRY(dmalloc(ctx, &ctx->hole, 2 * ms));
-  WM_TRY(dmalloc(ctx, &ctx->nhole, nseg));
-  WM_TRY(dmalloc(ctx, &ctx->nup, nseg));
-  WM_TRY(dmalloc(ctx, &ctx->ndn, nseg));
   if (ctx->form == WM_FORM_FP64) {
+    // particle_y workspace
+    const size_t ms = (size_t)nseg * ctx->mcap;
+    WM_TRY(dmalloc(ctx, &ctx->hole, 2 * ms));
+    WM_TRY(dmalloc(ctx, &ctx->nhole, nseg));
+    WM_TRY(dmalloc(ctx, &ctx->nup, nseg));
+    WM_TRY(dmalloc(ctx, &ctx->ndn, nseg));
     WM_TRY(alloc_bank(ctx, ctx->mv_up, ms));
     WM_TRY(alloc_bank(ctx, ctx->mv_dn, ms));
+    WM_TRY(dmalloc(ctx, &ctx->mvid_up, ms));
+    WM_TRY(dmalloc(ctx, &ctx->mvid_dn, ms));
   }
-  WM_TRY(dmalloc(ctx, &ctx->mvid_up, ms));
-  WM_TRY(dmalloc(ctx, &ctx->mvid_dn, ms));
 
   FieldGeom &fg = ctx->fg;
   fg.nx = g.nx;
@@ -471,22 +549,38 @@ int wm_alloc_device(wm_ctx *ctx) {
   return WM_OK;
 }
 
-int wm_check_derr(wm_ctx *ctx, const char *where) {
+int wm_fetch_derr(wm_ctx *ctx, int *out) {
   int *hp = (int *)ctx->h_pin;
   WM_HIP(hipMemcpyAsync(hp, ctx->derr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
-  const int h = *hp;
+  *out = *hp;
+  if (*out) {
+    WM_HIP(hipMemsetAsync(ctx->derr, 0, sizeof(int), ctx->stream));
+    WM_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return WM_OK;
+}
+
+int wm_report_derr(wm_ctx *ctx, int h, const char *where) {
   if (!h) return WM_OK;
-  WM_HIP(hipMemsetAsync(ctx->derr, 0, sizeof(int), ctx->stream));
-  WM_HIP(hipStreamSynchronize(ctx->stream));
   if (h & (DERR_NP_OVER | DERR_MOVER_OVER)) {
     wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np) in %s (%s)", where,
                  (h & DERR_NP_OVER) ? "row capacity" : "migration buffer");
     return WM_ERR_NP_OVER;
   }
+  if (h & DERR_RELAYOUT) {
+    wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np) in %s (a cell bin is full)", where);
+    return WM_ERR_NP_OVER;
+  }
   wm_set_error(ctx, WM_ERR_RANGE, "%s: particle outside the cells the reference indexes (%s)",
                where, (h & DERR_JUMP) ? "moved more than one cell" : "cell index out of range");
   return WM_ERR_RANGE;
+}
+
+int wm_check_derr(wm_ctx *ctx, const char *where) {
+  int h = 0;
+  WM_TRY(wm_fetch_derr(ctx, &h));
+  return wm_report_derr(ctx, h, where);
 }
 
 // ================================================================== C ABI
@@ -526,10 +620,6 @@ int wm_destroy(wm_ctx *ctx) {
   free_bank(ctx->bank[1]);
   free_bank(ctx->mv_up);
   free_bank(ctx->mv_dn);
-  free_rec32(ctx->b32[0]);
-  if (ctx->bins.q) (void)hipFree(ctx->bins.q);
-  if (ctx->bins.uz) (void)hipFree(ctx->bins.uz);
-  free_rec32(ctx->b32[1]);
   if (ctx->tmp32) (void)hipFree(ctx->tmp32);
   void *ptrs[] = {ctx->id[0], ctx->id[1], ctx->np2, ctx->cumcnt, ctx->hole, ctx->nhole,
                   ctx->nup, ctx->ndn, ctx->mvid_up, ctx->mvid_dn, ctx->uf, ctx->df, ctx->uj,
@@ -537,7 +627,10 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
                   ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum[0],
                   ctx->bin_sum[1], ctx->mv[0].rec, ctx->mv[1].rec, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
-                  ctx->pc_sc, ctx->fcg_s, ctx->mom, ctx->mom_part};
+                  ctx->pc_sc, ctx->fcg_s, ctx->mom, ctx->mom_part, ctx->bins.q, ctx->bins.uz,
+                  ctx->bstart, ctx->pcnt, ctx->mfill[0], ctx->mfill[1], ctx->rowpend, ctx->rowflag,
+                  ctx->fstat, ctx->rtmp.q, ctx->rtmp.uz, ctx->rtmp_id, ctx->rtmp_bstart,
+                  ctx->relay_list};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   wm_pc_free(ctx);
@@ -808,14 +901,19 @@ int wm_state_upload(wm_ctx *ctx, const double *up, const int *np2, const int *cu
   WM_TRY(require(ctx, ctx->pinit, "particle__init"));
   WM_TRY(wm_alloc_device(ctx));
   const int nseg = nseg_of(ctx);
-  const bool f32 = ctx->form == WM_FORM_FP32;
-  WM_TRY(upload_particles(ctx, up, np2, f32 ? 1 : 0));
-  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  WM_TRY(upload_cumcnt(ctx, cumcnt));
-  if (f32) WM_TRY(wm_k_dense_to_bins(ctx));
+  if (ctx->form == WM_FORM_FP32) {
+    // the rows go straight into the cell bins (cumcnt only checked)
+    WM_TRY(upload_cumcnt(ctx, cumcnt));
+    WM_TRY(fresh_bins(ctx));
+    WM_TRY(upload_bins(ctx, up, np2));
+  } else {
+    WM_TRY(upload_particles(ctx, up, np2, 0));
+    WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
+    WM_TRY(upload_cumcnt(ctx, cumcnt));
+  }
   WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
   WM_HIP(hipStreamSynchronize(ctx->stream));
-  if (f32) WM_TRY(wm_check_derr(ctx, "state upload"));
+  WM_TRY(wm_check_derr(ctx, "state upload"));
   ctx->resident = true;
   return WM_OK;
 }
@@ -825,10 +923,11 @@ int wm_state_download(wm_ctx *ctx, double *up, int *np2, int *cumcnt, double *uf
   if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
   const int nseg = nseg_of(ctx);
   const bool f32 = ctx->form == WM_FORM_FP32;
-  // FP32: the bins become dense rows (sorted by cell, as sort.f90 leaves them)
+  // FP32: the bins' counts give np2 and cumcnt (sorted by cell, as sort.f90
+  // leaves them)
   if (f32 && (up || np2 || cumcnt)) {
-    WM_TRY(wm_k_bins_to_dense(ctx));
-    WM_TRY(wm_check_derr(ctx, "state download"));
+    WM_TRY(wm_settle_movers(ctx));
+    WM_TRY(wm_k_bin_rows(ctx));
   }
   std::vector<int> n2(nseg);
   WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
@@ -836,8 +935,10 @@ int wm_state_download(wm_ctx *ctx, double *up, int *np2, int *cumcnt, double *uf
     WM_HIP(hipMemcpyAsync(cumcnt, ctx->cumcnt, sizeof(int) * (size_t)nseg * (ctx->g.nx + 1),
                           hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
+  if (f32) WM_TRY(wm_check_derr(ctx, "state download"));
   if (np2) memcpy(np2, n2.data(), sizeof(int) * nseg);
-  if (up) WM_TRY(download_particles(ctx, up, n2.data(), f32 ? 1 : 0));
+  if (up) WM_TRY(f32 ? download_bins(ctx, up, n2.data()) : download_particles(ctx, up, n2.data(), 0));
+  if (f32 && up) WM_TRY(wm_check_derr(ctx, "state download"));
   if (uf) WM_TRY(download_field(ctx, uf, 6, ctx->uf));
   return WM_OK;
 }
@@ -847,9 +948,15 @@ int wm_state_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double
   WM_TRY(require(ctx, ctx->pinit, "particle__init"));
   WM_TRY(require(ctx, ctx->sinit, "sort__init"));
   WM_TRY(wm_alloc_device(ctx));
-  WM_TRY(wm_k_init_weibel(ctx, seed, ppc, vth, udrift));
-  // stable by cell within each row: the order of sort__bucket
-  WM_TRY(ctx->form == WM_FORM_FP32 ? wm_k_dense_to_bins(ctx) : wm_k_sort(ctx, 1, 0));
+  if (ctx->form == WM_FORM_FP32) {
+    // straight into the cell bins, stable by cell within each row
+    WM_TRY(fresh_bins(ctx));
+    WM_TRY(wm_k_init_weibel(ctx, seed, ppc, vth, udrift));
+  } else {
+    WM_TRY(wm_k_init_weibel(ctx, seed, ppc, vth, udrift));
+    // stable by cell within each row: the order of sort__bucket
+    WM_TRY(wm_k_sort(ctx, 1, 0));
+  }
   WM_TRY(fill_zero(ctx, ctx->uf, 6 * ctx->fg.ps));
   WM_TRY(wm_check_derr(ctx, "initial state"));
   ctx->resident = true;
@@ -884,11 +991,15 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     if (f32) {
       // Cell bins: the push wraps x itself, and moves the last push's movers
       // into their bins before it pushes (the particle boundaries and the
-      // sort of app.f90:104-106 reduce to that), so nothing runs beside the
-      // solve.
+      // sort of app.f90:104-106 reduce to that).  Beside the solve, on the
+      // second stream, the fill check finds the rows whose bins would
+      // overflow with this push's movers; they are laid out anew below.
       WM_TRY(wm_k_push_deposit32(ctx));
       WM_HIP(hipEventRecord(e[2], ctx->stream));
+      WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
+      StreamSwap on2(ctx, ctx->stream2);
       WM_HIP(hipEventRecord(e[5], ctx->stream));
+      WM_TRY(wm_k_fill_check(ctx));
       WM_HIP(hipEventRecord(e[6], ctx->stream));
       WM_HIP(hipEventRecord(e[7], ctx->stream));
     } else {
@@ -925,7 +1036,20 @@ int wm_step(wm_ctx *ctx, int nsteps) {
       WM_HIP(hipEventElapsedTime(&ms, e[from[k]], e[to[k]]));
       acc[k] += ms;
     }
-    WM_TRY(wm_check_derr(ctx, "step"));
+    int h = 0;
+    WM_TRY(wm_fetch_derr(ctx, &h));
+    if (f32 && (h & DERR_RELAYOUT) && !(h & ~DERR_RELAYOUT)) {
+      // FP32: rows whose bins would overflow are laid out anew (timed as [5])
+      WM_HIP(hipEventRecord(e[9], ctx->stream));
+      WM_TRY(wm_relayout(ctx));
+      WM_HIP(hipEventRecord(e[10], ctx->stream));
+      WM_HIP(hipEventSynchronize(e[10]));
+      float ms = 0.f;
+      WM_HIP(hipEventElapsedTime(&ms, e[9], e[10]));
+      acc[5] += ms;
+      WM_TRY(wm_fetch_derr(ctx, &h));
+    }
+    WM_TRY(wm_report_derr(ctx, h & ~DERR_RELAYOUT, "step"));
   }
   for (int k = 0; k < WM_NPHASE; ++k) ctx->phase_ms[k] = acc[k];
   return WM_OK;
@@ -1057,6 +1181,22 @@ int wm_particle_count(wm_ctx *ctx, long long *count) {
   for (int v : c) t += v;
   *count = t;
   return WM_OK;
+}
+
+int wm_capacity_stats(wm_ctx *ctx, long long *out, int n) {
+  if (!ctx || !out) return WM_ERR_ARG;
+  if (!ctx->alloc_done || ctx->form != WM_FORM_FP32) {
+    wm_set_error(ctx, WM_ERR_ARG, "capacity statistics: no FP32 device state");
+    return WM_ERR_ARG;
+  }
+  int st[WM_NSTAT];
+  WM_HIP(hipMemcpyAsync(st, ctx->fstat, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  const long long v[8] = {st[0], st[1], st[2], st[3], ctx->relayouts, ctx->cap, ctx->mrow,
+                          wm_push32_pool()};
+  const int m = std::min(n, 8);
+  for (int k = 0; k < m; ++k) out[k] = v[k];
+  return m;
 }
 
 // ------------------------------------------- learned preconditioner

@@ -46,7 +46,9 @@ enum : int {
   DERR_NP_OVER = 2,   // a row exceeds the device capacity
   DERR_MOVER_OVER = 4,// a row has more migrating particles than the buffer
   DERR_JUMP = 8,      // a particle moved more than one cell in one step
-  DERR_CG_ITEMAX = 16 // CG reached ite_max
+  DERR_CG_ITEMAX = 16,// CG reached ite_max
+  DERR_RELAYOUT = 32  // FP32 form: a bin would overflow with the pending movers (not an
+                      // error: the host lays the flagged rows out anew before the next push)
 };
 
 // ------------------------------------------------------------- parameters
@@ -83,9 +85,8 @@ struct FcgScal {
   int active, it, bad, itmax;
 };
 
-// FP32 form, cell bins: the movers of one step, structure of arrays, one
-// region of WM_MG slots per cell of every push workgroup.
-#define WM_MG 64
+// FP32 form, cell bins: the movers of one push, one 32-B record each, in a
+// pool per row (all species) that the push workgroups fill chunk by chunk.
 struct __align__(16) MoverRec {   // 32 B: two 16-byte halves
   float ox1, oy1;              // offsets in the destination cell
   float ux, uy, uz;
@@ -100,12 +101,18 @@ struct Movers {
 // it, in one 32-byte load.
 #define WM_SUM_MV 8            // movers of a bin listed in its summary
 #define WM_SLOT_BITS 13        // source slots below 2^13 (bins of at most 8192 slots)
+#define WM_BIN_MAX (1 << WM_SLOT_BITS)
+#define WM_BIN_ALIGN 8         // bins start on 8-slot (128-byte float4) boundaries
 struct __align__(32) BinSum {
   uint8_t cnt[8];              // movers out per direction index
-  uint32_t out;                // (first record in the cell's region << 8) | movers out
+  uint32_t out;                // (first record in the row's pool << 8) | movers out
   uint32_t pad;
   uint16_t mv[WM_SUM_MV];      // the first movers in list order: slot | index << 13
 };
+// Device statistics of the FP32 form's capacities (maxima since the state was
+// made): [0] particles in one bin, [1] particles leaving one bin in a push,
+// [2] movers of one push workgroup, [3] records of one row's mover pool.
+#define WM_NSTAT 8
 
 using PBank = Rec64;
 
@@ -134,10 +141,9 @@ struct wm_ctx {
   // same particle banks, fields and counts, refuse from then on
   bool resident = false;
 
-  // particles: FP64 form (bank) or FP32 form (b32), chosen before allocation
+  // particles: FP64 form (bank) or FP32 form (bins), chosen before allocation
   int form = WM_FORM_FP64;
   PBank bank[2]{};
-  Rec32 b32[2]{};
   float *tmp32 = nullptr;  // cell-centred fields of the FP32 push
   Rec64 b64(int k) const { return bank[k]; }
   int64_t *id[2]{};
@@ -178,22 +184,38 @@ struct wm_ctx {
   int last_iters[3] = {0, 0, 0};
   double phase_ms[8] = {0};
   int *cnt_scratch = nullptr;    // [nseg] counts for layout conversion
-  // FP32 form: particles live in cell bins of CB slots, bin = seg * nx + i
-  // (bins, IDs id_bins); b32[1] with id[] is the dense row layout of the
-  // reference, used to convert state in and out
+  // FP32 form: particles live in cell bins inside the row segments.  Bin
+  // (s, jl, i), index seg * nx + i with seg = s * ny + jl, occupies slots
+  // [bstart[seg * (nx + 1) + i], bstart[seg * (nx + 1) + i + 1]) of segment seg
+  // (seg * cap ..) and holds bin_cnt[bin] particles at its front.  The layout
+  // gives each bin room beyond its count; a bin that would overflow gets its
+  // row laid out anew (wm_relayout).
   Bins32 bins;
-  int CB = 0;
+  int *bstart = nullptr;
   int *bin_cnt = nullptr;        // particles per bin
-  int64_t *id_bins = nullptr;
-  // The movers of a push (records per cell, summary per bin) join their
-  // destination bins in the next push's prologue, or in the settle kernel
-  // before the bins are read otherwise; two buffers, because a push reads
-  // the last push's movers while it writes its own.
+  int64_t *id_bins = nullptr;    // null when ndim = 5 (no IDs)
+  // The movers of a push (records in per-row pools, summary per bin) join
+  // their destination bins in the next push's prologue, in the settle kernel
+  // before the bins are read otherwise, or in a re-layout of their rows; two
+  // buffers, because a push reads the last push's movers while it writes
+  // its own.
   Movers mv[2]{};
+  int *mfill[2] = {nullptr, nullptr};   // records used per row
+  int mrow = 0;                  // records per row pool
   BinSum *bin_sum[2] = {nullptr, nullptr};
   int mv_last = 0;               // the buffer the last push wrote
-  bool mv_pending = false;       // its movers are not in their bins yet
-  int nregion = 0;
+  bool mv_pending = false;       // some rows' bins still await its movers
+  int *rowpend = nullptr;        // [ny] 1: the row's bins await the last push's movers
+  int *rowflag = nullptr;        // [ny] 1: lay the row out anew before the next push
+  int *pcnt = nullptr;           // [nbin] counts once the pending movers are placed
+  int *fstat = nullptr;          // [WM_NSTAT] capacity statistics (maxima)
+  long long relayouts = 0;       // rows laid out anew since the state was made
+  // re-layout workspace: a batch of relay_rows rows (all species)
+  int relay_rows = 0;
+  Bins32 rtmp;
+  int64_t *rtmp_id = nullptr;
+  int *rtmp_bstart = nullptr;
+  int *relay_list = nullptr;
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
   // moment diagnostics: mom in the reference layout, per-tile node sums
   double *mom = nullptr, *mom_part = nullptr;
@@ -216,6 +238,8 @@ struct wm_ctx {
 void wm_set_error(wm_ctx *ctx, int code, const char *fmt, ...);
 int wm_alloc_device(wm_ctx *ctx);
 int wm_check_derr(wm_ctx *ctx, const char *where);
+int wm_fetch_derr(wm_ctx *ctx, int *out);     // device error bits, cleared
+int wm_report_derr(wm_ctx *ctx, int h, const char *where);
 
 // ---------------------------------------------------- kernel launch layer
 // Particles (wm_particle.hip)
@@ -237,15 +261,17 @@ int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x);
 int wm_k_sort(wm_ctx *ctx, int src, int dst);
 // FP32 form, cell bins (wm_sort.hip): dense rows <-> bins, movers into
 // their destination bins, bin counts, row counts
-int wm_k_dense_to_bins(wm_ctx *ctx);
-int wm_k_bins_to_dense(wm_ctx *ctx);
+int wm_k_stage_to_bins(wm_ctx *ctx, int seg0, int nseg, int w);   // staged AoS rows -> bins
+int wm_k_bins_to_stage(wm_ctx *ctx, int seg0, int nseg, int w);   // bins -> staged AoS rows
+int wm_k_bin_rows(wm_ctx *ctx);      // np2 and cumcnt of the bins
 int wm_settle_movers(wm_ctx *ctx);   // the last push's movers into their bins, if pending
+int wm_k_fill_check(wm_ctx *ctx);    // projected bin counts after a push; flags rows to re-lay
+int wm_relayout(wm_ctx *ctx);        // flagged rows laid out anew, their movers placed
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);
 int wm_k_push_deposit32(wm_ctx *ctx);   // in place on the bins (pending movers first)
-int wm_push32_regions(int nx, int ny);   // mover regions of the push launch
+int wm_push32_pool();                     // movers one push workgroup lists
 int wm_k_deposit_gather32(wm_ctx *ctx);
-int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank);
 int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk);
 // Moment diagnostics (wm_mom.hip): CIC moments of bank 0 with cumcnt (FP64)
 // or of the settled bins (FP32) into ctx->mom (allocated by wm_mom_buffers on
@@ -281,6 +307,7 @@ void wm_pc_set_stats(wm_ctx *ctx, const int *fcg, const int *fb);
 void wm_pc_get_stats(const wm_ctx *ctx, int *fcg, int *fb);
 // Synthetic initial state (wm_init.hip)
 int wm_k_init_weibel(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udrift);
+int wm_k_init_weibel_bins(wm_ctx *ctx, uint64_t seed, int ppc, double vth, double udrift);
 
 // --------------------------------------------------------- small helpers
 static inline int wm_cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }

@@ -1,6 +1,7 @@
 // wm_migrate.h -- the FP32 form's migration of one cell bin: the particles
 // that left their cells in a push join their destination bins.  Used by the
-// settle kernel (k_migrate) and by the next push's prologue.
+// settle kernel (k_migrate), by the next push's prologue, and (placing the
+// bin anew) by the re-layout of a row.
 #pragma once
 #include "wm_internal.h"
 
@@ -9,18 +10,26 @@ namespace wm_mig {
 // Inlined into the caller, so that the structs below stay in registers.
 #define WM_MIG_FN __device__ __attribute__((always_inline)) inline
 
-struct Mig {                    // bins and mover regions of one migration
-  int nx, ny, nxr, CB;          // nxr: regions per row (cells rounded up to the push chunk)
+struct Mig {                    // bins and mover pools of one migration
+  int nx, ny, cap, mrow;        // cap: slots per row segment; mrow: records per row pool
   int r0, nr;                   // the migrated rows (every species; settle kernel)
   size_t nbin;                  // bins of those rows
   const BinSum *sum;            // per bin: the push's summary of the movers that left it
+  const int *bstart;            // [seg][nx + 1] first slot of each bin in its segment
   int *cnt;
 };
 
-// First record of the movers of the bin in cell (i, jl) in the cell's region
-// (out: the bin summary's (offset << 8) | count).
-WM_MIG_FN size_t bin_records(const Mig &G, int i, int jl, uint32_t out) {
-  return ((size_t)jl * G.nxr + i) * WM_MG + (out >> 8);
+// First record of the movers of a bin of row jl in the row's pool (out: the
+// bin summary's (offset << 8) | count).
+WM_MIG_FN size_t bin_records(const Mig &G, int jl, uint32_t out) {
+  return (size_t)jl * G.mrow + (out >> 8);
+}
+// First slot and capacity of bin i of segment seg.
+WM_MIG_FN void bin_span(const Mig &G, size_t seg, int i, size_t &base, int &capd) {
+  const int *b = G.bstart + seg * (size_t)(G.nx + 1) + i;
+  const int b0 = b[0];
+  base = seg * (size_t)G.cap + b0;
+  capd = b[1] - b0;
 }
 
 // Byte k of a summary's direction counts (words lo, hi), and entry j of its
@@ -78,14 +87,14 @@ WM_MIG_FN void plan_move(Move &v, MoveState &st, bool compact, const Movers &M, 
     v.qz = t.z;
     v.qw = t.w;
     v.uz = A.uz[base + v.src];
-    v.id = ids[base + v.src];
+    v.id = ids ? ids[base + v.src] : 0;
   }
 }
 WM_MIG_FN void store_move(const Move &v, const Bins32 &A, int64_t *ids, size_t base) {
   if (!v.ok) return;
   A.q[base + v.dst] = make_float4(v.qx, v.qy, v.qz, v.qw);
   A.uz[base + v.dst] = v.uz;
-  ids[base + v.dst] = v.id;
+  if (ids) ids[base + v.dst] = v.id;
 }
 
 // Direction index of entry j of a source's mover list.
@@ -121,23 +130,32 @@ WM_MIG_FN void plan_arrival(Arrival &w, ArrState &as, int k, const Movers &M, ui
     ++as.j;
   }
 }
-WM_MIG_FN void put_arrival(const Arrival &w, const Mig &G, const Bins32 &A, int64_t *ids, int *derr,
+WM_MIG_FN void put_arrival(const Arrival &w, int capd, const Bins32 &A, int64_t *ids, int *derr,
                            size_t base) {
   if (!w.ok) return;
-  if (w.slot >= G.CB) {
+  if (w.slot >= capd) {
     atomicOr(derr, DERR_NP_OVER);
     return;
   }
   const size_t p = base + w.slot;
   A.q[p] = make_float4(w.rec.ox1, w.rec.oy1, w.rec.ux, w.rec.uy);
   A.uz[p] = w.rec.uz;
-  ids[p] = w.rec.id;
+  if (ids) ids[p] = w.rec.id;
 }
 
-// The migration of bin d (cell i, row jl, species sp), by a group of 8
-// consecutive lanes (lane k of the group passes k) that alone writes d: no
-// lane reads what another group writes.  Lane k takes the arrivals from the
-// source bin in direction index k (directions 0, 1, 2, 3, 5, 6, 7, 8), in the
+// Source bin of direction index k (directions 0, 1, 2, 3, 5, 6, 7, 8) for
+// the bin in cell (i, jl) of species sp.
+WM_MIG_FN size_t source_bin(const Mig &G, int i, int jl, int sp, int k, int &si, int &sjl) {
+  const int dir = k < 4 ? k : k + 1;
+  si = wm_wrap(i - (dir % 3 - 1), G.nx);
+  sjl = wm_wrap(jl - (dir / 3 - 1), G.ny);
+  return ((size_t)sp * G.ny + sjl) * G.nx + si;
+}
+
+// The migration of bin d (cell i, row jl, species sp; first slot base,
+// capd slots), by a group of 8 consecutive lanes (lane k of the group
+// passes k) that alone writes d: no lane reads what another group writes.
+// Lane k takes the arrivals from the source bin in direction index k, in the
 // order the source listed them; directions follow one another, so arrival
 // index = the earlier directions' count (a prefix over the group's lanes) +
 // the rank within direction k.  d held n particles and lost m (its movers'
@@ -151,10 +169,9 @@ WM_MIG_FN void put_arrival(const Arrival &w, const Mig &G, const Bins32 &A, int6
 constexpr int DL = 8;
 // Returns d's new count to lane 0 of the group (-1 to the others).
 WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids, int *derr,
-                          size_t d, int i, int jl, int sp, int k) {
-  const int dir = k < 4 ? k : k + 1;
-  const int si = wm_wrap(i - (dir % 3 - 1), G.nx), sjl = wm_wrap(jl - (dir / 3 - 1), G.ny);
-  const size_t src = ((size_t)sp * G.ny + sjl) * G.nx + si;
+                          size_t d, int i, int jl, int sp, int k, size_t base, int capd) {
+  int si, sjl;
+  const size_t src = source_bin(G, i, jl, sp, k, si, sjl);
   const uint4 *own = (const uint4 *)(G.sum + d), *from = (const uint4 *)(G.sum + src);
   uint4 oh = own[0], ol = own[1], fh = from[0], fl = from[1];
   int nd = G.cnt[d];
@@ -162,10 +179,10 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   // only where they are first used, a round trip later)
   __asm__ volatile("" : "+v"(ol.x), "+v"(ol.y), "+v"(ol.z), "+v"(ol.w), "+v"(fh.x), "+v"(fh.y),
                    "+v"(nd));
-  const int n = min(nd, G.CB);
+  const int n = min(nd, capd);
   const int c = sum_cnt(fh.x, fh.y, k);
   const int m = (int)(oh.z & 0xFFu);
-  const size_t r0 = bin_records(G, i, jl, oh.z);
+  const size_t r0 = bin_records(G, jl, oh.z);
   int inc = c;                                   // inclusive prefix over the group
 #pragma unroll
   for (int o = 1; o < DL; o <<= 1) {
@@ -174,7 +191,6 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   }
   const int total = __shfl(inc, DL - 1, DL);
   const int n1 = n - m, nnew = n1 + total;
-  const size_t base = d * G.CB;
   // Lane 0's compaction (fewer arrivals than holes): the particles above the
   // new count that are not holes fill the holes left below it, in order.
   // Its sources lie at or above the new count and the arrivals' holes below
@@ -195,19 +211,19 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   // Arrivals from direction k, in the source's list order: the first two
   // records load together, the rest one by one.
   ArrState as{inc - c, c, 0, c > 0 ? (int)(fh.z & 0xFFu) : 0,
-              c > 0 ? bin_records(G, si, sjl, fh.z) : 0};
+              c > 0 ? bin_records(G, sjl, fh.z) : 0};
   Arrival w0, w1;
   plan_arrival(w0, as, k, M, fl, ol, r0, m, n);
   plan_arrival(w1, as, k, M, fl, ol, r0, m, n);
-  put_arrival(w0, G, A, ids, derr, base);
-  put_arrival(w1, G, A, ids, derr, base);
+  put_arrival(w0, capd, A, ids, derr, base);
+  put_arrival(w1, capd, A, ids, derr, base);
   for (; as.left > 0 && as.j < as.ms; ++as.j) {
     if (list_dir(M, fl, as.rs, as.j) != k) continue;
     Arrival w;
     w.rec = M.rec[as.rs + as.j];
     w.slot = as.a < m ? hole_slot(M, r0, ol, as.a) : n + (as.a - m);
     w.ok = true;
-    put_arrival(w, G, A, ids, derr, base);
+    put_arrival(w, capd, A, ids, derr, base);
     --as.left;
     ++as.a;
   }
@@ -227,17 +243,85 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
       const size_t dst = base + hole_slot(M, r0, ol, q), sf = base + from_slot;
       A.q[dst] = A.q[sf];
       A.uz[dst] = A.uz[sf];
-      ids[dst] = ids[sf];
+      if (ids) ids[dst] = ids[sf];
       ++q;
     }
   }
   int cn = n1 + total;
-  if (cn > G.CB) {
+  if (cn > capd) {
     atomicOr(derr, DERR_NP_OVER);
-    cn = G.CB;
+    cn = capd;
   }
   G.cnt[d] = cn;
   return cn;
 }
 
+// Bin d placed anew (re-layout of its row): its particles that stay, in slot
+// order, then its arrivals in the order migrate_bin would number them, written
+// from slot 0 of the bin's new place dbase in the arrays D (IDs dids), by the
+// same group of 8 lanes.  Returns the new count to lane 0 (-1 to the others).
+WM_MIG_FN int place_bin(const Mig &G, const Movers &M, const Bins32 &A, const int64_t *ids,
+                        const Bins32 &D, int64_t *dids, size_t d, int i, int jl, int sp, int k,
+                        size_t base, int capd, size_t dbase, int dcap, bool pending, int *derr) {
+  const int n = min(G.cnt[d], capd);
+  int m = 0, c = 0, si = 0, sjl = 0;
+  uint4 oh{}, ol{}, fh{}, fl{};
+  if (pending) {
+    const size_t src = source_bin(G, i, jl, sp, k, si, sjl);
+    const uint4 *own = (const uint4 *)(G.sum + d), *from = (const uint4 *)(G.sum + src);
+    oh = own[0];
+    ol = own[1];
+    fh = from[0];
+    fl = from[1];
+    c = sum_cnt(fh.x, fh.y, k);
+    m = (int)(oh.z & 0xFFu);
+  }
+  const size_t r0 = bin_records(G, jl, oh.z);
+  int inc = c;
+#pragma unroll
+  for (int o = 1; o < DL; o <<= 1) {
+    const int v = __shfl_up(inc, o, DL);
+    if (k >= o) inc += v;
+  }
+  const int total = __shfl(inc, DL - 1, DL);
+  const int n1 = n - m;
+  // the particles that stay: lane k takes slots t = k, k + 8, ... below n; its
+  // rank among them is t minus the holes below t (the holes ascend)
+  for (int t = k, h = 0; t < n; t += DL) {
+    while (h < m && hole_slot(M, r0, ol, h) < t) ++h;
+    if (h < m && hole_slot(M, r0, ol, h) == t) continue;
+    const int r = t - h;
+    if (r >= dcap) {
+      atomicOr(derr, DERR_NP_OVER);
+      continue;
+    }
+    D.q[dbase + r] = A.q[base + t];
+    D.uz[dbase + r] = A.uz[base + t];
+    if (ids) dids[dbase + r] = ids[base + t];
+  }
+  // arrivals of direction k after them
+  int a = inc - c;
+  const int ms = c > 0 ? (int)(fh.z & 0xFFu) : 0;
+  const size_t rs = c > 0 ? bin_records(G, sjl, fh.z) : 0;
+  for (int j = 0, left = c; left > 0 && j < ms; ++j) {
+    if (list_dir(M, fl, rs, j) != k) continue;
+    const MoverRec rec = M.rec[rs + j];
+    const int r = n1 + a;
+    ++a;
+    --left;
+    if (r >= dcap) {
+      atomicOr(derr, DERR_NP_OVER);
+      continue;
+    }
+    D.q[dbase + r] = make_float4(rec.ox1, rec.oy1, rec.ux, rec.uy);
+    D.uz[dbase + r] = rec.uz;
+    if (ids) dids[dbase + r] = rec.id;
+  }
+  if (k != 0) return -1;
+  return min(n1 + total, dcap);
+}
+
 }  // namespace wm_mig
+
+// The bins and the mover buffer buf of the context (wm_sort.hip).
+wm_mig::Mig wm_make_mig(const wm_ctx *ctx, int buf);

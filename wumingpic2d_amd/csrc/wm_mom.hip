@@ -31,7 +31,7 @@ static_assert(9 * NMOM <= MG * NST, "a group's slots hold its cell's sums");
 __device__ inline double *grp_w(double *St, int g) { return St + g * MG * NST; }
 
 struct MP {
-  int nx, ny, nxgs, nys, nsp, nchunk, nrb, cap, CB;
+  int nx, ny, nxgs, nys, nsp, nchunk, nrb, cap;
   double d_delx, c, delt;        // delt: the moment step, half the run's (mom_calc.f90:34)
   double q[WM_MAXSP], r[WM_MAXSP];
 };
@@ -60,11 +60,12 @@ struct Src32 {
   const float4 *q;
   const float *uz;
   const int *cnt;
+  const int *bstart;
   __device__ void range(const MP &P, int s, int jl, int i, size_t &base, int &kb, int &ke) const {
-    const size_t bin = ((size_t)s * P.ny + jl) * P.nx + i;
+    const size_t seg = (size_t)s * P.ny + jl;
     kb = 0;
-    ke = min(P.CB, cnt[bin]);
-    base = bin * (size_t)P.CB;
+    ke = cnt[seg * P.nx + i];
+    base = seg * (size_t)P.cap + bstart[seg * (P.nx + 1) + i];
   }
   __device__ Pt load(size_t p, int iabs, int jabs) const {
     const float4 v = q[p];
@@ -335,7 +336,6 @@ MP make_mp(const wm_ctx *ctx) {
   P.nchunk = wm_cdiv(g.nx, MC);
   P.nrb = wm_cdiv(g.ny, MR);
   P.cap = ctx->cap;
-  P.CB = ctx->CB;
   P.d_delx = 1. / g.delx;
   P.c = g.c;
   P.delt = g.delt * 0.5;
@@ -374,7 +374,7 @@ int wm_k_moments(wm_ctx *ctx, bool accl) {
   const MP P = make_mp(ctx);
   if (ctx->form == WM_FORM_FP32) {
     if (!accl) return WM_ERR_ARG;
-    const Src32 src{ctx->bins.q, ctx->bins.uz, ctx->bin_cnt};
+    const Src32 src{ctx->bins.q, ctx->bins.uz, ctx->bin_cnt, ctx->bstart};
     WM_TRY((launch_mom<Src32, true, false>(ctx, P, src, Rec64{})));
   } else {
     const PBank &A = ctx->bank[0];

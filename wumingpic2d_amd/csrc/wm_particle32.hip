@@ -227,21 +227,40 @@ __device__ inline size_t bin_of(int s, int jl, int i, int ny, int nx) {
   return ((size_t)s * ny + jl) * nx + i;
 }
 
-constexpr int XC = WM_MG;       // movers per cell whose deposit inputs stay in LDS (all)
+// The movers of one push workgroup are listed in LDS, per cell in (species,
+// slot) order, for their deposit, their bins' summaries and their records.
+// The list entries come from one pool: a cell's list is a chain of blocks of
+// MB entries that its group takes from the pool as it needs them, so a dense
+// cell can list more movers than an even share.  Entry j of cell c's list is
+// xent(c, j).  Which pool block a cell gets varies from run to run; what a
+// list holds, and so every result, does not.
+constexpr int MB = 16;                // entries per pool block
+constexpr int NBLK = 60;              // blocks in the pool
+constexpr int NPOOL = MB * NBLK;      // movers of one workgroup in one push
+constexpr int MAXB = 64;              // blocks of one cell's list
 
 // The push kernel's LDS (file scope, so device functions address it directly).
 __shared__ float T[6 * 3 * TW];                   // cell-centred fields of the chunk
 __shared__ float4 CF4[CHUNK * CFS / 4];           // per-cell gather coefficients
 __shared__ double cellbuf[CHUNK * NACC];          // per-cell stencil sums
-__shared__ int xl[CHUNK * WM_MG];                 // movers: species << 28 | dir << 20 | slot
-__shared__ float4 xpl[CHUNK * XC];                // movers' offsets before and after
-__shared__ float xql[CHUNK * XC];                 // movers' charge times v_z
-__shared__ int gcnt[CHUNK];                       // movers per cell
+__shared__ int xl[NPOOL];                         // movers: species << 28 | dir << 20 | slot
+__shared__ float4 xpl[NPOOL];                     // movers' offsets before and after
+__shared__ float xql[NPOOL];                      // movers' charge times v_z
+__shared__ uint8_t xblk[CHUNK * MAXB];            // pool block of each block of a cell's list
+__shared__ int xused;                             // pool blocks handed out
+__shared__ int gcnt[CHUNK];                       // movers listed per cell
 __shared__ int scnt[WM_MAXSP * CHUNK];            // particles per (species, cell)
+__shared__ int sbin[WM_MAXSP * CHUNK];            // first slot of each (species, cell) bin
+__shared__ int rfirst;                            // first record of the workgroup's movers in
+                                                  // its row's pool (-1: the pool is full)
 __shared__ float sqc[WM_MAXSP];                   // q delx / delt per species (an LDS read:
                                                   // a per-lane index into the kernel arguments
                                                   // is a memory load, and its wait would also
                                                   // wait for the stores in flight)
+
+__device__ inline int xent(int c, int j) {
+  return (int)xblk[c * MAXB + (j >> 4)] * MB + (j & (MB - 1));
+}
 
 // Shape weight at index a (0..4) of the 5-point stencil for offset o whose
 // 3-point support starts at index 1 + inc (inc = cell change, -1..1; the old
@@ -275,13 +294,13 @@ __device__ inline void place5(float sm, float s0, float sp, float nm, float n0, 
     X[k] = o[k] + 0.5f * ds[k];
   }
 }
-__device__ inline void mover_part(float *acc, int p, int nm, const float4 *xp, const float *xq,
-                                  const int *xg) {
+__device__ inline void mover_part(float *acc, int p, int nm, int mc) {
   const float twelfth = 1.0f / 12.0f;
   for (int j = p; j < nm; j += MQ) {
-    const float4 v = xp[j];
-    const float qv = xq[j];
-    const int xv = xg[j];
+    const int e = xent(mc, j);
+    const float4 v = xpl[e];
+    const float qv = xql[e];
+    const int xv = xl[e];
     const int dir = (xv >> 20) & 0xF;
     const int dy = dir / 3 - 1, dx = dir - 3 * (dy + 1) - 1;
     const float qc = sqc[xv >> 28];
@@ -330,21 +349,20 @@ struct Win {
 };
 
 struct Spc {                    // one species of one cell
-  int s, n, CB;
-  size_t base0;                 // first slot of the species' bin of the chunk's first cell
+  int s, n, off;                // off: first slot of the group's bin in its segment
+  size_t base0;                 // first slot of the species' row segment (workgroup-uniform)
   float q, fac1, txxx, fac2, qc;
 };
-// First slot of the group's bin: from the workgroup-uniform base0, so that no
-// 64-bit per-lane base stays live through the windows.
-__device__ inline size_t spc_base(const Spc &S) {
-  return S.base0 + (size_t)((threadIdx.x / GROUP) * S.CB);
-}
+// First slot of the group's bin: the workgroup-uniform segment start plus the
+// bin's 32-bit offset, so that no 64-bit per-lane base stays live through the
+// windows.
+__device__ inline size_t spc_base(const Spc &S) { return S.base0 + (size_t)(unsigned)S.off; }
 
-__device__ inline void make_spc(Spc &S, int s, int n, int jl, int il, int CB, const PP32 &P) {
+__device__ inline void make_spc(Spc &S, int s, int n, int off, int jl, const PP32 &P) {
   S.s = s;
   S.n = n;
-  S.CB = CB;
-  S.base0 = bin_of(s, jl, il - (int)(threadIdx.x / GROUP), P.ny, P.nx) * (size_t)CB;
+  S.off = off;
+  S.base0 = ((size_t)s * P.ny + jl) * (size_t)P.cap;
   S.q = P.q[s];
   S.fac1 = P.q[s] / P.r[s] * 0.5f * P.delt;
   S.txxx = S.fac1 * S.fac1;
@@ -369,24 +387,22 @@ __device__ inline void load_slot(Win &W, int w, const Bins32 &A, const Spc &S, i
 // Per-lane state of one group's pass over its cell; the group's indices
 // follow from threadIdx and blockIdx.
 struct Lane {
-  int xcnt;                     // movers of the cell so far (all species)
-  bool ok;
+  int xcnt;                     // movers of the cell listed so far (all species)
+  bool ok;                      // no particle jumped more than one cell
+  bool lost;                    // a mover found no room in the pool
 };
 __device__ inline int lane_g() { return threadIdx.x / GROUP; }
 __device__ inline int lane_sub() { return threadIdx.x % GROUP; }
-__device__ inline size_t lane_reg(const PP32 &P) {   // first record of the cell's mover region
-  return (((size_t)blockIdx.y * P.nchunk + blockIdx.x) * CHUNK + lane_g()) * WM_MG;
-}
 
 // One window of GROUP particles (slot k = w * GROUP + sub of the bin); the
 // group executes it together.  Every particle's new state is written back to
 // its slot (a mover's slot becomes a hole, so what it holds does not matter;
 // the window's stores stay contiguous).  A stayer's core stencil is
-// accumulated; a mover goes to the cell's mover region (all but its ID) and
-// to the LDS list, with the inputs of its deposit.
+// accumulated; a mover is listed in LDS with the inputs of its deposit (its
+// record is written after the loop, from its slot).
 __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float oy0, float cux,
-                                   float cuy, float cuz, const Spc &S, const PP32 &P, const Bins32 &A,
-                                   const Movers &M) {
+                                   float cuy, float cuz, const Spc &S, const PP32 &P,
+                                   const Bins32 &A) {
   // Every lane pushes and stores: a lane past the bin's count holds its last
   // particle (or slot 0 of an empty bin) and writes back the same values as
   // that particle's lane, to the same slot.  Unconditional stores keep the
@@ -410,22 +426,36 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
   if (on && !cross) {
     core32(core, ox0, oy0, ox1, oy1, S.qc, qv);
   }
-  // movers in (species, slot) order
+  // movers in (species, slot) order; the group's lane 0 takes the pool blocks
+  // that this window's movers start (at most two)
   const unsigned gm = (unsigned)(__ballot(cross) >> (GROUP * (lane_g() & 3))) & 0xFFFFu;
-  if (cross) {
-    const int pos = L.xcnt + __popc(gm & ((1u << lane_sub()) - 1u));
-    if (pos < WM_MG) {
-      const int dir = (dy + 1) * 3 + (dx + 1);
-      const int kd = dir - (dir > 4);              // direction index 0..7
-      float4 *rp = (float4 *)(M.rec + lane_reg(P) + pos);
-      rp[0] = make_float4(ox1, oy1, o.ux, o.uy);
-      ((float2 *)rp)[2] = make_float2(o.uz, __int_as_float(k | (kd << 24)));
-      xl[lane_g() * WM_MG + pos] = (S.s << 28) | (dir << 20) | k;
-      xpl[lane_g() * XC + pos] = make_float4(ox0, oy0, ox1, oy1);
-      xql[lane_g() * XC + pos] = qv;
+  const int nmv = __popc(gm);
+  if (nmv != 0 && !L.lost) {                       // uniform over the group
+    const int pos0 = L.xcnt;
+    const int b0 = pos0 / MB, b1 = (pos0 + nmv - 1) / MB;
+    const bool fresh = pos0 % MB == 0;
+    const int nnew = (b1 - b0) + (fresh ? 1 : 0);
+    int first = 0;
+    if (lane_sub() == 0 && nnew > 0) first = atomicAdd(&xused, nnew);
+    first = __shfl(first, (int)(threadIdx.x & 63) & ~(GROUP - 1), 64);
+    const bool room = first + nnew <= NBLK && b1 < MAXB;
+    if (lane_sub() == 0 && room) {
+      if (fresh) xblk[lane_g() * MAXB + b0] = (uint8_t)first;
+      if (b1 > b0) xblk[lane_g() * MAXB + b1] = (uint8_t)(first + (fresh ? 1 : 0));
     }
+    __builtin_amdgcn_wave_barrier();
+    if (cross && room) {
+      const int e = xent(lane_g(), pos0 + __popc(gm & ((1u << lane_sub()) - 1u)));
+      const int dir = (dy + 1) * 3 + (dx + 1);
+      xl[e] = (S.s << 28) | (dir << 20) | k;
+      xpl[e] = make_float4(ox0, oy0, ox1, oy1);
+      xql[e] = qv;
+    }
+    if (room)
+      L.xcnt += nmv;
+    else
+      L.lost = true;
   }
-  L.xcnt += __popc(gm);
   __builtin_amdgcn_sched_barrier(0);               // windows are not interleaved (registers)
 }
 
@@ -435,18 +465,17 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
 // of bins above PD * GROUP particles, loaded directly.
 template <bool MORE>
 __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, const Spc &Sn,
-                                    const PP32 &P, const Bins32 &A, const Movers &M) {
+                                    const PP32 &P, const Bins32 &A) {
 #pragma unroll
   for (int w = 0; w < PD; ++w) {
-    push_window(L, core, w * GROUP + lane_sub(), W.ox[w], W.oy[w], W.ux[w], W.uy[w], W.uz[w], S, P, A,
-                M);
+    push_window(L, core, w * GROUP + lane_sub(), W.ox[w], W.oy[w], W.ux[w], W.uy[w], W.uz[w], S, P, A);
     if (MORE) load_slot(W, w, A, Sn, lane_sub());
   }
   for (int w = PD * GROUP; w < S.n; w += GROUP) {
     const int k = w + lane_sub();
     const size_t p = spc_base(S) + min(k, S.n - 1);
     const float4 v = A.q[p];
-    push_window(L, core, k, v.x, v.y, v.z, v.w, A.uz[p], S, P, A, M);
+    push_window(L, core, k, v.x, v.y, v.z, v.w, A.uz[p], S, P, A);
     // nothing of this rare path stays in flight: otherwise the compiler, not
     // knowing whether the loop ran, would drain the register windows after it
     __builtin_amdgcn_s_waitcnt(0);
@@ -454,21 +483,23 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
 }
 
 // Push + deposit in place on the cell bins.  Every particle's new state is
-// written to its slot; one that leaves its cell also goes to the cell's mover
-// region (new state, source slot and direction; its ID at the kernel's end)
-// and its slot becomes a hole.  The movers' deposit is summed per cell from
-// the list in LDS.  Each bin's summary records how many of its particles
-// left, in total (with the offset of its first record in the region) and in
-// each of the 8 directions, and the slots and directions of the first
-// WM_SUM_MV of them.  The next push's prologue (or the settle kernel) then
-// moves the movers into their destination bins.
+// written to its slot; one that leaves its cell is listed in LDS and its slot
+// becomes a hole.  After the loop the movers' deposit is summed per cell from
+// the list, each mover's record (new state, source slot and direction, ID)
+// goes to the row's mover pool, in a range the workgroup takes with one
+// atomic, and each bin's summary records how many of its particles left, in
+// total (with the index of its first record in the pool) and in each of the 8
+// directions, and the slots and directions of the first WM_SUM_MV of them.
+// The next push's prologue (or the settle kernel, or a re-layout of the row)
+// then moves the movers into their destination bins.
 // PAIR: exactly two species (straight-line code, so the compiler keeps the
 // second species' windows in flight through the first without spilling).
 template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
-    PP32 P, int CB, const float *__restrict__ tmp, int TX, size_t tps, Bins32 A,
-    int64_t *ids, wm_mig::Mig G, Movers Mp, int pending, Movers M, BinSum *__restrict__ msum,
-    double *__restrict__ part, int *__restrict__ derr) {
+    PP32 P, const float *__restrict__ tmp, int TX, size_t tps, Bins32 A, int64_t *ids,
+    wm_mig::Mig G, Movers Mp, const int *__restrict__ rowpend, Movers M, int *__restrict__ mfill,
+    int mrow, BinSum *__restrict__ msum, double *__restrict__ part, int *__restrict__ stats,
+    int *__restrict__ derr) {
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
@@ -485,27 +516,32 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
                 ? tmp[(size_t)k * tps + (size_t)(i0 + col) + (size_t)TX * (jl + b)]
                 : 0.0f;
   }
-  // The last push's movers into this workgroup's bins (8 lanes per bin);
-  // they read only what the last push wrote, and write only these bins, so
-  // workgroups need not wait for one another.  Each bin's new count goes
-  // straight to LDS; the barrier below also makes the bins' new slots
-  // visible to the window loads.
-  if (pending) {
-    for (int u = threadIdx.x; u < P.nsp * CHUNK * wm_mig::DL; u += NT) {
-      const int k = u % wm_mig::DL, c = (u / wm_mig::DL) % CHUNK, s = u / (wm_mig::DL * CHUNK);
-      int cn = 0;
-      if (c < ncell)
-        cn = wm_mig::migrate_bin(G, Mp, A, ids, derr, bin_of(s, jl, i0 + c, P.ny, P.nx), i0 + c,
-                                 jl, s, k);
-      if (k == 0) scnt[s * CHUNK + c] = cn;
+  // The last push's movers into this workgroup's bins (8 lanes per bin),
+  // unless the row was laid out anew since (its bins hold them); they read
+  // only what the last push wrote, and write only these bins, so workgroups
+  // need not wait for one another.  Each bin's count and first slot go
+  // straight to LDS; the barrier below also makes the bins' new slots visible
+  // to the window loads.
+  const int pend = rowpend[jl];
+  for (int u = threadIdx.x; u < P.nsp * CHUNK * wm_mig::DL; u += NT) {
+    const int k = u % wm_mig::DL, c = (u / wm_mig::DL) % CHUNK, s = u / (wm_mig::DL * CHUNK);
+    int cn = 0, b0 = 0;
+    if (c < ncell) {
+      const size_t seg = (size_t)s * P.ny + jl;
+      size_t base;
+      int capd;
+      wm_mig::bin_span(G, seg, i0 + c, base, capd);
+      b0 = (int)(base - seg * (size_t)P.cap);
+      const size_t d = seg * P.nx + i0 + c;
+      cn = pend ? wm_mig::migrate_bin(G, Mp, A, ids, derr, d, i0 + c, jl, s, k, base, capd)
+                : min(capd, G.cnt[d]);
     }
-  } else {
-    for (int t = threadIdx.x; t < P.nsp * CHUNK; t += NT) {
-      const int c = t % CHUNK, s = t / CHUNK;
-      scnt[t] = c < ncell ? min(CB, G.cnt[bin_of(s, jl, i0 + c, P.ny, P.nx)]) : 0;
+    if (k == 0) {
+      scnt[s * CHUNK + c] = cn;
+      sbin[s * CHUNK + c] = b0;
     }
   }
-  const size_t rbase = ((size_t)jl * P.nchunk + chunk) * CHUNK;   // first region
+  if (threadIdx.x == 0) xused = 0;
   if (threadIdx.x < WM_MAXSP) sqc[threadIdx.x] = P.q[threadIdx.x] * P.qc_scale;
 #pragma unroll
   for (int q = 0; q < NTV; ++q)
@@ -519,7 +555,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   Win win;
   Spc sc{}, sn{};
   if (g < ncell) {
-    make_spc(sc, 0, scnt[g], jl, il, CB, P);
+    make_spc(sc, 0, scnt[g], sbin[g], jl, P);
 #pragma unroll
     for (int w = 0; w < PD; ++w) load_slot(win, w, A, sc, sub);
   }
@@ -531,26 +567,26 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   }
   __syncthreads();
   double *cb = cellbuf + g * NACC;
-  int *xg = xl + g * WM_MG;
   Lane L;
   L.xcnt = 0;
   L.ok = true;
+  L.lost = false;
   float core[NCORE];
 #pragma unroll
   for (int k = 0; k < NCORE; ++k) core[k] = 0.0f;
   if (g < ncell) {
     // the next species' windows load while this one is pushed
     if (PAIR) {
-      make_spc(sn, 1, scnt[CHUNK + g], jl, il, CB, P);
-      push_species<true>(L, core, win, sc, sn, P, A, M);
-      push_species<false>(L, core, win, sn, sn, P, A, M);
+      make_spc(sn, 1, scnt[CHUNK + g], sbin[CHUNK + g], jl, P);
+      push_species<true>(L, core, win, sc, sn, P, A);
+      push_species<false>(L, core, win, sn, sn, P, A);
     } else {
       for (int s = 0; s + 1 < P.nsp; ++s) {
-        make_spc(sn, s + 1, scnt[(s + 1) * CHUNK + g], jl, il, CB, P);
-        push_species<true>(L, core, win, sc, sn, P, A, M);
+        make_spc(sn, s + 1, scnt[(s + 1) * CHUNK + g], sbin[(s + 1) * CHUNK + g], jl, P);
+        push_species<true>(L, core, win, sc, sn, P, A);
         sc = sn;
       }
-      push_species<false>(L, core, win, sc, sn, P, A, M);
+      push_species<false>(L, core, win, sc, sn, P, A);
     }
     // stay-in-cell stencils of every species, summed over the group once
 #pragma unroll
@@ -559,23 +595,21 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
       if (sub == 0) cb[core_slot(c)] += (double)v;
     }
   }
-  const bool ok = L.ok;
-  const int xcnt = L.xcnt;
-  if (!ok) atomicOr(derr, DERR_JUMP);
-  if (xcnt > WM_MG && sub == 0) atomicOr(derr, DERR_MOVER_OVER);
-  if (sub == 0 && g < CHUNK) gcnt[g] = g < ncell ? min(xcnt, WM_MG) : 0;
-  __syncthreads();                                   // also orders the movers' stores
-  const int gm = g < ncell ? gcnt[g] : 0;
+  if (!L.ok) atomicOr(derr, DERR_JUMP);
+  if (L.lost && sub == 0) atomicOr(derr, DERR_MOVER_OVER);
+  if (sub == 0 && g < CHUNK) gcnt[g] = g < ncell ? L.xcnt : 0;
+  __syncthreads();                                   // the lists, and the movers' new state in
+                                                     // their slots
   // the movers' deposit, on the first wave alone: lane t takes the cell's
-  // movers j = t % MQ, t % MQ + MQ, ... of cell t / MQ
+  // movers j = t % MQ, t % MQ + MQ, ... of cell t / MQ; meanwhile the second
+  // wave takes the workgroup's records in the row's pool
   static_assert(CHUNK * MQ == 64, "the movers' deposit runs on one wave");
   if (threadIdx.x < CHUNK * MQ) {
     const int mc = threadIdx.x / MQ;
     float acc[NACC];
 #pragma unroll
     for (int e = 0; e < NACC; ++e) acc[e] = 0.0f;
-    mover_part(acc, threadIdx.x % MQ, mc < ncell ? gcnt[mc] : 0, xpl + mc * XC, xql + mc * XC,
-               xl + mc * WM_MG);
+    mover_part(acc, threadIdx.x % MQ, mc < ncell ? gcnt[mc] : 0, mc);
     double *mcb = cellbuf + mc * NACC;
     const bool lead = threadIdx.x % MQ == 0 && mc < ncell;
 #pragma unroll
@@ -583,30 +617,65 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
       const float v = quad_sum(acc[e]);
       if (lead) mcb[e] += (double)v;
     }
+  } else if (threadIdx.x == 64) {
+    int tot = 0;
+    for (int c = 0; c < CHUNK; ++c) tot += gcnt[c];
+    int f = tot > 0 ? atomicAdd(mfill + jl, tot) : 0;
+    if (f + tot > mrow) {
+      atomicOr(derr, DERR_MOVER_OVER);
+      f = -1;
+    }
+    rfirst = f;
+    atomicMax(stats + 2, tot);
+  }
+  __syncthreads();
+  const int rf = rfirst;
+  // each mover's record: new state from its slot, ID, source slot and direction
+  {
+    int tot = 0;
+    for (int c = 0; c < CHUNK; ++c) tot += gcnt[c];
+    for (int t = threadIdx.x; t < tot && rf >= 0; t += NT) {
+      int c = 0, j = t;
+      while (j >= gcnt[c]) j -= gcnt[c++];
+      const int xv = xl[xent(c, j)];
+      const int s = xv >> 28, dir = (xv >> 20) & 0xF, k = xv & 0xFFFFF;
+      const size_t p = ((size_t)s * P.ny + jl) * (size_t)P.cap + (unsigned)sbin[s * CHUNK + c] + k;
+      const float4 v = A.q[p];
+      const float uz = A.uz[p];
+      const int64_t id = ids ? ids[p] : 0;
+      float4 *rp = (float4 *)(M.rec + (size_t)jl * mrow + rf + t);
+      rp[0] = v;
+      rp[1] = make_float4(uz, __int_as_float(k | ((dir - (dir > 4)) << 24)),
+                          __int_as_float((int)(uint32_t)id), __int_as_float((int)(id >> 32)));
+    }
   }
   // per bin: movers in total and per direction
   if (g < ncell) {
-    int j0 = 0;
+    int j0 = 0, before = 0;
+    for (int c = 0; c < g; ++c) before += gcnt[c];
+    const int gm = gcnt[g];
     for (int s = 0; s < P.nsp; ++s) {
       int m = 0;
-      while (j0 + m < gm && (xg[j0 + m] >> 28) == s) ++m;
+      while (j0 + m < gm && (xl[xent(g, j0 + m)] >> 28) == s) ++m;
       BinSum *bs = msum + bin_of(s, jl, il, P.ny, P.nx);
       if (sub < 8) {
         const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
         int c = 0;
-        for (int q = 0; q < m; ++q) c += ((xg[j0 + q] >> 20) & 0xF) == d;
+        for (int q = 0; q < m; ++q) c += ((xl[xent(g, j0 + q)] >> 20) & 0xF) == d;
         bs->cnt[sub] = (uint8_t)c;
       }
       if (sub < min(m, WM_SUM_MV)) {
-        const int xv = xg[j0 + sub];
+        const int xv = xl[xent(g, j0 + sub)];
         const int dir = (xv >> 20) & 0xF;
         bs->mv[sub] = (uint16_t)((xv & 0xFFFFF) | ((dir - (dir > 4)) << WM_SLOT_BITS));
       }
-      if (sub == 0) bs->out = (uint32_t)((j0 << 8) | m);   // first record in the region, count
+      if (sub == 0) {
+        if (m > 255) atomicOr(derr, DERR_MOVER_OVER);
+        bs->out = (uint32_t)(((max(rf, 0) + before + j0) << 8) | min(m, 255));
+      }
       j0 += m;
     }
   }
-  __syncthreads();
   double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;
   for (int e = threadIdx.x; e < TILE; e += NT) {
     const int comp = e % 3;
@@ -617,18 +686,15 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(cellbuf + gg * NACC, comp, t - gg, b);
     out[e] = sum;
   }
-  // last, the movers' IDs into their records (the wait for these loads also
-  // waits for this wave's earlier stores, which are done by now)
-  for (int j = sub; j < gm; j += GROUP) {
-    const int xv = xg[j];
-    M.rec[(rbase + g) * WM_MG + j].id = ids[bin_of(xv >> 28, jl, il, P.ny, P.nx) * CB + (xv & 0xFFFFF)];
-  }
 }
 
 // FP32 cell-centred fields (particle.f90:68-81 averages, in FP64, rounded).
+// Also clears the row pools' record counts of the push that follows.
 __global__ void k_cell_fields32(FieldGeom fg, const double *__restrict__ uf,
-                                float *__restrict__ tmp, int TX, int TY, size_t tps) {
+                                float *__restrict__ tmp, int TX, int TY, size_t tps,
+                                int *__restrict__ mfill, int ny) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ny) mfill[t] = 0;
   if (t >= TX * TY) return;
   const int a = t % TX, b = t / TX;
   const size_t c0 = (size_t)(a + 1) + (size_t)fg.gx * (size_t)(b + 1);
@@ -644,9 +710,12 @@ __global__ void k_cell_fields32(FieldGeom fg, const double *__restrict__ uf,
 }
 
 // Sum of the FP32 kernel's tiles into uj (ghost-inclusive), fixed order.
+// Also marks every row's bins as awaiting the push's movers.
 __global__ void k_deposit_gather32(int nx, int ny, int nchunk, FieldGeom fg,
-                                   const double *__restrict__ part, double *__restrict__ uj) {
+                                   const double *__restrict__ part, double *__restrict__ uj,
+                                   int *__restrict__ rowpend) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ny) rowpend[t] = 1;
   if (t >= fg.gx * fg.gy) return;
   const int li = t % fg.gx, lj = t / fg.gx;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -669,60 +738,21 @@ __global__ void k_deposit_gather32(int nx, int ny, int nchunk, FieldGeom fg,
   uj[2 * fg.ps + t] = s2;
 }
 
-// ------------------------------------------ conversion to and from FP64
-// Reference-layout staging rows (seg, k) of ndim doubles <-> Rec32 banks:
-// ci = int(x) - nxgs, ox = x - int(x) rounded to FP32 (kept below 1), and
-// likewise oy against the row j.
-__global__ void k_aos_to_rec32(int ndim, int w, int cap, int ny, int nxgs, int nys,
-                               const int *__restrict__ cnt, const double *__restrict__ st,
-                               Rec32 B, int64_t *id) {
-  const int seg = blockIdx.y;
-  const int n = cnt[seg];
-  const int j = nys + seg % ny;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    const double *r = st + ((size_t)seg * w + k) * ndim;
-    const size_t p = (size_t)seg * cap + k;
-    const int ix = (int)r[0];
-    B.ci[p] = ix - nxgs;
-    B.ox[p] = wm_below_one((float)(r[0] - ix));
-    B.oy[p] = wm_below_one((float)(r[1] - j));
-    B.ux[p] = (float)r[2];
-    B.uy[p] = (float)r[3];
-    B.uz[p] = (float)r[4];
-    if (ndim >= 6) id[p] = __double_as_longlong(r[5]);
-  }
-}
-__global__ void k_rec32_to_aos(int ndim, int w, int cap, int ny, int nxgs, int nys,
-                               const int *__restrict__ cnt, double *__restrict__ st, Rec32 B,
-                               const int64_t *id) {
-  const int seg = blockIdx.y;
-  const int n = cnt[seg];
-  const int j = nys + seg % ny;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    double *r = st + ((size_t)seg * w + k) * ndim;
-    const size_t p = (size_t)seg * cap + k;
-    r[0] = (double)(B.ci[p] + nxgs) + (double)B.ox[p];
-    r[1] = (double)j + (double)B.oy[p];
-    r[2] = B.ux[p];
-    r[3] = B.uy[p];
-    r[4] = B.uz[p];
-    if (ndim >= 6) r[5] = __longlong_as_double(id[p]);
-  }
-}
-
 // FP64 energy sums of FP32 momenta: per-workgroup partials.
 // Workgroup (x, seg) sums the bins of cells x, x + nchunk, ... of segment seg.
-__global__ __launch_bounds__(256) void k_energy_kin32(int nx, int CB, const int *cnt,
-                                                      const float4 *q, const float *uz, double c,
-                                                      double *part, int nchunk) {
+__global__ __launch_bounds__(256) void k_energy_kin32(int nx, int cap, const int *bstart,
+                                                      const int *cnt, const float4 *q,
+                                                      const float *uz, double c, double *part,
+                                                      int nchunk) {
   __shared__ double sh[256];
   const int seg = blockIdx.y;
   double v = 0.0;
   for (int i = blockIdx.x; i < nx; i += nchunk) {
     const size_t bin = (size_t)seg * nx + i;
-    const int n = min(CB, cnt[bin]);
+    const int n = cnt[bin];
+    const size_t base = (size_t)seg * cap + bstart[(size_t)seg * (nx + 1) + i];
     for (int k = threadIdx.x; k < n; k += 256) {
-      const size_t p = bin * CB + k;
+      const size_t p = base + k;
       const double a = q[p].z, b = q[p].w, d = uz[p];
       v += sqrt(1 + (a * a + b * b + d * d) / (c * c)) - 1;
     }
@@ -763,60 +793,42 @@ size_t wm_deposit_part_elems32(int nx, int ny) {
 int wm_k_cell_fields32(wm_ctx *ctx) {
   const int TX = ctx->g.nx + 2, TY = ctx->g.ny + 2;
   const size_t tps = (size_t)TX * TY;
-  hipLaunchKernelGGL(k_cell_fields32, dim3(wm_cdiv((long long)TX * TY, 256)), dim3(256), 0,
-                     ctx->stream, ctx->fg, ctx->uf, ctx->tmp32, TX, TY, tps);
+  // the push that follows writes its movers to the other buffer when the
+  // last push's are pending (wm_k_push_deposit32)
+  const int out = ctx->mv_pending ? 1 - ctx->mv_last : ctx->mv_last;
+  hipLaunchKernelGGL(k_cell_fields32, dim3(wm_cdiv((long long)std::max(TX * TY, ctx->g.ny), 256)),
+                     dim3(256), 0, ctx->stream, ctx->fg, ctx->uf, ctx->tmp32, TX, TY, tps,
+                     ctx->mfill[out], ctx->g.ny);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
-
-int wm_push32_regions(int nx, int ny) { return ny * wm_cdiv(nx, CHUNK) * CHUNK; }
 
 int wm_k_push_deposit32(wm_ctx *ctx) {
   PP32 P = make_pp32(ctx);
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
-  // the last push's movers (if not yet settled) are read from its buffer,
+  // the last push's movers (if not yet placed) are read from its buffer,
   // this push's go to the other
   const int in = ctx->mv_last, out = ctx->mv_pending ? 1 - in : in;
-  wm_mig::Mig G;
-  G.nx = ctx->g.nx;
-  G.ny = ctx->g.ny;
-  G.nxr = (int)((size_t)ctx->nregion / (size_t)ctx->g.ny);
-  G.CB = ctx->CB;
-  G.r0 = 0;
-  G.nr = ctx->g.ny;
-  G.nbin = (size_t)ctx->g.nsp * G.ny * G.nx;
-  G.sum = ctx->bin_sum[in];
-  G.cnt = ctx->bin_cnt;
+  const wm_mig::Mig G = wm_make_mig(ctx, in);
   hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
-                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->CB, ctx->tmp32, TX,
-                     tps, ctx->bins, ctx->id_bins, G, ctx->mv[in], ctx->mv_pending ? 1 : 0,
-                     ctx->mv[out], ctx->bin_sum[out], ctx->dep_part, ctx->derr);
+                     dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->tmp32, TX, tps,
+                     ctx->bins, ctx->id_bins, G, ctx->mv[in], ctx->rowpend, ctx->mv[out],
+                     ctx->mfill[out], ctx->mrow, ctx->bin_sum[out], ctx->dep_part, ctx->fstat,
+                     ctx->derr);
   WM_HIP(hipGetLastError());
   ctx->mv_last = out;
   ctx->mv_pending = true;
   return WM_OK;
 }
 
+int wm_push32_pool() { return NPOOL; }
+
 int wm_k_deposit_gather32(wm_ctx *ctx) {
-  const int n = ctx->fg.gx * ctx->fg.gy;
+  const int n = std::max(ctx->fg.gx * ctx->fg.gy, ctx->g.ny);
   hipLaunchKernelGGL(k_deposit_gather32, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream,
                      ctx->g.nx, ctx->g.ny, wm_cdiv(ctx->g.nx, CHUNK), ctx->fg, ctx->dep_part,
-                     ctx->uj);
-  WM_HIP(hipGetLastError());
-  return WM_OK;
-}
-
-int wm_k_aos_rec32(wm_ctx *ctx, bool to_rec, int w, const int *dcnt, double *stage, int bank) {
-  const wm_grid &g = ctx->g;
-  const int nseg = g.nsp * g.ny;
-  const dim3 grid(std::min(64, wm_cdiv(w, 256)), nseg);
-  if (to_rec)
-    hipLaunchKernelGGL(k_aos_to_rec32, grid, dim3(256), 0, ctx->stream, g.ndim, w, ctx->cap, g.ny,
-                       g.nxgs, g.nys, dcnt, stage, ctx->b32[bank], ctx->id[ctx->idcur]);
-  else
-    hipLaunchKernelGGL(k_rec32_to_aos, grid, dim3(256), 0, ctx->stream, g.ndim, w, ctx->cap, g.ny,
-                       g.nxgs, g.nys, dcnt, stage, ctx->b32[bank], ctx->id[ctx->idcur]);
+                     ctx->uj, ctx->rowpend);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -825,7 +837,8 @@ int wm_k_energy_kin32(wm_ctx *ctx, int bank, double *part, int nchunk) {
   const int nseg = ctx->g.nsp * ctx->g.ny;
   (void)bank;                                      // the bins are the state
   hipLaunchKernelGGL(k_energy_kin32, dim3(nchunk, nseg), dim3(256), 0, ctx->stream, ctx->g.nx,
-                     ctx->CB, ctx->bin_cnt, ctx->bins.q, ctx->bins.uz, ctx->g.c, part, nchunk);
+                     ctx->cap, ctx->bstart, ctx->bin_cnt, ctx->bins.q, ctx->bins.uz, ctx->g.c, part,
+                     nchunk);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

@@ -15,11 +15,14 @@
 #include <utility>
 
 #include "wm_internal.h"
+#include "wm_bins.h"
 #include "wm_migrate.h"
 
 namespace {
 
 using namespace wm_mig;
+using wm_bins::WaveMatch;
+using wm_bins::wave_match;
 
 constexpr int SW = 64;     // one wavefront per segment
 constexpr int BT = 4;      // tiles in flight
@@ -27,30 +30,6 @@ constexpr int BT = 4;      // tiles in flight
 struct SP {
   int nx, nxgs, cap;
 };
-
-struct WaveMatch {
-  int rank;     // equal keys in lower lanes
-  int count;    // equal keys in the wavefront
-  bool leader;  // lowest lane holding the key
-};
-
-__device__ inline WaveMatch wave_match(int key, bool active) {
-  const int lane = threadIdx.x & 63;
-  WaveMatch w{0, 0, false};
-  unsigned long long rem = __ballot(active);
-  while (rem) {
-    const int l = __ffsll((long long)rem) - 1;
-    const int k = __shfl(key, l);
-    const unsigned long long m = __ballot(active && key == k);
-    if (active && key == k) {
-      w.rank = __popcll(m & ((1ULL << lane) - 1ULL));
-      w.count = __popcll(m);
-      w.leader = (lane == l);
-    }
-    rem &= ~m;
-  }
-  return w;
-}
 
 template <class R>
 __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, R src,
@@ -145,68 +124,85 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, 
 }
 
 // ------------------------------------------------------------------------
-// FP32 form: cell bins.  Bin (s, jl, i) = seg * nx + i holds up to CB
-// particles of cell i of row segment seg = s * ny + jl in slots
-// [bin * CB, bin * CB + cnt[bin]).
+// FP32 form: cell bins inside the row segments.  Bin (seg, i) occupies slots
+// [bstart[seg][i], bstart[seg][i + 1]) of segment seg (from seg * cap) and
+// holds cnt[seg * nx + i] particles at its front.
 
-// Dense rows (any order, np2 per segment) -> bins, stable: within a bin the
-// particles keep their row order.  One wavefront per segment, as k_sort.
-__global__ __launch_bounds__(SW) void k_dense_to_bins(SP P, int CB, const int *__restrict__ np2,
-                                                      Rec32 src, const int64_t *__restrict__ ids,
-                                                      Bins32 dst, int64_t *__restrict__ idd,
-                                                      int *__restrict__ cnt, int *derr) {
-  extern __shared__ int run[];             // particles written per cell so far
-  const int seg = blockIdx.x;
-  const int lane = threadIdx.x;
-  const int n = np2[seg];
-  const size_t base = (size_t)seg * P.cap;
+struct SB {
+  int nx, ny, nxgs, nys, cap, nsp;
+};
+
+__device__ inline size_t seg_at(const SB &P, int seg) { return (size_t)seg * P.cap; }
+
+// Staged reference-layout rows -> bins, stable within a bin (row order).
+// Segment seg0 + b holds cnt[b] records of ndim doubles at st + (b * w + k) *
+// ndim.  One wavefront per segment: the cell histogram, the segment's layout
+// (wm_bins::wave_layout), then the scatter in row order.  ci = int(x) - nxgs,
+// ox = x - int(x) rounded to FP32 and kept below 1, oy likewise against the
+// row j.
+__global__ __launch_bounds__(SW) void k_stage_to_bins(SB P, int seg0, int w, int ndim,
+                                                      const int *__restrict__ cnt,
+                                                      const double *__restrict__ st, Bins32 A,
+                                                      int64_t *__restrict__ ids,
+                                                      int *__restrict__ bstart,
+                                                      int *__restrict__ bin_cnt, int *derr) {
+  extern __shared__ int run[];             // particles per cell, then written per cell
+  const int b = blockIdx.x, seg = seg0 + b, lane = threadIdx.x;
+  const int n = cnt[b];
+  const int j = P.nys + seg % P.ny;
+  const double *rows = st + (size_t)b * w * ndim;
+  int *bs = bstart + (size_t)seg * (P.nx + 1);
   int bad = 0;
   for (int i = lane; i < P.nx; i += SW) run[i] = 0;
   __syncthreads();
   for (int k0 = 0; k0 < n; k0 += SW) {
     const int k = k0 + lane;
-    Rec32::Val v{};
-    int key = -1;
-    int64_t vid = 0;
-    if (k < n) {
-      v = src.load(base + k);
-      vid = ids[base + k];
-      key = v.ci;
-      if (key < 0 || key >= P.nx) {
-        bad |= DERR_RANGE;
-        key = -1;
-      }
+    int key = k < n ? (int)rows[(size_t)k * ndim] - P.nxgs : -1;
+    if (k < n && (key < 0 || key >= P.nx)) {
+      bad |= DERR_RANGE;
+      key = -1;
     }
-    const bool act = key >= 0;
-    const WaveMatch m = wave_match(key, act);
-    if (act) {
-      const int slot = run[key] + m.rank;
-      if (slot < CB) {
-        const size_t d = ((size_t)seg * P.nx + key) * CB + slot;
-        dst.q[d] = make_float4(v.ox, v.oy, v.ux, v.uy);
-        dst.uz[d] = v.uz;
-        idd[d] = vid;
-      } else {
-        bad |= DERR_NP_OVER;
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (act && m.leader) run[key] += m.count;
-    __builtin_amdgcn_wave_barrier();
+    const WaveMatch m = wave_match(key, key >= 0);
+    if (key >= 0 && m.leader) run[key] += m.count;
   }
   __syncthreads();
-  for (int i = lane; i < P.nx; i += SW) cnt[(size_t)seg * P.nx + i] = min(run[i], CB);
+  if (!wm_bins::wave_layout([&](int i) { return run[i]; }, P.nx, P.cap, bs)) {
+    for (int i = lane; i <= P.nx; i += SW) bs[i] = 0;    // an empty, consistent layout
+    for (int i = lane; i < P.nx; i += SW) bin_cnt[(size_t)seg * P.nx + i] = 0;
+    if (lane == 0) atomicOr(derr, DERR_NP_OVER | bad);
+    return;
+  }
+  for (int i = lane; i < P.nx; i += SW) {
+    bin_cnt[(size_t)seg * P.nx + i] = run[i];
+    run[i] = 0;
+  }
+  __syncthreads();                         // the layout's stores before its loads below
+  for (int k0 = 0; k0 < n; k0 += SW) {
+    const int k = k0 + lane;
+    const double *r = rows + (size_t)min(k, n - 1) * ndim;
+    const double x = r[0], y = r[1];
+    const int ix = (int)x;
+    const bool act = k < n && ix - P.nxgs >= 0 && ix - P.nxgs < P.nx;
+    const int key = act ? ix - P.nxgs : -1;
+    const WaveMatch m = wave_match(key, act);
+    if (act) {
+      const size_t p = seg_at(P, seg) + bs[key] + run[key] + m.rank;
+      A.q[p] = make_float4(wm_below_one((float)(x - ix)), wm_below_one((float)(y - j)), (float)r[2],
+                           (float)r[3]);
+      A.uz[p] = (float)r[4];
+      if (ids) ids[p] = ndim >= 6 ? __double_as_longlong(r[5]) : 0;
+    }
+    __builtin_amdgcn_wave_barrier();       // every lane has read its offset before the
+    if (act && m.leader) run[key] += m.count;   // leaders advance them
+    __builtin_amdgcn_wave_barrier();
+  }
   if (bad) atomicOr(derr, bad);
 }
 
-// Bins -> dense rows sorted by cell, with cumcnt and np2 as sort.f90:64-69
-// leaves them.  One workgroup per segment.
-__global__ __launch_bounds__(256) void k_bins_to_dense(SP P, int CB, const int *__restrict__ cnt,
-                                                       Bins32 src, const int64_t *__restrict__ ids,
-                                                       Rec32 dst, int64_t *__restrict__ idd,
-                                                       int *__restrict__ cumcnt,
-                                                       int *__restrict__ np2, int *derr) {
-  extern __shared__ int cc[];              // nx + 1 prefix
+// Per segment: cumcnt (exclusive prefix of the bin counts, as sort.f90:64-69
+// leaves it) and np2.  One workgroup per segment.
+__global__ __launch_bounds__(256) void k_bin_rows(SB P, const int *__restrict__ cnt,
+                                                  int *__restrict__ cumcnt, int *__restrict__ np2) {
   __shared__ int sh[256];
   const int seg = blockIdx.x;
   const int *c = cnt + (size_t)seg * P.nx;
@@ -223,43 +219,176 @@ __global__ __launch_bounds__(256) void k_bins_to_dense(SP P, int CB, const int *
     __syncthreads();
   }
   int run = sh[threadIdx.x] - tot;
+  int *cc = cumcnt + (size_t)seg * (P.nx + 1);
   for (int i = c0; i < c1; ++i) {
     cc[i] = run;
     run += c[i];
   }
-  if (threadIdx.x == 255) cc[P.nx] = sh[255];
+  if (threadIdx.x == 255) {
+    cc[P.nx] = sh[255];
+    np2[seg] = sh[255];
+  }
+}
+
+// Bins -> staged reference-layout rows sorted by cell (segment seg0 + b at
+// st + b * w * ndim), with cumcnt from k_bin_rows: x = nxgs + i + ox and
+// y = j + oy in FP64.  One workgroup per segment.
+__global__ __launch_bounds__(256) void k_bins_to_stage(SB P, int seg0, int w, int ndim,
+                                                       const int *__restrict__ cumcnt,
+                                                       const int *__restrict__ bstart, Bins32 A,
+                                                       const int64_t *__restrict__ ids,
+                                                       double *__restrict__ st, int *derr) {
+  extern __shared__ int cc[];              // nx + 1 prefix
+  const int b = blockIdx.x, seg = seg0 + b;
+  const int j = P.nys + seg % P.ny;
+  const int *cg = cumcnt + (size_t)seg * (P.nx + 1);
+  for (int i = threadIdx.x; i <= P.nx; i += 256) cc[i] = cg[i];
   __syncthreads();
   const int n = cc[P.nx];
-  int *cco = cumcnt + (size_t)seg * (P.nx + 1);
-  for (int i = threadIdx.x; i <= P.nx; i += 256) cco[i] = cc[i];
-  if (threadIdx.x == 0) {
-    np2[seg] = n;
-    if (n > P.cap) atomicOr(derr, DERR_NP_OVER);
+  if (n > w) {
+    if (threadIdx.x == 0) atomicOr(derr, DERR_NP_OVER);
+    return;
   }
-  const size_t base = (size_t)seg * P.cap;
-  for (int k = threadIdx.x; k < min(n, P.cap); k += 256) {
+  const int *bs = bstart + (size_t)seg * (P.nx + 1);
+  double *rows = st + (size_t)b * w * ndim;
+  for (int k = threadIdx.x; k < n; k += 256) {
     int lo = 0, hi = P.nx - 1;                     // cell i with cc[i] <= k < cc[i + 1]
     while (lo < hi) {
       const int mid = (lo + hi + 1) / 2;
       if (cc[mid] <= k) lo = mid; else hi = mid - 1;
     }
-    const size_t f = ((size_t)seg * P.nx + lo) * CB + (k - cc[lo]);
-    const size_t d = base + k;
-    const float4 v = src.q[f];
-    dst.ci[d] = lo;
-    dst.ox[d] = v.x;
-    dst.oy[d] = v.y;
-    dst.ux[d] = v.z;
-    dst.uy[d] = v.w;
-    dst.uz[d] = src.uz[f];
-    idd[d] = ids[f];
+    const size_t f = seg_at(P, seg) + bs[lo] + (k - cc[lo]);
+    const float4 v = A.q[f];
+    double *r = rows + (size_t)k * ndim;
+    r[0] = (double)(lo + P.nxgs) + (double)v.x;
+    r[1] = (double)j + (double)v.y;
+    r[2] = v.z;
+    r[3] = v.w;
+    r[4] = A.uz[f];
+    if (ndim >= 6) r[5] = __longlong_as_double(ids ? ids[f] : 0);
+  }
+}
+
+// After a push: every bin's count once the push's movers are placed
+// (count - movers out + arrivals from the 8 neighbours' summaries), into pc.
+// A bin whose count would exceed its room flags its row for a re-layout.
+// Also keeps the capacity statistics.
+__global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
+                                                    int *__restrict__ rowflag,
+                                                    const int *__restrict__ mfill,
+                                                    int *__restrict__ stats, int *derr) {
+  const size_t d = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int cnew = 0, mout = 0;
+  bool over = false;
+  if (d < G.nbin) {
+    const int i = (int)(d % (unsigned)G.nx);
+    const unsigned sj = (unsigned)(d / (unsigned)G.nx);
+    const int jl = (int)(sj % (unsigned)G.ny), sp = (int)(sj / (unsigned)G.ny);
+    mout = (int)(((const uint4 *)(G.sum + d))[0].z & 0xFFu);
+    int arr = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      int si, sjl;
+      const size_t src = source_bin(G, i, jl, sp, k, si, sjl);
+      const uint2 c = ((const uint2 *)(G.sum + src))[0];
+      arr += sum_cnt(c.x, c.y, k);
+    }
+    cnew = G.cnt[d] - mout + arr;
+    pc[d] = cnew;
+    size_t base;
+    int capd;
+    bin_span(G, sj, i, base, capd);
+    over = cnew > capd;
+    if (over) atomicOr(rowflag + jl, 1);
+  }
+  int a = cnew, m = mout;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a = max(a, __shfl_xor(a, o, 64));
+    m = max(m, __shfl_xor(m, o, 64));
+  }
+  const bool any = __ballot(over) != 0ULL;
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(stats + 0, a);
+    atomicMax(stats + 1, m);
+    if (any) atomicOr(derr, DERR_RELAYOUT);
+  }
+  if (d < (size_t)G.ny) atomicMax(stats + 3, mfill[d]);
+}
+
+// Re-layout, step 1: the new layout of each segment of the listed rows
+// (batch row b, species s: block b * nsp + s) from the counts pc, into tbs.
+__global__ __launch_bounds__(SW) void k_relayout_plan(SB P, const int *__restrict__ list,
+                                                      const int *__restrict__ pc,
+                                                      int *__restrict__ tbs, int *derr) {
+  const int b = blockIdx.x / P.nsp, s = blockIdx.x % P.nsp;
+  const int seg = s * P.ny + list[b];
+  const int *c = pc + (size_t)seg * P.nx;
+  int *t = tbs + (size_t)blockIdx.x * (P.nx + 1);
+  if (!wm_bins::wave_layout([&](int i) { return c[i]; }, P.nx, P.cap, t)) {
+    for (int i = threadIdx.x; i <= P.nx; i += SW) t[i] = 0;   // nothing is placed
+    if (threadIdx.x == 0) atomicOr(derr, DERR_NP_OVER);
+  }
+}
+
+// Step 2: every bin of the listed rows placed anew (its particles that stay,
+// then the pending arrivals) into the workspace D at the new layout; the
+// bin's count becomes its new count.  8 lanes per bin.
+__global__ void k_relayout_move(Mig G, SB P, const int *__restrict__ list,
+                                const int *__restrict__ tbs, Movers M, Bins32 A,
+                                const int64_t *__restrict__ ids, Bins32 D,
+                                int64_t *__restrict__ dids, const int *__restrict__ rowpend,
+                                int nb, int *derr) {
+  const size_t nbin = (size_t)nb * P.nsp * P.nx;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < nbin;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const unsigned bl = (unsigned)(t / DL);
+    const int k = (int)(t % DL);
+    const int i = (int)(bl % (unsigned)P.nx);
+    const int bsg = (int)(bl / (unsigned)P.nx);         // batch segment b * nsp + s
+    const int s = bsg % P.nsp, jl = list[bsg / P.nsp];
+    const size_t seg = (size_t)s * P.ny + jl;
+    const size_t d = seg * P.nx + i;
+    size_t base;
+    int capd;
+    bin_span(G, seg, i, base, capd);
+    const int *tb = tbs + (size_t)bsg * (P.nx + 1);
+    const size_t dbase = (size_t)bsg * P.cap + tb[i];
+    const int cn = place_bin(G, M, A, ids, D, dids, d, i, jl, s, k, base, capd, dbase,
+                             tb[i + 1] - tb[i], rowpend[jl] != 0, derr);
+    if (k == 0) G.cnt[d] = cn;
+  }
+}
+
+// Step 3: the workspace back into the rows' segments, with their new
+// layout; the rows hold their movers now.
+__global__ void k_relayout_copy(SB P, const int *__restrict__ list, const int *__restrict__ tbs,
+                                Bins32 D, const int64_t *__restrict__ dids, Bins32 A,
+                                int64_t *__restrict__ ids, int *__restrict__ bstart,
+                                int *__restrict__ rowpend) {
+  const int bsg = blockIdx.y;
+  const int s = bsg % P.nsp, jl = list[bsg / P.nsp];
+  const int seg = s * P.ny + jl;
+  const int *tb = tbs + (size_t)bsg * (P.nx + 1);
+  const int total = tb[P.nx];
+  const size_t src = (size_t)bsg * P.cap, dst = seg_at(P, seg);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    A.q[dst + t] = D.q[src + t];
+    A.uz[dst + t] = D.uz[src + t];
+    if (ids) ids[dst + t] = dids[src + t];
+  }
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i <= P.nx; i += blockDim.x) bstart[(size_t)seg * (P.nx + 1) + i] = tb[i];
+    if (threadIdx.x == 0 && s == 0) rowpend[jl] = 0;
   }
 }
 
 // The settle kernel: the migration of every bin of rows r0 .. r0 + nr - 1 from
 // the last push's summaries and records, when no push follows to do it in
-// its prologue (state download, energy sums).
-__global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids, int *derr) {
+// its prologue (state download, energy sums); rows laid out anew since that
+// push hold their movers already.
+__global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids,
+                          const int *__restrict__ rowpend, int *derr) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < G.nbin;
        t += (size_t)gridDim.x * blockDim.x) {
     // cell (i, jl) and species sp of the group's bin (bins fit 32 bits)
@@ -267,58 +396,135 @@ __global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids, 
     const int i = (int)(du % (unsigned)G.nx);
     const unsigned sj = du / (unsigned)G.nx;
     const int jl = G.r0 + (int)(sj % (unsigned)G.nr), sp = (int)(sj / (unsigned)G.nr);
-    const size_t d = ((size_t)sp * G.ny + jl) * G.nx + i;
-    (void)migrate_bin(G, M, A, ids, derr, d, i, jl, sp, (int)(t % DL));
+    if (!rowpend[jl]) continue;
+    const size_t seg = (size_t)sp * G.ny + jl;
+    const size_t d = seg * G.nx + i;
+    size_t base;
+    int capd;
+    bin_span(G, seg, i, base, capd);
+    (void)migrate_bin(G, M, A, ids, derr, d, i, jl, sp, (int)(t % DL), base, capd);
   }
 }
 
+__global__ void k_fill_i32(int *a, size_t n, int v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
 }  // namespace
 
-int wm_k_dense_to_bins(wm_ctx *ctx) {
-  SP P;
+namespace {
+SB make_sb(const wm_ctx *ctx) {
+  SB P;
   P.nx = ctx->g.nx;
+  P.ny = ctx->g.ny;
   P.nxgs = ctx->g.nxgs;
+  P.nys = ctx->g.nys;
   P.cap = ctx->cap;
-  const int nseg = ctx->g.nsp * ctx->g.ny;
-  hipLaunchKernelGGL(k_dense_to_bins, dim3(nseg), dim3(SW), sizeof(int) * P.nx, ctx->stream, P,
-                     ctx->CB, ctx->np2, ctx->b32[1], ctx->id[ctx->idcur], ctx->bins,
-                     ctx->id_bins, ctx->bin_cnt, ctx->derr);
+  P.nsp = ctx->g.nsp;
+  return P;
+}
+}  // namespace
+
+wm_mig::Mig wm_make_mig(const wm_ctx *ctx, int buf) {
+  Mig G;
+  G.nx = ctx->g.nx;
+  G.ny = ctx->g.ny;
+  G.cap = ctx->cap;
+  G.mrow = ctx->mrow;
+  G.r0 = 0;
+  G.nr = ctx->g.ny;
+  G.nbin = (size_t)ctx->g.nsp * G.ny * G.nx;
+  G.sum = ctx->bin_sum[buf];
+  G.bstart = ctx->bstart;
+  G.cnt = ctx->bin_cnt;
+  return G;
+}
+
+int wm_k_stage_to_bins(wm_ctx *ctx, int seg0, int nseg, int w) {
+  const SB P = make_sb(ctx);
+  const size_t lds = sizeof(int) * (size_t)P.nx;
+  if (lds > 65536) {
+    wm_set_error(ctx, WM_ERR_ARG, "FP32 form: nx=%d exceeds the 16384 cells one LDS histogram holds",
+                 P.nx);
+    return WM_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_stage_to_bins, dim3(nseg), dim3(SW), lds, ctx->stream, P, seg0, w,
+                     ctx->g.ndim, ctx->cnt_scratch, ctx->stage, ctx->bins, ctx->id_bins,
+                     ctx->bstart, ctx->bin_cnt, ctx->derr);
   WM_HIP(hipGetLastError());
-  ctx->mv_pending = false;                         // the bins are the whole state now
   return WM_OK;
 }
 
-int wm_k_bins_to_dense(wm_ctx *ctx) {
-  WM_TRY(wm_settle_movers(ctx));
-  SP P;
-  P.nx = ctx->g.nx;
-  P.nxgs = ctx->g.nxgs;
-  P.cap = ctx->cap;
-  const int nseg = ctx->g.nsp * ctx->g.ny;
-  hipLaunchKernelGGL(k_bins_to_dense, dim3(nseg), dim3(256), sizeof(int) * (P.nx + 1),
-                     ctx->stream, P, ctx->CB, ctx->bin_cnt, ctx->bins, ctx->id_bins,
-                     ctx->b32[1], ctx->id[ctx->idcur], ctx->cumcnt, ctx->np2, ctx->derr);
+int wm_k_bin_rows(wm_ctx *ctx) {
+  const SB P = make_sb(ctx);
+  hipLaunchKernelGGL(k_bin_rows, dim3(P.nsp * P.ny), dim3(256), 0, ctx->stream, P, ctx->bin_cnt,
+                     ctx->cumcnt, ctx->np2);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_bins_to_stage(wm_ctx *ctx, int seg0, int nseg, int w) {
+  const SB P = make_sb(ctx);
+  hipLaunchKernelGGL(k_bins_to_stage, dim3(nseg), dim3(256), sizeof(int) * (size_t)(P.nx + 1),
+                     ctx->stream, P, seg0, w, ctx->g.ndim, ctx->cumcnt, ctx->bstart, ctx->bins,
+                     ctx->id_bins, ctx->stage, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
 int wm_settle_movers(wm_ctx *ctx) {
   if (ctx->form != WM_FORM_FP32 || !ctx->mv_pending) return WM_OK;
-  Mig G;
-  G.nx = ctx->g.nx;
-  G.ny = ctx->g.ny;
-  G.nxr = (int)((size_t)ctx->nregion / (size_t)ctx->g.ny);
-  G.CB = ctx->CB;
-  G.r0 = 0;
-  G.nr = ctx->g.ny;
-  G.nbin = (size_t)ctx->g.nsp * G.ny * G.nx;
-  G.sum = ctx->bin_sum[ctx->mv_last];
-  G.cnt = ctx->bin_cnt;
+  const Mig G = wm_make_mig(ctx, ctx->mv_last);
   hipLaunchKernelGGL(k_migrate, dim3((unsigned)std::min<size_t>((G.nbin * DL + 255) / 256, 65536)),
                      dim3(256), 0, ctx->stream, G, ctx->mv[ctx->mv_last], ctx->bins, ctx->id_bins,
-                     ctx->derr);
+                     ctx->rowpend, ctx->derr);
+  hipLaunchKernelGGL(k_fill_i32, dim3(wm_cdiv(ctx->g.ny, 256)), dim3(256), 0, ctx->stream,
+                     ctx->rowpend, (size_t)ctx->g.ny, 0);
   WM_HIP(hipGetLastError());
   ctx->mv_pending = false;
+  return WM_OK;
+}
+
+int wm_k_fill_check(wm_ctx *ctx) {
+  const Mig G = wm_make_mig(ctx, ctx->mv_last);
+  const size_t nthr = std::max(G.nbin, (size_t)ctx->g.ny);
+  hipLaunchKernelGGL(k_fill_check, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, ctx->stream,
+                     G, ctx->pcnt, ctx->rowflag, ctx->mfill[ctx->mv_last], ctx->fstat, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+// The rows the fill check flagged, laid out anew in batches of relay_rows
+// rows through the workspace; their pending movers are placed on the way.
+int wm_relayout(wm_ctx *ctx) {
+  const SB P = make_sb(ctx);
+  std::vector<int> flag(P.ny), list;
+  WM_HIP(hipMemcpyAsync(flag.data(), ctx->rowflag, sizeof(int) * P.ny, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  hipLaunchKernelGGL(k_fill_i32, dim3(wm_cdiv(P.ny, 256)), dim3(256), 0, ctx->stream, ctx->rowflag,
+                     (size_t)P.ny, 0);
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  for (int jl = 0; jl < P.ny; ++jl)
+    if (flag[jl]) list.push_back(jl);
+  const Mig G = wm_make_mig(ctx, ctx->mv_last);
+  for (size_t r0 = 0; r0 < list.size(); r0 += ctx->relay_rows) {
+    const int nb = (int)std::min<size_t>(ctx->relay_rows, list.size() - r0);
+    WM_HIP(hipMemcpyAsync(ctx->relay_list, list.data() + r0, sizeof(int) * nb,
+                          hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_relayout_plan, dim3(nb * P.nsp), dim3(SW), 0, ctx->stream, P,
+                       ctx->relay_list, ctx->pcnt, ctx->rtmp_bstart, ctx->derr);
+    const size_t nthr = (size_t)nb * P.nsp * P.nx * DL;
+    hipLaunchKernelGGL(k_relayout_move, dim3((unsigned)std::min<size_t>((nthr + 255) / 256, 65536)),
+                       dim3(256), 0, ctx->stream, G, P, ctx->relay_list, ctx->rtmp_bstart,
+                       ctx->mv[ctx->mv_last], ctx->bins, ctx->id_bins, ctx->rtmp, ctx->rtmp_id,
+                       ctx->rowpend, nb, ctx->derr);
+    hipLaunchKernelGGL(k_relayout_copy, dim3(std::max(1, std::min(64, wm_cdiv(P.cap, 256))),
+                       nb * P.nsp), dim3(256), 0, ctx->stream, P, ctx->relay_list, ctx->rtmp_bstart,
+                       ctx->rtmp, ctx->rtmp_id, ctx->bins, ctx->id_bins, ctx->bstart, ctx->rowpend);
+    WM_HIP(hipGetLastError());
+    WM_HIP(hipStreamSynchronize(ctx->stream));       // the list is host memory of this scope
+  }
+  ctx->relayouts += (long long)list.size();
   return WM_OK;
 }
 
@@ -334,7 +540,7 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst) {
     return WM_ERR_ARG;
   }
   if (ctx->form == WM_FORM_FP32) {
-    // the FP32 form keeps its particles in cell bins (wm_k_dense_to_bins)
+    // the FP32 form keeps its particles in cell bins (wm_k_stage_to_bins)
     wm_set_error(ctx, WM_ERR_ARG, "sort__bucket works on the FP64 particle form");
     return WM_ERR_ARG;
   }
