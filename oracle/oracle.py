@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "libwmoracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libwmref.so")
+REF_LIB_OMP = os.path.join(HERE, "_ref", "libwmref_omp.so")   # OpenMP loops on
 
 OR_OK, OR_ERR_CG_ITEMAX, OR_ERR_NP_OVER, OR_ERR_RANGE, OR_ERR_ARG = 0, 1, 2, 3, 4
 MAXSP = 8
@@ -192,8 +193,8 @@ class Oracle:
         return self.lib.or_add_rd(a, b)
 
 
-def ref_available() -> bool:
-    return os.path.exists(REF_LIB)
+def ref_available(omp: bool = False) -> bool:
+    return os.path.exists(REF_LIB_OMP if omp else REF_LIB)
 
 
 class RefBuild:
@@ -204,12 +205,12 @@ class RefBuild:
     arrays once per process, so every instance loads its own private copy of
     the shared object."""
 
-    def __init__(self, p: dict):
-        if not ref_available():
-            raise OracleError("oracle/_ref/libwmref.so not built")
+    def __init__(self, p: dict, omp: bool = False):
+        if not ref_available(omp):
+            raise OracleError("oracle/_ref/libwmref%s.so not built" % ("_omp" if omp else ""))
         self._tmp = tempfile.NamedTemporaryFile(suffix=".so", delete=False)
         self._tmp.close()
-        shutil.copyfile(REF_LIB, self._tmp.name)
+        shutil.copyfile(REF_LIB_OMP if omp else REF_LIB, self._tmp.name)
         self.lib = ctypes.CDLL(self._tmp.name, mode=os.RTLD_LOCAL)
         vp = ctypes.c_void_p
         self.lib._QMparticlePparticle__init.argtypes = [vp] * 14

@@ -265,6 +265,31 @@ def test_fp32_tight_rows_relayout_and_track_oracle():
     assert np.abs(guf - uf).max() < 1e-3 * np.abs(uf).max()
 
 
+def test_fp32_dense_hot_chunks_spill_their_mover_lists():
+    """16 cells per row (one push workgroup), 160 particles per cell and
+    species, u_th = 0.5 c: about a third of the 5120 particles of a
+    workgroup leave their cells each step, more than the 960 its LDS list
+    holds, so the list continues in the row's spill pool.  The trajectory
+    still tracks the FP64 oracle at the FP32 bars."""
+    p, up, np2, cc, uf = state(16, 8, 160, 51, vth=0.5)
+    ctx = _ctx32(p)
+    ctx.state_upload(up, np2, cc, uf)
+    orc = Oracle(p)
+    gp = up.copy(order="F")
+    for _ in range(3):
+        orc.step(up, gp, uf, cc, np2)
+        ctx.step(1)
+    st = ctx.capacity_stats()
+    assert st["max_workgroup_movers"] > st["workgroup_lds_list"] and st["max_row_spill"] > 0, st
+    gup, gnp2, guf = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
+    ctx.state_download(up=gup, np2=gnp2, uf=guf)
+    assert gnp2.sum() == np2.sum()
+    a, b = _by_id(up, np2), _by_id(gup, gnp2)
+    assert a.keys() == b.keys()
+    assert max(np.abs(a[k] - b[k]).max() for k in a) < 2e-5
+    assert np.abs(guf - uf).max() < 1e-3 * np.abs(uf).max()
+
+
 def test_fp32_host_array_entry_points_refuse():
     p, up, np2, cc, uf = state(16, 4, 2, 45)
     ctx = _ctx32(p)

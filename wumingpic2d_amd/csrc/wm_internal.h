@@ -104,14 +104,16 @@ struct Movers {
 #define WM_BIN_MAX (1 << WM_SLOT_BITS)
 #define WM_BIN_ALIGN 8         // bins start on 8-slot (128-byte float4) boundaries
 struct __align__(32) BinSum {
-  uint8_t cnt[8];              // movers out per direction index
-  uint32_t out;                // (first record in the row's pool << 8) | movers out
-  uint32_t pad;
+  uint8_t cnt[8];              // movers out per direction index (at most 255 each)
+  uint32_t first;              // the first of its movers' records in the row's pool
+  uint32_t m;                  // movers out
   uint16_t mv[WM_SUM_MV];      // the first movers in list order: slot | index << 13
 };
 // Device statistics of the FP32 form's capacities (maxima since the state was
 // made): [0] particles in one bin, [1] particles leaving one bin in a push,
-// [2] movers of one push workgroup, [3] records of one row's mover pool.
+// [2] movers of one push workgroup, [3] records of one row's mover pool,
+// [4] entries of one row's spill of the push's mover lists, [5] bins found
+// full by the fill check (a sum).
 #define WM_NSTAT 8
 
 using PBank = Rec64;
@@ -202,6 +204,14 @@ struct wm_ctx {
   Movers mv[2]{};
   int *mfill[2] = {nullptr, nullptr};   // records used per row
   int mrow = 0;                  // records per row pool
+  // The push lists its movers in LDS; a workgroup with more movers than its
+  // LDS list holds continues the list in blocks of its row's spill pool
+  // (srow entries per row; blocks used in sfill).
+  int *spill_l = nullptr;
+  float4 *spill_p = nullptr;
+  float *spill_q = nullptr;
+  int *sfill = nullptr;
+  int srow = 0;
   BinSum *bin_sum[2] = {nullptr, nullptr};
   int mv_last = 0;               // the buffer the last push wrote
   bool mv_pending = false;       // some rows' bins still await its movers
@@ -215,8 +225,13 @@ struct wm_ctx {
   Bins32 rtmp;
   int64_t *rtmp_id = nullptr;
   int *rtmp_bstart = nullptr;
+  int *rtmp_blk = nullptr;       // per batch segment and block of 64 cells: re-laid
   int *relay_list = nullptr;
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
+  // restart files (paraio__init): the directory and the run's decomposition
+  bool ioinit = false;
+  std::string iodir;
+  int io_nproc = 1, io_nrank = 0;
   // moment diagnostics: mom in the reference layout, per-tile node sums
   double *mom = nullptr, *mom_part = nullptr;
   long long cg_iters_total = 0;
@@ -264,6 +279,20 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst);
 int wm_k_stage_to_bins(wm_ctx *ctx, int seg0, int nseg, int w);   // staged AoS rows -> bins
 int wm_k_bins_to_stage(wm_ctx *ctx, int seg0, int nseg, int w);   // bins -> staged AoS rows
 int wm_k_bin_rows(wm_ctx *ctx);      // np2 and cumcnt of the bins
+// The device-resident state row by row in the reference's record layout
+// (wm_api.hip), for the restart files: batches of segments [s0, s1) of at
+// most 2^26 doubles (wm_batch_end, w records per segment), out of the state
+// into host h, or from host h into a state begun by wm_rows_begin and
+// finished by wm_rows_done (FP64: the rows sorted by cell, as sort__bucket
+// leaves them; FP32: the bins are sorted by construction).
+int wm_batch_end(const wm_ctx *ctx, const int *cnt, int s0, int *w);
+int wm_state_np2(wm_ctx *ctx, int *np2_host);
+int wm_rows_out(wm_ctx *ctx, int s0, int s1, int w, double *h);
+int wm_rows_begin(wm_ctx *ctx);
+int wm_rows_in(wm_ctx *ctx, int s0, int s1, int w, const int *cnt_host, const double *h);
+int wm_rows_done(wm_ctx *ctx, const int *np2_host);
+int wm_field_out(wm_ctx *ctx, double *uf);
+int wm_field_in(wm_ctx *ctx, const double *uf);
 int wm_settle_movers(wm_ctx *ctx);   // the last push's movers into their bins, if pending
 int wm_k_fill_check(wm_ctx *ctx);    // projected bin counts after a push; flags rows to re-lay
 int wm_relayout(wm_ctx *ctx);        // flagged rows laid out anew, their movers placed

@@ -19,10 +19,10 @@ struct Mig {                    // bins and mover pools of one migration
   int *cnt;
 };
 
-// First record of the movers of a bin of row jl in the row's pool (out: the
-// bin summary's (offset << 8) | count).
-WM_MIG_FN size_t bin_records(const Mig &G, int jl, uint32_t out) {
-  return (size_t)jl * G.mrow + (out >> 8);
+// First record of the movers of a bin of row jl in the row's pool (first:
+// the bin summary's index).
+WM_MIG_FN size_t bin_records(const Mig &G, int jl, uint32_t first) {
+  return (size_t)jl * G.mrow + first;
 }
 // First slot and capacity of bin i of segment seg.
 WM_MIG_FN void bin_span(const Mig &G, size_t seg, int i, size_t &base, int &capd) {
@@ -178,10 +178,10 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   // all loads in flight together (the compiler would otherwise issue some
   // only where they are first used, a round trip later)
   __asm__ volatile("" : "+v"(ol.x), "+v"(ol.y), "+v"(ol.z), "+v"(ol.w), "+v"(fh.x), "+v"(fh.y),
-                   "+v"(nd));
+                   "+v"(fh.z), "+v"(fh.w), "+v"(nd));
   const int n = min(nd, capd);
   const int c = sum_cnt(fh.x, fh.y, k);
-  const int m = (int)(oh.z & 0xFFu);
+  const int m = (int)oh.w;
   const size_t r0 = bin_records(G, jl, oh.z);
   int inc = c;                                   // inclusive prefix over the group
 #pragma unroll
@@ -210,8 +210,7 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   plan_move(v3, st, compact, M, r0, ol, m, n, base, A, ids);
   // Arrivals from direction k, in the source's list order: the first two
   // records load together, the rest one by one.
-  ArrState as{inc - c, c, 0, c > 0 ? (int)(fh.z & 0xFFu) : 0,
-              c > 0 ? bin_records(G, sjl, fh.z) : 0};
+  ArrState as{inc - c, c, 0, c > 0 ? (int)fh.w : 0, c > 0 ? bin_records(G, sjl, fh.z) : 0};
   Arrival w0, w1;
   plan_arrival(w0, as, k, M, fl, ol, r0, m, n);
   plan_arrival(w1, as, k, M, fl, ol, r0, m, n);
@@ -274,7 +273,7 @@ WM_MIG_FN int place_bin(const Mig &G, const Movers &M, const Bins32 &A, const in
     fh = from[0];
     fl = from[1];
     c = sum_cnt(fh.x, fh.y, k);
-    m = (int)(oh.z & 0xFFu);
+    m = (int)oh.w;
   }
   const size_t r0 = bin_records(G, jl, oh.z);
   int inc = c;
@@ -301,7 +300,7 @@ WM_MIG_FN int place_bin(const Mig &G, const Movers &M, const Bins32 &A, const in
   }
   // arrivals of direction k after them
   int a = inc - c;
-  const int ms = c > 0 ? (int)(fh.z & 0xFFu) : 0;
+  const int ms = c > 0 ? (int)fh.w : 0;
   const size_t rs = c > 0 ? bin_records(G, sjl, fh.z) : 0;
   for (int j = 0, left = c; left > 0 && j < ms; ++j) {
     if (list_dir(M, fl, rs, j) != k) continue;

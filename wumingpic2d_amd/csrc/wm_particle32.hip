@@ -227,17 +227,19 @@ __device__ inline size_t bin_of(int s, int jl, int i, int ny, int nx) {
   return ((size_t)s * ny + jl) * nx + i;
 }
 
-// The movers of one push workgroup are listed in LDS, per cell in (species,
-// slot) order, for their deposit, their bins' summaries and their records.
-// The list entries come from one pool: a cell's list is a chain of blocks of
-// MB entries that its group takes from the pool as it needs them, so a dense
-// cell can list more movers than an even share.  Entry j of cell c's list is
-// xent(c, j).  Which pool block a cell gets varies from run to run; what a
-// list holds, and so every result, does not.
+// The movers of one push workgroup are listed, per cell in (species, slot)
+// order, for their deposit, their bins' summaries and their records.  The
+// list entries come from a pool in LDS: a cell's list is a chain of blocks
+// of MB entries that its group takes from the pool as it needs them, so a
+// dense cell can list more movers than an even share.  Past the LDS pool the
+// chain continues in blocks of the row's spill pool in device memory (block
+// NBLK + b is entry block b of the row's spill).  Entry j of cell c's list is
+// xent(c, j).  Which block a cell gets varies from run to run; what a list
+// holds, and so every result, does not.
 constexpr int MB = 16;                // entries per pool block
-constexpr int NBLK = 60;              // blocks in the pool
-constexpr int NPOOL = MB * NBLK;      // movers of one workgroup in one push
-constexpr int MAXB = 64;              // blocks of one cell's list
+constexpr int NBLK = 60;              // blocks in the LDS pool
+constexpr int NPOOL = MB * NBLK;      // movers of one workgroup listed in LDS
+constexpr int MAXB = 64;              // blocks of one cell's list (1024 movers)
 
 // The push kernel's LDS (file scope, so device functions address it directly).
 __shared__ float T[6 * 3 * TW];                   // cell-centred fields of the chunk
@@ -246,8 +248,8 @@ __shared__ double cellbuf[CHUNK * NACC];          // per-cell stencil sums
 __shared__ int xl[NPOOL];                         // movers: species << 28 | dir << 20 | slot
 __shared__ float4 xpl[NPOOL];                     // movers' offsets before and after
 __shared__ float xql[NPOOL];                      // movers' charge times v_z
-__shared__ uint8_t xblk[CHUNK * MAXB];            // pool block of each block of a cell's list
-__shared__ int xused;                             // pool blocks handed out
+__shared__ uint16_t xblk[CHUNK * MAXB];           // block of each block of a cell's list
+__shared__ int xused;                             // LDS pool blocks handed out
 __shared__ int gcnt[CHUNK];                       // movers listed per cell
 __shared__ int scnt[WM_MAXSP * CHUNK];            // particles per (species, cell)
 __shared__ int sbin[WM_MAXSP * CHUNK];            // first slot of each (species, cell) bin
@@ -258,8 +260,27 @@ __shared__ float sqc[WM_MAXSP];                   // q delx / delt per species (
                                                   // is a memory load, and its wait would also
                                                   // wait for the stores in flight)
 
+// The row's spill of the mover lists: entries at base + (e - NPOOL).
+struct Spill {
+  int *l;
+  float4 *p;
+  float *q;
+  size_t base;
+  int *fill;                                      // the row's spill blocks used
+  int nblk;                                       // spill blocks per row
+};
+
 __device__ inline int xent(int c, int j) {
   return (int)xblk[c * MAXB + (j >> 4)] * MB + (j & (MB - 1));
+}
+__device__ inline int xl_at(int e, const Spill &X) {
+  return e < NPOOL ? xl[e] : X.l[X.base + (e - NPOOL)];
+}
+__device__ inline float4 xpl_at(int e, const Spill &X) {
+  return e < NPOOL ? xpl[e] : X.p[X.base + (e - NPOOL)];
+}
+__device__ inline float xql_at(int e, const Spill &X) {
+  return e < NPOOL ? xql[e] : X.q[X.base + (e - NPOOL)];
 }
 
 // Shape weight at index a (0..4) of the 5-point stencil for offset o whose
@@ -294,13 +315,13 @@ __device__ inline void place5(float sm, float s0, float sp, float nm, float n0, 
     X[k] = o[k] + 0.5f * ds[k];
   }
 }
-__device__ inline void mover_part(float *acc, int p, int nm, int mc) {
+__device__ inline void mover_part(float *acc, int p, int nm, int mc, const Spill &X) {
   const float twelfth = 1.0f / 12.0f;
   for (int j = p; j < nm; j += MQ) {
     const int e = xent(mc, j);
-    const float4 v = xpl[e];
-    const float qv = xql[e];
-    const int xv = xl[e];
+    const float4 v = xpl_at(e, X);
+    const float qv = xql_at(e, X);
+    const int xv = xl_at(e, X);
     const int dir = (xv >> 20) & 0xF;
     const int dy = dir / 3 - 1, dx = dir - 3 * (dy + 1) - 1;
     const float qc = sqc[xv >> 28];
@@ -402,7 +423,7 @@ __device__ inline int lane_sub() { return threadIdx.x % GROUP; }
 // record is written after the loop, from its slot).
 __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float oy0, float cux,
                                    float cuy, float cuz, const Spc &S, const PP32 &P,
-                                   const Bins32 &A) {
+                                   const Bins32 &A, const Spill &X) {
   // Every lane pushes and stores: a lane past the bin's count holds its last
   // particle (or slot 0 of an empty bin) and writes back the same values as
   // that particle's lane, to the same slot.  Unconditional stores keep the
@@ -436,20 +457,35 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
     const bool fresh = pos0 % MB == 0;
     const int nnew = (b1 - b0) + (fresh ? 1 : 0);
     int first = 0;
-    if (lane_sub() == 0 && nnew > 0) first = atomicAdd(&xused, nnew);
+    if (lane_sub() == 0 && nnew > 0) {
+      first = atomicAdd(&xused, nnew);
+      if (first + nnew > NBLK) {                   // LDS pool full: the row's spill pool
+        const int f = atomicAdd(X.fill, nnew);
+        first = f + nnew <= X.nblk ? NBLK + f : -1;
+      }
+    }
     first = __shfl(first, (int)(threadIdx.x & 63) & ~(GROUP - 1), 64);
-    const bool room = first + nnew <= NBLK && b1 < MAXB;
+    const bool room = first >= 0 && b1 < MAXB;
     if (lane_sub() == 0 && room) {
-      if (fresh) xblk[lane_g() * MAXB + b0] = (uint8_t)first;
-      if (b1 > b0) xblk[lane_g() * MAXB + b1] = (uint8_t)(first + (fresh ? 1 : 0));
+      if (fresh) xblk[lane_g() * MAXB + b0] = (uint16_t)first;
+      if (b1 > b0) xblk[lane_g() * MAXB + b1] = (uint16_t)(first + (fresh ? 1 : 0));
     }
     __builtin_amdgcn_wave_barrier();
     if (cross && room) {
       const int e = xent(lane_g(), pos0 + __popc(gm & ((1u << lane_sub()) - 1u)));
       const int dir = (dy + 1) * 3 + (dx + 1);
-      xl[e] = (S.s << 28) | (dir << 20) | k;
-      xpl[e] = make_float4(ox0, oy0, ox1, oy1);
-      xql[e] = qv;
+      const int v = (S.s << 28) | (dir << 20) | k;
+      const float4 pp = make_float4(ox0, oy0, ox1, oy1);
+      if (e < NPOOL) {
+        xl[e] = v;
+        xpl[e] = pp;
+        xql[e] = qv;
+      } else {
+        const size_t g = X.base + (e - NPOOL);
+        X.l[g] = v;
+        X.p[g] = pp;
+        X.q[g] = qv;
+      }
     }
     if (room)
       L.xcnt += nmv;
@@ -465,17 +501,18 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
 // of bins above PD * GROUP particles, loaded directly.
 template <bool MORE>
 __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, const Spc &Sn,
-                                    const PP32 &P, const Bins32 &A) {
+                                    const PP32 &P, const Bins32 &A, const Spill &X) {
 #pragma unroll
   for (int w = 0; w < PD; ++w) {
-    push_window(L, core, w * GROUP + lane_sub(), W.ox[w], W.oy[w], W.ux[w], W.uy[w], W.uz[w], S, P, A);
+    push_window(L, core, w * GROUP + lane_sub(), W.ox[w], W.oy[w], W.ux[w], W.uy[w], W.uz[w], S, P, A,
+                X);
     if (MORE) load_slot(W, w, A, Sn, lane_sub());
   }
   for (int w = PD * GROUP; w < S.n; w += GROUP) {
     const int k = w + lane_sub();
     const size_t p = spc_base(S) + min(k, S.n - 1);
     const float4 v = A.q[p];
-    push_window(L, core, k, v.x, v.y, v.z, v.w, A.uz[p], S, P, A);
+    push_window(L, core, k, v.x, v.y, v.z, v.w, A.uz[p], S, P, A, X);
     // nothing of this rare path stays in flight: otherwise the compiler, not
     // knowing whether the loop ran, would drain the register windows after it
     __builtin_amdgcn_s_waitcnt(0);
@@ -498,12 +535,14 @@ template <bool PAIR>
 __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     PP32 P, const float *__restrict__ tmp, int TX, size_t tps, Bins32 A, int64_t *ids,
     wm_mig::Mig G, Movers Mp, const int *__restrict__ rowpend, Movers M, int *__restrict__ mfill,
-    int mrow, BinSum *__restrict__ msum, double *__restrict__ part, int *__restrict__ stats,
-    int *__restrict__ derr) {
+    int mrow, Spill X, BinSum *__restrict__ msum, double *__restrict__ part,
+    int *__restrict__ stats, int *__restrict__ derr) {
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;
   const int i0 = chunk * CHUNK;                      // cell index relative to nxgs
   const int ncell = min(CHUNK, P.nx - i0);
+  X.base = (size_t)jl * (size_t)(X.nblk * MB);
+  X.fill += jl;
   // the chunk's cell-centred fields first: they do not depend on the
   // migration, so their loads overlap its
   constexpr int NTV = (6 * 3 * TW + NT - 1) / NT;
@@ -578,15 +617,15 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     // the next species' windows load while this one is pushed
     if (PAIR) {
       make_spc(sn, 1, scnt[CHUNK + g], sbin[CHUNK + g], jl, P);
-      push_species<true>(L, core, win, sc, sn, P, A);
-      push_species<false>(L, core, win, sn, sn, P, A);
+      push_species<true>(L, core, win, sc, sn, P, A, X);
+      push_species<false>(L, core, win, sn, sn, P, A, X);
     } else {
       for (int s = 0; s + 1 < P.nsp; ++s) {
         make_spc(sn, s + 1, scnt[(s + 1) * CHUNK + g], sbin[(s + 1) * CHUNK + g], jl, P);
-        push_species<true>(L, core, win, sc, sn, P, A);
+        push_species<true>(L, core, win, sc, sn, P, A, X);
         sc = sn;
       }
-      push_species<false>(L, core, win, sc, sn, P, A);
+      push_species<false>(L, core, win, sc, sn, P, A, X);
     }
     // stay-in-cell stencils of every species, summed over the group once
 #pragma unroll
@@ -609,7 +648,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     float acc[NACC];
 #pragma unroll
     for (int e = 0; e < NACC; ++e) acc[e] = 0.0f;
-    mover_part(acc, threadIdx.x % MQ, mc < ncell ? gcnt[mc] : 0, mc);
+    mover_part(acc, threadIdx.x % MQ, mc < ncell ? gcnt[mc] : 0, mc, X);
     double *mcb = cellbuf + mc * NACC;
     const bool lead = threadIdx.x % MQ == 0 && mc < ncell;
 #pragma unroll
@@ -637,7 +676,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     for (int t = threadIdx.x; t < tot && rf >= 0; t += NT) {
       int c = 0, j = t;
       while (j >= gcnt[c]) j -= gcnt[c++];
-      const int xv = xl[xent(c, j)];
+      const int xv = xl_at(xent(c, j), X);
       const int s = xv >> 28, dir = (xv >> 20) & 0xF, k = xv & 0xFFFFF;
       const size_t p = ((size_t)s * P.ny + jl) * (size_t)P.cap + (unsigned)sbin[s * CHUNK + c] + k;
       const float4 v = A.q[p];
@@ -656,22 +695,23 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     const int gm = gcnt[g];
     for (int s = 0; s < P.nsp; ++s) {
       int m = 0;
-      while (j0 + m < gm && (xl[xent(g, j0 + m)] >> 28) == s) ++m;
+      while (j0 + m < gm && (xl_at(xent(g, j0 + m), X) >> 28) == s) ++m;
       BinSum *bs = msum + bin_of(s, jl, il, P.ny, P.nx);
       if (sub < 8) {
         const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
         int c = 0;
-        for (int q = 0; q < m; ++q) c += ((xl[xent(g, j0 + q)] >> 20) & 0xF) == d;
-        bs->cnt[sub] = (uint8_t)c;
+        for (int q = 0; q < m; ++q) c += ((xl_at(xent(g, j0 + q), X) >> 20) & 0xF) == d;
+        if (c > 255) atomicOr(derr, DERR_MOVER_OVER);
+        bs->cnt[sub] = (uint8_t)min(c, 255);
       }
       if (sub < min(m, WM_SUM_MV)) {
-        const int xv = xl[xent(g, j0 + sub)];
+        const int xv = xl_at(xent(g, j0 + sub), X);
         const int dir = (xv >> 20) & 0xF;
         bs->mv[sub] = (uint16_t)((xv & 0xFFFFF) | ((dir - (dir > 4)) << WM_SLOT_BITS));
       }
       if (sub == 0) {
-        if (m > 255) atomicOr(derr, DERR_MOVER_OVER);
-        bs->out = (uint32_t)(((max(rf, 0) + before + j0) << 8) | min(m, 255));
+        bs->first = (uint32_t)(max(rf, 0) + before + j0);
+        bs->m = (uint32_t)m;
       }
       j0 += m;
     }
@@ -689,12 +729,16 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
 }
 
 // FP32 cell-centred fields (particle.f90:68-81 averages, in FP64, rounded).
-// Also clears the row pools' record counts of the push that follows.
+// Also clears the row pools' record counts and spill counts of the push that
+// follows.
 __global__ void k_cell_fields32(FieldGeom fg, const double *__restrict__ uf,
                                 float *__restrict__ tmp, int TX, int TY, size_t tps,
-                                int *__restrict__ mfill, int ny) {
+                                int *__restrict__ mfill, int *__restrict__ sfill, int ny) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ny) mfill[t] = 0;
+  if (t < ny) {
+    mfill[t] = 0;
+    sfill[t] = 0;
+  }
   if (t >= TX * TY) return;
   const int a = t % TX, b = t / TX;
   const size_t c0 = (size_t)(a + 1) + (size_t)fg.gx * (size_t)(b + 1);
@@ -798,7 +842,7 @@ int wm_k_cell_fields32(wm_ctx *ctx) {
   const int out = ctx->mv_pending ? 1 - ctx->mv_last : ctx->mv_last;
   hipLaunchKernelGGL(k_cell_fields32, dim3(wm_cdiv((long long)std::max(TX * TY, ctx->g.ny), 256)),
                      dim3(256), 0, ctx->stream, ctx->fg, ctx->uf, ctx->tmp32, TX, TY, tps,
-                     ctx->mfill[out], ctx->g.ny);
+                     ctx->mfill[out], ctx->sfill, ctx->g.ny);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -814,8 +858,9 @@ int wm_k_push_deposit32(wm_ctx *ctx) {
   hipLaunchKernelGGL(P.nsp == 2 ? k_push_deposit32<true> : k_push_deposit32<false>,
                      dim3(P.nchunk, P.ny), dim3(NT), 0, ctx->stream, P, ctx->tmp32, TX, tps,
                      ctx->bins, ctx->id_bins, G, ctx->mv[in], ctx->rowpend, ctx->mv[out],
-                     ctx->mfill[out], ctx->mrow, ctx->bin_sum[out], ctx->dep_part, ctx->fstat,
-                     ctx->derr);
+                     ctx->mfill[out], ctx->mrow,
+                     Spill{ctx->spill_l, ctx->spill_p, ctx->spill_q, 0, ctx->sfill, ctx->srow / MB},
+                     ctx->bin_sum[out], ctx->dep_part, ctx->fstat, ctx->derr);
   WM_HIP(hipGetLastError());
   ctx->mv_last = out;
   ctx->mv_pending = true;

@@ -276,6 +276,7 @@ __global__ __launch_bounds__(256) void k_bins_to_stage(SB P, int seg0, int w, in
 __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
                                                     int *__restrict__ rowflag,
                                                     const int *__restrict__ mfill,
+                                                    const int *__restrict__ sfill,
                                                     int *__restrict__ stats, int *derr) {
   const size_t d = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   int cnew = 0, mout = 0;
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
     const int i = (int)(d % (unsigned)G.nx);
     const unsigned sj = (unsigned)(d / (unsigned)G.nx);
     const int jl = (int)(sj % (unsigned)G.ny), sp = (int)(sj / (unsigned)G.ny);
-    mout = (int)(((const uint4 *)(G.sum + d))[0].z & 0xFFu);
+    mout = (int)(((const uint4 *)(G.sum + d))[0].w);
     int arr = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -299,8 +300,10 @@ __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
     int capd;
     bin_span(G, sj, i, base, capd);
     over = cnew > capd;
-    if (over) atomicOr(rowflag + jl, 1);
+    if (over && !rowflag[jl]) atomicOr(rowflag + jl, 1);
   }
+  // maxima: one lane per wave, and an atomic only where it raises the value
+  // (atomics on one address from every wave would serialise)
   int a = cnew, m = mout;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -309,37 +312,94 @@ __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
   }
   const bool any = __ballot(over) != 0ULL;
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(stats + 0, a);
-    atomicMax(stats + 1, m);
+    if (a > stats[0]) atomicMax(stats + 0, a);
+    if (m > stats[1]) atomicMax(stats + 1, m);
     if (any) atomicOr(derr, DERR_RELAYOUT);
+    if (any) atomicAdd(stats + 5, __popcll(__ballot(over)));
   }
-  if (d < (size_t)G.ny) atomicMax(stats + 3, mfill[d]);
+  if (d < (size_t)G.ny) {
+    if (mfill[d] > stats[3]) atomicMax(stats + 3, mfill[d]);
+    if (16 * sfill[d] > stats[4]) atomicMax(stats + 4, 16 * sfill[d]);   // blocks of 16
+  }
 }
 
-// Re-layout, step 1: the new layout of each segment of the listed rows
-// (batch row b, species s: block b * nsp + s) from the counts pc, into tbs.
+// Re-layout.  A row's bins are laid out anew block by block: a block of RB
+// cells with a bin that would overflow gets a new layout inside the slots
+// its bins span now, each bin its need plus a share of the block's free
+// slots in proportion to its need; when the block's span cannot hold its
+// bins' needs the whole segment is laid out anew (wm_bins::wave_layout).
+// Every other bin keeps its place and takes its pending movers in place, as
+// the next push's prologue would, so the whole row holds its movers after.
+constexpr int RB = 64;
+
+// Step 1, one wavefront per segment of the listed rows (batch row b,
+// species s: block b * nsp + s): the new first slots into tbs and, per
+// block of RB cells, whether its bins move (rblk).
 __global__ __launch_bounds__(SW) void k_relayout_plan(SB P, const int *__restrict__ list,
                                                       const int *__restrict__ pc,
-                                                      int *__restrict__ tbs, int *derr) {
-  const int b = blockIdx.x / P.nsp, s = blockIdx.x % P.nsp;
-  const int seg = s * P.ny + list[b];
+                                                      const int *__restrict__ bstart,
+                                                      int *__restrict__ tbs, int *__restrict__ rblk,
+                                                      int *derr) {
+  const int bsg = blockIdx.x, lane = threadIdx.x;
+  const int seg = (bsg % P.nsp) * P.ny + list[bsg / P.nsp];
   const int *c = pc + (size_t)seg * P.nx;
-  int *t = tbs + (size_t)blockIdx.x * (P.nx + 1);
+  const int *bs = bstart + (size_t)seg * (P.nx + 1);
+  const int nblk = (P.nx + RB - 1) / RB;
+  int *t = tbs + (size_t)bsg * (P.nx + 1);
+  int *rb = rblk + (size_t)bsg * nblk;
+  bool full = false;
+  for (int blk = 0; blk < nblk && !full; ++blk) {
+    const int i0 = blk * RB, i1 = min(P.nx, i0 + RB), i = i0 + lane;
+    const bool in = i < i1;
+    const int cnt = in ? c[i] : 0, b0 = in ? bs[i] : 0, b1 = in ? bs[i + 1] : 0;
+    if (__ballot(in && cnt > b1 - b0) == 0ULL) {             // no bin of the block is full
+      if (in) t[i] = b0;
+      if (lane == 0) rb[blk] = 0;
+      continue;
+    }
+    const int lo = bs[i0], hi = bs[i1];
+    const int need = in ? wm_bins::bin_need(cnt) : 0;
+    int S = need;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) S += __shfl_xor(S, o, 64);
+    if (S > hi - lo || __ballot(cnt > WM_BIN_MAX) != 0ULL) {
+      full = true;                                         // wave-uniform
+      break;
+    }
+    const long long units = (hi - lo - S) / WM_BIN_ALIGN;
+    constexpr int top = WM_BIN_MAX / WM_BIN_ALIGN * WM_BIN_ALIGN;
+    const int room = in ? (int)min((long long)top, need + units * need / S * WM_BIN_ALIGN) : 0;
+    int incl = room;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (in) t[i] = lo + incl - room;
+    if (lane == 0) rb[blk] = 1;
+  }
+  if (!full) {
+    if (lane == 0) t[P.nx] = bs[P.nx];
+    return;
+  }
+  for (int blk = lane; blk < nblk; blk += SW) rb[blk] = 1;
   if (!wm_bins::wave_layout([&](int i) { return c[i]; }, P.nx, P.cap, t)) {
-    for (int i = threadIdx.x; i <= P.nx; i += SW) t[i] = 0;   // nothing is placed
-    if (threadIdx.x == 0) atomicOr(derr, DERR_NP_OVER);
+    for (int i = lane; i <= P.nx; i += SW) t[i] = 0;       // nothing is placed
+    if (lane == 0) atomicOr(derr, DERR_NP_OVER);
   }
 }
 
-// Step 2: every bin of the listed rows placed anew (its particles that stay,
-// then the pending arrivals) into the workspace D at the new layout; the
-// bin's count becomes its new count.  8 lanes per bin.
+// Step 2: every bin of the listed rows takes its pending movers, 8 lanes per
+// bin: a bin of a re-laid block is placed anew (its particles that stay, then
+// its arrivals) into the workspace D at its new place; any other bin migrates
+// in place.  The bin's count becomes its new count.
 __global__ void k_relayout_move(Mig G, SB P, const int *__restrict__ list,
-                                const int *__restrict__ tbs, Movers M, Bins32 A,
-                                const int64_t *__restrict__ ids, Bins32 D,
+                                const int *__restrict__ tbs, const int *__restrict__ rblk,
+                                Movers M, Bins32 A, int64_t *__restrict__ ids, Bins32 D,
                                 int64_t *__restrict__ dids, const int *__restrict__ rowpend,
                                 int nb, int *derr) {
   const size_t nbin = (size_t)nb * P.nsp * P.nx;
+  const int nblk = (P.nx + RB - 1) / RB;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < nbin;
        t += (size_t)gridDim.x * blockDim.x) {
     const unsigned bl = (unsigned)(t / DL);
@@ -349,33 +409,48 @@ __global__ void k_relayout_move(Mig G, SB P, const int *__restrict__ list,
     const int s = bsg % P.nsp, jl = list[bsg / P.nsp];
     const size_t seg = (size_t)s * P.ny + jl;
     const size_t d = seg * P.nx + i;
+    const bool pend = rowpend[jl] != 0;
     size_t base;
     int capd;
     bin_span(G, seg, i, base, capd);
-    const int *tb = tbs + (size_t)bsg * (P.nx + 1);
-    const size_t dbase = (size_t)bsg * P.cap + tb[i];
-    const int cn = place_bin(G, M, A, ids, D, dids, d, i, jl, s, k, base, capd, dbase,
-                             tb[i + 1] - tb[i], rowpend[jl] != 0, derr);
-    if (k == 0) G.cnt[d] = cn;
+    if (rblk[(size_t)bsg * nblk + i / RB]) {
+      const int *tb = tbs + (size_t)bsg * (P.nx + 1);
+      const size_t dbase = (size_t)bsg * P.cap + tb[i];
+      const int cn = place_bin(G, M, A, ids, D, dids, d, i, jl, s, k, base, capd, dbase,
+                               tb[i + 1] - tb[i], pend, derr);
+      if (k == 0) G.cnt[d] = cn;
+    } else if (pend) {
+      (void)migrate_bin(G, M, A, ids, derr, d, i, jl, s, k, base, capd);
+    }
   }
 }
 
-// Step 3: the workspace back into the rows' segments, with their new
-// layout; the rows hold their movers now.
-__global__ void k_relayout_copy(SB P, const int *__restrict__ list, const int *__restrict__ tbs,
-                                Bins32 D, const int64_t *__restrict__ dids, Bins32 A,
-                                int64_t *__restrict__ ids, int *__restrict__ bstart,
-                                int *__restrict__ rowpend) {
+// Step 3: the re-laid bins from the workspace into their new places (a
+// wavefront per bin), the segments' new first slots; the rows hold their
+// movers now.
+__global__ __launch_bounds__(256) void k_relayout_copy(SB P, const int *__restrict__ list,
+                                                       const int *__restrict__ tbs,
+                                                       const int *__restrict__ rblk, Bins32 D,
+                                                       const int64_t *__restrict__ dids, Bins32 A,
+                                                       int64_t *__restrict__ ids,
+                                                       const int *__restrict__ cnt,
+                                                       int *__restrict__ bstart,
+                                                       int *__restrict__ rowpend) {
   const int bsg = blockIdx.y;
   const int s = bsg % P.nsp, jl = list[bsg / P.nsp];
   const int seg = s * P.ny + jl;
+  const int nblk = (P.nx + RB - 1) / RB;
   const int *tb = tbs + (size_t)bsg * (P.nx + 1);
-  const int total = tb[P.nx];
-  const size_t src = (size_t)bsg * P.cap, dst = seg_at(P, seg);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    A.q[dst + t] = D.q[src + t];
-    A.uz[dst + t] = D.uz[src + t];
-    if (ids) ids[dst + t] = dids[src + t];
+  const int lane = threadIdx.x & 63;
+  for (int i = blockIdx.x * 4 + threadIdx.x / 64; i < P.nx; i += gridDim.x * 4) {
+    if (!rblk[(size_t)bsg * nblk + i / RB]) continue;
+    const int n = cnt[(size_t)seg * P.nx + i];
+    const size_t src = (size_t)bsg * P.cap + tb[i], dst = seg_at(P, seg) + tb[i];
+    for (int k = lane; k < n; k += 64) {
+      A.q[dst + k] = D.q[src + k];
+      A.uz[dst + k] = D.uz[src + k];
+      if (ids) ids[dst + k] = dids[src + k];
+    }
   }
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i <= P.nx; i += blockDim.x) bstart[(size_t)seg * (P.nx + 1) + i] = tb[i];
@@ -489,7 +564,8 @@ int wm_k_fill_check(wm_ctx *ctx) {
   const Mig G = wm_make_mig(ctx, ctx->mv_last);
   const size_t nthr = std::max(G.nbin, (size_t)ctx->g.ny);
   hipLaunchKernelGGL(k_fill_check, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, ctx->stream,
-                     G, ctx->pcnt, ctx->rowflag, ctx->mfill[ctx->mv_last], ctx->fstat, ctx->derr);
+                     G, ctx->pcnt, ctx->rowflag, ctx->mfill[ctx->mv_last], ctx->sfill, ctx->fstat,
+                     ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -512,15 +588,17 @@ int wm_relayout(wm_ctx *ctx) {
     WM_HIP(hipMemcpyAsync(ctx->relay_list, list.data() + r0, sizeof(int) * nb,
                           hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_relayout_plan, dim3(nb * P.nsp), dim3(SW), 0, ctx->stream, P,
-                       ctx->relay_list, ctx->pcnt, ctx->rtmp_bstart, ctx->derr);
+                       ctx->relay_list, ctx->pcnt, ctx->bstart, ctx->rtmp_bstart, ctx->rtmp_blk,
+                       ctx->derr);
     const size_t nthr = (size_t)nb * P.nsp * P.nx * DL;
     hipLaunchKernelGGL(k_relayout_move, dim3((unsigned)std::min<size_t>((nthr + 255) / 256, 65536)),
                        dim3(256), 0, ctx->stream, G, P, ctx->relay_list, ctx->rtmp_bstart,
-                       ctx->mv[ctx->mv_last], ctx->bins, ctx->id_bins, ctx->rtmp, ctx->rtmp_id,
-                       ctx->rowpend, nb, ctx->derr);
-    hipLaunchKernelGGL(k_relayout_copy, dim3(std::max(1, std::min(64, wm_cdiv(P.cap, 256))),
-                       nb * P.nsp), dim3(256), 0, ctx->stream, P, ctx->relay_list, ctx->rtmp_bstart,
-                       ctx->rtmp, ctx->rtmp_id, ctx->bins, ctx->id_bins, ctx->bstart, ctx->rowpend);
+                       ctx->rtmp_blk, ctx->mv[ctx->mv_last], ctx->bins, ctx->id_bins, ctx->rtmp,
+                       ctx->rtmp_id, ctx->rowpend, nb, ctx->derr);
+    hipLaunchKernelGGL(k_relayout_copy, dim3(std::max(1, std::min(256, wm_cdiv(P.nx, 4))), nb * P.nsp),
+                       dim3(256), 0, ctx->stream, P, ctx->relay_list, ctx->rtmp_bstart,
+                       ctx->rtmp_blk, ctx->rtmp, ctx->rtmp_id, ctx->bins, ctx->id_bins,
+                       ctx->bin_cnt, ctx->bstart, ctx->rowpend);
     WM_HIP(hipGetLastError());
     WM_HIP(hipStreamSynchronize(ctx->stream));       // the list is host memory of this scope
   }
