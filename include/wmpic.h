@@ -178,9 +178,12 @@ int wm_particle_count(wm_ctx *ctx, long long *count);
  * overflows np, boundary_periodic.f90:228-236).  out[0] most particles in
  * one cell bin, [1] most particles leaving one bin in a push, [2] most
  * movers of one push workgroup, [3] most mover records of one row in a
- * push, [4] rows laid out anew (a bin would have overflowed), [5] slots per
- * (species, row) segment, [6] mover records per row, [7] movers one push
- * workgroup can list.  Returns the number written. */
+ * push, [4] most entries of one row's spill of the push's mover lists, [5]
+ * rows whose bins were laid out anew (a bin would have overflowed), [6]
+ * slots per (species, row) segment, [7] mover records per row, [8] movers
+ * one push workgroup lists in LDS, [9] spill entries per row, [10] bins found
+ * full (each re-laid with its block of 64 cells).  Returns the number
+ * written (at most n, at most 11). */
 int wm_capacity_stats(wm_ctx *ctx, long long *out, int n);
 
 /* ---- moment diagnostics of an output step (proj/weibel/app.f90:119-121)
@@ -206,6 +209,45 @@ int wm_bcp_mom(wm_ctx *ctx, double *mom);
  * and the moments in one pass over the particles, then the fold); only mom
  * is copied to the host.  Needs mom_calc__init. */
 int wm_moments(wm_ctx *ctx, double *mom);
+
+/* ---- restart files (common/paraio.f90): a JSON file of attributes and
+ *      dataset offsets beside a raw file, for one process (nproc = 1) ---- */
+/* paraio__init(ndim,np,nsp,nxgs,nxge,nygs,nyge,nys,nye,nproc,nrank,delx,
+ * delt,c,q,r,dir) (paraio.f90:34-76); the grid must be the modules'. */
+int wm_paraio_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
+                   int nygs, int nyge, int nys, int nye, int nproc, int nrank,
+                   double delx, double delt, double c, const double *q,
+                   const double *r, const char *dir);
+/* paraio__output(up,uf,np2,nxs,nxe,it,filename) (paraio.f90:93-228): writes
+ * dir//filename//'.json' and '.raw' from host arrays. */
+int wm_paraio_output(wm_ctx *ctx, const double *up, const double *uf,
+                     const int *np2, int nxs, int nxe, int it,
+                     const char *filename);
+/* paraio__input(up,uf,np2,nxs,nxe,it,filename) (paraio.f90:233-426): reads
+ * them back into host arrays, with the reference's checks and stops. */
+int wm_paraio_input(wm_ctx *ctx, double *up, double *uf, int *np2, int *nxs,
+                    int *nxe, int *it, const char *filename);
+/* The same files from the device-resident state, row by row (no host copy
+ * of the whole state; up's inactive slots are left holes of the file), and
+ * into it: the rows as io__input reads them, sorted by cell as the app's
+ * sort__bucket after io__input leaves them (proj/weibel/app.f90:351-352),
+ * with the CG warm start cleared as in a fresh process. */
+int wm_paraio_output_state(wm_ctx *ctx, int it, const char *filename);
+int wm_paraio_input_state(wm_ctx *ctx, int *it, const char *filename);
+/* The run's other outputs in the same format: paraio__param(n0,wpe,wpi,wge,
+ * wgi,vti,vte,filename) (paraio.f90:431-555); paraio__mom(mom,uf,it)
+ * (:557-713, divides mom's velocities and temperatures by the density in
+ * place, as the reference does); paraio__ptcl(up,uf,np2,it) and
+ * paraio__orb(up,uf,np2,it) (:715-863: every active particle, or those with
+ * a positive ID).  File names are filename, or it followed by _mom, _ptcl or
+ * _orb, under the directory of wm_paraio_init. */
+int wm_paraio_param(wm_ctx *ctx, int n0, double wpe, double wpi, double wge,
+                    double wgi, double vti, double vte, const char *filename);
+int wm_paraio_mom(wm_ctx *ctx, double *mom, const double *uf, int it);
+int wm_paraio_ptcl(wm_ctx *ctx, const double *up, const double *uf,
+                   const int *np2, int it);
+int wm_paraio_orb(wm_ctx *ctx, const double *up, const double *uf,
+                  const int *np2, int it);
 
 /* ---- learned preconditioner of the implicit solve (no reference
  *      counterpart: the reference runs unpreconditioned CG, field.f90:319-461).

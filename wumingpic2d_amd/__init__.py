@@ -67,8 +67,18 @@ EXPORTS = (
     "wm_state_download", "wm_state_init_weibel", "wm_step", "wm_energy", "wm_cg_iters",
     "wm_phase_times", "wm_particle_count", "wm_capacity_stats",
     "wm_set_solver", "wm_pc_collect", "wm_pc_pair_count", "wm_pc_init", "wm_pc_train",
-    "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_grad", "wm_solve_stats",
+    "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_grad",
+    "wm_solve_stats",
     "wm_mom_calc_init", "wm_mom_calc_accl", "wm_mom_calc_nvt", "wm_bcp_mom", "wm_moments",
+    "wm_paraio_init", "wm_paraio_output", "wm_paraio_input", "wm_paraio_output_state",
+    "wm_paraio_input_state", "wm_paraio_param", "wm_paraio_mom", "wm_paraio_ptcl", "wm_paraio_orb",
+    # include/wmpic_json.h: the JSON model behind the Fortran json_module
+    "wm_json_new_object", "wm_json_load", "wm_json_parse", "wm_json_free", "wm_json_child",
+    "wm_json_get_int", "wm_json_get_real", "wm_json_get_bool", "wm_json_get_str",
+    "wm_json_array_size", "wm_json_add_child", "wm_json_add_int", "wm_json_add_real",
+    "wm_json_add_bool", "wm_json_add_str", "wm_json_add_ints", "wm_json_add_reals",
+    "wm_json_update_str", "wm_json_update_int", "wm_json_update_real", "wm_json_print",
+    "wm_json_serialize",
 )
 
 
@@ -147,6 +157,15 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_mom_calc_nvt": [_P, _P, _P, _P],
         "wm_bcp_mom": [_P, _P],
         "wm_moments": [_P, _P],
+        "wm_paraio_init": [_P] + [_I] * 11 + [_D, _D, _D, _P, _P, ctypes.c_char_p],
+        "wm_paraio_output": [_P, _P, _P, _P, _I, _I, _I, ctypes.c_char_p],
+        "wm_paraio_input": [_P, _P, _P, _P, _P, _P, _P, ctypes.c_char_p],
+        "wm_paraio_output_state": [_P, _I, ctypes.c_char_p],
+        "wm_paraio_input_state": [_P, _P, ctypes.c_char_p],
+        "wm_paraio_param": [_P, _I] + [_D] * 6 + [ctypes.c_char_p],
+        "wm_paraio_mom": [_P, _P, _P, _I],
+        "wm_paraio_ptcl": [_P, _P, _P, _P, _I],
+        "wm_paraio_orb": [_P, _P, _P, _P, _I],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -379,6 +398,55 @@ class Context:
         self._check(self._lib.wm_moments(self._h, mom.ctypes.data))
         return mom
 
+    # ------------------------------------------ restart files (paraio)
+    def paraio__init(self, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, nproc, nrank, delx,
+                     delt, c, q, r, dir_: str) -> None:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        self._check(self._lib.wm_paraio_init(self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys,
+                                             nye, nproc, nrank, delx, delt, c, q.ctypes.data,
+                                             r.ctypes.data, dir_.encode()))
+
+    def paraio__output(self, up, uf, np2, nxs, nxe, it, filename: str) -> None:
+        self._check(self._lib.wm_paraio_output(self._h, self._a(up, "up"), self._a(uf, "uf"),
+                                               self._a(np2, "np2"), nxs, nxe, it,
+                                               filename.encode()))
+
+    def paraio__input(self, up, uf, np2, filename: str):
+        """Reads into up, uf, np2; returns (nxs, nxe, it)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.wm_paraio_input(self._h, self._a(up, "up"), self._a(uf, "uf"),
+                                              self._a(np2, "np2"), ctypes.byref(a),
+                                              ctypes.byref(b), ctypes.byref(c),
+                                              filename.encode()))
+        return a.value, b.value, c.value
+
+    def paraio__param(self, n0, wpe, wpi, wge, wgi, vti, vte, filename: str) -> None:
+        self._check(self._lib.wm_paraio_param(self._h, n0, wpe, wpi, wge, wgi, vti, vte,
+                                              filename.encode()))
+
+    def paraio__mom(self, mom, uf, it) -> None:
+        self._check(self._lib.wm_paraio_mom(self._h, self._a(mom, "mom"), self._a(uf, "uf"), it))
+
+    def paraio__ptcl(self, up, uf, np2, it) -> None:
+        self._check(self._lib.wm_paraio_ptcl(self._h, self._a(up, "up"), self._a(uf, "uf"),
+                                             self._a(np2, "np2"), it))
+
+    def paraio__orb(self, up, uf, np2, it) -> None:
+        self._check(self._lib.wm_paraio_orb(self._h, self._a(up, "up"), self._a(uf, "uf"),
+                                            self._a(np2, "np2"), it))
+
+    def save_restart(self, it: int, filename: str) -> None:
+        """paraio__output of the device-resident state, row by row."""
+        self._check(self._lib.wm_paraio_output_state(self._h, int(it), filename.encode()))
+
+    def load_restart(self, filename: str) -> int:
+        """paraio__input into the device-resident state (rows sorted by cell, CG
+        warm start cleared); returns it."""
+        it = ctypes.c_int()
+        self._check(self._lib.wm_paraio_input_state(self._h, ctypes.byref(it), filename.encode()))
+        return it.value
+
     def cg_iters(self) -> np.ndarray:
         out = np.zeros(3, dtype=np.int32)
         self._check(self._lib.wm_cg_iters(self._h, out.ctypes.data))
@@ -397,12 +465,13 @@ class Context:
     def capacity_stats(self) -> dict:
         """FP32 form: the particle store's capacities and their use since the
         state was made (maxima over the pushes)."""
-        out = np.zeros(8, dtype=np.int64)
-        n = self._lib.wm_capacity_stats(self._h, out.ctypes.data, 8)
-        if n != 8:                      # an error status (8 entries on success)
+        out = np.zeros(11, dtype=np.int64)
+        n = self._lib.wm_capacity_stats(self._h, out.ctypes.data, 11)
+        if n != 11:                     # an error status (11 entries on success)
             self._check(n)
         keys = ("max_bin_fill", "max_bin_movers", "max_workgroup_movers", "max_row_records",
-                "relayouts", "slots_per_row", "records_per_row", "workgroup_mover_list")
+                "max_row_spill", "relayouts", "slots_per_row", "records_per_row",
+                "workgroup_lds_list", "spill_per_row", "full_bins")
         return {k: int(v) for k, v in zip(keys, out[:n])}
 
     # ------------------------------------------ learned preconditioner

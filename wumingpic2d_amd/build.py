@@ -1,6 +1,7 @@
 """Build recipe of libwmpic.so (HIP, gfx950) -- run by __graft_entry__.build().
 
-Each ``csrc/*.hip`` file is compiled to an object in parallel, then linked
+Each ``csrc/*.hip`` file (and host-only ``csrc/*.cpp``) is compiled to an
+object in parallel, then linked
 into ``wumingpic2d_amd/libwmpic.so`` in-tree.  Packed FP32 instructions are
 disabled for every device compile.
 """
@@ -36,8 +37,8 @@ def _run(cmd: list[str]) -> None:
 
 def build_hip(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    objs = [os.path.join(BUILD, os.path.basename(s)[:-4] + ".o") for s in srcs]
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
+    objs = [os.path.join(BUILD, os.path.splitext(os.path.basename(s))[0] + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         futs = [ex.submit(_run, [HIPCC, *FLAGS, "-c", s, "-o", o]) for s, o in zip(srcs, objs)]
         for f in futs:
@@ -53,10 +54,16 @@ def build_hip(verbose: bool = False) -> str:
 FORTRAN = os.path.join(HERE, "fortran")
 FMOD = os.path.join(FORTRAN, "mod")
 FLIB = os.path.join(HERE, "libwmpic_f.so")
-FSRC = ["wmpic_capi.f90", "wm_mpi_set.f90", "wm_particle.f90", "wm_field.f90", "wm_sort.f90",
-        "wm_boundary_periodic.f90", "wm_mom_calc.f90"]
+FSRC = ["apps/mpi.f90", "wmpic_capi.f90", "wm_mpi_set.f90", "wm_particle.f90", "wm_field.f90",
+        "wm_sort.f90", "wm_boundary_periodic.f90", "wm_mom_calc.f90",
+        # what the reference's apps use besides the hot path (single process)
+        "apps/json_module.f90", "apps/mpiio.f90", "apps/jsonio.f90", "apps/wm_paraio.f90",
+        "apps/wuming_utils.f90", "apps/wuming2d.f90"]
+FCSRC = ["apps/wm_mpi_stub.c"]
 DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "weibel_loop.f90")
 DRIVER = os.path.join(ROOT, "tests", "fortran", "weibel_loop.x")
+JSON_SRC = os.path.join(ROOT, "tests", "fortran", "json_check.f90")
+JSON_X = os.path.join(ROOT, "tests", "fortran", "json_check.x")
 
 
 def build_fortran(verbose: bool = False) -> str:
@@ -65,14 +72,23 @@ def build_fortran(verbose: bool = False) -> str:
     and the Fortran test driver of the Weibel time loop."""
     fc = os.environ.get("FC", "amdflang")
     os.makedirs(FMOD, exist_ok=True)
+    os.makedirs(BUILD, exist_ok=True)
     srcs = [os.path.join(FORTRAN, f) for f in FSRC]
+    cobjs = []
+    for c in FCSRC:
+        o = os.path.join(BUILD, os.path.basename(c)[:-2] + ".o")
+        _run([os.environ.get("CC", "gcc"), "-O2", "-fPIC", "-c", os.path.join(FORTRAN, c), "-o", o])
+        cobjs.append(o)
     tmp = FLIB + ".tmp"
-    _run([fc, "-O2", "-fPIC", "-shared", "-module-dir", FMOD, *srcs, "-o", tmp,
+    _run([fc, "-O2", "-fPIC", "-shared", "-module-dir", FMOD, *srcs, *cobjs, "-o", tmp,
           f"-L{HERE}", "-lwmpic", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FLIB)
     _run([fc, "-O2", "-I", FMOD, DRIVER_SRC, "-o", DRIVER + ".tmp", f"-L{HERE}", "-lwmpic_f",
           "-lwmpic", "-Wl,-rpath,$ORIGIN/../../wumingpic2d_amd"])
     os.replace(DRIVER + ".tmp", DRIVER)
+    _run([fc, "-O2", "-I", FMOD, JSON_SRC, "-o", JSON_X + ".tmp", f"-L{HERE}", "-lwmpic_f",
+          "-lwmpic", "-Wl,-rpath,$ORIGIN/../../wumingpic2d_amd"])
+    os.replace(JSON_X + ".tmp", JSON_X)
     if verbose:
         print(f"built {FLIB} and {DRIVER}")
     return FLIB

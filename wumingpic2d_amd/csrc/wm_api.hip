@@ -259,7 +259,10 @@ int download_particles(wm_ctx *ctx, double *up, const int *cnt_host, int b, int 
 // in batches of segments of at most WM_STAGE_ELEMS doubles (at least one
 // segment), so that no dense copy of the state is kept on the device.
 #define WM_STAGE_ELEMS ((size_t)1 << 26)
-int batch_end(const wm_ctx *ctx, const int *cnt, int s0, int &w) {
+}  // namespace
+
+int wm_batch_end(const wm_ctx *ctx, const int *cnt, int s0, int *wout) {
+  int &w = *wout;
   const int nseg = nseg_of(ctx);
   w = std::max(1, cnt[s0]);
   int s1 = s0 + 1;
@@ -272,6 +275,7 @@ int batch_end(const wm_ctx *ctx, const int *cnt, int s0, int &w) {
   return s1;
 }
 
+namespace {
 int upload_bins(wm_ctx *ctx, const double *up, const int *cnt_host) {
   const wm_grid &g = ctx->g;
   const int nseg = nseg_of(ctx);
@@ -283,7 +287,7 @@ int upload_bins(wm_ctx *ctx, const double *up, const int *cnt_host) {
     }
   for (int s0 = 0; s0 < nseg;) {
     int w = 1;
-    const int s1 = batch_end(ctx, cnt_host, s0, w);
+    const int s1 = wm_batch_end(ctx, cnt_host, s0, &w);
     const size_t elems = (size_t)(s1 - s0) * w * g.ndim;
     WM_TRY(ensure_stage(ctx, elems));
     std::vector<double> h(elems);
@@ -313,7 +317,7 @@ int download_bins(wm_ctx *ctx, double *up, const int *cnt_host) {
     }
   for (int s0 = 0; s0 < nseg;) {
     int w = 1;
-    const int s1 = batch_end(ctx, cnt_host, s0, w);
+    const int s1 = wm_batch_end(ctx, cnt_host, s0, &w);
     const size_t elems = (size_t)(s1 - s0) * w * g.ndim;
     WM_TRY(ensure_stage(ctx, elems));
     WM_TRY(wm_k_bins_to_stage(ctx, s0, s1 - s0, w));
@@ -427,6 +431,91 @@ int check_bounds(wm_ctx *ctx, int nxs, int nxe) {
 
 }  // namespace
 
+// ------------------------------------------ the state row by row (restart I/O)
+int wm_state_np2(wm_ctx *ctx, int *np2_host) {
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(wm_settle_movers(ctx));
+    WM_TRY(wm_k_bin_rows(ctx));
+  }
+  WM_HIP(hipMemcpyAsync(np2_host, ctx->np2, sizeof(int) * nseg_of(ctx), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return wm_check_derr(ctx, "state rows");
+}
+
+int wm_rows_out(wm_ctx *ctx, int s0, int s1, int w, double *h) {
+  const wm_grid &g = ctx->g;
+  const size_t elems = (size_t)(s1 - s0) * w * g.ndim;
+  WM_TRY(ensure_stage(ctx, elems));
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(wm_k_bins_to_stage(ctx, s0, s1 - s0, w));
+  } else {
+    const size_t off = (size_t)s0 * ctx->cap;
+    const PBank &A = ctx->bank[0];
+    const PBank B{A.x + off, A.y + off, A.ux + off, A.uy + off, A.uz + off};
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(std::min(64, wm_cdiv(w, 256)), s1 - s0), dim3(256), 0,
+                       ctx->stream, g.ndim, w, ctx->cap, ctx->np2 + s0, ctx->stage, B,
+                       ctx->id[ctx->idcur] + off);
+    WM_HIP(hipGetLastError());
+  }
+  WM_HIP(hipMemcpyAsync(h, ctx->stage, sizeof(double) * elems, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int wm_rows_in(wm_ctx *ctx, int s0, int s1, int w, const int *cnt_host, const double *h) {
+  const wm_grid &g = ctx->g;
+  const size_t elems = (size_t)(s1 - s0) * w * g.ndim;
+  WM_TRY(ensure_stage(ctx, elems));
+  WM_HIP(hipMemcpyAsync(ctx->stage, h, sizeof(double) * elems, hipMemcpyHostToDevice, ctx->stream));
+  WM_HIP(hipMemcpyAsync(ctx->cnt_scratch, cnt_host + s0, sizeof(int) * (s1 - s0),
+                        hipMemcpyHostToDevice, ctx->stream));
+  if (ctx->form == WM_FORM_FP32) {
+    WM_TRY(wm_k_stage_to_bins(ctx, s0, s1 - s0, w));
+  } else {
+    const size_t off = (size_t)s0 * ctx->cap;
+    const PBank &A = ctx->bank[1];
+    const PBank B{A.x + off, A.y + off, A.ux + off, A.uy + off, A.uz + off};
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(std::min(64, wm_cdiv(w, 256)), s1 - s0), dim3(256), 0,
+                       ctx->stream, g.ndim, w, ctx->cap, ctx->cnt_scratch, ctx->stage, B,
+                       ctx->id[ctx->idcur] + off);
+    WM_HIP(hipGetLastError());
+  }
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+int wm_rows_begin(wm_ctx *ctx) {
+  WM_TRY(wm_alloc_device(ctx));
+  if (ctx->form == WM_FORM_FP32) WM_TRY(fresh_bins(ctx));
+  // a restarted run begins as a fresh process does: the CG warm start df is
+  // zero (field.f90:105-119)
+  WM_TRY(fill_zero(ctx, ctx->df, 6 * ctx->fg.ps));
+  ctx->x0_fresh = false;
+  return WM_OK;
+}
+
+int wm_rows_done(wm_ctx *ctx, const int *np2_host) {
+  if (ctx->form == WM_FORM_FP64) {
+    // the rows as read (bank 1), sorted by cell into bank 0 with cumcnt: the
+    // reference's io__input then sort__bucket (proj/weibel/app.f90:351-352)
+    WM_HIP(hipMemcpyAsync(ctx->np2, np2_host, sizeof(int) * nseg_of(ctx), hipMemcpyHostToDevice,
+                          ctx->stream));
+    WM_TRY(wm_k_sort(ctx, 1, 0));
+  }
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  WM_TRY(wm_check_derr(ctx, "state rows"));
+  ctx->resident = true;
+  return WM_OK;
+}
+
+int wm_field_out(wm_ctx *ctx, double *uf) { return download_field(ctx, uf, 6, ctx->uf); }
+int wm_field_in(wm_ctx *ctx, const double *uf) {
+  WM_TRY(upload_field(ctx, uf, 6, ctx->uf));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
 // ------------------------------------------------------------- allocation
 int wm_alloc_device(wm_ctx *ctx) {
   if (ctx->alloc_done) return WM_OK;
@@ -466,6 +555,15 @@ int wm_alloc_device(wm_ctx *ctx) {
       WM_TRY(dmalloc(ctx, &ctx->mfill[b], g.ny));
       WM_HIP(hipMemsetAsync(ctx->mfill[b], 0, sizeof(int) * g.ny, ctx->stream));
     }
+    // the spill of the push's mover lists: a sixteenth of a row's slots
+    // (block indices of the lists are 16-bit: at most 65535 - 60 spill blocks per row)
+    ctx->srow = (int)std::min(16LL * 65000, (std::max(4096LL, (long long)g.nsp * ctx->cap / 16) + 15) / 16 * 16);
+    const size_t sn = (size_t)g.ny * ctx->srow;
+    WM_TRY(dmalloc(ctx, &ctx->spill_l, sn));
+    WM_TRY(dmalloc(ctx, &ctx->spill_p, sn));
+    WM_TRY(dmalloc(ctx, &ctx->spill_q, sn));
+    WM_TRY(dmalloc(ctx, &ctx->sfill, g.ny));
+    WM_HIP(hipMemsetAsync(ctx->sfill, 0, sizeof(int) * g.ny, ctx->stream));
     WM_TRY(dmalloc(ctx, &ctx->rowpend, g.ny));
     WM_TRY(dmalloc(ctx, &ctx->rowflag, g.ny));
     WM_TRY(dmalloc(ctx, &ctx->fstat, WM_NSTAT));
@@ -477,6 +575,7 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(dmalloc(ctx, &ctx->rtmp.uz, rslots));
     if (g.ndim >= 6) WM_TRY(dmalloc(ctx, &ctx->rtmp_id, rslots));
     WM_TRY(dmalloc(ctx, &ctx->rtmp_bstart, (size_t)ctx->relay_rows * g.nsp * (g.nx + 1)));
+    WM_TRY(dmalloc(ctx, &ctx->rtmp_blk, (size_t)ctx->relay_rows * g.nsp * ((g.nx + 63) / 64)));
     WM_TRY(dmalloc(ctx, &ctx->relay_list, ctx->relay_rows));
     WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
     WM_HIP(hipMemsetAsync(ctx->bstart, 0, sizeof(int) * (size_t)nseg * (g.nx + 1), ctx->stream));
@@ -630,7 +729,8 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->pc_sc, ctx->fcg_s, ctx->mom, ctx->mom_part, ctx->bins.q, ctx->bins.uz,
                   ctx->bstart, ctx->pcnt, ctx->mfill[0], ctx->mfill[1], ctx->rowpend, ctx->rowflag,
                   ctx->fstat, ctx->rtmp.q, ctx->rtmp.uz, ctx->rtmp_id, ctx->rtmp_bstart,
-                  ctx->relay_list};
+                  ctx->relay_list, ctx->spill_l, ctx->spill_p, ctx->spill_q, ctx->sfill,
+                  ctx->rtmp_blk};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   wm_pc_free(ctx);
@@ -1192,9 +1292,9 @@ int wm_capacity_stats(wm_ctx *ctx, long long *out, int n) {
   int st[WM_NSTAT];
   WM_HIP(hipMemcpyAsync(st, ctx->fstat, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
-  const long long v[8] = {st[0], st[1], st[2], st[3], ctx->relayouts, ctx->cap, ctx->mrow,
-                          wm_push32_pool()};
-  const int m = std::min(n, 8);
+  const long long v[11] = {st[0], st[1], st[2], st[3], st[4], ctx->relayouts, ctx->cap,
+                           ctx->mrow, wm_push32_pool(), ctx->srow, st[5]};
+  const int m = std::min(n, 11);
   for (int k = 0; k < m; ++k) out[k] = v[k];
   return m;
 }

@@ -846,34 +846,46 @@ int conv_dw(wm_ctx *ctx, wm_pc *pc, int l, const bf16 *X, const bf16 *G, int H, 
   return WM_OK;
 }
 
-// Forward pass over three images r (stride rstride) with scales sc (device,
-// s[3] then 1/s[3]); z gets s * y (stride zstride), y (optional) the net output.
-int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const double *sc, double *z,
-            size_t zstride, float *y, const double *q = nullptr, double *dots = nullptr,
-            const FcgScal *S = nullptr) {
-  const int H = pc->H, W = pc->W, B = wm_pc::B;
-  const float *th = pc->theta;
+// The activations of one forward pass (BF16, NHWC, the three images batched).
+struct Acts {
+  bf16 *d1, *a1, *d2, *a2, *d3, *a3, *u2, *b2, *u1, *b1;
+};
+Acts acts_of(const wm_pc *pc) {
+  return Acts{pc->d1, pc->a1, pc->d2, pc->a2, pc->d3, pc->a3, pc->u2, pc->b2, pc->u1, pc->b1};
+}
+
+// Forward pass over three H x W images r (stride rstride) with scales sc
+// (device, s[3] then 1/s[3]), activations in X; z gets s * y (stride
+// zstride), y (optional) the net output.
+int forward_on(wm_ctx *ctx, const wm_pc *pc, const Acts &X, int H, int W, const double *r,
+               size_t rstride, const double *sc, double *z, size_t zstride, float *y,
+               const double *q = nullptr, double *dots = nullptr, const FcgScal *S = nullptr) {
+  const int B = wm_pc::B;
   const bf16 *wb = pc->wb;
   hipStream_t st = ctx->stream;
   const FcgScal *gate = dots ? S : nullptr;
   hipLaunchKernelGGL(k_lift_pool, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
-                     pc->d1, H, W, B, gate);
+                     X.d1, H, W, B, gate);
   // d2 = pool(a1) and d3 = pool(a2) leave the D1 and D2 convolutions with a1, a2
-  WM_TRY((conv<C0, C1, false>(ctx, pc->d1, wb + loff(L_D1), pc->a1, H >> 1, W >> 1, gate, pc->d2)));
-  WM_TRY((conv<C1, C2, false>(ctx, pc->d2, wb + loff(L_D2), pc->a2, H >> 2, W >> 2, gate, pc->d3)));
-  WM_TRY((conv<C2, C2, false>(ctx, pc->d3, wb + loff(L_D3), pc->a3, H >> 3, W >> 3, gate)));
+  WM_TRY((conv<C0, C1, false>(ctx, X.d1, wb + loff(L_D1), X.a1, H >> 1, W >> 1, gate, X.d2)));
+  WM_TRY((conv<C1, C2, false>(ctx, X.d2, wb + loff(L_D2), X.a2, H >> 2, W >> 2, gate, X.d3)));
+  WM_TRY((conv<C2, C2, false>(ctx, X.d3, wb + loff(L_D3), X.a3, H >> 3, W >> 3, gate)));
   // u2 = a2 + up(a3) and u1 = a1 + up(b2) are formed inside the U2/U1
   // convolutions; training keeps them for the weight gradients
   const bool keep = y != nullptr;
-  WM_TRY((conv_up<C2, C1>(ctx, pc->a2, pc->a3, keep ? pc->u2 : nullptr, wb + loff(L_U2), pc->b2,
-                          H >> 2, W >> 2, gate)));
-  WM_TRY((conv_up<C1, C0>(ctx, pc->a1, pc->b2, keep ? pc->u1 : nullptr, wb + loff(L_U1), pc->b1,
-                          H >> 1, W >> 1, gate)));
-  hipLaunchKernelGGL(k_up_proj, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold,
-                     pc->b1, z, zstride, y, H, W, B, q, dots, S);
-  (void)th;
+  WM_TRY((conv_up<C2, C1>(ctx, X.a2, X.a3, keep ? X.u2 : nullptr, wb + loff(L_U2), X.b2, H >> 2,
+                          W >> 2, gate)));
+  WM_TRY((conv_up<C1, C0>(ctx, X.a1, X.b2, keep ? X.u1 : nullptr, wb + loff(L_U1), X.b1, H >> 1,
+                          W >> 1, gate)));
+  hipLaunchKernelGGL(k_up_proj, tiles(H, W, B), dim3(256), 0, st, r, rstride, sc, pc->fold, X.b1,
+                     z, zstride, y, H, W, B, q, dots, S);
   WM_HIP(hipGetLastError());
   return WM_OK;
+}
+int forward(wm_ctx *ctx, wm_pc *pc, const double *r, size_t rstride, const double *sc, double *z,
+            size_t zstride, float *y, const double *q = nullptr, double *dots = nullptr,
+            const FcgScal *S = nullptr) {
+  return forward_on(ctx, pc, acts_of(pc), pc->H, pc->W, r, rstride, sc, z, zstride, y, q, dots, S);
 }
 
 // Loss and gradient (into pc->grad) of the pair (r, e) with scales sc;

@@ -1,17 +1,19 @@
 ! Drop-in module `mpi_set` (the public interface of common/mpi_set.f90:7-19)
 ! for one process that holds the whole y range on one device.  The MPI
-! handles it publishes are plain integers that the device library ignores.
+! handles it publishes are those of the single-rank module mpi, which the
+! apps pass to their own MPI calls; the device library does not use them.
 module mpi_set
   use iso_c_binding
   use wmpic_capi
+  use mpi, only: MPI_INTEGER, MPI_DOUBLE_PRECISION, MPI_CHARACTER, MPI_SUM, MPI_WTIME
   implicit none
   private
-  public :: mpi_set__init
+  public :: mpi_set__init, MPI_WTIME
 
-  integer, public, parameter   :: mnpi  = 0
-  integer, public, parameter   :: mnpr  = 0
-  integer, public, parameter   :: mnpc  = 0
-  integer, public, parameter   :: opsum = 0
+  integer, public, parameter   :: mnpi  = MPI_INTEGER
+  integer, public, parameter   :: mnpr  = MPI_DOUBLE_PRECISION
+  integer, public, parameter   :: mnpc  = MPI_CHARACTER
+  integer, public, parameter   :: opsum = MPI_SUM
   integer, public              :: nerr = 0, ncomw = 0, nsize = 1, nrank = 0
   integer, public              :: nstat(1) = 0
   integer, public              :: nys, nye, nup, ndown

@@ -16,6 +16,8 @@ module wmpic_capi
   public :: wm_bcp_particle_y, wm_bcp_dfield, wm_bcp_curre, wm_bcp_phi
   public :: wm_mpi_set_init, wm_set_capacity, WM_BC_PERIODIC
   public :: wm_mom_calc_init, wm_mom_calc_accl, wm_mom_calc_nvt, wm_bcp_mom
+  public :: wm_paraio_init, wm_paraio_output, wm_paraio_input, wm_paraio_param, wm_paraio_mom
+  public :: wm_paraio_ptcl, wm_paraio_orb
 
   integer(c_int), parameter :: WM_BC_PERIODIC = 0
   type(c_ptr), save :: wm_ctx = c_null_ptr
@@ -158,6 +160,62 @@ module wmpic_capi
        real(c_double), intent(in) :: up(*)
        integer(c_int), intent(in) :: np2(*)
      end function wm_mom_calc_nvt
+     integer(c_int) function wm_paraio_init(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, &
+          nproc, nrank, delx, delt, c, q, r, dir) bind(C, name="wm_paraio_init")
+       import :: c_ptr, c_int, c_double, c_char
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nproc, nrank
+       real(c_double), value :: delx, delt, c
+       real(c_double), intent(in) :: q(*), r(*)
+       character(kind=c_char), intent(in) :: dir(*)
+     end function wm_paraio_init
+     integer(c_int) function wm_paraio_output(ctx, up, uf, np2, nxs, nxe, it, filename) &
+          bind(C, name="wm_paraio_output")
+       import :: c_ptr, c_int, c_double, c_char
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: nxs, nxe, it
+       character(kind=c_char), intent(in) :: filename(*)
+     end function wm_paraio_output
+     integer(c_int) function wm_paraio_input(ctx, up, uf, np2, nxs, nxe, it, filename) &
+          bind(C, name="wm_paraio_input")
+       import :: c_ptr, c_int, c_double, c_char
+       type(c_ptr), value :: ctx
+       real(c_double) :: up(*), uf(*)
+       integer(c_int) :: np2(*)
+       integer(c_int), intent(out) :: nxs, nxe, it
+       character(kind=c_char), intent(in) :: filename(*)
+     end function wm_paraio_input
+     integer(c_int) function wm_paraio_param(ctx, n0, wpe, wpi, wge, wgi, vti, vte, filename) &
+          bind(C, name="wm_paraio_param")
+       import :: c_ptr, c_int, c_double, c_char
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n0
+       real(c_double), value :: wpe, wpi, wge, wgi, vti, vte
+       character(kind=c_char), intent(in) :: filename(*)
+     end function wm_paraio_param
+     integer(c_int) function wm_paraio_mom(ctx, mom, uf, it) bind(C, name="wm_paraio_mom")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: mom(*)
+       real(c_double), intent(in) :: uf(*)
+       integer(c_int), value :: it
+     end function wm_paraio_mom
+     integer(c_int) function wm_paraio_ptcl(ctx, up, uf, np2, it) bind(C, name="wm_paraio_ptcl")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: it
+     end function wm_paraio_ptcl
+     integer(c_int) function wm_paraio_orb(ctx, up, uf, np2, it) bind(C, name="wm_paraio_orb")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: up(*), uf(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: it
+     end function wm_paraio_orb
      integer(c_int) function wm_bcp_mom(ctx, mom) bind(C, name="wm_bcp_mom")
        import :: c_ptr, c_int, c_double
        type(c_ptr), value :: ctx
