@@ -38,10 +38,15 @@ METRIC = "particle-steps/s (push+deposit+sort) & implicit-solve ms: FP32+learned
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    # name: (nx, ny, ppc per species, capacity factor, full run length)
-    "weibel_2048": (2048, 2048, 64, 1.5, 2000),
+    # name: (nx, ny, ppc per species, capacity factor, full run length).
+    # Capacity: slots per (species, row) = factor x ppc x nx.  The reference
+    # sizes rows at 5 x (proj/weibel/app.f90:248); 2 x leaves a cell's bin
+    # room for about 6 standard deviations of its count (Poisson, 64 ppc)
+    # once the bins are rounded to 8 slots, so bins overflow (and are laid
+    # out anew) rarely even where the nonlinear phase moves particles fast.
+    "weibel_2048": (2048, 2048, 64, 2.0, 2000),
     "weibel_128": (128, 128, 64, 2.0, 200),
-    "weibel_1024": (1024, 1024, 64, 1.5, 2000),
+    "weibel_1024": (1024, 1024, 64, 2.0, 2000),
 }
 EVERY = 10            # steps between energy-history entries
 FIT = (60, 160)       # linear-phase window (steps) of the growth-rate fit
@@ -358,6 +363,8 @@ def full_run_line(head: dict, ref: dict | None, dt: float) -> dict:
            "capacity": head["caps"]}
     if head["late"]:
         out["late_ms_per_step"] = {"fp32_fcg": round(head["late"]["ms_per_step"], 3)}
+        out["late_phase_ms_per_step"] = {
+            "fp32_fcg": {n: round(float(v), 4) for n, v in zip(PHASES, head["late"]["phase"])}}
     if head["plain"] is not None:
         out["pc_training_amortized_ms_per_step"] = round(
             head["pc_training"]["seconds"] * 1e3 / max(out["steps"], 1), 4)
@@ -375,6 +382,8 @@ def full_run_line(head: dict, ref: dict | None, dt: float) -> dict:
         out["capacity_fp64"] = ref["caps"]
         if ref["late"]:
             out["late_ms_per_step"]["fp64_cg"] = round(ref["late"]["ms_per_step"], 3)
+            out["late_phase_ms_per_step"]["fp64_cg"] = {
+                n: round(float(v), 4) for n, v in zip(PHASES, ref["late"]["phase"])}
     return out
 
 
@@ -457,6 +466,7 @@ def main() -> int:
                                f"{args.form.upper()} form, {solver.upper()} solve",
                    "nx": nx, "ny": ny, "ppc_per_species": ppc, "species": 2, "particles": npart,
                    "dt": p["delt"], "form": args.form, "solver": solver,
+                   "slots_per_row": int(WORKLOADS[args.workload][3] * ppc * nx),
                    "parallelism": "single GPU"},
         "solve_ms_per_step": hl["solve_ms_per_step"],
         "cg_iters_per_step": hl["cg_iters_per_step"],

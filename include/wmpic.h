@@ -63,6 +63,8 @@ extern "C" {
 
 /* Boundary families for wm_field_fdtd_i's set_boundary_* callbacks. */
 #define WM_BC_PERIODIC 0    /* boundary_periodic__dfield/__curre/__phi */
+#define WM_BC_SHOCK 1       /* boundary_shock__*: conducting wall at nxs, open edge at nxe */
+#define WM_BC_RECONNECTION 2 /* boundary_reconnection__*: conducting walls at nxs, nxe */
 
 typedef struct wm_ctx wm_ctx;
 
@@ -110,6 +112,19 @@ int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
                 int nygs, int nyge, int nys, int nye, int nup, int ndown,
                 int mnpi, int mnpr, int ncomw, int nerr, const int *nstat,
                 int nstat_len, double delx, double delt, double c);
+/* boundary_shock__init (proj/shock/boundary_shock.f90:26-58) and
+ * boundary_reconnection__init (proj/reconnection/boundary_reconnection.f90:
+ * 26-58): the same arguments; they select the wall family, whose host-array
+ * calls take an active x range nxs..nxe inside nxgs..nxge.  A context takes
+ * one family. */
+int wm_bcs_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
+                int nygs, int nyge, int nys, int nye, int nup, int ndown,
+                int mnpi, int mnpr, int ncomw, int nerr, const int *nstat,
+                int nstat_len, double delx, double delt, double c);
+int wm_bcr_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
+                int nygs, int nyge, int nys, int nye, int nup, int ndown,
+                int mnpi, int mnpr, int ncomw, int nerr, const int *nstat,
+                int nstat_len, double delx, double delt, double c);
 
 /* ---- host-array entry points (reference call semantics) ---- */
 /* particle__solv(gp,up,uf,cumcnt,nxs,nxe) (particle.f90:48-177) */
@@ -139,6 +154,46 @@ int wm_bcp_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye,
 /* boundary_periodic__phi(phi,nxs,nxe,nys,nye,l) (boundary_periodic.f90:511-568) */
 int wm_bcp_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye,
                int l);
+
+/* ---- the shock family (proj/shock/boundary_shock.f90) ----
+ * particle_x (:62-100, the reference's signature is (up,np2,nxs,nxe)):
+ * mirrors about (nxs+1) delx and (nxe-1) delx with the momentum reversed.
+ * injection (:255-298): the left mirror, and beyond xend = nxe delx +
+ * u0/sqrt(1+u0^2/c^2) delt x mirrors about xend and u_x about u0.
+ * particle_y (:103-252): the periodic one.  dfield (:300-407), curre
+ * (:410-548), phi (:551-625): y periodic; a conducting wall at nxs (Bx, Ey,
+ * Ez odd about it, By, Bz, Ex even about column nxs), zero beyond nxe, no x
+ * fold of the current.  mom (:628-693): x ghosts fold into their own edge. */
+int wm_bcs_particle_x(wm_ctx *ctx, double *up, const int *np2, int nxs, int nxe);
+int wm_bcs_injection(wm_ctx *ctx, double *up, const int *np2, int nxs, int nxe,
+                     double u0);
+int wm_bcs_particle_y(wm_ctx *ctx, double *up, int *np2);
+int wm_bcs_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye,
+                  int nxgs, int nxge);
+int wm_bcs_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye,
+                 int nxgs, int nxge);
+int wm_bcs_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye,
+               int l);
+int wm_bcs_mom(wm_ctx *ctx, double *mom);
+/* ---- the reconnection family (proj/reconnection/boundary_reconnection.f90)
+ * particle_x (:61-99), particle_y (:102-251), dfield (:254-361; the right
+ * wall mirrors the left one about nxe - 1/2 and rewrites column nxe of Bx,
+ * Ey, Ez), curre (:364-502), phi (:505-579), mom (:582-647). */
+int wm_bcr_particle_x(wm_ctx *ctx, double *up, const int *np2, int nxs, int nxe);
+int wm_bcr_particle_y(wm_ctx *ctx, double *up, int *np2);
+int wm_bcr_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye,
+                  int nxgs, int nxge);
+int wm_bcr_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye,
+                 int nxgs, int nxge);
+int wm_bcr_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye,
+               int l);
+int wm_bcr_mom(wm_ctx *ctx, double *mom);
+/* Device-resident runs of a wall family: the active range nxs..nxe and the
+ * shock's u0, the arguments its loop passes every step
+ * (proj/shock/app.f90:112-117).  No reference counterpart (the reference
+ * passes them per call).  The wall families run in the FP64 form, with the
+ * reference's plain CG. */
+int wm_set_walls(wm_ctx *ctx, int nxs, int nxe, double u0);
 
 /* ---- device-resident state and step ---- */
 /* Ingest the reference's state (as after proj/weibel/app.f90:362 gp = up):
@@ -179,11 +234,12 @@ int wm_particle_count(wm_ctx *ctx, long long *count);
  * one cell bin, [1] most particles leaving one bin in a push, [2] most
  * movers of one push workgroup, [3] most mover records of one row in a
  * push, [4] most entries of one row's spill of the push's mover lists, [5]
- * rows whose bins were laid out anew (a bin would have overflowed), [6]
- * slots per (species, row) segment, [7] mover records per row, [8] movers
- * one push workgroup lists in LDS, [9] spill entries per row, [10] bins found
- * full (each re-laid with its block of 64 cells).  Returns the number
- * written (at most n, at most 11). */
+ * blocks of 64 cells whose bins were laid out anew (a bin would have
+ * overflowed), [6] slots per (species, row) segment, [7] mover records per
+ * row, [8] movers one push workgroup lists in LDS, [9] spill entries per
+ * row, [10] bins found full, [11] segments laid out anew as a whole (a
+ * block's slots could not hold its bins).  Returns the number written (at
+ * most n, at most 12). */
 int wm_capacity_stats(wm_ctx *ctx, long long *out, int n);
 
 /* ---- moment diagnostics of an output step (proj/weibel/app.f90:119-121)

@@ -33,7 +33,11 @@ class OrParams(ctypes.Structure):
                 ("nyge", ctypes.c_int), ("nys", ctypes.c_int), ("nye", ctypes.c_int),
                 ("delx", ctypes.c_double), ("delt", ctypes.c_double), ("c", ctypes.c_double),
                 ("gfac", ctypes.c_double), ("q", ctypes.c_double * MAXSP),
-                ("r", ctypes.c_double * MAXSP)]
+                ("r", ctypes.c_double * MAXSP), ("nxs", ctypes.c_int), ("nxe", ctypes.c_int),
+                ("bc", ctypes.c_int), ("u0", ctypes.c_double)]
+
+
+BC_PERIODIC, BC_SHOCK, BC_RECONNECTION = 0, 1, 2
 
 
 def _p(a):
@@ -70,6 +74,8 @@ class Oracle:
         L.or_add_rd.argtypes = [ctypes.c_double, ctypes.c_double]
         L.or_cgm.argtypes = [vp, vp, vp, ctypes.c_double, vp]
         L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_bc_phi_l.argtypes = [vp, vp, ctypes.c_int]
+        L.or_bc_injection.argtypes = [vp, vp, vp, ctypes.c_double]
         for n, k in (("or_particle_solv", 5), ("or_ele_cur", 5), ("or_bc_curre", 2),
                      ("or_bc_dfield", 2), ("or_bc_phi", 2), ("or_field_rhs", 4),
                      ("or_field_fdtd_i", 6), ("or_field_advance", 4), ("or_bc_particle_x", 3),
@@ -86,6 +92,11 @@ class Oracle:
         for s in range(p["nsp"]):
             self.par.q[s] = float(p["q"][s])
             self.par.r[s] = float(p["r"][s])
+        # boundary family and active x range (the whole grid unless given)
+        self.par.bc = int(p.get("bc", BC_PERIODIC))
+        self.par.nxs = int(p.get("nxs", p["nxgs"]))
+        self.par.nxe = int(p.get("nxe", p["nxge"]))
+        self.par.u0 = float(p.get("u0", 0.0))
         self.pp = ctypes.addressof(self.par)
         self.state = L.or_state_new(self.pp)
         if not self.state:
@@ -118,8 +129,16 @@ class Oracle:
     def bc_dfield(self, df):
         self.lib.or_bc_dfield(self.pp, _p(df))
 
-    def bc_phi(self, phi):
-        self.lib.or_bc_phi(self.pp, _p(phi))
+    def bc_phi(self, phi, l: int = 1):
+        self.lib.or_bc_phi_l(self.pp, _p(phi), int(l))
+
+    def bc_injection(self, up, np2, u0: float):
+        self._ck(self.lib.or_bc_injection(self.pp, _p(up), _p(np2), ctypes.c_double(u0)),
+                 "injection")
+
+    def set_active_x(self, nxs: int, nxe: int):
+        """The active x range [nxs, nxe] (the shock's box grows by relocate)."""
+        self.par.nxs, self.par.nxe = int(nxs), int(nxe)
 
     def field_rhs(self, gkl, uf, uj):
         self.lib.or_field_rhs(self.pp, _p(gkl), _p(uf), _p(uj))

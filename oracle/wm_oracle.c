@@ -74,6 +74,9 @@ static size_t n_uf(const or_params *p) {
 static size_t n_uj(const or_params *p) {
   return 3u * (size_t)(NXg(p) + 4) * (size_t)(NYl(p) + 4);
 }
+/* The active x range [nxs, nxe] (or_params; nxe = 0: nxgs..nxge). */
+static inline int XS(const or_params *p) { return p->nxe > 0 ? p->nxs : p->nxgs; }
+static inline int XE(const or_params *p) { return p->nxe > 0 ? p->nxe : p->nxge; }
 
 /* ----------------------------------------------------------------- state */
 struct or_state {
@@ -136,7 +139,7 @@ double or_add_rd(double a, double b) {
 /* ------------------------------------------------------- particle__solv */
 int or_particle_solv(const or_params *p, double *gp, const double *up,
                      const double *uf, const int *cumcnt) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   const int nxt = nxe - nxs + 3, nyt = nye - nys + 3;
   const double d_delx = 1. / p->delx;           /* particle.f90:41 */
   const double delt = p->delt, c = p->c;
@@ -239,7 +242,7 @@ int or_particle_solv(const or_params *p, double *gp, const double *up,
 /* Deposit of rows j0..j1 into uj (field.f90:212-313, one thread's share). */
 static int ele_cur_rows(const or_params *p, double *uj, const double *up,
                         const double *gp, const int *cumcnt, int j0, int j1) {
-  const int nxs = p->nxgs, nxe = p->nxge;
+  const int nxs = XS(p), nxe = XE(p);
   const double d_delx = 1. / p->delx, d_delt = 1. / p->delt;
   const double delx = p->delx, c = p->c;
   const double fac = 1.0 / 3.0; /* field.f90:198 */
@@ -380,7 +383,7 @@ int or_ele_cur(const or_params *p, double *uj, const double *up,
 /* ------------------------------------------------- boundary_periodic__* */
 void or_bc_curre(const or_params *p, double *uj) {
   /* boundary_periodic.f90:357-508 with nup = ndown = self. */
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   const int nb = nxe - nxs + 5;
   double *snd = (double *)malloc(sizeof(double) * 6u * (size_t)nb);
   double *rcv = (double *)malloc(sizeof(double) * 6u * (size_t)nb);
@@ -449,7 +452,14 @@ void or_bc_curre(const or_params *p, double *uj) {
       UJ(k, i, nys - 1) = rcv[ii + 3 + k - 1];
     }
   }
-  /* columns :495-506 (whole-array statements, executed in order) */
+  /* columns :495-506 (whole-array statements, executed in order); the wall
+   * families fold nothing in x (boundary_shock.f90:410-548,
+   * boundary_reconnection.f90:364-502 end after the rows) */
+  if (p->bc != OR_BC_PERIODIC) {
+    free(snd);
+    free(rcv);
+    return;
+  }
   for (int j = nys - 2; j <= nye + 2; ++j)
     for (int k = 1; k <= 3; ++k) UJ(k, nxe - 1, j) = UJ(k, nxe - 1, j) + UJ(k, nxs - 2, j);
   for (int j = nys - 2; j <= nye + 2; ++j)
@@ -473,7 +483,7 @@ void or_bc_curre(const or_params *p, double *uj) {
 
 void or_bc_dfield(const or_params *p, double *df) {
   /* boundary_periodic.f90:251-354 with nup = ndown = self. */
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
 #define DF(k, i, j) df[iuf(p, k, i, j)]
   /* rows nys, nys+1 -> nye+1, nye+2 :263-302 */
   for (int i = nxs; i <= nxe; ++i)
@@ -488,6 +498,30 @@ void or_bc_dfield(const or_params *p, double *df) {
       DF(k, i, nys - 2) = a;
       DF(k, i, nys - 1) = b;
     }
+  if (p->bc == OR_BC_SHOCK) {
+    /* conducting wall at the left, zero at the right edge
+     * (boundary_shock.f90:396-405) */
+    for (int j = nys - 2; j <= nye + 2; ++j) {
+      DF(1, nxs - 1, j) = -DF(1, nxs, j);
+      for (int k = 2; k <= 4; ++k) DF(k, nxs - 1, j) = DF(k, nxs + 1, j);
+      for (int k = 5; k <= 6; ++k) DF(k, nxs - 1, j) = -DF(k, nxs, j);
+      for (int k = 1; k <= 6; ++k) DF(k, nxe + 1, j) = 0.0;
+    }
+    return;
+  }
+  if (p->bc == OR_BC_RECONNECTION) {
+    /* conducting walls on both sides; the right one rewrites the last
+     * interior column of Bx, Ey, Ez (boundary_reconnection.f90:350-359) */
+    for (int j = nys - 2; j <= nye + 2; ++j) {
+      DF(1, nxs - 1, j) = -DF(1, nxs, j);
+      for (int k = 2; k <= 4; ++k) DF(k, nxs - 1, j) = DF(k, nxs + 1, j);
+      for (int k = 5; k <= 6; ++k) DF(k, nxs - 1, j) = -DF(k, nxs, j);
+      DF(1, nxe, j) = -DF(1, nxe - 1, j);
+      for (int k = 2; k <= 4; ++k) DF(k, nxe + 1, j) = DF(k, nxe - 1, j);
+      for (int k = 5; k <= 6; ++k) DF(k, nxe, j) = -DF(k, nxe - 1, j);
+    }
+    return;
+  }
   /* columns :347-352 */
   for (int j = nys - 2; j <= nye + 2; ++j)
     for (int k = 1; k <= 6; ++k) DF(k, nxs - 2, j) = DF(k, nxe - 1, j);
@@ -500,14 +534,25 @@ void or_bc_dfield(const or_params *p, double *df) {
 #undef DF
 }
 
-void or_bc_phi(const or_params *p, double *phi) {
-  /* boundary_periodic.f90:511-568 */
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+void or_bc_phi_l(const or_params *p, double *phi, int l) {
+  /* boundary_periodic.f90:511-568; the wall families' rows are the same
+   * (boundary_shock.f90:551-602) and their columns depend on l (:603-623,
+   * boundary_reconnection.f90:557-577): Bx odd about the wall, By and Bz
+   * even about column nxs, zero beyond nxe */
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   for (int i = nxs; i <= nxe; ++i) phi[iphi(p, i, nye + 1)] = phi[iphi(p, i, nys)];
   for (int i = nxs; i <= nxe; ++i) phi[iphi(p, i, nys - 1)] = phi[iphi(p, i, nye)];
+  if (p->bc != OR_BC_PERIODIC) {
+    for (int j = nys - 1; j <= nye + 1; ++j) {
+      phi[iphi(p, nxs - 1, j)] = l == 1 ? -phi[iphi(p, nxs, j)] : phi[iphi(p, nxs + 1, j)];
+      phi[iphi(p, nxe + 1, j)] = 0.0;
+    }
+    return;
+  }
   for (int j = nys - 1; j <= nye + 1; ++j) phi[iphi(p, nxs - 1, j)] = phi[iphi(p, nxe, j)];
   for (int j = nys - 1; j <= nye + 1; ++j) phi[iphi(p, nxe + 1, j)] = phi[iphi(p, nxs, j)];
 }
+void or_bc_phi(const or_params *p, double *phi) { or_bc_phi_l(p, phi, 1); }
 
 /* ------------------------------------------------------------ field RHS */
 static void field_consts(const or_params *p, double *f1, double *f2, double *f3,
@@ -524,7 +569,7 @@ static void field_consts(const or_params *p, double *f1, double *f2, double *f3,
 
 void or_field_rhs(const or_params *p, double *gkl, const double *uf,
                   const double *uj) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   double f1, f2, f3, f4, f5;
   field_consts(p, &f1, &f2, &f3, &f4, &f5);
   (void)f4;
@@ -556,11 +601,12 @@ void or_field_rhs(const or_params *p, double *gkl, const double *uf,
 /* ------------------------------------------------------------------ cgm */
 int or_cgm(const or_params *p, double *db, const double *gkl, double err,
            int *iters3) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   const int ite_max = 100; /* field.f90:337 */
   double f1, f2, f3, f4, f5;
   field_consts(p, &f1, &f2, &f3, &f4, &f5);
-  const size_t nphi = (size_t)(nxe - nxs + 3) * (size_t)(nye - nys + 3);
+  /* phi and p span nxgs-1..nxge+1 (iphi); the solve uses nxs-1..nxe+1 */
+  const size_t nphi = (size_t)(NXg(p) + 2) * (size_t)(nye - nys + 3);
   const size_t nin = (size_t)(nxe - nxs + 1) * (size_t)(nye - nys + 1);
   double *phi = (double *)calloc(nphi, sizeof(double));
   double *pp = (double *)calloc(nphi, sizeof(double));
@@ -583,7 +629,7 @@ int or_cgm(const or_params *p, double *db, const double *gkl, double err,
       }
     double sum_g = sum;                /* :362 (single rank) */
     const double eps = sqrt(sum_g) * err; /* :364 */
-    or_bc_phi(p, phi);                 /* :367 */
+    or_bc_phi_l(p, phi, l);            /* :367 */
     double sumr = 0.0;
     for (int j = nys; j <= nye; ++j)
       for (int i = nxs; i <= nxe; ++i) {
@@ -596,7 +642,7 @@ int or_cgm(const or_params *p, double *db, const double *gkl, double err,
     if (sqrt(sumr_g) > eps) {          /* :385 */
       while (sum_g > eps) {            /* :387 -- sum_g starts as |b|^2 */
         ite = ite + 1;
-        or_bc_phi(p, pp);              /* :392 */
+        or_bc_phi_l(p, pp, l);         /* :392 */
         sumr = 0.0;
         double sum2 = 0.0;
         for (int j = nys; j <= nye; ++j)
@@ -654,7 +700,7 @@ int or_field_fdtd_i(const or_params *p, or_state *s, double *uf,
 }
 
 int or_field_advance(const or_params *p, or_state *s, double *uf, const double *uj) {
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   double f1, f2, f3, f4, f5;
   field_consts(p, &f1, &f2, &f3, &f4, &f5);
   const double gfac = p->gfac, delt = p->delt;
@@ -694,7 +740,64 @@ int or_field_advance(const or_params *p, or_state *s, double *uf, const double *
 }
 
 /* ---------------------------------------------- particle boundary: x, y */
+/* The wall families' particle_x (boundary_shock.f90:62-100 and
+ * boundary_reconnection.f90:61-99, the same statements): a particle in a
+ * cell below nxs+1 is mirrored about x = (nxs+1) delx, one in a cell at or
+ * above nxe-1 about x = (nxe-1) delx, and its momentum reversed. */
+static void wall_x(const or_params *p, double *up, const int *np2) {
+  const int nxs = XS(p), nxe = XE(p);
+  const double delx = p->delx;
+  for (int isp = 1; isp <= p->nsp; ++isp)
+    for (int j = p->nys; j <= p->nye; ++j)
+      for (int ii = 1; ii <= np2[inp2(p, j, isp)]; ++ii) {
+        double *u = up + iup(p, 1, ii, j, isp);
+        const int ipos = (int)(u[0] / delx);
+        double w;
+        if (ipos < nxs + 1)
+          w = 2. * (nxs + 1) * delx;
+        else if (ipos >= nxe - 1)
+          w = 2. * (nxe - 1) * delx;
+        else
+          continue;
+        u[0] = w - u[0];
+        u[2] = -u[2];
+        u[3] = -u[3];
+        u[4] = -u[4];
+      }
+}
+
+int or_bc_injection(const or_params *p, double *up, const int *np2, double u0) {
+  /* boundary_shock.f90:255-298: the left wall as in particle_x; beyond the
+   * right edge xend, which moves with the upstream flow during the step,
+   * x mirrors about xend and u_x about u0 (the flow's frame) */
+  const int nxs = XS(p), nxe = XE(p);
+  const double delx = p->delx, c = p->c;
+  const double xend = nxe * delx + u0 / sqrt(1 + (u0 * u0) / (c * c)) * p->delt;
+  for (int isp = 1; isp <= p->nsp; ++isp)
+    for (int j = p->nys; j <= p->nye; ++j)
+      for (int ii = 1; ii <= np2[inp2(p, j, isp)]; ++ii) {
+        double *u = up + iup(p, 1, ii, j, isp);
+        const int ipos = (int)(u[0] / delx);
+        if (ipos < nxs + 1) {
+          u[0] = 2. * (nxs + 1) * delx - u[0];
+          u[2] = -u[2];
+          u[3] = -u[3];
+          u[4] = -u[4];
+        } else if (u[0] > xend) {
+          u[0] = 2. * xend - u[0];
+          u[2] = 2. * u0 - u[2];
+          u[3] = -u[3];
+          u[4] = -u[4];
+        }
+      }
+  return OR_OK;
+}
+
 int or_bc_particle_x(const or_params *p, double *up, const int *np2) {
+  if (p->bc != OR_BC_PERIODIC) {
+    wall_x(p, up, np2);
+    return OR_OK;
+  }
   /* boundary_periodic.f90:61-96, rounding toward -inf (:74). */
   const int nx = NXg(p);
   const double span = (double)nx * p->delx;
@@ -817,7 +920,7 @@ int or_bc_particle_y(const or_params *p, or_state *s, double *up, int *np2) {
 /* ---------------------------------------------------------- sort__bucket */
 int or_sort_bucket(const or_params *p, double *gp, const double *up,
                    int *cumcnt, const int *np2) {
-  const int nxs = p->nxgs, nxe = p->nxge, ndim = p->ndim;
+  const int nxs = XS(p), nxe = XE(p), ndim = p->ndim;
   int bad = 0;
   for (int isp = 1; isp <= p->nsp; ++isp)
 #pragma omp parallel for reduction(| : bad)
@@ -872,7 +975,7 @@ int or_mom_calc_accl(const or_params *p, double *gp, const double *up,
   /* mom_calc.f90:48-164: particle.f90's velocity update with the step that
    * mom_calc__init keeps, delt*0.5 (mom_calc.f90:34), and no move: the
    * positions are copied.  gp(6) is not written. */
-  const int nxs = p->nxgs, nxe = p->nxge, nys = p->nys, nye = p->nye;
+  const int nxs = XS(p), nxe = XE(p), nys = p->nys, nye = p->nye;
   const int nxt = nxe - nxs + 3, nyt = nye - nys + 3;
   const double d_delx = 1. / p->delx;           /* mom_calc.f90:41 */
   const double delt = p->delt * 0.5, c = p->c;  /* :34 */
@@ -1003,11 +1106,14 @@ void or_bc_mom(const or_params *p, double *mom) {
    * included (:579-584); then, per species, row nys-1 into row nye (:586-608)
    * and row nye+1 into row nys (:611-633), ghost columns included. */
   const int nxgs = p->nxgs, nxge = p->nxge, nys = p->nys, nye = p->nye;
+  const int wall = p->bc != OR_BC_PERIODIC;
+  /* the wall families fold each x ghost into its own edge
+   * (boundary_shock.f90:636-641, boundary_reconnection.f90:590-595) */
   for (int isp = 1; isp <= p->nsp; ++isp)
     for (int j = nys - 1; j <= nye + 1; ++j)
       for (int k = 1; k <= 7; ++k) {
-        mom[imom(p, k, nxgs, j, isp)] += mom[imom(p, k, nxge + 1, j, isp)];
-        mom[imom(p, k, nxge, j, isp)] += mom[imom(p, k, nxgs - 1, j, isp)];
+        mom[imom(p, k, nxgs, j, isp)] += mom[imom(p, k, wall ? nxgs - 1 : nxge + 1, j, isp)];
+        mom[imom(p, k, nxge, j, isp)] += mom[imom(p, k, wall ? nxge + 1 : nxgs - 1, j, isp)];
       }
   for (int isp = 1; isp <= p->nsp; ++isp) {
     for (int i = nxgs - 1; i <= nxge + 1; ++i)
@@ -1048,12 +1154,15 @@ void or_energy(const or_params *p, const double *up, const double *uf,
 /* ----------------------------------------------------------------- step */
 int or_step(const or_params *p, or_state *s, double *up, double *gp,
             double *uf, int *cumcnt, int *np2) {
-  /* proj/weibel/app.f90:102-106 */
+  /* proj/weibel/app.f90:102-106, proj/reconnection/app.f90:101-106; the
+   * shock's injection edge comes between the push and the current
+   * (proj/shock/app.f90:112-117) and it has no particle_x */
   int st = or_particle_solv(p, gp, up, uf, cumcnt);
   if (st) return st;
+  if (p->bc == OR_BC_SHOCK) or_bc_injection(p, gp, np2, p->u0);
   st = or_field_fdtd_i(p, s, uf, up, gp, cumcnt);
   if (st) return st;
-  st = or_bc_particle_x(p, gp, np2);
+  if (p->bc != OR_BC_SHOCK) st = or_bc_particle_x(p, gp, np2);
   if (st) return st;
   st = or_bc_particle_y(p, s, gp, np2);
   if (st) return st;

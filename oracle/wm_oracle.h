@@ -69,7 +69,17 @@ typedef struct {
   int nxgs, nxge, nygs, nyge, nys, nye;
   double delx, delt, c, gfac;
   double q[OR_MAXSP], r[OR_MAXSP];
+  /* The active x range [nxs, nxe] the reference passes to particle__solv,
+   * field__fdtd_i, sort__bucket (nxe = 0: the whole grid), the boundary
+   * family (OR_BC_*), and the shock's upstream four-velocity u0
+   * (proj/shock/app.f90:113). */
+  int nxs, nxe, bc;
+  double u0;
 } or_params;
+
+#define OR_BC_PERIODIC 0       /* common/boundary_periodic.f90 */
+#define OR_BC_SHOCK 1          /* proj/shock/boundary_shock.f90 */
+#define OR_BC_RECONNECTION 2   /* proj/reconnection/boundary_reconnection.f90 */
 
 /* Error codes (mirror the reference's fatal stops). */
 #define OR_OK 0
@@ -105,6 +115,11 @@ void or_bc_curre(const or_params *p, double *uj);
 void or_bc_dfield(const or_params *p, double *df);
 /* boundary_periodic__phi (boundary_periodic.f90:511-568) */
 void or_bc_phi(const or_params *p, double *phi);
+/* bc_phi of component l (the wall families' ghost column depends on it) */
+void or_bc_phi_l(const or_params *p, double *phi, int l);
+/* boundary_shock__injection (boundary_shock.f90:255-298): the left wall and
+ * the moving right edge xend = nxe delx + u0/sqrt(1+u0^2/c^2) delt. */
+int or_bc_injection(const or_params *p, double *up, const int *np2, double u0);
 /* RHS gkl of field__fdtd_i (field.f90:125-146) */
 void or_field_rhs(const or_params *p, double *gkl, const double *uf,
                   const double *uj);

@@ -59,7 +59,10 @@ def test_unchanged_weibel_app_resumes_on_the_device_and_matches_the_oracle(tmp_p
                          "mass_ratio": 1.0, "sigma_e": 0.0, "omega_pe": 0.125, "v_the": 0.1,
                          "v_thi": 0.1, "t_ani": 1.0}}
     json.dump(cfg, open(d + "config.json", "w"), indent=2)
-    r = subprocess.run([APP, d + "config.json"], cwd=d, capture_output=True, text=True, timeout=300)
+    # no argument: the app reads config.json in its directory (app.f90:22,
+    # 184-185); its argument form passes an unallocated deferred-length
+    # string to get_command_argument (app.f90:24, 188), which reads as blank
+    r = subprocess.run([APP], cwd=d, capture_output=True, text=True, timeout=300)
     print(r.stdout[-1500:], r.stderr[-1500:])
     assert r.returncode == 0
     assert r.stdout.count("Time step:") == it1 - it0

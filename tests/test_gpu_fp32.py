@@ -162,11 +162,14 @@ def _one_row_state(p, rows):
 
 
 def test_fp32_full_bin_is_laid_out_anew():
-    """Row segments of 160 slots leave no room beyond each bin's need (16
-    cells of at least 8 slots; 30 particles in cell 5 get 32).  Ten fast
-    particles of cell 4 arrive in cell 5: the fill check after the push
-    sees 40 > 32, and the row is laid out anew (placing the movers) before
-    the next push.  The state then matches the FP64 oracle's."""
+    """Row segments of 176 slots leave one 8-slot block of room beyond the
+    bins' needs (16 cells of at least 8 slots; 30 particles in cell 5 get
+    32).  Ten fast particles of cell 4 arrive in cell 5: the fill check
+    after the push sees 40 > 32, and the block of cells is laid out anew
+    before the next push.  Bins move whole, their movers' holes with them,
+    so cell 4 keeps its 10 slots until the next push places its movers: the
+    new layout needs 14 x 8 + 16 + 40 = 168 slots.  The state then matches
+    the FP64 oracle's."""
     from wumingpic2d_amd import weibel
     p = weibel.grid_params(16, 8, 4)
     jl, y = 3, p["nys"] + 3 + 0.5
@@ -174,7 +177,7 @@ def test_fp32_full_bin_is_laid_out_anew():
           [(p["nxgs"] + 5 + f, y, 0.0, 0.0) for f in np.linspace(0.1, 0.6, 30)]
     up, np2, cc, uf = _one_row_state(p, {jl: row})
     ctx = _ctx32(p)
-    ctx.set_capacity(160)
+    ctx.set_capacity(176)
     ctx.state_upload(up, np2, cc, uf)
     orc = Oracle(p)
     gp = up.copy(order="F")
@@ -228,7 +231,7 @@ def test_fp32_tight_rows_relayout_and_track_oracle():
     """Row segments of 1.375 times a row's 512 particles: bins of about 16
     particles take 20 slots on average once rounded up to 8-slot blocks, so
     each bin has a few slots of room, a hot plasma (u_th = 0.2 c) fills bins
-    every step and rows are laid out anew again and again.  The trajectory
+    every step and blocks of cells are laid out anew again and again.  The trajectory
     still tracks the FP64 oracle at the FP32 bars, and two runs are
     bit-identical."""
     p, up0, np20, cc0, uf0 = state(32, 16, 16, 50, vth=0.2)

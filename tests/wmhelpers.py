@@ -76,3 +76,39 @@ def gauss_residual(p, uf, up, np2):
     ey = uf[4, 2:-2, 2:-2]
     div = (np.roll(ex, -1, 0) - ex) + (np.roll(ey, -1, 1) - ey)
     return div - 4 * np.pi * rho_tsc(p, up, np2)
+
+
+def wall_state(bc, nx, ny, ppc, seed, vth=0.1, extra=0, u0=-0.2):
+    """A state for a wall family (oracle.BC_SHOCK / BC_RECONNECTION): the
+    Weibel pairs, on a grid `extra` columns wider than they fill for the
+    shock (its box nxs..nxe grows into the rest, proj/shock/app.f90:613-623;
+    its particles fill [nxs, nxe), :436), with none left deep in cell nxs
+    (the shock mirrors a particle pushed below x = nxs two cells up, a move
+    its deposit does not index).  p carries bc, nxs, nxe and u0."""
+    from oracle import BC_SHOCK, Oracle
+
+    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth)
+    p = dict(p, bc=bc, nxs=p["nxgs"], nxe=p["nxge"], u0=u0 if bc == BC_SHOCK else 0.0)
+    if bc != BC_SHOCK:
+        return p, up, np2, cc, uf
+    q = dict(p, nx=nx + extra + 1, nxge=p["nxge"] + extra + 1)
+    q["nxe"] = q["nxge"] - extra
+    cc = np.asfortranarray(np.concatenate([cc, np.repeat(cc[-1:], extra + 1, axis=0)], axis=0))
+    uf2 = np.zeros((6, q["nx"] + 4, ny + 4), order="F")
+    uf2[:, :nx + 4] = uf
+    x = up[0]
+    x[x < q["nxs"] + 1] += 1.0
+    up2 = np.zeros_like(up)
+    Oracle(q).sort__bucket(up2, up, cc, np2)
+    return q, up2, np2, cc, uf2
+
+
+def make_wall_ctx(p, init=("particle", "field", "sort")):
+    """A context with the wall family of p["bc"] initialised (its __init
+    takes boundary_periodic__init's arguments)."""
+    from oracle import BC_SHOCK
+
+    ctx = make_ctx(p, init=init)
+    init_bc = ctx.boundary_shock__init if p["bc"] == BC_SHOCK else ctx.boundary_reconnection__init
+    init_bc(*grid_args(p), 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
+    return ctx

@@ -52,6 +52,8 @@ WM_ERR_HIP = 6
 WM_ERR_NOMEM = 7
 WM_ERR_NODEV = 8
 BC_PERIODIC = 0
+BC_SHOCK = 1
+BC_RECONNECTION = 2
 FORM_FP64 = 0
 FORM_FP32 = 1
 SOLVER_CG = 0
@@ -72,6 +74,10 @@ EXPORTS = (
     "wm_mom_calc_init", "wm_mom_calc_accl", "wm_mom_calc_nvt", "wm_bcp_mom", "wm_moments",
     "wm_paraio_init", "wm_paraio_output", "wm_paraio_input", "wm_paraio_output_state",
     "wm_paraio_input_state", "wm_paraio_param", "wm_paraio_mom", "wm_paraio_ptcl", "wm_paraio_orb",
+    "wm_bcs_init", "wm_bcs_particle_x", "wm_bcs_injection", "wm_bcs_particle_y", "wm_bcs_dfield",
+    "wm_bcs_curre", "wm_bcs_phi", "wm_bcs_mom", "wm_bcr_init", "wm_bcr_particle_x",
+    "wm_bcr_particle_y", "wm_bcr_dfield", "wm_bcr_curre", "wm_bcr_phi", "wm_bcr_mom",
+    "wm_set_walls",
     # include/wmpic_json.h: the JSON model behind the Fortran json_module
     "wm_json_new_object", "wm_json_load", "wm_json_parse", "wm_json_free", "wm_json_child",
     "wm_json_get_int", "wm_json_get_real", "wm_json_get_bool", "wm_json_get_str",
@@ -166,7 +172,17 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_paraio_mom": [_P, _P, _P, _I],
         "wm_paraio_ptcl": [_P, _P, _P, _P, _I],
         "wm_paraio_orb": [_P, _P, _P, _P, _I],
+        "wm_set_walls": [_P, _I, _I, _D],
+        "wm_bcs_injection": [_P, _P, _P, _I, _I, _D],
     }
+    for fam in ("bcs", "bcr"):
+        sig[f"wm_{fam}_init"] = sig["wm_bcp_init"]
+        sig[f"wm_{fam}_particle_x"] = [_P, _P, _P, _I, _I]
+        sig[f"wm_{fam}_particle_y"] = [_P, _P, _P]
+        sig[f"wm_{fam}_dfield"] = sig["wm_bcp_dfield"]
+        sig[f"wm_{fam}_curre"] = sig["wm_bcp_curre"]
+        sig[f"wm_{fam}_phi"] = sig["wm_bcp_phi"]
+        sig[f"wm_{fam}_mom"] = [_P, _P]
     for name, args in sig.items():
         f = getattr(lib, name)
         f.argtypes = args
@@ -364,6 +380,73 @@ class Context:
     def boundary_periodic__mom(self, mom) -> None:
         self._check(self._lib.wm_bcp_mom(self._h, self._a(mom, "mom")))
 
+    # ------------------- the wall families (boundary_shock, boundary_reconnection)
+    def _bc_init(self, fam, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, nup, ndown, mnpi,
+                 mnpr, ncomw, nerr, nstat, delx, delt, c) -> None:
+        ns = np.ascontiguousarray(nstat, dtype=np.int32)
+        self._check(getattr(self._lib, f"wm_{fam}_init")(
+            self._h, ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye, nup, ndown, mnpi, mnpr,
+            ncomw, nerr, ns.ctypes.data, ns.size, delx, delt, c))
+        self._remember(ndim, np_, nsp, nxgs, nxge, nygs, nyge, nys, nye)
+
+    def boundary_shock__init(self, *args) -> None:
+        self._bc_init("bcs", *args)
+
+    def boundary_reconnection__init(self, *args) -> None:
+        self._bc_init("bcr", *args)
+
+    def _phi_n(self, nxs, nxe):
+        return (nxe - nxs + 3) * (self.grid["ny"] + 2) if self.grid else None
+
+    def _walls(self, fam):
+        lib, a = self._lib, self._a
+
+        class W:
+            def particle_x(_, up, np2, nxs, nxe):
+                self._check(getattr(lib, f"wm_{fam}_particle_x")(self._h, a(up, "up"),
+                                                                   a(np2, "np2"), nxs, nxe))
+
+            def particle_y(_, up, np2):
+                self._check(getattr(lib, f"wm_{fam}_particle_y")(self._h, a(up, "up"),
+                                                                   a(np2, "np2")))
+
+            def dfield(_, df, nxs, nxe, nys, nye, nxgs, nxge):
+                self._check(getattr(lib, f"wm_{fam}_dfield")(self._h, a(df, "uf", "df"), nxs,
+                                                               nxe, nys, nye, nxgs, nxge))
+
+            def curre(_, uj, nxs, nxe, nys, nye, nxgs, nxge):
+                self._check(getattr(lib, f"wm_{fam}_curre")(self._h, a(uj, "uj"), nxs, nxe,
+                                                              nys, nye, nxgs, nxge))
+
+            def phi(_, phi, nxs, nxe, nys, nye, l):
+                p = _ptr(phi, np.float64, "phi", self._phi_n(nxs, nxe))
+                self._check(getattr(lib, f"wm_{fam}_phi")(self._h, p, nxs, nxe, nys, nye, l))
+
+            def mom(_, mom):
+                self._check(getattr(lib, f"wm_{fam}_mom")(self._h, a(mom, "mom")))
+
+        return W()
+
+    @property
+    def boundary_shock(self):
+        """boundary_shock__* (proj/shock/boundary_shock.f90) as methods:
+        particle_x, particle_y, dfield, curre, phi, mom (and injection on
+        the context)."""
+        return self._walls("bcs")
+
+    @property
+    def boundary_reconnection(self):
+        """boundary_reconnection__* (proj/reconnection/boundary_reconnection.f90)."""
+        return self._walls("bcr")
+
+    def boundary_shock__injection(self, up, np2, nxs, nxe, u0: float) -> None:
+        self._check(self._lib.wm_bcs_injection(self._h, self._a(up, "up"), self._a(np2, "np2"),
+                                               nxs, nxe, float(u0)))
+
+    def set_walls(self, nxs: int, nxe: int, u0: float = 0.0) -> None:
+        """Active columns and the shock's u0 for device-resident steps."""
+        self._check(self._lib.wm_set_walls(self._h, nxs, nxe, float(u0)))
+
     # ------------------------------------------------- resident run
     def state_upload(self, up, np2, cumcnt, uf) -> None:
         self._check(self._lib.wm_state_upload(self._h, self._a(up, "up"),
@@ -465,13 +548,13 @@ class Context:
     def capacity_stats(self) -> dict:
         """FP32 form: the particle store's capacities and their use since the
         state was made (maxima over the pushes)."""
-        out = np.zeros(11, dtype=np.int64)
-        n = self._lib.wm_capacity_stats(self._h, out.ctypes.data, 11)
-        if n != 11:                     # an error status (11 entries on success)
+        out = np.zeros(12, dtype=np.int64)
+        n = self._lib.wm_capacity_stats(self._h, out.ctypes.data, 12)
+        if n != 12:                     # an error status (12 entries on success)
             self._check(n)
         keys = ("max_bin_fill", "max_bin_movers", "max_workgroup_movers", "max_row_records",
                 "max_row_spill", "relayouts", "slots_per_row", "records_per_row",
-                "workgroup_lds_list", "spill_per_row", "full_bins")
+                "workgroup_lds_list", "spill_per_row", "full_bins", "segment_relayouts")
         return {k: int(v) for k, v in zip(keys, out[:n])}
 
     # ------------------------------------------ learned preconditioner

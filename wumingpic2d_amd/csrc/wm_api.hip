@@ -341,7 +341,6 @@ int fresh_bins(wm_ctx *ctx) {
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, ctx->stream, ctx->fstat, (size_t)WM_NSTAT, 0);
   WM_HIP(hipGetLastError());
   ctx->mv_pending = false;
-  ctx->relayouts = 0;
   return WM_OK;
 }
 
@@ -368,8 +367,26 @@ int download_field(wm_ctx *ctx, double *a, int ncomp, const double *planes) {
   return WM_OK;
 }
 
-int upload_cumcnt(wm_ctx *ctx, const int *cumcnt) {
+// cumcnt as the device reads it: the wall families' entries outside
+// nxs .. nxe+1 are whatever the last sort left there (sort.f90:66-69 writes
+// only those), so the cells outside the active range are made empty.
+std::vector<int> active_cumcnt(const wm_ctx *ctx, const int *cumcnt) {
+  const int nseg = nseg_of(ctx), w = ctx->g.nx + 1;
+  std::vector<int> c(cumcnt, cumcnt + (size_t)nseg * w);
+  if (ctx->bcf == WM_BC_PERIODIC || ctx->xe == 0) return c;
+  const int a = ctx->xs - ctx->g.nxgs, b = ctx->xe + 1 - ctx->g.nxgs;
+  for (int s = 0; s < nseg; ++s) {
+    int *r = c.data() + (size_t)s * w;
+    for (int i = 0; i < a; ++i) r[i] = 0;
+    for (int i = b + 1; i < w; ++i) r[i] = r[b];
+  }
+  return c;
+}
+
+int upload_cumcnt(wm_ctx *ctx, const int *cumcnt_in) {
   const size_t n = (size_t)nseg_of(ctx) * (ctx->g.nx + 1);
+  const std::vector<int> act = active_cumcnt(ctx, cumcnt_in);
+  const int *cumcnt = act.data();
   // cumcnt must be a prefix table inside the row capacity (sort.f90:64-69)
   for (int s = 0; s < nseg_of(ctx); ++s) {
     const int *c = cumcnt + (size_t)s * (ctx->g.nx + 1);
@@ -381,13 +398,17 @@ int upload_cumcnt(wm_ctx *ctx, const int *cumcnt) {
     }
   }
   WM_HIP(hipMemcpyAsync(ctx->cumcnt, cumcnt, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  // the copy reads act before this scope ends
+  WM_HIP(hipStreamSynchronize(ctx->stream));
   return WM_OK;
 }
 
 std::vector<int> counts_from_cumcnt(const wm_ctx *ctx, const int *cumcnt) {
   const int nseg = nseg_of(ctx);
+  const int last = ctx->bcf == WM_BC_PERIODIC || ctx->xe == 0 ? ctx->g.nx
+                                                               : ctx->xe + 1 - ctx->g.nxgs;
   std::vector<int> c(nseg);
-  for (int s = 0; s < nseg; ++s) c[s] = cumcnt[(size_t)s * (ctx->g.nx + 1) + ctx->g.nx];
+  for (int s = 0; s < nseg; ++s) c[s] = cumcnt[(size_t)s * (ctx->g.nx + 1) + last];
   return c;
 }
 
@@ -420,12 +441,25 @@ int require_no_resident(wm_ctx *ctx, const char *what) {
   return WM_OK;
 }
 
+// The active x range of a call: the whole grid on one periodic rank; the
+// wall families' [nxs, nxe] inside it (the shock's box grows,
+// proj/shock/app.f90:613-623), kept for the device kernels.
 int check_bounds(wm_ctx *ctx, int nxs, int nxe) {
-  if (nxs != ctx->g.nxgs || nxe != ctx->g.nxge) {
-    wm_set_error(ctx, WM_ERR_ARG, "nxs:nxe = %d:%d must equal nxgs:nxge = %d:%d on one rank", nxs,
-                 nxe, ctx->g.nxgs, ctx->g.nxge);
+  if (ctx->bcf == WM_BC_PERIODIC) {
+    if (nxs != ctx->g.nxgs || nxe != ctx->g.nxge) {
+      wm_set_error(ctx, WM_ERR_ARG, "nxs:nxe = %d:%d must equal nxgs:nxge = %d:%d on one rank",
+                   nxs, nxe, ctx->g.nxgs, ctx->g.nxge);
+      return WM_ERR_ARG;
+    }
+    return WM_OK;
+  }
+  if (nxs < ctx->g.nxgs || nxe > ctx->g.nxge || nxe < nxs + 2) {
+    wm_set_error(ctx, WM_ERR_ARG, "nxs:nxe = %d:%d must lie in nxgs:nxge = %d:%d, at least 3 cells",
+                 nxs, nxe, ctx->g.nxgs, ctx->g.nxge);
     return WM_ERR_ARG;
   }
+  ctx->xs = nxs;
+  ctx->xe = nxe;
   return WM_OK;
 }
 
@@ -565,22 +599,31 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(dmalloc(ctx, &ctx->sfill, g.ny));
     WM_HIP(hipMemsetAsync(ctx->sfill, 0, sizeof(int) * g.ny, ctx->stream));
     WM_TRY(dmalloc(ctx, &ctx->rowpend, g.ny));
-    WM_TRY(dmalloc(ctx, &ctx->rowflag, g.ny));
     WM_TRY(dmalloc(ctx, &ctx->fstat, WM_NSTAT));
-    // the re-layout's workspace: rows of at most 512 MB together
-    const size_t row_bytes = (size_t)g.nsp * ctx->cap * (20 + (g.ndim >= 6 ? 8 : 0));
-    ctx->relay_rows = (int)std::max<size_t>(1, std::min<size_t>(g.ny, ((size_t)512 << 20) / row_bytes));
-    const size_t rslots = (size_t)ctx->relay_rows * g.nsp * ctx->cap;
-    WM_TRY(dmalloc(ctx, &ctx->rtmp.q, rslots));
-    WM_TRY(dmalloc(ctx, &ctx->rtmp.uz, rslots));
-    if (g.ndim >= 6) WM_TRY(dmalloc(ctx, &ctx->rtmp_id, rslots));
-    WM_TRY(dmalloc(ctx, &ctx->rtmp_bstart, (size_t)ctx->relay_rows * g.nsp * (g.nx + 1)));
-    WM_TRY(dmalloc(ctx, &ctx->rtmp_blk, (size_t)ctx->relay_rows * g.nsp * ((g.nx + 63) / 64)));
-    WM_TRY(dmalloc(ctx, &ctx->relay_list, ctx->relay_rows));
+    // the re-layout's lists and workspace: relay_nwb block workgroups of
+    // relay_wsz slots (four times a block's mean span), or relay_nws (256:
+    // one per CU) segment workgroups of a segment's slots
+    const size_t nblk = (size_t)nseg * wm_cdiv(g.nx, WM_RB);
+    WM_TRY(dmalloc(ctx, &ctx->relay_ctl, 4));
+    WM_TRY(dmalloc(ctx, &ctx->relay_blocks, nblk));
+    WM_TRY(dmalloc(ctx, &ctx->blkflag, nblk));
+    WM_TRY(dmalloc(ctx, &ctx->relay_segs, nseg));
+    WM_TRY(dmalloc(ctx, &ctx->segflag, nseg));
+    ctx->relay_wsz = std::min(ctx->cap, std::max(1024, 4 * WM_RB * wm_cdiv(ctx->cap, g.nx) + 63) / 64 * 64);
+    ctx->relay_nwb = (int)std::min<size_t>(512, nblk);
+    ctx->relay_nws = std::min(256, nseg);
+    const size_t wslots = std::max((size_t)ctx->relay_nwb * ctx->relay_wsz,
+                                   (size_t)ctx->relay_nws * ctx->cap);
+    WM_TRY(dmalloc(ctx, &ctx->rws.q, wslots));
+    WM_TRY(dmalloc(ctx, &ctx->rws.uz, wslots));
+    if (g.ndim >= 6) WM_TRY(dmalloc(ctx, &ctx->rws_id, wslots));
+    WM_TRY(dmalloc(ctx, &ctx->rws_bs, (size_t)ctx->relay_nws * 2 * (g.nx + 1)));
+    WM_HIP(hipMemsetAsync(ctx->relay_ctl, 0, sizeof(int) * 4, ctx->stream));
+    WM_HIP(hipMemsetAsync(ctx->blkflag, 0, sizeof(int) * nblk, ctx->stream));
+    WM_HIP(hipMemsetAsync(ctx->segflag, 0, sizeof(int) * nseg, ctx->stream));
     WM_HIP(hipMemsetAsync(ctx->bin_cnt, 0, sizeof(int) * nbin, ctx->stream));
     WM_HIP(hipMemsetAsync(ctx->bstart, 0, sizeof(int) * (size_t)nseg * (g.nx + 1), ctx->stream));
     WM_HIP(hipMemsetAsync(ctx->rowpend, 0, sizeof(int) * g.ny, ctx->stream));
-    WM_HIP(hipMemsetAsync(ctx->rowflag, 0, sizeof(int) * g.ny, ctx->stream));
     WM_HIP(hipMemsetAsync(ctx->fstat, 0, sizeof(int) * WM_NSTAT, ctx->stream));
     ctx->mv_pending = false;
   } else {
@@ -667,10 +710,6 @@ int wm_report_derr(wm_ctx *ctx, int h, const char *where) {
                  (h & DERR_NP_OVER) ? "row capacity" : "migration buffer");
     return WM_ERR_NP_OVER;
   }
-  if (h & DERR_RELAYOUT) {
-    wm_set_error(ctx, WM_ERR_NP_OVER, "memory over (np2 > np) in %s (a cell bin is full)", where);
-    return WM_ERR_NP_OVER;
-  }
   wm_set_error(ctx, WM_ERR_RANGE, "%s: particle outside the cells the reference indexes (%s)",
                where, (h & DERR_JUMP) ? "moved more than one cell" : "cell index out of range");
   return WM_ERR_RANGE;
@@ -727,10 +766,10 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum[0],
                   ctx->bin_sum[1], ctx->mv[0].rec, ctx->mv[1].rec, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s, ctx->mom, ctx->mom_part, ctx->bins.q, ctx->bins.uz,
-                  ctx->bstart, ctx->pcnt, ctx->mfill[0], ctx->mfill[1], ctx->rowpend, ctx->rowflag,
-                  ctx->fstat, ctx->rtmp.q, ctx->rtmp.uz, ctx->rtmp_id, ctx->rtmp_bstart,
-                  ctx->relay_list, ctx->spill_l, ctx->spill_p, ctx->spill_q, ctx->sfill,
-                  ctx->rtmp_blk};
+                  ctx->bstart, ctx->pcnt, ctx->mfill[0], ctx->mfill[1], ctx->rowpend,
+                  ctx->fstat, ctx->relay_ctl, ctx->relay_blocks, ctx->blkflag, ctx->relay_segs,
+                  ctx->segflag, ctx->rws.q, ctx->rws.uz, ctx->rws_id, ctx->rws_bs,
+                  ctx->spill_l, ctx->spill_p, ctx->spill_q, ctx->sfill};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   wm_pc_free(ctx);
@@ -820,10 +859,13 @@ int wm_sort_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int
   return WM_OK;
 }
 
-int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
-                int nys, int nye, int nup, int ndown, int mnpi, int mnpr, int ncomw, int nerr,
-                const int *nstat, int nstat_len, double delx, double delt, double c) {
-  (void)mnpi; (void)mnpr; (void)ncomw; (void)nerr; (void)nstat; (void)nstat_len;
+namespace {
+// boundary_*__init of one family (boundary_periodic.f90:24-58,
+// boundary_shock.f90:26-58, boundary_reconnection.f90:26-58): the same
+// arguments; the family picks the device's field, current, potential and
+// particle boundaries.
+int bc_init(wm_ctx *ctx, int family, int ndim, int np, int nsp, int nxgs, int nxge, int nygs,
+            int nyge, int nys, int nye, int nup, int ndown, double delx, double delt, double c) {
   if (!ctx) return WM_ERR_ARG;
   if (nup != 0 || ndown != 0) {
     wm_set_error(ctx, WM_ERR_ARG, "one device context is one periodic rank (nup = ndown = 0)");
@@ -838,8 +880,37 @@ int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int 
     ctx->g.delt = delt;
     ctx->g.c = c;
   }
+  if (ctx->binit && ctx->bcf != family) {
+    wm_set_error(ctx, WM_ERR_ARG, "a context takes one boundary family (%d, then %d)", ctx->bcf,
+                 family);
+    return WM_ERR_ARG;
+  }
+  ctx->bcf = family;
   ctx->binit = true;
   return WM_OK;
+}
+}  // namespace
+
+int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+                int nys, int nye, int nup, int ndown, int mnpi, int mnpr, int ncomw, int nerr,
+                const int *nstat, int nstat_len, double delx, double delt, double c) {
+  (void)mnpi; (void)mnpr; (void)ncomw; (void)nerr; (void)nstat; (void)nstat_len;
+  return bc_init(ctx, WM_BC_PERIODIC, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nup, ndown,
+                 delx, delt, c);
+}
+int wm_bcs_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+                int nys, int nye, int nup, int ndown, int mnpi, int mnpr, int ncomw, int nerr,
+                const int *nstat, int nstat_len, double delx, double delt, double c) {
+  (void)mnpi; (void)mnpr; (void)ncomw; (void)nerr; (void)nstat; (void)nstat_len;
+  return bc_init(ctx, WM_BC_SHOCK, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nup, ndown,
+                 delx, delt, c);
+}
+int wm_bcr_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge, int nygs, int nyge,
+                int nys, int nye, int nup, int ndown, int mnpi, int mnpr, int ncomw, int nerr,
+                const int *nstat, int nstat_len, double delx, double delt, double c) {
+  (void)mnpi; (void)mnpr; (void)ncomw; (void)nerr; (void)nstat; (void)nstat_len;
+  return bc_init(ctx, WM_BC_RECONNECTION, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, nup,
+                 ndown, delx, delt, c);
 }
 
 // ------------------------------------------------- host-array entry points
@@ -871,11 +942,13 @@ int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
                     const int *cumcnt, int nxs, int nxe, int bc_family) {
   if (!ctx) return WM_ERR_ARG;
   WM_TRY(require(ctx, ctx->finit, "field__init"));
-  if (bc_family != WM_BC_PERIODIC) {
+  static const char *const init_of[3] = {"boundary_periodic__init", "boundary_shock__init",
+                                         "boundary_reconnection__init"};
+  if (bc_family < WM_BC_PERIODIC || bc_family > WM_BC_RECONNECTION) {
     wm_set_error(ctx, WM_ERR_ARG, "boundary family %d is not available", bc_family);
     return WM_ERR_ARG;
   }
-  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require(ctx, ctx->binit && ctx->bcf == bc_family, init_of[bc_family]));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   WM_TRY(require_fp64(ctx, "field__fdtd_i"));
   WM_TRY(require_no_resident(ctx, "field__fdtd_i"));
@@ -896,7 +969,7 @@ int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
 
 int wm_bcp_particle_x(wm_ctx *ctx, double *up, const int *np2) {
   if (!ctx) return WM_ERR_ARG;
-  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require(ctx, ctx->binit && ctx->bcf == WM_BC_PERIODIC, "boundary_periodic__init"));
   WM_TRY(require_fp64(ctx, "boundary_periodic__particle_x"));
   WM_TRY(require_no_resident(ctx, "boundary_periodic__particle_x"));
   WM_TRY(wm_alloc_device(ctx));
@@ -917,7 +990,7 @@ int wm_bcp_particle_y(wm_ctx *ctx, double *up, int *np2) {
   WM_TRY(upload_particles(ctx, up, np2, 0));
   const int nseg = nseg_of(ctx);
   WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  WM_TRY(wm_k_particle_y(ctx, 0, false));
+  WM_TRY(wm_k_particle_y(ctx, 0, 0));
   WM_TRY(wm_check_derr(ctx, "boundary_periodic__particle_y"));
   std::vector<int> n2(nseg);
   WM_HIP(hipMemcpyAsync(n2.data(), ctx->np2, sizeof(int) * nseg, hipMemcpyDeviceToHost, ctx->stream));
@@ -946,16 +1019,22 @@ int wm_sort_bucket(wm_ctx *ctx, double *gp, const double *up, int *cumcnt, const
   WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg, hipMemcpyHostToDevice, ctx->stream));
   WM_TRY(wm_k_sort(ctx, 1, 0));
   WM_TRY(wm_check_derr(ctx, "sort__bucket"));
-  WM_HIP(hipMemcpyAsync(cumcnt, ctx->cumcnt, sizeof(int) * (size_t)nseg * (ctx->g.nx + 1),
-                        hipMemcpyDeviceToHost, ctx->stream));
+  const int w = ctx->g.nx + 1;
+  std::vector<int> cc((size_t)nseg * w);
+  WM_HIP(hipMemcpyAsync(cc.data(), ctx->cumcnt, sizeof(int) * cc.size(), hipMemcpyDeviceToHost,
+                        ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
+  // sort.f90:66-69 writes cumcnt(nxs:nxe+1) only
+  const int a = nxs - ctx->g.nxgs, b = nxe + 1 - ctx->g.nxgs;
+  for (int sg = 0; sg < nseg; ++sg)
+    memcpy(cumcnt + (size_t)sg * w + a, cc.data() + (size_t)sg * w + a, sizeof(int) * (b - a + 1));
   return download_particles(ctx, gp, np2, 0);
 }
 
 int wm_bcp_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye, int nxgs,
                   int nxge) {
   if (!ctx) return WM_ERR_ARG;
-  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require(ctx, ctx->binit && ctx->bcf == WM_BC_PERIODIC, "boundary_periodic__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   if (nys != ctx->g.nys || nye != ctx->g.nye || nxgs != ctx->g.nxgs || nxge != ctx->g.nxge)
     return WM_ERR_ARG;
@@ -968,7 +1047,7 @@ int wm_bcp_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye, i
 int wm_bcp_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye, int nxgs,
                  int nxge) {
   if (!ctx) return WM_ERR_ARG;
-  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require(ctx, ctx->binit && ctx->bcf == WM_BC_PERIODIC, "boundary_periodic__init"));
   WM_TRY(check_bounds(ctx, nxs, nxe));
   if (nys != ctx->g.nys || nye != ctx->g.nye || nxgs != ctx->g.nxgs || nxge != ctx->g.nxge)
     return WM_ERR_ARG;
@@ -978,19 +1057,146 @@ int wm_bcp_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye, in
   return download_field(ctx, uj, 3, ctx->fscratch);
 }
 
-int wm_bcp_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye, int l) {
-  (void)l;
-  if (!ctx) return WM_ERR_ARG;
-  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+namespace {
+const char *bc_init_name(int family) {
+  return family == WM_BC_SHOCK ? "boundary_shock__init"
+         : family == WM_BC_RECONNECTION ? "boundary_reconnection__init"
+                                        : "boundary_periodic__init";
+}
+int require_bc(wm_ctx *ctx, int family) {
+  return require(ctx, ctx->binit && ctx->bcf == family, bc_init_name(family));
+}
+
+int bc_dfield(wm_ctx *ctx, int family, double *df, int nxs, int nxe, int nys, int nye, int nxgs,
+              int nxge) {
+  if (!ctx || !df) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, family));
   WM_TRY(check_bounds(ctx, nxs, nxe));
-  if (nys != ctx->g.nys || nye != ctx->g.nye) return WM_ERR_ARG;
+  if (nys != ctx->g.nys || nye != ctx->g.nye || nxgs != ctx->g.nxgs || nxge != ctx->g.nxge)
+    return WM_ERR_ARG;
   WM_TRY(wm_alloc_device(ctx));
-  const size_t n = (size_t)(ctx->g.nx + 2) * (ctx->g.ny + 2);
+  WM_TRY(upload_field(ctx, df, 6, ctx->fscratch));
+  WM_TRY(wm_k_bc_dfield(ctx, ctx->fscratch));
+  return download_field(ctx, df, 6, ctx->fscratch);
+}
+
+int bc_curre(wm_ctx *ctx, int family, double *uj, int nxs, int nxe, int nys, int nye, int nxgs,
+             int nxge) {
+  if (!ctx || !uj) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, family));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  if (nys != ctx->g.nys || nye != ctx->g.nye || nxgs != ctx->g.nxgs || nxge != ctx->g.nxge)
+    return WM_ERR_ARG;
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(upload_field(ctx, uj, 3, ctx->fscratch));
+  WM_TRY(wm_k_bc_curre(ctx, ctx->fscratch));
+  return download_field(ctx, uj, 3, ctx->fscratch);
+}
+
+// phi(nxs-1:nxe+1, nys-1:nye+1) of component l
+int bc_phi(wm_ctx *ctx, int family, double *phi, int nxs, int nxe, int nys, int nye, int l) {
+  if (!ctx || !phi) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, family));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  if (nys != ctx->g.nys || nye != ctx->g.nye || l < 1 || l > 3) return WM_ERR_ARG;
+  WM_TRY(wm_alloc_device(ctx));
+  const size_t n = (size_t)(nxe - nxs + 3) * (ctx->g.ny + 2);
   WM_TRY(ensure_stage(ctx, n));
   WM_HIP(hipMemcpyAsync(ctx->stage, phi, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
-  WM_TRY(wm_k_bc_phi(ctx, ctx->stage));
+  WM_TRY(wm_k_bc_phi(ctx, ctx->stage, l));
   WM_HIP(hipMemcpyAsync(phi, ctx->stage, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
+// The walls' particle_x (family shock or reconnection) or the shock's
+// injection edge (inj) on host arrays.
+int bc_walls_x(wm_ctx *ctx, int family, bool inj, double *up, const int *np2, int nxs, int nxe,
+               double u0, const char *what) {
+  if (!ctx || !up || !np2) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, family));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  WM_TRY(require_fp64(ctx, what));
+  WM_TRY(require_no_resident(ctx, what));
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(upload_particles(ctx, up, np2, 0));
+  WM_HIP(hipMemcpyAsync(ctx->np2, np2, sizeof(int) * nseg_of(ctx), hipMemcpyHostToDevice,
+                        ctx->stream));
+  WM_TRY(inj ? wm_k_injection(ctx, 0, u0) : wm_k_particle_x(ctx, 0));
+  WM_TRY(wm_check_derr(ctx, what));
+  return download_particles(ctx, up, np2, 0);
+}
+
+int bc_mom(wm_ctx *ctx, int family, double *mom) {
+  if (!ctx || !mom) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, family));
+  WM_TRY(wm_alloc_device(ctx));
+  WM_TRY(wm_mom_buffers(ctx));
+  const size_t n = wm_mom_elems(ctx);
+  WM_HIP(hipMemcpyAsync(ctx->mom, mom, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  WM_TRY(wm_k_bc_mom(ctx, ctx->mom));
+  WM_HIP(hipMemcpyAsync(mom, ctx->mom, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+}  // namespace
+
+int wm_bcp_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye, int l) {
+  return bc_phi(ctx, WM_BC_PERIODIC, phi, nxs, nxe, nys, nye, l);
+}
+
+// ---- boundary_shock__* (proj/shock/boundary_shock.f90)
+int wm_bcs_particle_x(wm_ctx *ctx, double *up, const int *np2, int nxs, int nxe) {
+  return bc_walls_x(ctx, WM_BC_SHOCK, false, up, np2, nxs, nxe, 0.0, "boundary_shock__particle_x");
+}
+int wm_bcs_injection(wm_ctx *ctx, double *up, const int *np2, int nxs, int nxe, double u0) {
+  return bc_walls_x(ctx, WM_BC_SHOCK, true, up, np2, nxs, nxe, u0, "boundary_shock__injection");
+}
+int wm_bcs_particle_y(wm_ctx *ctx, double *up, int *np2) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, WM_BC_SHOCK));
+  return wm_bcp_particle_y(ctx, up, np2);
+}
+int wm_bcs_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye, int nxgs, int nxge) {
+  return bc_dfield(ctx, WM_BC_SHOCK, df, nxs, nxe, nys, nye, nxgs, nxge);
+}
+int wm_bcs_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye, int nxgs, int nxge) {
+  return bc_curre(ctx, WM_BC_SHOCK, uj, nxs, nxe, nys, nye, nxgs, nxge);
+}
+int wm_bcs_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye, int l) {
+  return bc_phi(ctx, WM_BC_SHOCK, phi, nxs, nxe, nys, nye, l);
+}
+int wm_bcs_mom(wm_ctx *ctx, double *mom) { return bc_mom(ctx, WM_BC_SHOCK, mom); }
+
+// ---- boundary_reconnection__* (proj/reconnection/boundary_reconnection.f90)
+int wm_bcr_particle_x(wm_ctx *ctx, double *up, const int *np2, int nxs, int nxe) {
+  return bc_walls_x(ctx, WM_BC_RECONNECTION, false, up, np2, nxs, nxe, 0.0,
+                    "boundary_reconnection__particle_x");
+}
+int wm_bcr_particle_y(wm_ctx *ctx, double *up, int *np2) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require_bc(ctx, WM_BC_RECONNECTION));
+  return wm_bcp_particle_y(ctx, up, np2);
+}
+int wm_bcr_dfield(wm_ctx *ctx, double *df, int nxs, int nxe, int nys, int nye, int nxgs, int nxge) {
+  return bc_dfield(ctx, WM_BC_RECONNECTION, df, nxs, nxe, nys, nye, nxgs, nxge);
+}
+int wm_bcr_curre(wm_ctx *ctx, double *uj, int nxs, int nxe, int nys, int nye, int nxgs, int nxge) {
+  return bc_curre(ctx, WM_BC_RECONNECTION, uj, nxs, nxe, nys, nye, nxgs, nxge);
+}
+int wm_bcr_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye, int l) {
+  return bc_phi(ctx, WM_BC_RECONNECTION, phi, nxs, nxe, nys, nye, l);
+}
+int wm_bcr_mom(wm_ctx *ctx, double *mom) { return bc_mom(ctx, WM_BC_RECONNECTION, mom); }
+
+// Device-resident runs of a wall family: the active columns and the shock's
+// upstream four-velocity (the arguments the reference passes each step).
+int wm_set_walls(wm_ctx *ctx, int nxs, int nxe, double u0) {
+  if (!ctx) return WM_ERR_ARG;
+  WM_TRY(require(ctx, ctx->binit && ctx->bcf != WM_BC_PERIODIC,
+                 "boundary_shock__init or boundary_reconnection__init"));
+  WM_TRY(check_bounds(ctx, nxs, nxe));
+  ctx->u0 = u0;
   return WM_OK;
 }
 
@@ -1079,6 +1285,10 @@ int wm_step(wm_ctx *ctx, int nsteps) {
   WM_TRY(require(ctx, ctx->finit, "field__init"));
   WM_TRY(require(ctx, ctx->sinit, "sort__init"));
   WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  if (ctx->bcf != WM_BC_PERIODIC && ctx->form == WM_FORM_FP32) {
+    wm_set_error(ctx, WM_ERR_ARG, "the wall boundary families run in the FP64 particle form");
+    return WM_ERR_ARG;
+  }
   WM_TRY(wm_alloc_device(ctx));
   double acc[WM_NPHASE] = {0};
   for (int it = 0; it < nsteps; ++it) {
@@ -1092,8 +1302,8 @@ int wm_step(wm_ctx *ctx, int nsteps) {
       // Cell bins: the push wraps x itself, and moves the last push's movers
       // into their bins before it pushes (the particle boundaries and the
       // sort of app.f90:104-106 reduce to that).  Beside the solve, on the
-      // second stream, the fill check finds the rows whose bins would
-      // overflow with this push's movers; they are laid out anew below.
+      // second stream, the fill check lists the blocks of cells whose bins
+      // would overflow with this push's movers and they are laid out anew.
       WM_TRY(wm_k_push_deposit32(ctx));
       WM_HIP(hipEventRecord(e[2], ctx->stream));
       WM_HIP(hipStreamWaitEvent(ctx->stream2, e[2], 0));
@@ -1101,6 +1311,7 @@ int wm_step(wm_ctx *ctx, int nsteps) {
       WM_HIP(hipEventRecord(e[5], ctx->stream));
       WM_TRY(wm_k_fill_check(ctx));
       WM_HIP(hipEventRecord(e[6], ctx->stream));
+      WM_TRY(wm_relayout(ctx));
       WM_HIP(hipEventRecord(e[7], ctx->stream));
     } else {
       // The particle boundaries and the sort (app.f90:104-106) read only the
@@ -1113,7 +1324,11 @@ int wm_step(wm_ctx *ctx, int nsteps) {
       StreamSwap on2(ctx, ctx->stream2);
       WM_HIP(hipEventRecord(e[5], ctx->stream));
       {
-        WM_TRY(wm_k_particle_y(ctx, 1, true));    // app.f90:104-105 (x fused into y pass 1)
+        // app.f90:104-105 (x fused into y pass 1): the periodic wrap, the
+        // reconnection's walls (proj/reconnection/app.f90:104), or none
+        // (the shock's edges came with the push, proj/shock/app.f90:113)
+        const int xmode = ctx->bcf == WM_BC_PERIODIC ? 1 : ctx->bcf == WM_BC_RECONNECTION ? 2 : 0;
+        WM_TRY(wm_k_particle_y(ctx, 1, xmode));
         WM_HIP(hipEventRecord(e[6], ctx->stream));
         WM_TRY(wm_k_sort(ctx, 1, 0));             // app.f90:106
       }
@@ -1129,27 +1344,14 @@ int wm_step(wm_ctx *ctx, int nsteps) {
     WM_HIP(hipEventSynchronize(e[8]));
     // phases: [0..3] on the first stream (e0..e4), [4] particle boundaries
     // (e5..e6) and [5] sort (e6..e7) on the second (FP64; in the FP32 form
-    // the push holds them)
+    // the push holds them, and [4] is the fill check, [5] the re-layout)
     const int from[WM_NPHASE] = {0, 1, 2, 3, 5, 6}, to[WM_NPHASE] = {1, 2, 3, 4, 6, 7};
     for (int k = 0; k < WM_NPHASE; ++k) {
       float ms = 0.f;
       WM_HIP(hipEventElapsedTime(&ms, e[from[k]], e[to[k]]));
       acc[k] += ms;
     }
-    int h = 0;
-    WM_TRY(wm_fetch_derr(ctx, &h));
-    if (f32 && (h & DERR_RELAYOUT) && !(h & ~DERR_RELAYOUT)) {
-      // FP32: rows whose bins would overflow are laid out anew (timed as [5])
-      WM_HIP(hipEventRecord(e[9], ctx->stream));
-      WM_TRY(wm_relayout(ctx));
-      WM_HIP(hipEventRecord(e[10], ctx->stream));
-      WM_HIP(hipEventSynchronize(e[10]));
-      float ms = 0.f;
-      WM_HIP(hipEventElapsedTime(&ms, e[9], e[10]));
-      acc[5] += ms;
-      WM_TRY(wm_fetch_derr(ctx, &h));
-    }
-    WM_TRY(wm_report_derr(ctx, h & ~DERR_RELAYOUT, "step"));
+    WM_TRY(wm_check_derr(ctx, "step"));
   }
   for (int k = 0; k < WM_NPHASE; ++k) ctx->phase_ms[k] = acc[k];
   return WM_OK;
@@ -1216,7 +1418,7 @@ int wm_mom_calc_nvt(wm_ctx *ctx, double *mom, const double *up, const int *np2) 
 
 int wm_bcp_mom(wm_ctx *ctx, double *mom) {
   if (!ctx || !mom) return WM_ERR_ARG;
-  WM_TRY(require(ctx, ctx->binit, "boundary_periodic__init"));
+  WM_TRY(require(ctx, ctx->binit && ctx->bcf == WM_BC_PERIODIC, "boundary_periodic__init"));
   WM_TRY(wm_alloc_device(ctx));
   WM_TRY(wm_mom_buffers(ctx));
   const size_t n = wm_mom_elems(ctx);
@@ -1292,9 +1494,9 @@ int wm_capacity_stats(wm_ctx *ctx, long long *out, int n) {
   int st[WM_NSTAT];
   WM_HIP(hipMemcpyAsync(st, ctx->fstat, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
   WM_HIP(hipStreamSynchronize(ctx->stream));
-  const long long v[11] = {st[0], st[1], st[2], st[3], st[4], ctx->relayouts, ctx->cap,
-                           ctx->mrow, wm_push32_pool(), ctx->srow, st[5]};
-  const int m = std::min(n, 11);
+  const long long v[12] = {st[0], st[1], st[2], st[3], st[4], st[6], ctx->cap,
+                           ctx->mrow, wm_push32_pool(), ctx->srow, st[5], st[7]};
+  const int m = std::min(n, 12);
   for (int k = 0; k < m; ++k) out[k] = v[k];
   return m;
 }

@@ -1,6 +1,9 @@
-// wm_bc.hip -- periodic particle boundaries (boundary_periodic.f90:61-248).
+// wm_bc.hip -- particle boundaries: periodic (boundary_periodic.f90:61-248),
+// the conducting walls of the shock and reconnection families and the
+// shock's injection edge (boundary_shock.f90:62-100, 255-298,
+// boundary_reconnection.f90:61-99); their particle_y is the periodic one.
 //
-// particle_x is elementwise.  particle_y keeps the reference's single-rank
+// particle_x and the injection edge are elementwise.  particle_y keeps the reference's single-rank
 // result exactly, including the order particles end up in inside each row,
 // with two kernels over (species, row) segments:
 //  pass 1 (one workgroup per segment, in index order): leavers get their y
@@ -22,6 +25,8 @@ namespace {
 struct BP {
   int nx, ny, nxgs, nxge, nygs, nyge, nys, nsp, cap, mcap;
   double delx, spanx, spany;
+  int xs, xe;          // active columns nxs, nxe (the wall families)
+  double wl, wr;       // mirror points 2 (nxs+1) delx and 2 (nxe-1) delx
 };
 
 __device__ inline double pred_d(double s) {
@@ -45,6 +50,69 @@ __device__ inline int wrap_x(double &x, const BP &P) {
   if (ipos < P.nxgs) x = add_rd(x, P.spanx);
   else if (ipos >= P.nxge + 1) x = add_rd(x, -P.spanx);
   return ipos;
+}
+
+// The wall families' particle_x (boundary_shock.f90:77-89,
+// boundary_reconnection.f90:76-88): a particle in a cell below nxs+1 is
+// mirrored about (nxs+1) delx, one in a cell at or above nxe-1 about
+// (nxe-1) delx, and its momentum reversed.  Returns the new cell.
+__device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) {
+  const double x = R.x[p];
+  const int ipos = (int)(x / P.delx);
+  double w;
+  if (ipos < P.xs + 1) w = P.wl;
+  else if (ipos >= P.xe - 1) w = P.wr;
+  else return ipos;
+  const double xn = w - x;
+  R.x[p] = xn;
+  R.ux[p] = -R.ux[p];
+  R.uy[p] = -R.uy[p];
+  R.uz[p] = -R.uz[p];
+  return (int)(xn / P.delx);
+}
+
+__global__ void k_wall_x(BP P, const int *__restrict__ np2, Rec64 B, int *derr) {
+  const int seg = blockIdx.y;
+  const int n = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  bool bad = false;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int inew = wall_x(B, base + k, P);
+    bad = bad || inew < P.nxgs || inew > P.nxge;
+  }
+  if (bad) atomicOr(derr, DERR_RANGE);
+}
+
+// boundary_shock__injection (boundary_shock.f90:266-294): the left wall as
+// in particle_x; beyond xend (the right edge moved with the upstream flow
+// over the step) x mirrors about xend and u_x about u0.
+__global__ void k_injection(BP P, const int *__restrict__ np2, Rec64 B, double xend, double u0,
+                            int *derr) {
+  const int seg = blockIdx.y;
+  const int n = np2[seg];
+  const size_t base = (size_t)seg * P.cap;
+  bool bad = false;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const size_t p = base + k;
+    const double x = B.x[p];
+    const int ipos = (int)(x / P.delx);
+    double xn = x;
+    if (ipos < P.xs + 1) {
+      xn = P.wl - x;
+      B.ux[p] = -B.ux[p];
+    } else if (x > xend) {
+      xn = 2.0 * xend - x;
+      B.ux[p] = 2.0 * u0 - B.ux[p];
+    } else {
+      continue;
+    }
+    B.x[p] = xn;
+    B.uy[p] = -B.uy[p];
+    B.uz[p] = -B.uz[p];
+    const int inew = (int)(xn / P.delx);
+    bad = bad || inew < P.nxgs || inew > P.nxge;
+  }
+  if (bad) atomicOr(derr, DERR_RANGE);
 }
 
 __global__ void k_particle_x(BP P, const int *__restrict__ np2, double *__restrict__ x,
@@ -74,10 +142,14 @@ __device__ inline int lanes_below(unsigned long long m) {
 
 // Row change of particle p of row j: -1, 0, +1, or 2 (a jump the reference
 // does not index).  FP64: jpos = int(y/delx) (boundary_periodic.f90:147),
-// with particle_x's round-down wrap of x applied first when fused.
-__device__ inline int row_change(const Rec64 &R, size_t p, int j, bool fuse_x, const BP &P,
+// with particle_x applied first when fused: xmode 1 the periodic round-down
+// wrap, 2 the walls' mirrors.
+__device__ inline int row_change(const Rec64 &R, size_t p, int j, int xmode, const BP &P,
                                  int &bad) {
-  if (fuse_x) {
+  if (xmode == 2) {
+    const int inew = wall_x(R, p, P);
+    if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
+  } else if (xmode == 1) {
     double xv = R.x[p];
     const double x0 = xv;
     wrap_x(xv, P);
@@ -102,7 +174,7 @@ __device__ inline Rec64::Val arriving(const Rec64 &R, size_t p, int j, int d, co
 
 // pass 1: one wavefront per segment walks it in index order.
 template <class R>
-__global__ __launch_bounds__(YW) void k_particle_y1(BP P, bool fuse_x,
+__global__ __launch_bounds__(YW) void k_particle_y1(BP P, int xmode,
                                                     const int *__restrict__ np2, R B,
                                                     const int64_t *__restrict__ id,
                                                     int *__restrict__ hole,
@@ -125,7 +197,7 @@ __global__ __launch_bounds__(YW) void k_particle_y1(BP P, bool fuse_x,
 #pragma unroll
     for (int t = 0; t < YT; ++t) {
       const int k = k0 + t * YW + lane;
-      dir[t] = k < n ? row_change(B, base + k, j, fuse_x, P, bad) : 0;
+      dir[t] = k < n ? row_change(B, base + k, j, xmode, P, bad) : 0;
     }
 #pragma unroll
     for (int t = 0; t < YT; ++t) {
@@ -243,6 +315,10 @@ BP make_bp(const wm_ctx *ctx) {
   P.delx = g.delx;
   P.spanx = (double)(g.nxge - g.nxgs + 1) * g.delx;
   P.spany = (double)(g.nyge - g.nygs + 1) * g.delx;
+  P.xs = ctx->xe > 0 ? ctx->xs : g.nxgs;
+  P.xe = ctx->xe > 0 ? ctx->xe : g.nxge;
+  P.wl = 2.0 * (P.xs + 1) * g.delx;
+  P.wr = 2.0 * (P.xe - 1) * g.delx;
   return P;
 }
 
@@ -252,13 +328,35 @@ int wm_k_particle_x(wm_ctx *ctx, int bank) {
   BP P = make_bp(ctx);
   const int nseg = ctx->g.nsp * ctx->g.ny;
   const int nb = std::min(64, std::max(1, wm_cdiv(ctx->cap, 256)));
-  hipLaunchKernelGGL(k_particle_x, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->np2,
-                     ctx->bank[bank].x, ctx->derr);
+  if (ctx->bcf != WM_BC_PERIODIC)
+    hipLaunchKernelGGL(k_wall_x, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->np2,
+                       ctx->bank[bank], ctx->derr);
+  else
+    hipLaunchKernelGGL(k_particle_x, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->np2,
+                       ctx->bank[bank].x, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
-int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x) {
+// xend of boundary_shock__injection (boundary_shock.f90:269), as the
+// reference evaluates it.
+double wm_injection_xend(const wm_ctx *ctx, double u0) {
+  const wm_grid &g = ctx->g;
+  const int nxe = ctx->xe > 0 ? ctx->xe : g.nxge;
+  return nxe * g.delx + u0 / sqrt(1 + (u0 * u0) / (g.c * g.c)) * g.delt;
+}
+
+int wm_k_injection(wm_ctx *ctx, int bank, double u0) {
+  BP P = make_bp(ctx);
+  const int nseg = ctx->g.nsp * ctx->g.ny;
+  const int nb = std::min(64, std::max(1, wm_cdiv(ctx->cap, 256)));
+  hipLaunchKernelGGL(k_injection, dim3(nb, nseg), dim3(256), 0, ctx->stream, P, ctx->np2,
+                     ctx->bank[bank], wm_injection_xend(ctx, u0), u0, ctx->derr);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_particle_y(wm_ctx *ctx, int bank, int xmode) {
   BP P = make_bp(ctx);
   const int nseg = ctx->g.nsp * ctx->g.ny;
   int64_t *id = ctx->id[ctx->idcur];
@@ -269,7 +367,7 @@ int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x) {
   }
   {
     const Rec64 &B = ctx->bank[bank];
-    hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, fuse_x,
+    hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, xmode,
                        ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
                        ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
     hipLaunchKernelGGL(k_particle_y2<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2,

@@ -1,5 +1,6 @@
-// wm_bins.h -- the FP32 form's bin layout of one row segment, and the
-// wavefront key matching of the stable scatters that fill bins in row order.
+// wm_bins.h -- the FP32 form's bin layout of one row segment (or of a window
+// of it), and the wavefront key matching of the stable scatters that fill
+// bins in row order.
 #pragma once
 #include "wm_internal.h"
 
@@ -33,42 +34,76 @@ __device__ inline WaveMatch wave_match(int key, bool active) {
 
 __device__ inline int round_up(int v, int a) { return (v + a - 1) / a * a; }
 
-// Room of a bin of c particles before the slack: at least one aligned block.
-__device__ inline int bin_need(int c) { return round_up(max(c, 1), WM_BIN_ALIGN); }
+// The layout policy of a window of nb bins over `span` slots: bin i gets its
+// need -- its count raised to the window's mean count (floor), rounded up to
+// WM_BIN_ALIGN slots, at least one block -- plus a share of the window's
+// free slots in proportion to that need, in whole blocks.  Raising the small
+// bins to the mean keeps a bin that is momentarily light from being laid out
+// tight and overflowing as soon as its count returns to the mean; the share
+// in proportion gives a dense bin, which gains particles fastest where a
+// plasma pinches, headroom for as many steps as a mean one.  When the raised
+// needs do not fit, the plain counts are used (floor 0).
+struct Policy {
+  int floor;        // counts below it are raised to it
+  long long units;  // free blocks of WM_BIN_ALIGN slots after the needs
+  long long S;      // the needs' sum
+};
+__device__ inline int need_of(int c, int floor) { return round_up(max(max(c, floor), 1), WM_BIN_ALIGN); }
+__device__ inline int room_of(const Policy &p, int c) {
+  const int nd = need_of(c, p.floor);
+  return nd + (int)(p.units * nd / p.S) * WM_BIN_ALIGN;
+}
+// From the window's count sum C, the sum of needs raised to the mean (S1)
+// and the plain one (S0): false when even the plain needs exceed the span.
+__device__ inline bool make_policy(long long C, long long S1, long long S0, int nb, long long span,
+                                   Policy &p) {
+  if (S1 <= span) {
+    p.floor = (int)(C / nb);
+    p.S = S1;
+  } else {
+    p.floor = 0;
+    p.S = S0;
+  }
+  p.units = (span - p.S) / WM_BIN_ALIGN;
+  return p.S <= span;
+}
 
 // The layout of one segment of `cap` slots, computed by one wavefront (all
-// 64 lanes call it) from the counts count(i), i < nx: bin i gets its need
-// (count rounded up to WM_BIN_ALIGN slots, at least one block) plus a share
-// of the segment's free slots in proportion to that need, in whole blocks,
-// at most WM_BIN_MAX slots.  Writes the first slots bs[0..nx] (bs[nx] = slots
-// laid out) and returns false when the counts do not fit: the row's
-// capacity is exceeded (the reference's memory over) or one bin's count is
-// above WM_BIN_MAX.
+// 64 lanes call it) from the counts count(i), i < nx, with the policy above.
+// Writes the first slots bs[0..nx] (bs[nx] = cap) and returns false when the
+// counts do not fit: the row's capacity is exceeded (the reference's memory
+// over) or one bin's count is above WM_BIN_MAX.
 template <class F>
 __device__ bool wave_layout(F count, int nx, int cap, int *bs) {
   const int lane = threadIdx.x & 63;
   const int per = (nx + 63) / 64;
   const int lo = min(nx, lane * per), hi = min(nx, lo + per);
-  long long need = 0;
+  long long C = 0;
   int bad = 0;
   for (int i = lo; i < hi; ++i) {
     const int c = count(i);
     bad |= c > WM_BIN_MAX;
-    need += bin_need(c);
+    C += c;
   }
-  long long S = need;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) S += __shfl_xor(S, o, 64);
-  if (__ballot(bad) != 0ULL || S > cap) return false;
-  const long long units = (cap - S) / WM_BIN_ALIGN;
-  constexpr int top = WM_BIN_MAX / WM_BIN_ALIGN * WM_BIN_ALIGN;
-  auto room = [&](int c) {
-    const int nd = bin_need(c);
-    const long long extra = units * nd / S * WM_BIN_ALIGN;
-    return (int)min((long long)top, nd + extra);
-  };
+  for (int o = 32; o > 0; o >>= 1) C += __shfl_xor(C, o, 64);
+  if (__ballot(bad) != 0ULL) return false;
+  const int mean = (int)(C / nx);
+  long long S1 = 0, S0 = 0;
+  for (int i = lo; i < hi; ++i) {
+    const int c = count(i);
+    S1 += need_of(c, mean);
+    S0 += need_of(c, 0);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    S1 += __shfl_xor(S1, o, 64);
+    S0 += __shfl_xor(S0, o, 64);
+  }
+  Policy p;
+  if (!make_policy(C, S1, S0, nx, cap, p)) return false;
   int mine = 0;
-  for (int i = lo; i < hi; ++i) mine += room(count(i));
+  for (int i = lo; i < hi; ++i) mine += room_of(p, count(i));
   int incl = mine;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -78,10 +113,9 @@ __device__ bool wave_layout(F count, int nx, int cap, int *bs) {
   int run = incl - mine;
   for (int i = lo; i < hi; ++i) {
     bs[i] = run;
-    run += room(count(i));
+    run += room_of(p, count(i));
   }
-  const int total = __shfl(incl, 63, 64);
-  if (lane == 0) bs[nx] = total;
+  if (lane == 0) bs[nx] = cap;     // the slack after the last block joins the last bin
   return true;
 }
 

@@ -1,10 +1,14 @@
 // wm_field.hip -- implicit-FDTD field solve (field.f90:66-186, 319-461) and
-// the periodic field/current/potential boundaries (boundary_periodic.f90).
+// the field/current/potential boundaries: periodic (boundary_periodic.f90)
+// and the conducting walls of the shock and reconnection families
+// (proj/shock/boundary_shock.f90, proj/reconnection/boundary_reconnection.f90).
 //
 // Fields are component planes of (nx+4) x (ny+4), x fastest.  The CG works
 // directly on the interior of the delta-B planes of df (which also carry the
 // previous step's solution, the reference's warm start field.f90:98,355) and
-// takes the periodic 1-cell halo of boundary_periodic__phi by index wrap.
+// takes the 1-cell halo of set_boundary_phi by index: the periodic wrap, or
+// the wall ghosts.  The wall families solve over the active columns
+// [nxs, nxe] only (the shock's box grows, proj/shock/app.f90:613-690).
 // The three Helmholtz solves are batched in one launch per sweep
 // (blockIdx.y = component); components that have converged skip their work.
 // Dot products are reduced deterministically: fixed per-workgroup trees, then
@@ -37,11 +41,12 @@ FC make_fc(const wm_grid &g) {
 
 // ------------------------------------------------------------ boundaries
 // boundary_periodic__curre (boundary_periodic.f90:357-506), row phase: one
-// thread per (column, component) runs the fold/copy sequence of that column.
-__global__ void k_curre_rows(FieldGeom fg, double *uj) {
+// thread per (column, component) runs the fold/copy sequence of that column,
+// for the ghost-inclusive columns c0 .. c0 + cnt - 1 (nxs-2 .. nxe+2).
+__global__ void k_curre_rows(FieldGeom fg, double *uj, int c0, int cnt) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= fg.gx * 3) return;
-  const int li = t % fg.gx, comp = t / fg.gx;
+  if (t >= cnt * 3) return;
+  const int li = c0 + t % cnt, comp = t / cnt;
   double *U = uj + (size_t)comp * fg.ps + li;
   const size_t gx = fg.gx;
   const int ny = fg.ny;
@@ -81,11 +86,12 @@ __global__ void k_curre_cols(FieldGeom fg, double *uj) {
 }
 
 // boundary_periodic__dfield (boundary_periodic.f90:263-345): ghost rows of
-// the interior columns.
-__global__ void k_dfield_rows(FieldGeom fg, double *df, int ncomp) {
+// the interior columns x0 .. x0 + cnt - 1 (all of them, or the wall
+// families' active ones, boundary_shock.f90:326-392).
+__global__ void k_dfield_rows(FieldGeom fg, double *df, int ncomp, int x0, int cnt) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= fg.nx * ncomp) return;
-  const int li = 2 + t % fg.nx, comp = t / fg.nx;
+  if (t >= cnt * ncomp) return;
+  const int li = 2 + x0 + t % cnt, comp = t / cnt;
   double *U = df + (size_t)comp * fg.ps + li;
   const size_t gx = fg.gx;
   const int ny = fg.ny;
@@ -108,6 +114,29 @@ __global__ void k_dfield_cols(FieldGeom fg, double *df, int ncomp) {
   R[nx + 3] = R[3];
 }
 
+// The conducting walls' ghost columns of df, every row (after the rows):
+// the shock's (boundary_shock.f90:396-405) -- Bx, Ey, Ez odd about the wall
+// at nxs - 1/2, By, Bz, Ex even about column nxs, zero at nxe+1 -- or the
+// reconnection's (boundary_reconnection.f90:350-359), whose right wall is
+// the mirror image at nxe - 1/2 and rewrites column nxe of Bx, Ey, Ez.
+__global__ void k_dfield_walls(FieldGeom fg, double *df, XSide X) {
+  const int lj = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lj >= fg.gy) return;
+  double *R = df + (size_t)lj * fg.gx;
+  const size_t ps = fg.ps;
+  const int L = X.xa + 2, E = X.xb + 2;      // ghost-inclusive columns of nxs, nxe
+  R[L - 1] = -R[L];
+  for (int k = 1; k <= 3; ++k) R[k * ps + L - 1] = R[k * ps + L + 1];
+  for (int k = 4; k <= 5; ++k) R[k * ps + L - 1] = -R[k * ps + L];
+  if (X.wall == WM_BC_SHOCK) {
+    for (int k = 0; k < 6; ++k) R[k * ps + E + 1] = 0.0;
+  } else {
+    R[E] = -R[E - 1];
+    for (int k = 1; k <= 3; ++k) R[k * ps + E + 1] = R[k * ps + E - 1];
+    for (int k = 4; k <= 5; ++k) R[k * ps + E] = -R[k * ps + E - 1];
+  }
+}
+
 // boundary_periodic__phi (boundary_periodic.f90:511-568) on an
 // (nx+2) x (ny+2) array: rows, then columns.
 __global__ void k_phi_rows(int nx, int ny, double *phi) {
@@ -124,14 +153,24 @@ __global__ void k_phi_cols(int nx, int ny, double *phi) {
   phi[(size_t)b * w] = phi[(size_t)b * w + nx];
   phi[(size_t)b * w + nx + 1] = phi[(size_t)b * w + 1];
 }
+// The wall families' columns of phi(nxs-1:nxe+1, ...) for component l
+// (boundary_shock.f90:603-623, boundary_reconnection.f90:557-577).
+__global__ void k_phi_walls(int nx, int ny, double *phi, int l) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > ny + 1) return;
+  double *R = phi + (size_t)b * (nx + 2);
+  R[0] = l == 1 ? -R[1] : R[2];
+  R[nx + 1] = 0.0;
+}
 
 // ----------------------------------------------------------- RHS, dE, +=
 // gkl (field.f90:125-146), interior cells, gkl planes nx x ny.
-__global__ void k_rhs(FieldGeom fg, FC f, const double *__restrict__ uf,
+__global__ void k_rhs(FieldGeom fg, FC f, XSide X, const double *__restrict__ uf,
                       const double *__restrict__ uj, double *__restrict__ gkl) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= fg.nx * fg.ny) return;
   const int ii = t % fg.nx, jj = t / fg.nx;
+  if (ii < X.xa || ii > X.xb) return;
   const size_t gx = fg.gx, ps = fg.ps;
   const size_t c = (size_t)(ii + 2) + gx * (size_t)(jj + 2);
   const double *B1 = uf, *B2 = uf + ps, *B3 = uf + 2 * ps;
@@ -151,11 +190,12 @@ __global__ void k_rhs(FieldGeom fg, FC f, const double *__restrict__ uf,
 }
 
 // delta-E (field.f90:154-171), interior cells.
-__global__ void k_delta_e(FieldGeom fg, FC f, const double *__restrict__ uf,
+__global__ void k_delta_e(FieldGeom fg, FC f, XSide X, const double *__restrict__ uf,
                           const double *__restrict__ uj, double *__restrict__ df) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= fg.nx * fg.ny) return;
   const int ii = t % fg.nx, jj = t / fg.nx;
+  if (ii < X.xa || ii > X.xb) return;
   const size_t gx = fg.gx, ps = fg.ps;
   const size_t c = (size_t)(ii + 2) + gx * (size_t)(jj + 2);
   const double *dB1 = df, *dB2 = df + ps, *dB3 = df + 2 * ps;
@@ -174,6 +214,18 @@ __global__ void k_add(size_t n, double *__restrict__ uf, const double *__restric
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) uf[t] = uf[t] + df[t];
 }
+// The same over columns nxs-2 .. nxe+2 only (an active range short of the
+// grid): ghost-inclusive columns xa .. xb + 4 of every row and component.
+__global__ void k_add_cols(FieldGeom fg, XSide X, double *__restrict__ uf,
+                           const double *__restrict__ df) {
+  const int w = X.xb - X.xa + 5;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)6 * fg.gy * w) return;
+  const int a = (int)(t % w);
+  const size_t rc = t / w;                     // component * gy + row
+  const size_t q = (rc / fg.gy) * fg.ps + (rc % fg.gy) * fg.gx + X.xa + a;
+  uf[q] = uf[q] + df[q];
+}
 
 // -------------------------------------------------------------------- CG
 constexpr int CGB = 256;   // threads per CG workgroup
@@ -185,6 +237,7 @@ struct CgArgs {
   size_t n;                // nx*ny
   double f4, f5, err;
   int ite_max;
+  XSide X;                 // periodic, or the walls and the active columns
 };
 
 // Fixed-order workgroup sum of NV values: a butterfly within each wave, then
@@ -259,9 +312,22 @@ __device__ inline CgCell cg_cell(const CgArgs &A) {
   c.b = xcd_order(blockIdx.x, A.nblk);
   c.jj = c.b / A.nbx;
   c.ii = (c.b - c.jj * A.nbx) * CGB + threadIdx.x;
-  c.in = c.ii < A.nx;
+  c.in = c.ii >= A.X.xa && c.ii <= A.X.xb;
   c.t = c.jj * A.nx + c.ii;
   return c;
+}
+
+// Value at column a, row b of component l of a CG field v(x, y): the
+// periodic wrap, or the wall ghosts of set_boundary_phi
+// (boundary_shock.f90:603-623): left of the active columns -v(xa) for Bx
+// and v(xa + 1) for By and Bz, right of them zero.
+template <class F>
+__device__ inline double cg_at(const CgArgs &A, int l, int a, int b, F v) {
+  const int bw = wrapm(b, A.ny);
+  if (!A.X.wall) return v(wrapm(a, A.nx), bw);
+  if (a < A.X.xa) return l == 0 ? -v(A.X.xa, bw) : v(A.X.xa + 1, bw);
+  if (a > A.X.xb) return 0.0;
+  return v(a, bw);
 }
 
 // Initial residual (field.f90:350-385): b = f5*gkl, r = b + N4(phi) -
@@ -280,7 +346,9 @@ __global__ __launch_bounds__(CGB) void k_cg_init(CgArgs A, const double *__restr
   if (C.in) {
     const double *phi = df + (size_t)l * A.ps;
     auto PH = [&](int a, int b) {
-      return phi[(size_t)(wrapm(a, A.nx) + 2) + (size_t)A.gx * (size_t)(wrapm(b, A.ny) + 2)];
+      return cg_at(A, l, a, b, [&](int x, int y) {
+        return phi[(size_t)(x + 2) + (size_t)A.gx * (size_t)(y + 2)];
+      });
     };
     const double b = A.f5 * gkl[(size_t)l * A.n + t];
     const double rr = b + PH(ii, jj - 1) + PH(ii - 1, jj) - A.f4 * PH(ii, jj) +
@@ -322,8 +390,10 @@ __global__ __launch_bounds__(CGB) void k_cg_pap(CgArgs A, const double *__restri
   if (C.in) {
     const double *R = r + (size_t)l * A.n, *P = pold + (size_t)l * A.n;
     auto PN = [&](int a, int b) {
-      const int q = wrapm(a, A.nx) + A.nx * wrapm(b, A.ny);
-      return __fma_rn(bv, P[q], R[q]);
+      return cg_at(A, l, a, b, [&](int x, int y) {
+        const int q = x + A.nx * y;
+        return __fma_rn(bv, P[q], R[q]);
+      });
     };
     const double pc = PN(ii, jj);
     const double ap = apply_a(A.f4, pc, PN(ii, jj - 1), PN(ii - 1, jj), PN(ii + 1, jj), PN(ii, jj + 1));
@@ -347,7 +417,9 @@ __global__ __launch_bounds__(CGB) void k_cg_update(CgArgs A, double *__restrict_
   double v[1] = {0.0};
   if (C.in) {
     const double *P = p + (size_t)l * A.n;
-    auto PP = [&](int a, int b) { return P[wrapm(a, A.nx) + A.nx * wrapm(b, A.ny)]; };
+    auto PP = [&](int a, int b) {
+      return cg_at(A, l, a, b, [&](int x, int y) { return P[x + A.nx * y]; });
+    };
     const double pc = P[t];
     const double ap = apply_a(A.f4, pc, PP(ii, jj - 1), PP(ii - 1, jj), PP(ii + 1, jj), PP(ii, jj + 1));
     double *phi = df + (size_t)l * A.ps + (size_t)(ii + 2) + (size_t)A.gx * (size_t)(jj + 2);
@@ -581,26 +653,55 @@ __global__ __launch_bounds__(256) void k_energy_fld(FieldGeom fg, const double *
 }  // namespace
 
 // -------------------------------------------------------------- launchers
+XSide wm_xside(const wm_ctx *ctx) {
+  XSide X;
+  X.wall = ctx->bcf;
+  X.xa = ctx->xe > 0 ? ctx->xs - ctx->g.nxgs : 0;
+  X.xb = ctx->xe > 0 ? ctx->xe - ctx->g.nxgs : ctx->g.nx - 1;
+  return X;
+}
+
+// The wall families fold the current in y only, over columns nxs-2 ..
+// nxe+2 (boundary_shock.f90:410-548).
 int wm_k_bc_curre(wm_ctx *ctx, double *uj) {
   const FieldGeom &fg = ctx->fg;
-  hipLaunchKernelGGL(k_curre_rows, dim3(wm_cdiv(fg.gx * 3, 256)), dim3(256), 0, ctx->stream, fg, uj);
-  hipLaunchKernelGGL(k_curre_cols, dim3(wm_cdiv(fg.gy * 3, 256)), dim3(256), 0, ctx->stream, fg, uj);
+  const XSide X = wm_xside(ctx);
+  const int cnt = X.xb - X.xa + 5;
+  hipLaunchKernelGGL(k_curre_rows, dim3(wm_cdiv(cnt * 3, 256)), dim3(256), 0, ctx->stream, fg, uj,
+                     X.xa, cnt);
+  if (ctx->bcf == WM_BC_PERIODIC)
+    hipLaunchKernelGGL(k_curre_cols, dim3(wm_cdiv(fg.gy * 3, 256)), dim3(256), 0, ctx->stream, fg, uj);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
 int wm_k_bc_dfield(wm_ctx *ctx, double *df) {
   const FieldGeom &fg = ctx->fg;
-  hipLaunchKernelGGL(k_dfield_rows, dim3(wm_cdiv(fg.nx * 6, 256)), dim3(256), 0, ctx->stream, fg, df, 6);
-  hipLaunchKernelGGL(k_dfield_cols, dim3(wm_cdiv(fg.gy * 6, 256)), dim3(256), 0, ctx->stream, fg, df, 6);
+  const XSide X = wm_xside(ctx);
+  const int cnt = X.xb - X.xa + 1;
+  hipLaunchKernelGGL(k_dfield_rows, dim3(wm_cdiv(cnt * 6, 256)), dim3(256), 0, ctx->stream, fg, df,
+                     6, X.xa, cnt);
+  if (X.wall)
+    hipLaunchKernelGGL(k_dfield_walls, dim3(wm_cdiv(fg.gy, 256)), dim3(256), 0, ctx->stream, fg,
+                       df, X);
+  else
+    hipLaunchKernelGGL(k_dfield_cols, dim3(wm_cdiv(fg.gy * 6, 256)), dim3(256), 0, ctx->stream, fg,
+                       df, 6);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
-int wm_k_bc_phi(wm_ctx *ctx, double *phi) {
-  const int nx = ctx->g.nx, ny = ctx->g.ny;
+// phi(nxs-1:nxe+1, nys-1:nye+1) of component l on the device.
+int wm_k_bc_phi(wm_ctx *ctx, double *phi, int l) {
+  const XSide X = wm_xside(ctx);
+  const int nx = X.xb - X.xa + 1, ny = ctx->g.ny;
   hipLaunchKernelGGL(k_phi_rows, dim3(wm_cdiv(nx, 256)), dim3(256), 0, ctx->stream, nx, ny, phi);
-  hipLaunchKernelGGL(k_phi_cols, dim3(wm_cdiv(ny + 2, 256)), dim3(256), 0, ctx->stream, nx, ny, phi);
+  if (X.wall)
+    hipLaunchKernelGGL(k_phi_walls, dim3(wm_cdiv(ny + 2, 256)), dim3(256), 0, ctx->stream, nx, ny,
+                       phi, l);
+  else
+    hipLaunchKernelGGL(k_phi_cols, dim3(wm_cdiv(ny + 2, 256)), dim3(256), 0, ctx->stream, nx, ny,
+                       phi);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
@@ -682,7 +783,8 @@ int wm_k_field_solve(wm_ctx *ctx) {
   const FieldGeom &fg = ctx->fg;
   const FC f = make_fc(g);
   const int n = g.nx * g.ny;
-  hipLaunchKernelGGL(k_rhs, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, ctx->uf,
+  const XSide X = wm_xside(ctx);
+  hipLaunchKernelGGL(k_rhs, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, X, ctx->uf,
                      ctx->uj, ctx->gkl);
   WM_HIP(hipGetLastError());
 
@@ -691,20 +793,24 @@ int wm_k_field_solve(wm_ctx *ctx) {
   A.nbx = wm_cdiv(g.nx, CGB);
   A.nblk = A.nbx * g.ny;
   A.f4 = f.f4; A.f5 = f.f5; A.err = ctx->cg_err; A.ite_max = ctx->ite_max;
+  A.X = X;
   const dim3 grid(A.nblk, 3);
   int fcg_it[3] = {0, 0, 0};
-  if (ctx->solver == WM_SOLVER_FCG) WM_TRY(fcg_solve(ctx, A, fcg_it));
+  // the learned preconditioner was trained on the periodic operator: the
+  // wall families solve with the reference's plain CG
+  const bool fcg = ctx->solver == WM_SOLVER_FCG && !X.wall;
+  if (fcg) WM_TRY(fcg_solve(ctx, A, fcg_it));
   // the reference's plain CG; after flexible CG it recomputes the true
   // residual and only iterates if that misses the tolerance
   double *pbuf[2] = {ctx->cg_p[0], ctx->cg_p[1]};
   // after flexible CG the judge also copies phi to cg_x0: if no plain
   // iteration follows, that is the next flexible solve's warm start
-  double *x0 = ctx->solver == WM_SOLVER_FCG ? ctx->cg_x0 : nullptr;
+  double *x0 = fcg ? ctx->cg_x0 : nullptr;
   hipLaunchKernelGGL(k_cg_init, grid, dim3(CGB), 0, ctx->stream, A, ctx->gkl, ctx->df,
                      ctx->cg_r, pbuf[0], ctx->red, x0);
   hipLaunchKernelGGL(k_cg_fin, dim3(3), dim3(CGB), 0, ctx->stream, A, 0, ctx->red, ctx->cg_s);
   WM_HIP(hipGetLastError());
-  const bool collect = ctx->solver == WM_SOLVER_CG && wm_pc_collecting(ctx);
+  const bool collect = ctx->solver == WM_SOLVER_CG && !X.wall && wm_pc_collecting(ctx);
   static_assert(3 * sizeof(CgScal) <= WM_PIN_BYTES, "pinned scratch too small");
   CgScal *hs = (CgScal *)ctx->h_pin;
   int iters = 0, cur = 0;
@@ -748,20 +854,26 @@ int wm_k_field_solve(wm_ctx *ctx) {
   ctx->x0_fresh = x0 != nullptr && iters == 0;
   if (iters > 0) ctx->cg_hint = std::max(hs[0].ite, std::max(hs[1].ite, hs[2].ite));
   for (int l = 0; l < 3; ++l) ctx->last_iters[l] = fcg_it[l] + hs[l].ite;
-  if (ctx->solver == WM_SOLVER_FCG) {
+  if (fcg) {
     int fb[3] = {hs[0].ite, hs[1].ite, hs[2].ite};
     wm_pc_set_stats(ctx, fcg_it, fb);
   }
   ctx->cg_iters_total += ctx->last_iters[0] + ctx->last_iters[1] + ctx->last_iters[2];
 
   WM_TRY(wm_k_bc_dfield(ctx, ctx->df));
-  hipLaunchKernelGGL(k_delta_e, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, ctx->uf,
-                     ctx->uj, ctx->df);
+  hipLaunchKernelGGL(k_delta_e, dim3(wm_cdiv(n, 256)), dim3(256), 0, ctx->stream, fg, f, X,
+                     ctx->uf, ctx->uj, ctx->df);
   WM_HIP(hipGetLastError());
   WM_TRY(wm_k_bc_dfield(ctx, ctx->df));
-  const size_t nall = 6 * fg.ps;
-  hipLaunchKernelGGL(k_add, dim3(wm_cdiv((long long)nall, 256)), dim3(256), 0, ctx->stream, nall,
-                     ctx->uf, ctx->df);
+  if (X.xa == 0 && X.xb == g.nx - 1) {
+    const size_t nall = 6 * fg.ps;
+    hipLaunchKernelGGL(k_add, dim3(wm_cdiv((long long)nall, 256)), dim3(256), 0, ctx->stream, nall,
+                       ctx->uf, ctx->df);
+  } else {
+    const long long nadd = 6LL * fg.gy * (X.xb - X.xa + 5);
+    hipLaunchKernelGGL(k_add_cols, dim3(wm_cdiv(nadd, 256)), dim3(256), 0, ctx->stream, fg, X,
+                       ctx->uf, ctx->df);
+  }
   WM_HIP(hipGetLastError());
   return WM_OK;
 }

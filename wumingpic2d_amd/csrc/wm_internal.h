@@ -46,9 +46,7 @@ enum : int {
   DERR_NP_OVER = 2,   // a row exceeds the device capacity
   DERR_MOVER_OVER = 4,// a row has more migrating particles than the buffer
   DERR_JUMP = 8,      // a particle moved more than one cell in one step
-  DERR_CG_ITEMAX = 16,// CG reached ite_max
-  DERR_RELAYOUT = 32  // FP32 form: a bin would overflow with the pending movers (not an
-                      // error: the host lays the flagged rows out anew before the next push)
+  DERR_CG_ITEMAX = 16 // CG reached ite_max
 };
 
 // ------------------------------------------------------------- parameters
@@ -58,6 +56,12 @@ struct wm_grid {
   int nx, ny;
   double delx, delt, c, gfac;
   double q[WM_MAXSP], r[WM_MAXSP];
+};
+
+// The x side of the boundaries as the kernels see it: wall = 0 periodic,
+// else the family; [xa, xb] the active interior columns, 0-based.
+struct XSide {
+  int wall, xa, xb;
 };
 
 // Field plane geometry (ghost-inclusive, 2 ghost cells).
@@ -113,8 +117,9 @@ struct __align__(32) BinSum {
 // made): [0] particles in one bin, [1] particles leaving one bin in a push,
 // [2] movers of one push workgroup, [3] records of one row's mover pool,
 // [4] entries of one row's spill of the push's mover lists, [5] bins found
-// full by the fill check (a sum).
+// full by the fill check, [6] blocks and [7] segments laid out anew (sums).
 #define WM_NSTAT 8
+#define WM_RB 64               // cells of a re-layout block
 
 using PBank = Rec64;
 
@@ -130,6 +135,12 @@ struct wm_ctx {
   // init flags and parameters (one grid per context)
   bool pinit = false, finit = false, sinit = false, binit = false, minit = false;
   bool have_grid = false;
+  // The boundary family (WM_BC_*) whose __init was called, the active x
+  // range [xs, xe] the reference passes as nxs, nxe (xe = 0: nxgs..nxge),
+  // and the shock's upstream four-velocity u0 (its injection edge).
+  int bcf = 0;
+  int xs = 0, xe = 0;
+  double u0 = 0.0;
   wm_grid g{};
   double cg_err = 1e-6;    // field.f90:339
   int ite_max = 100;       // field.f90:337
@@ -191,7 +202,7 @@ struct wm_ctx {
   // [bstart[seg * (nx + 1) + i], bstart[seg * (nx + 1) + i + 1]) of segment seg
   // (seg * cap ..) and holds bin_cnt[bin] particles at its front.  The layout
   // gives each bin room beyond its count; a bin that would overflow gets its
-  // row laid out anew (wm_relayout).
+  // block of cells laid out anew (wm_relayout).
   Bins32 bins;
   int *bstart = nullptr;
   int *bin_cnt = nullptr;        // particles per bin
@@ -216,17 +227,22 @@ struct wm_ctx {
   int mv_last = 0;               // the buffer the last push wrote
   bool mv_pending = false;       // some rows' bins still await its movers
   int *rowpend = nullptr;        // [ny] 1: the row's bins await the last push's movers
-  int *rowflag = nullptr;        // [ny] 1: lay the row out anew before the next push
   int *pcnt = nullptr;           // [nbin] counts once the pending movers are placed
-  int *fstat = nullptr;          // [WM_NSTAT] capacity statistics (maxima)
-  long long relayouts = 0;       // rows laid out anew since the state was made
-  // re-layout workspace: a batch of relay_rows rows (all species)
-  int relay_rows = 0;
-  Bins32 rtmp;
-  int64_t *rtmp_id = nullptr;
-  int *rtmp_bstart = nullptr;
-  int *rtmp_blk = nullptr;       // per batch segment and block of 64 cells: re-laid
-  int *relay_list = nullptr;
+  int *fstat = nullptr;          // [WM_NSTAT] capacity statistics
+  // Re-layout, on the device after the fill check: blocks of WM_RB cells
+  // with a bin that would overflow are listed (blkflag dedups), each laid out
+  // anew inside its own span; a block whose span cannot hold its bins lists
+  // its segment, laid out anew as a whole.  Bins move verbatim (holes
+  // included), so the pending movers' bin-relative slots stay valid.
+  int *relay_ctl = nullptr;      // [4] listed blocks, listed segments
+  int *relay_blocks = nullptr;   // [nseg * nblk] listed blocks (seg * nblk + block)
+  int *relay_segs = nullptr;     // [nseg]
+  int *blkflag = nullptr;        // [nseg * nblk]
+  int *segflag = nullptr;        // [nseg]
+  Bins32 rws;                    // workspace: relay_nwb blocks of relay_wsz slots, or
+  int64_t *rws_id = nullptr;     // relay_nws segments of cap slots
+  int *rws_bs = nullptr;         // [relay_nws][2 (nx + 1)] a segment's new layout, count prefix
+  int relay_wsz = 0, relay_nwb = 0, relay_nws = 0;
   double *fscratch = nullptr;    // 6 planes for the host-array boundary calls
   // restart files (paraio__init): the directory and the run's decomposition
   bool ioinit = false;
@@ -265,13 +281,17 @@ int wm_k_push_deposit(wm_ctx *ctx, int src, int dst);
 int wm_k_deposit_gather(wm_ctx *ctx);
 // Fields (wm_field.hip)
 int wm_k_bc_curre(wm_ctx *ctx, double *uj);
+XSide wm_xside(const wm_ctx *ctx);
 int wm_k_bc_dfield(wm_ctx *ctx, double *df);
-int wm_k_bc_phi(wm_ctx *ctx, double *phi);
+int wm_k_bc_phi(wm_ctx *ctx, double *phi, int l);   // phi(nxs-1:nxe+1, nys-1:nye+1)
 int wm_k_field_solve(wm_ctx *ctx);   // RHS .. uf += df (after the current is ready)
 int wm_k_energy(wm_ctx *ctx, int bank, double *out_host);
 // Particle boundaries (wm_bc.hip)
 int wm_k_particle_x(wm_ctx *ctx, int bank);
-int wm_k_particle_y(wm_ctx *ctx, int bank, bool fuse_x);
+// xmode: 0 y only, 1 with the periodic x wrap fused, 2 with the walls' mirrors fused
+int wm_k_particle_y(wm_ctx *ctx, int bank, int xmode);
+int wm_k_injection(wm_ctx *ctx, int bank, double u0);   // boundary_shock__injection
+double wm_injection_xend(const wm_ctx *ctx, double u0);
 // Sort (wm_sort.hip)
 int wm_k_sort(wm_ctx *ctx, int src, int dst);
 // FP32 form, cell bins (wm_sort.hip): dense rows <-> bins, movers into
@@ -294,8 +314,8 @@ int wm_rows_done(wm_ctx *ctx, const int *np2_host);
 int wm_field_out(wm_ctx *ctx, double *uf);
 int wm_field_in(wm_ctx *ctx, const double *uf);
 int wm_settle_movers(wm_ctx *ctx);   // the last push's movers into their bins, if pending
-int wm_k_fill_check(wm_ctx *ctx);    // projected bin counts after a push; flags rows to re-lay
-int wm_relayout(wm_ctx *ctx);        // flagged rows laid out anew, their movers placed
+int wm_k_fill_check(wm_ctx *ctx);    // projected bin counts after a push; lists blocks to re-lay
+int wm_relayout(wm_ctx *ctx);        // listed blocks (segments) laid out anew, on the device
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);
 int wm_k_push_deposit32(wm_ctx *ctx);   // in place on the bins (pending movers first)

@@ -1,7 +1,6 @@
 // wm_migrate.h -- the FP32 form's migration of one cell bin: the particles
 // that left their cells in a push join their destination bins.  Used by the
-// settle kernel (k_migrate), by the next push's prologue, and (placing the
-// bin anew) by the re-layout of a row.
+// settle kernel (k_migrate) and by the next push's prologue.
 #pragma once
 #include "wm_internal.h"
 
@@ -253,71 +252,6 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   }
   G.cnt[d] = cn;
   return cn;
-}
-
-// Bin d placed anew (re-layout of its row): its particles that stay, in slot
-// order, then its arrivals in the order migrate_bin would number them, written
-// from slot 0 of the bin's new place dbase in the arrays D (IDs dids), by the
-// same group of 8 lanes.  Returns the new count to lane 0 (-1 to the others).
-WM_MIG_FN int place_bin(const Mig &G, const Movers &M, const Bins32 &A, const int64_t *ids,
-                        const Bins32 &D, int64_t *dids, size_t d, int i, int jl, int sp, int k,
-                        size_t base, int capd, size_t dbase, int dcap, bool pending, int *derr) {
-  const int n = min(G.cnt[d], capd);
-  int m = 0, c = 0, si = 0, sjl = 0;
-  uint4 oh{}, ol{}, fh{}, fl{};
-  if (pending) {
-    const size_t src = source_bin(G, i, jl, sp, k, si, sjl);
-    const uint4 *own = (const uint4 *)(G.sum + d), *from = (const uint4 *)(G.sum + src);
-    oh = own[0];
-    ol = own[1];
-    fh = from[0];
-    fl = from[1];
-    c = sum_cnt(fh.x, fh.y, k);
-    m = (int)oh.w;
-  }
-  const size_t r0 = bin_records(G, jl, oh.z);
-  int inc = c;
-#pragma unroll
-  for (int o = 1; o < DL; o <<= 1) {
-    const int v = __shfl_up(inc, o, DL);
-    if (k >= o) inc += v;
-  }
-  const int total = __shfl(inc, DL - 1, DL);
-  const int n1 = n - m;
-  // the particles that stay: lane k takes slots t = k, k + 8, ... below n; its
-  // rank among them is t minus the holes below t (the holes ascend)
-  for (int t = k, h = 0; t < n; t += DL) {
-    while (h < m && hole_slot(M, r0, ol, h) < t) ++h;
-    if (h < m && hole_slot(M, r0, ol, h) == t) continue;
-    const int r = t - h;
-    if (r >= dcap) {
-      atomicOr(derr, DERR_NP_OVER);
-      continue;
-    }
-    D.q[dbase + r] = A.q[base + t];
-    D.uz[dbase + r] = A.uz[base + t];
-    if (ids) dids[dbase + r] = ids[base + t];
-  }
-  // arrivals of direction k after them
-  int a = inc - c;
-  const int ms = c > 0 ? (int)fh.w : 0;
-  const size_t rs = c > 0 ? bin_records(G, sjl, fh.z) : 0;
-  for (int j = 0, left = c; left > 0 && j < ms; ++j) {
-    if (list_dir(M, fl, rs, j) != k) continue;
-    const MoverRec rec = M.rec[rs + j];
-    const int r = n1 + a;
-    ++a;
-    --left;
-    if (r >= dcap) {
-      atomicOr(derr, DERR_NP_OVER);
-      continue;
-    }
-    D.q[dbase + r] = make_float4(rec.ox1, rec.oy1, rec.ux, rec.uy);
-    D.uz[dbase + r] = rec.uz;
-    if (ids) dids[dbase + r] = rec.id;
-  }
-  if (k != 0) return -1;
-  return min(n1 + total, dcap);
 }
 
 }  // namespace wm_mig

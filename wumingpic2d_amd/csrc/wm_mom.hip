@@ -303,14 +303,17 @@ __global__ void k_mom_gather(MP P, const double *__restrict__ part, double *__re
   mom[t] = sum;
 }
 
-// boundary_periodic.f90:579-584: the x ghosts into the opposite edge, every row.
-__global__ void k_mom_fold_x(int nx, int ny, int nsp, double *mom) {
+// boundary_periodic.f90:579-584: the x ghosts into the opposite edge, every
+// row; the wall families fold each into its own edge (boundary_shock.f90:
+// 636-641, boundary_reconnection.f90:590-595).
+__global__ void k_mom_fold_x(int nx, int ny, int nsp, double *mom, int wall) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= NMOM * (ny + 2) * nsp) return;
   const int m = t % NMOM, row = t / NMOM;        // row = b + (ny+2) * s
   double *r = mom + m + (size_t)NMOM * (nx + 2) * row;
-  r[NMOM * 1] += r[NMOM * (nx + 1)];
-  r[NMOM * nx] += r[0];
+  const double lo = r[0], hi = r[NMOM * (nx + 1)];
+  r[NMOM * 1] += wall ? lo : hi;
+  r[NMOM * nx] += wall ? hi : lo;
 }
 
 // boundary_periodic.f90:586-633 on one rank: row nys-1 into row nye, then
@@ -402,7 +405,7 @@ int wm_k_bc_mom(wm_ctx *ctx, double *mom) {
   const wm_grid &g = ctx->g;
   const int nxr = NMOM * (g.ny + 2) * g.nsp, nyr = NMOM * (g.nx + 2) * g.nsp;
   hipLaunchKernelGGL(k_mom_fold_x, dim3(wm_cdiv(nxr, 256)), dim3(256), 0, ctx->stream, g.nx, g.ny,
-                     g.nsp, mom);
+                     g.nsp, mom, ctx->bcf != WM_BC_PERIODIC);
   WM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_mom_fold_y, dim3(wm_cdiv(nyr, 256)), dim3(256), 0, ctx->stream, g.nx, g.ny,
                      g.nsp, mom);

@@ -30,7 +30,27 @@ struct PP {                           // particle-kernel parameters
   int nx, ny, nxgs, nys, nsp, cap, nchunk;
   double d_delx, delt, c, qc_scale;   // qc_scale = delx / delt
   double q[WM_MAXSP], r[WM_MAXSP];
+  // the shock's injection edge, applied to the pushed particle before its
+  // current (proj/shock/app.f90:112-114): inj, nxs, the left mirror point
+  // 2 (nxs+1) delx, xend and u0 (boundary_shock.f90:266-294)
+  int inj, xs;
+  double delx, wl, xend, u0;
 };
+
+__device__ inline void injection_edge(double &x, double &ux, double &uy, double &uz,
+                                      const PP &P) {
+  if ((int)(x / P.delx) < P.xs + 1) {
+    x = P.wl - x;
+    ux = -ux;
+  } else if (x > P.xend) {
+    x = 2.0 * P.xend - x;
+    ux = 2.0 * P.u0 - ux;
+  } else {
+    return;
+  }
+  uy = -uy;
+  uz = -uz;
+}
 
 __device__ inline void shape(double dh, double &sm, double &s0, double &sp) {
   // 2nd-order (TSC) weights: particle.f90:98-100
@@ -349,6 +369,7 @@ __global__ __launch_bounds__(256, 3) void k_particle(
         __asm__ volatile("" ::: "memory");
         if (PUSH) {
           PushOut o = boris(T, g, x0, y0, cux, cuy, cuz, i, j, P, fac1, txxx, fac2);
+          if (P.inj) injection_edge(o.x, o.ux, o.uy, o.uz, P);
           xb[p] = o.x; yb[p] = o.y; uxb[p] = o.ux; uyb[p] = o.uy; uzb[p] = o.uz;
           x1 = o.x; y1 = o.y; ux = o.ux; uy = o.uy; uz = o.uz;
         } else {
@@ -463,12 +484,23 @@ PP make_pp(const wm_ctx *ctx) {
     P.q[s] = s < g.nsp ? g.q[s] : 0.0;
     P.r[s] = s < g.nsp ? g.r[s] : 1.0;
   }
+  P.inj = 0;
+  P.xs = ctx->xe > 0 ? ctx->xs : g.nxgs;
+  P.delx = g.delx;
+  P.wl = 2.0 * (P.xs + 1) * g.delx;
+  P.xend = 0.0;
+  P.u0 = 0.0;
   return P;
 }
 
 template <bool PUSH, bool DEP>
-int launch_particle(wm_ctx *ctx, int a, int b) {
+int launch_particle(wm_ctx *ctx, int a, int b, bool inj = false) {
   PP P = make_pp(ctx);
+  if (inj) {
+    P.inj = 1;
+    P.u0 = ctx->u0;
+    P.xend = wm_injection_xend(ctx, ctx->u0);
+  }
   dim3 grid(P.nchunk, P.ny);
   const int TX = ctx->g.nx + 2;
   const size_t tps = (size_t)TX * (size_t)(ctx->g.ny + 2);
@@ -498,7 +530,11 @@ int wm_k_cell_fields(wm_ctx *ctx) {
 
 int wm_k_push(wm_ctx *ctx, int src, int dst) { return launch_particle<true, false>(ctx, src, dst); }
 int wm_k_deposit(wm_ctx *ctx, int oldb, int newb) { return launch_particle<false, true>(ctx, oldb, newb); }
-int wm_k_push_deposit(wm_ctx *ctx, int src, int dst) { return launch_particle<true, true>(ctx, src, dst); }
+// The device-resident shock step fuses its injection edge between the push
+// and the current.
+int wm_k_push_deposit(wm_ctx *ctx, int src, int dst) {
+  return launch_particle<true, true>(ctx, src, dst, ctx->bcf == WM_BC_SHOCK);
+}
 
 int wm_k_deposit_gather(wm_ctx *ctx) {
   PP P = make_pp(ctx);

@@ -269,12 +269,22 @@ __global__ __launch_bounds__(256) void k_bins_to_stage(SB P, int seg0, int w, in
   }
 }
 
+// The re-layout's lists and workspace (wm_ctx::relay_*).
+struct Relay {
+  int *ctl;                      // [0] listed blocks, [1] listed segments
+  int *blocks, *segs, *blkflag, *segflag;
+  Bins32 ws;
+  int64_t *ws_id;
+  int *ws_bs;                    // per segment workgroup: new layout, count prefix
+  int wsz;                       // slots per block workgroup
+};
+
 // After a push: every bin's count once the push's movers are placed
 // (count - movers out + arrivals from the 8 neighbours' summaries), into pc.
-// A bin whose count would exceed its room flags its row for a re-layout.
+// A bin whose count would exceed its room lists its block of WM_RB cells for
+// the re-layout; a count above WM_BIN_MAX is the reference's memory over.
 // Also keeps the capacity statistics.
-__global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
-                                                    int *__restrict__ rowflag,
+__global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc, Relay R,
                                                     const int *__restrict__ mfill,
                                                     const int *__restrict__ sfill,
                                                     int *__restrict__ stats, int *derr) {
@@ -300,7 +310,12 @@ __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
     int capd;
     bin_span(G, sj, i, base, capd);
     over = cnew > capd;
-    if (over && !rowflag[jl]) atomicOr(rowflag + jl, 1);
+    if (cnew > WM_BIN_MAX) atomicOr(derr, DERR_NP_OVER);
+    if (over) {
+      const int nblk = (G.nx + WM_RB - 1) / WM_RB;
+      const int b = (int)sj * nblk + i / WM_RB;
+      if (atomicOr(R.blkflag + b, 1) == 0) R.blocks[atomicAdd(R.ctl, 1)] = b;
+    }
   }
   // maxima: one lane per wave, and an atomic only where it raises the value
   // (atomics on one address from every wave would serialise)
@@ -310,12 +325,11 @@ __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
     a = max(a, __shfl_xor(a, o, 64));
     m = max(m, __shfl_xor(m, o, 64));
   }
-  const bool any = __ballot(over) != 0ULL;
+  const unsigned long long fb = __ballot(over);
   if ((threadIdx.x & 63) == 0) {
     if (a > stats[0]) atomicMax(stats + 0, a);
     if (m > stats[1]) atomicMax(stats + 1, m);
-    if (any) atomicOr(derr, DERR_RELAYOUT);
-    if (any) atomicAdd(stats + 5, __popcll(__ballot(over)));
+    if (fb) atomicAdd(stats + 5, __popcll(fb));
   }
   if (d < (size_t)G.ny) {
     if (mfill[d] > stats[3]) atomicMax(stats + 3, mfill[d]);
@@ -323,138 +337,177 @@ __global__ __launch_bounds__(256) void k_fill_check(Mig G, int *__restrict__ pc,
   }
 }
 
-// Re-layout.  A row's bins are laid out anew block by block: a block of RB
-// cells with a bin that would overflow gets a new layout inside the slots
-// its bins span now, each bin its need plus a share of the block's free
-// slots in proportion to its need; when the block's span cannot hold its
-// bins' needs the whole segment is laid out anew (wm_bins::wave_layout).
-// Every other bin keeps its place and takes its pending movers in place, as
-// the next push's prologue would, so the whole row holds its movers after.
-constexpr int RB = 64;
+// Copies the n(j) particles of bins j < nb (old first slot ob(j), new first
+// slot nb_(j), count prefix pre(j)) of segment seg through the workspace W:
+// first all of them out, flat in prefix order, then all back at their new
+// places.  Bins move verbatim, holes included: slot k of a bin stays slot k.
+// Every thread of the workgroup calls it.
+template <class OB, class NB, class PRE>
+__device__ void move_bins(const SB &P, int seg, int nb, OB ob, NB nb_, PRE pre, Bins32 A,
+                          int64_t *ids, Bins32 W, int64_t *wid) {
+  const int T = pre(nb);
+  const size_t s0 = seg_at(P, seg);
+  const int stride = blockDim.x;
+  // flat index f -> bin j: f grows by the block size, j follows it
+  int j = 0;
+  for (int f = threadIdx.x; f < T; f += stride) {
+    while (f >= pre(j + 1)) ++j;
+    const size_t src = s0 + ob(j) + (f - pre(j));
+    W.q[f] = A.q[src];
+    W.uz[f] = A.uz[src];
+    if (ids) wid[f] = ids[src];
+  }
+  __syncthreads();
+  j = 0;
+  for (int f = threadIdx.x; f < T; f += stride) {
+    while (f >= pre(j + 1)) ++j;
+    const size_t dst = s0 + nb_(j) + (f - pre(j));
+    A.q[dst] = W.q[f];
+    A.uz[dst] = W.uz[f];
+    if (ids) ids[dst] = wid[f];
+  }
+  __syncthreads();
+}
 
-// Step 1, one wavefront per segment of the listed rows (batch row b,
-// species s: block b * nsp + s): the new first slots into tbs and, per
-// block of RB cells, whether its bins move (rblk).
-__global__ __launch_bounds__(SW) void k_relayout_plan(SB P, const int *__restrict__ list,
+// Step 1: each listed block (WM_RB cells; a workgroup of 256 threads per
+// block at a time) is laid out anew inside the slots its bins span now, by
+// wm_bins' policy on the counts max(count now, count once the movers are
+// placed): wave 0 plans, the workgroup moves the bins.  A block whose span
+// cannot hold its needs with 1/16 of it to spare, or whose particles exceed
+// the workspace, lists its segment for step 2.
+constexpr int RW = 256;   // threads of a re-layout workgroup
+__global__ __launch_bounds__(RW) void k_relayout_blocks(SB P, Relay R, const int *__restrict__ cnt,
+                                                        const int *__restrict__ pc,
+                                                        int *__restrict__ bstart, Bins32 A,
+                                                        int64_t *__restrict__ ids) {
+  __shared__ int s_old[WM_RB], s_new[WM_RB], s_pre[WM_RB + 1];
+  __shared__ int s_go;
+  const int nblk = (P.nx + WM_RB - 1) / WM_RB;
+  const int nlist = R.ctl[0];
+  Bins32 W{R.ws.q + (size_t)blockIdx.x * R.wsz, R.ws.uz + (size_t)blockIdx.x * R.wsz};
+  int64_t *wid = R.ws_id ? R.ws_id + (size_t)blockIdx.x * R.wsz : nullptr;
+  for (int e = blockIdx.x; e < nlist; e += gridDim.x) {
+    const int b = R.blocks[e];
+    const int seg = b / nblk, i0 = (b % nblk) * WM_RB, i1 = min(P.nx, i0 + WM_RB), nb = i1 - i0;
+    int *bs = bstart + (size_t)seg * (P.nx + 1);
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x, i = i0 + lane;
+      const bool in = lane < nb;
+      const int n = in ? cnt[(size_t)seg * P.nx + i] : 0;
+      const int c = in ? max(n, pc[(size_t)seg * P.nx + i]) : 0;
+      const int lo = bs[i0], hi = bs[i1];
+      long long C = c;
+      int T = n;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        C += __shfl_xor(C, o, 64);
+        T += __shfl_xor(T, o, 64);
+      }
+      const int mean = (int)(C / nb);
+      long long S1 = in ? wm_bins::need_of(c, mean) : 0, S0 = in ? wm_bins::need_of(c, 0) : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        S1 += __shfl_xor(S1, o, 64);
+        S0 += __shfl_xor(S0, o, 64);
+      }
+      // a block left with less than 1/16 of its span free would fill again
+      // within steps: its whole segment is laid out anew instead, spreading
+      // the row's free slots over every bin
+      wm_bins::Policy pol;
+      const bool ok = wm_bins::make_policy(C, S1, S0, nb, hi - lo, pol) && T <= R.wsz &&
+                      16 * pol.units * WM_BIN_ALIGN >= hi - lo;
+      const int room = in ? wm_bins::room_of(pol, c) : 0;
+      int incl = room, ninc = n;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64), w = __shfl_up(ninc, o, 64);
+        if (lane >= o) {
+          incl += v;
+          ninc += w;
+        }
+      }
+      if (in) {
+        s_old[lane] = bs[i];
+        s_new[lane] = lo + incl - room;
+        s_pre[lane + 1] = ninc;
+      }
+      if (lane == 0) {
+        s_pre[0] = 0;
+        s_go = ok;
+        R.blkflag[b] = 0;
+        if (!ok && atomicOr(R.segflag + seg, 1) == 0) R.segs[atomicAdd(R.ctl + 1, 1)] = seg;
+      }
+    }
+    __syncthreads();
+    if (s_go) {
+      move_bins(
+          P, seg, nb, [&](int j) { return s_old[j]; }, [&](int j) { return s_new[j]; },
+          [&](int j) { return s_pre[j]; }, A, ids, W, wid);
+      if (threadIdx.x < nb) bs[i0 + threadIdx.x] = s_new[threadIdx.x];
+    }
+    __syncthreads();
+  }
+}
+
+// Step 2: each listed segment (a workgroup at a time) is laid out anew as a
+// whole (wm_bins::wave_layout on the same counts) and its bins moved through
+// a workspace of a segment's slots.  A segment whose counts do not fit is
+// the reference's memory over.
+__global__ __launch_bounds__(RW) void k_relayout_segs(SB P, Relay R, const int *__restrict__ cnt,
                                                       const int *__restrict__ pc,
-                                                      const int *__restrict__ bstart,
-                                                      int *__restrict__ tbs, int *__restrict__ rblk,
-                                                      int *derr) {
-  const int bsg = blockIdx.x, lane = threadIdx.x;
-  const int seg = (bsg % P.nsp) * P.ny + list[bsg / P.nsp];
-  const int *c = pc + (size_t)seg * P.nx;
-  const int *bs = bstart + (size_t)seg * (P.nx + 1);
-  const int nblk = (P.nx + RB - 1) / RB;
-  int *t = tbs + (size_t)bsg * (P.nx + 1);
-  int *rb = rblk + (size_t)bsg * nblk;
-  bool full = false;
-  for (int blk = 0; blk < nblk && !full; ++blk) {
-    const int i0 = blk * RB, i1 = min(P.nx, i0 + RB), i = i0 + lane;
-    const bool in = i < i1;
-    const int cnt = in ? c[i] : 0, b0 = in ? bs[i] : 0, b1 = in ? bs[i + 1] : 0;
-    if (__ballot(in && cnt > b1 - b0) == 0ULL) {             // no bin of the block is full
-      if (in) t[i] = b0;
-      if (lane == 0) rb[blk] = 0;
-      continue;
-    }
-    const int lo = bs[i0], hi = bs[i1];
-    const int need = in ? wm_bins::bin_need(cnt) : 0;
-    int S = need;
+                                                      int *__restrict__ bstart, Bins32 A,
+                                                      int64_t *__restrict__ ids, int *derr) {
+  __shared__ int s_ok;
+  const int nlist = R.ctl[1];
+  const size_t wsz = (size_t)P.cap;
+  Bins32 W{R.ws.q + (size_t)blockIdx.x * wsz, R.ws.uz + (size_t)blockIdx.x * wsz};
+  int64_t *wid = R.ws_id ? R.ws_id + (size_t)blockIdx.x * wsz : nullptr;
+  int *tb = R.ws_bs + (size_t)blockIdx.x * 2 * (P.nx + 1), *pre = tb + P.nx + 1;
+  for (int e = blockIdx.x; e < nlist; e += gridDim.x) {
+    const int seg = R.segs[e];
+    const int *cn = cnt + (size_t)seg * P.nx, *pn = pc + (size_t)seg * P.nx;
+    int *bs = bstart + (size_t)seg * (P.nx + 1);
+    if (threadIdx.x < 64) {
+      const bool ok = wm_bins::wave_layout([&](int i) { return max(cn[i], pn[i]); }, P.nx, P.cap, tb);
+      // count prefix for the flat copy, 64 bins at a time
+      int run = 0;
+      for (int i0 = 0; i0 < P.nx; i0 += 64) {
+        const int i = i0 + threadIdx.x;
+        const int n = i < P.nx ? cn[i] : 0;
+        int incl = n;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) S += __shfl_xor(S, o, 64);
-    if (S > hi - lo || __ballot(cnt > WM_BIN_MAX) != 0ULL) {
-      full = true;                                         // wave-uniform
-      break;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(incl, o, 64);
+          if ((int)threadIdx.x >= o) incl += v;
+        }
+        if (i < P.nx) pre[i + 1] = run + incl;
+        run += __shfl(incl, 63, 64);
+      }
+      if (threadIdx.x == 0) {
+        pre[0] = 0;
+        s_ok = ok;
+        R.segflag[seg] = 0;
+        if (!ok) atomicOr(derr, DERR_NP_OVER);
+      }
     }
-    const long long units = (hi - lo - S) / WM_BIN_ALIGN;
-    constexpr int top = WM_BIN_MAX / WM_BIN_ALIGN * WM_BIN_ALIGN;
-    const int room = in ? (int)min((long long)top, need + units * need / S * WM_BIN_ALIGN) : 0;
-    int incl = room;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int v = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += v;
+    __syncthreads();
+    if (s_ok) {
+      move_bins(
+          P, seg, P.nx, [&](int j) { return bs[j]; }, [&](int j) { return tb[j]; },
+          [&](int j) { return pre[j]; }, A, ids, W, wid);
+      for (int i = threadIdx.x; i <= P.nx; i += blockDim.x) bs[i] = tb[i];
     }
-    if (in) t[i] = lo + incl - room;
-    if (lane == 0) rb[blk] = 1;
-  }
-  if (!full) {
-    if (lane == 0) t[P.nx] = bs[P.nx];
-    return;
-  }
-  for (int blk = lane; blk < nblk; blk += SW) rb[blk] = 1;
-  if (!wm_bins::wave_layout([&](int i) { return c[i]; }, P.nx, P.cap, t)) {
-    for (int i = lane; i <= P.nx; i += SW) t[i] = 0;       // nothing is placed
-    if (lane == 0) atomicOr(derr, DERR_NP_OVER);
+    __syncthreads();
   }
 }
 
-// Step 2: every bin of the listed rows takes its pending movers, 8 lanes per
-// bin: a bin of a re-laid block is placed anew (its particles that stay, then
-// its arrivals) into the workspace D at its new place; any other bin migrates
-// in place.  The bin's count becomes its new count.
-__global__ void k_relayout_move(Mig G, SB P, const int *__restrict__ list,
-                                const int *__restrict__ tbs, const int *__restrict__ rblk,
-                                Movers M, Bins32 A, int64_t *__restrict__ ids, Bins32 D,
-                                int64_t *__restrict__ dids, const int *__restrict__ rowpend,
-                                int nb, int *derr) {
-  const size_t nbin = (size_t)nb * P.nsp * P.nx;
-  const int nblk = (P.nx + RB - 1) / RB;
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < nbin;
-       t += (size_t)gridDim.x * blockDim.x) {
-    const unsigned bl = (unsigned)(t / DL);
-    const int k = (int)(t % DL);
-    const int i = (int)(bl % (unsigned)P.nx);
-    const int bsg = (int)(bl / (unsigned)P.nx);         // batch segment b * nsp + s
-    const int s = bsg % P.nsp, jl = list[bsg / P.nsp];
-    const size_t seg = (size_t)s * P.ny + jl;
-    const size_t d = seg * P.nx + i;
-    const bool pend = rowpend[jl] != 0;
-    size_t base;
-    int capd;
-    bin_span(G, seg, i, base, capd);
-    if (rblk[(size_t)bsg * nblk + i / RB]) {
-      const int *tb = tbs + (size_t)bsg * (P.nx + 1);
-      const size_t dbase = (size_t)bsg * P.cap + tb[i];
-      const int cn = place_bin(G, M, A, ids, D, dids, d, i, jl, s, k, base, capd, dbase,
-                               tb[i + 1] - tb[i], pend, derr);
-      if (k == 0) G.cnt[d] = cn;
-    } else if (pend) {
-      (void)migrate_bin(G, M, A, ids, derr, d, i, jl, s, k, base, capd);
-    }
-  }
-}
-
-// Step 3: the re-laid bins from the workspace into their new places (a
-// wavefront per bin), the segments' new first slots; the rows hold their
-// movers now.
-__global__ __launch_bounds__(256) void k_relayout_copy(SB P, const int *__restrict__ list,
-                                                       const int *__restrict__ tbs,
-                                                       const int *__restrict__ rblk, Bins32 D,
-                                                       const int64_t *__restrict__ dids, Bins32 A,
-                                                       int64_t *__restrict__ ids,
-                                                       const int *__restrict__ cnt,
-                                                       int *__restrict__ bstart,
-                                                       int *__restrict__ rowpend) {
-  const int bsg = blockIdx.y;
-  const int s = bsg % P.nsp, jl = list[bsg / P.nsp];
-  const int seg = s * P.ny + jl;
-  const int nblk = (P.nx + RB - 1) / RB;
-  const int *tb = tbs + (size_t)bsg * (P.nx + 1);
-  const int lane = threadIdx.x & 63;
-  for (int i = blockIdx.x * 4 + threadIdx.x / 64; i < P.nx; i += gridDim.x * 4) {
-    if (!rblk[(size_t)bsg * nblk + i / RB]) continue;
-    const int n = cnt[(size_t)seg * P.nx + i];
-    const size_t src = (size_t)bsg * P.cap + tb[i], dst = seg_at(P, seg) + tb[i];
-    for (int k = lane; k < n; k += 64) {
-      A.q[dst + k] = D.q[src + k];
-      A.uz[dst + k] = D.uz[src + k];
-      if (ids) ids[dst + k] = dids[src + k];
-    }
-  }
-  if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i <= P.nx; i += blockDim.x) bstart[(size_t)seg * (P.nx + 1) + i] = tb[i];
-    if (threadIdx.x == 0 && s == 0) rowpend[jl] = 0;
+// Step 3: the lists' totals into the statistics, the lists emptied.
+__global__ void k_relayout_done(Relay R, int *__restrict__ stats) {
+  if (threadIdx.x == 0) {
+    stats[6] += R.ctl[0];
+    stats[7] += R.ctl[1];
+    R.ctl[0] = 0;
+    R.ctl[1] = 0;
   }
 }
 
@@ -560,49 +613,44 @@ int wm_settle_movers(wm_ctx *ctx) {
   return WM_OK;
 }
 
+namespace {
+Relay make_relay(const wm_ctx *ctx) {
+  Relay R;
+  R.ctl = ctx->relay_ctl;
+  R.blocks = ctx->relay_blocks;
+  R.segs = ctx->relay_segs;
+  R.blkflag = ctx->blkflag;
+  R.segflag = ctx->segflag;
+  R.ws = ctx->rws;
+  R.ws_id = ctx->rws_id;
+  R.ws_bs = ctx->rws_bs;
+  R.wsz = ctx->relay_wsz;
+  return R;
+}
+}  // namespace
+
 int wm_k_fill_check(wm_ctx *ctx) {
   const Mig G = wm_make_mig(ctx, ctx->mv_last);
   const size_t nthr = std::max(G.nbin, (size_t)ctx->g.ny);
   hipLaunchKernelGGL(k_fill_check, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, ctx->stream,
-                     G, ctx->pcnt, ctx->rowflag, ctx->mfill[ctx->mv_last], ctx->sfill, ctx->fstat,
-                     ctx->derr);
+                     G, ctx->pcnt, make_relay(ctx), ctx->mfill[ctx->mv_last], ctx->sfill,
+                     ctx->fstat, ctx->derr);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
-// The rows the fill check flagged, laid out anew in batches of relay_rows
-// rows through the workspace; their pending movers are placed on the way.
+// The blocks the fill check listed, then the segments they escalated, laid
+// out anew on the device (no host round trip: the grids loop over the lists,
+// whose lengths only the device knows).
 int wm_relayout(wm_ctx *ctx) {
   const SB P = make_sb(ctx);
-  std::vector<int> flag(P.ny), list;
-  WM_HIP(hipMemcpyAsync(flag.data(), ctx->rowflag, sizeof(int) * P.ny, hipMemcpyDeviceToHost,
-                        ctx->stream));
-  hipLaunchKernelGGL(k_fill_i32, dim3(wm_cdiv(P.ny, 256)), dim3(256), 0, ctx->stream, ctx->rowflag,
-                     (size_t)P.ny, 0);
-  WM_HIP(hipStreamSynchronize(ctx->stream));
-  for (int jl = 0; jl < P.ny; ++jl)
-    if (flag[jl]) list.push_back(jl);
-  const Mig G = wm_make_mig(ctx, ctx->mv_last);
-  for (size_t r0 = 0; r0 < list.size(); r0 += ctx->relay_rows) {
-    const int nb = (int)std::min<size_t>(ctx->relay_rows, list.size() - r0);
-    WM_HIP(hipMemcpyAsync(ctx->relay_list, list.data() + r0, sizeof(int) * nb,
-                          hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_relayout_plan, dim3(nb * P.nsp), dim3(SW), 0, ctx->stream, P,
-                       ctx->relay_list, ctx->pcnt, ctx->bstart, ctx->rtmp_bstart, ctx->rtmp_blk,
-                       ctx->derr);
-    const size_t nthr = (size_t)nb * P.nsp * P.nx * DL;
-    hipLaunchKernelGGL(k_relayout_move, dim3((unsigned)std::min<size_t>((nthr + 255) / 256, 65536)),
-                       dim3(256), 0, ctx->stream, G, P, ctx->relay_list, ctx->rtmp_bstart,
-                       ctx->rtmp_blk, ctx->mv[ctx->mv_last], ctx->bins, ctx->id_bins, ctx->rtmp,
-                       ctx->rtmp_id, ctx->rowpend, nb, ctx->derr);
-    hipLaunchKernelGGL(k_relayout_copy, dim3(std::max(1, std::min(256, wm_cdiv(P.nx, 4))), nb * P.nsp),
-                       dim3(256), 0, ctx->stream, P, ctx->relay_list, ctx->rtmp_bstart,
-                       ctx->rtmp_blk, ctx->rtmp, ctx->rtmp_id, ctx->bins, ctx->id_bins,
-                       ctx->bin_cnt, ctx->bstart, ctx->rowpend);
-    WM_HIP(hipGetLastError());
-    WM_HIP(hipStreamSynchronize(ctx->stream));       // the list is host memory of this scope
-  }
-  ctx->relayouts += (long long)list.size();
+  const Relay R = make_relay(ctx);
+  hipLaunchKernelGGL(k_relayout_blocks, dim3(ctx->relay_nwb), dim3(RW), 0, ctx->stream, P, R,
+                     ctx->bin_cnt, ctx->pcnt, ctx->bstart, ctx->bins, ctx->id_bins);
+  hipLaunchKernelGGL(k_relayout_segs, dim3(ctx->relay_nws), dim3(RW), 0, ctx->stream, P, R,
+                     ctx->bin_cnt, ctx->pcnt, ctx->bstart, ctx->bins, ctx->id_bins, ctx->derr);
+  hipLaunchKernelGGL(k_relayout_done, dim3(1), dim3(64), 0, ctx->stream, R, ctx->fstat);
+  WM_HIP(hipGetLastError());
   return WM_OK;
 }
 
