@@ -333,6 +333,17 @@ int wm_pc_num_params(void);
 int wm_pc_params(wm_ctx *ctx, float *theta, int n, int set);
 /* z = M(r) for three host images r[3][ny][nx] (x fastest). */
 int wm_pc_apply(wm_ctx *ctx, const double *r, double *z);
+/* z = M(r) like wm_pc_apply, for device images r, z [3][ny][nx] (x fastest)
+ * of any size with nx, ny positive multiples of 8 and nx * ny <= 262144,
+ * allocating nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  The activations live in a static device array of the
+ * library (about 27 MB, held on each GPU that calls this until the process
+ * ends), shared by every call on a GPU: calls that overlap on one GPU race.
+ * The context supplies the parameters (set or trained beforehand); the call
+ * returns when z is written.  Larger images return WM_ERR_ARG. */
+int wm_pc_apply_noalloc(wm_ctx *ctx, const double *r, double *z, int nx,
+                        int ny);
 /* Loss and dL/dtheta for host images r, e [3][ny][nx]. */
 int wm_pc_grad(wm_ctx *ctx, const double *r, const double *e, float *grad,
                double *loss);

@@ -69,7 +69,7 @@ EXPORTS = (
     "wm_state_download", "wm_state_init_weibel", "wm_step", "wm_energy", "wm_cg_iters",
     "wm_phase_times", "wm_particle_count", "wm_capacity_stats",
     "wm_set_solver", "wm_pc_collect", "wm_pc_pair_count", "wm_pc_init", "wm_pc_train",
-    "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_grad",
+    "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_apply_noalloc", "wm_pc_grad",
     "wm_solve_stats",
     "wm_mom_calc_init", "wm_mom_calc_accl", "wm_mom_calc_nvt", "wm_bcp_mom", "wm_moments",
     "wm_paraio_init", "wm_paraio_output", "wm_paraio_input", "wm_paraio_output_state",
@@ -156,6 +156,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_pc_num_params": [],
         "wm_pc_params": [_P, _P, _I, _I],
         "wm_pc_apply": [_P, _P, _P],
+        "wm_pc_apply_noalloc": [_P, _P, _P, _I, _I],
         "wm_pc_grad": [_P, _P, _P, _P, _P],
         "wm_solve_stats": [_P, _P, _P],
         "wm_mom_calc_init": [_P] + [_I] * 9 + [_D, _D, _D, _P, _P],
@@ -596,6 +597,13 @@ class Context:
         z = np.zeros_like(r)
         self._check(self._lib.wm_pc_apply(self._h, r.ctypes.data, z.ctypes.data))
         return z
+
+    def pc_apply_noalloc(self, r_dev: int, z_dev: int, nx: int, ny: int) -> None:
+        """z = M(r) on device images (addresses r_dev, z_dev of [3, ny, nx]
+        doubles), allocating nothing: the activations live in the library's
+        static scratch array (nx * ny <= 262144)."""
+        self._check(self._lib.wm_pc_apply_noalloc(self._h, ctypes.c_void_p(r_dev),
+                                                  ctypes.c_void_p(z_dev), int(nx), int(ny)))
 
     def pc_grad(self, r: np.ndarray, e: np.ndarray):
         """(loss, dL/dtheta) for images r, e [3, ny, nx]."""

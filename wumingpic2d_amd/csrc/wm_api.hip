@@ -1554,6 +1554,15 @@ int wm_pc_apply(wm_ctx *ctx, const double *r, double *z) {
   return wm_pc_apply_host(ctx, r, z);
 }
 
+int wm_pc_apply_noalloc(wm_ctx *ctx, const double *r, double *z, int nx, int ny) {
+  if (!ctx || !r || !z) return WM_ERR_ARG;
+  if (!ctx->pc) {                    // the parameters live in the context: nothing to apply
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: set or train the parameters first");
+    return WM_ERR_ARG;
+  }
+  return wm_pc_apply_noalloc_dev(ctx, r, z, nx, ny);
+}
+
 int wm_pc_grad(wm_ctx *ctx, const double *r, const double *e, float *grad, double *loss) {
   if (!ctx || !r || !e || !grad || !loss) return WM_ERR_ARG;
   WM_TRY(require_grid(ctx));

@@ -339,6 +339,11 @@ int wm_pc_init_params(wm_ctx *ctx, uint64_t seed);
 int wm_pc_set_params_dev(wm_ctx *ctx, const float *theta_host, int n);
 int wm_pc_get_params_dev(wm_ctx *ctx, float *theta_host, int n);
 int wm_pc_apply_host(wm_ctx *ctx, const double *r, double *z);
+// z = M(r) on device images of at most WM_PC_NOALLOC_CELLS pixels, activations
+// in a static device array (no allocation)
+#define WM_PC_NOALLOC_CELLS (512 * 512)
+int wm_pc_apply_noalloc_dev(wm_ctx *ctx, const double *r, double *z, int nx, int ny);
+size_t wm_pc_noalloc_bytes();
 int wm_pc_grad_host(wm_ctx *ctx, const double *r, const double *e, float *grad, double *loss);
 int wm_pc_collect_cfg(wm_ctx *ctx, int nslot, int per_solve);
 bool wm_pc_collecting(const wm_ctx *ctx);

@@ -1087,6 +1087,95 @@ int wm_pc_apply_host(wm_ctx *ctx, const double *r, double *z) {
   return WM_OK;
 }
 
+// ---------------------------------------------- no-allocation application
+// z = M(r) for callers that cannot allocate (code under graph capture, tight
+// serving loops): the activations and the residual scales live in this file's
+// device array `scratch`, sized for images of at most WM_PC_NOALLOC_CELLS
+// pixels.  Each GPU holds its own copy of the array, at its own address,
+// until the process ends; every call on a GPU shares it, so calls that
+// overlap on one GPU race.  Nothing in it needs to start at zero: every
+// kernel of the pass writes all of what the next one reads.
+namespace {
+constexpr size_t NOALLOC_CELLS = WM_PC_NOALLOC_CELLS;
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+// bytes of one activation at level l (pixels (ny >> l) (nx >> l) <= cells >> 2l)
+constexpr size_t act_bytes(int l, int ch) {
+  return align256((size_t)wm_pc::B * (NOALLOC_CELLS >> (2 * l)) * ch * sizeof(bf16));
+}
+constexpr size_t SC_BYTES = align256(6 * sizeof(double));
+constexpr size_t SCRATCH_BYTES = SC_BYTES + act_bytes(1, C0) + act_bytes(1, C1) + act_bytes(1, C0) +
+                                 act_bytes(2, C1) + act_bytes(2, C2) + act_bytes(2, C1) +
+                                 act_bytes(3, C2) + act_bytes(3, C2);
+}  // namespace
+
+// In host code `scratch` names a host-side stand-in: kernels get its device
+// address from hipGetSymbolAddress (scratch_of), never the name itself.
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEV = 64;
+void *scratch_ptr[MAX_DEV];            // the array's device address, per GPU (set up once)
+
+int scratch_of(wm_ctx *ctx, unsigned char **out) {
+  if (ctx->device < 0 || ctx->device >= MAX_DEV) {
+    wm_set_error(ctx, WM_ERR_NODEV, "pc_apply_noalloc: device %d beyond %d", ctx->device, MAX_DEV);
+    return WM_ERR_NODEV;
+  }
+  if (!scratch_ptr[ctx->device]) {
+    int cur = 0;
+    WM_HIP(hipGetDevice(&cur));
+    WM_HIP(hipSetDevice(ctx->device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    WM_HIP(hipSetDevice(cur));
+    WM_HIP(e);
+    scratch_ptr[ctx->device] = p;
+  }
+  *out = (unsigned char *)scratch_ptr[ctx->device];
+  return WM_OK;
+}
+}  // namespace
+
+size_t wm_pc_noalloc_bytes() { return SCRATCH_BYTES; }
+
+int wm_pc_apply_noalloc_dev(wm_ctx *ctx, const double *r, double *z, int nx, int ny) {
+  wm_pc *pc = ctx->pc;
+  if (!pc || !pc->trained) {
+    wm_set_error(ctx, WM_ERR_ARG, "learned preconditioner: set or train the parameters first");
+    return WM_ERR_ARG;
+  }
+  if (nx <= 0 || ny <= 0 || nx % 8 || ny % 8 || (size_t)nx * ny > NOALLOC_CELLS) {
+    wm_set_error(ctx, WM_ERR_ARG,
+                 "pc_apply_noalloc: nx=%d, ny=%d must be positive multiples of 8 with nx*ny <= %zu "
+                 "(its scratch array's size)", nx, ny, NOALLOC_CELLS);
+    return WM_ERR_ARG;
+  }
+  unsigned char *b = nullptr;
+  WM_TRY(scratch_of(ctx, &b));
+  double *sc = (double *)b;
+  b += SC_BYTES;
+  Acts X{};
+  auto take = [&](int l, int ch) {
+    bf16 *p = (bf16 *)b;
+    b += act_bytes(l, ch);
+    return p;
+  };
+  X.d1 = take(1, C0);
+  X.a1 = take(1, C1);
+  X.b1 = take(1, C0);
+  X.d2 = take(2, C1);
+  X.a2 = take(2, C2);
+  X.b2 = take(2, C1);
+  X.d3 = take(3, C2);
+  X.a3 = take(3, C2);
+  const size_t n = (size_t)nx * ny;
+  hipLaunchKernelGGL(k_img_scale, dim3(3), dim3(256), 0, ctx->stream, r, n, sc);
+  WM_HIP(hipGetLastError());
+  WM_TRY(forward_on(ctx, pc, X, ny, nx, r, n, sc, z, n, nullptr));
+  WM_HIP(hipStreamSynchronize(ctx->stream));
+  return WM_OK;
+}
+
 // Loss and gradient for host pair images (r, e)[3][ny][nx].
 int wm_pc_grad_host(wm_ctx *ctx, const double *r, const double *e, float *grad, double *loss) {
   wm_pc *pc = ctx->pc;
