@@ -112,7 +112,7 @@ int wm_bcp_init(wm_ctx *ctx, int ndim, int np, int nsp, int nxgs, int nxge,
                 int nygs, int nyge, int nys, int nye, int nup, int ndown,
                 int mnpi, int mnpr, int ncomw, int nerr, const int *nstat,
                 int nstat_len, double delx, double delt, double c);
-/* boundary_shock__init (proj/shock/boundary_shock.f90:26-58) and
+/* boundary_shock__init (proj/shock/boundary_shock.f90:27-59) and
  * boundary_reconnection__init (proj/reconnection/boundary_reconnection.f90:
  * 26-58): the same arguments; they select the wall family, whose host-array
  * calls take an active x range nxs..nxe inside nxgs..nxge.  A context takes
@@ -132,7 +132,9 @@ int wm_particle_solv(wm_ctx *ctx, double *gp, const double *up,
                      const double *uf, const int *cumcnt, int nxs, int nxe);
 /* field__fdtd_i(uf,up,gp,cumcnt,nxs,nxe,bc_dfield,bc_curre,bc_phi)
  * (field.f90:66-186).  The three boundary callbacks of the reference are
- * selected by bc_family and run on the device. */
+ * selected by bc_family (WM_BC_*; negative: the family whose __init was
+ * called) and run on the device.  The wall families solve with the
+ * reference's plain CG. */
 int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up,
                     const double *gp, const int *cumcnt, int nxs, int nxe,
                     int bc_family);
@@ -158,7 +160,7 @@ int wm_bcp_phi(wm_ctx *ctx, double *phi, int nxs, int nxe, int nys, int nye,
 /* ---- the shock family (proj/shock/boundary_shock.f90) ----
  * particle_x (:62-100, the reference's signature is (up,np2,nxs,nxe)):
  * mirrors about (nxs+1) delx and (nxe-1) delx with the momentum reversed.
- * injection (:255-298): the left mirror, and beyond xend = nxe delx +
+ * injection (:255-297): the left mirror, and beyond xend = nxe delx +
  * u0/sqrt(1+u0^2/c^2) delt x mirrors about xend and u_x about u0.
  * particle_y (:103-252): the periodic one.  dfield (:300-407), curre
  * (:410-548), phi (:551-625): y periodic; a conducting wall at nxs (Bx, Ey,

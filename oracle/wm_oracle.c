@@ -172,7 +172,7 @@ int or_particle_solv(const or_params *p, double *gp, const double *up,
         const double txxx = fac1 * fac1;         /* :91 */
         const double fac2 = q * delt / r;        /* :92 */
         const int i0 = cumcnt[icc(p, i, j, isp)], i1 = cumcnt[icc(p, i + 1, j, isp)];
-        if (i0 < 0 || i1 > p->np || i1 < i0) { bad = 1; continue; }
+        if (i0 < 0 || i1 > p->np) { bad = 1; continue; }   /* i1 < i0: an empty range */
         for (int ii = i0 + 1; ii <= i1; ++ii) {
           const double *u = up + iup(p, 1, ii, j, isp);
           double *g = gp + iup(p, 1, ii, j, isp);
@@ -256,7 +256,7 @@ static int ele_cur_rows(const or_params *p, double *uj, const double *up,
       for (int isp = 1; isp <= p->nsp; ++isp) {
         const double q = p->q[isp - 1];
         const int i0 = cumcnt[icc(p, i, j, isp)], i1 = cumcnt[icc(p, i + 1, j, isp)];
-        if (i0 < 0 || i1 > p->np || i1 < i0) { bad = 1; continue; }
+        if (i0 < 0 || i1 > p->np) { bad = 1; continue; }   /* i1 < i0: an empty range */
         for (int ii = i0 + 1; ii <= i1; ++ii) {
           const double *u = up + iup(p, 1, ii, j, isp);
           const double *g = gp + iup(p, 1, ii, j, isp);
@@ -1003,7 +1003,7 @@ int or_mom_calc_accl(const or_params *p, double *gp, const double *up,
         const double txxx = fac1 * fac1;         /* :91 */
         const double fac2 = q * delt / r;        /* :92 */
         const int i0 = cumcnt[icc(p, i, j, isp)], i1 = cumcnt[icc(p, i + 1, j, isp)];
-        if (i0 < 0 || i1 > p->np || i1 < i0) { bad = 1; continue; }
+        if (i0 < 0 || i1 > p->np) { bad = 1; continue; }   /* i1 < i0: an empty range */
         for (int ii = i0 + 1; ii <= i1; ++ii) {
           const double *u = up + iup(p, 1, ii, j, isp);
           double *g = gp + iup(p, 1, ii, j, isp);

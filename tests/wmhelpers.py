@@ -78,7 +78,7 @@ def gauss_residual(p, uf, up, np2):
     return div - 4 * np.pi * rho_tsc(p, up, np2)
 
 
-def wall_state(bc, nx, ny, ppc, seed, vth=0.1, extra=0, u0=-0.2):
+def wall_state(bc, nx, ny, ppc, seed, vth=0.1, extra=0, u0=-0.2, np_cap=None):
     """A state for a wall family (oracle.BC_SHOCK / BC_RECONNECTION): the
     Weibel pairs, on a grid `extra` columns wider than they fill for the
     shock (its box nxs..nxe grows into the rest, proj/shock/app.f90:613-623;
@@ -87,7 +87,8 @@ def wall_state(bc, nx, ny, ppc, seed, vth=0.1, extra=0, u0=-0.2):
     its deposit does not index).  p carries bc, nxs, nxe and u0."""
     from oracle import BC_SHOCK, Oracle
 
-    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth)
+    kw = {"np_cap": np_cap} if np_cap else {}
+    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=vth, **kw)
     p = dict(p, bc=bc, nxs=p["nxgs"], nxe=p["nxge"], u0=u0 if bc == BC_SHOCK else 0.0)
     if bc != BC_SHOCK:
         return p, up, np2, cc, uf

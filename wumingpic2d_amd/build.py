@@ -55,7 +55,8 @@ FORTRAN = os.path.join(HERE, "fortran")
 FMOD = os.path.join(FORTRAN, "mod")
 FLIB = os.path.join(HERE, "libwmpic_f.so")
 FSRC = ["apps/mpi.f90", "wmpic_capi.f90", "wm_mpi_set.f90", "wm_particle.f90", "wm_field.f90",
-        "wm_sort.f90", "wm_boundary_periodic.f90", "wm_mom_calc.f90",
+        "wm_sort.f90", "wm_boundary_periodic.f90", "wm_boundary_shock.f90",
+        "wm_boundary_reconnection.f90", "wm_mom_calc.f90",
         # what the reference's apps use besides the hot path (single process)
         "apps/json_module.f90", "apps/mpiio.f90", "apps/jsonio.f90", "apps/wm_paraio.f90",
         "apps/wuming_utils.f90", "apps/wuming2d.f90"]

@@ -387,11 +387,22 @@ int upload_cumcnt(wm_ctx *ctx, const int *cumcnt_in) {
   const size_t n = (size_t)nseg_of(ctx) * (ctx->g.nx + 1);
   const std::vector<int> act = active_cumcnt(ctx, cumcnt_in);
   const int *cumcnt = act.data();
-  // cumcnt must be a prefix table inside the row capacity (sort.f90:64-69)
+  // cumcnt gives cell i the particles cumcnt(i)+1 .. cumcnt(i+1)
+  // (particle.f90:94, field.f90:218): inside the row capacity, and the
+  // non-empty ranges in order without overlap.  A range may be empty with
+  // cumcnt(i+1) < cumcnt(i): the shock's inject() appends particles to cell
+  // nxe-1 by raising cumcnt(nxe) alone (proj/shock/app.f90:832-835).
+  const int lim = std::min(ctx->g.np, ctx->cap);
   for (int s = 0; s < nseg_of(ctx); ++s) {
     const int *c = cumcnt + (size_t)s * (ctx->g.nx + 1);
-    bool ok = c[0] >= 0 && c[ctx->g.nx] <= std::min(ctx->g.np, ctx->cap);
-    for (int i = 0; ok && i < ctx->g.nx; ++i) ok = c[i + 1] >= c[i];
+    bool ok = true;
+    int end = 0;
+    for (int i = 0; ok && i <= ctx->g.nx; ++i) ok = c[i] >= 0 && c[i] <= lim;
+    for (int i = 0; ok && i < ctx->g.nx; ++i)
+      if (c[i + 1] > c[i]) {
+        ok = c[i] >= end;
+        end = c[i + 1];
+      }
     if (!ok) {
       wm_set_error(ctx, WM_ERR_RANGE, "cumcnt of row segment %d is not a prefix table within capacity", s);
       return WM_ERR_RANGE;
@@ -403,12 +414,17 @@ int upload_cumcnt(wm_ctx *ctx, const int *cumcnt_in) {
   return WM_OK;
 }
 
+// The particles a cumcnt table hands out: the largest entry of the active
+// range (the end of its last non-empty cell range).
 std::vector<int> counts_from_cumcnt(const wm_ctx *ctx, const int *cumcnt) {
   const int nseg = nseg_of(ctx);
-  const int last = ctx->bcf == WM_BC_PERIODIC || ctx->xe == 0 ? ctx->g.nx
-                                                               : ctx->xe + 1 - ctx->g.nxgs;
+  const bool all = ctx->bcf == WM_BC_PERIODIC || ctx->xe == 0;
+  const int a = all ? 0 : ctx->xs - ctx->g.nxgs, b = all ? ctx->g.nx : ctx->xe + 1 - ctx->g.nxgs;
   std::vector<int> c(nseg);
-  for (int s = 0; s < nseg; ++s) c[s] = cumcnt[(size_t)s * (ctx->g.nx + 1) + last];
+  for (int s = 0; s < nseg; ++s) {
+    const int *r = cumcnt + (size_t)s * (ctx->g.nx + 1);
+    c[s] = *std::max_element(r + a, r + b + 1);
+  }
   return c;
 }
 
@@ -586,6 +602,7 @@ int wm_alloc_device(wm_ctx *ctx) {
     for (int b = 0; b < 2; ++b) {
       WM_TRY(dmalloc(ctx, &ctx->bin_sum[b], nbin));
       WM_TRY(dmalloc(ctx, &ctx->mv[b].rec, (size_t)g.ny * ctx->mrow));
+      WM_TRY(dmalloc(ctx, &ctx->mv[b].perm, (size_t)g.ny * ctx->mrow));
       WM_TRY(dmalloc(ctx, &ctx->mfill[b], g.ny));
       WM_HIP(hipMemsetAsync(ctx->mfill[b], 0, sizeof(int) * g.ny, ctx->stream));
     }
@@ -764,7 +781,7 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->tmp, ctx->gkl, ctx->cg_r, ctx->cg_p[0], ctx->cg_p[1], ctx->cg_s, ctx->red,
                   ctx->red_cnt, ctx->dep_part, ctx->diag, ctx->derr, ctx->stage,
                   ctx->cnt_scratch, ctx->fscratch, ctx->id_bins, ctx->bin_cnt, ctx->bin_sum[0],
-                  ctx->bin_sum[1], ctx->mv[0].rec, ctx->mv[1].rec, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
+                  ctx->bin_sum[1], ctx->mv[0].rec, ctx->mv[1].rec, ctx->mv[0].perm, ctx->mv[1].perm, ctx->cg_z, ctx->cg_q, ctx->cg_x0,
                   ctx->pc_sc, ctx->fcg_s, ctx->mom, ctx->mom_part, ctx->bins.q, ctx->bins.uz,
                   ctx->bstart, ctx->pcnt, ctx->mfill[0], ctx->mfill[1], ctx->rowpend,
                   ctx->fstat, ctx->relay_ctl, ctx->relay_blocks, ctx->blkflag, ctx->relay_segs,
@@ -944,7 +961,8 @@ int wm_field_fdtd_i(wm_ctx *ctx, double *uf, const double *up, const double *gp,
   WM_TRY(require(ctx, ctx->finit, "field__init"));
   static const char *const init_of[3] = {"boundary_periodic__init", "boundary_shock__init",
                                          "boundary_reconnection__init"};
-  if (bc_family < WM_BC_PERIODIC || bc_family > WM_BC_RECONNECTION) {
+  if (bc_family < 0) bc_family = ctx->binit ? ctx->bcf : WM_BC_PERIODIC;   // the initialised one
+  if (bc_family > WM_BC_RECONNECTION) {
     wm_set_error(ctx, WM_ERR_ARG, "boundary family %d is not available", bc_family);
     return WM_ERR_ARG;
   }

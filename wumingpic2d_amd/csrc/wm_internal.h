@@ -100,6 +100,10 @@ struct __align__(16) MoverRec {   // 32 B: two 16-byte halves
 };
 struct Movers {
   MoverRec *rec = nullptr;
+  // per bin with more than WM_SUM_MV movers, beside its records: the
+  // records' indices (within the bin's) in direction order, so the migration
+  // reads the movers of one direction without scanning the bin's list
+  int *perm = nullptr;
 };
 // Per bin after the push: what the migration needs of the movers that left
 // it, in one 32-byte load.

@@ -96,39 +96,12 @@ WM_MIG_FN void store_move(const Move &v, const Bins32 &A, int64_t *ids, size_t b
   if (ids) ids[base + v.dst] = v.id;
 }
 
-// Direction index of entry j of a source's mover list.
-WM_MIG_FN int list_dir(const Movers &M, uint4 fl, size_t rs, int j) {
-  return j < WM_SUM_MV ? sum_mv(fl.x, fl.y, fl.z, fl.w, j) >> WM_SLOT_BITS
-                       : M.rec[rs + j].sd >> 24;
-}
-// Arrivals from one direction: next arrival index (a), how many are still to
-// come (left), next position in the source's list (j), its length (ms) and
-// first record (rs).
-struct ArrState {
-  int a, left, j, ms;
-  size_t rs;
-};
+// One arrival: the record and the slot it takes.
 struct Arrival {
   MoverRec rec;
   int slot;
   bool ok;
 };
-// Finds the next arrival of direction k in the source's list and issues its
-// record's load.
-WM_MIG_FN void plan_arrival(Arrival &w, ArrState &as, int k, const Movers &M, uint4 fl, uint4 ol,
-                            size_t r0, int m, int n) {
-  w.ok = false;
-  w.slot = 0;
-  while (as.left > 0 && as.j < as.ms && list_dir(M, fl, as.rs, as.j) != k) ++as.j;
-  if (as.left > 0 && as.j < as.ms) {
-    w.rec = M.rec[as.rs + as.j];
-    w.slot = as.a < m ? hole_slot(M, r0, ol, as.a) : n + (as.a - m);
-    w.ok = true;
-    --as.left;
-    ++as.a;
-    ++as.j;
-  }
-}
 WM_MIG_FN void put_arrival(const Arrival &w, int capd, const Bins32 &A, int64_t *ids, int *derr,
                            size_t base) {
   if (!w.ok) return;
@@ -154,17 +127,19 @@ WM_MIG_FN size_t source_bin(const Mig &G, int i, int jl, int sp, int k, int &si,
 // The migration of bin d (cell i, row jl, species sp; first slot base,
 // capd slots), by a group of 8 consecutive lanes (lane k of the group
 // passes k) that alone writes d: no lane reads what another group writes.
-// Lane k takes the arrivals from the source bin in direction index k, in the
-// order the source listed them; directions follow one another, so arrival
-// index = the earlier directions' count (a prefix over the group's lanes) +
-// the rank within direction k.  d held n particles and lost m (its movers'
-// slots, ascending, are its holes): the a-th arrival takes the a-th hole
-// while there are holes, then slot n + (a - m).  With fewer arrivals than
-// holes, lane 0 fills the holes left below the new count, in order, with the
-// particles above it.  The summaries of d and of the source (two 32-byte
-// loads, issued together) give the counts and the first WM_SUM_MV movers'
-// slots and directions; only a bin with more movers reads the rest from its
-// records.
+// Lane k looks up the arrivals from the source bin in direction index k:
+// their count (the source's summary) and where they are (a source with at
+// most WM_SUM_MV movers lists them in its summary; one with more lists its
+// records of one direction together through its perm index, in list order).  Arrival index = the earlier directions' count (a prefix over the
+// group's lanes) + the rank within direction k, and the group's lanes take
+// arrival indices k, k + 8, ..., whatever their direction.  d held n
+// particles and lost m (its movers' slots, ascending, are its holes): the
+// a-th arrival takes the a-th hole while there are holes, then slot
+// n + (a - m).  With fewer arrivals than holes, lane 0 fills the holes left
+// below the new count, in order, with the particles above it.  The
+// summaries of d and of the source (32 bytes each) give the counts, the
+// record offsets and the first WM_SUM_MV holes; only a bin with more movers
+// reads the rest of its holes from its records.
 constexpr int DL = 8;
 // Returns d's new count to lane 0 of the group (-1 to the others).
 WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids, int *derr,
@@ -177,7 +152,8 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   // all loads in flight together (the compiler would otherwise issue some
   // only where they are first used, a round trip later)
   __asm__ volatile("" : "+v"(ol.x), "+v"(ol.y), "+v"(ol.z), "+v"(ol.w), "+v"(fh.x), "+v"(fh.y),
-                   "+v"(fh.z), "+v"(fh.w), "+v"(nd));
+                   "+v"(fh.z), "+v"(fh.w), "+v"(fl.x), "+v"(fl.y), "+v"(fl.z), "+v"(fl.w),
+                   "+v"(nd));
   const int n = min(nd, capd);
   const int c = sum_cnt(fh.x, fh.y, k);
   const int m = (int)oh.w;
@@ -194,8 +170,8 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   // new count that are not holes fill the holes left below it, in order.
   // Its sources lie at or above the new count and the arrivals' holes below
   // it, so the first four moves load together with the arrivals' records
-  // (the rest follow one by one).  Each batched move and arrival is a named
-  // variable, not an array element: an indexed array would go to memory.
+  // (the rest follow one by one).  Each batched move is a named variable,
+  // not an array element: an indexed array would go to memory.
   const bool compact = k == 0 && total < m;
   MoveState st{total, total, 0, nnew};
   if (compact) {
@@ -207,23 +183,59 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   plan_move(v1, st, compact, M, r0, ol, m, n, base, A, ids);
   plan_move(v2, st, compact, M, r0, ol, m, n, base, A, ids);
   plan_move(v3, st, compact, M, r0, ol, m, n, base, A, ids);
-  // Arrivals from direction k, in the source's list order: the first two
-  // records load together, the rest one by one.
-  ArrState as{inc - c, c, 0, c > 0 ? (int)fh.w : 0, c > 0 ? bin_records(G, sjl, fh.z) : 0};
-  Arrival w0, w1;
-  plan_arrival(w0, as, k, M, fl, ol, r0, m, n);
-  plan_arrival(w1, as, k, M, fl, ol, r0, m, n);
-  put_arrival(w0, capd, A, ids, derr, base);
-  put_arrival(w1, capd, A, ids, derr, base);
-  for (; as.left > 0 && as.j < as.ms; ++as.j) {
-    if (list_dir(M, fl, as.rs, as.j) != k) continue;
+  // direction k's records of the source: perm[fs + pre + r], r < c, index
+  // records fs + perm[..]; pre = the source's movers of directions before k
+  int pre = 0;
+#pragma unroll
+  for (int q = 0; q < DL - 1; ++q) pre += q < k ? sum_cnt(fh.x, fh.y, q) : 0;
+  const size_t fs = c > 0 ? bin_records(G, sjl, fh.z) : 0;
+  const unsigned fs_lo = (unsigned)fs, fs_hi = (unsigned)(fs >> 32);
+  // arrival a: its direction kk (the group's inclusive prefixes ascend),
+  // rank a - (the prefix before kk), and lane kk's source.  The shuffles run
+  // in every lane of the group (a lane past the last arrival reads and
+  // drops).
+  auto arrival = [&](int a, Arrival &w) {
+    w.ok = a < total;
+    int kk = 0, before = 0;
+#pragma unroll
+    for (int q = 0; q < DL; ++q) {
+      const int iq = __shfl(inc, q, DL);
+      if (iq <= a) {
+        kk = q + 1;
+        before = iq;
+      }
+    }
+    kk = min(kk, DL - 1);
+    const size_t f = (size_t)(unsigned)__shfl((int)fs_lo, kk, DL) |
+                     ((size_t)(unsigned)__shfl((int)fs_hi, kk, DL) << 32);
+    const int pk = __shfl(pre, kk, DL);
+    const int ms = __shfl((int)fh.w, kk, DL);
+    const uint32_t l0 = __shfl(fl.x, kk, DL), l1 = __shfl(fl.y, kk, DL);
+    const uint32_t l2 = __shfl(fl.z, kk, DL), l3 = __shfl(fl.w, kk, DL);
+    if (!w.ok) return;
+    const int r = a - before;
+    int q;
+    if (ms <= WM_SUM_MV) {
+      // a source with few movers lists them all in its summary (direction
+      // index in the top bits): the r-th of direction kk, without a load
+      q = 0;
+      for (int j = 0, seen = 0; j < ms; ++j)
+        if (sum_mv(l0, l1, l2, l3, j) >> WM_SLOT_BITS == kk) {
+          if (seen == r) q = j;
+          ++seen;
+        }
+    } else {
+      q = M.perm[f + pk + r];
+    }
+    w.rec = M.rec[f + q];
+    w.slot = a < m ? hole_slot(M, r0, ol, a) : n + (a - m);
+  };
+  // every lane of the group runs the same number of rounds (the shuffles
+  // need the whole group)
+  for (int a = k; a - k < total; a += DL) {
     Arrival w;
-    w.rec = M.rec[as.rs + as.j];
-    w.slot = as.a < m ? hole_slot(M, r0, ol, as.a) : n + (as.a - m);
-    w.ok = true;
+    arrival(a, w);
     put_arrival(w, capd, A, ids, derr, base);
-    --as.left;
-    ++as.a;
   }
   store_move(v0, A, ids, base);
   store_move(v1, A, ids, base);

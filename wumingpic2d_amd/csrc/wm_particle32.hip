@@ -283,14 +283,6 @@ __device__ inline float xql_at(int e, const Spill &X) {
   return e < NPOOL ? xql[e] : X.q[X.base + (e - NPOOL)];
 }
 
-// Shape weight at index a (0..4) of the 5-point stencil for offset o whose
-// 3-point support starts at index 1 + inc (inc = cell change, -1..1; the old
-// position's support starts at index 1).
-__device__ inline float shape5(float sm, float s0, float sp, int inc, int a) {
-  const int r = a - 1 - inc;
-  return r == 0 ? sm : (r == 1 ? s0 : (r == 2 ? sp : 0.0f));
-}
-
 // The deposit of one cell's movers j = p, p + MQ, ... (part p of MQ) into
 // the 65 stencil sums of the cell (Jx a < 4 at a*5 + b, Jy b < 4 at
 // 20 + a*4 + b, Jz at 40 + a*5 + b): field.f90:224-268's Esirkepov weights
@@ -298,7 +290,7 @@ __device__ inline float shape5(float sm, float s0, float sp, int inc, int a) {
 // summed over the part's movers in FP32.  Jz uses
 // s0x s0y + ds_x s0y / 2 + s0x ds_y / 2 + ds_x ds_y / 3 = X Y + ds_x ds_y / 12
 // with X = s0x + ds_x / 2, Y = s0y + ds_y / 2.
-constexpr int MQ = 4;
+constexpr int MQ = 16;
 // The 5-point old shape (support at 1..3) and new shape (support at
 // 1 + inc .. 3 + inc): ds = new - old and X = old + ds / 2.  Selects on
 // three flags rather than on inc's value, which the compiler would turn into
@@ -352,14 +344,6 @@ __device__ inline void mover_part(float *acc, int p, int nm, int mc, const Spill
       for (int b = 0; b < 5; ++b) acc[40 + a * 5 + b] += za * Y[b] + zb * dY[b];
     }
   }
-}
-
-// Sum over the four lanes of a DPP quad; lane 0 of the quad holds
-// (v0 + v1) + (v2 + v3).
-__device__ inline float quad_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
-  return v;
 }
 
 constexpr int PD = 5;           // windows of a bin held in registers (80 particles)
@@ -639,24 +623,12 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   if (sub == 0 && g < CHUNK) gcnt[g] = g < ncell ? L.xcnt : 0;
   __syncthreads();                                   // the lists, and the movers' new state in
                                                      // their slots
-  // the movers' deposit, on the first wave alone: lane t takes the cell's
-  // movers j = t % MQ, t % MQ + MQ, ... of cell t / MQ; meanwhile the second
-  // wave takes the workgroup's records in the row's pool
-  static_assert(CHUNK * MQ == 64, "the movers' deposit runs on one wave");
-  if (threadIdx.x < CHUNK * MQ) {
-    const int mc = threadIdx.x / MQ;
-    float acc[NACC];
-#pragma unroll
-    for (int e = 0; e < NACC; ++e) acc[e] = 0.0f;
-    mover_part(acc, threadIdx.x % MQ, mc < ncell ? gcnt[mc] : 0, mc, X);
-    double *mcb = cellbuf + mc * NACC;
-    const bool lead = threadIdx.x % MQ == 0 && mc < ncell;
-#pragma unroll
-    for (int e = 0; e < NACC; ++e) {
-      const float v = quad_sum(acc[e]);
-      if (lead) mcb[e] += (double)v;
-    }
-  } else if (threadIdx.x == 64) {
+  // the workgroup's records in the row's pool (one atomic), while every
+  // cell's group deposits the cell's movers: lane sub takes movers j = sub,
+  // sub + MQ, ... (a dense cell of the nonlinear phase loses a hundred
+  // particles a step), and the group sums its lanes in a fixed order
+  static_assert(CHUNK * MQ == NT && MQ == GROUP, "the movers' deposit runs on every cell group");
+  if (threadIdx.x == 0) {
     int tot = 0;
     for (int c = 0; c < CHUNK; ++c) tot += gcnt[c];
     int f = tot > 0 ? atomicAdd(mfill + jl, tot) : 0;
@@ -666,6 +638,19 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     }
     rfirst = f;
     atomicMax(stats + 2, tot);
+  }
+  {
+    float acc[NACC];
+#pragma unroll
+    for (int e = 0; e < NACC; ++e) acc[e] = 0.0f;
+    mover_part(acc, sub, g < ncell ? gcnt[g] : 0, g, X);
+    double *mcb = cellbuf + g * NACC;
+    const bool lead = sub == 0 && g < ncell;
+#pragma unroll
+    for (int e = 0; e < NACC; ++e) {
+      const float v = row_sum16(acc[e]);
+      if (lead) mcb[e] += (double)v;
+    }
   }
   __syncthreads();
   const int rf = rfirst;
@@ -697,12 +682,29 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
       int m = 0;
       while (j0 + m < gm && (xl_at(xent(g, j0 + m), X) >> 28) == s) ++m;
       BinSum *bs = msum + bin_of(s, jl, il, P.ny, P.nx);
-      if (sub < 8) {
-        const int d = sub < 4 ? sub : sub + 1;       // directions 0..8 without 4
-        int c = 0;
+      // lane sub < 8 of the cell's group: direction index sub (directions
+      // 0..8 without 4); its count, and (a bin with more movers than its
+      // summary lists) the bin's records of that direction in list order at
+      // perm[first + (counts of the directions before) + rank]
+      const int d = sub < 4 ? sub : sub + 1;
+      int c = 0;
+      if (sub < 8)
         for (int q = 0; q < m; ++q) c += ((xl_at(xent(g, j0 + q), X) >> 20) & 0xF) == d;
+      int pre = c;
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) {
+        const int v = __shfl_up(pre, o, 8);
+        if (sub >= o) pre += v;
+      }
+      pre -= c;
+      if (sub < 8) {
         if (c > 255) atomicOr(derr, DERR_MOVER_OVER);
         bs->cnt[sub] = (uint8_t)min(c, 255);
+        if (rf >= 0 && c > 0 && m > WM_SUM_MV) {       // else the summary lists them
+          int *pp = M.perm + (size_t)jl * mrow + rf + before + j0 + pre;
+          for (int q = 0, r = 0; r < c; ++q)
+            if (((xl_at(xent(g, j0 + q), X) >> 20) & 0xF) == d) pp[r++] = q;
+        }
       }
       if (sub < min(m, WM_SUM_MV)) {
         const int xv = xl_at(xent(g, j0 + sub), X);

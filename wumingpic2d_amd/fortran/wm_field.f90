@@ -55,7 +55,10 @@ contains
     real(8), intent(in)    :: up(ndim,np,nys:nye,nsp)
     real(8), intent(inout) :: uf(6,nxgs-2:nxge+2,nys-2:nye+2)
     call wmpic_context()
-    call wmpic_check(wm_field_fdtd_i(wm_ctx, uf, up, gp, cumcnt, nxs, nxe, WM_BC_PERIODIC))
+    ! the set_boundary_* arguments are the initialised boundary module's
+    ! (boundary_periodic, boundary_shock or boundary_reconnection), which
+    ! the device applies itself
+    call wmpic_check(wm_field_fdtd_i(wm_ctx, uf, up, gp, cumcnt, nxs, nxe, WM_BC_INITIALISED))
   end subroutine field__fdtd_i
 
 end module field

@@ -14,12 +14,17 @@ module wmpic_capi
   public :: wm_particle_init, wm_particle_solv, wm_field_init, wm_field_fdtd_i
   public :: wm_sort_init, wm_sort_bucket, wm_bcp_init, wm_bcp_particle_x
   public :: wm_bcp_particle_y, wm_bcp_dfield, wm_bcp_curre, wm_bcp_phi
-  public :: wm_mpi_set_init, wm_set_capacity, WM_BC_PERIODIC
+  public :: wm_mpi_set_init, wm_set_capacity, WM_BC_PERIODIC, WM_BC_INITIALISED
+  public :: wm_bcs_init, wm_bcs_particle_x, wm_bcs_injection, wm_bcs_particle_y, wm_bcs_dfield
+  public :: wm_bcs_curre, wm_bcs_phi, wm_bcs_mom, wm_bcr_init, wm_bcr_particle_x
+  public :: wm_bcr_particle_y, wm_bcr_dfield, wm_bcr_curre, wm_bcr_phi, wm_bcr_mom
   public :: wm_mom_calc_init, wm_mom_calc_accl, wm_mom_calc_nvt, wm_bcp_mom
   public :: wm_paraio_init, wm_paraio_output, wm_paraio_input, wm_paraio_param, wm_paraio_mom
   public :: wm_paraio_ptcl, wm_paraio_orb
 
   integer(c_int), parameter :: WM_BC_PERIODIC = 0
+  ! field__fdtd_i: the family whose boundary module was initialised
+  integer(c_int), parameter :: WM_BC_INITIALISED = -1
   type(c_ptr), save :: wm_ctx = c_null_ptr
 
   interface
@@ -136,6 +141,113 @@ module wmpic_capi
        real(c_double) :: phi(*)
        integer(c_int), value :: nxs, nxe, nys, nye, l
      end function wm_bcp_phi
+     integer(c_int) function wm_bcs_init(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, &
+          nup, ndown, mnpi, mnpr, ncomw, nerr, nstat, nstat_len, delx, delt, c) &
+          bind(C, name="wm_bcs_init")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye
+       integer(c_int), value :: nup, ndown, mnpi, mnpr, ncomw, nerr, nstat_len
+       integer(c_int), intent(in) :: nstat(*)
+       real(c_double), value :: delx, delt, c
+     end function wm_bcs_init
+     integer(c_int) function wm_bcs_particle_x(ctx, up, np2, nxs, nxe) &
+          bind(C, name="wm_bcs_particle_x")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: up(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: nxs, nxe
+     end function wm_bcs_particle_x
+     integer(c_int) function wm_bcs_particle_y(ctx, up, np2) bind(C, name="wm_bcs_particle_y")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: up(*)
+       integer(c_int) :: np2(*)
+     end function wm_bcs_particle_y
+     integer(c_int) function wm_bcs_dfield(ctx, df, nxs, nxe, nys, nye, nxgs, nxge) &
+          bind(C, name="wm_bcs_dfield")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: df(*)
+       integer(c_int), value :: nxs, nxe, nys, nye, nxgs, nxge
+     end function wm_bcs_dfield
+     integer(c_int) function wm_bcs_curre(ctx, uj, nxs, nxe, nys, nye, nxgs, nxge) &
+          bind(C, name="wm_bcs_curre")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: uj(*)
+       integer(c_int), value :: nxs, nxe, nys, nye, nxgs, nxge
+     end function wm_bcs_curre
+     integer(c_int) function wm_bcs_phi(ctx, phi, nxs, nxe, nys, nye, l) bind(C, name="wm_bcs_phi")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: phi(*)
+       integer(c_int), value :: nxs, nxe, nys, nye, l
+     end function wm_bcs_phi
+     integer(c_int) function wm_bcs_mom(ctx, mom) bind(C, name="wm_bcs_mom")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: mom(*)
+     end function wm_bcs_mom
+     integer(c_int) function wm_bcr_init(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye, &
+          nup, ndown, mnpi, mnpr, ncomw, nerr, nstat, nstat_len, delx, delt, c) &
+          bind(C, name="wm_bcr_init")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, nye
+       integer(c_int), value :: nup, ndown, mnpi, mnpr, ncomw, nerr, nstat_len
+       integer(c_int), intent(in) :: nstat(*)
+       real(c_double), value :: delx, delt, c
+     end function wm_bcr_init
+     integer(c_int) function wm_bcr_particle_x(ctx, up, np2, nxs, nxe) &
+          bind(C, name="wm_bcr_particle_x")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: up(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: nxs, nxe
+     end function wm_bcr_particle_x
+     integer(c_int) function wm_bcr_particle_y(ctx, up, np2) bind(C, name="wm_bcr_particle_y")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: up(*)
+       integer(c_int) :: np2(*)
+     end function wm_bcr_particle_y
+     integer(c_int) function wm_bcr_dfield(ctx, df, nxs, nxe, nys, nye, nxgs, nxge) &
+          bind(C, name="wm_bcr_dfield")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: df(*)
+       integer(c_int), value :: nxs, nxe, nys, nye, nxgs, nxge
+     end function wm_bcr_dfield
+     integer(c_int) function wm_bcr_curre(ctx, uj, nxs, nxe, nys, nye, nxgs, nxge) &
+          bind(C, name="wm_bcr_curre")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: uj(*)
+       integer(c_int), value :: nxs, nxe, nys, nye, nxgs, nxge
+     end function wm_bcr_curre
+     integer(c_int) function wm_bcr_phi(ctx, phi, nxs, nxe, nys, nye, l) bind(C, name="wm_bcr_phi")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: phi(*)
+       integer(c_int), value :: nxs, nxe, nys, nye, l
+     end function wm_bcr_phi
+     integer(c_int) function wm_bcr_mom(ctx, mom) bind(C, name="wm_bcr_mom")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: mom(*)
+     end function wm_bcr_mom
+     integer(c_int) function wm_bcs_injection(ctx, up, np2, nxs, nxe, u0) &
+          bind(C, name="wm_bcs_injection")
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: up(*)
+       integer(c_int), intent(in) :: np2(*)
+       integer(c_int), value :: nxs, nxe
+       real(c_double), value :: u0
+     end function wm_bcs_injection
      integer(c_int) function wm_mom_calc_init(ctx, ndim, np, nsp, nxgs, nxge, nygs, nyge, nys, &
           nye, delx, delt, c, q, r) bind(C, name="wm_mom_calc_init")
        import :: c_ptr, c_int, c_double
