@@ -230,6 +230,11 @@ int wm_cg_iters(wm_ctx *ctx, int *iters3);
 int wm_phase_times(wm_ctx *ctx, double *ms, int n);
 /* Total active particles (sum of np2). */
 int wm_particle_count(wm_ctx *ctx, long long *count);
+/* The active particles' IDs (up(6,...) as 64-bit integers) summed and
+ * xor-ed with 64-bit wrap, computed on the device: sum_xor[0], sum_xor[1].
+ * A property of the particle set that no step may change, for states too
+ * large to download.  No reference counterpart. */
+int wm_id_checksum(wm_ctx *ctx, unsigned long long *sum_xor);
 /* FP32 form: the particle store's capacities and their use since the state
  * was made (no reference counterpart; the reference stops when a row
  * overflows np, boundary_periodic.f90:228-236).  out[0] most particles in

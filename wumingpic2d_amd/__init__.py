@@ -67,7 +67,7 @@ EXPORTS = (
     "wm_particle_solv", "wm_field_fdtd_i", "wm_bcp_particle_x", "wm_bcp_particle_y",
     "wm_sort_bucket", "wm_bcp_dfield", "wm_bcp_curre", "wm_bcp_phi", "wm_state_upload",
     "wm_state_download", "wm_state_init_weibel", "wm_step", "wm_energy", "wm_cg_iters",
-    "wm_phase_times", "wm_particle_count", "wm_capacity_stats",
+    "wm_phase_times", "wm_particle_count", "wm_capacity_stats", "wm_id_checksum",
     "wm_set_solver", "wm_pc_collect", "wm_pc_pair_count", "wm_pc_init", "wm_pc_train",
     "wm_pc_num_params", "wm_pc_params", "wm_pc_apply", "wm_pc_apply_noalloc", "wm_pc_grad",
     "wm_solve_stats",
@@ -147,6 +147,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "wm_cg_iters": [_P, _P],
         "wm_phase_times": [_P, _P, _I],
         "wm_particle_count": [_P, ctypes.POINTER(ctypes.c_longlong)],
+        "wm_id_checksum": [_P, _P],
         "wm_capacity_stats": [_P, _P, _I],
         "wm_set_solver": [_P, _I],
         "wm_pc_collect": [_P, _I, _I],
@@ -545,6 +546,12 @@ class Context:
         c = ctypes.c_longlong()
         self._check(self._lib.wm_particle_count(self._h, ctypes.byref(c)))
         return c.value
+
+    def id_checksum(self) -> tuple:
+        """(sum, xor) of the active particles' 64-bit IDs, on the device."""
+        out = np.zeros(2, dtype=np.uint64)
+        self._check(self._lib.wm_id_checksum(self._h, out.ctypes.data))
+        return int(out[0]), int(out[1])
 
     def capacity_stats(self) -> dict:
         """FP32 form: the particle store's capacities and their use since the

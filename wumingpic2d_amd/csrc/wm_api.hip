@@ -1503,6 +1503,24 @@ int wm_particle_count(wm_ctx *ctx, long long *count) {
   return WM_OK;
 }
 
+int wm_id_checksum(wm_ctx *ctx, unsigned long long *sum_xor) {
+  if (!ctx || !sum_xor) return WM_ERR_ARG;
+  if (!ctx->alloc_done) return WM_ERR_NOT_INIT;
+  if (ctx->form == WM_FORM_FP32) WM_TRY(wm_settle_movers(ctx));
+  unsigned long long *d = nullptr;
+  WM_HIP(hipMalloc((void **)&d, 2 * sizeof(unsigned long long)));
+  int st = WM_OK;
+  if (hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), ctx->stream) != hipSuccess)
+    st = WM_ERR_HIP;
+  if (st == WM_OK) st = wm_k_id_checksum(ctx, d);
+  if (st == WM_OK && (hipMemcpyAsync(sum_xor, d, 2 * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                      hipStreamSynchronize(ctx->stream) != hipSuccess))
+    st = WM_ERR_HIP;
+  (void)hipFree(d);
+  return st;
+}
+
 int wm_capacity_stats(wm_ctx *ctx, long long *out, int n) {
   if (!ctx || !out) return WM_ERR_ARG;
   if (!ctx->alloc_done || ctx->form != WM_FORM_FP32) {

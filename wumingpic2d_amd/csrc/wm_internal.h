@@ -319,6 +319,7 @@ int wm_field_out(wm_ctx *ctx, double *uf);
 int wm_field_in(wm_ctx *ctx, const double *uf);
 int wm_settle_movers(wm_ctx *ctx);   // the last push's movers into their bins, if pending
 int wm_k_fill_check(wm_ctx *ctx);    // projected bin counts after a push; lists blocks to re-lay
+int wm_k_id_checksum(wm_ctx *ctx, unsigned long long *out_dev);   // [sum, xor] of the IDs
 int wm_relayout(wm_ctx *ctx);        // listed blocks (segments) laid out anew, on the device
 // FP32 form (wm_particle32.hip)
 int wm_k_cell_fields32(wm_ctx *ctx);

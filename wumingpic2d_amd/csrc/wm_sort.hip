@@ -534,6 +534,44 @@ __global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids,
   }
 }
 
+// The IDs of the active particles of one segment per workgroup, summed and
+// xor-ed (64-bit integer wrap: the order does not matter): bins of the FP32
+// form (bs, cnt) or rows of the FP64 form (np2).
+__global__ __launch_bounds__(256) void k_id_checksum(const int64_t *__restrict__ ids, int nx,
+                                                     int cap, const int *__restrict__ bstart,
+                                                     const int *__restrict__ cnt,
+                                                     const int *__restrict__ np2,
+                                                     unsigned long long *out) {
+  const int seg = blockIdx.x;
+  unsigned long long sum = 0, x = 0;
+  const int64_t *row = ids + (size_t)seg * cap;
+  if (bstart) {
+    const int *bs = bstart + (size_t)seg * (nx + 1);
+    const int *c = cnt + (size_t)seg * nx;
+    for (int i = threadIdx.x >> 6; i < nx; i += blockDim.x >> 6)
+      for (int k = threadIdx.x & 63; k < c[i]; k += 64) {
+        const unsigned long long v = (unsigned long long)row[bs[i] + k];
+        sum += v;
+        x ^= v;
+      }
+  } else {
+    for (int k = threadIdx.x; k < np2[seg]; k += blockDim.x) {
+      const unsigned long long v = (unsigned long long)row[k];
+      sum += v;
+      x ^= v;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += (unsigned long long)__shfl_xor((long long)sum, o, 64);
+    x ^= (unsigned long long)__shfl_xor((long long)x, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(out, sum);
+    atomicXor(out + 1, x);
+  }
+}
+
 __global__ void k_fill_i32(int *a, size_t n, int v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = v;
@@ -650,6 +688,20 @@ int wm_relayout(wm_ctx *ctx) {
   hipLaunchKernelGGL(k_relayout_segs, dim3(ctx->relay_nws), dim3(RW), 0, ctx->stream, P, R,
                      ctx->bin_cnt, ctx->pcnt, ctx->bstart, ctx->bins, ctx->id_bins, ctx->derr);
   hipLaunchKernelGGL(k_relayout_done, dim3(1), dim3(64), 0, ctx->stream, R, ctx->fstat);
+  WM_HIP(hipGetLastError());
+  return WM_OK;
+}
+
+int wm_k_id_checksum(wm_ctx *ctx, unsigned long long *out_dev) {
+  const bool f32 = ctx->form == WM_FORM_FP32;
+  const int64_t *ids = f32 ? ctx->id_bins : ctx->id[ctx->idcur];
+  if (!ids) {
+    wm_set_error(ctx, WM_ERR_ARG, "ID checksum: the particles carry no IDs (ndim = 5)");
+    return WM_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_id_checksum, dim3(ctx->g.nsp * ctx->g.ny), dim3(256), 0, ctx->stream, ids,
+                     ctx->g.nx, ctx->cap, f32 ? ctx->bstart : nullptr,
+                     f32 ? ctx->bin_cnt : nullptr, ctx->np2, out_dev);
   WM_HIP(hipGetLastError());
   return WM_OK;
 }
