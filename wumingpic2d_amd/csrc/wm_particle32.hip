@@ -192,30 +192,46 @@ __device__ inline void core32(float *core, float ox0, float oy0, float ox1, floa
   }
 }
 
-// Slot of core value c in a cell's 65 stencil sums (Jx 0..19 as a*5 + b with
-// a < 4, Jy 20..39 as 20 + a*4 + b with b < 4, Jz 40..64 as 40 + a*5 + b); the
-// core sits at 5-point indices 1..3.
-__device__ inline int core_slot(int c) {
-  if (c < 6) return (c / 3 + 1) * 5 + (c % 3 + 1);
-  if (c < 12) return 20 + ((c - 6) / 2 + 1) * 4 + ((c - 6) % 2 + 1);
-  return 40 + ((c - 12) / 3 + 1) * 5 + ((c - 12) % 3 + 1);
+// A cell's stencil sums at tile offset (a, b) of component comp: the
+// movers' sums, 65 per cell (Jx 0..19 as (a-1)*5 + b with a >= 1, Jy 20..39
+// as 20 + a*4 + (b-1) with b >= 1, Jz 40..64 as 40 + a*5 + b), plus the
+// stayers' 21 core sums where the core has the offset (Jx a = 2..3, Jy
+// b = 2..3, offsets 1..3 otherwise; the layout of core32).  Both are FP32
+// sums of the group's lanes, added here in FP64.
+constexpr int CCS = 24;               // floats per cell of the core sums (21, padded)
+constexpr int CMS = 68;               // floats per cell of the movers' sums (65, padded)
+__shared__ float ccore[CHUNK * CCS];
+__shared__ float cmov[CHUNK * CMS];
+__device__ inline int core_of(int comp, int a, int b) {
+  if (comp == 0) return (a >= 2 && a <= 3 && b >= 1 && b <= 3) ? (a - 2) * 3 + (b - 1) : -1;
+  if (comp == 1) return (a >= 1 && a <= 3 && b >= 2 && b <= 3) ? 6 + (a - 1) * 2 + (b - 2) : -1;
+  return (a >= 1 && a <= 3 && b >= 1 && b <= 3) ? 12 + (a - 1) * 3 + (b - 1) : -1;
 }
-
-__device__ inline double acc_at(const double *cb, int comp, int a, int b) {
+__device__ inline double acc_at(int cell, int comp, int a, int b) {
   const int idx = comp == 0 ? (a - 1) * 5 + b : (comp == 1 ? 20 + a * 4 + (b - 1) : 40 + a * 5 + b);
   const bool none = (comp == 0 && a == 0) || (comp == 1 && b == 0);
-  const double v = cb[none ? 0 : idx];
-  return none ? 0.0 : v;
+  const double mv = (double)cmov[cell * CMS + (none ? 0 : idx)];
+  const int c = core_of(comp, a, b);
+  const double cv = (double)ccore[cell * CCS + (c < 0 ? 0 : c)];
+  return none ? 0.0 : (c < 0 ? mv : cv + mv);
 }
 
 // Sum over the 16 lanes of one DPP row (a group): quad butterflies, then
 // rotations by 4 and 8 within the row.  Every lane ends with the total; the
-// callers use lane 0's, whose association order is fixed.
+// callers use lane 0's, whose association order is fixed.  The callers run
+// it with every lane of the row active, so each DPP source lane is valid;
+// written as update_dpp with bound_ctrl (zero for an invalid source), each
+// step compiles to a single v_add_f32_dpp.
+template <int CTRL>
+__device__ inline float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
 __device__ inline float row_sum16(float v) {
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x124, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xF, 0xF, false));
+  v = dpp_add<0xB1>(v);         // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);         // quad_perm [2,3,0,1]
+  v = dpp_add<0x124>(v);        // row_ror 4
+  v = dpp_add<0x128>(v);        // row_ror 8
+  __asm__ volatile("" : "+v"(v));   // the last add stays fused, not sunk into the caller's branch
   return v;
 }
 
@@ -228,28 +244,26 @@ __device__ inline size_t bin_of(int s, int jl, int i, int ny, int nx) {
 }
 
 // The movers of one push workgroup are listed, per cell in (species, slot)
-// order, for their deposit, their bins' summaries and their records.  The
-// list entries come from a pool in LDS: a cell's list is a chain of blocks
-// of MB entries that its group takes from the pool as it needs them, so a
-// dense cell can list more movers than an even share.  Past the LDS pool the
-// chain continues in blocks of the row's spill pool in device memory (block
-// NBLK + b is entry block b of the row's spill).  Entry j of cell c's list is
-// xent(c, j).  Which block a cell gets varies from run to run; what a list
-// holds, and so every result, does not.
-constexpr int MB = 16;                // entries per pool block
-constexpr int NBLK = 60;              // blocks in the LDS pool
-constexpr int NPOOL = MB * NBLK;      // movers of one workgroup listed in LDS
-constexpr int MAXB = 64;              // blocks of one cell's list (1024 movers)
+// order, for their deposit, their bins' summaries and their records.  A
+// cell's first CAPG entries sit in its own part of an LDS pool; a dense cell
+// continues its list in blocks of MB entries of the row's spill pool in
+// device memory, which its group takes as it reaches them (entry index
+// NPOOL + b * MB + r is entry r of the row's spill block b).  Entry j of cell
+// c's list is xent(c, j).  Which spill block a cell gets varies from run to
+// run; what a list holds, and so every result, does not.
+constexpr int MB = 16;                // entries per spill block
+constexpr int CAPG = 60;              // entries of one cell's list in LDS
+constexpr int NPOOL = CHUNK * CAPG;   // movers of one workgroup listed in LDS
+constexpr int MAXB = 64;              // spill blocks of one cell's list
 
 // The push kernel's LDS (file scope, so device functions address it directly).
 __shared__ float T[6 * 3 * TW];                   // cell-centred fields of the chunk
 __shared__ float4 CF4[CHUNK * CFS / 4];           // per-cell gather coefficients
-__shared__ double cellbuf[CHUNK * NACC];          // per-cell stencil sums
 __shared__ int xl[NPOOL];                         // movers: species << 28 | dir << 20 | slot
 __shared__ float4 xpl[NPOOL];                     // movers' offsets before and after
 __shared__ float xql[NPOOL];                      // movers' charge times v_z
-__shared__ uint16_t xblk[CHUNK * MAXB];           // block of each block of a cell's list
-__shared__ int xused;                             // LDS pool blocks handed out
+__shared__ uint16_t xblk[CHUNK * MAXB];           // spill block of each block of a cell's
+                                                  // list past CAPG
 __shared__ int gcnt[CHUNK];                       // movers listed per cell
 __shared__ int scnt[WM_MAXSP * CHUNK];            // particles per (species, cell)
 __shared__ int sbin[WM_MAXSP * CHUNK];            // first slot of each (species, cell) bin
@@ -271,7 +285,8 @@ struct Spill {
 };
 
 __device__ inline int xent(int c, int j) {
-  return (int)xblk[c * MAXB + (j >> 4)] * MB + (j & (MB - 1));
+  return j < CAPG ? c * CAPG + j
+                  : NPOOL + (int)xblk[c * MAXB + ((j - CAPG) >> 4)] * MB + ((j - CAPG) & (MB - 1));
 }
 __device__ inline int xl_at(int e, const Spill &X) {
   return e < NPOOL ? xl[e] : X.l[X.base + (e - NPOOL)];
@@ -431,32 +446,32 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
   if (on && !cross) {
     core32(core, ox0, oy0, ox1, oy1, S.qc, qv);
   }
-  // movers in (species, slot) order; the group's lane 0 takes the pool blocks
-  // that this window's movers start (at most two)
+  // movers in (species, slot) order: the window's movers are entries
+  // pos0 .. pos0 + nmv - 1 of the cell's list
   const unsigned gm = (unsigned)(__ballot(cross) >> (GROUP * (lane_g() & 3))) & 0xFFFFu;
   const int nmv = __popc(gm);
   if (nmv != 0 && !L.lost) {                       // uniform over the group
     const int pos0 = L.xcnt;
-    const int b0 = pos0 / MB, b1 = (pos0 + nmv - 1) / MB;
-    const bool fresh = pos0 % MB == 0;
-    const int nnew = (b1 - b0) + (fresh ? 1 : 0);
-    int first = 0;
-    if (lane_sub() == 0 && nnew > 0) {
-      first = atomicAdd(&xused, nnew);
-      if (first + nnew > NBLK) {                   // LDS pool full: the row's spill pool
+    const int j = pos0 + __popc(gm & ((1u << lane_sub()) - 1u));
+    bool room = true;
+    if (pos0 + nmv > CAPG) {
+      // the list runs past the cell's LDS entries: the group's lane 0 takes
+      // the spill blocks that this window's movers start
+      const int taken = pos0 > CAPG ? (pos0 - CAPG + MB - 1) / MB : 0;
+      const int need = (pos0 + nmv - CAPG + MB - 1) / MB;
+      const int nnew = need - taken;
+      int first = 0;
+      if (lane_sub() == 0 && nnew > 0) {
         const int f = atomicAdd(X.fill, nnew);
-        first = f + nnew <= X.nblk ? NBLK + f : -1;
+        first = f + nnew <= X.nblk ? f : -1;
       }
+      first = __shfl(first, (int)(threadIdx.x & 63) & ~(GROUP - 1), 64);
+      room = first >= 0 && need <= MAXB;
+      if (room && lane_sub() < nnew) xblk[lane_g() * MAXB + taken + lane_sub()] = (uint16_t)(first + lane_sub());
+      __builtin_amdgcn_wave_barrier();
     }
-    first = __shfl(first, (int)(threadIdx.x & 63) & ~(GROUP - 1), 64);
-    const bool room = first >= 0 && b1 < MAXB;
-    if (lane_sub() == 0 && room) {
-      if (fresh) xblk[lane_g() * MAXB + b0] = (uint16_t)first;
-      if (b1 > b0) xblk[lane_g() * MAXB + b1] = (uint16_t)(first + (fresh ? 1 : 0));
-    }
-    __builtin_amdgcn_wave_barrier();
     if (cross && room) {
-      const int e = xent(lane_g(), pos0 + __popc(gm & ((1u << lane_sub()) - 1u)));
+      const int e = xent(lane_g(), j);
       const int dir = (dy + 1) * 3 + (dx + 1);
       const int v = (S.s << 28) | (dir << 20) | k;
       const float4 pp = make_float4(ox0, oy0, ox1, oy1);
@@ -564,12 +579,10 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
       sbin[s * CHUNK + c] = b0;
     }
   }
-  if (threadIdx.x == 0) xused = 0;
   if (threadIdx.x < WM_MAXSP) sqc[threadIdx.x] = P.q[threadIdx.x] * P.qc_scale;
 #pragma unroll
   for (int q = 0; q < NTV; ++q)
     if (threadIdx.x + q * NT < 6 * 3 * TW) T[threadIdx.x + q * NT] = tv[q];
-  for (int e = threadIdx.x; e < CHUNK * NACC; e += NT) cellbuf[e] = 0.0;
   __syncthreads();
   const int g = threadIdx.x / GROUP;
   const int sub = threadIdx.x % GROUP;
@@ -589,7 +602,6 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     cf[12 * k + 9] = cf[12 * k + 10] = cf[12 * k + 11] = 0.0f;
   }
   __syncthreads();
-  double *cb = cellbuf + g * NACC;
   Lane L;
   L.xcnt = 0;
   L.ok = true;
@@ -615,7 +627,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
 #pragma unroll
     for (int c = 0; c < NCORE; ++c) {
       const float v = row_sum16(core[c]);
-      if (sub == 0) cb[core_slot(c)] += (double)v;
+      if (sub == 0) ccore[g * CCS + c] = v;
     }
   }
   if (!L.ok) atomicOr(derr, DERR_JUMP);
@@ -644,12 +656,11 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
 #pragma unroll
     for (int e = 0; e < NACC; ++e) acc[e] = 0.0f;
     mover_part(acc, sub, g < ncell ? gcnt[g] : 0, g, X);
-    double *mcb = cellbuf + g * NACC;
     const bool lead = sub == 0 && g < ncell;
 #pragma unroll
     for (int e = 0; e < NACC; ++e) {
       const float v = row_sum16(acc[e]);
-      if (lead) mcb[e] += (double)v;
+      if (lead) cmov[g * CMS + e] = v;
     }
   }
   __syncthreads();
@@ -725,7 +736,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     const int b = e / (3 * DW);
     double sum = 0.0;
     const int glo = max(0, t - 4), ghi = min(ncell - 1, t);
-    for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(cellbuf + gg * NACC, comp, t - gg, b);
+    for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(gg, comp, t - gg, b);
     out[e] = sum;
   }
 }
