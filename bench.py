@@ -480,10 +480,17 @@ def main() -> int:
     }
     if head["plain"] is not None:
         pl = head["plain"]
+        run_len = WORKLOADS[args.workload][4]
+        train_ms = head["pc_training"]["seconds"] * 1e3 / run_len
+        eff = hl["ms_per_step"] + train_ms
         out["solve_fp32_same_state"] = {
             "solve_ms_fcg": hl["solve_ms_per_step"], "solve_ms_plain": round(float(pl["phase"][3]), 4),
             "iters_fcg": hl["fcg_iters_per_step"], "iters_plain": pl["iters"],
             "ms_per_step_fcg": round(hl["ms_per_step"], 3), "ms_per_step_plain": round(pl["ms_per_step"], 3),
+            # the preconditioner's training spread over the workload's whole run
+            "training_ms_per_step": round(train_ms, 4), "run_steps": run_len,
+            "effective_ms_per_step_fcg": round(eff, 3),
+            "learned_pc_pays": bool(eff < pl["ms_per_step"]),
             "what": f"{args.steps} steps of each, one after the other on the same FP32 state; "
                     "the second stream holds only the bins' fill check"}
     ref = None

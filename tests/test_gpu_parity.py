@@ -22,7 +22,9 @@ def _close(a, b, rtol=RTOL, scale=None):
     return np.abs(a - b).max() <= rtol * s
 
 
-@pytest.mark.parametrize("nx,ny,ppc,seed", [(32, 16, 8, 1), (64, 8, 20, 2), (16, 3, 5, 3)])
+# the last two have many scatter rounds per row (8 tiles of 64 particles each)
+@pytest.mark.parametrize("nx,ny,ppc,seed", [(32, 16, 8, 1), (64, 8, 20, 2), (16, 3, 5, 3),
+                                            (256, 4, 40, 12), (1024, 2, 70, 13)])
 def test_sort_bucket_bitexact(nx, ny, ppc, seed):
     p, up, np2, cc, uf = state(nx, ny, ppc, seed)
     rng = np.random.default_rng(seed)
@@ -82,9 +84,11 @@ def test_field_fdtd_i(nx, ny, ppc, seed):
         assert _close(uf2, uf1)
 
 
-def test_particle_x_y_exact():
-    p, up, np2, cc, uf = state(32, 12, 10, 8, vth=0.3)
-    random_fields(uf, 8, amp=0.3)
+# the second has rows of more than 65536 particles: two rounds of pass 1
+@pytest.mark.parametrize("nx,ny,ppc,seed", [(32, 12, 10, 8), (1024, 4, 72, 11)])
+def test_particle_x_y_exact(nx, ny, ppc, seed):
+    p, up, np2, cc, uf = state(nx, ny, ppc, seed, vth=0.3)
+    random_fields(uf, seed, amp=0.3)
     orc = Oracle(p)
     gp = np.zeros_like(up)
     orc.particle__solv(gp, up, uf, cc)     # moves particles across cells/rows

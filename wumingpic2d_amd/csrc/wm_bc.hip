@@ -6,10 +6,13 @@
 // particle_x and the injection edge are elementwise.  particle_y keeps the reference's single-rank
 // result exactly, including the order particles end up in inside each row,
 // with two kernels over (species, row) segments:
-//  pass 1 (one workgroup per segment, in index order): leavers get their y
+//  pass 1 (one workgroup of YNW wavefronts per segment): leavers get their y
 //   wrapped in round-down arithmetic and are compacted, order preserving,
 //   into per-row "up" and "down" mover lists; their slots become the row's
-//   ascending hole list (the reference's flag(:,j), :143-169);
+//   ascending hole list (the reference's flag(:,j), :143-169).  The segment
+//   is taken YST tiles of 64 particles at a time: the wavefronts classify
+//   the tiles (ballot masks in LDS), the workgroup scans the tiles' mover
+//   counts, and each tile's movers then go to their places in index order;
 //  pass 2 (one workgroup per segment): the arrival list of row j is the
 //   concatenation the self-SENDRECVs produce (:174-189) -- for rows other
 //   than nys, up-movers of row j-1 then down-movers of row j+1; for row nys,
@@ -132,8 +135,12 @@ __global__ void k_particle_x(BP P, const int *__restrict__ np2, double *__restri
   if (bad) atomicOr(derr, DERR_RANGE);
 }
 
-constexpr int YW = 64;     // one wavefront per segment
-constexpr int YT = 4;      // tiles unrolled per walk step
+constexpr int YW = 64;     // wavefront
+constexpr int YNW = 8;     // wavefronts per segment (pass 1)
+constexpr int YTH = YNW * YW;
+constexpr int YST = 1024;  // tiles of 64 particles per pass-1 round
+constexpr int YT = 4;      // tiles in flight per wavefront
+constexpr int Y2T = 256;   // pass-2 workgroup
 
 __device__ inline int lanes_below(unsigned long long m) {
   const int lane = threadIdx.x & 63;
@@ -172,58 +179,122 @@ __device__ inline Rec64::Val arriving(const Rec64 &R, size_t p, int j, int d, co
   return v;
 }
 
-// pass 1: one wavefront per segment walks it in index order.
+// Wavefront-inclusive prefix sum.
+__device__ inline int wave_incl(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// pass 1: the tiles of a round are classified by all wavefronts (tile t by
+// wavefront t % YNW), scanned, and their movers placed in index order.
 template <class R>
-__global__ __launch_bounds__(YW) void k_particle_y1(BP P, int xmode,
-                                                    const int *__restrict__ np2, R B,
-                                                    const int64_t *__restrict__ id,
-                                                    int *__restrict__ hole,
-                                                    int *__restrict__ nhole,
-                                                    int *__restrict__ nup,
-                                                    int *__restrict__ ndn, R mu,
-                                                    int64_t *mid_u, R md, int64_t *mid_d,
-                                                    int *derr) {
+__global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
+                                                     const int *__restrict__ np2, R B,
+                                                     const int64_t *__restrict__ id,
+                                                     int *__restrict__ hole,
+                                                     int *__restrict__ nhole,
+                                                     int *__restrict__ nup,
+                                                     int *__restrict__ ndn, R mu,
+                                                     int64_t *mid_u, R md, int64_t *mid_d,
+                                                     int *derr) {
+  __shared__ unsigned long long smu[YST], smd[YST];   // per tile: lanes moving up / down
+  __shared__ int scu[YST], scd[YST];                   // movers up / down before the tile
+  __shared__ int wtu[YNW], wtd[YNW];
   const int seg = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int jl = seg % P.ny;
   const int j = P.nys + jl;
   const int n = np2[seg];
   const size_t base = (size_t)seg * P.cap;
   const size_t mb = (size_t)seg * P.mcap;
-  int cu = 0, cd = 0;
+  int cu = 0, cd = 0;                                  // movers of earlier rounds
   int bad = 0;
-  for (int k0 = 0; k0 < n; k0 += YW * YT) {
-    int dir[YT];
+  for (int k0 = 0; k0 < n; k0 += YST * YW) {
+    const int ntile = min(YST, (n - k0 + YW - 1) / YW);
+    // classify (the periodic x wrap or the walls' mirrors are applied here)
+    for (int t0 = w; t0 < ntile; t0 += YNW * YT) {
+      int dir[YT];
 #pragma unroll
-    for (int t = 0; t < YT; ++t) {
-      const int k = k0 + t * YW + lane;
-      dir[t] = k < n ? row_change(B, base + k, j, xmode, P, bad) : 0;
+      for (int b = 0; b < YT; ++b) {
+        const int k = k0 + (t0 + b * YNW) * YW + lane;
+        dir[b] = t0 + b * YNW < ntile && k < n ? row_change(B, base + k, j, xmode, P, bad) : 0;
+      }
+#pragma unroll
+      for (int b = 0; b < YT; ++b) {
+        const int t = t0 + b * YNW;
+        if (dir[b] == 2) bad |= DERR_JUMP;
+        const unsigned long long u = __ballot(dir[b] == 1), d = __ballot(dir[b] == -1);
+        if (t < ntile && lane == 0) {
+          smu[t] = u;
+          smd[t] = d;
+        }
+      }
     }
+    __syncthreads();
+    // exclusive prefix of the tiles' counts: thread th owns tiles 2 th, 2 th + 1
+    int a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    const int t2 = 2 * threadIdx.x;
+    if (t2 < ntile) {
+      a0 = __popcll(smu[t2]);
+      b0 = __popcll(smd[t2]);
+    }
+    if (t2 + 1 < ntile) {
+      a1 = __popcll(smu[t2 + 1]);
+      b1 = __popcll(smd[t2 + 1]);
+    }
+    const int iu = wave_incl(a0 + a1), id_ = wave_incl(b0 + b1);
+    if (lane == YW - 1) {
+      wtu[w] = iu;
+      wtd[w] = id_;
+    }
+    __syncthreads();
+    int pu = 0, pd = 0, tu = 0, td = 0;
 #pragma unroll
-    for (int t = 0; t < YT; ++t) {
-      const int k = k0 + t * YW + lane;
-      if (dir[t] == 2) bad |= DERR_JUMP;
-      const bool up = dir[t] == 1, dn = dir[t] == -1;
-      const unsigned long long mu_ = __ballot(up), md_ = __ballot(dn);
+    for (int v = 0; v < YNW; ++v) {
+      pu += v < w ? wtu[v] : 0;
+      pd += v < w ? wtd[v] : 0;
+      tu += wtu[v];
+      td += wtd[v];
+    }
+    if (t2 < YST) {
+      const int eu = cu + pu + iu - (a0 + a1), ed = cd + pd + id_ - (b0 + b1);
+      scu[t2] = eu;
+      scd[t2] = ed;
+      scu[t2 + 1] = eu + a0;
+      scd[t2 + 1] = ed + b0;
+    }
+    __syncthreads();
+    // place each tile's movers
+    for (int t = w; t < ntile; t += YNW) {
+      const unsigned long long mu_ = smu[t], md_ = smd[t];
+      if ((mu_ | md_) == 0ULL) continue;
+      const bool up = (mu_ >> lane) & 1ULL, dn = (md_ >> lane) & 1ULL;
       if (up || dn) {
+        const int k = k0 + t * YW + lane;
         // holes in ascending index order: flag(:,j) of :163-164
-        const int hpos = cu + cd + lanes_below(mu_ | md_);
+        const int hpos = scu[t] + scd[t] + lanes_below(mu_ | md_);
         if (hpos < 2 * P.mcap) hole[2 * mb + hpos] = k;
         else bad |= DERR_MOVER_OVER;
-        const int pos = up ? cu + lanes_below(mu_) : cd + lanes_below(md_);
+        const int pos = up ? scu[t] + lanes_below(mu_) : scd[t] + lanes_below(md_);
         if (pos < P.mcap) {
           const size_t q = mb + pos;
-          (up ? mu : md).store(q, arriving(B, base + k, j, dir[t], P));
+          (up ? mu : md).store(q, arriving(B, base + k, j, up ? 1 : -1, P));
           (up ? mid_u : mid_d)[q] = id[base + k];
         } else {
           bad |= DERR_MOVER_OVER;
         }
       }
-      cu += __popcll(mu_);
-      cd += __popcll(md_);
     }
+    cu += tu;
+    cd += td;
+    __syncthreads();                                   // the round's LDS is reused
   }
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     nhole[seg] = min(cu + cd, 2 * P.mcap);
     nup[seg] = min(cu, P.mcap);
     ndn[seg] = min(cd, P.mcap);
@@ -231,9 +302,10 @@ __global__ __launch_bounds__(YW) void k_particle_y1(BP P, int xmode,
   if (bad) atomicOr(derr, bad);
 }
 
-// pass 2: one wavefront per segment.
+// pass 2: one workgroup per segment; the arrivals are independent, and the
+// first wavefront alone compacts the row when fewer arrive than left.
 template <class R>
-__global__ __launch_bounds__(YW) void k_particle_y2(BP P, int *__restrict__ np2, R B,
+__global__ __launch_bounds__(Y2T) void k_particle_y2(BP P, int *__restrict__ np2, R B,
                                                     int64_t *__restrict__ id,
                                                     const int *__restrict__ hole,
                                                     const int *__restrict__ nhole,
@@ -242,7 +314,7 @@ __global__ __launch_bounds__(YW) void k_particle_y2(BP P, int *__restrict__ np2,
                                                     const int64_t *mid_u, R md,
                                                     const int64_t *mid_d, int *derr) {
   const int seg = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x;                    // the compaction's lanes: wavefront 0
   const int s = seg / P.ny, jl = seg % P.ny;
   const int below = s * P.ny + (jl == 0 ? P.ny - 1 : jl - 1);
   const int above = s * P.ny + (jl == P.ny - 1 ? 0 : jl + 1);
@@ -269,14 +341,16 @@ __global__ __launch_bounds__(YW) void k_particle_y2(BP P, int *__restrict__ np2,
     return;
   }
   // holes filled by arrivals, and surplus arrivals appended
-  for (int a = lane; a < A; a += YW) {
+  for (int a = threadIdx.x; a < A; a += Y2T) {
     const int dst = a < H ? hl[a] : N + (a - H);
     const bool f = a < nF;
     const size_t q = (size_t)(f ? Fseg : Sseg) * P.mcap + (f ? a : a - nF);
     B.store(base + dst, (f ? FB : SB).load(q));
     id[base + dst] = (f ? Fid : Sid)[q];
   }
-  if (A < H) {
+  if (A < H && threadIdx.x < YW) {
+    // sources are the non-holes of [N - m, N) and destinations holes the
+    // arrivals do not take, so this runs beside the arrivals' stores
     const int m = H - A;
     int done = 0;    // non-hole tail slots already placed
     for (int q0 = 0; q0 < m; q0 += YW) {
@@ -304,7 +378,8 @@ __global__ __launch_bounds__(YW) void k_particle_y2(BP P, int *__restrict__ np2,
       done += __popcll(mk);
     }
   }
-  if (lane == 0) np2[seg] = newn;
+  __syncthreads();                                 // every lane has read np2[seg]
+  if (threadIdx.x == 0) np2[seg] = newn;
 }
 
 BP make_bp(const wm_ctx *ctx) {
@@ -367,10 +442,10 @@ int wm_k_particle_y(wm_ctx *ctx, int bank, int xmode) {
   }
   {
     const Rec64 &B = ctx->bank[bank];
-    hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, xmode,
+    hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YTH), 0, ctx->stream, P, xmode,
                        ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
                        ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
-    hipLaunchKernelGGL(k_particle_y2<Rec64>, dim3(nseg), dim3(YW), 0, ctx->stream, P, ctx->np2,
+    hipLaunchKernelGGL(k_particle_y2<Rec64>, dim3(nseg), dim3(Y2T), 0, ctx->stream, P, ctx->np2,
                        B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
                        ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
   }

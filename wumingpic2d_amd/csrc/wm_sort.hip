@@ -1,14 +1,17 @@
 // wm_sort.hip -- stable per-row bucket sort by x-cell (sort.f90:36-82).
 //
-// One wavefront per (species, row) segment, so the ordered walk through the
-// segment needs no workgroup barriers: the cell histogram and the running
-// write offsets live in LDS (nx counters), and the wavefront walks the
-// segment in index order, 64 particles at a time, with the loads of four
-// such tiles in flight together.  Inside a tile, lanes with equal keys are
-// grouped by a ballot loop over the distinct keys (sorted input has one to
-// three per tile), which gives each lane its rank among equal keys below it,
-// so the destination of every particle is
-//   start[key] + (equal keys already written) + rank,
+// One workgroup of NWS wavefronts per (species, row) segment.  The cell
+// histogram and the running write offsets live in LDS (nx counters): the
+// wavefronts count the segment's keys with LDS atomics (tile t of 64
+// particles by wavefront t % NWS, BT tiles in flight), the workgroup scans
+// the counts into cumcnt, and the scatter then walks the segment in rounds
+// of NWS consecutive tiles, wavefront w taking tile w of the round.  Inside a
+// tile, lanes with equal keys are grouped by a ballot loop over the distinct
+// keys (sorted input has one to three per tile), which gives each lane its
+// rank among equal keys below it; each wavefront lists its tile's (key,
+// count) pairs in LDS, and a lane adds the counts of its key in the round's
+// lower tiles, so the destination of every particle is
+//   start[key] + (equal keys of earlier rounds) + (of lower tiles) + rank,
 // exactly the reference's stable scatter.  Keys are int(x) as in sort.f90:60
 // (the reference assumes delx = 1).
 #include <algorithm>
@@ -24,61 +27,74 @@ using namespace wm_mig;
 using wm_bins::WaveMatch;
 using wm_bins::wave_match;
 
-constexpr int SW = 64;     // one wavefront per segment
-constexpr int BT = 4;      // tiles in flight
+constexpr int SW = 64;     // wavefront
+constexpr int NWS = 8;     // wavefronts per segment
+constexpr int STH = NWS * SW;
+constexpr int BT = 4;      // tiles in flight per wavefront (count)
 
 struct SP {
   int nx, nxgs, cap;
 };
 
 template <class R>
-__global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, R src,
-                                             const int64_t *__restrict__ ids, R dst,
-                                             int64_t *__restrict__ idd,
-                                             int *__restrict__ cumcnt, int *derr) {
+__global__ __launch_bounds__(STH) void k_sort(SP P, const int *__restrict__ np2, R src,
+                                              const int64_t *__restrict__ ids, R dst,
+                                              int64_t *__restrict__ idd,
+                                              int *__restrict__ cumcnt, int *derr) {
   extern __shared__ int cnt[];             // nx counters, then running offsets
+  __shared__ int lkey[NWS][SW], lcnt[NWS][SW], nl[NWS], wsum[NWS];
   using V = typename R::Val;
   const int seg = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n = np2[seg];
+  const int nt = (n + SW - 1) / SW;
   const size_t base = (size_t)seg * P.cap;
   int bad = 0;
-  for (int i = lane; i < P.nx; i += SW) cnt[i] = 0;
+  for (int i = threadIdx.x; i < P.nx; i += STH) cnt[i] = 0;
   __syncthreads();
   // count (sort.f90:59-62)
-  for (int k0 = 0; k0 < n; k0 += SW * BT) {
+  for (int t0 = w; t0 < nt; t0 += NWS * BT) {
     int key[BT];
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
-      const int k = k0 + b * SW + lane;
+      const int k = (t0 + b * NWS) * SW + lane;
       key[b] = k < n ? src.key(src.load(base + k), P.nxgs) : -1;
     }
 #pragma unroll
     for (int b = 0; b < BT; ++b) {
-      const int k = k0 + b * SW + lane;
+      const int k = (t0 + b * NWS) * SW + lane;
       bool act = k < n;
       if (act && (key[b] < 0 || key[b] >= P.nx)) {
         bad = 1;
         act = false;
       }
       const WaveMatch m = wave_match(key[b], act);
-      if (act && m.leader) cnt[key[b]] += m.count;
+      if (act && m.leader) atomicAdd(&cnt[key[b]], m.count);
     }
   }
   __syncthreads();
   // exclusive prefix -> cumcnt(nxs:nxe+1) (sort.f90:64-69); cnt becomes the
-  // running write offset of each cell
+  // running write offset of each cell.  Thread th owns a run of per cells.
   {
-    const int per = (P.nx + SW - 1) / SW;
-    const int lo = min(P.nx, lane * per), hi = min(P.nx, lo + per);
+    const int per = (P.nx + STH - 1) / STH;
+    const int lo = min(P.nx, (int)threadIdx.x * per), hi = min(P.nx, lo + per);
     int s = 0;
     for (int i = lo; i < hi; ++i) s += cnt[i];
     int incl = s;
+#pragma unroll
     for (int off = 1; off < SW; off <<= 1) {
       const int v = __shfl_up(incl, off);
       if (lane >= off) incl += v;
     }
-    int run = incl - s;
+    if (lane == SW - 1) wsum[w] = incl;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < NWS; ++v) {
+      pre += v < w ? wsum[v] : 0;
+      tot += wsum[v];
+    }
+    int run = pre + incl - s;
     int *cc = cumcnt + (size_t)seg * (P.nx + 1);
     for (int i = lo; i < hi; ++i) {
       const int c = cnt[i];
@@ -86,39 +102,57 @@ __global__ __launch_bounds__(SW) void k_sort(SP P, const int *__restrict__ np2, 
       cnt[i] = run;
       run += c;
     }
-    if (lane == SW - 1) cc[P.nx] = incl;
+    if (threadIdx.x == 0) cc[P.nx] = tot;
   }
   __syncthreads();
-  // stable scatter (sort.f90:71-75), in index order
-  for (int k0 = 0; k0 < n; k0 += SW * BT) {
-    int key[BT];
-    V v[BT];
-    int64_t vid[BT];
-#pragma unroll
-    for (int b = 0; b < BT; ++b) {
-      const int k = k0 + b * SW + lane;
-      key[b] = -1;
+  // stable scatter (sort.f90:71-75): rounds of NWS tiles, the next round's
+  // tile loading while this one is placed
+  V v{}, nv{};
+  int64_t vid = 0, nvid = 0;
+  int key = -1, nkey = -1;
+  {
+    const int k = w * SW + lane;
+    if (k < n) {
+      v = src.load(base + k);
+      vid = ids[base + k];
+      key = src.key(v, P.nxgs);
+    }
+  }
+  for (int r0 = 0; r0 < nt; r0 += NWS) {
+    {
+      const int k = (r0 + NWS + w) * SW + lane;
+      nkey = -1;
       if (k < n) {
-        const size_t s = base + k;
-        v[b] = src.load(s);
-        vid[b] = ids[s];
-        key[b] = src.key(v[b], P.nxgs);
+        nv = src.load(base + k);
+        nvid = ids[base + k];
+        nkey = src.key(nv, P.nxgs);
       }
     }
-#pragma unroll
-    for (int b = 0; b < BT; ++b) {
-      const bool act = key[b] >= 0 && key[b] < P.nx;
-      const WaveMatch m = wave_match(key[b], act);
-      if (act) {
-        const size_t d = base + cnt[key[b]] + m.rank;
-        dst.store(d, v[b]);
-        idd[d] = vid[b];
-      }
-      // every lane has read its offset before the leaders advance them
-      __builtin_amdgcn_wave_barrier();
-      if (act && m.leader) cnt[key[b]] += m.count;
-      __builtin_amdgcn_wave_barrier();
+    const bool act = key >= 0 && key < P.nx;
+    const WaveMatch m = wave_match(key, act);
+    const unsigned long long lm = __ballot(act && m.leader);
+    if (act && m.leader) {
+      const int li = __popcll(lm & ((1ULL << lane) - 1ULL));
+      lkey[w][li] = key;
+      lcnt[w][li] = m.count;
     }
+    if (lane == 0) nl[w] = __popcll(lm);
+    __syncthreads();
+    if (act) {
+      int prior = 0;                       // equal keys in the round's lower tiles
+      for (int u = 0; u < w; ++u) {
+        const int nu = nl[u];
+        for (int i = 0; i < nu; ++i) prior += lkey[u][i] == key ? lcnt[u][i] : 0;
+      }
+      const size_t d = base + cnt[key] + prior + m.rank;
+      dst.store(d, v);
+      idd[d] = vid;
+    }
+    __syncthreads();                       // every lane has read the offsets and lists
+    if (act && m.leader) atomicAdd(&cnt[key], m.count);
+    v = nv;
+    vid = nvid;
+    key = nkey;
   }
   if (bad) atomicOr(derr, DERR_RANGE);
 }
@@ -723,7 +757,7 @@ int wm_k_sort(wm_ctx *ctx, int src, int dst) {
     return WM_ERR_ARG;
   }
   {
-    hipLaunchKernelGGL(k_sort<Rec64>, dim3(nseg), dim3(SW), lds, ctx->stream, P, ctx->np2,
+    hipLaunchKernelGGL(k_sort<Rec64>, dim3(nseg), dim3(STH), lds, ctx->stream, P, ctx->np2,
                        ctx->b64(src), ctx->id[ctx->idcur], ctx->b64(dst), ctx->id[1 - ctx->idcur],
                        ctx->cumcnt, ctx->derr);
   }
