@@ -59,8 +59,7 @@ __device__ inline int wrap_x(double &x, const BP &P) {
 // boundary_reconnection.f90:76-88): a particle in a cell below nxs+1 is
 // mirrored about (nxs+1) delx, one in a cell at or above nxe-1 about
 // (nxe-1) delx, and its momentum reversed.  Returns the new cell.
-__device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) {
-  const double x = R.x[p];
+__device__ inline int wall_x_at(const Rec64 &R, size_t p, double x, const BP &P) {
   const int ipos = (int)(x / P.delx);
   double w;
   if (ipos < P.xs + 1) w = P.wl;
@@ -73,6 +72,7 @@ __device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) {
   R.uz[p] = -R.uz[p];
   return (int)(xn / P.delx);
 }
+__device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) { return wall_x_at(R, p, R.x[p], P); }
 
 __global__ void k_wall_x(BP P, const int *__restrict__ np2, Rec64 B, int *derr) {
   const int seg = blockIdx.y;
@@ -141,30 +141,32 @@ constexpr int YTH = YNW * YW;
 constexpr int YST = 1024;  // tiles of 64 particles per pass-1 round
 constexpr int YT = 4;      // tiles in flight per wavefront
 constexpr int Y2T = 256;   // pass-2 workgroup
+constexpr int Y2TAIL = 8192;  // top slots whose holes pass 2 marks in LDS
 
 __device__ inline int lanes_below(unsigned long long m) {
   const int lane = threadIdx.x & 63;
   return __popcll(m & ((1ULL << lane) - 1ULL));
 }
 
-// Row change of particle p of row j: -1, 0, +1, or 2 (a jump the reference
-// does not index).  FP64: jpos = int(y/delx) (boundary_periodic.f90:147),
-// with particle_x applied first when fused: xmode 1 the periodic round-down
-// wrap, 2 the walls' mirrors.
-__device__ inline int row_change(const Rec64 &R, size_t p, int j, int xmode, const BP &P,
-                                 int &bad) {
+// Row change of particle p of row j, whose x and y the caller loaded: -1,
+// 0, +1, or 2 (a jump the reference does not index).  FP64: jpos =
+// int(y/delx) (boundary_periodic.f90:147), with particle_x applied first
+// when fused: xmode 1 the periodic round-down wrap, 2 the walls' mirrors.
+// Taking the loaded values lets a caller issue several particles' loads
+// before the first store of a wrapped x.
+__device__ inline int row_change(const Rec64 &R, size_t p, double x, double y, int j, int xmode,
+                                 const BP &P, int &bad) {
   if (xmode == 2) {
-    const int inew = wall_x(R, p, P);
+    const int inew = wall_x_at(R, p, x, P);
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   } else if (xmode == 1) {
-    double xv = R.x[p];
-    const double x0 = xv;
+    double xv = x;
     wrap_x(xv, P);
-    if (xv != x0) R.x[p] = xv;
+    if (xv != x) R.x[p] = xv;
     const int inew = (int)(xv / P.delx);
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   }
-  const int jpos = (int)(R.y[p] / P.delx);
+  const int jpos = (int)(y / P.delx);
   const int d = jpos - j;
   return (d >= -1 && d <= 1) ? d : 2;
 }
@@ -218,11 +220,24 @@ __global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
     const int ntile = min(YST, (n - k0 + YW - 1) / YW);
     // classify (the periodic x wrap or the walls' mirrors are applied here)
     for (int t0 = w; t0 < ntile; t0 += YNW * YT) {
+      // every tile's x and y in flight before the first wrapped x is stored
+      double xv[YT], yv[YT];
+#pragma unroll
+      for (int b = 0; b < YT; ++b) {
+        const int k = k0 + (t0 + b * YNW) * YW + lane;
+        xv[b] = yv[b] = 0.0;
+        if (t0 + b * YNW < ntile && k < n) {
+          xv[b] = B.x[base + k];
+          yv[b] = B.y[base + k];
+        }
+      }
       int dir[YT];
 #pragma unroll
       for (int b = 0; b < YT; ++b) {
         const int k = k0 + (t0 + b * YNW) * YW + lane;
-        dir[b] = t0 + b * YNW < ntile && k < n ? row_change(B, base + k, j, xmode, P, bad) : 0;
+        dir[b] = t0 + b * YNW < ntile && k < n
+                     ? row_change(B, base + k, xv[b], yv[b], j, xmode, P, bad)
+                     : 0;
       }
 #pragma unroll
       for (int b = 0; b < YT; ++b) {
@@ -313,6 +328,7 @@ __global__ __launch_bounds__(Y2T) void k_particle_y2(BP P, int *__restrict__ np2
                                                     const int *__restrict__ ndn, R mu,
                                                     const int64_t *mid_u, R md,
                                                     const int64_t *mid_d, int *derr) {
+  __shared__ unsigned char tail[Y2TAIL];           // holes among the top slots
   const int seg = blockIdx.x;
   const int lane = threadIdx.x;                    // the compaction's lanes: wavefront 0
   const int s = seg / P.ny, jl = seg % P.ny;
@@ -348,10 +364,24 @@ __global__ __launch_bounds__(Y2T) void k_particle_y2(BP P, int *__restrict__ np2
     B.store(base + dst, (f ? FB : SB).load(q));
     id[base + dst] = (f ? Fid : Sid)[q];
   }
+  // With fewer arrivals than holes the top m = H - A slots empty: their
+  // non-holes fill the holes the arrivals leave.  Sources are the non-holes
+  // of [N - m, N) and destinations holes the arrivals do not take, so this
+  // runs beside the arrivals' stores.  The holes among the top m slots are
+  // the last entries of the ascending list (at most m of them, all in
+  // hl[A..H)): the workgroup marks them in LDS first.
+  const int m = H - A;
+  const bool marked = A < H && m <= Y2TAIL;
+  if (marked) {
+    for (int i = threadIdx.x; i < m; i += Y2T) tail[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += Y2T) {
+      const int v = hl[H - 1 - i];
+      if (v >= N - m) tail[v - (N - m)] = 1;
+    }
+    __syncthreads();
+  }
   if (A < H && threadIdx.x < YW) {
-    // sources are the non-holes of [N - m, N) and destinations holes the
-    // arrivals do not take, so this runs beside the arrivals' stores
-    const int m = H - A;
     int done = 0;    // non-hole tail slots already placed
     for (int q0 = 0; q0 < m; q0 += YW) {
       const int qi = q0 + lane;          // slot N-1-qi, walking down from the top
@@ -359,14 +389,18 @@ __global__ __launch_bounds__(Y2T) void k_particle_y2(BP P, int *__restrict__ np2
       int src = 0;
       if (qi < m) {
         src = N - 1 - qi;
-        // src is a hole iff it is in the sorted list hl[A..H)
-        int lo = A, hi = H;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (hl[mid] < src) lo = mid + 1;
-          else hi = mid;
+        if (marked) {
+          nonhole = tail[m - 1 - qi] == 0;
+        } else {
+          // src is a hole iff it is in the sorted list hl[A..H)
+          int lo = A, hi = H;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (hl[mid] < src) lo = mid + 1;
+            else hi = mid;
+          }
+          nonhole = !(lo < H && hl[lo] == src);
         }
-        nonhole = !(lo < H && hl[lo] == src);
       }
       const unsigned long long mk = __ballot(nonhole);
       const int nrank = done + lanes_below(mk);

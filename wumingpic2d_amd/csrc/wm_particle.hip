@@ -5,8 +5,11 @@
 // workgroup's 256 lanes form 32 groups of 8 lanes, group g owns cell
 // i0+g and its 8 lanes stride through that cell's particles of every species
 // (the reference's cell -> species -> particle loop, particle.f90:85-94 and
-// field.f90:212-221).  Particles of one cell are contiguous (sorted), so each
-// group reads 64 contiguous bytes per array per iteration.
+// field.f90:212-221).  Particles of one cell are contiguous (sorted); a
+// group's windows start at the 8-particle boundary at or below the cell's
+// first particle (lanes before it idle in the first window), so each of its
+// loads and stores is one aligned 64-byte half line per array, shared at
+// the cell's ends with the neighbouring groups of the same wavefront.
 //
 // Push fields come from a per-workgroup LDS tile of the cell-centred fields.
 // Deposit: each lane accumulates the 5x5 stencils of its particles in
@@ -344,14 +347,15 @@ __global__ __launch_bounds__(256, 3) void k_particle(
       unsigned long long mask = 0ULL;
       // the next particle's state is loaded while the current one is
       // computed (one step of software pipelining per lane); windows of
-      // GROUP consecutive particles start at the cell's first particle
-      const int ka = kb;
+      // GROUP consecutive particles start at the GROUP boundary at or below
+      // the cell's first particle
+      const int ka = kb - kb % GROUP;
       // segment bases: the same for the whole workgroup, 32-bit indices below
       const double *Xa = xa + base, *Ya = ya + base;
       const double *Uxa = uxa + base, *Uya = uya + base, *Uza = uza + base;
       int k = ka + sub;
       double nx0 = 0, ny0 = 0, nux = 0, nuy = 0, nuz = 0;
-      if (k < ke) {
+      if (k >= kb && k < ke) {
         nx0 = Xa[k]; ny0 = Ya[k];
         if (PUSH) { nux = Uxa[k]; nuy = Uya[k]; nuz = Uza[k]; }
       }
@@ -363,6 +367,7 @@ __global__ __launch_bounds__(256, 3) void k_particle(
           nx0 = Xa[kn]; ny0 = Ya[kn];
           if (PUSH) { nux = Uxa[kn]; nuy = Uya[kn]; nuz = Uza[kn]; }
         }
+        if (k < kb) continue;                  // before the cell (first window only)
         double x1, y1, ux, uy, uz;
         // keep the 54 tile values in LDS instead of hoisting them into
         // registers for the whole cell loop (occupancy)
