@@ -47,7 +47,20 @@ WORKLOADS = {
     "weibel_2048": (2048, 2048, 64, 2.0, 2000),
     "weibel_128": (128, 128, 64, 2.0, 200),
     "weibel_1024": (1024, 1024, 64, 2.0, 2000),
+    # 8192 x 8192, 32 ppc/species (4.3e9 particles): rows of 1.4 x the mean,
+    # 32768 mover records per row and two training pairs per slot ring keep
+    # the FP32 state near 250 GB of the card's 288; the FP64 form of it
+    # (412 GB of particle banks) does not fit, so it has no comparison leg
+    "weibel_8192": (8192, 8192, 32, 1.4, 500),
 }
+# per workload: FP32 mover records per row (0: a quarter of the row's
+# slots), the learned preconditioner's pair ring and pairs per solve, and
+# whether the FP64 comparison leg fits
+WORKLOAD_EXTRA = {"weibel_8192": {"mrow": 32768, "pairs": (2, 2), "compare": False}}
+
+
+def extra(workload: str, key: str, default):
+    return WORKLOAD_EXTRA.get(workload, {}).get(key, default)
 EVERY = 10            # steps between energy-history entries
 FIT = (60, 160)       # linear-phase window (steps) of the growth-rate fit
 
@@ -207,7 +220,7 @@ def make_context(wm, weibel, workload: str, form: str):
     cap = int(capf * ppc * nx)
     ctx = wm.Context(0)
     ctx.set_form(wm.FORM_FP32 if form == "fp32" else wm.FORM_FP64)
-    ctx.set_capacity(cap, 0 if form == "fp32" else max(1024, cap // 16))
+    ctx.set_capacity(cap, extra(workload, "mrow", 0) if form == "fp32" else max(1024, cap // 16))
     g = (p["ndim"], p["np"], p["nsp"], p["nxgs"], p["nxge"], p["nygs"], p["nyge"], p["nys"], p["nye"])
     ctx.boundary_periodic__init(*g, 0, 0, 0, 0, 0, 0, [0], p["delx"], p["delt"], p["c"])
     ctx.particle__init(*g, p["delx"], p["delt"], p["c"], p["q"], p["r"])
@@ -269,7 +282,7 @@ def run_leg(wm, weibel, args, form: str, solver: str) -> dict:
     S = Steps(ctx, np)
     train = {}
     if solver == "fcg":
-        ctx.pc_collect(16, 4)
+        ctx.pc_collect(*extra(args.workload, "pairs", (16, 4)))
     S.run(args.warmup, f"{form} warm-up")
     if solver == "fcg":
         t0 = time.perf_counter()
@@ -494,7 +507,8 @@ def main() -> int:
             "what": f"{args.steps} steps of each, one after the other on the same FP32 state; "
                     "the second stream holds only the bins' fill check"}
     ref = None
-    if not args.no_compare and not (args.form == "fp64" and solver == "cg"):
+    if (not args.no_compare and extra(args.workload, "compare", True)
+            and not (args.form == "fp64" and solver == "cg")):
         ref = run_leg(wm, weibel, args, "fp64", "cg")
         rl = leg_line(ref)
         e1, e0 = head["energy"], ref["energy"]
