@@ -162,18 +162,19 @@ def _one_row_state(p, rows):
 
 
 def test_fp32_full_bin_is_laid_out_anew():
-    """Row segments of 176 slots leave one 8-slot block of room beyond the
-    bins' needs (16 cells of at least 8 slots; 30 particles in cell 5 get
-    32).  Ten fast particles of cell 4 arrive in cell 5: the fill check
-    after the push sees 40 > 32, and the block of cells is laid out anew
-    before the next push.  Bins move whole, their movers' holes with them,
-    so cell 4 keeps its 10 slots until the next push places its movers: the
-    new layout needs 14 x 8 + 16 + 40 = 168 slots.  The state then matches
-    the FP64 oracle's."""
+    """Row segments of 176 slots leave two 8-slot blocks of room beyond the
+    bins' needs (16 cells of at least 8 slots; 12 particles in cell 4 get 16,
+    30 in cell 5 get 32).  Shared out by the running sum of the needs, one of
+    them goes to cell 5: 40 slots.  Twelve fast particles of cell 4 arrive in
+    cell 5: the fill check after the push sees 42 > 40.  Bins move whole,
+    their movers' holes with them, so cell 4 keeps its 12 slots until the
+    next push places its movers: the new layout needs 14 x 8 + 16 + 48 = 176
+    slots, the whole span, so the block is not re-laid alone but its segment
+    is, before the next push.  The state then matches the FP64 oracle's."""
     from wumingpic2d_amd import weibel
     p = weibel.grid_params(16, 8, 4)
     jl, y = 3, p["nys"] + 3 + 0.5
-    row = [(p["nxgs"] + 4.95, y, 0.9, 0.0)] * 10 + \
+    row = [(p["nxgs"] + 4.95, y, 0.9, 0.0)] * 12 + \
           [(p["nxgs"] + 5 + f, y, 0.0, 0.0) for f in np.linspace(0.1, 0.6, 30)]
     up, np2, cc, uf = _one_row_state(p, {jl: row})
     ctx = _ctx32(p)
@@ -185,10 +186,10 @@ def test_fp32_full_bin_is_laid_out_anew():
         ctx.step(1)
         orc.step(up, gp, uf, cc, np2)
     st = ctx.capacity_stats()
-    assert st["relayouts"] >= 1 and st["max_bin_fill"] >= 40, st
+    assert st["relayouts"] >= 1 and st["segment_relayouts"] >= 1 and st["max_bin_fill"] >= 42, st
     gup, gnp2, guf = np.zeros_like(up), np.zeros_like(np2), np.zeros_like(uf)
     ctx.state_download(up=gup, np2=gnp2, uf=guf)
-    assert gnp2.sum() == np2.sum() == 40
+    assert gnp2.sum() == np2.sum() == 42
     a, b = _by_id(up, np2), _by_id(gup, gnp2)
     assert a.keys() == b.keys()
     assert max(np.abs(a[k] - b[k]).max() for k in a) < 2e-5

@@ -41,17 +41,25 @@ __device__ inline int round_up(int v, int a) { return (v + a - 1) / a * a; }
 // bins to the mean keeps a bin that is momentarily light from being laid out
 // tight and overflowing as soon as its count returns to the mean; the share
 // in proportion gives a dense bin, which gains particles fastest where a
-// plasma pinches, headroom for as many steps as a mean one.  When the raised
-// needs do not fit, the plain counts are used (floor 0).
+// plasma pinches, headroom for as many steps as a mean one.  The shares are
+// apportioned by the running sum of the needs: bin i starts at
+// P + WM_BIN_ALIGN * floor(units * P / S), P the needs of the bins before it,
+// so the free blocks are all handed out and a bin's share differs from its
+// proportion by less than one block.  (Rounding each share down on its own
+// gave nothing to every bin whose proportion was below one block: with 32
+// particles per cell and rows of 1.4 times the mean, every bin at the mean
+// count was laid out full.)  When the raised needs do not fit, the plain
+// counts are used (floor 0).
 struct Policy {
   int floor;        // counts below it are raised to it
   long long units;  // free blocks of WM_BIN_ALIGN slots after the needs
   long long S;      // the needs' sum
 };
 __device__ inline int need_of(int c, int floor) { return round_up(max(max(c, floor), 1), WM_BIN_ALIGN); }
-__device__ inline int room_of(const Policy &p, int c) {
-  const int nd = need_of(c, p.floor);
-  return nd + (int)(p.units * nd / p.S) * WM_BIN_ALIGN;
+// First slot (from the window's start) of the bin whose predecessors' needs
+// sum to P.
+__device__ inline int start_of(const Policy &p, long long P) {
+  return (int)(P + (long long)WM_BIN_ALIGN * (p.units * P / p.S));
 }
 // From the window's count sum C, the sum of needs raised to the mean (S1)
 // and the plain one (S0): false when even the plain needs exceed the span.
@@ -102,18 +110,18 @@ __device__ bool wave_layout(F count, int nx, int cap, int *bs) {
   }
   Policy p;
   if (!make_policy(C, S1, S0, nx, cap, p)) return false;
-  int mine = 0;
-  for (int i = lo; i < hi; ++i) mine += room_of(p, count(i));
-  int incl = mine;
+  long long mine = 0;
+  for (int i = lo; i < hi; ++i) mine += need_of(count(i), p.floor);
+  long long incl = mine;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
+    const long long v = __shfl_up(incl, o, 64);
     if (lane >= o) incl += v;
   }
-  int run = incl - mine;
+  long long run = incl - mine;                  // needs of the bins before lo
   for (int i = lo; i < hi; ++i) {
-    bs[i] = run;
-    run += room_of(p, count(i));
+    bs[i] = start_of(p, run);
+    run += need_of(count(i), p.floor);
   }
   if (lane == 0) bs[nx] = cap;     // the slack after the last block joins the last bin
   return true;

@@ -192,12 +192,14 @@ __device__ inline void core32(float *core, float ox0, float oy0, float ox1, floa
   }
 }
 
-// A cell's stencil sums at tile offset (a, b) of component comp: the
-// movers' sums, 65 per cell (Jx 0..19 as (a-1)*5 + b with a >= 1, Jy 20..39
-// as 20 + a*4 + (b-1) with b >= 1, Jz 40..64 as 40 + a*5 + b), plus the
-// stayers' 21 core sums where the core has the offset (Jx a = 2..3, Jy
-// b = 2..3, offsets 1..3 otherwise; the layout of core32).  Both are FP32
-// sums of the group's lanes, added here in FP64.
+// A cell's stencil sums: the movers' sums, 65 per cell (Jx 0..19 as
+// (a-1)*5 + b with tile offset a >= 1, Jy 20..39 as 20 + a*4 + (b-1) with
+// b >= 1, Jz 40..64 as 40 + a*5 + b), and the stayers' 21 core sums where
+// the core has the offset (Jx a = 2..3, Jy b = 2..3, offsets 1..3
+// otherwise; the layout of core32).  Both are FP32 sums of the group's
+// lanes; after the movers' deposit each group adds its cell's pair in FP64
+// (csum, which takes the place of the movers' list offsets in LDS) for the
+// tile assembly.
 constexpr int CCS = 24;               // floats per cell of the core sums (21, padded)
 constexpr int CMS = 68;               // floats per cell of the movers' sums (65, padded)
 __shared__ float ccore[CHUNK * CCS];
@@ -207,13 +209,22 @@ __device__ inline int core_of(int comp, int a, int b) {
   if (comp == 1) return (a >= 1 && a <= 3 && b >= 2 && b <= 3) ? 6 + (a - 1) * 2 + (b - 2) : -1;
   return (a >= 1 && a <= 3 && b >= 1 && b <= 3) ? 12 + (a - 1) * 3 + (b - 1) : -1;
 }
-__device__ inline double acc_at(int cell, int comp, int a, int b) {
+// Entry e of the 65 sums in FP64: core + movers (the core term first).
+__device__ inline double cell_sum(int cell, int e) {
+  const int comp = e < 20 ? 0 : (e < 40 ? 1 : 2);
+  const int a = comp == 0 ? e / 5 + 1 : (comp == 1 ? (e - 20) / 4 : (e - 40) / 5);
+  const int b = comp == 0 ? e % 5 : (comp == 1 ? (e - 20) % 4 + 1 : (e - 40) % 5);
+  const int c = core_of(comp, a, b);
+  const double mv = (double)cmov[cell * CMS + e];
+  const double cv = (double)ccore[cell * CCS + (c < 0 ? 0 : c)];
+  return c < 0 ? mv : cv + mv;
+}
+// Sum of component comp at tile offset (a, b) of a cell, from csum.
+__device__ inline double acc_at(const double *csum, int cell, int comp, int a, int b) {
   const int idx = comp == 0 ? (a - 1) * 5 + b : (comp == 1 ? 20 + a * 4 + (b - 1) : 40 + a * 5 + b);
   const bool none = (comp == 0 && a == 0) || (comp == 1 && b == 0);
-  const double mv = (double)cmov[cell * CMS + (none ? 0 : idx)];
-  const int c = core_of(comp, a, b);
-  const double cv = (double)ccore[cell * CCS + (c < 0 ? 0 : c)];
-  return none ? 0.0 : (c < 0 ? mv : cv + mv);
+  const double v = csum[cell * NACC + (none ? 0 : idx)];
+  return none ? 0.0 : v;
 }
 
 // Sum over the 16 lanes of one DPP row (a group): quad butterflies, then
@@ -267,6 +278,8 @@ __shared__ uint16_t xblk[CHUNK * MAXB];           // spill block of each block o
 __shared__ int gcnt[CHUNK];                       // movers listed per cell
 __shared__ int scnt[WM_MAXSP * CHUNK];            // particles per (species, cell)
 __shared__ int sbin[WM_MAXSP * CHUNK];            // first slot of each (species, cell) bin
+__shared__ int gsb[WM_MAXSP * CHUNK];             // movers listed per cell up to the end of
+                                                  // each species (the lists' species bounds)
 __shared__ int rfirst;                            // first record of the workgroup's movers in
                                                   // its row's pool (-1: the pool is full)
 __shared__ float sqc[WM_MAXSP];                   // q delx / delt per species (an LDS read:
@@ -373,16 +386,29 @@ struct Spc {                    // one species of one cell
   size_t base0;                 // first slot of the species' row segment (workgroup-uniform)
   float q, fac1, txxx, fac2, qc;
 };
+constexpr int BUF_WORD3 = 0x00020000;   // raw (untyped) buffer access on gfx9-family GPUs
+// Buffer resource of nbytes from p, which is the same in every lane (a row
+// segment's start): read from the first lane into scalar registers, so that
+// the buffer instructions take it directly.
+__device__ inline __amdgpu_buffer_rsrc_t segment_rsrc(const void *p, int nbytes) {
+  const unsigned long long a = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void *u = (void *)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(u, 0, nbytes, BUF_WORD3);
+}
 // First slot of the group's bin: the workgroup-uniform segment start plus the
 // bin's 32-bit offset, so that no 64-bit per-lane base stays live through the
 // windows.
 __device__ inline size_t spc_base(const Spc &S) { return S.base0 + (size_t)(unsigned)S.off; }
 
-__device__ inline void make_spc(Spc &S, int s, int n, int off, int jl, const PP32 &P) {
+__device__ inline void make_spc(Spc &S, int s, int n, int off, int jl, const PP32 &P,
+                                const Bins32 &A) {
   S.s = s;
   S.n = n;
   S.off = off;
   S.base0 = ((size_t)s * P.ny + jl) * (size_t)P.cap;
+
   S.q = P.q[s];
   S.fac1 = P.q[s] / P.r[s] * 0.5f * P.delt;
   S.txxx = S.fac1 * S.fac1;
@@ -420,28 +446,22 @@ __device__ inline int lane_sub() { return threadIdx.x % GROUP; }
 // the window's stores stay contiguous).  A stayer's core stencil is
 // accumulated; a mover is listed in LDS with the inputs of its deposit (its
 // record is written after the loop, from its slot).
-__device__ inline void push_window(Lane &L, float *core, int k, float ox0, float oy0, float cux,
-                                   float cuy, float cuz, const Spc &S, const PP32 &P,
-                                   const Bins32 &A, const Spill &X) {
-  // Every lane pushes and stores: a lane past the bin's count holds its last
-  // particle (or slot 0 of an empty bin) and writes back the same values as
-  // that particle's lane, to the same slot.  Unconditional stores keep the
-  // compiler's count of memory operations in flight exact, so its waits for
-  // the register windows do not also wait for recent stores.
-  const bool on = k < S.n;
+// The push of one window's particles: the new state, the stayers' stencils,
+// the movers' list entries.
+__device__ inline void push_compute(Lane &L, float *core, int k, bool on, float ox0, float oy0,
+                                    float cux, float cuy, float cuz, const Spc &S, const PP32 &P,
+                                    const Spill &X, float &ox1, float &oy1, float &nux, float &nuy,
+                                    float &nuz) {
   __asm__ volatile("" ::: "memory");               // gather coefficients re-read from LDS
   const Push32 o = boris32(CF4 + lane_g() * (CFS / 4), ox0, oy0, cux, cuy, cuz, P, S.fac1, S.txxx,
                            S.fac2);
-  float ox1, oy1;
   const int dx = rebase(o.ox, ox1);
   const int dy = rebase(o.oy, oy1);
+  nux = o.ux;
+  nuy = o.uy;
+  nuz = o.uz;
   L.ok = L.ok & (!on | (fmaxf(fabsf(o.ox - 0.5f), fabsf(o.oy - 0.5f)) < 1.5f));
   const bool cross = on && (dx | dy) != 0;
-  int kk = min(k, max(S.n - 1, 0));
-  __asm__ volatile("" : "+v"(kk));                 // store addresses formed here, not earlier
-  const size_t p = spc_base(S) + kk;
-  A.q[p] = make_float4(ox1, oy1, o.ux, o.uy);
-  A.uz[p] = o.uz;
   const float qv = S.q * (o.uz * o.ig);            // charge times v_z
   if (on && !cross) {
     core32(core, ox0, oy0, ox1, oy1, S.qc, qv);
@@ -491,8 +511,35 @@ __device__ inline void push_window(Lane &L, float *core, int k, float ox0, float
     else
       L.lost = true;
   }
+}
+
+__device__ inline void push_window(Lane &L, float *core, int k, float ox0, float oy0, float cux,
+                                   float cuy, float cuz, const Spc &S, const PP32 &P,
+                                   const Bins32 &A, const Spill &X) {
+  // A window none of whose lanes holds a particle (the register windows past
+  // the wave's fullest bin) skips the push; every window still issues its
+  // stores, which the hardware drops for a lane past the bin's count (its
+  // offset lies past the segment).  Stores outside any branch keep the
+  // compiler's count of memory operations in flight exact, so its waits for
+  // the register windows do not also wait for recent stores.
+  const bool on = k < S.n;
+  // (what a skipped window stores is dropped: no lane is on)
+  float ox1 = 0.0f, oy1 = 0.0f, nux = 0.0f, nuy = 0.0f, nuz = 0.0f;
+  if (__builtin_amdgcn_ballot_w64(on) != 0ULL)
+    push_compute(L, core, k, on, ox0, oy0, cux, cuy, cuz, S, P, X, ox1, oy1, nux, nuy, nuz);
+  // buffer resources of the segment's q and uz slots: a store at an offset
+  // past the segment is dropped by the hardware
+  const __amdgpu_buffer_rsrc_t rq = segment_rsrc(A.q + S.base0, P.cap * 16);
+  const __amdgpu_buffer_rsrc_t ru = segment_rsrc(A.uz + S.base0, P.cap * 4);
+  const unsigned sl = (unsigned)(S.off + k);
+  const unsigned bq = on ? sl * 16u : 0xFFFFFFF0u, bu = on ? sl * 4u : 0xFFFFFFFCu;
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(nuz), ru, bu, 0, 0);
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 qv4 = {__float_as_uint(ox1), __float_as_uint(oy1), __float_as_uint(nux), __float_as_uint(nuy)};
+  __builtin_amdgcn_raw_buffer_store_b128(qv4, rq, bq, 0, 0);
   __builtin_amdgcn_sched_barrier(0);               // windows are not interleaved (registers)
 }
+
 
 // All windows of one species of the group's cell: the PD register slots
 // (all of them, whatever the count), each refilled with the same window of
@@ -507,12 +554,25 @@ __device__ inline void push_species(Lane &L, float *core, Win &W, const Spc &S, 
                 X);
     if (MORE) load_slot(W, w, A, Sn, lane_sub());
   }
-  for (int w = PD * GROUP; w < S.n; w += GROUP) {
-    const int k = w + lane_sub();
-    const size_t p = spc_base(S) + min(k, S.n - 1);
-    const float4 v = A.q[p];
-    push_window(L, core, k, v.x, v.y, v.z, v.w, A.uz[p], S, P, A, X);
-    // nothing of this rare path stays in flight: otherwise the compiler, not
+  // Bins above PD windows (dense cells of the nonlinear phase): each window's
+  // load is issued before the window ahead of it is pushed.  A window's
+  // stores touch only its own slots; the next window's loads (a lane past
+  // the count re-reads the last slot, which lies in that next window) read
+  // slots they do not.
+  if (PD * GROUP < S.n) {
+    int w = PD * GROUP;
+    size_t p = spc_base(S) + min(w + lane_sub(), S.n - 1);
+    float4 v = A.q[p];
+    float u = A.uz[p];
+    for (; w < S.n; w += GROUP) {
+      const float4 cv = v;
+      const float cu = u;
+      p = spc_base(S) + min(w + GROUP + lane_sub(), S.n - 1);
+      v = A.q[p];
+      u = A.uz[p];
+      push_window(L, core, w + lane_sub(), cv.x, cv.y, cv.z, cv.w, cu, S, P, A, X);
+    }
+    // nothing of this path stays in flight: otherwise the compiler, not
     // knowing whether the loop ran, would drain the register windows after it
     __builtin_amdgcn_s_waitcnt(0);
   }
@@ -591,7 +651,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   Win win;
   Spc sc{}, sn{};
   if (g < ncell) {
-    make_spc(sc, 0, scnt[g], sbin[g], jl, P);
+    make_spc(sc, 0, scnt[g], sbin[g], jl, P, A);
 #pragma unroll
     for (int w = 0; w < PD; ++w) load_slot(win, w, A, sc, sub);
   }
@@ -612,16 +672,23 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   if (g < ncell) {
     // the next species' windows load while this one is pushed
     if (PAIR) {
-      make_spc(sn, 1, scnt[CHUNK + g], sbin[CHUNK + g], jl, P);
+      make_spc(sn, 1, scnt[CHUNK + g], sbin[CHUNK + g], jl, P, A);
       push_species<true>(L, core, win, sc, sn, P, A, X);
+      const int x0 = L.xcnt;
       push_species<false>(L, core, win, sn, sn, P, A, X);
+      if (sub == 0) {
+        gsb[g] = x0;
+        gsb[CHUNK + g] = L.xcnt;
+      }
     } else {
       for (int s = 0; s + 1 < P.nsp; ++s) {
-        make_spc(sn, s + 1, scnt[(s + 1) * CHUNK + g], sbin[(s + 1) * CHUNK + g], jl, P);
+        make_spc(sn, s + 1, scnt[(s + 1) * CHUNK + g], sbin[(s + 1) * CHUNK + g], jl, P, A);
         push_species<true>(L, core, win, sc, sn, P, A, X);
+        if (sub == 0) gsb[s * CHUNK + g] = L.xcnt;
         sc = sn;
       }
       push_species<false>(L, core, win, sc, sn, P, A, X);
+      if (sub == 0) gsb[(P.nsp - 1) * CHUNK + g] = L.xcnt;
     }
     // stay-in-cell stencils of every species, summed over the group once
 #pragma unroll
@@ -665,33 +732,31 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
   }
   __syncthreads();
   const int rf = rfirst;
-  // each mover's record: new state from its slot, ID, source slot and direction
-  {
-    int tot = 0;
-    for (int c = 0; c < CHUNK; ++c) tot += gcnt[c];
-    for (int t = threadIdx.x; t < tot && rf >= 0; t += NT) {
-      int c = 0, j = t;
-      while (j >= gcnt[c]) j -= gcnt[c++];
-      const int xv = xl_at(xent(c, j), X);
+  double *csum = (double *)xpl;                      // the list offsets are read by now
+  if (g < ncell) {
+    int before = 0;
+    for (int c = 0; c < g; ++c) before += gcnt[c];
+    const int gm = gcnt[g];
+    // each of the cell's movers' records, by its group: new state from its
+    // slot, ID, source slot and direction
+    for (int j = sub; j < gm && rf >= 0; j += GROUP) {
+      const int xv = xl_at(xent(g, j), X);
       const int s = xv >> 28, dir = (xv >> 20) & 0xF, k = xv & 0xFFFFF;
-      const size_t p = ((size_t)s * P.ny + jl) * (size_t)P.cap + (unsigned)sbin[s * CHUNK + c] + k;
+      const size_t p = ((size_t)s * P.ny + jl) * (size_t)P.cap + (unsigned)sbin[s * CHUNK + g] + k;
       const float4 v = A.q[p];
       const float uz = A.uz[p];
       const int64_t id = ids ? ids[p] : 0;
-      float4 *rp = (float4 *)(M.rec + (size_t)jl * mrow + rf + t);
+      float4 *rp = (float4 *)(M.rec + (size_t)jl * mrow + rf + before + j);
       rp[0] = v;
       rp[1] = make_float4(uz, __int_as_float(k | ((dir - (dir > 4)) << 24)),
                           __int_as_float((int)(uint32_t)id), __int_as_float((int)(id >> 32)));
     }
-  }
-  // per bin: movers in total and per direction
-  if (g < ncell) {
-    int j0 = 0, before = 0;
-    for (int c = 0; c < g; ++c) before += gcnt[c];
-    const int gm = gcnt[g];
+    // the cell's sums in FP64 for the tile assembly
+    for (int e = sub; e < NACC; e += GROUP) csum[g * NACC + e] = cell_sum(g, e);
+    // per bin: movers in total and per direction
     for (int s = 0; s < P.nsp; ++s) {
-      int m = 0;
-      while (j0 + m < gm && (xl_at(xent(g, j0 + m), X) >> 28) == s) ++m;
+      const int j0 = s ? gsb[(s - 1) * CHUNK + g] : 0;
+      const int m = gsb[s * CHUNK + g] - j0;
       BinSum *bs = msum + bin_of(s, jl, il, P.ny, P.nx);
       // lane sub < 8 of the cell's group: direction index sub (directions
       // 0..8 without 4); its count, and (a bin with more movers than its
@@ -726,9 +791,9 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
         bs->first = (uint32_t)(max(rf, 0) + before + j0);
         bs->m = (uint32_t)m;
       }
-      j0 += m;
     }
   }
+  __syncthreads();                                   // every cell's FP64 sums
   double *out = part + ((size_t)jl * P.nchunk + chunk) * TILE;
   for (int e = threadIdx.x; e < TILE; e += NT) {
     const int comp = e % 3;
@@ -736,7 +801,7 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
     const int b = e / (3 * DW);
     double sum = 0.0;
     const int glo = max(0, t - 4), ghi = min(ncell - 1, t);
-    for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(gg, comp, t - gg, b);
+    for (int gg = glo; gg <= ghi; ++gg) sum += acc_at(csum, gg, comp, t - gg, b);
     out[e] = sum;
   }
 }

@@ -450,8 +450,8 @@ __global__ __launch_bounds__(RW) void k_relayout_blocks(SB P, Relay R, const int
       wm_bins::Policy pol;
       const bool ok = wm_bins::make_policy(C, S1, S0, nb, hi - lo, pol) && T <= R.wsz &&
                       16 * pol.units * WM_BIN_ALIGN >= hi - lo;
-      const int room = in ? wm_bins::room_of(pol, c) : 0;
-      int incl = room, ninc = n;
+      const int nd = in ? wm_bins::need_of(c, pol.floor) : 0;
+      int incl = nd, ninc = n;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const int v = __shfl_up(incl, o, 64), w = __shfl_up(ninc, o, 64);
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(RW) void k_relayout_blocks(SB P, Relay R, const int
       }
       if (in) {
         s_old[lane] = bs[i];
-        s_new[lane] = lo + incl - room;
+        s_new[lane] = lo + wm_bins::start_of(pol, incl - nd);
         s_pre[lane + 1] = ninc;
       }
       if (lane == 0) {
