@@ -1,4 +1,4 @@
-"""C5 at its full size: the Weibel pair plasma on 8192 x 8192 cells with 32
+"""The largest Weibel workload at its full size: the pair plasma on 8192 x 8192 cells with 32
 particles per cell and species (4.3e9 particles), in the FP32 form with
 flexible CG and the learned preconditioner, on one GPU.  Rows of 1.4 times
 their mean occupancy and mover pools of 32768 records per row keep the

@@ -617,9 +617,11 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_HIP(hipMemsetAsync(ctx->sfill, 0, sizeof(int) * g.ny, ctx->stream));
     WM_TRY(dmalloc(ctx, &ctx->rowpend, g.ny));
     WM_TRY(dmalloc(ctx, &ctx->fstat, WM_NSTAT));
-    // the re-layout's lists and workspace: relay_nwb block workgroups of
-    // relay_wsz slots (four times a block's mean span), or relay_nws (256:
-    // one per CU) segment workgroups of a segment's slots
+    // the re-layout's lists and workspace: relay_nwb block workgroups (2048:
+    // eight per CU; a step at 8192 x 8192 cells, 32 per cell and species,
+    // re-lays about 270k blocks) of relay_wsz
+    // slots (four times a block's mean span), or relay_nws (256: one per CU)
+    // segment workgroups of a segment's slots
     const size_t nblk = (size_t)nseg * wm_cdiv(g.nx, WM_RB);
     WM_TRY(dmalloc(ctx, &ctx->relay_ctl, 4));
     WM_TRY(dmalloc(ctx, &ctx->relay_blocks, nblk));
@@ -627,7 +629,7 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(dmalloc(ctx, &ctx->relay_segs, nseg));
     WM_TRY(dmalloc(ctx, &ctx->segflag, nseg));
     ctx->relay_wsz = std::min(ctx->cap, std::max(1024, 4 * WM_RB * wm_cdiv(ctx->cap, g.nx) + 63) / 64 * 64);
-    ctx->relay_nwb = (int)std::min<size_t>(512, nblk);
+    ctx->relay_nwb = (int)std::min<size_t>(2048, nblk);
     ctx->relay_nws = std::min(256, nseg);
     const size_t wslots = std::max((size_t)ctx->relay_nwb * ctx->relay_wsz,
                                    (size_t)ctx->relay_nws * ctx->cap);

@@ -42,11 +42,20 @@ WM_MIG_FN int sum_mv(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) 
   const uint32_t v = j < 4 ? (j < 2 ? w0 : w1) : (j < 6 ? w2 : w3);
   return (int)((v >> (16 * (j & 1))) & 0xFFFFu);
 }
-// Slot of bin d's h-th hole: from its summary for the first WM_SUM_MV, else
-// from the record.
-WM_MIG_FN int hole_slot(const Movers &M, size_t r0, uint4 l, int h) {
-  return h < WM_SUM_MV ? sum_mv(l.x, l.y, l.z, l.w, h) & ((1 << WM_SLOT_BITS) - 1)
-                       : M.rec[r0 + h].sd & 0xFFFFFF;
+// Bin d's holes (its movers' slots, ascending): the first WM_SUM_MV from its
+// summary (words l), the next mc from the group's LDS cache hc (read from
+// the records once, by the whole group), the rest from the records.
+struct Holes {
+  const Movers *M;
+  size_t r0;                    // d's first record in its row's pool
+  uint4 l;
+  const int *hc;
+  int mc;
+};
+WM_MIG_FN int hole_slot(const Holes &H, int h) {
+  if (h < WM_SUM_MV) return sum_mv(H.l.x, H.l.y, H.l.z, H.l.w, h) & ((1 << WM_SLOT_BITS) - 1);
+  if (h - WM_SUM_MV < H.mc) return H.hc[h - WM_SUM_MV];
+  return H.M->rec[H.r0 + h].sd & 0xFFFFFF;
 }
 
 // The compaction's running state: next hole to fill (q), end of the holes
@@ -62,17 +71,17 @@ struct Move {                    // one compaction move: particle at src -> slot
 };
 // Plans the next move (skipping holes above the new count) and issues its
 // source's loads.
-WM_MIG_FN void plan_move(Move &v, MoveState &st, bool compact, const Movers &M, size_t r0, uint4 ol,
-                         int m, int n, size_t base, const Bins32 &A, const int64_t *ids) {
+WM_MIG_FN void plan_move(Move &v, MoveState &st, bool compact, const Holes &H, int m, int n,
+                         size_t base, const Bins32 &A, const int64_t *ids) {
   v.ok = false;
   v.dst = v.src = 0;
   if (compact) {
-    while (st.from < n && st.h < m && hole_slot(M, r0, ol, st.h) == st.from) {
+    while (st.from < n && st.h < m && hole_slot(H, st.h) == st.from) {
       ++st.h;
       ++st.from;
     }
     if (st.from < n && st.q < st.hb) {
-      v.dst = hole_slot(M, r0, ol, st.q);
+      v.dst = hole_slot(H, st.q);
       v.src = st.from;
       v.ok = true;
       ++st.q;
@@ -141,9 +150,11 @@ WM_MIG_FN size_t source_bin(const Mig &G, int i, int jl, int sp, int k, int &si,
 // record offsets and the first WM_SUM_MV holes; only a bin with more movers
 // reads the rest of its holes from its records.
 constexpr int DL = 8;
+// hc: the group's LDS cache of hcn holes past the summary's (nullptr: none).
 // Returns d's new count to lane 0 of the group (-1 to the others).
 WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_t *ids, int *derr,
-                          size_t d, int i, int jl, int sp, int k, size_t base, int capd) {
+                          size_t d, int i, int jl, int sp, int k, size_t base, int capd,
+                          int *hc, int hcn) {
   int si, sjl;
   const size_t src = source_bin(G, i, jl, sp, k, si, sjl);
   const uint4 *own = (const uint4 *)(G.sum + d), *from = (const uint4 *)(G.sum + src);
@@ -166,6 +177,24 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   }
   const int total = __shfl(inc, DL - 1, DL);
   const int n1 = n - m, nnew = n1 + total;
+  // a bin with more holes than its summary lists: the group reads the next
+  // ones from its records into the LDS cache, four per lane in flight
+  const int mc = hc ? min(max(m - WM_SUM_MV, 0), hcn) : 0;
+  for (int e0 = 0; e0 < mc; e0 += 4 * DL) {
+    int hv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = e0 + u * DL + k;
+      hv[u] = e < mc ? (int)(M.rec[r0 + WM_SUM_MV + e].sd & 0xFFFFFF) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = e0 + u * DL + k;
+      if (e < mc) hc[e] = hv[u];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();                 // the group reads what its lanes wrote
+  const Holes H{&M, r0, ol, hc, mc};
   // Lane 0's compaction (fewer arrivals than holes): the particles above the
   // new count that are not holes fill the holes left below it, in order.
   // Its sources lie at or above the new count and the arrivals' holes below
@@ -175,14 +204,14 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   const bool compact = k == 0 && total < m;
   MoveState st{total, total, 0, nnew};
   if (compact) {
-    while (st.hb < m && hole_slot(M, r0, ol, st.hb) < nnew) ++st.hb;   // holes below the new count
+    while (st.hb < m && hole_slot(H, st.hb) < nnew) ++st.hb;   // holes below the new count
     st.h = st.hb;
   }
   Move v0, v1, v2, v3;
-  plan_move(v0, st, compact, M, r0, ol, m, n, base, A, ids);
-  plan_move(v1, st, compact, M, r0, ol, m, n, base, A, ids);
-  plan_move(v2, st, compact, M, r0, ol, m, n, base, A, ids);
-  plan_move(v3, st, compact, M, r0, ol, m, n, base, A, ids);
+  plan_move(v0, st, compact, H, m, n, base, A, ids);
+  plan_move(v1, st, compact, H, m, n, base, A, ids);
+  plan_move(v2, st, compact, H, m, n, base, A, ids);
+  plan_move(v3, st, compact, H, m, n, base, A, ids);
   // direction k's records of the source: perm[fs + pre + r], r < c, index
   // records fs + perm[..]; pre = the source's movers of directions before k
   int pre = 0;
@@ -228,7 +257,7 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
       q = M.perm[f + pk + r];
     }
     w.rec = M.rec[f + q];
-    w.slot = a < m ? hole_slot(M, r0, ol, a) : n + (a - m);
+    w.slot = a < m ? hole_slot(H, a) : n + (a - m);
   };
   // every lane of the group runs the same number of rounds (the shuffles
   // need the whole group)
@@ -241,21 +270,19 @@ WM_MIG_FN int migrate_bin(const Mig &G, const Movers &M, const Bins32 &A, int64_
   store_move(v1, A, ids, base);
   store_move(v2, A, ids, base);
   store_move(v3, A, ids, base);
-  int from_slot = st.from, h = st.h, q = st.q;
-  const int hb = st.hb;
   if (k != 0) return -1;
-  if (compact) {
-    for (; from_slot < n && q < hb; ++from_slot) {
-      if (h < m && hole_slot(M, r0, ol, h) == from_slot) {     // a hole above the new count
-        ++h;
-        continue;
-      }
-      const size_t dst = base + hole_slot(M, r0, ol, q), sf = base + from_slot;
-      A.q[dst] = A.q[sf];
-      A.uz[dst] = A.uz[sf];
-      if (ids) ids[dst] = ids[sf];
-      ++q;
-    }
+  // the rest of the compaction, four moves at a time (a bin that lost many
+  // particles and gained few)
+  while (compact && st.from < n && st.q < st.hb) {
+    Move u0, u1, u2, u3;
+    plan_move(u0, st, compact, H, m, n, base, A, ids);
+    plan_move(u1, st, compact, H, m, n, base, A, ids);
+    plan_move(u2, st, compact, H, m, n, base, A, ids);
+    plan_move(u3, st, compact, H, m, n, base, A, ids);
+    store_move(u0, A, ids, base);
+    store_move(u1, A, ids, base);
+    store_move(u2, A, ids, base);
+    store_move(u3, A, ids, base);
   }
   int cn = n1 + total;
   if (cn > capd) {

@@ -266,6 +266,7 @@ constexpr int MB = 16;                // entries per spill block
 constexpr int CAPG = 60;              // entries of one cell's list in LDS
 constexpr int NPOOL = CHUNK * CAPG;   // movers of one workgroup listed in LDS
 constexpr int MAXB = 64;              // spill blocks of one cell's list
+constexpr int HCN = NPOOL * 4 / (NT / 8);   // migration prologue: holes cached per bin group
 
 // The push kernel's LDS (file scope, so device functions address it directly).
 __shared__ float T[6 * 3 * TW];                   // cell-centred fields of the chunk
@@ -631,7 +632,9 @@ __global__ __launch_bounds__(256, 4) void k_push_deposit32(
       wm_mig::bin_span(G, seg, i0 + c, base, capd);
       b0 = (int)(base - seg * (size_t)P.cap);
       const size_t d = seg * P.nx + i0 + c;
-      cn = pend ? wm_mig::migrate_bin(G, Mp, A, ids, derr, d, i0 + c, jl, s, k, base, capd)
+      // (the movers' offsets pool, unused until the windows, caches holes)
+      int *hc = (int *)xpl + (threadIdx.x / wm_mig::DL) * HCN;
+      cn = pend ? wm_mig::migrate_bin(G, Mp, A, ids, derr, d, i0 + c, jl, s, k, base, capd, hc, HCN)
                 : min(capd, G.cnt[d]);
     }
     if (k == 0) {

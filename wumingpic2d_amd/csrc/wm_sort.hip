@@ -549,8 +549,12 @@ __global__ void k_relayout_done(Relay R, int *__restrict__ stats) {
 // the last push's summaries and records, when no push follows to do it in
 // its prologue (state download, energy sums); rows laid out anew since that
 // push hold their movers already.
-__global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids,
-                          const int *__restrict__ rowpend, int *derr) {
+constexpr int MIG_HC = 120;   // holes cached in LDS per bin group
+__global__ __launch_bounds__(256) void k_migrate(Mig G, Movers M, Bins32 A,
+                                                 int64_t *__restrict__ ids,
+                                                 const int *__restrict__ rowpend, int *derr) {
+  __shared__ int hcs[256 / DL * MIG_HC];
+  int *hc = hcs + (threadIdx.x / DL) * MIG_HC;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t / DL < G.nbin;
        t += (size_t)gridDim.x * blockDim.x) {
     // cell (i, jl) and species sp of the group's bin (bins fit 32 bits)
@@ -564,7 +568,7 @@ __global__ void k_migrate(Mig G, Movers M, Bins32 A, int64_t *__restrict__ ids,
     size_t base;
     int capd;
     bin_span(G, seg, i, base, capd);
-    (void)migrate_bin(G, M, A, ids, derr, d, i, jl, sp, (int)(t % DL), base, capd);
+    (void)migrate_bin(G, M, A, ids, derr, d, i, jl, sp, (int)(t % DL), base, capd, hc, MIG_HC);
   }
 }
 
