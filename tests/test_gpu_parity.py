@@ -159,6 +159,40 @@ def test_resident_steps_match_oracle(nx, ny, ppc, seed, steps):
     assert np.allclose(e1, e2, rtol=1e-11, atol=1e-18)
 
 
+def test_resident_row_losing_half_its_particles_matches_oracle():
+    """Every particle in the left half of row 3's cells moves up a row in one
+    push (y near the top of the cell, u_y = 0.9 c): the row keeps its right
+    half and loses more than 8192 particles net, so the top slots' particles
+    that fill its holes (bp.f90:193-236) are more than the fused pass 2 +
+    sort lists in LDS and are found through the hole list instead.  The
+    state after two resident steps matches the oracle's, order and IDs bit
+    for bit."""
+    p, up, np2, cc, uf = state(512, 6, 40, 21, vth=0.05)
+    jl, nxgs, nys = 3, p["nxgs"], p["nys"]
+    for s in range(p["nsp"]):
+        n = np2[jl, s]
+        left = up[0, :n, jl, s] - nxgs < 256
+        up[1, :n, jl, s][left] = nys + jl + 0.95
+        up[3, :n, jl, s][left] = 0.9
+    ctx = make_ctx(p, mcap=16384)
+    ctx.state_upload(up, np2, cc, uf)
+    orc = Oracle(p)
+    gp = up.copy(order="F")
+    for it in range(2):
+        orc.step(up, gp, uf, cc, np2)
+        ctx.step(1)
+    gup, gnp2 = np.zeros_like(up), np.zeros_like(np2)
+    gcc, guf = np.zeros_like(cc), np.zeros_like(uf)
+    ctx.state_download(up=gup, np2=gnp2, cumcnt=gcc, uf=guf)
+    assert np2[jl, 0] < 20480 - 8192
+    assert np.array_equal(gnp2, np2)
+    assert np.array_equal(gcc, cc)
+    for s in range(p["nsp"]):
+        for j in range(p["ny"]):
+            assert np.array_equal(bits(up[5, :np2[j, s], j, s]), bits(gup[5, :np2[j, s], j, s]))
+            assert np.allclose(active(gup, np2, j, s), active(up, np2, j, s), rtol=RTOL, atol=1e-13)
+
+
 def test_resident_fields_per_component_and_gauss_law():
     """The north star's FP64 bars over the first 10 steps: after every step
     each of the six field components within 1e-12 of the oracle's, relative

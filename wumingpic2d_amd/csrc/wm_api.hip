@@ -664,6 +664,11 @@ int wm_alloc_device(wm_ctx *ctx) {
     WM_TRY(alloc_bank(ctx, ctx->mv_dn, ms));
     WM_TRY(dmalloc(ctx, &ctx->mvid_up, ms));
     WM_TRY(dmalloc(ctx, &ctx->mvid_dn, ms));
+    ctx->ytiles = wm_cdiv(ctx->cap, 64);
+    WM_TRY(dmalloc(ctx, &ctx->ytmask, (size_t)nseg * ctx->ytiles));
+    WM_TRY(dmalloc(ctx, &ctx->ytpre, (size_t)nseg * ctx->ytiles));
+    WM_TRY(dmalloc(ctx, &ctx->yhist, (size_t)nseg * g.nx));
+    WM_HIP(hipMemsetAsync(ctx->yhist, 0, sizeof(int) * (size_t)nseg * g.nx, ctx->stream));
   }
 
   FieldGeom &fg = ctx->fg;
@@ -788,7 +793,8 @@ int wm_destroy(wm_ctx *ctx) {
                   ctx->bstart, ctx->pcnt, ctx->mfill[0], ctx->mfill[1], ctx->rowpend,
                   ctx->fstat, ctx->relay_ctl, ctx->relay_blocks, ctx->blkflag, ctx->relay_segs,
                   ctx->segflag, ctx->rws.q, ctx->rws.uz, ctx->rws_id, ctx->rws_bs,
-                  ctx->spill_l, ctx->spill_p, ctx->spill_q, ctx->sfill};
+                  ctx->spill_l, ctx->spill_p, ctx->spill_q, ctx->sfill, ctx->ytmask,
+                  ctx->ytpre, ctx->yhist};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   wm_pc_free(ctx);
@@ -1348,9 +1354,8 @@ int wm_step(wm_ctx *ctx, int nsteps) {
         // reconnection's walls (proj/reconnection/app.f90:104), or none
         // (the shock's edges came with the push, proj/shock/app.f90:113)
         const int xmode = ctx->bcf == WM_BC_PERIODIC ? 1 : ctx->bcf == WM_BC_RECONNECTION ? 2 : 0;
-        WM_TRY(wm_k_particle_y(ctx, 1, xmode));
-        WM_HIP(hipEventRecord(e[6], ctx->stream));
-        WM_TRY(wm_k_sort(ctx, 1, 0));             // app.f90:106
+        // app.f90:105-106: pass 1, then pass 2 fused with the sort
+        WM_TRY(wm_k_particle_y_sort(ctx, 1, 0, xmode, e[6]));
       }
       WM_HIP(hipEventRecord(e[7], ctx->stream));
     }

@@ -22,6 +22,7 @@
 //   shrinks by m = H-A to L = np2-m, and the k-th remaining hole below L
 //   takes the k-th non-hole slot of [L, np2) counted from the top.
 #include "wm_internal.h"
+#include "wm_bins.h"
 
 namespace {
 
@@ -30,6 +31,14 @@ struct BP {
   double delx, spanx, spany;
   int xs, xe;          // active columns nxs, nxe (the wall families)
   double wl, wr;       // mirror points 2 (nxs+1) delx and 2 (nxe-1) delx
+};
+
+// The outputs pass 1 gives the fused pass 2 + sort (all null otherwise).
+struct YFuse {
+  unsigned long long *tmask;    // per tile of 64 slots: the lanes that leave
+  int *tpre;                    // per tile: the row's holes before it
+  int ntiles;                   // tiles per segment
+  int *hist;                    // [seg][nx]: cells after pass 2 (added to, zero before)
 };
 
 __device__ inline double pred_d(double s) {
@@ -59,20 +68,25 @@ __device__ inline int wrap_x(double &x, const BP &P) {
 // boundary_reconnection.f90:76-88): a particle in a cell below nxs+1 is
 // mirrored about (nxs+1) delx, one in a cell at or above nxe-1 about
 // (nxe-1) delx, and its momentum reversed.  Returns the new cell.
-__device__ inline int wall_x_at(const Rec64 &R, size_t p, double x, const BP &P) {
+__device__ inline int wall_x_at(const Rec64 &R, size_t p, double x, const BP &P, double &xf) {
   const int ipos = (int)(x / P.delx);
   double w;
+  xf = x;
   if (ipos < P.xs + 1) w = P.wl;
   else if (ipos >= P.xe - 1) w = P.wr;
   else return ipos;
   const double xn = w - x;
+  xf = xn;
   R.x[p] = xn;
   R.ux[p] = -R.ux[p];
   R.uy[p] = -R.uy[p];
   R.uz[p] = -R.uz[p];
   return (int)(xn / P.delx);
 }
-__device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) { return wall_x_at(R, p, R.x[p], P); }
+__device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) {
+  double xf;
+  return wall_x_at(R, p, R.x[p], P, xf);
+}
 
 __global__ void k_wall_x(BP P, const int *__restrict__ np2, Rec64 B, int *derr) {
   const int seg = blockIdx.y;
@@ -155,14 +169,16 @@ __device__ inline int lanes_below(unsigned long long m) {
 // Taking the loaded values lets a caller issue several particles' loads
 // before the first store of a wrapped x.
 __device__ inline int row_change(const Rec64 &R, size_t p, double x, double y, int j, int xmode,
-                                 const BP &P, int &bad) {
+                                 const BP &P, int &bad, double &xf) {
+  xf = x;                                          // the particle's x once particle_x is done
   if (xmode == 2) {
-    const int inew = wall_x_at(R, p, x, P);
+    const int inew = wall_x_at(R, p, x, P, xf);
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   } else if (xmode == 1) {
     double xv = x;
     wrap_x(xv, P);
     if (xv != x) R.x[p] = xv;
+    xf = xv;
     const int inew = (int)(xv / P.delx);
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   }
@@ -203,7 +219,7 @@ __global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
                                                      int *__restrict__ nup,
                                                      int *__restrict__ ndn, R mu,
                                                      int64_t *mid_u, R md, int64_t *mid_d,
-                                                     int *derr) {
+                                                     YFuse F, int *derr) {
   __shared__ unsigned long long smu[YST], smd[YST];   // per tile: lanes moving up / down
   __shared__ int scu[YST], scd[YST];                   // movers up / down before the tile
   __shared__ int wtu[YNW], wtd[YNW];
@@ -232,12 +248,34 @@ __global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
         }
       }
       int dir[YT];
+      double xf[YT];
 #pragma unroll
       for (int b = 0; b < YT; ++b) {
         const int k = k0 + (t0 + b * YNW) * YW + lane;
+        xf[b] = 0.0;
         dir[b] = t0 + b * YNW < ntile && k < n
-                     ? row_change(B, base + k, xv[b], yv[b], j, xmode, P, bad)
+                     ? row_change(B, base + k, xv[b], yv[b], j, xmode, P, bad, xf[b])
                      : 0;
+      }
+      if (F.hist) {
+        // the histogram of the cells each row holds after pass 2 (the sort's
+        // count, int(x) as sort.f90:60): a staying particle's in this row's,
+        // one atomic per distinct cell of the wavefront; a leaver's in the
+        // receiving row's
+#pragma unroll
+        for (int b = 0; b < YT; ++b) {
+          const int k = k0 + (t0 + b * YNW) * YW + lane;
+          const bool in = t0 + b * YNW < ntile && k < n;
+          const int key = (int)xf[b] - P.nxgs;
+          const bool ok = in && key >= 0 && key < P.nx;
+          const bool stay = ok && dir[b] == 0;
+          const wm_bins::WaveMatch mm = wm_bins::wave_match(key, stay);
+          if (stay && mm.leader) atomicAdd(F.hist + (size_t)seg * P.nx + key, mm.count);
+          if (ok && (dir[b] == 1 || dir[b] == -1)) {
+            const int s = seg / P.ny, dj = wm_wrap(jl + dir[b], P.ny);
+            atomicAdd(F.hist + ((size_t)s * P.ny + dj) * P.nx + key, 1);
+          }
+        }
       }
 #pragma unroll
       for (int b = 0; b < YT; ++b) {
@@ -282,6 +320,17 @@ __global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
       scd[t2] = ed;
       scu[t2 + 1] = eu + a0;
       scd[t2 + 1] = ed + b0;
+      if (F.tmask) {                                   // each tile's leavers and the holes before it
+        const size_t tb = (size_t)seg * F.ntiles + k0 / YW;
+        if (t2 < ntile) {
+          F.tmask[tb + t2] = smu[t2] | smd[t2];
+          F.tpre[tb + t2] = eu + ed;
+        }
+        if (t2 + 1 < ntile) {
+          F.tmask[tb + t2 + 1] = smu[t2 + 1] | smd[t2 + 1];
+          F.tpre[tb + t2 + 1] = eu + a0 + ed + b0;
+        }
+      }
     }
     __syncthreads();
     // place each tile's movers
@@ -416,6 +465,237 @@ __global__ __launch_bounds__(Y2T) void k_particle_y2(BP P, int *__restrict__ np2
   if (threadIdx.x == 0) np2[seg] = newn;
 }
 
+// pass 2 fused with the sort (the resident step; sort.f90:36-82 after
+// bp.f90:143-236).  The row pass 2 would leave -- hole h filled by arrival h
+// while there are arrivals, surplus arrivals appended, or with fewer
+// arrivals the top m = H - A slots' non-holes moved, from the top, into the
+// holes the arrivals leave -- is never written: slot k of it is read where
+// it comes from (the bank, an arrival list, or the slot the compaction
+// moves), through pass 1's tile masks and hole prefixes, and sorted straight
+// into the other bank.  The counts come from pass 1's histogram (cleared
+// here for the next step); the scatter is k_sort's: rounds of FW tiles, one
+// per wavefront, ranks among equal keys by a ballot loop, each wavefront's
+// (key, count) list in LDS for the lanes of the higher tiles.
+constexpr int FW = 8;      // wavefronts per segment
+constexpr int FTH = FW * YW;
+
+struct YSrc {              // the lists of one destination row's arrivals and its bank
+  int nF, nS, A, H, N, newn, m;
+  size_t base, Fq, Sq;     // the arrival lists' first records
+  bool first_row;
+};
+
+template <class R>
+__global__ __launch_bounds__(FTH) void k_y2_sort(BP P, int *__restrict__ np2, R B,
+                                                 const int64_t *__restrict__ id,
+                                                 const int *__restrict__ hole,
+                                                 const int *__restrict__ nhole,
+                                                 const int *__restrict__ nup,
+                                                 const int *__restrict__ ndn, R mu,
+                                                 const int64_t *mid_u, R md, const int64_t *mid_d,
+                                                 YFuse F, R dst, int64_t *__restrict__ idd,
+                                                 int *__restrict__ cumcnt, int *derr) {
+  extern __shared__ int cnt[];                        // nx running offsets
+  __shared__ int lkey[FW][YW], lcnt[FW][YW], nl[FW], wsum[FW];
+  __shared__ int csrc[Y2TAIL];                        // the compaction's sources, in hole order
+  using V = typename R::Val;
+  const int seg = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int s = seg / P.ny, jl = seg % P.ny;
+  const int below = s * P.ny + (jl == 0 ? P.ny - 1 : jl - 1);
+  const int above = s * P.ny + (jl == P.ny - 1 ? 0 : jl + 1);
+  // arrival lists, in the order the self-SENDRECVs append them (:174-189)
+  const bool first_row = (jl == 0);
+  const R FB = first_row ? md : mu, SB = first_row ? mu : md;
+  const int64_t *Fid = first_row ? mid_d : mid_u, *Sid = first_row ? mid_u : mid_d;
+  const int Fseg = first_row ? above : below, Sseg = first_row ? below : above;
+  const int nF = first_row ? ndn[above] : nup[below];
+  const int nS = first_row ? nup[below] : ndn[above];
+  const int A = nF + nS, H = nhole[seg], N = np2[seg];
+  const int newn = N + A - H, m = H - A;
+  const size_t base = (size_t)seg * P.cap;
+  const int *hl = hole + 2 * (size_t)seg * P.mcap;
+  const unsigned long long *tm = F.tmask + (size_t)seg * F.ntiles;
+  const int *tp = F.tpre + (size_t)seg * F.ntiles;
+  int *hist = F.hist + (size_t)seg * P.nx;
+  if (newn > P.cap) {
+    if (threadIdx.x == 0) atomicOr(derr, DERR_NP_OVER);
+    return;
+  }
+  // is slot k (< N) a hole of pass 1, and its rank among the holes
+  auto hole_rank = [&](int k, int &h) {
+    const unsigned long long mk = tm[k >> 6];
+    const int b = k & 63;
+    h = tp[k >> 6] + __popcll(mk & ((1ULL << b) - 1ULL));
+    return (bool)((mk >> b) & 1ULL);
+  };
+  // the compaction's sources: the non-holes of [newn, N) from the top, in
+  // order, for the holes of rank A, A + 1, ... below newn
+  const bool in_lds = m <= Y2TAIL;
+  if (m > 0 && in_lds) {
+    int run = 0;
+    for (int j0 = 0; j0 < m; j0 += FTH) {
+      const int j = j0 + (int)threadIdx.x;
+      int h;
+      const bool nh = j < m && !hole_rank(N - 1 - j, h);
+      const unsigned long long bm = __ballot(nh);
+      if (lane == 0) wsum[w] = __popcll(bm);
+      __syncthreads();
+      int pre = 0, tot = 0;
+#pragma unroll
+      for (int v = 0; v < FW; ++v) {
+        pre += v < w ? wsum[v] : 0;
+        tot += wsum[v];
+      }
+      if (nh) csrc[run + pre + lanes_below(bm)] = N - 1 - j;
+      __syncthreads();
+      run += tot;
+    }
+  }
+  // the r-th source without the LDS list (more than Y2TAIL slots vacated):
+  // the smallest j with r + 1 non-holes among slots N-1 .. N-1-j
+  auto source_far = [&](int r) {
+    int lo = r, hi = m - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1, x = N - 1 - mid;
+      int a = 0, b = H;                              // holes >= x: H - (first hole >= x)
+      while (a < b) {
+        const int c = (a + b) >> 1;
+        if (hl[c] < x) a = c + 1;
+        else b = c;
+      }
+      if (mid + 1 - (H - a) >= r + 1) hi = mid;
+      else lo = mid + 1;
+    }
+    return N - 1 - lo;
+  };
+  // where slot k of the row after pass 2 comes from: the bank (returns the
+  // slot) or an arrival list (returns -1 - its record, F or S)
+  auto source = [&](int k, bool &fl, size_t &q) {
+    int h = 0;
+    int a = -1;
+    if (k >= N) {
+      a = H + (k - N);
+    } else if (hole_rank(k, h)) {
+      if (h < A) a = h;
+      else if (m > 0 && h - A < m) return in_lds ? csrc[h - A] : source_far(h - A);
+      else return -2;                                // inconsistent lists (a mover overflow)
+    } else {
+      return k;
+    }
+    fl = a < nF;
+    q = fl ? (size_t)Fseg * P.mcap + a : (size_t)Sseg * P.mcap + (a - nF);
+    return -1;
+  };
+  auto fetch = [&](int k, V &v, int64_t &vid) {
+    bool fl = false;
+    size_t q = 0;
+    int sl = source(k, fl, q);
+    if (sl == -2 || sl >= N) {                         // (only after a reported overflow)
+      atomicOr(derr, DERR_MOVER_OVER);
+      sl = 0;
+    }
+    if (sl >= 0) {
+      v = B.load(base + sl);
+      vid = id[base + sl];
+    } else {
+      v = (fl ? FB : SB).load(q);
+      vid = (fl ? Fid : Sid)[q];
+    }
+  };
+  // exclusive prefix of the histogram -> cumcnt and the running offsets;
+  // the histogram is left zero for the next step
+  {
+    const int per = (P.nx + FTH - 1) / FTH;
+    const int lo = min(P.nx, (int)threadIdx.x * per), hi = min(P.nx, lo + per);
+    int sum = 0;
+    for (int i = lo; i < hi; ++i) sum += hist[i];
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < YW; off <<= 1) {
+      const int v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
+    }
+    if (lane == YW - 1) wsum[w] = incl;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int v = 0; v < FW; ++v) {
+      pre += v < w ? wsum[v] : 0;
+      tot += wsum[v];
+    }
+    int run = pre + incl - sum;
+    int *cc = cumcnt + (size_t)seg * (P.nx + 1);
+    for (int i = lo; i < hi; ++i) {
+      const int c = hist[i];
+      hist[i] = 0;
+      cc[i] = run;
+      cnt[i] = run;
+      run += c;
+    }
+    if (threadIdx.x == 0) {
+      cc[P.nx] = tot;
+      if (tot != newn) atomicOr(derr, DERR_RANGE);   // a particle outside the row's cells
+    }
+  }
+  __syncthreads();
+  // stable scatter (sort.f90:71-75) of slots 0 .. newn-1 of the row after
+  // pass 2, rounds of FW tiles, the next round's tile loading meanwhile
+  const int nt = (newn + YW - 1) / YW;
+  V v{}, nv{};
+  int64_t vid = 0, nvid = 0;
+  int key = -1, nkey = -1;
+  {
+    const int k = w * YW + lane;
+    if (k < newn) {
+      fetch(k, v, vid);
+      key = B.key(v, P.nxgs);
+    }
+  }
+  for (int r0 = 0; r0 < nt; r0 += FW) {
+    {
+      const int k = (r0 + FW + w) * YW + lane;
+      nkey = -1;
+      if (k < newn) {
+        fetch(k, nv, nvid);
+        nkey = B.key(nv, P.nxgs);
+      }
+    }
+    const bool act = key >= 0 && key < P.nx;
+    const wm_bins::WaveMatch mt = wm_bins::wave_match(key, act);
+    const unsigned long long lm = __ballot(act && mt.leader);
+    if (act && mt.leader) {
+      const int li = __popcll(lm & ((1ULL << lane) - 1ULL));
+      lkey[w][li] = key;
+      lcnt[w][li] = mt.count;
+    }
+    if (lane == 0) nl[w] = __popcll(lm);
+    __syncthreads();
+    if (act) {
+      int prior = 0;                                   // equal keys in the round's lower tiles
+      for (int u = 0; u < w; ++u) {
+        const int nu = nl[u];
+        for (int i = 0; i < nu; ++i) prior += lkey[u][i] == key ? lcnt[u][i] : 0;
+      }
+      const int dpos = cnt[key] + prior + mt.rank;
+      if (dpos < newn) {
+        const size_t d = base + dpos;
+        dst.store(d, v);
+        idd[d] = vid;
+      } else {
+        atomicOr(derr, DERR_RANGE);
+      }
+    }
+    __syncthreads();                                   // every lane has read the offsets and lists
+    if (act && mt.leader) atomicAdd(&cnt[key], mt.count);
+    v = nv;
+    vid = nvid;
+    key = nkey;
+  }
+  __syncthreads();                                     // every lane has read np2[seg]
+  if (threadIdx.x == 0) np2[seg] = newn;
+}
+
 BP make_bp(const wm_ctx *ctx) {
   const wm_grid &g = ctx->g;
   BP P;
@@ -465,6 +745,30 @@ int wm_k_injection(wm_ctx *ctx, int bank, double u0) {
   return WM_OK;
 }
 
+int wm_k_particle_y_sort(wm_ctx *ctx, int src, int dst, int xmode, hipEvent_t mid) {
+  BP P = make_bp(ctx);
+  const int nseg = ctx->g.nsp * ctx->g.ny;
+  const size_t lds = sizeof(int) * (size_t)P.nx;
+  if (ctx->form == WM_FORM_FP32 || lds > 65536) {
+    wm_set_error(ctx, WM_ERR_ARG, "particle_y + sort: FP64 rows of at most 16384 cells");
+    return WM_ERR_ARG;
+  }
+  int64_t *id = ctx->id[ctx->idcur];
+  const YFuse F{ctx->ytmask, ctx->ytpre, ctx->ytiles, ctx->yhist};
+  const Rec64 &B = ctx->bank[src];
+  hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YTH), 0, ctx->stream, P, xmode,
+                     ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
+                     ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, F, ctx->derr);
+  if (mid) WM_HIP(hipEventRecord(mid, ctx->stream));
+  hipLaunchKernelGGL(k_y2_sort<Rec64>, dim3(nseg), dim3(FTH), lds, ctx->stream, P, ctx->np2, B,
+                     id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up, ctx->mvid_up,
+                     ctx->mv_dn, ctx->mvid_dn, F, ctx->b64(dst), ctx->id[1 - ctx->idcur],
+                     ctx->cumcnt, ctx->derr);
+  WM_HIP(hipGetLastError());
+  ctx->idcur = 1 - ctx->idcur;
+  return WM_OK;
+}
+
 int wm_k_particle_y(wm_ctx *ctx, int bank, int xmode) {
   BP P = make_bp(ctx);
   const int nseg = ctx->g.nsp * ctx->g.ny;
@@ -478,7 +782,7 @@ int wm_k_particle_y(wm_ctx *ctx, int bank, int xmode) {
     const Rec64 &B = ctx->bank[bank];
     hipLaunchKernelGGL(k_particle_y1<Rec64>, dim3(nseg), dim3(YTH), 0, ctx->stream, P, xmode,
                        ctx->np2, B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
-                       ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);
+                       ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, YFuse{}, ctx->derr);
     hipLaunchKernelGGL(k_particle_y2<Rec64>, dim3(nseg), dim3(Y2T), 0, ctx->stream, P, ctx->np2,
                        B, id, ctx->hole, ctx->nhole, ctx->nup, ctx->ndn, ctx->mv_up,
                        ctx->mvid_up, ctx->mv_dn, ctx->mvid_dn, ctx->derr);

@@ -173,6 +173,13 @@ struct wm_ctx {
   int *nhole = nullptr, *nup = nullptr, *ndn = nullptr;  // [nseg]
   PBank mv_up{}, mv_dn{};
   int64_t *mvid_up = nullptr, *mvid_dn = nullptr;
+  // the resident step's fused pass 2 + sort: per tile of 64 slots of a row
+  // segment, pass 1's leaver mask and the holes before the tile; per
+  // segment, the histogram of the cells the row holds after pass 2
+  unsigned long long *ytmask = nullptr;   // [nseg*ytiles]
+  int *ytpre = nullptr;                   // [nseg*ytiles]
+  int *yhist = nullptr;                   // [nseg*nx], zero between steps
+  int ytiles = 0;
 
   // fields
   FieldGeom fg{};
@@ -294,6 +301,10 @@ int wm_k_energy(wm_ctx *ctx, int bank, double *out_host);
 int wm_k_particle_x(wm_ctx *ctx, int bank);
 // xmode: 0 y only, 1 with the periodic x wrap fused, 2 with the walls' mirrors fused
 int wm_k_particle_y(wm_ctx *ctx, int bank, int xmode);
+// The resident step's particle_y and sort: pass 1, then pass 2 fused with
+// the sort, from bank src into bank dst (the ID arrays follow); mid is
+// recorded between the two
+int wm_k_particle_y_sort(wm_ctx *ctx, int src, int dst, int xmode, hipEvent_t mid);
 int wm_k_injection(wm_ctx *ctx, int bank, double u0);   // boundary_shock__injection
 double wm_injection_xend(const wm_ctx *ctx, double u0);
 // Sort (wm_sort.hip)
