@@ -235,18 +235,22 @@ __global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
   for (int k0 = 0; k0 < n; k0 += YST * YW) {
     const int ntile = min(YST, (n - k0 + YW - 1) / YW);
     // classify (the periodic x wrap or the walls' mirrors are applied here)
-    for (int t0 = w; t0 < ntile; t0 += YNW * YT) {
-      // every tile's x and y in flight before the first wrapped x is stored
-      double xv[YT], yv[YT];
+    // the next YT tiles' x and y are in flight while these are classified;
+    // indices are clamped into the segment (lanes past its end read its last
+    // particle and are not classified), so the loads are unconditional
+    double xv[YT], yv[YT];
+    auto fetch = [&](int t0, double *xs, double *ys) {
 #pragma unroll
       for (int b = 0; b < YT; ++b) {
-        const int k = k0 + (t0 + b * YNW) * YW + lane;
-        xv[b] = yv[b] = 0.0;
-        if (t0 + b * YNW < ntile && k < n) {
-          xv[b] = B.x[base + k];
-          yv[b] = B.y[base + k];
-        }
+        const int k = min(k0 + (t0 + b * YNW) * YW + lane, n - 1);
+        xs[b] = B.x[base + k];
+        ys[b] = B.y[base + k];
       }
+    };
+    fetch(w, xv, yv);
+    for (int t0 = w; t0 < ntile; t0 += YNW * YT) {
+      double xn[YT], yn[YT];
+      fetch(t0 + YNW * YT, xn, yn);
       int dir[YT];
       double xf[YT];
 #pragma unroll
@@ -286,6 +290,11 @@ __global__ __launch_bounds__(YTH) void k_particle_y1(BP P, int xmode,
           smu[t] = u;
           smd[t] = d;
         }
+      }
+#pragma unroll
+      for (int b = 0; b < YT; ++b) {
+        xv[b] = xn[b];
+        yv[b] = yn[b];
       }
     }
     __syncthreads();

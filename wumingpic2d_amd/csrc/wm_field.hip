@@ -883,10 +883,8 @@ int wm_k_energy(wm_ctx *ctx, int bank, double *out) {
   const int nseg = g.nsp * g.ny;
   const int nchunk = 8;
   double *part = ctx->diag;
-  if (ctx->form == WM_FORM_FP32) {
-    WM_TRY(wm_settle_movers(ctx));
+  if (ctx->form == WM_FORM_FP32)   // no settle: a pending mover's slot holds its pushed state
     WM_TRY(wm_k_energy_kin32(ctx, bank, part, nchunk));
-  }
   else
     hipLaunchKernelGGL(k_energy_kin, dim3(nchunk, nseg), dim3(256), 0, ctx->stream, g.ny,
                        ctx->cap, g.nsp, ctx->np2, ctx->bank[bank].ux, ctx->bank[bank].uy,

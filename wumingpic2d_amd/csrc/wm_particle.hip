@@ -78,7 +78,7 @@ __device__ inline void load_tmp_tile(double *T, const double *tmp, int TX,
   }
 }
 
-struct PushOut { double x, y, ux, uy, uz; };
+struct PushOut { double x, y, ux, uy, uz, ig; };   // ig = 1 / gamma of the new u
 
 __device__ inline PushOut boris(const double *T, int gcol, double x, double y,
                                 double ux, double uy, double uz, int i, int j,
@@ -117,6 +117,7 @@ __device__ inline PushOut boris(const double *T, int gcol, double x, double y,
   const double ig = 1.0 / sqrt(1.0 + (o.ux * o.ux + o.uy * o.uy + o.uz * o.uz) / (P.c * P.c));
   o.x = x + o.ux * P.delt * ig;
   o.y = y + o.uy * P.delt * ig;
+  o.ig = ig;
   return o;
 }
 
@@ -302,6 +303,16 @@ __device__ inline void crossing_pass(double *cb, int sub, unsigned long long mas
   }
 }
 
+// Element k of a row segment that starts at seg (the same in every lane):
+// a 32-bit byte offset from a scalar base, so that no per-lane 64-bit
+// address is kept per array.  A segment holds fewer than 2^29 doubles.
+__device__ inline double seg_ld(const double *seg, int k) {
+  return *(const double *)((const char *)seg + (unsigned)k * 8u);
+}
+__device__ inline void seg_st(double *seg, int k, double v) {
+  *(double *)((char *)seg + (unsigned)k * 8u) = v;
+}
+
 template <bool PUSH, bool DEPOSIT>
 __global__ __launch_bounds__(256, 3) void k_particle(
     PP P, const int *__restrict__ cumcnt,
@@ -345,51 +356,67 @@ __global__ __launch_bounds__(256, 3) void k_particle(
       const double txxx = fac1 * fac1;
       const double fac2 = q * P.delt / r;
       unsigned long long mask = 0ULL;
-      // the next particle's state is loaded while the current one is
-      // computed (one step of software pipelining per lane); windows of
-      // GROUP consecutive particles start at the GROUP boundary at or below
-      // the cell's first particle
-      const int ka = kb - kb % GROUP;
+      if (kb >= ke) continue;                  // no particle of species s here
+      // windows of 2 GROUP consecutive particles start at the 2 GROUP
+      // boundary at or below the cell's first particle, so that each array's
+      // load is one aligned 128-byte line for the group; lane sub takes the
+      // window's particles sub and GROUP + sub, in that order (the order of
+      // GROUP-particle windows, m counting them from ka)
+      const int ka = kb - kb % (2 * GROUP);
       // segment bases: the same for the whole workgroup, 32-bit indices below
       const double *Xa = xa + base, *Ya = ya + base;
       const double *Uxa = uxa + base, *Uya = uya + base, *Uza = uza + base;
-      int k = ka + sub;
-      double nx0 = 0, ny0 = 0, nux = 0, nuy = 0, nuz = 0;
-      if (k >= kb && k < ke) {
-        nx0 = Xa[k]; ny0 = Ya[k];
-        if (PUSH) { nux = Uxa[k]; nuy = Uya[k]; nuz = Uza[k]; }
-      }
-      for (int m = 0; k < ke; k += GROUP, ++m) {
-        const size_t p = base + (size_t)k;
-        const double x0 = nx0, y0 = ny0, cux = nux, cuy = nuy, cuz = nuz;
-        if (k + GROUP < ke) {
-          const int kn = k + GROUP;
-          nx0 = Xa[kn]; ny0 = Ya[kn];
-          if (PUSH) { nux = Uxa[kn]; nuy = Uya[kn]; nuz = Uza[kn]; }
-        }
-        if (k < kb) continue;                  // before the cell (first window only)
-        double x1, y1, ux, uy, uz;
+      double *Xb = xb + base, *Yb = yb + base;
+      double *Uxb = uxb + base, *Uyb = uyb + base, *Uzb = uzb + base;
+      auto one = [&](int k, int m, double x0, double y0, double cux, double cuy, double cuz) {
+        if (k < kb || k >= ke) return;         // before or after the cell
+        double x1, y1, ux, uy, uz, ig = 0.0;
         // keep the 54 tile values in LDS instead of hoisting them into
         // registers for the whole cell loop (occupancy)
         __asm__ volatile("" ::: "memory");
         if (PUSH) {
           PushOut o = boris(T, g, x0, y0, cux, cuy, cuz, i, j, P, fac1, txxx, fac2);
           if (P.inj) injection_edge(o.x, o.ux, o.uy, o.uz, P);
-          xb[p] = o.x; yb[p] = o.y; uxb[p] = o.ux; uyb[p] = o.uy; uzb[p] = o.uz;
-          x1 = o.x; y1 = o.y; ux = o.ux; uy = o.uy; uz = o.uz;
+          seg_st(Xb, k, o.x); seg_st(Yb, k, o.y);
+          seg_st(Uxb, k, o.ux); seg_st(Uyb, k, o.uy); seg_st(Uzb, k, o.uz);
+          x1 = o.x; y1 = o.y; ux = o.ux; uy = o.uy; uz = o.uz; ig = o.ig;
         } else {
-          x1 = xb[p]; y1 = yb[p]; ux = uxb[p]; uy = uyb[p]; uz = uzb[p];
+          x1 = seg_ld(Xb, k); y1 = seg_ld(Yb, k);
+          ux = seg_ld(Uxb, k); uy = seg_ld(Uyb, k); uz = seg_ld(Uzb, k);
         }
         if (DEPOSIT) {
           const int i2 = (int)(x1 * P.d_delx), j2 = (int)(y1 * P.d_delx);
           if (i2 == i && j2 == j) {
-            const double gvz = uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
+            // field.f90:270's v_z = u_z / gamma; the fused kernel reuses the
+            // push's 1 / gamma of the same u (a product instead of a
+            // quotient: within an ulp), except where the shock's injection
+            // edge may have reflected the pushed u
+            const double gvz = PUSH && !P.inj
+                                   ? uz * ig
+                                   : uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
             stencil_core(core, x0, y0, x1, y1, gvz, i, j, q, P);
           } else {
             ok = ok && (i2 - i >= -1 && i2 - i <= 1 && j2 - j >= -1 && j2 - j <= 1);
             if (m < 64) mask |= 1ULL << m;
           }
         }
+      };
+      for (int m = 0, k = ka + sub; k < ke; k += 2 * GROUP, m += 2) {
+        // both halves' loads issued together, each index clamped into the
+        // cell's particles (a lane outside them reads a neighbour and idles)
+        const int kA = min(max(k, kb), ke - 1), kB = min(max(k + GROUP, kb), ke - 1);
+        const double xA = seg_ld(Xa, kA), yA = seg_ld(Ya, kA);
+        const double xB = seg_ld(Xa, kB), yB = seg_ld(Ya, kB);
+        double uxA = 0, uyA = 0, uzA = 0, uxB = 0, uyB = 0, uzB = 0;
+        if (PUSH) {
+          uxA = seg_ld(Uxa, kA); uyA = seg_ld(Uya, kA); uzA = seg_ld(Uza, kA);
+          uxB = seg_ld(Uxa, kB); uyB = seg_ld(Uya, kB); uzB = seg_ld(Uza, kB);
+        }
+        // one particle at a time: interleaving the two would need twice the
+        // registers of one push and deposit
+        one(k, m, xA, yA, uxA, uyA, uzA);
+        __builtin_amdgcn_sched_barrier(0);
+        one(k + GROUP, m + 1, xB, yB, uxB, uyB, uzB);
       }
       if (DEPOSIT) {
         // add the stay-in-cell stencils of this species over the 8 lanes

@@ -864,31 +864,35 @@ __global__ void k_deposit_gather32(int nx, int ny, int nchunk, FieldGeom fg,
 }
 
 // FP64 energy sums of FP32 momenta: per-workgroup partials.
-// Workgroup (x, seg) sums the bins of cells x, x + nchunk, ... of segment seg.
+// Workgroup (x, seg) sums the bins of cells x, x + nchunk, ... of segment seg,
+// a wavefront per bin (wavefront w takes every fourth of them) and a lane
+// per slot, in a fixed order.  Between a push and the next migration a
+// mover's slot still holds its pushed state (its record is a copy), so the
+// bins' slots below their counts hold every particle once.
 __global__ __launch_bounds__(256) void k_energy_kin32(int nx, int cap, const int *bstart,
                                                       const int *cnt, const float4 *q,
                                                       const float *uz, double c, double *part,
                                                       int nchunk) {
-  __shared__ double sh[256];
+  __shared__ double sh[4];
   const int seg = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double v = 0.0;
-  for (int i = blockIdx.x; i < nx; i += nchunk) {
+  for (int i = blockIdx.x + nchunk * w; i < nx; i += 4 * nchunk) {
     const size_t bin = (size_t)seg * nx + i;
     const int n = cnt[bin];
     const size_t base = (size_t)seg * cap + bstart[(size_t)seg * (nx + 1) + i];
-    for (int k = threadIdx.x; k < n; k += 256) {
+    for (int k = lane; k < n; k += 64) {
       const size_t p = base + k;
-      const double a = q[p].z, b = q[p].w, d = uz[p];
+      const float4 t = q[p];
+      const double a = t.z, b = t.w, d = uz[p];
       v += sqrt(1 + (a * a + b * b + d * d) / (c * c)) - 1;
     }
   }
-  sh[threadIdx.x] = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (lane == 0) sh[w] = v;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[(size_t)seg * nchunk + blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) part[(size_t)seg * nchunk + blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 PP32 make_pp32(const wm_ctx *ctx) {
