@@ -150,7 +150,7 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
             "push_ms_reference": round(push_ref, 2), "push_ms_port": round(push_port, 2),
             "sort_ms_reference": round(sort_ref, 2), "sort_ms_port": round(sort_port, 2),
             "push_bits_equal": same,
-            "ratio_port_over_reference": round((push_ref + sort_ref) / (push_port + sort_port), 3)}
+            "reference_time_over_port_time": round((push_ref + sort_ref) / (push_port + sort_port), 3)}
         log(f"reference calibration: {out['reference_calibration']}")
     # the output-step moment diagnostics on the same sample (serial C port)
     mom = np.zeros((7, nx + 2, ny + 2, p["nsp"]), order="F")

@@ -488,12 +488,6 @@ __global__ __launch_bounds__(Y2T) void k_particle_y2(BP P, int *__restrict__ np2
 constexpr int FW = 8;      // wavefronts per segment
 constexpr int FTH = FW * YW;
 
-struct YSrc {              // the lists of one destination row's arrivals and its bank
-  int nF, nS, A, H, N, newn, m;
-  size_t base, Fq, Sq;     // the arrival lists' first records
-  bool first_row;
-};
-
 template <class R>
 __global__ __launch_bounds__(FTH) void k_y2_sort(BP P, int *__restrict__ np2, R B,
                                                  const int64_t *__restrict__ id,
