@@ -176,10 +176,12 @@ __device__ inline int row_change(const Rec64 &R, size_t p, double x, double y, i
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   } else if (xmode == 1) {
     double xv = x;
-    wrap_x(xv, P);
-    if (xv != x) R.x[p] = xv;
+    int inew = wrap_x(xv, P);                      // int(x / delx), unless it wrapped
+    if (xv != x) {
+      R.x[p] = xv;
+      inew = (int)(xv / P.delx);
+    }
     xf = xv;
-    const int inew = (int)(xv / P.delx);
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   }
   const int jpos = (int)(y / P.delx);
