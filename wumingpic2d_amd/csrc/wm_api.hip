@@ -575,7 +575,12 @@ int wm_alloc_device(wm_ctx *ctx) {
   }
   const wm_grid &g = ctx->g;
   if (ctx->cap <= 0) ctx->cap = g.np;
-  ctx->cap = (ctx->cap + 15) / 16 * 16;   // segments start on 64-byte sectors
+  ctx->cap = (ctx->cap + 15) / 16 * 16;   // segments start on 128-byte lines
+  // the FP64 push addresses a segment's doubles by 32-bit byte offsets
+  if (ctx->cap >= (1 << 29)) {
+    wm_set_error(ctx, WM_ERR_ARG, "%d slots per row segment exceed 2^29", ctx->cap);
+    return WM_ERR_ARG;
+  }
   if (ctx->form == WM_FORM_FP64 && ctx->mcap <= 0) ctx->mcap = std::max(1024, ctx->cap / 8);
   const int nseg = g.nsp * g.ny;
   const size_t slots = (size_t)nseg * ctx->cap;
