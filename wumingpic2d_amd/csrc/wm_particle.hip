@@ -28,6 +28,7 @@ constexpr int TW = CHUNK + 2;         // push tile width (halo 1)
 constexpr int DW = CHUNK + 4;         // deposit tile width (halo 2)
 constexpr int TILE = 5 * DW * 3;      // doubles per deposit tile
 constexpr int NACC = 65;              // Jx 4x5 + Jy 5x4 + Jz 5x5
+constexpr int MCAP = 16;              // a cell's movers listed in LDS per species
 
 struct PP {                           // particle-kernel parameters
   int nx, ny, nxgs, nys, nsp, cap, nchunk;
@@ -261,20 +262,28 @@ __device__ inline double group_sum(double v) {
   return v;
 }
 
-// Contribution of one component of this lane's cell-crossing particles
-// (bit m of mask = particle kb+sub+GROUP*m; particles beyond the mask are
-// re-examined), combined over the group and added by lane 0 to cellbuf.
+// Contribution of one component of a cell's crossing particles, combined
+// over the group and added by lane 0 to cellbuf.  The cell's first MCAP
+// movers of the species come from its LDS list L (x0, y0, x1, y1, q v_z;
+// lane sub takes entries sub, sub + GROUP, ...); a mover past them from the
+// lane's own mask (bit m = the lane's m-th particle from ka, the state read
+// back from the banks); particles beyond the mask are re-examined.
 template <int C>
-__device__ inline void crossing_pass(double *cb, int sub, unsigned long long mask, int ka,
-                                     int ke, size_t base, const double *xa, const double *ya,
-                                     const double *xb, const double *yb, const double *uxb,
-                                     const double *uyb, const double *uzb, int i, int j,
-                                     double q, const PP &P) {
+__device__ inline void crossing_pass(double *cb, int sub, const double *L, int nl,
+                                     unsigned long long mask, int ka, int ke, size_t base,
+                                     const double *xa, const double *ya, const double *xb,
+                                     const double *yb, const double *uxb, const double *uyb,
+                                     const double *uzb, int i, int j, double q, const PP &P) {
   constexpr int N = C == 2 ? 25 : 20;
   double acc[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) acc[k] = 0.0;
   const double qc = q * P.qc_scale;
+  for (int e = sub; e < nl; e += GROUP) {
+    Move m;
+    make_move(m, L[e], L[MCAP + e], L[2 * MCAP + e], L[3 * MCAP + e], i, j, P);
+    stencil<C>(acc, m, qc, L[4 * MCAP + e]);
+  }
   auto one = [&](int k, bool known) {
     const size_t p = base + (size_t)k;
     Move m;
@@ -326,6 +335,10 @@ __global__ __launch_bounds__(256, 3) void k_particle(
     double *__restrict__ part, int *__restrict__ derr) {
   __shared__ double T[PUSH ? 6 * 3 * TW : 1];
   __shared__ double cellbuf[DEPOSIT ? CHUNK * NACC : 1];
+  // per cell: its first MCAP movers of the species in hand, as
+  // [x0 | y0 | x1 | y1 | q v_z][entry], so that their full stencils need not
+  // read the banks again (their lines are gone from L2 by then)
+  __shared__ double mvl[DEPOSIT ? CHUNK * 5 * MCAP : 1];
   const int chunk = blockIdx.x;
   const int jl = blockIdx.y;                 // row j = nys + jl
   const int j = P.nys + jl;
@@ -339,6 +352,8 @@ __global__ __launch_bounds__(256, 3) void k_particle(
   const int sub = threadIdx.x % GROUP;
   const int i = i0 + g;
   double *cb = cellbuf + g * NACC;
+  double *L = mvl + (DEPOSIT ? g * 5 * MCAP : 0);
+  const int wg = (threadIdx.x & 63) / GROUP;   // the group's place in its wavefront
   double core[DEPOSIT ? 27 : 1];
   if (DEPOSIT) {
 #pragma unroll
@@ -368,9 +383,13 @@ __global__ __launch_bounds__(256, 3) void k_particle(
       const double *Uxa = uxa + base, *Uya = uya + base, *Uza = uza + base;
       double *Xb = xb + base, *Yb = yb + base;
       double *Uxb = uxb + base, *Uyb = uyb + base, *Uzb = uzb + base;
-      auto one = [&](int k, int m, double x0, double y0, double cux, double cuy, double cuz) {
-        if (k < kb || k >= ke) return;         // before or after the cell
-        double x1, y1, ux, uy, uz, ig = 0.0;
+      int mcnt = 0;                            // the cell's movers so far (lane 0's is kept)
+      // particle k (the lane's m-th): pushed and stored, its stay-in-cell
+      // stencil added; a crossing one returns true with its deposit inputs
+      auto one = [&](int k, int m, double x0, double y0, double cux, double cuy, double cuz,
+                     double &x1, double &y1, double &qv) {
+        if (k < kb || k >= ke) return false;   // before or after the cell
+        double ux, uy, uz, ig = 0.0;
         // keep the 54 tile values in LDS instead of hoisting them into
         // registers for the whole cell loop (occupancy)
         __asm__ volatile("" ::: "memory");
@@ -386,20 +405,42 @@ __global__ __launch_bounds__(256, 3) void k_particle(
         }
         if (DEPOSIT) {
           const int i2 = (int)(x1 * P.d_delx), j2 = (int)(y1 * P.d_delx);
+          // field.f90:270's v_z = u_z / gamma; the fused kernel reuses the
+          // push's 1 / gamma of the same u (a product instead of a
+          // quotient: within an ulp), except where the shock's injection
+          // edge may have reflected the pushed u
+          const double gvz = PUSH && !P.inj
+                                 ? uz * ig
+                                 : uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
           if (i2 == i && j2 == j) {
-            // field.f90:270's v_z = u_z / gamma; the fused kernel reuses the
-            // push's 1 / gamma of the same u (a product instead of a
-            // quotient: within an ulp), except where the shock's injection
-            // edge may have reflected the pushed u
-            const double gvz = PUSH && !P.inj
-                                   ? uz * ig
-                                   : uz / sqrt(1.0 + (ux * ux + uy * uy + uz * uz) / (P.c * P.c));
             stencil_core(core, x0, y0, x1, y1, gvz, i, j, q, P);
           } else {
             ok = ok && (i2 - i >= -1 && i2 - i <= 1 && j2 - j >= -1 && j2 - j <= 1);
-            if (m < 64) mask |= 1ULL << m;
+            qv = q * gvz;
+            return m < 64;                     // later ones: re-examined from the banks
           }
         }
+        return false;
+      };
+      // a crossing particle goes to the cell's LDS list in (window, lane)
+      // order, ranked by a ballot over the group (lane 0 of the group runs
+      // every window any of its lanes runs); past MCAP to the lane's mask
+      auto record = [&](bool cross, int m, double x0, double y0, double x1, double y1,
+                        double qv) {
+        const unsigned gb = (unsigned)(__ballot(cross) >> (GROUP * wg)) & 0xFFu;
+        if (cross) {
+          const int e = mcnt + __popc(gb & ((1u << sub) - 1u));
+          if (e < MCAP) {
+            L[e] = x0;
+            L[MCAP + e] = y0;
+            L[2 * MCAP + e] = x1;
+            L[3 * MCAP + e] = y1;
+            L[4 * MCAP + e] = qv;
+          } else {
+            mask |= 1ULL << m;
+          }
+        }
+        mcnt += __popc(gb);
       };
       for (int m = 0, k = ka + sub; k < ke; k += 2 * GROUP, m += 2) {
         // both halves' loads issued together, each index clamped into the
@@ -414,9 +455,12 @@ __global__ __launch_bounds__(256, 3) void k_particle(
         }
         // one particle at a time: interleaving the two would need twice the
         // registers of one push and deposit
-        one(k, m, xA, yA, uxA, uyA, uzA);
+        double x1, y1, qv = 0.0;
+        bool cr = one(k, m, xA, yA, uxA, uyA, uzA, x1, y1, qv);
+        if (DEPOSIT) record(cr, m, xA, yA, x1, y1, qv);
         __builtin_amdgcn_sched_barrier(0);
-        one(k + GROUP, m + 1, xB, yB, uxB, uyB, uzB);
+        cr = one(k + GROUP, m + 1, xB, yB, uxB, uyB, uzB, x1, y1, qv);
+        if (DEPOSIT) record(cr, m + 1, xB, yB, x1, y1, qv);
       }
       if (DEPOSIT) {
         // add the stay-in-cell stencils of this species over the 8 lanes
@@ -427,13 +471,20 @@ __global__ __launch_bounds__(256, 3) void k_particle(
           if (sub == 0) cb[core_slot(c)] += v;
         }
       }
-      if (DEPOSIT && __ballot(mask != 0ULL || ke - ka > GROUP * 64)) {
+      mcnt = __shfl(mcnt, 0, GROUP);
+      __builtin_amdgcn_wave_barrier();         // the group reads what its lanes listed
+      const int nl = DEPOSIT ? min(mcnt, MCAP) : 0;
+      if (DEPOSIT && __ballot(nl > 0 || mask != 0ULL || ke - ka > GROUP * 64)) {
         // cell-crossing particles: full 5x5 stencils, one component at a time
-        // (stream-ordered after this lane's own writes of the new state)
-        crossing_pass<0>(cb, sub, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
-        crossing_pass<1>(cb, sub, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
-        crossing_pass<2>(cb, sub, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j, q, P);
+        // (the banks' new state stream-ordered after this lane's own writes)
+        crossing_pass<0>(cb, sub, L, nl, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j,
+                         q, P);
+        crossing_pass<1>(cb, sub, L, nl, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j,
+                         q, P);
+        crossing_pass<2>(cb, sub, L, nl, mask, ka, ke, base, xa, ya, xb, yb, uxb, uyb, uzb, i, j,
+                         q, P);
       }
+      __builtin_amdgcn_wave_barrier();         // the list is read before the next species
     }
   }
   if (!ok) atomicOr(derr, DERR_JUMP);
