@@ -110,6 +110,40 @@ def test_particle_x_y_exact(nx, ny, ppc, seed):
             assert np.array_equal(bits(active(a, n1, jl, s, 6)), bits(active(b, n2, jl, s, 6)))
 
 
+def test_particle_x_y_exact_cell_width_not_one():
+    """particle_x and particle_y with delx = 0.75 (not a power of two): the
+    cell index is int(x / delx) by division, the path the kernels skip when
+    delx is 1.  Particles moved up to 1.5 cells in x (some across the
+    periodic edge) and 0.9 in y, bit-exact against the oracle."""
+    p, up, np2, cc, uf = state(64, 8, 20, 5)
+    d = 0.75
+    rng = np.random.default_rng(5)
+    g = up.copy(order="F")
+    for s in range(p["nsp"]):
+        for jl in range(p["ny"]):
+            n = np2[jl, s]
+            g[0, :n, jl, s] = (up[0, :n, jl, s] + rng.uniform(-1.5, 1.5, n)) * d
+            g[1, :n, jl, s] = (up[1, :n, jl, s] + rng.uniform(-0.9, 0.9, n)) * d
+    p = dict(p, delx=d)
+    orc = Oracle(p)
+    ctx = make_ctx(p, init=("bc",))
+    a, b = g.copy(order="F"), g.copy(order="F")
+    n1, n2 = np2.copy(order="F"), np2.copy(order="F")
+    orc.particle_x(a, n1)
+    ctx.boundary_periodic__particle_x(b, n2)
+    assert not np.array_equal(a[0], g[0])                # some particles wrapped
+    for s in range(p["nsp"]):
+        for jl in range(p["ny"]):
+            assert np.array_equal(bits(active(a, n1, jl, s, 6)), bits(active(b, n2, jl, s, 6)))
+    assert orc.particle_y(a, n1) == 0
+    ctx.boundary_periodic__particle_y(b, n2)
+    assert np.array_equal(n1, n2)
+    assert not np.array_equal(n1, np2)                   # rows exchanged particles
+    for s in range(p["nsp"]):
+        for jl in range(p["ny"]):
+            assert np.array_equal(bits(active(a, n1, jl, s, 6)), bits(active(b, n2, jl, s, 6)))
+
+
 def test_boundary_field_maps_exact():
     p, up, np2, cc, uf = state(20, 10, 1, 9)
     rng = np.random.default_rng(9)

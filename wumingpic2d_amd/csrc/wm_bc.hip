@@ -29,6 +29,7 @@ namespace {
 struct BP {
   int nx, ny, nxgs, nxge, nygs, nyge, nys, nsp, cap, mcap;
   double delx, spanx, spany;
+  int unit;            // delx == 1 (the reference's apps): v / delx is v itself
   int xs, xe;          // active columns nxs, nxe (the wall families)
   double wl, wr;       // mirror points 2 (nxs+1) delx and 2 (nxe-1) delx
 };
@@ -40,6 +41,13 @@ struct YFuse {
   int ntiles;                   // tiles per segment
   int *hist;                    // [seg][nx]: cells after pass 2 (added to, zero before)
 };
+
+// int(v / delx) of the reference: with delx == 1 the quotient is v exactly
+// (a uniform branch on the kernel argument skips the division).
+__device__ inline int cell_div(double v, const BP &P) {
+  if (P.unit) return (int)v;
+  return (int)(v / P.delx);
+}
 
 __device__ inline double pred_d(double s) {
   // largest double below s (s finite)
@@ -58,7 +66,7 @@ __device__ inline double add_rd(double a, double b) {
 }
 
 __device__ inline int wrap_x(double &x, const BP &P) {
-  const int ipos = (int)(x / P.delx);
+  const int ipos = cell_div(x, P);
   if (ipos < P.nxgs) x = add_rd(x, P.spanx);
   else if (ipos >= P.nxge + 1) x = add_rd(x, -P.spanx);
   return ipos;
@@ -69,7 +77,7 @@ __device__ inline int wrap_x(double &x, const BP &P) {
 // mirrored about (nxs+1) delx, one in a cell at or above nxe-1 about
 // (nxe-1) delx, and its momentum reversed.  Returns the new cell.
 __device__ inline int wall_x_at(const Rec64 &R, size_t p, double x, const BP &P, double &xf) {
-  const int ipos = (int)(x / P.delx);
+  const int ipos = cell_div(x, P);
   double w;
   xf = x;
   if (ipos < P.xs + 1) w = P.wl;
@@ -81,7 +89,7 @@ __device__ inline int wall_x_at(const Rec64 &R, size_t p, double x, const BP &P,
   R.ux[p] = -R.ux[p];
   R.uy[p] = -R.uy[p];
   R.uz[p] = -R.uz[p];
-  return (int)(xn / P.delx);
+  return cell_div(xn, P);
 }
 __device__ inline int wall_x(const Rec64 &R, size_t p, const BP &P) {
   double xf;
@@ -112,7 +120,7 @@ __global__ void k_injection(BP P, const int *__restrict__ np2, Rec64 B, double x
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const size_t p = base + k;
     const double x = B.x[p];
-    const int ipos = (int)(x / P.delx);
+    const int ipos = cell_div(x, P);
     double xn = x;
     if (ipos < P.xs + 1) {
       xn = P.wl - x;
@@ -126,7 +134,7 @@ __global__ void k_injection(BP P, const int *__restrict__ np2, Rec64 B, double x
     B.x[p] = xn;
     B.uy[p] = -B.uy[p];
     B.uz[p] = -B.uz[p];
-    const int inew = (int)(xn / P.delx);
+    const int inew = cell_div(xn, P);
     bad = bad || inew < P.nxgs || inew > P.nxge;
   }
   if (bad) atomicOr(derr, DERR_RANGE);
@@ -143,7 +151,7 @@ __global__ void k_particle_x(BP P, const int *__restrict__ np2, double *__restri
     const double v0 = v;
     wrap_x(v, P);
     if (v != v0) x[base + k] = v;
-    const int inew = (int)(v / P.delx);
+    const int inew = cell_div(v, P);
     bad = bad || inew < P.nxgs || inew > P.nxge;
   }
   if (bad) atomicOr(derr, DERR_RANGE);
@@ -179,12 +187,12 @@ __device__ inline int row_change(const Rec64 &R, size_t p, double x, double y, i
     int inew = wrap_x(xv, P);                      // int(x / delx), unless it wrapped
     if (xv != x) {
       R.x[p] = xv;
-      inew = (int)(xv / P.delx);
+      inew = cell_div(xv, P);
     }
     xf = xv;
     if (inew < P.nxgs || inew > P.nxge) bad |= DERR_RANGE;
   }
-  const int jpos = (int)(y / P.delx);
+  const int jpos = cell_div(y, P);
   const int d = jpos - j;
   return (d >= -1 && d <= 1) ? d : 2;
 }
@@ -707,6 +715,7 @@ BP make_bp(const wm_ctx *ctx) {
   P.nx = g.nx; P.ny = g.ny; P.nxgs = g.nxgs; P.nxge = g.nxge; P.nygs = g.nygs;
   P.nyge = g.nyge; P.nys = g.nys; P.nsp = g.nsp; P.cap = ctx->cap; P.mcap = ctx->mcap;
   P.delx = g.delx;
+  P.unit = g.delx == 1.0;
   P.spanx = (double)(g.nxge - g.nxgs + 1) * g.delx;
   P.spany = (double)(g.nyge - g.nygs + 1) * g.delx;
   P.xs = ctx->xe > 0 ? ctx->xs : g.nxgs;
