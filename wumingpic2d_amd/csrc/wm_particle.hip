@@ -6,16 +6,18 @@
 // i0+g and its 8 lanes stride through that cell's particles of every species
 // (the reference's cell -> species -> particle loop, particle.f90:85-94 and
 // field.f90:212-221).  Particles of one cell are contiguous (sorted); a
-// group's windows start at the 8-particle boundary at or below the cell's
-// first particle (lanes before it idle in the first window), so each of its
-// loads and stores is one aligned 64-byte half line per array, shared at
-// the cell's ends with the neighbouring groups of the same wavefront.
+// group's windows of 16 particles start at the 16-particle boundary at or
+// below the cell's first particle (lanes outside the cell idle), and lane
+// sub takes the window's particles sub and 8 + sub, so the group's two
+// loads of an array cover one aligned 128-byte line.
 //
 // Push fields come from a per-workgroup LDS tile of the cell-centred fields.
-// Deposit: each lane accumulates the 5x5 stencils of its particles in
-// registers (all in the same cell), the 8 lanes of a group combine by
-// butterfly shuffles, and the workgroup assembles its 5 x (CHUNK+4) current
-// tile in LDS in a fixed order.  Tiles go to a partial buffer that a gather
+// Deposit: each lane accumulates the stay-in-cell stencils of its particles
+// in registers (all in the same cell), the 8 lanes of a group combine by
+// butterfly shuffles; cell-crossing particles are listed in LDS as they are
+// found and their full 5x5 stencils summed after the cell's loop.  The
+// workgroup assembles its 5 x (CHUNK+4) current tile in LDS in a fixed
+// order.  Tiles go to a partial buffer that a gather
 // kernel sums in a fixed order, so the deposit is deterministic (no
 // floating-point atomics).
 #include "wm_internal.h"
